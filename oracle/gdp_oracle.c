@@ -1,0 +1,662 @@
+/*
+ * gdp_oracle.c -- CPU restatement of GMAP's banded affine-gap DP core.
+ *
+ * TEST INFRASTRUCTURE ONLY (see gdp_oracle.h).  Scalar C, one cell at a time,
+ * in the same visiting order as the reference's AVX2 build: row blocks of
+ * 32 (int8) or 16 (int16) query rows, columns left to right inside a block,
+ * the E/H update for every row of the block, then the vertical (F) chain over
+ * the in-band rows in 32-bit ints.  Reference paths are relative to src/.
+ */
+#include "gdp_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------- tables */
+
+#define MM_HIGHQ (-3)  /* dynprog.h:48-50 */
+#define MM_MEDQ (-2)
+#define MM_LOWQ (-1)
+#define MM_ENDQ (-5)   /* dynprog.c:103 */
+#define SC_FULLMATCH 3 /* dynprog.h:40-42 */
+#define SC_HALFMATCH 1
+#define SC_AMBIGUOUS 3
+
+static short pd[4][128][128];
+static int nt2int[128];
+static unsigned char consistent_tab[3][128][128];
+static int use8p[4];
+
+static int lc(int ch) { return (ch >= 'A' && ch <= 'Z') ? ch - 'A' + 'a' : ch; }
+
+/* permute_cases (dynprog.c:908-977): every case combination, both orders. */
+static void set_pair(int a, int b, short score, int mode) {
+  int ca[2] = {a, lc(a)}, cb[2] = {b, lc(b)};
+  int stranded = (mode == 0 || mode == 1 || mode == 3 || mode == 5);
+  for (int i = 0; i < 2; i++) {
+    for (int j = 0; j < 2; j++) {
+      int x = ca[i], y = cb[j];
+      if (stranded) {
+        consistent_tab[0][x][y] = consistent_tab[0][y][x] = 1;
+      } else {
+        consistent_tab[1][x][y] = consistent_tab[1][y][x] = 1;
+        consistent_tab[2][x][y] = consistent_tab[2][y][x] = 1;
+      }
+      for (int t = 0; t < 4; t++) pd[t][x][y] = pd[t][y][x] = score;
+    }
+  }
+}
+
+/* permute_cases_oneway (dynprog.c:980-1003): one direction, one genestrand. */
+static void set_oneway(int a, int b, short score, int genestrand) {
+  int ca[2] = {a, lc(a)}, cb[2] = {b, lc(b)};
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) {
+      consistent_tab[genestrand][ca[i]][cb[j]] = 1;
+      for (int t = 0; t < 4; t++) pd[t][ca[i]][cb[j]] = score;
+    }
+}
+
+/* Dynprog_init (dynprog.c:1006-1196). */
+void orc_init(int mode) {
+  static const int mm[4] = {MM_HIGHQ, MM_MEDQ, MM_LOWQ, MM_ENDQ};
+  memset(pd, 0, sizeof(pd));
+  memset(consistent_tab, 0, sizeof(consistent_tab));
+  for (int j = 0; j < 128; j++) nt2int[j] = 4;
+  nt2int['A'] = nt2int['a'] = 0;
+  nt2int['C'] = nt2int['c'] = 1;
+  nt2int['G'] = nt2int['g'] = 2;
+  nt2int['T'] = nt2int['t'] = 3;
+  for (int t = 0; t < 4; t++) use8p[t] = -128 / mm[t] - 1;
+
+  /* Mismatch everywhere in 'A'..'z' x 'A'..'y' (note: second index < 'z'). */
+  for (int c1 = 'A'; c1 <= 'z'; c1++)
+    for (int c2 = 'A'; c2 < 'z'; c2++)
+      for (int t = 0; t < 4; t++) pd[t][c1][c2] = (short)mm[t];
+
+  for (int c = 'A'; c < 'Z'; c++) set_pair(c, c, SC_FULLMATCH, mode);
+  set_pair('U', 'T', SC_FULLMATCH, mode);
+  static const char *half[] = {"RA", "RG", "YT", "YC", "WA", "WT", "SG", "SC", "MA", "MC", "KG", "KT"};
+  for (unsigned k = 0; k < sizeof(half) / sizeof(half[0]); k++) set_pair(half[k][0], half[k][1], SC_HALFMATCH, mode);
+  static const char *amb[] = {"HA", "HT", "HC", "BG", "BC", "BT", "VG", "VA", "VC", "DG", "DA", "DT",
+                              "NT", "NC", "NA", "NG", "XT", "XC", "XA", "XG", "NN", "XX"};
+  for (unsigned k = 0; k < sizeof(amb) / sizeof(amb[0]); k++) set_pair(amb[k][0], amb[k][1], SC_AMBIGUOUS, mode);
+
+  switch (mode) {
+  case 1: set_oneway('T', 'C', SC_FULLMATCH, 0); break;                                        /* CMET_STRANDED */
+  case 2: set_oneway('T', 'C', SC_FULLMATCH, 1); set_oneway('A', 'G', SC_FULLMATCH, 2); break; /* CMET_NONSTRANDED */
+  case 3: set_oneway('G', 'A', SC_FULLMATCH, 0); break;                                        /* ATOI_STRANDED */
+  case 4: set_oneway('G', 'A', SC_FULLMATCH, 1); set_oneway('C', 'T', SC_FULLMATCH, 2); break; /* ATOI_NONSTRANDED */
+  case 5: set_oneway('C', 'T', SC_FULLMATCH, 0); break;                                        /* TTOC_STRANDED */
+  case 6: set_oneway('C', 'T', SC_FULLMATCH, 1); set_oneway('G', 'A', SC_FULLMATCH, 2); break; /* TTOC_NONSTRANDED */
+  default: break;
+  }
+}
+
+const short *orc_pairdistance(int mismatchtype) { return &pd[mismatchtype][0][0]; }
+const int *orc_nt_to_int(void) { return nt2int; }
+int orc_use8p_size(int mismatchtype) { return use8p[mismatchtype]; }
+
+/* Dynprog_consistent_p (dynprog.c:893-900). */
+int orc_consistent(int genestrand, int c, int g) { return consistent_tab[genestrand][c & 127][g & 127]; }
+
+void orc_compute_bands(int *lband, int *uband, int rlength, int glength, int extraband, int widebandp) {
+  if (!widebandp) {
+    *lband = extraband;
+    *uband = extraband;
+  } else if (glength >= rlength) {
+    *uband = glength - rlength + extraband;
+    *lband = extraband;
+  } else {
+    *lband = rlength - glength + extraband;
+    *uband = extraband;
+  }
+}
+
+/* ------------------------------------------------------------ fill helpers */
+
+static const char LETTERS[5] = {'A', 'C', 'G', 'T', 'N'};
+
+static inline int imax(int a, int b) { return a > b ? a : b; }
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int clampw(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+/* Tie rule: jump-late uses >=, jump-early uses >. */
+static inline int wins(int a, int b, int late) { return late ? (a >= b) : (a > b); }
+
+static inline int seq_at(const char *s, int revp, int pos1) {
+  /* Reference indexing: forward s[pos-1], reversed s[1-pos]. */
+  return (int)(signed char)(revp ? s[1 - pos1] : s[pos1 - 1]);
+}
+
+#define DIR_H 1
+#define DIR_V 2
+#define DIR_E 4
+#define DIR_F 8
+
+/* ------------------------------------------------------------- full band */
+
+void orc_fill_full(int width, const char *rseq, const char *gseq, const char *gseq_alt,
+                   int rl, int gl, int mismatchtype, int open, int extend,
+                   int lband, int uband, int late, int revp,
+                   short *H, unsigned char *dirs) {
+  const int NEG = width == 8 ? -128 : -32768, POS = width == 8 ? 127 : 32767;
+  const int S = width == 8 ? 32 : 16; /* SIMD_NCHARS / SIMD_NSHORTS, dynprog.h:122-155 */
+  const int ld = rl + 1;
+  int (*ps)[5] = malloc(sizeof(int[5]) * (size_t)(rl + 1));
+  int *FF = malloc(sizeof(int) * (size_t)(gl + 1));
+  int Hprev[32], Hold[32], E[32];
+
+  /* Query profile over the five genome classes (dynprog_simd.c:3164-3189). */
+  for (int x = 0; x < 5; x++) ps[0][x] = pd[mismatchtype]['N'][(int)LETTERS[x]];
+  for (int r = 1; r <= rl; r++) {
+    int q = seq_at(rseq, revp, r);
+    for (int x = 0; x < 5; x++) ps[r][x] = pd[mismatchtype][q & 127][(int)LETTERS[x]];
+  }
+
+  for (int rlo = 0; rlo <= rl; rlo += S) {
+    int rhigh = imin(rlo + S - 1, rl), nl = rhigh - rlo + 1;
+    int c = imax(0, rlo - lband);
+    /* Block start (dynprog_simd.c:3221-3254): E = -inf (+1 when jump-early),
+       H = -inf - open so that H + open = -inf. */
+    for (int i = 0; i < S; i++) {
+      Hprev[i] = NEG - open;
+      E[i] = late ? NEG : NEG + 1;
+    }
+    for (; c <= rhigh + uband && c <= gl; c++) {
+      int x = 4, xa = 4, X;
+      if (c == 0) {
+        X = rlo == 0 ? 0 : NEG;
+      } else {
+        x = nt2int[seq_at(gseq, revp, c) & 127];
+        xa = nt2int[seq_at(gseq_alt, revp, c) & 127];
+        X = rlo == 0 ? NEG : H[(size_t)(c - 1) * ld + rlo - 1];
+      }
+      memcpy(Hold, Hprev, sizeof(Hold));
+      for (int i = 0; i < nl; i++) {
+        int r = rlo + i;
+        /* E: horizontal gap (dynprog_simd.c:3316-3333). */
+        int T1 = clampw(Hold[i] + open, NEG, POS);
+        int dE = wins(E[i], T1, late);
+        E[i] = clampw(imax(E[i], T1) + extend, NEG, POS);
+        /* H from the diagonal plus the best of the two genome alternatives
+           (3337-3388); column 0 scores are 0 at r=0 and -inf below. */
+        int diag = i == 0 ? X : Hold[i - 1];
+        int sc = c == 0 ? (r == 0 ? 0 : NEG) : imax(ps[r][x], ps[r][xa]);
+        int Hd = clampw(diag + sc, NEG, POS);
+        int dH = wins(E[i], Hd, late);
+        H[(size_t)c * ld + r] = (short)imax(Hd, E[i]);
+        dirs[(size_t)c * ld + r] = (unsigned char)((dH ? DIR_H : 0) | (dE ? DIR_E : 0));
+      }
+
+      /* Bottom band edge: diagonal only, E is kept (3390-3416). */
+      int rhigh_calc = rhigh;
+      if (rhigh >= c + lband) {
+        rhigh_calc = c + lband;
+        if (c > 0) {
+          int r = rhigh_calc;
+          int sc = imax(ps[r][x], ps[r][xa]);
+          H[(size_t)c * ld + r] = (short)clampw(H[(size_t)(c - 1) * ld + r - 1] + sc, NEG, POS);
+          dirs[(size_t)c * ld + r] &= (unsigned char)~(DIR_H | DIR_E);
+        }
+      }
+
+      /* F: vertical gap in 32-bit ints over the in-band rows (3419-3476). */
+      int rlo_calc = imax(rlo, c - uband);
+      int cg, last;
+      if (rlo == 0 || c >= rlo + uband) {
+        cg = last = -32768;
+      } else {
+        cg = FF[c];
+        last = H[(size_t)c * ld + rlo_calc - 1];
+      }
+      int r = rlo_calc;
+      if (r == c - uband) { /* top band edge: F restarts, no F direction */
+        cg = last + open + extend;
+        last = H[(size_t)c * ld + r];
+        r++;
+      }
+      for (; r <= rhigh_calc; r++) {
+        size_t k = (size_t)c * ld + r;
+        if (wins(cg, last + open, late)) {
+          cg += extend;
+          dirs[k] |= DIR_F;
+        } else {
+          cg = last + open + extend;
+        }
+        last = H[k];
+        if (wins(cg, last, late)) {
+          last = cg;
+          H[k] = (short)clampw(cg, NEG, POS);
+          dirs[k] = (unsigned char)((dirs[k] & ~DIR_H) | DIR_V);
+        }
+      }
+      FF[c] = cg;
+      for (int i = 0; i < nl; i++) Hprev[i] = H[(size_t)c * ld + rlo + i];
+    }
+  }
+  free(FF);
+  free(ps);
+}
+
+/* -------------------------------------------------------- one-sided fills */
+
+/* Shared body of the upper (row-blocked, [c][r]) and lower (column-blocked,
+   [r][c]) triangles.  "lanes" are the blocked index (query rows for upper,
+   genome columns for lower); "steps" are the swept index.  prof[lane][x] is
+   the lane's score against class x; cls[step] the class of the swept index.
+   E is capped to -inf on and beyond the main diagonal and the diagonal's
+   directions are forced to DIAG (dynprog_simd.c:7867-7960, 8759-8860). */
+static void onesided(int width, int nlanes_total, int nsteps_total, int band,
+                     const int (*prof)[5], const int *cls, const int *cls_alt,
+                     int open, int extend, int late, short *H, unsigned char *dirs, int ld) {
+  const int NEG = width == 8 ? -128 : -32768, POS = width == 8 ? 127 : 32767;
+  const int S = width == 8 ? 32 : 16;
+  int Hprev[32], Hold[32], E[32];
+  for (int lo = 0; lo <= nlanes_total; lo += S) {
+    int hi = imin(lo + S - 1, nlanes_total), nl = hi - lo + 1;
+    for (int i = 0; i < S; i++) {
+      Hprev[i] = NEG - open;
+      E[i] = late ? NEG : NEG + 1;
+    }
+    for (int s = lo; s <= hi + band && s <= nsteps_total; s++) {
+      int X = s == 0 ? 0 : (lo == 0 ? NEG : H[(size_t)(lo - 1) * ld + s - 1]);
+      memcpy(Hold, Hprev, sizeof(Hold));
+      for (int i = 0; i < nl; i++) {
+        int lane = lo + i;
+        int cap = lane >= s ? NEG : POS;
+        int e0 = imin(E[i], cap);
+        int T1 = clampw(Hold[i] + open, NEG, POS);
+        int dE = wins(e0, T1, late);
+        E[i] = imin(clampw(imax(e0, T1) + extend, NEG, POS), cap);
+        int diag = i == 0 ? X : Hold[i - 1];
+        int sc = imax(prof[lane][cls[s]], prof[lane][cls_alt[s]]);
+        int Hd = clampw(diag + sc, NEG, POS);
+        int dH = wins(E[i], Hd, late);
+        H[(size_t)lane * ld + s] = (short)imax(Hd, E[i]);
+        dirs[(size_t)lane * ld + s] = (unsigned char)((dH ? DIR_H : 0) | (dE ? DIR_E : 0));
+      }
+      if (hi >= s) dirs[(size_t)s * ld + s] = 0;
+      for (int i = 0; i < nl; i++) Hprev[i] = H[(size_t)(lo + i) * ld + s];
+    }
+  }
+}
+
+/* The helper indexes H[lane*ld + step]; upper stores [c][r] = [step][lane],
+   so upper runs on a transposed scratch and is copied back. */
+void orc_fill_upper(int width, const char *rseq, const char *gseq, const char *gseq_alt,
+                    int rl, int gl, int mismatchtype, int open, int extend,
+                    int uband, int late, int revp, short *H, unsigned char *dirs) {
+  int (*prof)[5] = malloc(sizeof(int[5]) * (size_t)(rl + 1));
+  int *cls = malloc(sizeof(int) * (size_t)(gl + 1)), *clsa = malloc(sizeof(int) * (size_t)(gl + 1));
+  short *Ht = malloc(sizeof(short) * (size_t)(rl + 1) * (gl + 1));
+  unsigned char *Dt = malloc((size_t)(rl + 1) * (gl + 1));
+  memset(Ht, 0, sizeof(short) * (size_t)(rl + 1) * (gl + 1));
+  memset(Dt, 0, (size_t)(rl + 1) * (gl + 1));
+  /* Query profile (dynprog_simd.c:7795-7820); column 0 uses class N (7841). */
+  for (int x = 0; x < 5; x++) prof[0][x] = pd[mismatchtype]['N'][(int)LETTERS[x]];
+  for (int r = 1; r <= rl; r++) {
+    int q = seq_at(rseq, revp, r);
+    for (int x = 0; x < 5; x++) prof[r][x] = pd[mismatchtype][q & 127][(int)LETTERS[x]];
+  }
+  cls[0] = clsa[0] = 4;
+  for (int c = 1; c <= gl; c++) {
+    cls[c] = nt2int[seq_at(gseq, revp, c) & 127];
+    clsa[c] = nt2int[seq_at(gseq_alt, revp, c) & 127];
+  }
+  onesided(width, rl, gl, uband, (const int(*)[5])prof, cls, clsa, open, extend, late, Ht, Dt, gl + 1);
+  /* Only cells the reference computes are defined; copy those back. */
+  const int S = width == 8 ? 32 : 16;
+  for (int rlo = 0; rlo <= rl; rlo += S) {
+    int rhigh = imin(rlo + S - 1, rl);
+    for (int c = rlo; c <= rhigh + uband && c <= gl; c++)
+      for (int r = rlo; r <= rhigh; r++) {
+        H[(size_t)c * (rl + 1) + r] = Ht[(size_t)r * (gl + 1) + c];
+        dirs[(size_t)c * (rl + 1) + r] = Dt[(size_t)r * (gl + 1) + c];
+      }
+  }
+  free(Dt);
+  free(Ht);
+  free(clsa);
+  free(cls);
+  free(prof);
+}
+
+void orc_fill_lower(int width, const char *rseq, const char *gseq, const char *gseq_alt,
+                    int rl, int gl, int mismatchtype, int open, int extend,
+                    int lband, int late, int revp, short *H, unsigned char *dirs) {
+  int (*prof)[5] = malloc(sizeof(int[5]) * (size_t)(gl + 1));
+  int *cls = malloc(sizeof(int) * (size_t)(rl + 1));
+  short *Ht = malloc(sizeof(short) * (size_t)(rl + 1) * (gl + 1));
+  unsigned char *Dt = malloc((size_t)(rl + 1) * (gl + 1));
+  memset(Ht, 0, sizeof(short) * (size_t)(rl + 1) * (gl + 1));
+  memset(Dt, 0, (size_t)(rl + 1) * (gl + 1));
+  /* Genome profile over the five query classes, max over the two genome
+     alternatives taken up front (dynprog_simd.c:8660-8720).  Column 0 is
+     scored against 'N' in the int16 kernel (8658) but against the byte
+     value 4 in the int8 kernel (5457), which the table leaves at 0. */
+  for (int x = 0; x < 5; x++) prof[0][x] = pd[mismatchtype][(int)LETTERS[x]][width == 8 ? 4 : 'N'];
+  for (int c = 1; c <= gl; c++) {
+    int g = seq_at(gseq, revp, c) & 127, ga = seq_at(gseq_alt, revp, c) & 127;
+    for (int x = 0; x < 5; x++) prof[c][x] = imax(pd[mismatchtype][(int)LETTERS[x]][g], pd[mismatchtype][(int)LETTERS[x]][ga]);
+  }
+  cls[0] = 4;
+  for (int r = 1; r <= rl; r++) cls[r] = nt2int[seq_at(rseq, revp, r) & 127];
+  /* lanes = genome columns, steps = query rows: the helper stores
+     Ht[c*(rl+1) + r]. */
+  onesided(width, gl, rl, lband, (const int(*)[5])prof, cls, cls, open, extend, late, Ht, Dt, rl + 1);
+  const int S = width == 8 ? 32 : 16;
+  for (int clo = 0; clo <= gl; clo += S) {
+    int chigh = imin(clo + S - 1, gl);
+    for (int r = clo; r <= chigh + lband && r <= rl; r++)
+      for (int c = clo; c <= chigh; c++) {
+        H[(size_t)r * (gl + 1) + c] = Ht[(size_t)c * (rl + 1) + r];
+        dirs[(size_t)r * (gl + 1) + c] = Dt[(size_t)c * (rl + 1) + r];
+      }
+  }
+  free(Dt);
+  free(Ht);
+  free(cls);
+  free(prof);
+}
+
+/* ------------------------------------------------------------- traceback */
+
+#define TB_MATCH 1 /* scores.h:4-9 */
+#define TB_MISMATCH (-3)
+#define TB_QOPEN (-3)
+#define TB_QINDEL (-1)
+#define TB_TOPEN (-3)
+#define TB_TINDEL (-1)
+#define MICROINTRON_LENGTH 9 /* pairpool.c:18 */
+
+typedef struct {
+  int *pairs;
+  int n, cap;
+} pairbuf_t;
+
+/* Pairpool_push (pairpool.c:179-259) drops pairs at negative coordinates. */
+static void push_pair(pairbuf_t *pb, int qpos, int gpos, int cdna, int comp, int g, int galt) {
+  if (qpos < 0 || gpos < 0) return;
+  if (pb->n < pb->cap) {
+    int *p = pb->pairs + (size_t)pb->n * ORC_PAIR_FIELDS;
+    p[0] = qpos; p[1] = gpos; p[2] = cdna; p[3] = comp; p[4] = g; p[5] = galt; p[6] = 0; p[7] = 0;
+  }
+  pb->n++;
+}
+
+/* Pairpool_push_gapholder (pairpool.c:374-452) with queryjump 0. */
+static void push_gapholder(pairbuf_t *pb, int genomejump) {
+  if (pb->n < pb->cap) {
+    int *p = pb->pairs + (size_t)pb->n * ORC_PAIR_FIELDS;
+    p[0] = -1; p[1] = -1; p[2] = ' '; p[3] = ' '; p[4] = ' '; p[5] = ' '; p[6] = 1; p[7] = genomejump;
+  }
+  pb->n++;
+}
+
+static int complement_char(int ch) {
+  switch (ch) {
+  case 'A': return 'T';
+  case 'C': return 'G';
+  case 'G': return 'C';
+  case 'T': return 'A';
+  case 'a': return 't';
+  case 'c': return 'g';
+  case 'g': return 'c';
+  case 't': return 'a';
+  default: return ch; /* N, n, '*' map to themselves in COMPLEMENT_LC */
+  }
+}
+
+static int genome_char(const orc_genome_t *genome, unsigned int pos) {
+  if (genome == NULL || pos >= genome->length) return 'N';
+  if (genome->nflags && genome->nflags[pos]) return 'N';
+  return "ACGT"[genome->codes[pos] & 3];
+}
+
+/* get_genomic_nt (pairpool.c:1016-1063); the alternate genome is the genome. */
+static int genomic_nt(int *alt, const orc_genome_t *genome, int genomicpos,
+                      unsigned int chroffset, unsigned int chrhigh, int watsonp) {
+  unsigned int pos;
+  if (watsonp) {
+    pos = chroffset + (unsigned int)genomicpos;
+    if (pos < chroffset || pos >= chrhigh) { *alt = '*'; return '*'; }
+    int ch = genome_char(genome, pos);
+    *alt = ch;
+    return ch;
+  }
+  pos = chrhigh - (unsigned int)genomicpos;
+  if (pos < chroffset || pos >= chrhigh) { *alt = '*'; return '*'; }
+  int ch = complement_char(genome_char(genome, pos));
+  *alt = ch;
+  return ch;
+}
+
+/* Pairpool_add_queryskip (pairpool.c:980-1010). */
+static void queryskip(pairbuf_t *pb, int r, int c, int dist, const char *rseq,
+                      int queryoffset, int genomeoffset, int revp) {
+  int qc = r - 1, gc = c - 1, step = -1;
+  if (revp) { qc = -qc; gc = -gc; step = +1; }
+  for (int j = 0; j < dist; j++) {
+    push_pair(pb, queryoffset + qc, genomeoffset + gc, rseq[qc], '-', ' ', ' ');
+    qc += step;
+  }
+}
+
+/* Pairpool_add_genomeskip (pairpool.c:1067-1180) with genomesequence NULL. */
+static int genomeskip(pairbuf_t *pb, int r, int c, int dist, int queryoffset, int genomeoffset,
+                      const orc_genome_t *genome, int revp, unsigned int chroffset,
+                      unsigned int chrhigh, int watsonp) {
+  int qc = r - 1, left = c - dist, right = c - 1, step = -1;
+  if (revp) {
+    int t = left;
+    qc = -qc;
+    left = -right;
+    right = -t;
+    step = +1;
+  }
+  if (dist < MICROINTRON_LENGTH) {
+    int gc = revp ? left : right;
+    for (int j = 0; j < dist; j++) {
+      int alt, ch = genomic_nt(&alt, genome, genomeoffset + gc, chroffset, chrhigh, watsonp);
+      push_pair(pb, queryoffset + qc, genomeoffset + gc, ' ', '-', ch, alt);
+      gc += step;
+    }
+    return 1;
+  }
+  push_gapholder(pb, dist);
+  return 0;
+}
+
+static void diag_pair(pairbuf_t *pb, int *counters, int r, int c, const char *rseq, const char *rsequc,
+                      const char *gseq, const char *gseq_alt, int queryoffset, int genomeoffset,
+                      int revp, int genestrand) {
+  int qc = r - 1, gc = c - 1;
+  if (revp) { qc = -qc; gc = -gc; }
+  int c1 = rseq[qc], c1uc = rsequc[qc], c2 = gseq[gc], c2a = gseq_alt[gc];
+  if (c2 == '*') return; /* past the chromosome end: nothing pushed */
+  if (c1uc == c2 || c1uc == c2a) {
+    counters[0] += TB_MATCH; counters[1] += 1;
+    push_pair(pb, queryoffset + qc, genomeoffset + gc, c1, '*', c2, c2a);
+  } else if (orc_consistent(genestrand, c1uc, c2) || orc_consistent(genestrand, c1uc, c2a)) {
+    counters[0] += TB_MATCH; counters[1] += 1;
+    push_pair(pb, queryoffset + qc, genomeoffset + gc, c1, ':', c2, c2a);
+  } else {
+    counters[0] += TB_MISMATCH; counters[2] += 1;
+    push_pair(pb, queryoffset + qc, genomeoffset + gc, c1, ' ', c2, c2a);
+  }
+}
+
+int orc_traceback(int kind, const unsigned char *dirs, int rl, int gl, int r, int c,
+                  const char *rseq, const char *rsequc, const char *gseq, const char *gseq_alt,
+                  int queryoffset, int genomeoffset, const orc_genome_t *genome,
+                  int revp, unsigned int chroffset, unsigned int chrhigh, int watsonp,
+                  int genestrand, int dynprogindex, int *counters, int *pairs, int maxpairs) {
+  pairbuf_t pb = {pairs, 0, maxpairs};
+  (void)dynprogindex;
+  const int ldc = rl + 1, ldr = gl + 1;
+#define DCR(cc, rr) dirs[(size_t)(cc) * ldc + (rr)] /* [c][r] for full/upper */
+#define DRC(rr, cc) dirs[(size_t)(rr) * ldr + (cc)] /* [r][c] for lower */
+  if (kind == ORC_KIND_FULL) {
+    while (r > 0 && c > 0) {
+      int d = DCR(c, r);
+      if ((d & DIR_H) && !(d & DIR_V)) {
+        int dist = 1;
+        while (c > 0 && (DCR(c--, r) & DIR_E)) dist++;
+        if (genomeskip(&pb, r, c + dist, dist, queryoffset, genomeoffset, genome, revp, chroffset, chrhigh, watsonp)) {
+          counters[0] += TB_TOPEN + dist * TB_TINDEL; counters[3] += 1; counters[4] += dist;
+        }
+      } else if (d & DIR_V) {
+        int dist = 1;
+        while (r > 0 && (DCR(c, r--) & DIR_F)) dist++;
+        queryskip(&pb, r + dist, c, dist, rseq, queryoffset, genomeoffset, revp);
+        counters[0] += TB_QOPEN + dist * TB_QINDEL; counters[3] += 1; counters[4] += dist;
+      } else {
+        diag_pair(&pb, counters, r, c, rseq, rsequc, gseq, gseq_alt, queryoffset, genomeoffset, revp, genestrand);
+        r--; c--;
+      }
+    }
+    if (r == 0 && c == 0) {
+    } else if (c == 0) {
+      queryskip(&pb, r, 1, r, rseq, queryoffset, genomeoffset, revp);
+      counters[0] += TB_QOPEN + r * TB_QINDEL; counters[3] += 1; counters[4] += r;
+    } else {
+      int dist = c;
+      if (genomeskip(&pb, 1, c, dist, queryoffset, genomeoffset, genome, revp, chroffset, chrhigh, watsonp)) {
+        counters[0] += TB_TOPEN + dist * TB_TINDEL; counters[3] += 1; counters[4] += dist;
+      }
+    }
+  } else if (kind == ORC_KIND_UPPER) {
+    while (r > 0 && c > 0) {
+      if (DCR(c, r) & (DIR_H | DIR_V)) {
+        int dist = 1;
+        while (DCR(c--, r) & DIR_E) dist++;
+        if (genomeskip(&pb, r, c + dist, dist, queryoffset, genomeoffset, genome, revp, chroffset, chrhigh, watsonp)) {
+          counters[0] += TB_TOPEN + dist * TB_TINDEL; counters[3] += 1; counters[4] += dist;
+        }
+      } else {
+        diag_pair(&pb, counters, r, c, rseq, rsequc, gseq, gseq_alt, queryoffset, genomeoffset, revp, genestrand);
+        r--; c--;
+      }
+    }
+    if (c != 0) {
+      int dist = c;
+      if (genomeskip(&pb, 1, c, dist, queryoffset, genomeoffset, genome, revp, chroffset, chrhigh, watsonp)) {
+        counters[0] += TB_TOPEN + dist * TB_TINDEL; counters[3] += 1; counters[4] += dist;
+      }
+    }
+  } else {
+    while (r > 0 && c > 0) {
+      if (DRC(r, c) & (DIR_H | DIR_V)) {
+        int dist = 1;
+        while (DRC(r--, c) & DIR_E) dist++;
+        queryskip(&pb, r + dist, c, dist, rseq, queryoffset, genomeoffset, revp);
+        counters[0] += TB_QOPEN + dist * TB_QINDEL; counters[3] += 1; counters[4] += dist;
+      } else {
+        diag_pair(&pb, counters, r, c, rseq, rsequc, gseq, gseq_alt, queryoffset, genomeoffset, revp, genestrand);
+        r--; c--;
+      }
+    }
+    if (r != 0) {
+      queryskip(&pb, r, 1, r, rseq, queryoffset, genomeoffset, revp);
+      counters[0] += TB_QOPEN + r * TB_QINDEL; counters[3] += 1; counters[4] += r;
+    }
+  }
+#undef DCR
+#undef DRC
+  return pb.n;
+}
+
+/* ------------------------------------------------------ best end-point scan */
+
+void orc_best_endpoint(int queryend_indels, const short *Hupper, const short *Hlower,
+                       int width, int rl, int gl, int lband, int uband, int late,
+                       int *finalscore, int *bestr, int *bestc) {
+  int best = queryend_indels ? (width == 8 ? -128 : -32768) : 0;
+  int r0 = queryend_indels ? rl : 1;
+  *bestr = queryend_indels ? rl : 0;
+  *bestc = 0;
+  for (int r = r0; r <= rl; r++) {
+    int clo = imax(1, r - lband), chigh = imin(gl, r + uband), c;
+    for (c = clo; c < r; c++) {
+      int v = Hlower[(size_t)r * (gl + 1) + c];
+      if (wins(v, best, late)) { *bestr = r; *bestc = c; best = v; }
+    }
+    for (; c <= chigh; c++) {
+      int v = Hupper[(size_t)c * (rl + 1) + r];
+      if (wins(v, best, late)) { *bestr = r; *bestc = c; best = v; }
+    }
+  }
+  *finalscore = best;
+}
+
+/* ---------------------------------------------------------- genome segment */
+
+static void revcomp_inplace(char *s, unsigned int n) {
+  for (unsigned int i = 0, j = n ? n - 1 : 0; i < n / 2; i++, j--) {
+    char t = (char)complement_char(s[i]);
+    s[i] = (char)complement_char(s[j]);
+    s[j] = t;
+  }
+  if (n % 2) s[n / 2] = (char)complement_char(s[n / 2]);
+}
+
+void orc_segment_right(char *out, const orc_genome_t *genome, unsigned int left,
+                       unsigned int length, unsigned int chrhigh, int revcomp) {
+  unsigned int oob = 0;
+  if (length == 0) { out[0] = 0; return; }
+  if (left >= chrhigh) {
+    memset(out, '*', length);
+    out[length] = 0;
+    return;
+  }
+  if (left + length >= chrhigh) oob = left + length - chrhigh;
+  for (unsigned int i = 0; i < length - oob; i++) out[i] = (char)genome_char(genome, left + i);
+  for (unsigned int i = length - oob; i < length; i++) out[i] = '*';
+  out[length] = 0;
+  if (revcomp) revcomp_inplace(out, length);
+}
+
+void orc_segment_left(char *out, const orc_genome_t *genome, unsigned int right,
+                      unsigned int length, unsigned int chroffset, int revcomp) {
+  unsigned int oob = 0;
+  if (length == 0) { out[0] = 0; return; }
+  if (right < chroffset) {
+    memset(out, '*', length);
+    out[length] = 0;
+    return;
+  }
+  if (right < chroffset + length) oob = chroffset + length - right;
+  for (unsigned int i = 0; i < oob; i++) out[i] = '*';
+  for (unsigned int i = oob; i < length; i++) out[i] = (char)genome_char(genome, right - length + i);
+  out[length] = 0;
+  if (revcomp) revcomp_inplace(out, length);
+}
+
+/* ------------------------------------------------------------- cell counts */
+
+long long orc_cells_full(int rl, int gl, int lband, int uband) {
+  long long n = 0;
+  for (int c = 1; c <= gl; c++) {
+    int lo = imax(1, c - uband), hi = imin(rl, c + lband);
+    if (hi >= lo) n += hi - lo + 1;
+  }
+  return n;
+}
+
+long long orc_cells_upper(int rl, int gl, int uband) {
+  long long n = 0;
+  for (int c = 1; c <= gl; c++) {
+    int lo = imax(1, c - uband), hi = imin(c, rl);
+    if (hi >= lo) n += hi - lo + 1;
+  }
+  return n;
+}
+
+long long orc_cells_lower(int rl, int gl, int lband) {
+  long long n = 0;
+  for (int r = 1; r <= rl; r++) {
+    int lo = imax(1, r - lband), hi = imin(r, gl);
+    if (hi >= lo) n += hi - lo + 1;
+  }
+  return n;
+}

@@ -1,0 +1,109 @@
+/*
+ * gdp_oracle.h -- CPU restatement of GMAP's banded affine-gap DP core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product path never calls it.
+ *
+ * Every function restates the reference algorithm in plain scalar C,
+ * following the reference file:line cited beside it (paths relative to the
+ * reference's src/ directory).  Output encodings are this project's own:
+ *
+ *   score matrices : int16, full/upper kernels store [c][r] with stride rl+1,
+ *                    the lower kernel stores [r][c] with stride gl+1
+ *                    (mirrors the reference's matrix[c][r] / matrix[r][c]).
+ *   direction byte : bit0 nogap==HORIZ, bit1 nogap==VERT, bit2 Egap!=DIAG,
+ *                    bit3 Fgap!=DIAG (VERT excludes HORIZ in bit0/1).
+ *   pair record    : 8 x int32 {querypos, genomepos, cdna, comp, genome,
+ *                    genomealt, gapp, genomejump}, in push order
+ *                    (the reference pushes from the end cell back to (0,0)).
+ */
+#ifndef GDP_ORACLE_H
+#define GDP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_HIGHQ 0
+#define ORC_MEDQ 1
+#define ORC_LOWQ 2
+#define ORC_ENDQ 3
+
+#define ORC_KIND_FULL 0
+#define ORC_KIND_UPPER 1
+#define ORC_KIND_LOWER 2
+
+#define ORC_PAIR_FIELDS 8
+
+/* Tables (dynprog.c:1006-1196).  mode follows mode.h (0 = STANDARD). */
+void orc_init(int mode);
+const short *orc_pairdistance(int mismatchtype); /* 128*128, [query][genome] */
+const int *orc_nt_to_int(void);                  /* 128 */
+int orc_consistent(int genestrand, int c, int g);
+int orc_use8p_size(int mismatchtype);
+
+/* Dynprog_compute_bands (dynprog.c:1245-1261). */
+void orc_compute_bands(int *lband, int *uband, int rlength, int glength, int extraband, int widebandp);
+
+/* Full-band fills: Dynprog_simd_8 (dynprog_simd.c:2986-3834) and
+   Dynprog_simd_16 (dynprog_simd.c:6561-7344), AVX2 block heights 32/16.
+   width is 8 or 16.  H and dirs are dense [c][r], (gl+1)*(rl+1). */
+void orc_fill_full(int width, const char *rseq, const char *gseq, const char *gseq_alt,
+                   int rl, int gl, int mismatchtype, int open, int extend,
+                   int lband, int uband, int jump_late, int revp,
+                   short *H, unsigned char *dirs);
+
+/* One-sided fills: Dynprog_simd_{8,16}_upper (dynprog_simd.c:4303-4863,
+   7713-8192) store [c][r]; Dynprog_simd_{8,16}_lower (5339-5927, 8585-9092)
+   store [r][c] with stride gl+1. */
+void orc_fill_upper(int width, const char *rseq, const char *gseq, const char *gseq_alt,
+                    int rl, int gl, int mismatchtype, int open, int extend,
+                    int uband, int jump_late, int revp, short *H, unsigned char *dirs);
+void orc_fill_lower(int width, const char *rseq, const char *gseq, const char *gseq_alt,
+                    int rl, int gl, int mismatchtype, int open, int extend,
+                    int lband, int jump_late, int revp, short *H, unsigned char *dirs);
+
+/* Genome model used for genome-skip characters in the tracebacks
+   (pairpool.c:1016-1063 get_genomic_nt over Genome_get_char genome.c:11003).
+   codes: 2-bit A=0,C=1,G=2,T=3 one byte per base; nflags: nonzero => 'N'. */
+typedef struct {
+  const unsigned char *codes;
+  const unsigned char *nflags;
+  unsigned int length;
+} orc_genome_t;
+
+/* Tracebacks: Dynprog_traceback_{8,16} (dynprog_simd.c:9153-9312, 9552-9711),
+   _upper (9318-9435, 9715-9832), _lower (9438-9548, 9835-9945).
+   dirs layout as produced by the matching fill.  Returns the number of
+   pair records written to pairs (capacity maxpairs records).
+   counters = {traceback_score, nmatches, nmismatches, nopens, nindels} (added to). */
+int orc_traceback(int kind, const unsigned char *dirs, int rl, int gl, int r, int c,
+                  const char *rseq, const char *rsequc, const char *gseq, const char *gseq_alt,
+                  int queryoffset, int genomeoffset, const orc_genome_t *genome,
+                  int revp, unsigned int chroffset, unsigned int chrhigh, int watsonp,
+                  int genestrand, int dynprogindex, int *counters, int *pairs, int maxpairs);
+
+/* find_best_endpoint_{8,16} (dynprog_end.c:143-290) when queryend_indels==0,
+   find_best_endpoint_to_queryend_indels_{8,16} (dynprog_end.c:358-508) otherwise.
+   Hupper is [c][r] stride rl+1, Hlower is [r][c] stride gl+1. */
+void orc_best_endpoint(int queryend_indels, const short *Hupper, const short *Hlower,
+                       int width, int rl, int gl, int lband, int uband, int jump_late,
+                       int *finalscore, int *bestr, int *bestc);
+
+/* Genome_get_segment_right / _left (genome.c:11022-11120) over orc_genome_t,
+   without SNP alternates (segmentalt == segment). out has length+1 bytes. */
+void orc_segment_right(char *out, const orc_genome_t *genome, unsigned int left,
+                       unsigned int length, unsigned int chrhigh, int revcomp);
+void orc_segment_left(char *out, const orc_genome_t *genome, unsigned int right,
+                      unsigned int length, unsigned int chroffset, int revcomp);
+
+/* Count of in-band cells (algorithmic work unit). */
+long long orc_cells_full(int rl, int gl, int lband, int uband);
+long long orc_cells_upper(int rl, int gl, int uband);
+long long orc_cells_lower(int rl, int gl, int lband);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

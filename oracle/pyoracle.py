@@ -1,0 +1,283 @@
+"""ctypes front end for the CPU checkers (test infrastructure only).
+
+Two libraries share one plain-array interface:
+  * ``Oracle`` -- oracle/liboracle.so, this project's restatement
+    (gdp_oracle.c), which also travels to the GPU host;
+  * ``Reference`` -- oracle/_ref/libgmapref.so, the reference's own DP
+    objects built in place by oracle/Makefile (only where the reference tree
+    exists).
+
+Sequences are passed the way the reference receives them: for ``revp`` the
+pointer addresses the *last* character and the kernels read ``s[1-pos]``
+(dynprog_end.c:1415-1451).  Helpers here take ordinary Python ``bytes`` and
+compute those pointers.  Only tests/, __graft_entry__.smoke() and bench.py's
+CPU-baseline leg may import this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libgmapref.so")
+
+KIND_FULL, KIND_UPPER, KIND_LOWER = 0, 1, 2
+PAIR_FIELDS = 8
+
+_i = ctypes.c_int
+_u = ctypes.c_uint
+_p = ctypes.c_void_p
+
+
+class _Seq:
+    """A NUL-padded ctypes copy of a sequence plus the pointer the kernels get."""
+
+    def __init__(self, data: bytes, revp: bool):
+        self.buf = ctypes.create_string_buffer(b"\0" + data + b"\0")
+        base = ctypes.addressof(self.buf) + 1
+        self.ptr = base + (len(data) - 1 if revp else 0)
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(_p)
+
+
+class _Common:
+    lib = None
+
+    def _seqs(self, rseq, gseq, galt, revp):
+        q = _Seq(rseq, revp)
+        g = _Seq(gseq, revp)
+        ga = g if galt is None or galt is gseq else _Seq(galt, revp)
+        return q, g, ga
+
+
+class Oracle(_Common):
+    """CPU restatement (gdp_oracle.c)."""
+
+    def __init__(self, mode: int = 0, path: str = ORACLE_SO):
+        self.lib = ctypes.CDLL(path)
+        L = self.lib
+        L.orc_init.argtypes = [_i]
+        L.orc_pairdistance.restype = ctypes.POINTER(ctypes.c_short)
+        L.orc_pairdistance.argtypes = [_i]
+        L.orc_nt_to_int.restype = ctypes.POINTER(ctypes.c_int)
+        L.orc_use8p_size.argtypes = [_i]
+        L.orc_consistent.argtypes = [_i, _i, _i]
+        L.orc_fill_full.argtypes = [_i, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]
+        L.orc_fill_upper.argtypes = [_i, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]
+        L.orc_fill_lower.argtypes = [_i, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]
+        L.orc_traceback.argtypes = [_i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _i, _i, _p, _i,
+                                    _u, _u, _i, _i, _i, _p, _p, _i]
+        L.orc_traceback.restype = _i
+        L.orc_best_endpoint.argtypes = [_i, _p, _p, _i, _i, _i, _i, _i, _i, _p, _p, _p]
+        L.orc_segment_right.argtypes = [_p, _p, _u, _u, _u, _i]
+        L.orc_segment_left.argtypes = [_p, _p, _u, _u, _u, _i]
+        for f in ("orc_cells_full",):
+            getattr(L, f).restype = ctypes.c_longlong
+            getattr(L, f).argtypes = [_i, _i, _i, _i]
+        for f in ("orc_cells_upper", "orc_cells_lower"):
+            getattr(L, f).restype = ctypes.c_longlong
+            getattr(L, f).argtypes = [_i, _i, _i]
+        L.orc_init(mode)
+        self._genome_keep = None
+
+    # tables -----------------------------------------------------------
+    def pairdistance(self, mt):
+        p = self.lib.orc_pairdistance(mt)
+        return np.ctypeslib.as_array(p, shape=(128 * 128,)).reshape(128, 128).copy()
+
+    def nt_to_int(self):
+        return np.ctypeslib.as_array(self.lib.orc_nt_to_int(), shape=(128,)).copy()
+
+    def use8p(self, mt):
+        return self.lib.orc_use8p_size(mt)
+
+    def consistent(self, gs, c, g):
+        return self.lib.orc_consistent(gs, c, g)
+
+    # fills ------------------------------------------------------------
+    def fill_full(self, width, rseq, gseq, galt, mt, open_, ext, lband, uband, late, revp):
+        rl, gl = len(rseq), len(gseq)
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        H = np.zeros((gl + 1, rl + 1), np.int16)
+        D = np.zeros((gl + 1, rl + 1), np.uint8)
+        self.lib.orc_fill_full(width, q.ptr, g.ptr, ga.ptr, rl, gl, mt, open_, ext, lband, uband,
+                               int(late), int(revp), _np_ptr(H), _np_ptr(D))
+        return H, D
+
+    def fill_upper(self, width, rseq, gseq, galt, mt, open_, ext, uband, late, revp):
+        rl, gl = len(rseq), len(gseq)
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        H = np.zeros((gl + 1, rl + 1), np.int16)
+        D = np.zeros((gl + 1, rl + 1), np.uint8)
+        self.lib.orc_fill_upper(width, q.ptr, g.ptr, ga.ptr, rl, gl, mt, open_, ext, uband,
+                                int(late), int(revp), _np_ptr(H), _np_ptr(D))
+        return H, D
+
+    def fill_lower(self, width, rseq, gseq, galt, mt, open_, ext, lband, late, revp):
+        rl, gl = len(rseq), len(gseq)
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        H = np.zeros((rl + 1, gl + 1), np.int16)
+        D = np.zeros((rl + 1, gl + 1), np.uint8)
+        self.lib.orc_fill_lower(width, q.ptr, g.ptr, ga.ptr, rl, gl, mt, open_, ext, lband,
+                                int(late), int(revp), _np_ptr(H), _np_ptr(D))
+        return H, D
+
+    # traceback ----------------------------------------------------------
+    def traceback(self, kind, dirs, r, c, rseq, rsequc, gseq, galt, queryoffset=0, genomeoffset=0,
+                  revp=False, chroffset=0, chrhigh=0, watsonp=True, genestrand=0, genome=None,
+                  maxpairs=8192):
+        rl, gl = len(rseq), len(gseq)
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        quc = _Seq(rsequc, revp)
+        dirs = np.ascontiguousarray(dirs, np.uint8)
+        out = np.zeros((maxpairs, PAIR_FIELDS), np.int32)
+        cnt = np.zeros(5, np.int32)
+        gptr = None
+        if genome is not None:
+            gptr = self._genome_struct(genome)
+        n = self.lib.orc_traceback(kind, _np_ptr(dirs), rl, gl, r, c, q.ptr, quc.ptr, g.ptr, ga.ptr,
+                                   queryoffset, genomeoffset, gptr, int(revp), chroffset, chrhigh,
+                                   int(watsonp), genestrand, 0, _np_ptr(cnt), _np_ptr(out), maxpairs)
+        return out[:n].copy(), cnt
+
+    def _genome_struct(self, genome):
+        codes, nflags = genome
+
+        class G(ctypes.Structure):
+            _fields_ = [("codes", _p), ("nflags", _p), ("length", _u)]
+
+        codes = np.ascontiguousarray(codes, np.uint8)
+        nflags = np.ascontiguousarray(nflags, np.uint8)
+        st = G(codes.ctypes.data, nflags.ctypes.data, len(codes))
+        self._genome_keep = (codes, nflags, st)
+        return ctypes.addressof(st)
+
+    def best_endpoint(self, queryend_indels, Hupper, Hlower, width, rl, gl, lband, uband, late):
+        fs, br, bc = _i(), _i(), _i()
+        Hu = np.ascontiguousarray(Hupper, np.int16)
+        Hl = np.ascontiguousarray(Hlower, np.int16)
+        self.lib.orc_best_endpoint(int(queryend_indels), _np_ptr(Hu), _np_ptr(Hl), width, rl, gl,
+                                   lband, uband, int(late), ctypes.byref(fs), ctypes.byref(br),
+                                   ctypes.byref(bc))
+        return fs.value, br.value, bc.value
+
+    def segment_right(self, genome, left, length, chrhigh, revcomp):
+        out = ctypes.create_string_buffer(length + 1)
+        self.lib.orc_segment_right(out, self._genome_struct(genome), left, length, chrhigh, int(revcomp))
+        return out.raw[:length]
+
+    def segment_left(self, genome, right, length, chroffset, revcomp):
+        out = ctypes.create_string_buffer(length + 1)
+        self.lib.orc_segment_left(out, self._genome_struct(genome), right, length, chroffset, int(revcomp))
+        return out.raw[:length]
+
+    def cells_full(self, rl, gl, lband, uband):
+        return self.lib.orc_cells_full(rl, gl, lband, uband)
+
+    def cells_upper(self, rl, gl, uband):
+        return self.lib.orc_cells_upper(rl, gl, uband)
+
+    def cells_lower(self, rl, gl, lband):
+        return self.lib.orc_cells_lower(rl, gl, lband)
+
+
+class Reference(_Common):
+    """The reference's AVX2 objects (oracle/_ref/libgmapref.so)."""
+
+    def __init__(self, mode: int = 0, path: str = REF_SO, max_rl: int = 1100, max_gl: int = 2200):
+        self.lib = ctypes.CDLL(path, mode=os.RTLD_LAZY)
+        L = self.lib
+        L.refh_init.argtypes = [_i]
+        L.refh_new.restype = _p
+        L.refh_new.argtypes = [_i, _i, _i]
+        L.refh_use8p.argtypes = [_i]
+        L.refh_pairdistance.argtypes = [_i, _p]
+        L.refh_consistent.argtypes = [_i, _i, _i]
+        L.refh_fill_full.argtypes = [_p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p, _i]
+        L.refh_fill_upper.argtypes = [_p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p, _i]
+        L.refh_fill_lower.argtypes = [_p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p, _i]
+        L.refh_traceback.argtypes = [_i, _i, _i, _i, _p, _p, _p, _p, _i, _i, _i, _u, _u, _i, _i, _i,
+                                     _p, _p, _i]
+        L.refh_traceback.restype = _i
+        L.refh_init(mode)
+        self.ws_single = L.refh_new(max_rl, max_gl, 0)
+        self.ws_double = L.refh_new(max_rl, max_gl, 1)
+        self._last = {}
+
+    def pairdistance(self, mt):
+        out = np.zeros(128 * 128, np.int16)
+        self.lib.refh_pairdistance(mt, _np_ptr(out))
+        return out.reshape(128, 128)
+
+    def use8p(self, mt):
+        return self.lib.refh_use8p(mt)
+
+    def consistent(self, gs, c, g):
+        return self.lib.refh_consistent(gs, c, g)
+
+    def fill_full(self, width, rseq, gseq, galt, mt, open_, ext, lband, uband, late, revp, garbage=0):
+        rl, gl = len(rseq), len(gseq)
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        H = np.zeros((gl + 1, rl + 1), np.int16)
+        D = np.zeros((gl + 1, rl + 1), np.uint8)
+        self.lib.refh_fill_full(self.ws_single, width, q.ptr, g.ptr, ga.ptr, rl, gl, mt, open_, ext,
+                                lband, uband, int(late), int(revp), _np_ptr(H), _np_ptr(D), garbage)
+        self._last[KIND_FULL] = width
+        return H, D
+
+    def fill_upper(self, width, rseq, gseq, galt, mt, open_, ext, uband, late, revp, garbage=0):
+        rl, gl = len(rseq), len(gseq)
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        H = np.zeros((gl + 1, rl + 1), np.int16)
+        D = np.zeros((gl + 1, rl + 1), np.uint8)
+        self.lib.refh_fill_upper(self.ws_double, width, q.ptr, g.ptr, ga.ptr, rl, gl, mt, open_, ext,
+                                 uband, int(late), int(revp), _np_ptr(H), _np_ptr(D), garbage)
+        self._last[KIND_UPPER] = width
+        return H, D
+
+    def fill_lower(self, width, rseq, gseq, galt, mt, open_, ext, lband, late, revp, garbage=0):
+        rl, gl = len(rseq), len(gseq)
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        H = np.zeros((rl + 1, gl + 1), np.int16)
+        D = np.zeros((rl + 1, gl + 1), np.uint8)
+        self.lib.refh_fill_lower(self.ws_double, width, q.ptr, g.ptr, ga.ptr, rl, gl, mt, open_, ext,
+                                 lband, int(late), int(revp), _np_ptr(H), _np_ptr(D), garbage)
+        self._last[KIND_LOWER] = width
+        return H, D
+
+    def traceback(self, kind, dirs, r, c, rseq, rsequc, gseq, galt, queryoffset=0, genomeoffset=0,
+                  revp=False, chroffset=0, chrhigh=0, watsonp=True, genestrand=0, genome=None,
+                  maxpairs=8192):
+        """Traceback over the *last* fill of this kind (dirs is ignored)."""
+        assert genome is None, "the reference harness has no genome; use chrhigh=0"
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        quc = _Seq(rsequc, revp)
+        out = np.zeros((maxpairs, PAIR_FIELDS), np.int32)
+        cnt = np.zeros(5, np.int32)
+        n = self.lib.refh_traceback(kind, self._last[kind], r, c, q.ptr, quc.ptr, g.ptr, ga.ptr,
+                                    queryoffset, genomeoffset, int(revp), chroffset, chrhigh,
+                                    int(watsonp), genestrand, 0, _np_ptr(cnt), _np_ptr(out), maxpairs)
+        return out[:n].copy(), cnt
+
+
+def band_mask_full(rl, gl, lband, uband):
+    """In-band cells B (r>=1, c>=1) as a [c][r] boolean mask."""
+    c = np.arange(gl + 1)[:, None]
+    r = np.arange(rl + 1)[None, :]
+    return (c >= 1) & (r >= 1) & (r >= c - uband) & (r <= c + lband)
+
+
+def band_mask_upper(rl, gl, uband):
+    c = np.arange(gl + 1)[:, None]
+    r = np.arange(rl + 1)[None, :]
+    return (c >= 1) & (r >= 1) & (r >= c - uband) & (r <= c)
+
+
+def band_mask_lower(rl, gl, lband):
+    """[r][c] mask of the lower triangle r >= c >= max(1, r-lband)."""
+    r = np.arange(rl + 1)[:, None]
+    c = np.arange(gl + 1)[None, :]
+    return (c >= 1) & (r >= 1) & (c >= r - lband) & (c <= r)
