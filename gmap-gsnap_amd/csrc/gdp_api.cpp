@@ -1,0 +1,181 @@
+// gdp_api.cpp -- host side of the batched C ABI (include/gmapdp.h): score
+// tables, problem validation and planning, and the launch wrappers.
+#include <hip/hip_runtime.h>
+
+#include <cctype>
+#include <cstdarg>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "gdp_internal.h"
+#include "gdp_tables.h"
+
+extern "C" int gdp_launch_fill(const void *, int, const char *, const char *, uint32_t *, int16_t *, int32_t *,
+                               hipStream_t);
+extern "C" int gdp_launch_traceback(const void *, const gdp_tb_request *, int, const char *, const char *,
+                                    const uint32_t *, const char *, uint32_t, gdp_pair *, int32_t *, hipStream_t);
+extern "C" int gdp_launch_best_endpoint(const void *, const int32_t *, int, int, const int16_t *, int32_t *,
+                                        hipStream_t);
+extern "C" int gdp_launch_export(const void *, int, const uint32_t *, const int16_t *, uint8_t *, int16_t *,
+                                 hipStream_t);
+extern "C" int gdp_upload_tables(const uint32_t *, const uint32_t *, const int8_t *, const uint8_t *);
+
+namespace {
+thread_local std::string g_err;
+std::mutex g_mu;
+bool g_init = false;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(const char *what, hipError_t e) {
+  return fail(GDP_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+}  // namespace
+
+extern "C" const char *gdp_last_error(void) { return g_err.c_str(); }
+
+extern "C" int gdp_init(int mode) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  gdp_tables_build(mode);
+  uint32_t PQ[4 * 128], PG[4 * 128];
+  int8_t NT[128];
+  gdp_tables_pack(PQ, PG, NT);
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return fail(GDP_EHIP, "gdp_init: no HIP device (%s)", hipGetErrorString(e));
+  if (gdp_upload_tables(PQ, PG, NT, gdp_tables_consistent()) != 0)
+    return fail(GDP_EHIP, "gdp_init: table upload failed");
+  g_init = true;
+  return GDP_OK;
+}
+
+extern "C" int32_t gdp_descriptor_bytes(void) { return (int32_t)sizeof(gdp_dev_problem); }
+
+static int validate(const gdp_problem &p, int i) {
+  if (p.kind < 0 || p.kind > 2) return fail(GDP_EINVAL, "problem %d: bad kind %d", i, p.kind);
+  if (p.width != 8 && p.width != 16) return fail(GDP_EINVAL, "problem %d: width must be 8 or 16", i);
+  if (p.mismatchtype < 0 || p.mismatchtype > 3) return fail(GDP_EINVAL, "problem %d: bad mismatchtype", i);
+  if (p.rl < 1 || p.gl < 1 || p.rl > GDP_MAX_LEN || p.gl > GDP_MAX_LEN)
+    return fail(GDP_ETOOBIG, "problem %d: lengths %d x %d outside [1, %d]", i, p.rl, p.gl, GDP_MAX_LEN);
+  if (p.lband < 0 || p.uband < 0) return fail(GDP_EINVAL, "problem %d: negative band", i);
+  if (p.open > 0 || p.open < -127 || p.extend > 0 || p.extend < -127)
+    return fail(GDP_EINVAL, "problem %d: gap penalties must lie in [-127, 0]", i);
+  int span = p.kind == GDP_KIND_FULL ? p.lband + p.uband : (p.kind == GDP_KIND_UPPER ? p.uband : p.lband);
+  if (span > GDP_MAX_SPAN) return fail(GDP_ETOOBIG, "problem %d: band span %d > %d", i, span, GDP_MAX_SPAN);
+  return GDP_OK;
+}
+
+extern "C" int gdp_plan(const gdp_problem *probs, int32_t n, void *descs_out, int64_t *dir_words,
+                        int64_t *score_slots) {
+  if (n < 0 || (n > 0 && (probs == nullptr || descs_out == nullptr)))
+    return fail(GDP_EINVAL, "gdp_plan: bad arguments");
+  gdp_dev_problem *d = (gdp_dev_problem *)descs_out;
+  int64_t dw = 0, ss = 0;
+  for (int32_t i = 0; i < n; i++) {
+    const gdp_problem &p = probs[i];
+    int rc = validate(p, i);
+    if (rc) return rc;
+    gdp_dev_problem q;
+    memset(&q, 0, sizeof q);
+    q.qoff = p.qoff;
+    q.goff = p.goff;
+    q.galtoff = p.galtoff;
+    q.rl = p.rl;
+    q.gl = p.gl;
+    q.lband = p.lband;
+    q.uband = p.uband;
+    q.kind = p.kind;
+    q.width = p.width;
+    q.mismatchtype = p.mismatchtype;
+    q.jump_late = p.jump_late ? 1 : 0;
+    q.open = p.open;
+    q.extend = p.extend;
+    gdp_geom G = gdp_make_geom(q);
+    q.dir_off = dw;
+    dw += gdp_dir_words(G);
+    if (p.keep_scores) {
+      q.score_off = ss;
+      ss += (int64_t)G.nsteps * GDP_GROUP;
+    } else {
+      q.score_off = -1;
+    }
+    d[i] = q;
+  }
+  if (dir_words) *dir_words = dw;
+  if (score_slots) *score_slots = ss;
+  return GDP_OK;
+}
+
+extern "C" int64_t gdp_cells(const gdp_problem *p) {
+  int64_t n = 0;
+  if (p->kind == GDP_KIND_FULL) {
+    for (int c = 1; c <= p->gl; c++) {
+      int lo = gdp_imax(1, c - p->uband), hi = gdp_imin(p->rl, c + p->lband);
+      if (hi >= lo) n += hi - lo + 1;
+    }
+  } else if (p->kind == GDP_KIND_UPPER) {
+    for (int c = 1; c <= p->gl; c++) {
+      int lo = gdp_imax(1, c - p->uband), hi = gdp_imin(c, p->rl);
+      if (hi >= lo) n += hi - lo + 1;
+    }
+  } else {
+    for (int r = 1; r <= p->rl; r++) {
+      int lo = gdp_imax(1, r - p->lband), hi = gdp_imin(r, p->gl);
+      if (hi >= lo) n += hi - lo + 1;
+    }
+  }
+  return n;
+}
+
+extern "C" int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
+                        int16_t *d_scores, int32_t *d_final, void *stream) {
+  if (!g_init) return fail(GDP_ENOINIT, "gdp_fill: gdp_init has not been called");
+  if (n < 0 || (n > 0 && (!d_descs || !d_qbuf || !d_gbuf || !d_dirs || !d_final)))
+    return fail(GDP_EINVAL, "gdp_fill: null argument");
+  if (gdp_launch_fill(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_scores, d_final, (hipStream_t)stream))
+    return hip_fail("gdp_fill launch", hipGetLastError());
+  return GDP_OK;
+}
+
+extern "C" int gdp_best_endpoint(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,
+                                 const int16_t *d_scores, int32_t *d_out, void *stream) {
+  if (!g_init) return fail(GDP_ENOINIT, "gdp_best_endpoint: gdp_init has not been called");
+  if (m < 0 || (m > 0 && (!d_descs || !d_ul || !d_scores || !d_out)))
+    return fail(GDP_EINVAL, "gdp_best_endpoint: null argument");
+  if (gdp_launch_best_endpoint(d_descs, d_ul, m, queryend_indels, d_scores, d_out, (hipStream_t)stream))
+    return hip_fail("gdp_best_endpoint launch", hipGetLastError());
+  return GDP_OK;
+}
+
+extern "C" int gdp_traceback(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq, const char *d_qbuf,
+                             const char *d_gbuf, const uint32_t *d_dirs, const char *d_genome, uint32_t genome_len,
+                             gdp_pair *d_pairs, int32_t *d_counts, void *stream) {
+  if (!g_init) return fail(GDP_ENOINIT, "gdp_traceback: gdp_init has not been called");
+  if (nreq < 0 || (nreq > 0 && (!d_descs || !d_reqs || !d_qbuf || !d_gbuf || !d_dirs || !d_pairs || !d_counts)))
+    return fail(GDP_EINVAL, "gdp_traceback: null argument");
+  if (gdp_launch_traceback(d_descs, d_reqs, nreq, d_qbuf, d_gbuf, d_dirs, d_genome, genome_len, d_pairs, d_counts,
+                           (hipStream_t)stream))
+    return hip_fail("gdp_traceback launch", hipGetLastError());
+  return GDP_OK;
+}
+
+extern "C" int gdp_export_dense(const void *d_descs, int32_t idx, const uint32_t *d_dirs, const int16_t *d_scores,
+                                uint8_t *d_dense_dirs, int16_t *d_dense_scores, void *stream) {
+  if (!g_init) return fail(GDP_ENOINIT, "gdp_export_dense: gdp_init has not been called");
+  if (!d_descs || !d_dirs || !d_dense_dirs) return fail(GDP_EINVAL, "gdp_export_dense: null argument");
+  if (gdp_launch_export(d_descs, idx, d_dirs, d_scores, d_dense_dirs, d_dense_scores, (hipStream_t)stream))
+    return hip_fail("gdp_export_dense launch", hipGetLastError());
+  return GDP_OK;
+}

@@ -1,0 +1,194 @@
+// gdp_internal.h -- device-side problem layout and shared constants of the
+// banded affine-gap DP core (GMAP dynprog_simd.c semantics) for gfx950.
+#pragma once
+#include <stdint.h>
+
+#include "../../include/gmapdp.h"
+
+// Lanes that sweep one problem.  Rows of the DP matrix are dealt round robin
+// to these lanes; a lane passes H and F of its cell to the next lane one step
+// later through a DPP row shift, and the last lane hands row r+15 to lane 0
+// through an LDS ring indexed by column.
+#define GDP_GROUP 16
+// Ring entries per group (power of two, > batch period - 15).
+#define GDP_RING 1024
+
+// Packed direction nibble, one per cell, same bits as the test oracle:
+// bit0 nogap==HORIZ, bit1 nogap==VERT, bit2 Egap!=DIAG, bit3 Fgap!=DIAG.
+#define GDP_DIR_H 1
+#define GDP_DIR_V 2
+#define GDP_DIR_E 4
+#define GDP_DIR_F 8
+
+// F is carried in 32 bits by the reference (dynprog_simd.c:2995); values at
+// or below this floor can never win a comparison against H + open, so F is
+// clamped here to keep it in 16 bits (+128 bias) in the LDS ring.
+#define GDP_F_FLOOR (-32896)
+
+// Device copy of one problem (64 bytes).  Row/column roles: for the full band
+// and the upper triangle rows are query positions r = 0..rl and columns are
+// genome positions c = 0..gl; the lower triangle is swept transposed (rows are
+// genome positions, columns query positions), which is how the reference's
+// column-blocked lower kernel visits it (dynprog_simd.c:8744-8860).
+struct gdp_dev_problem {
+  int64_t qoff;     // q(r) = qbuf[qoff + r - 1], r = 1..rl (already in sweep order)
+  int64_t goff;     // g(c) = gbuf[goff + c - 1]
+  int64_t galtoff;  // galt(c) = gbuf[galtoff + c - 1]
+  int64_t dir_off;  // first 32-bit direction word of this problem
+  int64_t score_off;// first int16 score slot (one per lane-step), -1 = not kept
+  int32_t rl, gl;
+  int16_t lband, uband;
+  int8_t kind, width, mismatchtype, jump_late;
+  int8_t open, extend;
+  int8_t pad[2];
+};
+static_assert(sizeof(gdp_dev_problem) == 64, "gdp_dev_problem must stay 64 bytes");
+static_assert(sizeof(gdp_problem) == 48, "gdp_problem is 48 bytes");
+static_assert(sizeof(gdp_tb_request) == 48, "gdp_tb_request is 48 bytes");
+static_assert(sizeof(gdp_pair) == 16, "gdp_pair is 16 bytes");
+
+// Sweep geometry shared by the fill, traceback, export and scan kernels.
+struct gdp_geom {
+  int R, C;        // last row / column index
+  int lb, ub;      // band: cells with c - ub <= r <= c + lb (one-sided: lb unused)
+  int S;           // emulated SIMD block height (32 int8 / 16 int16)
+  int full;        // 1 = full band, 0 = one-sided triangle
+  int P;           // steps between consecutive row batches
+  int nsteps;      // steps the sweep takes
+};
+
+#if defined(__HIPCC__)
+#define GDP_HD __host__ __device__ __forceinline__
+#else
+#define GDP_HD inline
+#endif
+
+GDP_HD int gdp_imax(int a, int b) { return a > b ? a : b; }
+GDP_HD int gdp_imin(int a, int b) { return a < b ? a : b; }
+
+GDP_HD gdp_geom gdp_make_geom(const gdp_dev_problem &p) {
+  gdp_geom g;
+  g.full = p.kind == GDP_KIND_FULL;
+  g.R = p.kind == GDP_KIND_LOWER ? p.gl : p.rl;
+  g.C = p.kind == GDP_KIND_LOWER ? p.rl : p.gl;
+  g.S = p.width == 8 ? 32 : 16;
+  if (g.full) {
+    g.lb = p.lband;
+    g.ub = p.uband;
+  } else {
+    g.lb = 0;
+    g.ub = p.kind == GDP_KIND_UPPER ? p.uband : p.lband;
+  }
+  // A lane's next row starts P+16 steps after its current row.  The largest
+  // span of one row (including the emulated out-of-band prefix of its block)
+  // bounds P from below; 16 keeps the handoff through the ring causal.
+  int span = (g.full ? g.lb + g.ub : g.ub) + g.S - GDP_GROUP;
+  g.P = gdp_imax(GDP_GROUP, span);
+  int nb = (g.R + GDP_GROUP) / GDP_GROUP;
+  g.nsteps = (nb - 1) * g.P + (GDP_GROUP - 1) + g.C + 1;
+  return g;
+}
+
+// First and last column a row computes.  The full band starts at the column
+// where the row's emulated block starts (dynprog_simd.c:3221-3222); the
+// one-sided triangles start at the block's first row (7853).
+GDP_HD int gdp_cstart(const gdp_geom &g, int r) {
+  int blk = (r / g.S) * g.S;
+  return g.full ? gdp_imax(0, blk - g.lb) : blk;
+}
+GDP_HD int gdp_cend(const gdp_geom &g, int r) { return gdp_imin(g.C, r + g.ub); }
+
+// Step at which row r, column c is computed, and its direction word/nibble.
+GDP_HD int gdp_step(const gdp_geom &g, int r, int c) {
+  return (r / GDP_GROUP) * g.P + (r % GDP_GROUP) + c;
+}
+GDP_HD int64_t gdp_dir_words(const gdp_geom &g) { return (int64_t)((g.nsteps + 7) / 8) * GDP_GROUP; }
+
+// Per-problem constants of the cell update.
+struct gdp_consts {
+  int NEG, POS;  // saturation bounds of the kernel width
+  int o, e;      // gap open / extend (negative)
+  int late;      // 1: ties go to the gap (>=), 0: strict (>)
+  int INIT_H;    // H seen left of a block start: -inf - open (dynprog_simd.c:3238)
+  int E_INIT;    // E at a block start: -inf, or -inf + 1 for jump-early (3506-3510)
+};
+
+GDP_HD gdp_consts gdp_make_consts(const gdp_dev_problem &p) {
+  gdp_consts k;
+  k.NEG = p.width == 8 ? -128 : -32768;
+  k.POS = p.width == 8 ? 127 : 32767;
+  k.o = p.open;
+  k.e = p.extend;
+  k.late = p.jump_late ? 1 : 0;
+  k.INIT_H = k.NEG - k.o;
+  k.E_INIT = k.late ? k.NEG : k.NEG + 1;
+  return k;
+}
+
+GDP_HD int gdp_clampw(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+GDP_HD int gdp_wins(int a, int b, int late) { return late ? (a >= b) : (a > b); }
+GDP_HD int gdp_field4(uint32_t w, int x) { return ((int)(w << (28 - 4 * x))) >> 28; }
+
+// One cell (r, c) of a row whose first computed column is cs.  Inputs: the
+// lane's left neighbour (Hl, El), the diagonal Hdg = H(r-1, c-1), the upper
+// neighbour (Hup, Fup) = (H, F)(r-1, c), the row profile and the column's two
+// genome classes.  Outputs H, F (F clamped at GDP_F_FLOOR), E and the
+// direction nibble.  Rules of the reference's AVX2 build:
+//   E/H update .................... dynprog_simd.c:3316-3388 (6862-6919)
+//   column 0 and row 0 ............ 3164-3189, 3259-3291
+//   block start (E, H seeds) ...... 3221-3254
+//   bottom band edge .............. 3390-3416
+//   F chain, top band edge ........ 3419-3476
+//   one-sided E cap, diagonal ..... 4478-4626, 7867-7960
+GDP_HD void gdp_cell(const gdp_consts &k, const gdp_geom &G, int r, int c, int cs, int Hl, int El, int Hdg, int Hup,
+                     int Fup, uint32_t prof, int x, int xa, int &H, int &F, int &E, uint32_t &nib) {
+  const bool rowstart = c == cs;
+  const int hl = rowstart ? k.INIT_H : Hl;
+  int el = rowstart ? k.E_INIT : El;
+  int diag = (rowstart && (r % G.S) != 0) ? k.INIT_H : Hdg;
+  if (r == 0) diag = c == 0 ? 0 : k.NEG;
+  const int sc = gdp_imax(gdp_field4(prof, x), gdp_field4(prof, xa));
+  const int capE = (!G.full && r >= c) ? k.NEG : k.POS;
+  el = gdp_imin(el, capE);
+  const int T1 = gdp_clampw(hl + k.o, k.NEG, k.POS);
+  int dE = gdp_wins(el, T1, k.late);
+  E = gdp_imin(gdp_clampw(gdp_imax(el, T1) + k.e, k.NEG, k.POS), capE);
+  int Hd;
+  if (G.full && c == 0) Hd = r == 0 ? 0 : k.NEG;
+  else Hd = gdp_clampw(diag + sc, k.NEG, k.POS);
+  int dH = gdp_wins(E, Hd, k.late);
+  int H0 = gdp_imax(Hd, E);
+  int dF = 0, dV = 0;
+  F = GDP_F_FLOOR;
+  if (G.full) {
+    const int kd = c - r;
+    if (kd == -G.lb && c > 0) {
+      H0 = Hd;
+      dE = 0;
+      dH = 0;
+    }
+    if (kd >= -G.lb) {
+      if (kd == G.ub) {
+        F = -32768 + k.o + k.e;
+        H = H0;
+      } else {
+        const int cg = r == 0 ? -32768 : Fup;
+        const int last = r == 0 ? -32768 : Hup;
+        dF = gdp_wins(cg, last + k.o, k.late);
+        F = dF ? cg + k.e : last + k.o + k.e;
+        dV = gdp_wins(F, H0, k.late);
+        H = dV ? F : H0;
+      }
+    } else {
+      H = H0;
+    }
+    F = gdp_imax(F, GDP_F_FLOOR);
+  } else {
+    if (c == r) {
+      dE = 0;
+      dH = 0;
+    }
+    H = H0;
+  }
+  nib = (uint32_t)((dV ? GDP_DIR_V : (dH ? GDP_DIR_H : 0)) | (dE ? GDP_DIR_E : 0) | (dF ? GDP_DIR_F : 0));
+}

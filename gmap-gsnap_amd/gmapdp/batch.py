@@ -1,0 +1,135 @@
+"""Device-resident batches of DP problems driven through libgmapdp.so.
+
+torch is used only as the device allocator and stream provider; every
+computation runs in the library's HIP kernels.
+"""
+import ctypes
+
+import numpy as np
+
+from .lib import PAIR_DTYPE, PROBLEM_DTYPE, TBREQ_DTYPE, load
+
+_initialized_mode = None
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("gmapdp needs a HIP device (torch.cuda.is_available() is False)")
+    return torch
+
+
+def init(mode=0):
+    """Builds and uploads the score tables once per process (gdp_init)."""
+    global _initialized_mode
+    L = load()
+    if _initialized_mode != mode:
+        _torch()
+        rc = L.gdp_init(mode)
+        if rc != 0:
+            raise RuntimeError("gdp_init failed: %s" % L.gdp_last_error().decode())
+        _initialized_mode = mode
+    return L
+
+
+def _check(L, rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed (%d): %s" % (what, rc, L.gdp_last_error().decode()))
+
+
+def plan(problems):
+    """Host planning (no device needed): descriptors and buffer sizes."""
+    L = load()
+    problems = np.ascontiguousarray(problems, PROBLEM_DTYPE)
+    n = len(problems)
+    descs = np.zeros(max(n, 1) * L.gdp_descriptor_bytes(), np.uint8)
+    dw, ss = ctypes.c_int64(), ctypes.c_int64()
+    _check(L, L.gdp_plan(problems.ctypes.data, n, descs.ctypes.data, ctypes.byref(dw), ctypes.byref(ss)), "gdp_plan")
+    return descs, dw.value, ss.value
+
+
+def cells(problems):
+    L = load()
+    problems = np.ascontiguousarray(problems, PROBLEM_DTYPE)
+    return np.array([L.gdp_cells(problems[i:i + 1].ctypes.data) for i in range(len(problems))], np.int64)
+
+
+class DeviceBatch:
+    """Problems, sequences and outputs resident in device memory."""
+
+    def __init__(self, problems, qbuf, gbuf, mode=0, device="cuda"):
+        torch = _torch()
+        self.torch = torch
+        self.L = init(mode)
+        self.problems = np.ascontiguousarray(problems, PROBLEM_DTYPE)
+        self.n = len(self.problems)
+        descs, dw, ss = plan(self.problems)
+        self.dir_words, self.score_slots = dw, ss
+        dev = torch.device(device)
+        self.d_descs = torch.from_numpy(descs).to(dev)
+        self.d_q = torch.from_numpy(np.frombuffer(bytes(qbuf) + b"\0", np.uint8).copy()).to(dev)
+        self.d_g = torch.from_numpy(np.frombuffer(bytes(gbuf) + b"\0", np.uint8).copy()).to(dev)
+        self.d_dirs = torch.zeros(max(dw, 1), dtype=torch.int32, device=dev)
+        self.d_scores = torch.zeros(max(ss, 1), dtype=torch.int16, device=dev)
+        self.d_final = torch.zeros(max(self.n, 1), dtype=torch.int32, device=dev)
+        self.dev = dev
+
+    @property
+    def stream(self):
+        return ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    def fill(self):
+        t = self
+        _check(t.L, t.L.gdp_fill(t.d_descs.data_ptr(), t.n, t.d_q.data_ptr(), t.d_g.data_ptr(), t.d_dirs.data_ptr(),
+                                 t.d_scores.data_ptr(), t.d_final.data_ptr(), t.stream), "gdp_fill")
+
+    def final_scores(self):
+        return self.d_final[: self.n].cpu().numpy()
+
+    def export(self, idx):
+        """Dense directions and scores of one problem ([col][row] of its sweep)."""
+        p = self.problems[idx]
+        R, C = (int(p["gl"]), int(p["rl"])) if p["kind"] == 2 else (int(p["rl"]), int(p["gl"]))
+        dd = self.torch.zeros((C + 1) * (R + 1), dtype=self.torch.uint8, device=self.dev)
+        ds = self.torch.zeros((C + 1) * (R + 1), dtype=self.torch.int16, device=self.dev)
+        _check(self.L, self.L.gdp_export_dense(self.d_descs.data_ptr(), idx, self.d_dirs.data_ptr(),
+                                               self.d_scores.data_ptr(), dd.data_ptr(), ds.data_ptr(), self.stream),
+               "gdp_export_dense")
+        return (dd.cpu().numpy().reshape(C + 1, R + 1), ds.cpu().numpy().reshape(C + 1, R + 1))
+
+    def traceback(self, reqs, genome=None):
+        """Runs the requests; returns (pairs per request, counts[n,6])."""
+        torch = self.torch
+        reqs = np.ascontiguousarray(reqs, TBREQ_DTYPE).copy()
+        off = 0
+        for k in range(len(reqs)):
+            reqs["pair_off"][k] = off
+            off += int(reqs["max_pairs"][k])
+        d_reqs = torch.from_numpy(reqs.view(np.uint8)).to(self.dev)
+        d_pairs = torch.zeros(max(off, 1) * PAIR_DTYPE.itemsize, dtype=torch.uint8, device=self.dev)
+        d_counts = torch.zeros(max(len(reqs), 1) * 6, dtype=torch.int32, device=self.dev)
+        gptr, glen = None, 0
+        if genome is not None:
+            d_gen = torch.from_numpy(np.ascontiguousarray(genome, np.uint8)).to(self.dev)
+            gptr, glen = d_gen.data_ptr(), len(genome)
+        _check(self.L, self.L.gdp_traceback(self.d_descs.data_ptr(), d_reqs.data_ptr(), len(reqs), self.d_q.data_ptr(),
+                                            self.d_g.data_ptr(), self.d_dirs.data_ptr(), gptr, glen,
+                                            d_pairs.data_ptr(), d_counts.data_ptr(), self.stream), "gdp_traceback")
+        counts = d_counts.cpu().numpy()[: 6 * len(reqs)].reshape(-1, 6)
+        pairs = d_pairs.cpu().numpy().view(PAIR_DTYPE)
+        out = []
+        for k in range(len(reqs)):
+            n = min(int(counts[k, 0]), int(reqs["max_pairs"][k]))
+            out.append(pairs[int(reqs["pair_off"][k]): int(reqs["pair_off"][k]) + n].copy())
+        return out, counts
+
+    def best_endpoint(self, ul, queryend_indels=False):
+        torch = self.torch
+        ul = np.ascontiguousarray(ul, np.int32).reshape(-1)
+        m = len(ul) // 2
+        d_ul = torch.from_numpy(ul).to(self.dev)
+        d_out = torch.zeros(max(m, 1) * 3, dtype=torch.int32, device=self.dev)
+        _check(self.L, self.L.gdp_best_endpoint(self.d_descs.data_ptr(), d_ul.data_ptr(), m, int(queryend_indels),
+                                                self.d_scores.data_ptr(), d_out.data_ptr(), self.stream),
+               "gdp_best_endpoint")
+        return d_out.cpu().numpy()[: 3 * m].reshape(-1, 3)

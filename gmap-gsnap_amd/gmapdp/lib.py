@@ -1,0 +1,86 @@
+"""ctypes binding of libgmapdp.so (declarations in include/gmapdp.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+GDP_FULL, GDP_UPPER, GDP_LOWER = 0, 1, 2
+HIGHQ, MEDQ, LOWQ, ENDQ = 0, 1, 2, 3
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def library_path():
+    return os.path.join(os.path.dirname(_HERE), "lib", "libgmapdp.so")
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("qoff", ctypes.c_int64), ("goff", ctypes.c_int64), ("galtoff", ctypes.c_int64),
+                ("rl", ctypes.c_int32), ("gl", ctypes.c_int32), ("lband", ctypes.c_int16),
+                ("uband", ctypes.c_int16), ("kind", ctypes.c_int8), ("width", ctypes.c_int8),
+                ("mismatchtype", ctypes.c_int8), ("jump_late", ctypes.c_int8), ("open", ctypes.c_int8),
+                ("extend", ctypes.c_int8), ("keep_scores", ctypes.c_int8), ("pad", ctypes.c_int8),
+                ("pad2", ctypes.c_int32)]
+
+
+class TbRequest(ctypes.Structure):
+    _fields_ = [("fill", ctypes.c_int32), ("r", ctypes.c_int32), ("c", ctypes.c_int32),
+                ("queryoffset", ctypes.c_int32), ("genomeoffset", ctypes.c_int32),
+                ("chroffset", ctypes.c_uint32), ("chrhigh", ctypes.c_uint32), ("watsonp", ctypes.c_int8),
+                ("revp", ctypes.c_int8), ("genestrand", ctypes.c_int8), ("pad", ctypes.c_int8),
+                ("pair_off", ctypes.c_int64), ("max_pairs", ctypes.c_int32), ("pad2", ctypes.c_int32)]
+
+
+class Pair(ctypes.Structure):
+    _fields_ = [("querypos", ctypes.c_int32), ("genomepos", ctypes.c_int32), ("cdna", ctypes.c_int8),
+                ("comp", ctypes.c_int8), ("genome", ctypes.c_int8), ("genomealt", ctypes.c_int8),
+                ("gapp", ctypes.c_int16), ("genomejump", ctypes.c_int16)]
+
+
+PROBLEM_DTYPE = np.dtype([("qoff", "<i8"), ("goff", "<i8"), ("galtoff", "<i8"), ("rl", "<i4"), ("gl", "<i4"),
+                          ("lband", "<i2"), ("uband", "<i2"), ("kind", "i1"), ("width", "i1"),
+                          ("mismatchtype", "i1"), ("jump_late", "i1"), ("open", "i1"), ("extend", "i1"),
+                          ("keep_scores", "i1"), ("pad", "i1"), ("pad2", "<i4")])
+TBREQ_DTYPE = np.dtype([("fill", "<i4"), ("r", "<i4"), ("c", "<i4"), ("queryoffset", "<i4"),
+                        ("genomeoffset", "<i4"), ("chroffset", "<u4"), ("chrhigh", "<u4"), ("watsonp", "i1"),
+                        ("revp", "i1"), ("genestrand", "i1"), ("pad", "i1"), ("pair_off", "<i8"),
+                        ("max_pairs", "<i4"), ("pad2", "<i4")])
+PAIR_DTYPE = np.dtype([("querypos", "<i4"), ("genomepos", "<i4"), ("cdna", "i1"), ("comp", "i1"),
+                       ("genome", "i1"), ("genomealt", "i1"), ("gapp", "<i2"), ("genomejump", "<i2")])
+assert PROBLEM_DTYPE.itemsize == 48 and TBREQ_DTYPE.itemsize == 48 and PAIR_DTYPE.itemsize == 16
+
+_lib = None
+
+
+def load(path=None):
+    """Loads libgmapdp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or library_path()
+    if not os.path.exists(p):
+        raise RuntimeError("libgmapdp.so not built (%s); run __graft_entry__.build()" % p)
+    L = ctypes.CDLL(p)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    sig = {
+        "gdp_init": (ctypes.c_int, [ctypes.c_int]),
+        "gdp_last_error": (ctypes.c_char_p, []),
+        "gdp_descriptor_bytes": (i32, []),
+        "gdp_plan": (ctypes.c_int, [vp, i32, vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+        "gdp_cells": (i64, [vp]),
+        "gdp_fill": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, vp, vp]),
+        "gdp_best_endpoint": (ctypes.c_int, [vp, vp, i32, i32, vp, vp, vp]),
+        "gdp_traceback": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp]),
+        "gdp_export_dense": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, vp]),
+        "gdp_pairdistance": (ctypes.POINTER(ctypes.c_int16), [ctypes.c_int]),
+        "gdp_nt_to_int": (ctypes.POINTER(ctypes.c_int32), []),
+        "gdp_use8p_size": (i32, [ctypes.c_int]),
+        "gdp_consistent": (i32, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "gdp_compute_bands": (None, [ctypes.POINTER(i32), ctypes.POINTER(i32), i32, i32, i32, i32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L

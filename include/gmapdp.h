@@ -1,0 +1,167 @@
+/*
+ * gmapdp.h -- C ABI of the MI355X (gfx950) banded affine-gap DP core.
+ *
+ * This is the batched interface: many independent DP problems are described
+ * in one array, their sequences sit in two byte buffers in device memory, and
+ * each call launches one kernel over the whole array.  The per-call
+ * interface with the reference's own signatures is in gmapdp_dynprog.h.
+ *
+ * Semantics follow the reference's AVX2 build of GMAP's DP core
+ * (src/dynprog_simd.c, src/dynprog.c, src/dynprog_end.c, src/pairpool.c);
+ * each entry point cites the reference function it stands in for.
+ *
+ * Conventions
+ *  - All "d_" pointers are device pointers on the current HIP device; the
+ *    caller allocates them (hipMalloc, or any framework allocator).
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *  - Functions return 0 on success and a negative GDP_E* code otherwise;
+ *    gdp_last_error() returns a message for the last failure on the thread.
+ *  - Sequences are stored in the order the fill consumes them: row r of a
+ *    problem reads query byte qbuf[qoff + r - 1] and column c reads genome
+ *    byte gbuf[goff + c - 1] (and gbuf[galtoff + c - 1] for the alternate
+ *    allele).  A caller holding the reference's reversed pointers
+ *    (revp = true, s[1 - r]) stores the reversed sequence.
+ */
+#ifndef GMAPDP_H
+#define GMAPDP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GDP_OK 0
+#define GDP_EINVAL (-1)
+#define GDP_ETOOBIG (-2)
+#define GDP_EHIP (-3)
+#define GDP_ENOINIT (-4)
+#define GDP_ENOMEM (-5)
+
+#define GDP_KIND_FULL 0  /* Dynprog_simd_8 / Dynprog_simd_16 */
+#define GDP_KIND_UPPER 1 /* Dynprog_simd_8_upper / Dynprog_simd_16_upper */
+#define GDP_KIND_LOWER 2 /* Dynprog_simd_8_lower / Dynprog_simd_16_lower */
+
+#define GDP_HIGHQ 0 /* Mismatchtype_T, dynprog.h:53 */
+#define GDP_MEDQ 1
+#define GDP_LOWQ 2
+#define GDP_ENDQ 3
+
+/* Largest band span a problem may use (lband + uband, or the one-sided band). */
+#define GDP_MAX_SPAN 960
+#define GDP_MAX_LEN 8000
+
+/* One DP problem (48 bytes, pad fields zero).  width 8 or 16 selects the reference's int8 or
+   int16 kernel; open/extend are the (negative) gap penalties; jump_late is
+   the reference's jump_late_p (ties prefer the gap); keep_scores asks the
+   fill to keep every computed score (needed by gdp_best_endpoint). */
+typedef struct gdp_problem {
+  int64_t qoff;
+  int64_t goff;
+  int64_t galtoff;
+  int32_t rl;
+  int32_t gl;
+  int16_t lband;
+  int16_t uband;
+  int8_t kind;
+  int8_t width;
+  int8_t mismatchtype;
+  int8_t jump_late;
+  int8_t open;
+  int8_t extend;
+  int8_t keep_scores;
+  int8_t pad;
+  int32_t pad2;
+} gdp_problem;
+
+/* One traceback request: walk problem `fill` from cell (r, c) (query row,
+   genome column) back to row or column 0, as Dynprog_traceback_{8,16}
+   (dynprog_simd.c:9153, 9552), _upper (9318, 9715) or _lower (9438, 9835)
+   do for the problem's kind.  Offsets, chromosome bounds, strand and
+   genestrand have the reference's meaning. */
+typedef struct gdp_tb_request {
+  int32_t fill;
+  int32_t r;
+  int32_t c;
+  int32_t queryoffset;
+  int32_t genomeoffset;
+  uint32_t chroffset;
+  uint32_t chrhigh;
+  int8_t watsonp;
+  int8_t revp;
+  int8_t genestrand;
+  int8_t pad;
+  int64_t pair_off;
+  int32_t max_pairs;
+  int32_t pad2;
+} gdp_tb_request;
+
+/* One traceback output record, in the order the reference pushes them onto
+   its pair list (from the end cell back to the start).  gapp = 1 marks a
+   gap holder (pairpool.c:374-452) whose genomejump is the skipped length. */
+typedef struct gdp_pair {
+  int32_t querypos;
+  int32_t genomepos;
+  int8_t cdna;
+  int8_t comp;
+  int8_t genome;
+  int8_t genomealt;
+  int16_t gapp;
+  int16_t genomejump;
+} gdp_pair;
+
+/* Builds the score tables for a GMAP mode (mode.h: 0 = STANDARD ...), as
+   Dynprog_init does (dynprog.c:1006-1196).  Must precede every other call. */
+int gdp_init(int mode);
+
+/* Message describing the last failure on this thread. */
+const char *gdp_last_error(void);
+
+/* Size in bytes of one device problem descriptor. */
+int32_t gdp_descriptor_bytes(void);
+
+/* Host-side planning: validates the problems and writes n device
+   descriptors (gdp_descriptor_bytes() each) into descs_out, assigning each
+   problem its slice of the direction and score buffers.  Returns the 32-bit
+   direction words and int16 score slots the batch needs. */
+int gdp_plan(const gdp_problem *probs, int32_t n, void *descs_out, int64_t *dir_words, int64_t *score_slots);
+
+/* In-band cell count of a problem (the throughput work unit). */
+int64_t gdp_cells(const gdp_problem *prob);
+
+/* Fills every problem: one kernel launch over the whole array.  d_final[i]
+   receives the score of cell (rl, gl) for full-band problems (the value
+   Dynprog_single_gap reads, dynprog_single.c:603), or INT32_MIN when that
+   cell is outside the band. */
+int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
+             int16_t *d_scores, int32_t *d_final, void *stream);
+
+/* Best end cell of end problems: for each of m pairs (upper, lower) of
+   one-sided fills with keep_scores set, find_best_endpoint_{8,16}
+   (dynprog_end.c:143-290) when queryend_indels = 0, or
+   find_best_endpoint_to_queryend_indels_{8,16} (358-508) otherwise.
+   d_out[3*k] = {finalscore, bestr, bestc}. */
+int gdp_best_endpoint(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,
+                      const int16_t *d_scores, int32_t *d_out, void *stream);
+
+/* Tracebacks, one GPU thread per request.  d_genome (may be NULL) holds the
+   chromosome as ASCII bases indexed by absolute position, used for genome
+   skips exactly as get_genomic_nt (pairpool.c:1016-1063) uses
+   Genome_get_char; positions outside [chroffset, chrhigh) give '*'.
+   d_counts[6*k] = {npairs, traceback_score, nmatches, nmismatches, nopens,
+   nindels}. */
+int gdp_traceback(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq, const char *d_qbuf,
+                  const char *d_gbuf, const uint32_t *d_dirs, const char *d_genome, uint32_t genome_len,
+                  gdp_pair *d_pairs, int32_t *d_counts, void *stream);
+
+/* Writes one problem's directions (and scores if kept) in the reference's
+   dense layout: [c][r] with stride rl+1 for full/upper, [r][c] with stride
+   gl+1 for lower; direction bytes use bit0 nogap HORIZ, bit1 nogap VERT,
+   bit2 Egap, bit3 Fgap.  Cells the fill does not compute are 0. */
+int gdp_export_dense(const void *d_descs, int32_t idx, const uint32_t *d_dirs, const int16_t *d_scores,
+                     uint8_t *d_dense_dirs, int16_t *d_dense_scores, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

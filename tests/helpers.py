@@ -1,0 +1,160 @@
+"""Shared generators and comparison helpers for the tests."""
+import numpy as np
+
+import pyoracle as po
+from gmapdp.lib import PROBLEM_DTYPE, TBREQ_DTYPE
+
+ALPH = np.frombuffer(b"ACGT", np.uint8)
+
+
+class Spec:
+    """One DP problem as the reference's callers would pass it."""
+
+    def __init__(self, kind, width, q, g, galt, mt, open_, ext, lband, uband, late, revp, start=None):
+        self.kind, self.width = kind, width
+        self.q, self.g, self.galt = q, g, galt
+        self.mt, self.open, self.ext = mt, open_, ext
+        self.lband, self.uband, self.late, self.revp = lband, uband, late, revp
+        self.start = start  # (r, c) traceback start
+
+    @property
+    def rl(self):
+        return len(self.q)
+
+    @property
+    def gl(self):
+        return len(self.g)
+
+
+def rand_seq(rng, n, nfrac=0.0):
+    s = ALPH[rng.integers(0, 4, n)].copy()
+    if nfrac:
+        s[rng.random(n) < nfrac] = ord("N")
+    return s.tobytes()
+
+
+def mutate(rng, src, sub, ind, maxind=3):
+    out = bytearray()
+    i = 0
+    while i < len(src):
+        u = rng.random()
+        if u < ind / 2:
+            out += rand_seq(rng, int(rng.integers(1, maxind + 1)))
+            continue
+        if u < ind:
+            i += int(rng.integers(1, maxind + 1))
+            continue
+        ch = src[i]
+        i += 1
+        if rng.random() < sub:
+            ch = rand_seq(rng, 1)[0]
+        out.append(ch)
+    return bytes(out)
+
+
+def random_specs(seed, n, kinds=(0, 1, 2), widths=(8, 16), maxrl8=60, maxrl16=200, lowercase=True):
+    """Adversarial random problems: all kinds, widths, tie rules, strands,
+    IUPAC/N characters, lower case, alternate alleles and narrow bands."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n):
+        width = int(rng.choice(widths))
+        rl = int(rng.integers(1, maxrl8 if width == 8 else maxrl16))
+        gl = int(rng.integers(max(1, rl - 10), rl + 40))
+        g = rand_seq(rng, gl, 0.02)
+        q = mutate(rng, g, float(rng.choice([0.02, 0.1, 0.3, 0.75])), 0.03)[:rl]
+        if len(q) < rl:
+            q = q + rand_seq(rng, rl - len(q))
+        if lowercase and rng.random() < 0.2:
+            q = bytes(ch + 32 if (65 <= ch <= 90 and rng.random() < 0.3) else ch for ch in q)
+        if rng.random() < 0.1:
+            q = bytes(b"RYKMSWBDHVNX"[rng.integers(0, 12)] if rng.random() < 0.1 else ch for ch in q)
+        galt = g if rng.random() < 0.8 else bytes(ch if rng.random() < 0.9 else ALPH[rng.integers(0, 4)] for ch in g)
+        mt = int(rng.integers(0, 4))
+        op = int(rng.choice([-8, -7, -6, -10, -20]))
+        ex = int(rng.choice([-3, -2, -1, -5]))
+        late = bool(rng.integers(0, 2))
+        revp = bool(rng.integers(0, 2))
+        eb = int(rng.integers(0, 25))
+        lb, ub = (eb, gl - rl + eb) if gl >= rl else (rl - gl + eb, eb)
+        kind = int(rng.choice(kinds))
+        specs.append(Spec(kind, width, q, g, galt, mt, op, ex, lb, ub, late, revp))
+    return specs
+
+
+def oracle_fill(O, s):
+    if s.kind == 0:
+        return O.fill_full(s.width, s.q, s.g, s.galt, s.mt, s.open, s.ext, s.lband, s.uband, s.late, s.revp)
+    if s.kind == 1:
+        return O.fill_upper(s.width, s.q, s.g, s.galt, s.mt, s.open, s.ext, s.uband, s.late, s.revp)
+    return O.fill_lower(s.width, s.q, s.g, s.galt, s.mt, s.open, s.ext, s.lband, s.late, s.revp)
+
+
+def band_mask(s):
+    if s.kind == 0:
+        return po.band_mask_full(s.rl, s.gl, s.lband, s.uband)
+    if s.kind == 1:
+        return po.band_mask_upper(s.rl, s.gl, s.uband)
+    return po.band_mask_lower(s.rl, s.gl, s.lband)
+
+
+def computed_mask(s):
+    """Cells the fill computes deterministically, as dense [col][row] of the
+    sweep (rows = query for full/upper, genome for lower): the band plus the
+    out-of-band prefix of each emulated 32/16-row block."""
+    S = 32 if s.width == 8 else 16
+    R, C = (s.gl, s.rl) if s.kind == 2 else (s.rl, s.gl)
+    r = np.arange(R + 1)[None, :]
+    c = np.arange(C + 1)[:, None]
+    blk = (r // S) * S
+    if s.kind == 0:
+        cs = np.maximum(0, blk - s.lband)
+        ce = np.minimum(C, r + s.uband)
+    else:
+        ub = s.uband if s.kind == 1 else s.lband
+        cs = blk
+        ce = np.minimum(C, r + ub)
+    return (c >= cs) & (c <= ce)
+
+
+def pack_batch(specs, keep_scores=False):
+    """Problems plus query/genome buffers in sweep order."""
+    probs = np.zeros(len(specs), PROBLEM_DTYPE)
+    qparts, gparts = [], []
+    qo = go = 0
+    for i, s in enumerate(specs):
+        q, g, ga = (s.q[::-1], s.g[::-1], s.galt[::-1]) if s.revp else (s.q, s.g, s.galt)
+        p = probs[i]
+        p["qoff"], p["goff"], p["galtoff"] = qo, go, go + len(g)
+        p["rl"], p["gl"], p["lband"], p["uband"] = s.rl, s.gl, s.lband, s.uband
+        p["kind"], p["width"], p["mismatchtype"], p["jump_late"] = s.kind, s.width, s.mt, int(s.late)
+        p["open"], p["extend"], p["keep_scores"] = s.open, s.ext, int(keep_scores)
+        qparts.append(q)
+        gparts.append(g + ga)
+        qo += len(q)
+        go += len(g) + len(ga)
+    return probs, b"".join(qparts), b"".join(gparts)
+
+
+def pairs_to_records(pairs):
+    """GPU pair records -> oracle 8-int records."""
+    out = np.zeros((len(pairs), 8), np.int32)
+    out[:, 0] = pairs["querypos"]
+    out[:, 1] = pairs["genomepos"]
+    out[:, 2] = pairs["cdna"]
+    out[:, 3] = pairs["comp"]
+    out[:, 4] = pairs["genome"]
+    out[:, 5] = pairs["genomealt"]
+    out[:, 6] = pairs["gapp"]
+    out[:, 7] = pairs["genomejump"]
+    return out
+
+
+def tb_request(i, s, r, c, qoff=5, goff=7, chroffset=0, chrhigh=0, watsonp=True, genestrand=0):
+    req = np.zeros(1, TBREQ_DTYPE)
+    req["fill"], req["r"], req["c"] = i, r, c
+    req["queryoffset"], req["genomeoffset"] = qoff, goff
+    req["chroffset"], req["chrhigh"], req["watsonp"] = chroffset, chrhigh, int(watsonp)
+    req["revp"], req["genestrand"] = int(s.revp), genestrand
+    req["max_pairs"] = s.rl + s.gl + 4
+    return req[0]

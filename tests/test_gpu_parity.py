@@ -1,0 +1,106 @@
+"""GPU parity: the HIP fill/traceback against the CPU oracle, bit for bit.
+
+The oracle (oracle/gdp_oracle.c) is pinned to the reference's AVX2 build by
+tests/test_oracle_reference.py (where the reference tree exists) and by the
+committed golden vectors in tests/golden (everywhere).
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(oracle, specs, keep_scores=True):
+    from gmapdp.batch import DeviceBatch
+    probs, qbuf, gbuf = H.pack_batch(specs, keep_scores=keep_scores)
+    b = DeviceBatch(probs, qbuf, gbuf)
+    b.fill()
+    return b
+
+
+def _check_fills(oracle, specs, b):
+    fin = b.final_scores()
+    for i, s in enumerate(specs):
+        Ho, Do = H.oracle_fill(oracle, s)
+        Dg, Sg = b.export(i)
+        B = H.band_mask(s)
+        M = H.computed_mask(s)
+        assert (Dg[B] == Do[B]).all(), ("in-band directions differ", i, s.kind, s.width, s.rl, s.gl)
+        assert (Sg[B] == Ho[B]).all(), ("in-band scores differ", i, s.kind, s.width, s.rl, s.gl)
+        assert (Dg[M] == Do[M]).all(), ("out-of-band block directions differ", i, s.kind, s.width)
+        assert (Sg[M] == Ho[M]).all(), ("out-of-band block scores differ", i, s.kind, s.width)
+        if s.kind == 0:
+            in_band = s.uband >= s.gl - s.rl and s.lband >= s.rl - s.gl
+            want = int(Ho[s.gl, s.rl]) if in_band else np.iinfo(np.int32).min
+            assert fin[i] == want, ("final score", i, fin[i], want)
+
+
+def _check_tracebacks(oracle, specs, b, seed):
+    rng = np.random.default_rng(seed)
+    reqs, starts = [], []
+    for i, s in enumerate(specs):
+        B = H.band_mask(s)
+        if s.kind == 0:
+            if not (s.uband >= s.gl - s.rl and s.lband >= s.rl - s.gl):
+                continue
+            r, c = s.rl, s.gl
+        else:
+            cells = np.argwhere(B)
+            if len(cells) == 0:
+                continue
+            a, bb = cells[rng.integers(0, len(cells))]
+            r, c = (int(bb), int(a)) if s.kind == 1 else (int(a), int(bb))
+        reqs.append(H.tb_request(i, s, r, c))
+        starts.append((i, r, c))
+    pairs, counts = b.traceback(np.array(reqs))
+    for k, (i, r, c) in enumerate(starts):
+        s = specs[i]
+        Ho, Do = H.oracle_fill(oracle, s)
+        Do = np.where(H.computed_mask(s), Do, 0).astype(np.uint8)
+        po, co = oracle.traceback(s.kind, Do, r, c, s.q, s.q.upper(), s.g, s.galt, 5, 7, s.revp)
+        got = H.pairs_to_records(pairs[k])
+        assert counts[k, 0] == len(po), ("pair count", i, counts[k, 0], len(po))
+        assert (got == po).all(), ("pairs differ", i, s.kind)
+        assert (counts[k, 1:] == co).all(), ("counters differ", i, counts[k, 1:], co)
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_random_all_kinds(oracle, seed):
+    specs = H.random_specs(seed, 400)
+    b = _run(oracle, specs)
+    _check_fills(oracle, specs, b)
+    _check_tracebacks(oracle, specs, b, seed)
+
+
+def test_wide_bands_and_long_queries(oracle):
+    rng = np.random.default_rng(7)
+    specs = []
+    for _ in range(40):
+        rl = int(rng.integers(300, 1000))
+        gl = int(rl * rng.uniform(1.0, 1.5))
+        g = H.rand_seq(rng, gl)
+        q = H.mutate(rng, g, 0.05, 0.01)[:rl]
+        q = q + H.rand_seq(rng, rl - len(q))
+        eb = int(rng.integers(3, 65))
+        specs.append(H.Spec(0, 16, q, g, g, 0, -8, -3, eb, gl - rl + eb, bool(rng.integers(0, 2)), False))
+    b = _run(oracle, specs)
+    _check_fills(oracle, specs, b)
+    _check_tracebacks(oracle, specs, b, 7)
+
+
+def test_edge_shapes(oracle):
+    specs = []
+    for rl, gl, lb, ub in [(1, 1, 0, 0), (1, 5, 0, 4), (5, 1, 4, 0), (2, 40, 0, 38), (40, 2, 38, 0),
+                           (33, 33, 0, 0), (64, 64, 1, 1), (31, 95, 3, 67), (16, 16, 15, 15)]:
+        for width in (8, 16):
+            for kind in (0, 1, 2):
+                rng = np.random.default_rng(rl * 1000 + gl)
+                g = H.rand_seq(rng, gl)
+                q = H.rand_seq(rng, rl)
+                for late in (False, True):
+                    specs.append(H.Spec(kind, width, q, g, g, 3, -10, -2, lb, ub, late, late))
+    b = _run(oracle, specs)
+    _check_fills(oracle, specs, b)
+    _check_tracebacks(oracle, specs, b, 3)
