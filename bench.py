@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the banded affine-gap DP core on one MI355X.
+
+A step is one pass of the hot path over one batch: every problem of the
+workload is filled (one batched launch) and traced back (one batched
+launch).  Inputs are resident in HBM before the timed region.  The default
+workload is the 10^6-problem single-gap batch (rl 150, gl 160-200,
+extraband 5-20, int16); --workload c3 selects the uneven 2x10^6 batch.
+
+Prints one JSON line (the fields the driver reads) with two extra objects:
+  roofline      -- the fill kernel against the packed-int16 VALU peak, its
+                   average duration measured with HIP events on its stream;
+  cpu_baseline  -- the reference's own AVX2 DP objects (oracle/_ref, built
+                   from the reference sources) or, where absent, the CPU
+                   restatement, timed on a bounded sample on the host cores.
+"""
+import argparse
+import concurrent.futures as cf
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gmap-gsnap_amd"))
+
+import numpy as np  # noqa: E402
+
+# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2 (packed int16) x 2.4 GHz.
+PEAK_INT16_TOPS = 78.6
+OPS_PER_CELL = 13  # algorithmic int16 ops per in-band cell (E 4, F 4, H 1+4)
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "c2": (2, 1_000_000, "single-gap rl=150 gl=U{160..200} extraband=U{5..20} int16, 1e6 problems"),
+    "c3": (3, 2_000_000, "single-gap rl=logU[20,1000] gl=rl*U(1,1.5) extraband=U{3..64} mixed int8/int16, 2e6 problems"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, sample_n, threads):
+    """Times the CPU DP (fill + traceback) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    probs = wl.problems[:sample_n]
+    qbuf, gbuf = wl.qbuf, wl.gbuf
+    cells = float(batch_cells(probs))
+    if os.path.exists(pyoracle.REF_SO):
+        L = ctypes.CDLL(pyoracle.REF_SO, mode=os.RTLD_LAZY)
+        L.refh_init.argtypes = [ctypes.c_int]
+        L.refh_init(0)
+        L.refh_run_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int]
+        L.refh_run_batch.restype = ctypes.c_longlong
+        maxrl, maxgl = int(probs["rl"].max()) + 8, int(probs["gl"].max()) + 8
+        run = lambda p: L.refh_run_batch(p.ctypes.data, len(p), qbuf.ctypes.data, gbuf.ctypes.data, 1, maxrl, maxgl)
+        kind = "reference"
+    else:
+        L = ctypes.CDLL(pyoracle.ORACLE_SO)
+        L.orc_init.argtypes = [ctypes.c_int]
+        L.orc_init(0)
+        L.orc_run_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.orc_run_batch.restype = ctypes.c_longlong
+        run = lambda p: L.orc_run_batch(p.ctypes.data, len(p), qbuf.ctypes.data, gbuf.ctypes.data, 1)
+        kind = "port"
+    parts = np.array_split(np.ascontiguousarray(probs), threads)
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        list(ex.map(run, parts))
+    dt = time.perf_counter() - t0
+    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": kind,
+            "sample": "%d problems of the same workload (first %d), fill + traceback, %.1f s" % (sample_n, sample_n, dt)}
+
+
+def batch_cells(probs):
+    from gmapdp.batch import cells
+    return int(cells(probs).sum())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--n", type=int, default=0, help="override the problem count")
+    ap.add_argument("--cpu-sample", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--percall", type=int, default=2000, help="problems timed one launch each (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (problems are independent; "
+              "see DESIGN.md for how batches would shard).  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from gmapdp.batch import DeviceBatch
+    from gmapdp.workload import Workload
+
+    config, n_default, desc = WORKLOADS[args.workload]
+    n = args.n or n_default
+    t0 = time.time()
+    wl = Workload(config, n)
+    probs = wl.problems
+    total_cells = batch_cells(probs)
+    log("workload %s: %d problems, %.3e cells, generated in %.1fs" % (args.workload, n, total_cells, time.time() - t0))
+
+    b = DeviceBatch(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes())
+    reqs = wl.tb_requests()
+    stream = torch.cuda.current_stream()
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        b.fill()
+        if ev is not None:
+            ev[1].record(stream)
+        b.traceback_device(reqs_dev)
+
+    reqs_dev = b.prepare_traceback(reqs, genome=wl.genome())
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    fill_ms = float(np.mean([a.elapsed_time(z) for a, z in evs]))
+    ms_per_step = dt * 1e3 / args.steps
+    gcups = total_cells * args.steps / dt / 1e9
+    achieved = total_cells * OPS_PER_CELL / (fill_ms * 1e-3) / 1e12
+    log("fill %.2f ms, step %.2f ms, %.1f GCUPS end-to-end, fill-only %.1f GCUPS"
+        % (fill_ms, ms_per_step, gcups, total_cells / (fill_ms * 1e-3) / 1e9))
+
+    # Plain one-launch-per-problem form on the same inputs.
+    percall = None
+    if args.percall:
+        m = min(args.percall, n)
+        sub = DeviceBatch(probs[:m], wl.qbuf.tobytes(), wl.gbuf.tobytes())
+        sreq = sub.prepare_traceback(wl.tb_requests()[:m], genome=wl.genome())
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for i in range(m):
+            sub.fill_one(i)
+            sub.traceback_one(sreq, i)
+            torch.cuda.synchronize()
+        dtp = time.perf_counter() - t1
+        percall = batch_cells(probs[:m]) / dtp / 1e9
+        log("per-call form: %d problems, %.3f GCUPS" % (m, percall))
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_fill")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        sample = args.cpu_sample or (400000 if args.workload == "c2" else 40000)
+        cpu = cpu_baseline(wl, min(sample, n), threads)
+        log("cpu baseline:", cpu)
+
+    out = {
+        "metric": "bit-exact GCUPS: batched persistent vs per-call GPU DP; serving p50/p99 ms",
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16",
+        "data": "synthetic (seeded generator of the workload)",
+        "config": {"workload": desc, "problems": n, "cells": total_cells, "step": "fill + traceback"},
+        "fill_gcups": round(total_cells / (fill_ms * 1e-3) / 1e9, 3),
+        "percall_gcups": None if percall is None else round(percall, 4),
+        "batched_vs_percall": None if percall is None else round(gcups / percall, 1),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
+                     "frac": round(achieved / PEAK_INT16_TOPS, 4), "traffic": traffic,
+                     "kernel": "gdp_fill_kernel", "ops_per_cell": OPS_PER_CELL},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,260 @@
+// gdp_gen.cpp -- synthetic workloads of the benchmark configurations.
+//
+// Every problem is generated from its own counter-based random stream
+// (splitmix64 of seed and index), so any slice [first, first+count) of a
+// workload is reproducible on its own and generation parallelises.
+//
+//   1  single-gap, 100-nt reads from a 2%-substituted, 0.2%-indel copy of a
+//      genomic span, extraband 10 (wide band), int16 HIGHQ or int8 LOWQ
+//   2  single-gap, rl 150, gl U{160..200}, extraband U{5..20}, int16 HIGHQ
+//   3  single-gap, rl log-uniform [20, 1000], gl = rl * U(1, 1.5),
+//      extraband U{3..64}, 10% of reads at 30% divergence, width by the
+//      reference's length rule (int8 when rl and gl < 41)
+//   4  end extensions (5' reversed / 3' forward halves), rl U{10..150},
+//      gl = rl + U{0..20}, 3% subs, 1% indels of 1-5, 1% N, ENDQ,
+//      extraband_end 6, int8 when rl or gl < 24; each yields an upper and a
+//      lower fill
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/gmapdp.h"
+#include "gdp_tables.h"
+
+namespace {
+
+struct Rng {
+  uint64_t s;
+  explicit Rng(uint64_t seed) : s(seed) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  }
+  double uni() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+  int range(int lo, int hi) { return lo + (int)(next() % (uint64_t)(hi - lo + 1)); }  // inclusive
+};
+
+const char kBases[4] = {'A', 'C', 'G', 'T'};
+
+// use8p_size (dynprog.c:1021-1024): -128 / mismatch - 1 = 41, 63, 127, 24.
+int use8p(int mismatchtype) {
+  static const int mm[4] = {-3, -2, -1, -5};
+  return -128 / mm[mismatchtype & 3] - 1;
+}
+
+uint64_t stream_seed(uint64_t seed, int64_t index) {
+  Rng r(seed ^ (0xd1b54a32d192ed03ull * (uint64_t)(index + 1)));
+  return r.next();
+}
+
+// Copies src with substitutions, indels and N's; stops at maxlen output bytes.
+int mutate(Rng &rng, const char *src, int srclen, char *out, int maxlen, double sub, double indel, int maxind,
+           double nfrac, int *consumed) {
+  int i = 0, o = 0;
+  while (i < srclen && o < maxlen) {
+    double u = rng.uni();
+    if (u < indel * 0.5) {  // insertion
+      int k = rng.range(1, maxind);
+      for (int j = 0; j < k && o < maxlen; j++) out[o++] = kBases[rng.next() & 3];
+      continue;
+    }
+    if (u < indel) {  // deletion
+      i += rng.range(1, maxind);
+      continue;
+    }
+    char ch = src[i++];
+    if (rng.uni() < sub) ch = kBases[rng.next() & 3];
+    if (nfrac > 0 && rng.uni() < nfrac) ch = 'N';
+    out[o++] = ch;
+  }
+  if (consumed) *consumed = i;
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Genome of iid uniform bases (deterministic in seed), ASCII.
+void gdp_gen_genome(uint64_t seed, char *out, int64_t length) {
+  const int nthreads = 16;
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; t++)
+    th.emplace_back([=]() {
+      int64_t lo = length * t / nthreads, hi = length * (t + 1) / nthreads;
+      for (int64_t b = lo & ~(int64_t)4095; b < hi; b += 4096) {
+        Rng r(stream_seed(seed ^ 0x6e6f6d65ull, b >> 12));
+        for (int64_t k = b; k < b + 4096 && k < hi; k++) {
+          uint64_t v = r.next();
+          if (k >= lo) out[k] = kBases[v & 3];
+        }
+      }
+    });
+  for (auto &t : th) t.join();
+}
+
+// Per-problem metadata the traceback needs (queryoffset etc.).
+typedef struct gdp_gen_meta {
+  int32_t queryoffset;
+  int32_t genomeoffset;
+  uint32_t chroffset;
+  uint32_t chrhigh;
+  int8_t watsonp;
+  int8_t revp;
+  int8_t pad[2];
+  int32_t pair;  // config 4: index of the partner fill (lower for an upper, -1 otherwise)
+} gdp_gen_meta;
+
+// Upper bound of query and private genome bytes one problem of a config uses.
+int32_t gdp_gen_max_bytes(int config) {
+  switch (config) {
+  case 1: return 100 + 200;
+  case 2: return 150 + 8;
+  case 3: return 1000 + 8;
+  case 4: return 2 * (150 + 170);
+  default: return 0;
+  }
+}
+
+// Generates problems [first, first+count) of a config.  Single-gap problems
+// read their genome window straight from `genome` (goff = window start); the
+// reversed genome windows of 5' end problems are written after the genome in
+// gbuf, starting at gbuf_extra (per-problem slots of gdp_gen_max_bytes).
+// Query bytes go to qbuf at qoff = (index-first) * gdp_gen_max_bytes.
+// Config 4 produces two fills per index (upper at 2k, lower at 2k+1).
+// variant selects config 1's int8 LOWQ run (1) instead of int16 HIGHQ (0).
+int gdp_gen_workload(int config, int variant, uint64_t seed, int64_t first, int64_t count, const char *genome,
+                     int64_t genome_len, gdp_problem *probs, gdp_gen_meta *meta, char *qbuf, char *gbuf,
+                     int64_t gbuf_extra) {
+  if (config < 1 || config > 4) return -1;
+  const int stride = gdp_gen_max_bytes(config);
+  const int nthreads = 16;
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; t++)
+    th.emplace_back([=]() {
+      std::vector<char> tmp(4096), src(4096);
+      for (int64_t k = count * t / nthreads; k < count * (t + 1) / nthreads; k++) {
+        const int64_t idx = first + k;
+        Rng rng(stream_seed(seed, idx));
+        char *q = qbuf + k * stride;
+        gdp_problem p;
+        memset(&p, 0, sizeof p);
+        gdp_gen_meta m;
+        memset(&m, 0, sizeof m);
+        m.watsonp = 1;
+        m.chroffset = 0;
+        m.chrhigh = (uint32_t)genome_len;
+        m.pair = -1;
+        p.qoff = k * stride;
+        if (config == 1) {
+          const int64_t pos = (int64_t)(rng.uni() * (double)(genome_len - 400));
+          int consumed = 0;
+          int rl = mutate(rng, genome + pos, 300, q, 100, 0.02, 0.002, 3, 0.0, &consumed);
+          p.rl = rl;
+          p.gl = consumed;
+          p.goff = p.galtoff = pos;
+          int lb, ub;
+          gdp_compute_bands(&lb, &ub, p.rl, p.gl, 10, 1);
+          p.lband = (int16_t)lb;
+          p.uband = (int16_t)ub;
+          p.kind = GDP_KIND_FULL;
+          if (variant == 0) {
+            p.mismatchtype = GDP_HIGHQ; p.open = -8; p.extend = -3;
+          } else {
+            p.mismatchtype = GDP_LOWQ; p.open = -6; p.extend = -1;
+          }
+          const int u8 = use8p(p.mismatchtype);
+          p.width = (p.rl < u8 && p.gl < u8) ? 8 : 16;
+          p.jump_late = (int8_t)(rng.next() & 1);
+          m.genomeoffset = (int32_t)pos;
+        } else if (config == 2 || config == 3) {
+          int rl, gl, eb;
+          double div = 0.02;
+          if (config == 2) {
+            rl = 150;
+            gl = rng.range(160, 200);
+            eb = rng.range(5, 20);
+          } else {
+            rl = (int)std::lround(std::exp(std::log(20.0) + rng.uni() * (std::log(1000.0) - std::log(20.0))));
+            gl = (int)std::lround(rl * (1.0 + 0.5 * rng.uni()));
+            eb = rng.range(3, 64);
+            if (rng.uni() < 0.10) div = 0.30;
+          }
+          const int64_t pos = (int64_t)(rng.uni() * (double)(genome_len - 2 * gl - 8));
+          // The read skips gl - rl genome bases at one point (a single gap),
+          // then carries substitutions and small indels.
+          const int skip = gl - rl, a = rng.range(0, rl);
+          memcpy(src.data(), genome + pos, a);
+          memcpy(src.data() + a, genome + pos + a + skip, rl - a);
+          int got = mutate(rng, src.data(), rl, q, rl, div, 0.002, 3, 0.0, nullptr);
+          for (int j = got; j < rl; j++) q[j] = kBases[rng.next() & 3];
+          p.rl = rl;
+          p.gl = gl;
+          p.goff = p.galtoff = pos;
+          p.lband = (int16_t)eb;
+          p.uband = (int16_t)(gl - rl + eb);
+          p.kind = GDP_KIND_FULL;
+          p.mismatchtype = GDP_HIGHQ;
+          p.open = -8;
+          p.extend = -3;
+          const int u8 = use8p(GDP_HIGHQ);
+          p.width = (config == 3 && rl < u8 && gl < u8) ? 8 : 16;
+          p.jump_late = (int8_t)(rng.next() & 1);
+          m.genomeoffset = (int32_t)pos;
+        } else {
+          // End extension: the read end follows the genome from the anchor
+          // outward; 5' ends are swept on reversed copies (revp).
+          const int rl = rng.range(10, 150), gl = rl + rng.range(0, 20);
+          const bool five = (rng.next() & 1) != 0;
+          const int64_t pos = 64 + (int64_t)(rng.uni() * (double)(genome_len - 2 * gl - 128));
+          char *gx = gbuf + gbuf_extra + k * stride;
+          for (int j = 0; j < gl; j++) gx[j] = five ? genome[pos + gl - 1 - j] : genome[pos + j];
+          int got = mutate(rng, gx, gl, q, rl, 0.03, 0.01, 5, 0.01, nullptr);
+          for (int j = got; j < rl; j++) q[j] = kBases[rng.next() & 3];
+          int lb, ub;
+          gdp_compute_bands(&lb, &ub, rl, gl, 6, 1);
+          const bool u8 = rl < use8p(GDP_ENDQ) || gl < use8p(GDP_ENDQ);
+          const bool late = (rng.next() & 1) != 0;
+          p.rl = rl;
+          p.gl = gl;
+          p.goff = p.galtoff = gbuf_extra + k * stride;
+          p.mismatchtype = GDP_ENDQ;
+          p.open = -10;
+          p.extend = -2;
+          p.width = u8 ? 8 : 16;
+          // 5' callers hand the reversed sequences and the flipped tie rule
+          // to the kernels (dynprog_end.c:1415-1451).
+          p.jump_late = (int8_t)(five ? !late : late);
+          p.keep_scores = 1;
+          m.revp = five ? 1 : 0;
+          m.genomeoffset = five ? (int32_t)(pos + gl - 1) : (int32_t)pos;
+          m.queryoffset = five ? rl - 1 : 0;
+          gdp_problem up = p, lo = p;
+          up.kind = GDP_KIND_UPPER;
+          up.uband = (int16_t)ub;
+          up.lband = (int16_t)lb;
+          lo.kind = GDP_KIND_LOWER;
+          lo.uband = (int16_t)ub;
+          lo.lband = (int16_t)lb;
+          probs[2 * k] = up;
+          probs[2 * k + 1] = lo;
+          m.pair = (int32_t)(2 * k + 1);
+          meta[2 * k] = m;
+          gdp_gen_meta ml = m;
+          ml.pair = (int32_t)(2 * k);
+          meta[2 * k + 1] = ml;
+          continue;
+        }
+        probs[k] = p;
+        meta[k] = m;
+      }
+    });
+  for (auto &t : th) t.join();
+  return 0;
+}
+
+}  // extern "C"

@@ -133,3 +133,42 @@ class DeviceBatch:
                                                 self.d_scores.data_ptr(), d_out.data_ptr(), self.stream),
                "gdp_best_endpoint")
         return d_out.cpu().numpy()[: 3 * m].reshape(-1, 3)
+
+    # -- device-resident traceback (timed paths) ---------------------------
+    def prepare_traceback(self, reqs, genome=None):
+        """Uploads requests and allocates outputs once; returns a handle."""
+        torch = self.torch
+        reqs = np.ascontiguousarray(reqs, TBREQ_DTYPE)
+        npairs = int((reqs["pair_off"].astype(np.int64) + reqs["max_pairs"]).max()) if len(reqs) else 1
+        h = {"n": len(reqs), "reqs": reqs,
+             "d_reqs": torch.from_numpy(reqs.view(np.uint8).copy()).to(self.dev),
+             "d_pairs": torch.zeros(max(npairs, 1) * PAIR_DTYPE.itemsize, dtype=torch.uint8, device=self.dev),
+             "d_counts": torch.zeros(max(len(reqs), 1) * 6, dtype=torch.int32, device=self.dev),
+             "d_genome": None, "glen": 0}
+        if genome is not None:
+            h["d_genome"] = torch.from_numpy(np.ascontiguousarray(genome, np.uint8)).to(self.dev)
+            h["glen"] = len(genome)
+        return h
+
+    def traceback_device(self, h, first=0, count=None):
+        count = h["n"] - first if count is None else count
+        gptr = h["d_genome"].data_ptr() if h["d_genome"] is not None else None
+        _check(self.L, self.L.gdp_traceback(self.d_descs.data_ptr(),
+                                            h["d_reqs"].data_ptr() + first * TBREQ_DTYPE.itemsize, count,
+                                            self.d_q.data_ptr(), self.d_g.data_ptr(), self.d_dirs.data_ptr(),
+                                            gptr, h["glen"], h["d_pairs"].data_ptr(),
+                                            h["d_counts"].data_ptr() + first * 24, self.stream), "gdp_traceback")
+
+    def traceback_results(self, h):
+        counts = h["d_counts"].cpu().numpy()[: 6 * h["n"]].reshape(-1, 6)
+        return counts
+
+    def fill_one(self, i):
+        """One problem per launch (the per-call form)."""
+        desc = self.d_descs.data_ptr() + i * self.L.gdp_descriptor_bytes()
+        _check(self.L, self.L.gdp_fill(desc, 1, self.d_q.data_ptr(), self.d_g.data_ptr(), self.d_dirs.data_ptr(),
+                                       self.d_scores.data_ptr(), self.d_final.data_ptr() + 4 * i, self.stream),
+               "gdp_fill")
+
+    def traceback_one(self, h, i):
+        self.traceback_device(h, i, 1)

@@ -1,0 +1,78 @@
+"""Synthetic workloads of the benchmark configurations (generated by the
+library's C++ generator, gdp_gen.cpp)."""
+import ctypes
+
+import numpy as np
+
+from .lib import PROBLEM_DTYPE, TBREQ_DTYPE, load
+
+META_DTYPE = np.dtype([("queryoffset", "<i4"), ("genomeoffset", "<i4"), ("chroffset", "<u4"), ("chrhigh", "<u4"),
+                       ("watsonp", "i1"), ("revp", "i1"), ("pad", "i1", (2,)), ("pair", "<i4")])
+assert META_DTYPE.itemsize == 24
+
+SEEDS = {1: 1001, 2: 1002, 3: 1003, 4: 1004, 5: 1005}
+
+
+def _bind(L):
+    if getattr(L, "_gen_bound", False):
+        return L
+    L.gdp_gen_genome.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int64]
+    L.gdp_gen_genome.restype = None
+    L.gdp_gen_max_bytes.argtypes = [ctypes.c_int]
+    L.gdp_gen_max_bytes.restype = ctypes.c_int32
+    L.gdp_gen_workload.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,
+                                   ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    L.gdp_gen_workload.restype = ctypes.c_int
+    L._gen_bound = True
+    return L
+
+
+class Workload:
+    """problems (PROBLEM_DTYPE), meta (META_DTYPE), qbuf and gbuf (uint8) of
+    one configuration; gbuf starts with the genome."""
+
+    def __init__(self, config, count, genome_len=1 << 24, variant=0, seed=None, first=0):
+        L = _bind(load())
+        self.config = config
+        seed = SEEDS[config] if seed is None else seed
+        stride = L.gdp_gen_max_bytes(config)
+        nprob = 2 * count if config == 4 else count
+        self.genome_len = genome_len
+        extra = count * stride if config == 4 else 0
+        self.gbuf = np.zeros(genome_len + extra + 8, np.uint8)
+        L.gdp_gen_genome(seed, self.gbuf.ctypes.data, genome_len)
+        self.qbuf = np.zeros(count * stride + 8, np.uint8)
+        self.problems = np.zeros(nprob, PROBLEM_DTYPE)
+        self.meta = np.zeros(nprob, META_DTYPE)
+        rc = L.gdp_gen_workload(config, variant, seed, first, count, self.gbuf.ctypes.data, genome_len,
+                                self.problems.ctypes.data, self.meta.ctypes.data, self.qbuf.ctypes.data,
+                                self.gbuf.ctypes.data, genome_len)
+        if rc != 0:
+            raise RuntimeError("gdp_gen_workload failed")
+
+    def genome(self):
+        return self.gbuf[: self.genome_len]
+
+    def tb_requests(self, start_cells=None):
+        """Traceback requests: full-band problems from (rl, gl); for end
+        problems pass start_cells[k] = (fill index, r, c)."""
+        p, m = self.problems, self.meta
+        if start_cells is None:
+            idx = np.arange(len(p))
+            r, c = p["rl"], p["gl"]
+        else:
+            idx, r, c = start_cells[:, 0], start_cells[:, 1], start_cells[:, 2]
+        req = np.zeros(len(idx), TBREQ_DTYPE)
+        req["fill"], req["r"], req["c"] = idx, r, c
+        req["queryoffset"] = m["queryoffset"][idx]
+        req["genomeoffset"] = m["genomeoffset"][idx]
+        req["chroffset"] = m["chroffset"][idx]
+        req["chrhigh"] = m["chrhigh"][idx]
+        req["watsonp"] = m["watsonp"][idx]
+        req["revp"] = m["revp"][idx]
+        req["max_pairs"] = p["rl"][idx] + p["gl"][idx] + 4
+        off = np.zeros(len(idx), np.int64)
+        off[1:] = np.cumsum(req["max_pairs"].astype(np.int64))[:-1]
+        req["pair_off"] = off
+        return req

@@ -660,3 +660,41 @@ long long orc_cells_lower(int rl, int gl, int lband) {
   }
   return n;
 }
+
+/* ------------------------------------------------- batch timing (baseline) */
+
+/* Problem record layout of the batched interface (include/gmapdp.h). */
+typedef struct {
+  long long qoff, goff, galtoff;
+  int rl, gl;
+  short lband, uband;
+  signed char kind, width, mismatchtype, jump_late, open, extend, keep_scores, pad;
+  int pad2;
+} orc_problem_t;
+
+long long orc_run_batch(const void *problems, int n, const char *qbuf, const char *gbuf, int do_traceback) {
+  const orc_problem_t *pr = (const orc_problem_t *)problems;
+  long long check = 0;
+  for (int i = 0; i < n; i++) {
+    const orc_problem_t *p = &pr[i];
+    int rl = p->rl, gl = p->gl;
+    size_t nc = (size_t)(rl + 1) * (gl + 1);
+    short *H = malloc(nc * sizeof(short));
+    unsigned char *D = calloc(nc, 1);
+    const char *q = qbuf + p->qoff, *g = gbuf + p->goff, *ga = gbuf + p->galtoff;
+    orc_fill_full(p->width, q, g, ga, rl, gl, p->mismatchtype, p->open, p->extend, p->lband, p->uband,
+                  p->jump_late, 0, H, D);
+    check += H[(size_t)gl * (rl + 1) + rl];
+    if (do_traceback) {
+      int counters[5] = {0, 0, 0, 0, 0};
+      int *pairs = malloc(sizeof(int) * ORC_PAIR_FIELDS * (size_t)(rl + gl + 4));
+      orc_traceback(ORC_KIND_FULL, D, rl, gl, rl, gl, q, q, g, ga, 0, 0, NULL, 0, 0, 0, 1, 0, 0, counters, pairs,
+                    rl + gl + 4);
+      check += counters[0];
+      free(pairs);
+    }
+    free(D);
+    free(H);
+  }
+  return check;
+}

@@ -205,3 +205,51 @@ int refh_traceback(int kind, int width, int r, int c, char *rseq, char *rsequc, 
   }
   return n;
 }
+
+/* ------------------------------------------------- batch timing (baseline) */
+
+typedef struct {
+  long long qoff, goff, galtoff;
+  int rl, gl;
+  short lband, uband;
+  signed char kind, width, mismatchtype, jump_late, open, extend, keep_scores, pad;
+  int pad2;
+} refh_problem_t;
+
+/* Runs full-band problems the way Dynprog_single_gap does (fill, read the
+   corner score, traceback), on a private workspace and pair pool so several
+   threads can time disjoint slices.  Genome skips resolve to '*' (chrhigh 0). */
+long long refh_run_batch(const void *problems, int n, const char *qbuf, const char *gbuf, int do_traceback,
+                         int maxrl, int maxgl) {
+  const refh_problem_t *pr = (const refh_problem_t *)problems;
+  Dynprog_T ws = (Dynprog_T)refh_new(maxrl, maxgl, 0);
+  Pairpool_T pp = Pairpool_new();
+  long long check = 0;
+  for (int i = 0; i < n; i++) {
+    const refh_problem_t *p = &pr[i];
+    char *q = (char *)qbuf + p->qoff, *g = (char *)gbuf + p->goff, *ga = (char *)gbuf + p->galtoff;
+    int sc = 0, nm = 0, nmm = 0, no = 0, ni = 0;
+    Pairpool_reset(pp);
+    if (p->width == 8) {
+      Direction8_T **dn, **de, **df;
+      Score8_T **m = Dynprog_simd_8(&dn, &de, &df, ws, q, g, ga, p->rl, p->gl, (Mismatchtype_T)p->mismatchtype,
+                                    p->open, p->extend, p->lband, p->uband, p->jump_late ? true : false, false);
+      check += m[p->gl][p->rl];
+      if (do_traceback)
+        Dynprog_traceback_8(NULL, &sc, &nm, &nmm, &no, &ni, dn, de, df, p->rl, p->gl, q, q, g, ga, 0, 0, NULL, NULL,
+                            pp, false, 0, 0, true, 0, 0);
+    } else {
+      Direction16_T **dn, **de, **df;
+      Score16_T **m = Dynprog_simd_16(&dn, &de, &df, ws, q, g, ga, p->rl, p->gl, (Mismatchtype_T)p->mismatchtype,
+                                      p->open, p->extend, p->lband, p->uband, p->jump_late ? true : false, false);
+      check += m[p->gl][p->rl];
+      if (do_traceback)
+        Dynprog_traceback_16(NULL, &sc, &nm, &nmm, &no, &ni, dn, de, df, p->rl, p->gl, q, q, g, ga, 0, 0, NULL,
+                             NULL, pp, false, 0, 0, true, 0, 0);
+    }
+    check += sc;
+  }
+  Pairpool_free(&pp);
+  Dynprog_free(&ws);
+  return check;
+}
