@@ -1,0 +1,51 @@
+"""Reader of tests/golden/golden_dp.npz (reference outputs, see make_golden.py)."""
+import hashlib
+import os
+
+import numpy as np
+
+import helpers as H
+
+PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden_dp.npz")
+
+
+def load():
+    z = np.load(PATH, allow_pickle=False)
+    g = {k: z[k] for k in z.files}
+    specs = []
+    for i in range(len(g["kind"])):
+        qo, go, gao = int(g["qoff"][i]), int(g["goff"][i]), int(g["galtoff"][i])
+        rl, gl = int(g["rl"][i]), int(g["gl"][i])
+        q = bytes(g["qbuf"][qo:qo + rl])
+        gg = bytes(g["gbuf"][go:go + gl])
+        ga = bytes(g["gbuf"][gao:gao + gl])
+        specs.append(H.Spec(int(g["kind"][i]), int(g["width"][i]), q, gg, ga, int(g["mt"][i]), int(g["open"][i]),
+                            int(g["ext"][i]), int(g["lband"][i]), int(g["uband"][i]), bool(g["late"][i]),
+                            bool(g["revp"][i])))
+    return g, specs
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def check_fill(g, i, s, D, Hs):
+    """D, Hs: dense [col][row] directions and scores of problem i (oracle layout)."""
+    B = H.band_mask(s)
+    off = int(g["dense_off"][i])
+    if off >= 0:
+        n = int(B.sum())
+        assert (D[B] == g["dense_dirs"][off:off + n]).all(), ("golden directions", i)
+        assert (Hs[B] == g["dense_scores"][off:off + n]).all(), ("golden scores", i)
+    else:
+        assert digest(D[B].astype(np.uint8)) + digest(Hs[B].astype(np.int16)) == str(g["digests"][i]), ("golden digest", i)
+    if g["final"][i] != -(1 << 31):
+        assert int(Hs[s.gl, s.rl]) == int(g["final"][i]), ("golden corner score", i)
+
+
+def traceback_expect(g, i):
+    n = int(g["tb_n"][i])
+    if g["tb_r"][i] < 0:
+        return None
+    off = int(g["tb_off"][i])
+    return int(g["tb_r"][i]), int(g["tb_c"][i]), g["tb_pairs"][off:off + n], g["tb_counts"][i]

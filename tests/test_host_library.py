@@ -1,0 +1,136 @@
+"""Host side of libgmapdp.so: exported C ABI, score tables, planning,
+workload generator.  No GPU calls."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import helpers as H
+from gmapdp import lib
+from gmapdp.lib import PROBLEM_DTYPE
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gdp_[a-z0-9_]+|Dynprog_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def L():
+    return lib.load()
+
+
+def test_library_exports_every_declared_symbol(L):
+    for header in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not header.endswith(".h"):
+            continue
+        names = _declared(header)
+        assert names, header
+        for name in names:
+            target = L if header == "gmapdp.h" else ctypes.CDLL(lib.library_path().replace("libgmapdp", "libgmapdp_dynprog"),
+                                                                mode=os.RTLD_LAZY)
+            assert hasattr(target, name), "%s declares %s but the library does not export it" % (header, name)
+
+
+def test_struct_sizes(L):
+    assert L.gdp_descriptor_bytes() == 64
+    assert PROBLEM_DTYPE.itemsize == ctypes.sizeof(lib.Problem) == 48
+    assert ctypes.sizeof(lib.TbRequest) == 48 and ctypes.sizeof(lib.Pair) == 16
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5, 6])
+def test_tables_match_oracle(L, mode):
+    import pyoracle
+    from gmapdp.lib import load
+    O = pyoracle.Oracle(mode)
+    L2 = load()
+    # gdp_tables_build runs inside gdp_init, which needs a device; build the
+    # host tables directly through the table entry used by gdp_init.
+    fn = getattr(L2, "_Z16gdp_tables_buildi")
+    fn.argtypes = [ctypes.c_int]
+    fn(mode)
+    for mt in range(4):
+        got = np.ctypeslib.as_array(L2.gdp_pairdistance(mt), shape=(128 * 128,)).reshape(128, 128)
+        assert (got == O.pairdistance(mt)).all()
+        assert L2.gdp_use8p_size(mt) == O.use8p(mt)
+    assert (np.ctypeslib.as_array(L2.gdp_nt_to_int(), shape=(128,)) == O.nt_to_int()).all()
+    for gs in range(3):
+        for c in range(0, 128, 5):
+            for g in range(128):
+                assert L2.gdp_consistent(gs, c, g) == O.consistent(gs, c, g)
+
+
+def test_compute_bands(L):
+    lb, ub = ctypes.c_int32(), ctypes.c_int32()
+    for rl, gl, eb, wide, want in [(100, 120, 10, 1, (10, 30)), (120, 100, 10, 1, (30, 10)), (5, 9, 3, 0, (3, 3))]:
+        L.gdp_compute_bands(ctypes.byref(lb), ctypes.byref(ub), rl, gl, eb, wide)
+        assert (lb.value, ub.value) == want
+
+
+def test_plan_validates_and_sizes(L):
+    from gmapdp.batch import plan
+    specs = H.random_specs(9, 30)
+    probs, q, g = H.pack_batch(specs, keep_scores=True)
+    descs, dw, ss = plan(probs)
+    assert dw > 0 and ss > 0
+    bad = probs.copy()
+    bad["width"][3] = 12
+    with pytest.raises(RuntimeError, match="width"):
+        plan(bad)
+    bad = probs.copy()
+    bad["lband"][0], bad["uband"][0], bad["kind"][0] = 700, 700, 0
+    with pytest.raises(RuntimeError, match="span"):
+        plan(bad)
+    bad = probs.copy()
+    bad["open"][1] = 5
+    with pytest.raises(RuntimeError, match="penalties"):
+        plan(bad)
+
+
+def test_cells_match_oracle(L, oracle):
+    from gmapdp.batch import cells
+    specs = H.random_specs(10, 60)
+    probs, _, _ = H.pack_batch(specs)
+    got = cells(probs)
+    for k, s in enumerate(specs):
+        want = {0: lambda: oracle.cells_full(s.rl, s.gl, s.lband, s.uband),
+                1: lambda: oracle.cells_upper(s.rl, s.gl, s.uband),
+                2: lambda: oracle.cells_lower(s.rl, s.gl, s.lband)}[s.kind]()
+        assert got[k] == want
+
+
+def test_workload_generator_is_deterministic_and_sliceable():
+    from gmapdp.workload import Workload
+    a = Workload(3, 400, genome_len=1 << 20)
+    b = Workload(3, 100, genome_len=1 << 20, first=300)
+    for k in range(100):
+        pa, pb = a.problems[300 + k], b.problems[k]
+        for f in ("rl", "gl", "lband", "uband", "width", "jump_late", "goff"):
+            assert pa[f] == pb[f]
+        qa = a.qbuf[pa["qoff"]: pa["qoff"] + pa["rl"]]
+        qb = b.qbuf[pb["qoff"]: pb["qoff"] + pb["rl"]]
+        assert (qa == qb).all()
+
+
+def test_workload_parameter_ranges():
+    from gmapdp.workload import Workload
+    w = Workload(2, 3000, genome_len=1 << 20).problems
+    assert (w["rl"] == 150).all() and w["gl"].min() >= 160 and w["gl"].max() <= 200
+    assert w["lband"].min() >= 5 and w["lband"].max() <= 20 and (w["uband"] == w["gl"] - 150 + w["lband"]).all()
+    assert (w["width"] == 16).all()
+    w3 = Workload(3, 20000, genome_len=1 << 22).problems
+    assert w3["rl"].min() >= 20 and w3["rl"].max() <= 1000
+    r = w3["gl"] / w3["rl"]
+    assert r.min() >= 0.99 and r.max() <= 1.51
+    small = (w3["rl"] < 41) & (w3["gl"] < 41)
+    assert ((w3["width"] == 8) == small).all()
+    w4 = Workload(4, 2000, genome_len=1 << 20)
+    p = w4.problems
+    assert (p["kind"][0::2] == 1).all() and (p["kind"][1::2] == 2).all()
+    assert ((p["width"] == 8) == ((p["rl"] < 24) | (p["gl"] < 24))).all()
+    assert (p["mismatchtype"] == 3).all()
