@@ -21,6 +21,8 @@ extern "C" int gdp_launch_best_endpoint(const void *, const int32_t *, int, int,
                                         hipStream_t);
 extern "C" int gdp_launch_export(const void *, int, const uint32_t *, const int16_t *, uint8_t *, int16_t *,
                                  hipStream_t);
+extern "C" int gdp_launch_export_ref(const void *, int, int, const uint32_t *, const int16_t *, int, void *, void *,
+                                     void *, void *, hipStream_t);
 extern "C" int gdp_upload_tables(const uint32_t *, const uint32_t *, const int8_t *, const uint8_t *);
 
 namespace {
@@ -46,9 +48,7 @@ int hip_fail(const char *what, hipError_t e) {
 
 extern "C" const char *gdp_last_error(void) { return g_err.c_str(); }
 
-extern "C" int gdp_init(int mode) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  gdp_tables_build(mode);
+static int upload_current_tables() {
   uint32_t PQ[4 * 128], PG[4 * 128];
   int8_t NT[128];
   gdp_tables_pack(PQ, PG, NT);
@@ -59,6 +59,24 @@ extern "C" int gdp_init(int mode) {
     return fail(GDP_EHIP, "gdp_init: table upload failed");
   g_init = true;
   return GDP_OK;
+}
+
+extern "C" int gdp_init(int mode) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  gdp_tables_build(mode);
+  return upload_current_tables();
+}
+
+extern "C" int gdp_init_tables(const int16_t *pd, const int32_t *nt, const uint8_t *cons) {
+  if (!pd || !nt || !cons) return fail(GDP_EINVAL, "gdp_init_tables: null table");
+  std::lock_guard<std::mutex> lk(g_mu);
+  gdp_tables_set(pd, nt, cons);
+  return upload_current_tables();
+}
+
+extern "C" void gdp_host_tables(int mode) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  gdp_tables_build(mode);
 }
 
 extern "C" int32_t gdp_descriptor_bytes(void) { return (int32_t)sizeof(gdp_dev_problem); }
@@ -177,5 +195,22 @@ extern "C" int gdp_export_dense(const void *d_descs, int32_t idx, const uint32_t
   if (!d_descs || !d_dirs || !d_dense_dirs) return fail(GDP_EINVAL, "gdp_export_dense: null argument");
   if (gdp_launch_export(d_descs, idx, d_dirs, d_scores, d_dense_dirs, d_dense_scores, (hipStream_t)stream))
     return hip_fail("gdp_export_dense launch", hipGetLastError());
+  return GDP_OK;
+}
+
+extern "C" int gdp_export_reference(const void *d_descs, int32_t idx, const uint32_t *d_dirs, const int16_t *d_scores,
+                                    int32_t stride, void *d_score, void *d_nogap, void *d_egap, void *d_fgap,
+                                    void *stream) {
+  if (!g_init) return fail(GDP_ENOINIT, "gdp_export_reference: gdp_init has not been called");
+  if (!d_descs || !d_dirs || !d_score || !d_nogap || !d_egap || stride <= 0)
+    return fail(GDP_EINVAL, "gdp_export_reference: bad argument");
+  gdp_dev_problem p;
+  if (hipMemcpy(&p, (const gdp_dev_problem *)d_descs + idx, sizeof p, hipMemcpyDeviceToHost) != hipSuccess)
+    return hip_fail("gdp_export_reference descriptor read", hipGetLastError());
+  gdp_geom G = gdp_make_geom(p);
+  if (stride < G.R + 1) return fail(GDP_EINVAL, "gdp_export_reference: stride %d < %d rows", stride, G.R + 1);
+  if (gdp_launch_export_ref(d_descs, idx, p.width, d_dirs, d_scores, stride, d_score, d_nogap, d_egap, d_fgap,
+                            (hipStream_t)stream))
+    return hip_fail("gdp_export_reference launch", hipGetLastError());
   return GDP_OK;
 }

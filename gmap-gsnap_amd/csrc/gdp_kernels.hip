@@ -271,8 +271,28 @@ __global__ void gdp_traceback_kernel(const gdp_dev_problem *__restrict__ probs, 
   auto dir = [&](int rr, int cc) -> uint32_t { return lower ? nib_at(dirs, p, G, cc, rr) : nib_at(dirs, p, G, rr, cc); };
   auto qch = [&](int rr) -> int { return qbuf[p.qoff + rr - 1]; };
 
+  const bool ops = (q.flags & GDP_TB_OPS) != 0;
+  auto push_op = [&](int type, int rr, int cc, int arg, int comp) {
+    if (out.n < out.cap) {
+      gdp_pair pr;
+      pr.querypos = rr;
+      pr.genomepos = cc;
+      pr.cdna = (int8_t)type;
+      pr.comp = (int8_t)comp;
+      pr.genome = 0;
+      pr.genomealt = 0;
+      pr.gapp = 0;
+      pr.genomejump = (int16_t)arg;
+      out.pairs[out.n] = pr;
+    }
+    out.n++;
+  };
   // Pairpool_add_queryskip (pairpool.c:980-1010).
   auto queryskip = [&](int rr, int cc, int dist) {
+    if (ops) {
+      push_op('V', rr, cc, dist, 0);
+      return;
+    }
     int qc = rr - 1, gc = cc - 1, step = -1;
     if (revp) {
       qc = -qc;
@@ -294,6 +314,10 @@ __global__ void gdp_traceback_kernel(const gdp_dev_problem *__restrict__ probs, 
       left = -right;
       right = -t;
       step = +1;
+    }
+    if (ops) {
+      push_op('H', rr, cc, dist, 0);
+      return dist < MICROINTRON_LENGTH;
     }
     if (dist < MICROINTRON_LENGTH) {
       int gc = revp ? left : right;
@@ -344,7 +368,8 @@ __global__ void gdp_traceback_kernel(const gdp_dev_problem *__restrict__ probs, 
       sc += TB_MATCH;
       nm += 1;
     }
-    push_pair(out, q.queryoffset + qc, q.genomeoffset + gc, c1, comp, c2, c2a);
+    if (ops) push_op('D', rr, cc, 0, comp);
+    else push_pair(out, q.queryoffset + qc, q.genomeoffset + gc, c1, comp, c2, c2a);
   };
 
   if (p.kind == GDP_KIND_FULL) {
@@ -497,7 +522,41 @@ __global__ void gdp_export_kernel(const gdp_dev_problem *__restrict__ probs, int
   }
 }
 
+// Reference workspace layout and encoding (dynprog_simd.c:2158-2282).
+template <typename T>
+__global__ void gdp_export_ref_kernel(const gdp_dev_problem *__restrict__ probs, int idx,
+                                      const uint32_t *__restrict__ dirs, const int16_t *__restrict__ scores, int stride,
+                                      T *__restrict__ sc, T *__restrict__ nogap, T *__restrict__ egap,
+                                      T *__restrict__ fgap) {
+  const gdp_dev_problem p = probs[idx];
+  const gdp_geom G = gdp_make_geom(p);
+  const int64_t ncell = (int64_t)(G.C + 1) * stride;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ncell; k += (int64_t)gridDim.x * blockDim.x) {
+    const int col = (int)(k / stride), row = (int)(k % stride);
+    const bool comp = row <= G.R && col >= gdp_cstart(G, row) && col <= gdp_cend(G, row);
+    const uint32_t nib = comp ? nib_at(dirs, p, G, row, col) : 0;
+    sc[k] = (T)((comp && p.score_off >= 0) ? score_at(scores, p, G, row, col) : 0);
+    nogap[k] = (T)((nib & GDP_DIR_V) ? -2 : ((nib & GDP_DIR_H) ? -1 : 0));
+    egap[k] = (T)((nib & GDP_DIR_E) ? -1 : 0);
+    if (fgap != nullptr) fgap[k] = (T)((nib & GDP_DIR_F) ? -2 : 0);
+  }
+}
+
 // ------------------------------------------------------------ host launchers
+
+extern "C" int gdp_launch_export_ref(const void *d_descs, int idx, int width, const uint32_t *d_dirs,
+                                     const int16_t *d_scores, int stride, void *sc, void *nogap, void *egap,
+                                     void *fgap, hipStream_t stream) {
+  if (width == 8)
+    hipLaunchKernelGGL(gdp_export_ref_kernel<int8_t>, dim3(64), dim3(256), 0, stream,
+                       (const gdp_dev_problem *)d_descs, idx, d_dirs, d_scores, stride, (int8_t *)sc, (int8_t *)nogap,
+                       (int8_t *)egap, (int8_t *)fgap);
+  else
+    hipLaunchKernelGGL(gdp_export_ref_kernel<int16_t>, dim3(64), dim3(256), 0, stream,
+                       (const gdp_dev_problem *)d_descs, idx, d_dirs, d_scores, stride, (int16_t *)sc,
+                       (int16_t *)nogap, (int16_t *)egap, (int16_t *)fgap);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 extern "C" int gdp_launch_fill(const void *d_descs, int n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
                                int16_t *d_scores, int32_t *d_final, hipStream_t stream) {

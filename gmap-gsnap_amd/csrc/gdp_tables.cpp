@@ -91,6 +91,13 @@ void gdp_tables_pack(uint32_t *PQ, uint32_t *PG, int8_t *NT) {
 
 const uint8_t *gdp_tables_consistent(void) { return &cons[0][0][0]; }
 
+void gdp_tables_set(const int16_t *pd_in, const int32_t *nt_in, const uint8_t *cons_in) {
+  memcpy(pd, pd_in, sizeof pd);
+  memcpy(nt2int, nt_in, sizeof nt2int);
+  memcpy(cons, cons_in, sizeof cons);
+  for (int t = 0; t < 4; t++) use8p[t] = -128 / kMismatch[t] - 1;
+}
+
 extern "C" const int16_t *gdp_pairdistance(int mismatchtype) { return &pd[mismatchtype & 3][0][0]; }
 extern "C" const int32_t *gdp_nt_to_int(void) { return nt2int; }
 extern "C" int32_t gdp_use8p_size(int mismatchtype) { return use8p[mismatchtype & 3]; }

@@ -9,6 +9,8 @@ void gdp_tables_build(int mode);
 void gdp_tables_pack(uint32_t *PQ, uint32_t *PG, int8_t *NT);
 // consistent_array[genestrand][c][g] as 3*128*128 bytes.
 const uint8_t *gdp_tables_consistent(void);
+// Installs tables built elsewhere (the host program's Dynprog_init).
+void gdp_tables_set(const int16_t *pd, const int32_t *nt, const uint8_t *cons);
 
 extern "C" {
 // Read-only views for callers and tests.

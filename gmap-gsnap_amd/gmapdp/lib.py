@@ -27,7 +27,7 @@ class TbRequest(ctypes.Structure):
     _fields_ = [("fill", ctypes.c_int32), ("r", ctypes.c_int32), ("c", ctypes.c_int32),
                 ("queryoffset", ctypes.c_int32), ("genomeoffset", ctypes.c_int32),
                 ("chroffset", ctypes.c_uint32), ("chrhigh", ctypes.c_uint32), ("watsonp", ctypes.c_int8),
-                ("revp", ctypes.c_int8), ("genestrand", ctypes.c_int8), ("pad", ctypes.c_int8),
+                ("revp", ctypes.c_int8), ("genestrand", ctypes.c_int8), ("flags", ctypes.c_int8),
                 ("pair_off", ctypes.c_int64), ("max_pairs", ctypes.c_int32), ("pad2", ctypes.c_int32)]
 
 
@@ -43,13 +43,24 @@ PROBLEM_DTYPE = np.dtype([("qoff", "<i8"), ("goff", "<i8"), ("galtoff", "<i8"), 
                           ("keep_scores", "i1"), ("pad", "i1"), ("pad2", "<i4")])
 TBREQ_DTYPE = np.dtype([("fill", "<i4"), ("r", "<i4"), ("c", "<i4"), ("queryoffset", "<i4"),
                         ("genomeoffset", "<i4"), ("chroffset", "<u4"), ("chrhigh", "<u4"), ("watsonp", "i1"),
-                        ("revp", "i1"), ("genestrand", "i1"), ("pad", "i1"), ("pair_off", "<i8"),
+                        ("revp", "i1"), ("genestrand", "i1"), ("flags", "i1"), ("pair_off", "<i8"),
                         ("max_pairs", "<i4"), ("pad2", "<i4")])
 PAIR_DTYPE = np.dtype([("querypos", "<i4"), ("genomepos", "<i4"), ("cdna", "i1"), ("comp", "i1"),
                        ("genome", "i1"), ("genomealt", "i1"), ("gapp", "<i2"), ("genomejump", "<i2")])
 assert PROBLEM_DTYPE.itemsize == 48 and TBREQ_DTYPE.itemsize == 48 and PAIR_DTYPE.itemsize == 16
 
 _lib = None
+
+
+def runtime_first():
+    """torch bundles its own libamdhip64.so.7 under the same soname as
+    /opt/rocm's; whichever is loaded first serves the whole process, and torch
+    cannot run on the other one.  Load torch's before any library of this
+    package so that torch and the kernels share one HIP runtime."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def load(path=None):
@@ -60,6 +71,7 @@ def load(path=None):
     p = path or library_path()
     if not os.path.exists(p):
         raise RuntimeError("libgmapdp.so not built (%s); run __graft_entry__.build()" % p)
+    runtime_first()
     L = ctypes.CDLL(p)
     vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
     sig = {

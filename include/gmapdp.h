@@ -90,11 +90,19 @@ typedef struct gdp_tb_request {
   int8_t watsonp;
   int8_t revp;
   int8_t genestrand;
-  int8_t pad;
+  int8_t flags; /* GDP_TB_OPS: emit pair-list operations instead of pairs */
   int64_t pair_off;
   int32_t max_pairs;
   int32_t pad2;
 } gdp_tb_request;
+
+/* With GDP_TB_OPS the walk records, instead of pairs, the calls the
+   reference's traceback makes on its pair pool, in order: cdna = 'D' for
+   Pairpool_push of a diagonal pair at (querypos, genomepos) = (r, c) with
+   comp set; 'H' for Pairpool_add_genomeskip(r, c, dist) and 'V' for
+   Pairpool_add_queryskip(r, c, dist), dist in genomejump.  A caller that owns
+   a host pair pool and genome replays them (gmapdp_dynprog.h). */
+#define GDP_TB_OPS 1
 
 /* One traceback output record, in the order the reference pushes them onto
    its pair list (from the end cell back to the start).  gapp = 1 marks a
@@ -113,6 +121,23 @@ typedef struct gdp_pair {
 /* Builds the score tables for a GMAP mode (mode.h: 0 = STANDARD ...), as
    Dynprog_init does (dynprog.c:1006-1196).  Must precede every other call. */
 int gdp_init(int mode);
+
+/* As gdp_init, but with tables the host program already built (its own
+   Dynprog_init): pd = pairdistance_array as 4*128*128 int16 [mt][query][genome],
+   nt = nt_to_int_array (128), cons = consistent_array as 3*128*128 bytes. */
+int gdp_init_tables(const int16_t *pd, const int32_t *nt, const uint8_t *cons);
+
+/* Builds the host-side tables only (no device needed); read them back with
+   gdp_pairdistance / gdp_nt_to_int / gdp_consistent / gdp_use8p_size. */
+void gdp_host_tables(int mode);
+const int16_t *gdp_pairdistance(int mismatchtype);
+const int32_t *gdp_nt_to_int(void);
+int32_t gdp_use8p_size(int mismatchtype);
+int32_t gdp_consistent(int genestrand, int c, int g);
+
+/* Dynprog_compute_bands (dynprog.c:1245-1261). */
+void gdp_compute_bands(int32_t *lband, int32_t *uband, int32_t rlength, int32_t glength, int32_t extraband,
+                       int32_t widebandp);
 
 /* Message describing the last failure on this thread. */
 const char *gdp_last_error(void);
@@ -160,6 +185,17 @@ int gdp_traceback(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nre
    bit2 Egap, bit3 Fgap.  Cells the fill does not compute are 0. */
 int gdp_export_dense(const void *d_descs, int32_t idx, const uint32_t *d_dirs, const int16_t *d_scores,
                      uint8_t *d_dense_dirs, int16_t *d_dense_scores, void *stream);
+
+/* Writes one problem in the reference's own workspace layout and encoding
+   (aligned_score{8,16}_alloc / aligned_directions{8,16}_alloc,
+   dynprog_simd.c:2158-2282): element (outer, inner) at outer*stride + inner,
+   outer = genome column c and inner = query row r for full/upper, outer = r
+   and inner = c for lower; stride is the reference's rlength_ceil /
+   glength_ceil.  Elements are int8 (width 8) or int16 (width 16): scores,
+   nogap in {0 DIAG, -1 HORIZ, -2 VERT}, Egap in {0, -1}, Fgap in {0, -2}
+   (d_fgap may be NULL for one-sided problems).  Cells not computed are 0. */
+int gdp_export_reference(const void *d_descs, int32_t idx, const uint32_t *d_dirs, const int16_t *d_scores,
+                         int32_t stride, void *d_score, void *d_nogap, void *d_egap, void *d_fgap, void *stream);
 
 #ifdef __cplusplus
 }

@@ -253,3 +253,23 @@ long long refh_run_batch(const void *problems, int n, const char *qbuf, const ch
   Dynprog_free(&ws);
   return check;
 }
+
+/* Pair pools and list flattening for tests that call a Dynprog_* library
+   directly (the pair list comes back as the reference's List_T). */
+void *refh_pool_new(void) { return (void *)Pairpool_new(); }
+void refh_pool_reset(void *pool) { Pairpool_reset((Pairpool_T)pool); }
+
+int refh_flatten(void *list, int *out, int maxpairs) {
+  List_T p;
+  int n = 0;
+  for (p = (List_T)list; p != NULL; p = p->rest) n++;
+  int k = n - 1;
+  for (p = (List_T)list; p != NULL; p = p->rest, k--) {
+    if (k >= maxpairs) continue;
+    Pair_T pr = (Pair_T)p->first;
+    int *o = out + (size_t)k * 8;
+    o[0] = pr->querypos; o[1] = (int)pr->genomepos; o[2] = pr->cdna; o[3] = pr->comp;
+    o[4] = pr->genome; o[5] = pr->genomealt; o[6] = pr->gapp ? 1 : 0; o[7] = pr->genomejump;
+  }
+  return n;
+}

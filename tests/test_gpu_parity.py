@@ -104,3 +104,45 @@ def test_edge_shapes(oracle):
     b = _run(oracle, specs)
     _check_fills(oracle, specs, b)
     _check_tracebacks(oracle, specs, b, 3)
+
+
+def test_end_extension_pipeline(oracle):
+    """Config-4 style problems: upper and lower fills, best end cell over both
+    triangles, traceback through the triangle holding it (dynprog_end.c:1576-1607)."""
+    from gmapdp.batch import DeviceBatch
+    from gmapdp.workload import Workload
+    w = Workload(4, 300, genome_len=1 << 20)
+    p, m = w.problems, w.meta
+    b = DeviceBatch(p, w.qbuf.tobytes(), w.gbuf.tobytes())
+    b.fill()
+    ul = np.stack([np.arange(0, len(p), 2), np.arange(1, len(p), 2)], 1).astype(np.int32)
+    for qi in (False, True):
+        best = b.best_endpoint(ul, queryend_indels=qi)
+        starts = []
+        for k in range(len(ul)):
+            pu = p[ul[k, 0]]
+            revp = bool(m["revp"][ul[k, 0]])
+            q = bytes(w.qbuf[pu["qoff"]: pu["qoff"] + pu["rl"]])
+            g = bytes(w.gbuf[pu["goff"]: pu["goff"] + pu["gl"]])
+            if revp:
+                q, g = q[::-1], g[::-1]
+            su = H.Spec(1, int(pu["width"]), q, g, g, 3, int(pu["open"]), int(pu["extend"]), int(pu["lband"]),
+                        int(pu["uband"]), bool(pu["jump_late"]), revp)
+            sl = H.Spec(2, su.width, q, g, g, 3, su.open, su.ext, su.lband, su.uband, su.late, revp)
+            Hu, Du = H.oracle_fill(oracle, su)
+            Hl, Dl = H.oracle_fill(oracle, sl)
+            want = oracle.best_endpoint(qi, Hu, Hl, su.width, su.rl, su.gl, su.lband, su.uband, su.late)
+            assert tuple(best[k]) == want, ("best end cell", k, tuple(best[k]), want)
+            fs, br, bc = want
+            if br == 0 and bc == 0:
+                continue
+            kind = 1 if bc >= br else 2
+            s = su if kind == 1 else sl
+            D = np.where(H.computed_mask(s), Du if kind == 1 else Dl, 0).astype(np.uint8)
+            qo, go = int(m["queryoffset"][ul[k, 0]]), int(m["genomeoffset"][ul[k, 0]])
+            po, co = oracle.traceback(kind, D, br, bc, s.q, s.q.upper(), s.g, s.galt, qo, go, revp)
+            req = H.tb_request(int(ul[k, 0 if kind == 1 else 1]), s, br, bc, qoff=qo, goff=go)
+            starts.append((req, po, co))
+        pairs, counts = b.traceback(np.array([x[0] for x in starts]))
+        for j, (_, po, co) in enumerate(starts):
+            assert (H.pairs_to_records(pairs[j]) == po).all() and (counts[j, 1:] == co).all(), ("end traceback", j)

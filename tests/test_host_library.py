@@ -25,16 +25,41 @@ def L():
     return lib.load()
 
 
-def test_library_exports_every_declared_symbol(L):
-    for header in sorted(os.listdir(os.path.join(ROOT, "include"))):
-        if not header.endswith(".h"):
+def _elf_exports(path):
+    """Defined global function symbols of an ELF64 shared object (.dynsym),
+    read without loading it: libgmapdp_dynprog.so leaves the host program's
+    tables and Pairpool_* undefined, so it only loads inside such a program."""
+    import struct
+    data = open(path, "rb").read()
+    assert data[:4] == b"\x7fELF" and data[4] == 2
+    shoff, = struct.unpack_from("<Q", data, 0x28)
+    shentsize, shnum = struct.unpack_from("<HH", data, 0x3A)
+    secs = [struct.unpack_from("<IIQQQQIIQQ", data, shoff + i * shentsize) for i in range(shnum)]
+    out = set()
+    for sh in secs:
+        if sh[1] != 11:  # SHT_DYNSYM
             continue
+        strtab = secs[sh[6]]
+        for k in range(sh[5] // 24):
+            name, info, _, shndx, _, _ = struct.unpack_from("<IBBHQQ", data, sh[4] + k * 24)
+            if shndx != 0 and (info >> 4) in (1, 2) and (info & 15) == 2:  # defined GLOBAL/WEAK FUNC
+                end = data.index(b"\0", strtab[4] + name)
+                out.add(data[strtab[4] + name:end].decode())
+    return out
+
+
+def test_library_exports_every_declared_symbol(L):
+    libs = {"gmapdp.h": lib.library_path(),
+            "gmapdp_dynprog.h": os.path.join(os.path.dirname(lib.library_path()), "libgmapdp_dynprog.so")}
+    headers = sorted(h for h in os.listdir(os.path.join(ROOT, "include")) if h.endswith(".h"))
+    assert set(headers) == set(libs), headers
+    for header in headers:
         names = _declared(header)
         assert names, header
+        exported = _elf_exports(libs[header])
         for name in names:
-            target = L if header == "gmapdp.h" else ctypes.CDLL(lib.library_path().replace("libgmapdp", "libgmapdp_dynprog"),
-                                                                mode=os.RTLD_LAZY)
-            assert hasattr(target, name), "%s declares %s but the library does not export it" % (header, name)
+            assert name in exported, "%s declares %s but %s does not export it" % (header, name, libs[header])
+    assert {n for n in _elf_exports(libs["gmapdp.h"]) if n.startswith("gdp_")} >= set(_declared("gmapdp.h"))
 
 
 def test_struct_sizes(L):
@@ -49,11 +74,7 @@ def test_tables_match_oracle(L, mode):
     from gmapdp.lib import load
     O = pyoracle.Oracle(mode)
     L2 = load()
-    # gdp_tables_build runs inside gdp_init, which needs a device; build the
-    # host tables directly through the table entry used by gdp_init.
-    fn = getattr(L2, "_Z16gdp_tables_buildi")
-    fn.argtypes = [ctypes.c_int]
-    fn(mode)
+    L2.gdp_host_tables(mode)
     for mt in range(4):
         got = np.ctypeslib.as_array(L2.gdp_pairdistance(mt), shape=(128 * 128,)).reshape(128, 128)
         assert (got == O.pairdistance(mt)).all()
