@@ -143,7 +143,7 @@ def main():
     percall = None
     if args.percall:
         m = min(args.percall, n)
-        sub = DeviceBatch(probs[:m], wl.qbuf.tobytes(), wl.gbuf.tobytes())
+        sub = DeviceBatch(probs[:m], wl.qbuf.tobytes(), wl.gbuf.tobytes(), single=True)
         sreq = sub.prepare_traceback(wl.tb_requests()[:m], genome=wl.genome())
         torch.cuda.synchronize()
         t1 = time.perf_counter()

@@ -9,6 +9,9 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <algorithm>
+#include <utility>
+#include <vector>
 
 #include "gdp_internal.h"
 #include "gdp_tables.h"
@@ -95,12 +98,29 @@ static int validate(const gdp_problem &p, int i) {
   return GDP_OK;
 }
 
-extern "C" int gdp_plan(const gdp_problem *probs, int32_t n, void *descs_out, int64_t *dir_words,
-                        int64_t *score_slots) {
-  if (n < 0 || (n > 0 && (probs == nullptr || descs_out == nullptr)))
-    return fail(GDP_EINVAL, "gdp_plan: bad arguments");
+extern "C" int64_t gdp_plan_bytes(int32_t n) {
+  if (n < 0) n = 0;
+  return (int64_t)n * (int64_t)(sizeof(gdp_dev_problem) + sizeof(gdp_wave)) + (int64_t)sizeof(gdp_pair_hdr);
+}
+
+// Full-band int16 fills without kept scores are swept in pairs, 2*(64/G) per
+// wave; the rest (one-sided triangles, int8, kept scores, very wide bands)
+// each get a 16-lane group of their own.
+static int pair_lanes(const gdp_problem &p) {
+  if (p.kind != GDP_KIND_FULL || p.width != 16 || p.keep_scores) return 0;
+  const int W = p.lband + p.uband;
+  if (W <= 224) return 16;
+  if (W <= 936) return 64;
+  return 0;
+}
+
+extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, void *descs_out, int64_t *dir_words,
+                           int64_t *score_slots) {
+  if (n < 0 || (n > 0 && probs == nullptr) || descs_out == nullptr) return fail(GDP_EINVAL, "gdp_plan: bad arguments");
+  uint8_t *base = (uint8_t *)descs_out;
   gdp_dev_problem *d = (gdp_dev_problem *)descs_out;
   int64_t dw = 0, ss = 0;
+  std::vector<int32_t> paired;
   for (int32_t i = 0; i < n; i++) {
     const gdp_problem &p = probs[i];
     int rc = validate(p, i);
@@ -120,20 +140,116 @@ extern "C" int gdp_plan(const gdp_problem *probs, int32_t n, void *descs_out, in
     q.jump_late = p.jump_late ? 1 : 0;
     q.open = p.open;
     q.extend = p.extend;
+    q.score_off = -1;
+    if (!(flags & GDP_PLAN_SINGLE) && pair_lanes(p)) {
+      d[i] = q;
+      paired.push_back(i);
+      continue;
+    }
     gdp_geom G = gdp_make_geom(q);
     q.dir_off = dw;
     dw += gdp_dir_words(G);
     if (p.keep_scores) {
       q.score_off = ss;
       ss += (int64_t)G.nsteps * GDP_GROUP;
-    } else {
-      q.score_off = -1;
     }
     d[i] = q;
   }
+
+  // Waves: problems with the same tie rule and band, in order of size.
+  std::stable_sort(paired.begin(), paired.end(), [&](int32_t a, int32_t b) {
+    const gdp_problem &x = probs[a], &y = probs[b];
+    const int kx = pair_lanes(x), ky = pair_lanes(y);
+    if (kx != ky) return kx < ky;
+    if ((x.jump_late != 0) != (y.jump_late != 0)) return (x.jump_late != 0) < (y.jump_late != 0);
+    if (x.lband != y.lband) return x.lband < y.lband;
+    if (x.uband != y.uband) return x.uband < y.uband;
+    if (x.rl != y.rl) return x.rl < y.rl;
+    return x.gl < y.gl;
+  });
+  std::vector<gdp_wave> waves[2];
+  size_t k = 0;
+  while (k < paired.size()) {
+    const gdp_problem &p0 = probs[paired[k]];
+    const int G = pair_lanes(p0), slots = 2 * (64 / G);
+    size_t e = k;
+    while (e < paired.size() && e - k < (size_t)slots) {
+      const gdp_problem &x = probs[paired[e]];
+      if (pair_lanes(x) != G || (x.jump_late != 0) != (p0.jump_late != 0) || x.lband != p0.lband ||
+          x.uband != p0.uband)
+        break;
+      e++;
+    }
+    gdp_wave w;
+    memset(&w, 0, sizeof w);
+    for (int s = 0; s < 8; s++) {
+      w.prob[s] = -1;
+      w.fin_t[s] = INT32_MAX;
+    }
+    int R = 0, C = 0;
+    for (size_t j = k; j < e; j++) {
+      R = gdp_imax(R, probs[paired[j]].rl);
+      C = gdp_imax(C, probs[paired[j]].gl);
+    }
+    const int lb = p0.lband, ub = p0.uband, P = gdp_p_period(G, lb, ub);
+    w.R = R;
+    w.C = C;
+    w.P = P;
+    w.nb = (R + G) / G;
+    w.nsteps = gdp_p_nsteps(G, P, R);
+    w.lb = (int16_t)lb;
+    w.ub = (int16_t)ub;
+    w.late = p0.jump_late ? 1 : 0;
+    w.glog = G == 16 ? 4 : 6;
+    std::vector<std::pair<int32_t, int8_t>> fin;
+    for (size_t j = k; j < e; j++) {
+      const int s = (int)(j - k), idx = paired[j];
+      const gdp_problem &x = probs[idx];
+      w.prob[s] = idx;
+      gdp_dev_problem &q = d[idx];
+      q.layout = 1;
+      q.glog = (int8_t)w.glog;
+      q.sR = R;
+      q.sC = C;
+      q.P = P;
+      q.nsteps = w.nsteps;
+      q.dir_off = dw;
+      dw += (int64_t)(w.nsteps / 8) * G;
+      if (x.rl <= x.gl + lb && x.rl >= x.gl - ub) fin.push_back({gdp_p_step(G, P, lb, x.rl, x.gl), (int8_t)s});
+    }
+    std::sort(fin.begin(), fin.end());
+    for (size_t j = 0; j < fin.size(); j++) {
+      w.fin_t[j] = fin[j].first;
+      w.fin_slot[j] = fin[j].second;
+    }
+    waves[G == 16 ? 0 : 1].push_back(w);
+    k = e;
+  }
+  // Longest sweeps first, so the queue ends on short ones.
+  for (int c = 0; c < 2; c++)
+    std::stable_sort(waves[c].begin(), waves[c].end(),
+                     [](const gdp_wave &a, const gdp_wave &b) { return a.nsteps > b.nsteps; });
+
+  gdp_pair_hdr h;
+  memset(&h, 0, sizeof h);
+  h.magic = GDP_PAIR_MAGIC ^ (uint64_t)n;
+  h.n = n;
+  h.nw[0] = (int32_t)waves[0].size();
+  h.nw[1] = (int32_t)waves[1].size();
+  const int64_t hoff = (int64_t)n * sizeof(gdp_dev_problem);
+  h.off[0] = hoff + (int64_t)sizeof(gdp_pair_hdr);
+  h.off[1] = h.off[0] + (int64_t)waves[0].size() * (int64_t)sizeof(gdp_wave);
+  memcpy(base + hoff, &h, sizeof h);
+  for (int c = 0; c < 2; c++)
+    if (!waves[c].empty()) memcpy(base + h.off[c], waves[c].data(), waves[c].size() * sizeof(gdp_wave));
   if (dir_words) *dir_words = dw;
   if (score_slots) *score_slots = ss;
   return GDP_OK;
+}
+
+extern "C" int gdp_plan(const gdp_problem *probs, int32_t n, void *descs_out, int64_t *dir_words,
+                        int64_t *score_slots) {
+  return gdp_plan_ex(probs, n, 0, descs_out, dir_words, score_slots);
 }
 
 extern "C" int64_t gdp_cells(const gdp_problem *p) {

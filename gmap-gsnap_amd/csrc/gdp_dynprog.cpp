@@ -174,10 +174,10 @@ void **run_fill(Dynprog_T dynprog, int kind, int width, char *rseq, char *gseq, 
   p.open = (int8_t)open;
   p.extend = (int8_t)extend;
   p.keep_scores = 1;
-  uint8_t desc[64];
+  std::vector<uint8_t> desc((size_t)gdp_plan_bytes(1));
   int64_t dw = 0, ss = 0;
-  if (gdp_plan(&p, 1, desc, &dw, &ss) != 0) die("gdp_plan");
-  hipcheck(hipMemcpy(sl.desc.need(64), desc, 64, hipMemcpyHostToDevice), "upload descriptor");
+  if (gdp_plan(&p, 1, desc.data(), &dw, &ss) != 0) die("gdp_plan");
+  hipcheck(hipMemcpy(sl.desc.need(desc.size()), desc.data(), desc.size(), hipMemcpyHostToDevice), "upload descriptor");
   hipcheck(hipMemcpy(sl.q.need(rl + 1), q.data(), rl, hipMemcpyHostToDevice), "upload query");
   uint8_t *dg = sl.g.need(2 * (size_t)gl + 1);
   hipcheck(hipMemcpy(dg, g.data(), gl, hipMemcpyHostToDevice), "upload genome");

@@ -25,11 +25,16 @@
 // clamped here to keep it in 16 bits (+128 bias) in the LDS ring.
 #define GDP_F_FLOOR (-32896)
 
-// Device copy of one problem (64 bytes).  Row/column roles: for the full band
+// Device copy of one problem (96 bytes).  Row/column roles: for the full band
 // and the upper triangle rows are query positions r = 0..rl and columns are
 // genome positions c = 0..gl; the lower triangle is swept transposed (rows are
 // genome positions, columns query positions), which is how the reference's
 // column-blocked lower kernel visits it (dynprog_simd.c:8744-8860).
+//
+// layout 0: the problem is swept alone by a 16-lane group (gdp_fill_kernel);
+// layout 1: it is one half of a lane pair swept together with problems of the
+// same band and tie rule (gdp_fill2_kernel), over the padded sweep sR x sC
+// with batch period P and 2^glog lanes per group.
 struct gdp_dev_problem {
   int64_t qoff;     // q(r) = qbuf[qoff + r - 1], r = 1..rl (already in sweep order)
   int64_t goff;     // g(c) = gbuf[goff + c - 1]
@@ -39,13 +44,47 @@ struct gdp_dev_problem {
   int32_t rl, gl;
   int16_t lband, uband;
   int8_t kind, width, mismatchtype, jump_late;
-  int8_t open, extend;
-  int8_t pad[2];
+  int8_t open, extend, layout, glog;
+  int32_t sR, sC;   // layout 1: padded sweep extent shared by the wave
+  int32_t P;        // layout 1: steps between row batches
+  int32_t nsteps;   // layout 1: steps of the sweep (multiple of 8)
+  int32_t pad[5];
 };
-static_assert(sizeof(gdp_dev_problem) == 64, "gdp_dev_problem must stay 64 bytes");
+static_assert(sizeof(gdp_dev_problem) == 96, "gdp_dev_problem must stay 96 bytes");
 static_assert(sizeof(gdp_problem) == 48, "gdp_problem is 48 bytes");
 static_assert(sizeof(gdp_tb_request) == 48, "gdp_tb_request is 48 bytes");
 static_assert(sizeof(gdp_pair) == 16, "gdp_pair is 16 bytes");
+
+// One wave of the paired fill: 64 lanes = 64/G groups of G lanes, each lane
+// carrying two problems in the low and high 16 bits of its registers.  Slot
+// 2*g + h is group g, half h.  Every problem of a wave has the same band and
+// tie rule; rows and columns are padded to the largest problem.
+struct gdp_wave {
+  int32_t prob[8];      // problem index per slot, -1 = empty
+  int32_t fin_t[8];     // steps at which final cells (rl, gl) are computed, ascending, INT32_MAX pads
+  int8_t fin_slot[8];   // slot of each fin_t entry
+  int32_t R, C;         // padded sweep: rows 0..R, columns 0..C
+  int32_t P, nb, nsteps;
+  int16_t lb, ub;
+  int8_t late, glog, pad0[2];
+  int32_t pad[7];
+};
+static_assert(sizeof(gdp_wave) == 128, "gdp_wave must stay 128 bytes");
+
+// Header written after the n problem descriptors.  The fill checks magic and
+// n before it touches the wave lists, so a descriptor array handed over
+// with a different n never reaches them.
+#define GDP_PAIR_MAGIC 0x6764703270616972ull
+struct gdp_pair_hdr {
+  uint64_t magic;       // GDP_PAIR_MAGIC ^ n
+  int32_t n;
+  int32_t nw[2];        // waves with 16- and 64-lane groups
+  int32_t pad0;
+  int64_t off[2];       // byte offset of each wave list from the descriptor base
+  int32_t queue[2][2];  // per list: next wave to hand out, waves that have left the queue
+  int32_t pad[2];
+};
+static_assert(sizeof(gdp_pair_hdr) == 64, "gdp_pair_hdr must stay 64 bytes");
 
 // Sweep geometry shared by the fill, traceback, export and scan kernels.
 struct gdp_geom {
@@ -103,6 +142,26 @@ GDP_HD int gdp_step(const gdp_geom &g, int r, int c) {
   return (r / GDP_GROUP) * g.P + (r % GDP_GROUP) + c;
 }
 GDP_HD int64_t gdp_dir_words(const gdp_geom &g) { return (int64_t)((g.nsteps + 7) / 8) * GDP_GROUP; }
+
+// Paired sweep (layout 1).  Row r = G*b + i is handled by lane i of its
+// group during batch b; its cell in column c is computed at step
+//   t = b*P + i + (c - cs0(b)),   cs0(b) = max(0, G*b - lb),
+// so cell (r-1, c) is always one step older than (r, c).  Each lane spends
+// P steps per row: from cs0(b) on (the emulated 16-row block start lies at
+// or after it) to past the top band edge.
+GDP_HD int gdp_p_period(int G, int lb, int ub) {
+  int P = gdp_imax(2 * G, G + lb + ub);
+  return (P + 7) & ~7;
+}
+GDP_HD int gdp_p_cs0(int G, int lb, int b) { return gdp_imax(0, G * b - lb); }
+GDP_HD int gdp_p_nsteps(int G, int P, int R) {
+  int nb = (R + G) / G;
+  return (nb * P + G + 7) & ~7;
+}
+GDP_HD int gdp_p_step(int G, int P, int lb, int r, int c) {
+  int b = r / G;
+  return b * P + (r % G) + (c - gdp_p_cs0(G, lb, b));
+}
 
 // Per-problem constants of the cell update.
 struct gdp_consts {

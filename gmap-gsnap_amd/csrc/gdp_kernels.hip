@@ -100,6 +100,7 @@ __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_probl
   uint32_t *ring = ring_all + grp * GDP_RING;
 
   const gdp_dev_problem p = probs[pid];
+  if (p.layout != 0) return;  // swept by gdp_fill2_kernel
   const gdp_geom G = gdp_make_geom(p);
   const gdp_consts K = gdp_make_consts(p);
   const bool full = G.full;
@@ -175,6 +176,13 @@ __device__ __forceinline__ uint32_t nib_at(const uint32_t *dirs, const gdp_dev_p
                                            int col) {
   if (row < 0 || row > G.R || col < 0 || col > G.C) return 0;
   if (col < gdp_cstart(G, row) || col > gdp_cend(G, row)) return 0;  // not computed: DIAG
+  if (p.layout == 1) {  // paired sweep: byte 0 Egap, 1 HORIZ, 2 Fgap, 3 VERT; bit = step & 7
+    const int g = 1 << p.glog;
+    const int t = gdp_p_step(g, p.P, p.lband, row, col);
+    const uint32_t w = dirs[p.dir_off + (int64_t)(t >> 3) * g + (row & (g - 1))] >> (t & 7);
+    return ((w >> 24) & 1 ? GDP_DIR_V : ((w >> 8) & 1 ? GDP_DIR_H : 0)) | (w & 1 ? GDP_DIR_E : 0) |
+           ((w >> 16) & 1 ? GDP_DIR_F : 0);
+  }
   int s = gdp_step(G, row, col);
   uint32_t w = dirs[p.dir_off + (int64_t)(s >> 3) * GDP_GROUP + (row % GDP_GROUP)];
   return (w >> (4 * (s & 7))) & 15;
@@ -542,6 +550,8 @@ __global__ void gdp_export_ref_kernel(const gdp_dev_problem *__restrict__ probs,
   }
 }
 
+#include "gdp_fill2.inc"
+
 // ------------------------------------------------------------ host launchers
 
 extern "C" int gdp_launch_export_ref(const void *d_descs, int idx, int width, const uint32_t *d_dirs,
@@ -558,12 +568,38 @@ extern "C" int gdp_launch_export_ref(const void *d_descs, int idx, int width, co
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Persistent grid of the paired fill: as many workgroups as fit on the device
+// at once, each wave taking waves off the descriptor's queue.
+template <int G>
+static int p2_grid() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
+  if (cached[dev] == 0) {
+    int ncu = 0, per = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill2_kernel<G>, P2_THREADS, p2cfg<G>::LDS_BYTES) !=
+        hipSuccess)
+      return -1;
+    cached[dev] = ncu * (per > 0 ? per : 1);
+  }
+  return cached[dev];
+}
+
 extern "C" int gdp_launch_fill(const void *d_descs, int n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
                                int16_t *d_scores, int32_t *d_final, hipStream_t stream) {
   if (n <= 0) return 0;
   int blocks = (n + GDP_GROUPS_PER_BLOCK - 1) / GDP_GROUPS_PER_BLOCK;
   hipLaunchKernelGGL(gdp_fill_kernel, dim3(blocks), dim3(GDP_BLOCK), 0, stream, (const gdp_dev_problem *)d_descs, n,
                      d_qbuf, d_gbuf, d_dirs, d_scores, d_final);
+  if (hipGetLastError() != hipSuccess) return -1;
+  const int g16 = p2_grid<16>(), g64 = p2_grid<64>();
+  if (g16 <= 0 || g64 <= 0) return -1;
+  hipLaunchKernelGGL(gdp_fill2_kernel<16>, dim3(g16), dim3(P2_THREADS), p2cfg<16>::LDS_BYTES, stream,
+                     (const uint8_t *)d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final);
+  if (hipGetLastError() != hipSuccess) return -1;
+  hipLaunchKernelGGL(gdp_fill2_kernel<64>, dim3(g64), dim3(P2_THREADS), p2cfg<64>::LDS_BYTES, stream,
+                     (const uint8_t *)d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -37,14 +37,18 @@ def _check(L, rc, what):
         raise RuntimeError("%s failed (%d): %s" % (what, rc, L.gdp_last_error().decode()))
 
 
-def plan(problems):
+PLAN_SINGLE = 1  # GDP_PLAN_SINGLE: every problem swept on its own
+
+
+def plan(problems, flags=0):
     """Host planning (no device needed): descriptors and buffer sizes."""
     L = load()
     problems = np.ascontiguousarray(problems, PROBLEM_DTYPE)
     n = len(problems)
-    descs = np.zeros(max(n, 1) * L.gdp_descriptor_bytes(), np.uint8)
+    descs = np.zeros(int(L.gdp_plan_bytes(n)), np.uint8)
     dw, ss = ctypes.c_int64(), ctypes.c_int64()
-    _check(L, L.gdp_plan(problems.ctypes.data, n, descs.ctypes.data, ctypes.byref(dw), ctypes.byref(ss)), "gdp_plan")
+    _check(L, L.gdp_plan_ex(problems.ctypes.data, n, flags, descs.ctypes.data, ctypes.byref(dw), ctypes.byref(ss)),
+           "gdp_plan")
     return descs, dw.value, ss.value
 
 
@@ -57,13 +61,16 @@ def cells(problems):
 class DeviceBatch:
     """Problems, sequences and outputs resident in device memory."""
 
-    def __init__(self, problems, qbuf, gbuf, mode=0, device="cuda"):
+    def __init__(self, problems, qbuf, gbuf, mode=0, device="cuda", single=False):
+        """single=True plans every problem on its own sweep (required for
+        fill_one, the one-launch-per-problem form)."""
         torch = _torch()
         self.torch = torch
         self.L = init(mode)
         self.problems = np.ascontiguousarray(problems, PROBLEM_DTYPE)
         self.n = len(self.problems)
-        descs, dw, ss = plan(self.problems)
+        self.single = single
+        descs, dw, ss = plan(self.problems, PLAN_SINGLE if single else 0)
         self.dir_words, self.score_slots = dw, ss
         dev = torch.device(device)
         self.d_descs = torch.from_numpy(descs).to(dev)
@@ -165,6 +172,8 @@ class DeviceBatch:
 
     def fill_one(self, i):
         """One problem per launch (the per-call form)."""
+        if not self.single:
+            raise RuntimeError("fill_one needs a batch planned with single=True")
         desc = self.d_descs.data_ptr() + i * self.L.gdp_descriptor_bytes()
         _check(self.L, self.L.gdp_fill(desc, 1, self.d_q.data_ptr(), self.d_g.data_ptr(), self.d_dirs.data_ptr(),
                                        self.d_scores.data_ptr(), self.d_final.data_ptr() + 4 * i, self.stream),

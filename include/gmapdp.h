@@ -145,19 +145,38 @@ const char *gdp_last_error(void);
 /* Size in bytes of one device problem descriptor. */
 int32_t gdp_descriptor_bytes(void);
 
+/* Bytes gdp_plan writes for n problems: n descriptors followed by the wave
+   lists of the paired sweep. */
+int64_t gdp_plan_bytes(int32_t n);
+
 /* Host-side planning: validates the problems and writes n device
-   descriptors (gdp_descriptor_bytes() each) into descs_out, assigning each
-   problem its slice of the direction and score buffers.  Returns the 32-bit
-   direction words and int16 score slots the batch needs. */
+   descriptors (gdp_descriptor_bytes() each) into descs_out, followed by the
+   wave lists (gdp_plan_bytes(n) in all), assigning each problem its slice of
+   the direction and score buffers.  Full-band int16 problems without kept
+   scores are grouped into waves of problems that share band and tie rule
+   and are swept two per lane; gdp_fill must then be given the whole array
+   with the same n.  Returns the 32-bit direction words and int16 score
+   slots the batch needs.  No counterpart in the reference (the batched
+   form the drop-in functions are built on). */
 int gdp_plan(const gdp_problem *probs, int32_t n, void *descs_out, int64_t *dir_words, int64_t *score_slots);
+
+/* As gdp_plan; flags GDP_PLAN_SINGLE gives every problem a sweep of its own,
+   so that any single descriptor can be filled by itself (gdp_fill on
+   descs + i*gdp_descriptor_bytes() with n = 1): the one-launch-per-problem
+   form. */
+#define GDP_PLAN_SINGLE 1
+int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, void *descs_out, int64_t *dir_words,
+                int64_t *score_slots);
 
 /* In-band cell count of a problem (the throughput work unit). */
 int64_t gdp_cells(const gdp_problem *prob);
 
-/* Fills every problem: one kernel launch over the whole array.  d_final[i]
-   receives the score of cell (rl, gl) for full-band problems (the value
-   Dynprog_single_gap reads, dynprog_single.c:603), or INT32_MIN when that
-   cell is outside the band. */
+/* Fills every problem of a plan: the single-sweep problems in one launch,
+   then the paired waves in persistent launches that take waves off a queue
+   kept in the descriptor array (so two fills of the same descriptors must
+   not overlap in time).  d_final[i] receives the score of cell (rl, gl) for
+   full-band problems (the value Dynprog_single_gap reads,
+   dynprog_single.c:603), or INT32_MIN when that cell is outside the band. */
 int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
              int16_t *d_scores, int32_t *d_final, void *stream);
 

@@ -8,17 +8,19 @@ import helpers as H
 pytestmark = pytest.mark.gpu
 
 
-def test_gpu_matches_reference_golden_vectors():
+@pytest.mark.parametrize("keep", [True, False])
+def test_gpu_matches_reference_golden_vectors(keep):
+    # keep=False: full-band int16 problems take the paired sweep
     from gmapdp.batch import DeviceBatch
     g, specs = golden_io.load()
-    probs, qbuf, gbuf = H.pack_batch(specs, keep_scores=True)
+    probs, qbuf, gbuf = H.pack_batch(specs, keep_scores=keep)
     b = DeviceBatch(probs, qbuf, gbuf)
     b.fill()
     fin = b.final_scores()
     reqs, idx = [], []
     for i, s in enumerate(specs):
         D, S = b.export(i)
-        golden_io.check_fill(g, i, s, D, S)
+        golden_io.check_fill(g, i, s, D, S, scores=keep)
         if g["final"][i] != -(1 << 31):
             assert fin[i] == g["final"][i], ("corner score", i)
         exp = golden_io.traceback_expect(g, i)

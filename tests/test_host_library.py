@@ -63,7 +63,7 @@ def test_library_exports_every_declared_symbol(L):
 
 
 def test_struct_sizes(L):
-    assert L.gdp_descriptor_bytes() == 64
+    assert L.gdp_descriptor_bytes() == 96
     assert PROBLEM_DTYPE.itemsize == ctypes.sizeof(lib.Problem) == 48
     assert ctypes.sizeof(lib.TbRequest) == 48 and ctypes.sizeof(lib.Pair) == 16
 
