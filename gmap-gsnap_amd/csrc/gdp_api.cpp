@@ -225,6 +225,12 @@ extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, v
     waves[G == 16 ? 0 : 1].push_back(w);
     k = e;
   }
+  // Empty slots of partial waves store into one shared sink.
+  int64_t sink_words = 0;
+  for (int c = 0; c < 2; c++)
+    for (const gdp_wave &w : waves[c]) sink_words = std::max<int64_t>(sink_words, (int64_t)(w.nsteps / 8) * (c ? 64 : 16));
+  const int64_t sink_off = dw;
+  dw += sink_words;
   // Longest sweeps first, so the queue ends on short ones.
   for (int c = 0; c < 2; c++)
     std::stable_sort(waves[c].begin(), waves[c].end(),
@@ -239,6 +245,7 @@ extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, v
   const int64_t hoff = (int64_t)n * sizeof(gdp_dev_problem);
   h.off[0] = hoff + (int64_t)sizeof(gdp_pair_hdr);
   h.off[1] = h.off[0] + (int64_t)waves[0].size() * (int64_t)sizeof(gdp_wave);
+  h.sink_off = sink_off;
   memcpy(base + hoff, &h, sizeof h);
   for (int c = 0; c < 2; c++)
     if (!waves[c].empty()) memcpy(base + h.off[c], waves[c].data(), waves[c].size() * sizeof(gdp_wave));

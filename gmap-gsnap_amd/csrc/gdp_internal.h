@@ -82,7 +82,7 @@ struct gdp_pair_hdr {
   int32_t pad0;
   int64_t off[2];       // byte offset of each wave list from the descriptor base
   int32_t queue[2][2];  // per list: next wave to hand out, waves that have left the queue
-  int32_t pad[2];
+  int64_t sink_off;     // direction words written by the empty slots of partial waves
 };
 static_assert(sizeof(gdp_pair_hdr) == 64, "gdp_pair_hdr must stay 64 bytes");
 
@@ -149,8 +149,10 @@ GDP_HD int64_t gdp_dir_words(const gdp_geom &g) { return (int64_t)((g.nsteps + 7
 // so cell (r-1, c) is always one step older than (r, c).  Each lane spends
 // P steps per row: from cs0(b) on (the emulated 16-row block start lies at
 // or after it) to past the top band edge.
+// P >= 2G + 8 keeps the ring distance D = P - (G - 1) - (cs0(b) - cs0(b-1))
+// at 8 or more, so a block of 8 steps never reads what it writes.
 GDP_HD int gdp_p_period(int G, int lb, int ub) {
-  int P = gdp_imax(2 * G, G + lb + ub);
+  int P = gdp_imax(2 * G + 8, G + lb + ub);
   return (P + 7) & ~7;
 }
 GDP_HD int gdp_p_cs0(int G, int lb, int b) { return gdp_imax(0, G * b - lb); }
