@@ -249,6 +249,7 @@ extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, v
   memcpy(base + hoff, &h, sizeof h);
   for (int c = 0; c < 2; c++)
     if (!waves[c].empty()) memcpy(base + h.off[c], waves[c].data(), waves[c].size() * sizeof(gdp_wave));
+  dw += 4;  // the traceback reads direction words in aligned 16-byte windows
   if (dir_words) *dir_words = dw;
   if (score_slots) *score_slots = ss;
   return GDP_OK;

@@ -178,8 +178,8 @@ void **run_fill(Dynprog_T dynprog, int kind, int width, char *rseq, char *gseq, 
   int64_t dw = 0, ss = 0;
   if (gdp_plan(&p, 1, desc.data(), &dw, &ss) != 0) die("gdp_plan");
   hipcheck(hipMemcpy(sl.desc.need(desc.size()), desc.data(), desc.size(), hipMemcpyHostToDevice), "upload descriptor");
-  hipcheck(hipMemcpy(sl.q.need(rl + 1), q.data(), rl, hipMemcpyHostToDevice), "upload query");
-  uint8_t *dg = sl.g.need(2 * (size_t)gl + 1);
+  hipcheck(hipMemcpy(sl.q.need(rl + 16), q.data(), rl, hipMemcpyHostToDevice), "upload query");
+  uint8_t *dg = sl.g.need(2 * (size_t)gl + 16);
   hipcheck(hipMemcpy(dg, g.data(), gl, hipMemcpyHostToDevice), "upload genome");
   hipcheck(hipMemcpy(dg + gl, ga.data(), gl, hipMemcpyHostToDevice), "upload genome alt");
   if (gdp_fill(sl.desc.p, 1, (const char *)sl.q.p, (const char *)sl.g.p, sl.dirs.need(dw), sl.scores.need(ss),

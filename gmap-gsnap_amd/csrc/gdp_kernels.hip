@@ -32,6 +32,8 @@ __device__ uint32_t gdp_tabPQ[4 * 128];
 __device__ uint32_t gdp_tabPG[4 * 128];
 __device__ int8_t gdp_tabNT[128];
 __device__ uint8_t gdp_tabCons[3 * 128 * 128];
+// The same as bits: bit g of word [(gs*128 + c)*4 + g/32].
+__device__ uint32_t gdp_tabConsBits[3 * 128 * 4];
 
 
 // Packed five-field maximum (lower-triangle profiles take the max over the
@@ -255,216 +257,315 @@ __device__ int genomic_nt(int *alt, const char *genome, uint32_t glen, int genom
   return ch;
 }
 
-__global__ void gdp_traceback_kernel(const gdp_dev_problem *__restrict__ probs, const gdp_tb_request *__restrict__ reqs,
-                                     int nreq, const char *__restrict__ qbuf, const char *__restrict__ gbuf,
-                                     const uint32_t *__restrict__ dirs, const char *__restrict__ genome, uint32_t glen,
-                                     gdp_pair *__restrict__ pairs, int32_t *__restrict__ counts) {
+// 16-byte window of device memory kept in registers: a walk reads its
+// directions and characters backwards a few bytes at a time, so one aligned
+// 16-byte load serves several steps.  Buffers read this way need 16 bytes
+// of padding past their last used byte.
+#define WIN_DECL(W)                                                                             \
+  uintptr_t W##_b = ~(uintptr_t)0;                                                              \
+  uint32_t W##_0 = 0, W##_1 = 0, W##_2 = 0, W##_3 = 0
+// Word at addr through window W (one aligned 16-byte load when addr leaves it).
+#define WIN_WORD(W, addr, out)                                                                  \
+  do {                                                                                          \
+    const uintptr_t a_ = (uintptr_t)(addr), b_ = a_ & ~(uintptr_t)15;                           \
+    if (b_ != W##_b) {                                                                          \
+      const uint4 v_ = *(const uint4 *)b_;                                                      \
+      W##_0 = v_.x;                                                                             \
+      W##_1 = v_.y;                                                                             \
+      W##_2 = v_.z;                                                                             \
+      W##_3 = v_.w;                                                                             \
+      W##_b = b_;                                                                               \
+    }                                                                                           \
+    const int k_ = (int)(a_ - b_) >> 2;                                                         \
+    (out) = k_ < 2 ? (k_ == 0 ? W##_0 : W##_1) : (k_ == 2 ? W##_2 : W##_3);                     \
+  } while (0)
+
+// Direction lookup for the walks: the geometry held in registers, the
+// divisions of gdp_step / gdp_p_step turned into shifts.
+struct tb_dirs {
+  const uint32_t *base;
+  int layout, glog, gmask, P, lb, ub, R, C, full, smask;
+  __device__ __forceinline__ void init(const uint32_t *dirs, const gdp_dev_problem &p, const gdp_geom &G) {
+    base = dirs + p.dir_off;
+    layout = p.layout;
+    glog = p.layout == 1 ? p.glog : 4;
+    gmask = (1 << glog) - 1;
+    P = p.layout == 1 ? p.P : G.P;
+    lb = G.lb;
+    ub = G.ub;
+    R = G.R;
+    C = G.C;
+    full = G.full;
+    smask = ~(G.S - 1);
+  }
+  // Word holding sweep cell (row, col) and the cell's step within it;
+  // ok = false outside the computed cells (direction DIAG there).
+  __device__ __forceinline__ const uint32_t *where(int row, int col, int &t, bool &ok) const {
+    const int blk = row & smask;
+    const int cs = full ? (blk - lb > 0 ? blk - lb : 0) : blk;
+    const int ce = row + ub < C ? row + ub : C;
+    ok = (unsigned)row <= (unsigned)R && col >= cs && col <= ce && col >= 0;
+    const int b = row >> glog, i = row & gmask;
+    if (layout == 1) {
+      const int c0 = (b << glog) - lb;
+      t = b * P + i + col - (c0 > 0 ? c0 : 0);
+      return ok ? base + ((t >> 3) << glog) + i : base;
+    }
+    t = b * P + i + col;
+    return ok ? base + ((t >> 3) << 4) + i : base;
+  }
+  __device__ __forceinline__ uint32_t nib(uint32_t w, int t, bool ok) const {
+    if (!ok) return 0;
+    if (layout == 1) {
+      w >>= t & 7;
+      return ((w >> 24) & 1 ? GDP_DIR_V : ((w >> 8) & 1 ? GDP_DIR_H : 0)) | (w & 1 ? GDP_DIR_E : 0) |
+             ((w >> 16) & 1 ? GDP_DIR_F : 0);
+    }
+    return (w >> (4 * (t & 7))) & 15;
+  }
+};
+
+#define WIN_BYTE(W, addr, out)                                                                  \
+  do {                                                                                          \
+    uint32_t w__;                                                                               \
+    WIN_WORD(W, addr, w__);                                                                     \
+    (out) = (int)(int8_t)(w__ >> (8 * ((uintptr_t)(addr) & 3)));                                \
+  } while (0)
+
+// One pair record (at most cap stored; n counts them all).
+#define TB_PUT(qpos, gpos, cdna_, comp_, g_, ga_, gapp_, jump_)                                 \
+  do {                                                                                          \
+    if (n < cap) {                                                                              \
+      gdp_pair pr_;                                                                             \
+      pr_.querypos = (qpos);                                                                    \
+      pr_.genomepos = (gpos);                                                                   \
+      pr_.cdna = (int8_t)(cdna_);                                                               \
+      pr_.comp = (int8_t)(comp_);                                                               \
+      pr_.genome = (int8_t)(g_);                                                                \
+      pr_.genomealt = (int8_t)(ga_);                                                            \
+      pr_.gapp = (int16_t)(gapp_);                                                              \
+      pr_.genomejump = (int16_t)(jump_);                                                        \
+      obuf[n] = pr_;                                                                            \
+    }                                                                                           \
+    n++;                                                                                        \
+  } while (0)
+// Pairpool_push (pairpool.c:179-259): pairs at negative coordinates are dropped.
+#define TB_PUSH(qpos, gpos, cdna_, comp_, g_, ga_)                                              \
+  do {                                                                                          \
+    const int qp__ = (qpos), gp__ = (gpos);                                                     \
+    if (qp__ >= 0 && gp__ >= 0) TB_PUT(qp__, gp__, cdna_, comp_, g_, ga_, 0, 0);                \
+  } while (0)
+
+// The walk visits one cell, or emits one pair of a pending gap, per
+// iteration, so the lanes of a wave run the same short body:
+//   DECIDE  diagonal pair, or the start of a gap run, at (r, c);
+//   RUN_H   one cell of a run to the left: full (dynprog_simd.c:9552-9711)
+//           follows Egap guarded by c > 0, upper (9715-9832) unguarded;
+//   RUN_V   one cell of a run upwards: full follows Fgap guarded by r > 0,
+//           lower (9835-9945) follows its Egap unguarded;
+//   EMIT_H  one pair of Pairpool_add_genomeskip (pairpool.c:1067-1180);
+//   EMIT_V  one pair of Pairpool_add_queryskip (pairpool.c:980-1010);
+//   TAIL    the leftover query or genome skip once r or c reaches 0.
+// Directions and characters are read through 16-byte register windows.
+enum { TB_DECIDE, TB_RUN_H, TB_RUN_V, TB_EMIT_H, TB_EMIT_V, TB_TAIL, TB_DONE };
+
+__global__ __launch_bounds__(256) void gdp_traceback_kernel(
+    const gdp_dev_problem *__restrict__ probs, const gdp_tb_request *__restrict__ reqs, int nreq,
+    const char *__restrict__ qbuf, const char *__restrict__ gbuf, const uint32_t *__restrict__ dirs,
+    const char *__restrict__ genome, uint32_t glen, gdp_pair *__restrict__ pairs, int32_t *__restrict__ counts) {
+  __shared__ uint32_t consb[3 * 128 * 4];
+  for (int j = threadIdx.x; j < 3 * 128 * 4; j += blockDim.x) consb[j] = gdp_tabConsBits[j];
+  __syncthreads();
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nreq) return;
   const gdp_tb_request q = reqs[k];
+  const int queryoffset = q.queryoffset, genomeoffset = q.genomeoffset, watsonp = q.watsonp, revp = q.revp;
+  const uint32_t chroffset = q.chroffset, chrhigh = q.chrhigh;
+  const int consrow = q.genestrand * 128;
+  const bool ops = (q.flags & GDP_TB_OPS) != 0;
+  const int cap = q.max_pairs;
+  gdp_pair *const obuf = pairs + q.pair_off;
   int32_t *cnt = counts + 6 * (int64_t)k;
-  const gdp_dev_problem p = probs[q.fill];
-  const gdp_geom G = gdp_make_geom(p);
-  if (q.r < 0 || q.r > p.rl || q.c < 0 || q.c > p.gl) {  // start cell outside the matrix
+  const gdp_dev_problem *pp = probs + q.fill;
+  const int kind = pp->kind;
+  const int64_t p_qoff = pp->qoff, p_goff = pp->goff, p_galtoff = pp->galtoff;
+  if (q.r < 0 || q.r > pp->rl || q.c < 0 || q.c > pp->gl) {  // start cell outside the matrix
     cnt[0] = -1;
     return;
   }
-  const bool lower = p.kind == GDP_KIND_LOWER;
-  const int revp = q.revp;
-  tb_out out{pairs + q.pair_off, 0, q.max_pairs};
-  int sc = 0, nm = 0, nmm = 0, nop = 0, nind = 0;
+  tb_dirs TD;
+  TD.init(dirs, *pp, gdp_make_geom(*pp));
+  const bool lower = kind == GDP_KIND_LOWER;
+  const uint32_t vbit = kind == GDP_KIND_FULL ? GDP_DIR_F : GDP_DIR_E;
+
+  WIN_DECL(wd);
+  WIN_DECL(wq);
+  WIN_DECL(wg);
+  WIN_DECL(wga);
+  int n = 0, sc = 0, nm = 0, nmm = 0, nop = 0, nind = 0;
   int r = q.r, c = q.c;
+  int mode = TB_DECIDE, after = TB_DECIDE, dist = 0;
+  int eq = 0, eg = 0, estep = 0, left = 0;  // emission cursor
+  int sk_rr = 0, sk_cc = 0, sk_d = 0, sk_next = 0, sk_kind = 0;  // pending skip
 
-  // Direction nibble at query row rr, genome column cc.
-  auto dir = [&](int rr, int cc) -> uint32_t { return lower ? nib_at(dirs, p, G, cc, rr) : nib_at(dirs, p, G, rr, cc); };
-  auto qch = [&](int rr) -> int { return qbuf[p.qoff + rr - 1]; };
-
-  const bool ops = (q.flags & GDP_TB_OPS) != 0;
-  auto push_op = [&](int type, int rr, int cc, int arg, int comp) {
-    if (out.n < out.cap) {
-      gdp_pair pr;
-      pr.querypos = rr;
-      pr.genomepos = cc;
-      pr.cdna = (int8_t)type;
-      pr.comp = (int8_t)comp;
-      pr.genome = 0;
-      pr.genomealt = 0;
-      pr.gapp = 0;
-      pr.genomejump = (int16_t)arg;
-      out.pairs[out.n] = pr;
-    }
-    out.n++;
-  };
-  // Pairpool_add_queryskip (pairpool.c:980-1010).
-  auto queryskip = [&](int rr, int cc, int dist) {
-    if (ops) {
-      push_op('V', rr, cc, dist, 0);
-      return;
-    }
-    int qc = rr - 1, gc = cc - 1, step = -1;
-    if (revp) {
-      qc = -qc;
-      gc = -gc;
-      step = +1;
-    }
-    for (int j = 0; j < dist; j++) {
-      // rsequence[qc] is q(|qc| + 1) in sweep order for both orientations.
-      push_pair(out, q.queryoffset + qc, q.genomeoffset + gc, qch((qc < 0 ? -qc : qc) + 1), '-', ' ', ' ');
-      qc += step;
-    }
-  };
-  // Pairpool_add_genomeskip (pairpool.c:1067-1180) with genomesequence NULL.
-  auto genomeskip = [&](int rr, int cc, int dist) -> bool {
-    int qc = rr - 1, left = cc - dist, right = cc - 1, step = -1;
-    if (revp) {
-      int t = left;
-      qc = -qc;
-      left = -right;
-      right = -t;
-      step = +1;
-    }
-    if (ops) {
-      push_op('H', rr, cc, dist, 0);
-      return dist < MICROINTRON_LENGTH;
-    }
-    if (dist < MICROINTRON_LENGTH) {
-      int gc = revp ? left : right;
-      for (int j = 0; j < dist; j++) {
-        int alt, ch = genomic_nt(&alt, genome, glen, q.genomeoffset + gc, q.chroffset, q.chrhigh, q.watsonp);
-        push_pair(out, q.queryoffset + qc, q.genomeoffset + gc, ' ', '-', ch, alt);
-        gc += step;
+  for (;;) {
+    if (mode == TB_DONE) break;
+    sk_kind = 0;
+    if (mode == TB_DECIDE) {
+      if (r <= 0 || c <= 0) {
+        mode = TB_TAIL;
+        continue;
       }
-      return true;
-    }
-    if (out.n < out.cap) {  // Pairpool_push_gapholder (pairpool.c:374-452)
-      gdp_pair pr;
-      pr.querypos = -1;
-      pr.genomepos = -1;
-      pr.cdna = ' ';
-      pr.comp = ' ';
-      pr.genome = ' ';
-      pr.genomealt = ' ';
-      pr.gapp = 1;
-      pr.genomejump = (int16_t)dist;
-      out.pairs[out.n] = pr;
-    }
-    out.n++;
-    return false;
-  };
-  auto diag_pair = [&](int rr, int cc) {
-    int qc = rr - 1, gc = cc - 1;
-    if (revp) {
-      qc = -qc;
-      gc = -gc;
-    }
-    int c1 = qch(rr), c1uc = (c1 >= 'a' && c1 <= 'z') ? c1 - 32 : c1;
-    int c2 = gbuf[p.goff + cc - 1], c2a = gbuf[p.galtoff + cc - 1];
-    if (c2 == '*') return;
-    int comp;
-    if (c1uc == c2 || c1uc == c2a) {
-      comp = '*';
-    } else if (gdp_tabCons[(q.genestrand * 128 + (c1uc & 127)) * 128 + (c2 & 127)] ||
-               gdp_tabCons[(q.genestrand * 128 + (c1uc & 127)) * 128 + (c2a & 127)]) {
-      comp = ':';
-    } else {
-      comp = ' ';
-    }
-    if (comp == ' ') {
-      sc += TB_MISMATCH;
-      nmm += 1;
-    } else {
-      sc += TB_MATCH;
-      nm += 1;
-    }
-    if (ops) push_op('D', rr, cc, 0, comp);
-    else push_pair(out, q.queryoffset + qc, q.genomeoffset + gc, c1, comp, c2, c2a);
-  };
-
-  if (p.kind == GDP_KIND_FULL) {
-    while (r > 0 && c > 0) {
-      uint32_t d = dir(r, c);
-      if ((d & GDP_DIR_H) && !(d & GDP_DIR_V)) {
-        int dist = 1;
-        while (c > 0 && (dir(r, c--) & GDP_DIR_E)) dist++;
-        if (genomeskip(r, c + dist, dist)) {
-          sc += TB_TOPEN + dist * TB_TINDEL;
-          nop += 1;
-          nind += dist;
+      // characters of a diagonal pair, fetched together with the directions
+      int c1, c2, c2a, t;
+      bool ok;
+      WIN_BYTE(wq, qbuf + p_qoff + r - 1, c1);
+      WIN_BYTE(wg, gbuf + p_goff + c - 1, c2);
+      WIN_BYTE(wga, gbuf + p_galtoff + c - 1, c2a);
+      const uint32_t *a = lower ? TD.where(c, r, t, ok) : TD.where(r, c, t, ok);
+      uint32_t w;
+      WIN_WORD(wd, a, w);
+      const uint32_t d = TD.nib(w, t, ok);
+      int m;
+      if (kind == GDP_KIND_FULL) m = (d & GDP_DIR_V) ? TB_RUN_V : ((d & GDP_DIR_H) ? TB_RUN_H : TB_DECIDE);
+      else m = (d & (GDP_DIR_H | GDP_DIR_V)) ? (kind == GDP_KIND_UPPER ? TB_RUN_H : TB_RUN_V) : TB_DECIDE;
+      if (m == TB_DECIDE) {
+        if (c2 != '*') {
+          const int c1uc = (c1 >= 'a' && c1 <= 'z') ? c1 - 32 : c1;
+          const int row = (consrow + (c1uc & 127)) << 2;
+          int comp;
+          if (c1uc == c2 || c1uc == c2a) comp = '*';
+          else if (((consb[row + ((c2 & 127) >> 5)] >> (c2 & 31)) & 1) ||
+                   ((consb[row + ((c2a & 127) >> 5)] >> (c2a & 31)) & 1))
+            comp = ':';  // consistent_array (dynprog.c:893-900)
+          else comp = ' ';
+          if (comp == ' ') {
+            sc += TB_MISMATCH;
+            nmm += 1;
+          } else {
+            sc += TB_MATCH;
+            nm += 1;
+          }
+          if (ops) TB_PUT(r, c, 'D', comp, 0, 0, 0, 0);
+          else TB_PUSH(queryoffset + (revp ? 1 - r : r - 1), genomeoffset + (revp ? 1 - c : c - 1), c1, comp, c2, c2a);
         }
-      } else if (d & GDP_DIR_V) {
-        int dist = 1;
-        while (r > 0 && (dir(r--, c) & GDP_DIR_F)) dist++;
-        queryskip(r + dist, c, dist);
-        sc += TB_QOPEN + dist * TB_QINDEL;
-        nop += 1;
-        nind += dist;
-      } else {
-        diag_pair(r, c);
         r--;
         c--;
+      } else {
+        mode = m;
+        dist = 1;
       }
-    }
-    if (r == 0 && c == 0) {
-    } else if (c == 0) {
-      queryskip(r, 1, r);
-      sc += TB_QOPEN + r * TB_QINDEL;
-      nop += 1;
-      nind += r;
-    } else {
-      int dist = c;
-      if (genomeskip(1, c, dist)) {
-        sc += TB_TOPEN + dist * TB_TINDEL;
-        nop += 1;
-        nind += dist;
+    } else if (mode == TB_RUN_H || mode == TB_RUN_V) {
+      const bool h = mode == TB_RUN_H;
+      bool more = false;
+      if (kind != GDP_KIND_FULL || (h ? c > 0 : r > 0)) {
+        int t;
+        bool ok;
+        const uint32_t *a = lower ? TD.where(c, r, t, ok) : TD.where(r, c, t, ok);
+        uint32_t w;
+        WIN_WORD(wd, a, w);
+        more = (TD.nib(w, t, ok) & (h ? GDP_DIR_E : vbit)) != 0;
+        if (h) c--;
+        else r--;
       }
-    }
-  } else if (p.kind == GDP_KIND_UPPER) {
-    while (r > 0 && c > 0) {
-      if (dir(r, c) & (GDP_DIR_H | GDP_DIR_V)) {
-        int dist = 1;
-        while (dir(r, c--) & GDP_DIR_E) dist++;
-        if (genomeskip(r, c + dist, dist)) {
-          sc += TB_TOPEN + dist * TB_TINDEL;
-          nop += 1;
-          nind += dist;
+      if (more) {
+        dist++;
+      } else {
+        sk_kind = h ? 1 : 2;
+        sk_rr = h ? r : r + dist;
+        sk_cc = h ? c + dist : c;
+        sk_d = dist;
+        sk_next = TB_DECIDE;
+      }
+    } else if (mode == TB_EMIT_H) {
+      // get_genomic_nt (pairpool.c:1016-1063) over the device genome
+      const int gpos = genomeoffset + eg;
+      const uint32_t pos = watsonp ? chroffset + (uint32_t)gpos : chrhigh - (uint32_t)gpos;
+      int ch;
+      if (pos < chroffset || pos >= chrhigh) ch = '*';
+      else {
+        ch = (genome != nullptr && pos < glen) ? genome[pos] : 'N';
+        if (!watsonp) ch = compl_char(ch);
+      }
+      TB_PUSH(queryoffset + eq, gpos, ' ', '-', ch, ch);
+      eg += estep;
+      if (--left == 0) mode = after;
+    } else if (mode == TB_EMIT_V) {
+      // rsequence[eq] is q(|eq| + 1) in sweep order for both orientations
+      int ch;
+      WIN_BYTE(wq, qbuf + p_qoff + (eq < 0 ? -eq : eq), ch);
+      TB_PUSH(queryoffset + eq, genomeoffset + eg, ch, '-', ' ', ' ');
+      eq += estep;
+      if (--left == 0) mode = after;
+    } else {  // TB_TAIL
+      mode = TB_DONE;
+      if (kind == GDP_KIND_FULL) {
+        if (r == 0 && c == 0) {
+        } else if (c == 0) {
+          sk_kind = 2, sk_rr = r, sk_cc = 1, sk_d = r;
+        } else {
+          sk_kind = 1, sk_rr = 1, sk_cc = c, sk_d = c;
         }
+      } else if (kind == GDP_KIND_UPPER) {
+        if (c != 0) sk_kind = 1, sk_rr = 1, sk_cc = c, sk_d = c;
       } else {
-        diag_pair(r, c);
-        r--;
-        c--;
+        if (r != 0) sk_kind = 2, sk_rr = r, sk_cc = 1, sk_d = r;
       }
+      sk_next = TB_DONE;
     }
-    if (c != 0) {
-      int dist = c;
-      if (genomeskip(1, c, dist)) {
-        sc += TB_TOPEN + dist * TB_TINDEL;
+    if (sk_kind == 1) {
+      // Pairpool_add_genomeskip with genomesequence NULL: a gap of 9 or more
+      // (MICROINTRON_LENGTH) becomes one gap holder and is not scored.
+      int qc = sk_rr - 1, lo = sk_cc - sk_d, hi = sk_cc - 1, stp = -1;
+      if (revp) {
+        const int tmp = lo;
+        qc = -qc;
+        lo = -hi;
+        hi = -tmp;
+        stp = +1;
+      }
+      mode = sk_next;
+      if (sk_d < MICROINTRON_LENGTH) {
+        sc += TB_TOPEN + sk_d * TB_TINDEL;
         nop += 1;
-        nind += dist;
+        nind += sk_d;
       }
-    }
-  } else {
-    while (r > 0 && c > 0) {
-      if (dir(r, c) & (GDP_DIR_H | GDP_DIR_V)) {
-        int dist = 1;
-        while (dir(r--, c) & GDP_DIR_E) dist++;
-        queryskip(r + dist, c, dist);
-        sc += TB_QOPEN + dist * TB_QINDEL;
-        nop += 1;
-        nind += dist;
-      } else {
-        diag_pair(r, c);
-        r--;
-        c--;
+      if (ops) {
+        TB_PUT(sk_rr, sk_cc, 'H', 0, 0, 0, 0, sk_d);
+      } else if (sk_d >= MICROINTRON_LENGTH) {
+        TB_PUT(-1, -1, ' ', ' ', ' ', ' ', 1, sk_d);  // Pairpool_push_gapholder (pairpool.c:374-452)
+      } else if (sk_d > 0) {
+        eq = qc;
+        eg = revp ? lo : hi;
+        estep = stp;
+        left = sk_d;
+        after = sk_next;
+        mode = TB_EMIT_H;
       }
-    }
-    if (r != 0) {
-      queryskip(r, 1, r);
-      sc += TB_QOPEN + r * TB_QINDEL;
+    } else if (sk_kind == 2) {
+      int qc = sk_rr - 1, gc = sk_cc - 1, stp = -1;
+      if (revp) {
+        qc = -qc;
+        gc = -gc;
+        stp = +1;
+      }
+      sc += TB_QOPEN + sk_d * TB_QINDEL;
       nop += 1;
-      nind += r;
+      nind += sk_d;
+      mode = sk_next;
+      if (ops) {
+        TB_PUT(sk_rr, sk_cc, 'V', 0, 0, 0, 0, sk_d);
+      } else if (sk_d > 0) {
+        eq = qc;
+        eg = gc;
+        estep = stp;
+        left = sk_d;
+        after = sk_next;
+        mode = TB_EMIT_V;
+      }
     }
   }
-  cnt[0] = out.n;
+  cnt[0] = n;
   cnt[1] = sc;
   cnt[2] = nm;
   cnt[3] = nmm;
@@ -607,7 +708,7 @@ extern "C" int gdp_launch_traceback(const void *d_descs, const gdp_tb_request *d
                                     const char *d_gbuf, const uint32_t *d_dirs, const char *d_genome, uint32_t glen,
                                     gdp_pair *d_pairs, int32_t *d_counts, hipStream_t stream) {
   if (nreq <= 0) return 0;
-  hipLaunchKernelGGL(gdp_traceback_kernel, dim3((nreq + 127) / 128), dim3(128), 0, stream,
+  hipLaunchKernelGGL(gdp_traceback_kernel, dim3((nreq + 255) / 256), dim3(256), 0, stream,
                      (const gdp_dev_problem *)d_descs, d_reqs, nreq, d_qbuf, d_gbuf, d_dirs, d_genome, glen, d_pairs,
                      d_counts);
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -633,5 +734,9 @@ extern "C" int gdp_upload_tables(const uint32_t *PQ, const uint32_t *PG, const i
   if (hipMemcpyToSymbol(HIP_SYMBOL(gdp_tabPG), PG, sizeof(uint32_t) * 4 * 128) != hipSuccess) return -1;
   if (hipMemcpyToSymbol(HIP_SYMBOL(gdp_tabNT), NT, 128) != hipSuccess) return -1;
   if (hipMemcpyToSymbol(HIP_SYMBOL(gdp_tabCons), cons, 3 * 128 * 128) != hipSuccess) return -1;
+  uint32_t bits[3 * 128 * 4] = {0};
+  for (int k = 0; k < 3 * 128 * 128; k++)
+    if (cons[k]) bits[(k >> 7) * 4 + ((k & 127) >> 5)] |= 1u << (k & 31);
+  if (hipMemcpyToSymbol(HIP_SYMBOL(gdp_tabConsBits), bits, sizeof bits) != hipSuccess) return -1;
   return 0;
 }

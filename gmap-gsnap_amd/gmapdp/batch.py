@@ -74,8 +74,9 @@ class DeviceBatch:
         self.dir_words, self.score_slots = dw, ss
         dev = torch.device(device)
         self.d_descs = torch.from_numpy(descs).to(dev)
-        self.d_q = torch.from_numpy(np.frombuffer(bytes(qbuf) + b"\0", np.uint8).copy()).to(dev)
-        self.d_g = torch.from_numpy(np.frombuffer(bytes(gbuf) + b"\0", np.uint8).copy()).to(dev)
+        # 16 bytes of padding: the traceback reads sequences in aligned 16-byte windows
+        self.d_q = torch.from_numpy(np.frombuffer(bytes(qbuf) + b"\0" * 16, np.uint8).copy()).to(dev)
+        self.d_g = torch.from_numpy(np.frombuffer(bytes(gbuf) + b"\0" * 16, np.uint8).copy()).to(dev)
         self.d_dirs = torch.zeros(max(dw, 1), dtype=torch.int32, device=dev)
         self.d_scores = torch.zeros(max(ss, 1), dtype=torch.int16, device=dev)
         self.d_final = torch.zeros(max(self.n, 1), dtype=torch.int32, device=dev)

@@ -188,7 +188,10 @@ int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_g
 int gdp_best_endpoint(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,
                       const int16_t *d_scores, int32_t *d_out, void *stream);
 
-/* Tracebacks, one GPU thread per request.  d_genome (may be NULL) holds the
+/* Tracebacks, one GPU thread per request.  d_qbuf, d_gbuf and d_dirs are
+   read in aligned 16-byte windows: they must be 16-byte aligned and readable
+   for 16 bytes past their last used byte (the dir_words of gdp_plan include
+   that padding).  d_genome (may be NULL) holds the
    chromosome as ASCII bases indexed by absolute position, used for genome
    skips exactly as get_genomic_nt (pairpool.c:1016-1063) uses
    Genome_get_char; positions outside [chroffset, chrhigh) give '*'.
