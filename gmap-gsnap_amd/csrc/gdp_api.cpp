@@ -100,7 +100,8 @@ static int validate(const gdp_problem &p, int i) {
 
 extern "C" int64_t gdp_plan_bytes(int32_t n) {
   if (n < 0) n = 0;
-  return (int64_t)n * (int64_t)(sizeof(gdp_dev_problem) + sizeof(gdp_wave)) + (int64_t)sizeof(gdp_pair_hdr);
+  return (int64_t)n * (int64_t)(sizeof(gdp_dev_problem) + sizeof(gdp_wave) + sizeof(int32_t)) +
+         (int64_t)sizeof(gdp_pair_hdr);
 }
 
 // Full-band int16 fills without kept scores are swept in pairs, 2*(64/G) per
@@ -109,10 +110,13 @@ extern "C" int64_t gdp_plan_bytes(int32_t n) {
 static int pair_lanes(const gdp_problem &p) {
   if (p.kind != GDP_KIND_FULL || p.width != 16 || p.keep_scores) return 0;
   const int W = p.lband + p.uband;
-  if (W <= 224) return 16;
-  if (W <= 936) return 64;
+  if (W <= 240) return 16;
+  if (W <= 960) return 64;
   return 0;
 }
+
+// Wave class: the ring of a class holds at least P step slots.
+static int pair_class(int G, int P) { return G == 64 ? 2 : (P <= 128 ? 0 : 1); }
 
 extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, void *descs_out, int64_t *dir_words,
                            int64_t *score_slots) {
@@ -167,7 +171,7 @@ extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, v
     if (x.rl != y.rl) return x.rl < y.rl;
     return x.gl < y.gl;
   });
-  std::vector<gdp_wave> waves[2];
+  std::vector<gdp_wave> waves[GDP_NCLS];
   size_t k = 0;
   while (k < paired.size()) {
     const gdp_problem &p0 = probs[paired[k]];
@@ -222,17 +226,17 @@ extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, v
       w.fin_t[j] = fin[j].first;
       w.fin_slot[j] = fin[j].second;
     }
-    waves[G == 16 ? 0 : 1].push_back(w);
+    waves[pair_class(G, P)].push_back(w);
     k = e;
   }
   // Empty slots of partial waves store into one shared sink.
   int64_t sink_words = 0;
-  for (int c = 0; c < 2; c++)
-    for (const gdp_wave &w : waves[c]) sink_words = std::max<int64_t>(sink_words, (int64_t)(w.nsteps / 8) * (c ? 64 : 16));
+  for (int c = 0; c < GDP_NCLS; c++)
+    for (const gdp_wave &w : waves[c]) sink_words = std::max<int64_t>(sink_words, (int64_t)(w.nsteps / 8) * (1 << w.glog));
   const int64_t sink_off = dw;
   dw += sink_words;
   // Longest sweeps first, so the queue ends on short ones.
-  for (int c = 0; c < 2; c++)
+  for (int c = 0; c < GDP_NCLS; c++)
     std::stable_sort(waves[c].begin(), waves[c].end(),
                      [](const gdp_wave &a, const gdp_wave &b) { return a.nsteps > b.nsteps; });
 
@@ -240,15 +244,21 @@ extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, v
   memset(&h, 0, sizeof h);
   h.magic = GDP_PAIR_MAGIC ^ (uint64_t)n;
   h.n = n;
-  h.nw[0] = (int32_t)waves[0].size();
-  h.nw[1] = (int32_t)waves[1].size();
+  h.ncls = GDP_NCLS;
   const int64_t hoff = (int64_t)n * sizeof(gdp_dev_problem);
-  h.off[0] = hoff + (int64_t)sizeof(gdp_pair_hdr);
-  h.off[1] = h.off[0] + (int64_t)waves[0].size() * (int64_t)sizeof(gdp_wave);
+  int64_t off = hoff + (int64_t)sizeof(gdp_pair_hdr);
+  for (int c = 0; c < GDP_NCLS; c++) {
+    h.nw[c] = (int32_t)waves[c].size();
+    h.off[c] = off;
+    if (!waves[c].empty()) memcpy(base + off, waves[c].data(), waves[c].size() * sizeof(gdp_wave));
+    off += (int64_t)waves[c].size() * (int64_t)sizeof(gdp_wave);
+  }
+  for (int c = 0; c < GDP_NCLS; c++) {
+    h.gen_off[c] = off;
+    off += (int64_t)waves[c].size() * (int64_t)sizeof(int32_t);
+  }
   h.sink_off = sink_off;
   memcpy(base + hoff, &h, sizeof h);
-  for (int c = 0; c < 2; c++)
-    if (!waves[c].empty()) memcpy(base + h.off[c], waves[c].data(), waves[c].size() * sizeof(gdp_wave));
   dw += 4;  // the traceback reads direction words in aligned 16-byte windows
   if (dir_words) *dir_words = dw;
   if (score_slots) *score_slots = ss;

@@ -71,20 +71,27 @@ struct gdp_wave {
 };
 static_assert(sizeof(gdp_wave) == 128, "gdp_wave must stay 128 bytes");
 
-// Header written after the n problem descriptors.  The fill checks magic and
-// n before it touches the wave lists, so a descriptor array handed over
-// with a different n never reaches them.
+// Header written after the n problem descriptors, followed by the wave
+// lists of the classes (16-lane groups with P <= 128, 16-lane groups with
+// P <= 256, 64-lane groups) and one int32 list per class of waves handed to
+// the general-lookup pass.  The fill checks magic and n before it touches
+// the lists, so a descriptor array handed over with another n never does.
 #define GDP_PAIR_MAGIC 0x6764703270616972ull
+#define GDP_NCLS 3
 struct gdp_pair_hdr {
-  uint64_t magic;       // GDP_PAIR_MAGIC ^ n
+  uint64_t magic;           // GDP_PAIR_MAGIC ^ n
   int32_t n;
-  int32_t nw[2];        // waves with 16- and 64-lane groups
-  int32_t pad0;
-  int64_t off[2];       // byte offset of each wave list from the descriptor base
-  int32_t queue[2][2];  // per list: next wave to hand out, waves that have left the queue
-  int64_t sink_off;     // direction words written by the empty slots of partial waves
+  int32_t ncls;
+  int32_t nw[4];            // waves per class
+  int64_t off[4];           // byte offset of each class's wave list from the descriptor base
+  int64_t gen_off[4];       // byte offset of each class's general-lookup list
+  int32_t queue[4][2];      // per class: next wave to hand out, waves that have left the queue
+  int32_t gen_queue[4][2];  // the same for the general-lookup pass
+  int32_t gen_count[4];     // waves in each general-lookup list
+  int64_t sink_off;         // direction words written by the empty slots of partial waves
+  int32_t pad[18];
 };
-static_assert(sizeof(gdp_pair_hdr) == 64, "gdp_pair_hdr must stay 64 bytes");
+static_assert(sizeof(gdp_pair_hdr) == 256, "gdp_pair_hdr must stay 256 bytes");
 
 // Sweep geometry shared by the fill, traceback, export and scan kernels.
 struct gdp_geom {

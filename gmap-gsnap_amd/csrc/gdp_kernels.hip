@@ -671,20 +671,24 @@ extern "C" int gdp_launch_export_ref(const void *d_descs, int idx, int width, co
 
 // Persistent grid of the paired fill: as many workgroups as fit on the device
 // at once, each wave taking waves off the descriptor's queue.
-template <int G>
-static int p2_grid() {
+template <int G, int RING, bool GENK>
+static int p2_launch(const void *d_descs, int n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
+                     int32_t *d_final, hipStream_t stream) {
+  typedef p2cfg<G, RING, GENK ? 4 : 1> cfg;
   static int cached[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
   if (cached[dev] == 0) {
     int ncu = 0, per = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill2_kernel<G>, P2_THREADS, p2cfg<G>::LDS_BYTES) !=
-        hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill2_kernel<G, RING, GENK>, P2_THREADS,
+                                                     cfg::LDS_BYTES) != hipSuccess)
       return -1;
     cached[dev] = ncu * (per > 0 ? per : 1);
   }
-  return cached[dev];
+  hipLaunchKernelGGL((gdp_fill2_kernel<G, RING, GENK>), dim3(cached[dev]), dim3(P2_THREADS), cfg::LDS_BYTES, stream,
+                     (const uint8_t *)d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 extern "C" int gdp_launch_fill(const void *d_descs, int n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
@@ -694,14 +698,13 @@ extern "C" int gdp_launch_fill(const void *d_descs, int n, const char *d_qbuf, c
   hipLaunchKernelGGL(gdp_fill_kernel, dim3(blocks), dim3(GDP_BLOCK), 0, stream, (const gdp_dev_problem *)d_descs, n,
                      d_qbuf, d_gbuf, d_dirs, d_scores, d_final);
   if (hipGetLastError() != hipSuccess) return -1;
-  const int g16 = p2_grid<16>(), g64 = p2_grid<64>();
-  if (g16 <= 0 || g64 <= 0) return -1;
-  hipLaunchKernelGGL(gdp_fill2_kernel<16>, dim3(g16), dim3(P2_THREADS), p2cfg<16>::LDS_BYTES, stream,
-                     (const uint8_t *)d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final);
-  if (hipGetLastError() != hipSuccess) return -1;
-  hipLaunchKernelGGL(gdp_fill2_kernel<64>, dim3(g64), dim3(P2_THREADS), p2cfg<64>::LDS_BYTES, stream,
-                     (const uint8_t *)d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  if (p2_launch<16, 128, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if (p2_launch<16, 256, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if (p2_launch<64, 1024, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if (p2_launch<16, 128, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if (p2_launch<16, 256, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if (p2_launch<64, 1024, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  return 0;
 }
 
 extern "C" int gdp_launch_traceback(const void *d_descs, const gdp_tb_request *d_reqs, int nreq, const char *d_qbuf,
