@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--percall", type=int, default=2000, help="problems timed one launch each (0 = skip)")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="before the timed region, copy MIB MiB once on the device (a known-size kernel that "
+                         "calibrates HBM byte counters in a profiled run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (problems are independent; "
@@ -113,37 +116,48 @@ def main():
     reqs = wl.tb_requests()
     stream = torch.cuda.current_stream()
 
-    def step(ev=None):
+    def step(ev=None, tev=None):
         if ev is not None:
             ev[0].record(stream)
         b.fill()
         if ev is not None:
             ev[1].record(stream)
         b.traceback_device(reqs_dev)
+        if tev is not None:
+            tev.record(stream)
 
     reqs_dev = b.prepare_traceback(reqs, genome=wl.genome())
+    if args.calib_copy:
+        src = torch.ones(args.calib_copy << 18, dtype=torch.int32, device=b.dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB read, %d MiB written" % (args.calib_copy, args.calib_copy))
+        del src, dst
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    tevs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        step(evs[k])
+        step(evs[k], tevs[k])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     fill_ms = float(np.mean([a.elapsed_time(z) for a, z in evs]))
+    tb_ms = float(np.mean([z.elapsed_time(t) for (a, z), t in zip(evs, tevs)]))
     ms_per_step = dt * 1e3 / args.steps
     gcups = total_cells * args.steps / dt / 1e9
     achieved = total_cells * OPS_PER_CELL / (fill_ms * 1e-3) / 1e12
-    log("fill %.2f ms, step %.2f ms, %.1f GCUPS end-to-end, fill-only %.1f GCUPS"
-        % (fill_ms, ms_per_step, gcups, total_cells / (fill_ms * 1e-3) / 1e9))
+    log("fill %.2f ms, traceback %.2f ms, step %.2f ms, %.1f GCUPS end-to-end, fill-only %.1f GCUPS"
+        % (fill_ms, tb_ms, ms_per_step, gcups, total_cells / (fill_ms * 1e-3) / 1e9))
 
     # Plain one-launch-per-problem form on the same inputs.
     percall = None
     if args.percall:
         m = min(args.percall, n)
-        sub = DeviceBatch(probs[:m], wl.qbuf.tobytes(), wl.gbuf.tobytes(), single=True)
+        sub = DeviceBatch(probs[:m], wl.qbuf.tobytes(), wl.gbuf.tobytes(), single=True)  # one sweep per problem
         sreq = sub.prepare_traceback(wl.tb_requests()[:m], genome=wl.genome())
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -182,12 +196,15 @@ def main():
         "dtype": "int16",
         "data": "synthetic (seeded generator of the workload)",
         "config": {"workload": desc, "problems": n, "cells": total_cells, "step": "fill + traceback"},
+        "fill_ms": round(fill_ms, 3),
+        "traceback_ms": round(tb_ms, 3),
         "fill_gcups": round(total_cells / (fill_ms * 1e-3) / 1e9, 3),
         "percall_gcups": None if percall is None else round(percall, 4),
         "batched_vs_percall": None if percall is None else round(gcups / percall, 1),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
                      "frac": round(achieved / PEAK_INT16_TOPS, 4), "traffic": traffic,
-                     "kernel": "gdp_fill_kernel", "ops_per_cell": OPS_PER_CELL},
+                     "kernel": "gdp_fill2_kernel (one fill: every launch of gdp_fill, HIP events on its stream)",
+                     "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -16,8 +16,8 @@
 #include "gdp_internal.h"
 #include "gdp_tables.h"
 
-extern "C" int gdp_launch_fill(const void *, int, const char *, const char *, uint32_t *, int16_t *, int32_t *,
-                               hipStream_t);
+extern "C" int gdp_launch_fill(const void *, int, uint32_t, const char *, const char *, uint32_t *, int16_t *,
+                               int32_t *, hipStream_t);
 extern "C" int gdp_launch_traceback(const void *, const gdp_tb_request *, int, const char *, const char *,
                                     const uint32_t *, const char *, uint32_t, gdp_pair *, int32_t *, hipStream_t);
 extern "C" int gdp_launch_best_endpoint(const void *, const int32_t *, int, int, const int16_t *, int32_t *,
@@ -291,14 +291,36 @@ extern "C" int64_t gdp_cells(const gdp_problem *p) {
   return n;
 }
 
-extern "C" int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
-                        int16_t *d_scores, int32_t *d_final, void *stream) {
+extern "C" uint32_t gdp_plan_kernels(const void *descs, int32_t n) {
+  if (descs == nullptr || n < 0) return GDP_FILL_ALL;
+  const gdp_dev_problem *d = (const gdp_dev_problem *)descs;
+  gdp_pair_hdr h;
+  memcpy(&h, (const uint8_t *)descs + (int64_t)n * sizeof(gdp_dev_problem), sizeof h);
+  if (h.magic != (GDP_PAIR_MAGIC ^ (uint64_t)n)) return GDP_FILL_ALL;
+  uint32_t mask = 0;
+  for (int32_t i = 0; i < n; i++)
+    if (d[i].layout == 0) {
+      mask |= 1u;
+      break;
+    }
+  for (int c = 0; c < GDP_NCLS; c++)
+    if (h.nw[c] > 0) mask |= 2u << c;
+  return mask;
+}
+
+extern "C" int gdp_fill_ex(const void *d_descs, int32_t n, uint32_t kernels, const char *d_qbuf, const char *d_gbuf,
+                           uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, void *stream) {
   if (!g_init) return fail(GDP_ENOINIT, "gdp_fill: gdp_init has not been called");
   if (n < 0 || (n > 0 && (!d_descs || !d_qbuf || !d_gbuf || !d_dirs || !d_final)))
     return fail(GDP_EINVAL, "gdp_fill: null argument");
-  if (gdp_launch_fill(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_scores, d_final, (hipStream_t)stream))
+  if (gdp_launch_fill(d_descs, n, kernels, d_qbuf, d_gbuf, d_dirs, d_scores, d_final, (hipStream_t)stream))
     return hip_fail("gdp_fill launch", hipGetLastError());
   return GDP_OK;
+}
+
+extern "C" int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
+                        int16_t *d_scores, int32_t *d_final, void *stream) {
+  return gdp_fill_ex(d_descs, n, GDP_FILL_ALL, d_qbuf, d_gbuf, d_dirs, d_scores, d_final, stream);
 }
 
 extern "C" int gdp_best_endpoint(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,

@@ -383,6 +383,7 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
   const uint32_t chroffset = q.chroffset, chrhigh = q.chrhigh;
   const int consrow = q.genestrand * 128;
   const bool ops = (q.flags & GDP_TB_OPS) != 0;
+  const bool nogaps = (q.flags & GDP_TB_NOGAPS) != 0;
   const int cap = q.max_pairs;
   gdp_pair *const obuf = pairs + q.pair_off;
   int32_t *cnt = counts + 6 * (int64_t)k;
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
       const uint32_t *a = lower ? TD.where(c, r, t, ok) : TD.where(r, c, t, ok);
       uint32_t w;
       WIN_WORD(wd, a, w);
-      const uint32_t d = TD.nib(w, t, ok);
+      const uint32_t d = nogaps ? 0u : TD.nib(w, t, ok);
       int m;
       if (kind == GDP_KIND_FULL) m = (d & GDP_DIR_V) ? TB_RUN_V : ((d & GDP_DIR_H) ? TB_RUN_H : TB_DECIDE);
       else m = (d & (GDP_DIR_H | GDP_DIR_V)) ? (kind == GDP_KIND_UPPER ? TB_RUN_H : TB_RUN_V) : TB_DECIDE;
@@ -499,7 +500,9 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
       if (--left == 0) mode = after;
     } else {  // TB_TAIL
       mode = TB_DONE;
-      if (kind == GDP_KIND_FULL) {
+      if (nogaps) {
+        // traceback_nogaps (dynprog_end.c:648-722) stops without a skip
+      } else if (kind == GDP_KIND_FULL) {
         if (r == 0 && c == 0) {
         } else if (c == 0) {
           sk_kind = 2, sk_rr = r, sk_cc = 1, sk_d = r;
@@ -691,19 +694,22 @@ static int p2_launch(const void *d_descs, int n, const char *d_qbuf, const char 
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-extern "C" int gdp_launch_fill(const void *d_descs, int n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
-                               int16_t *d_scores, int32_t *d_final, hipStream_t stream) {
+// kernels: bit 0 single sweeps, bits 1..3 the paired classes (gdp_plan_kernels).
+extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, const char *d_qbuf, const char *d_gbuf,
+                               uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, hipStream_t stream) {
   if (n <= 0) return 0;
-  int blocks = (n + GDP_GROUPS_PER_BLOCK - 1) / GDP_GROUPS_PER_BLOCK;
-  hipLaunchKernelGGL(gdp_fill_kernel, dim3(blocks), dim3(GDP_BLOCK), 0, stream, (const gdp_dev_problem *)d_descs, n,
-                     d_qbuf, d_gbuf, d_dirs, d_scores, d_final);
-  if (hipGetLastError() != hipSuccess) return -1;
-  if (p2_launch<16, 128, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
-  if (p2_launch<16, 256, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
-  if (p2_launch<64, 1024, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
-  if (p2_launch<16, 128, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
-  if (p2_launch<16, 256, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
-  if (p2_launch<64, 1024, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if (kernels & 1u) {
+    int blocks = (n + GDP_GROUPS_PER_BLOCK - 1) / GDP_GROUPS_PER_BLOCK;
+    hipLaunchKernelGGL(gdp_fill_kernel, dim3(blocks), dim3(GDP_BLOCK), 0, stream, (const gdp_dev_problem *)d_descs, n,
+                       d_qbuf, d_gbuf, d_dirs, d_scores, d_final);
+    if (hipGetLastError() != hipSuccess) return -1;
+  }
+  if ((kernels & 2u) && p2_launch<16, 128, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if ((kernels & 4u) && p2_launch<16, 256, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if ((kernels & 8u) && p2_launch<64, 1024, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if ((kernels & 2u) && p2_launch<16, 128, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if ((kernels & 4u) && p2_launch<16, 256, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if ((kernels & 8u) && p2_launch<64, 1024, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
   return 0;
 }
 

@@ -71,6 +71,7 @@ class DeviceBatch:
         self.n = len(self.problems)
         self.single = single
         descs, dw, ss = plan(self.problems, PLAN_SINGLE if single else 0)
+        self.kernels = int(self.L.gdp_plan_kernels(descs.ctypes.data, self.n))
         self.dir_words, self.score_slots = dw, ss
         dev = torch.device(device)
         self.d_descs = torch.from_numpy(descs).to(dev)
@@ -88,8 +89,9 @@ class DeviceBatch:
 
     def fill(self):
         t = self
-        _check(t.L, t.L.gdp_fill(t.d_descs.data_ptr(), t.n, t.d_q.data_ptr(), t.d_g.data_ptr(), t.d_dirs.data_ptr(),
-                                 t.d_scores.data_ptr(), t.d_final.data_ptr(), t.stream), "gdp_fill")
+        _check(t.L, t.L.gdp_fill_ex(t.d_descs.data_ptr(), t.n, t.kernels, t.d_q.data_ptr(), t.d_g.data_ptr(),
+                                    t.d_dirs.data_ptr(), t.d_scores.data_ptr(), t.d_final.data_ptr(), t.stream),
+               "gdp_fill")
 
     def final_scores(self):
         return self.d_final[: self.n].cpu().numpy()

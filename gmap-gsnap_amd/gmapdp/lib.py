@@ -81,6 +81,8 @@ def load(path=None):
         "gdp_plan": (ctypes.c_int, [vp, i32, vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "gdp_plan_ex": (ctypes.c_int, [vp, i32, i32, vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "gdp_plan_bytes": (i64, [i32]),
+        "gdp_plan_kernels": (ctypes.c_uint32, [vp, i32]),
+        "gdp_fill_ex": (ctypes.c_int, [vp, i32, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]),
         "gdp_cells": (i64, [vp]),
         "gdp_fill": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, vp, vp]),
         "gdp_best_endpoint": (ctypes.c_int, [vp, vp, i32, i32, vp, vp, vp]),

@@ -3,7 +3,7 @@
  *
  * This is the batched interface: many independent DP problems are described
  * in one array, their sequences sit in two byte buffers in device memory, and
- * each call launches one kernel over the whole array.  The per-call
+ * each call launches the kernels the whole array needs.  The per-call
  * interface with the reference's own signatures is in gmapdp_dynprog.h.
  *
  * Semantics follow the reference's AVX2 build of GMAP's DP core
@@ -104,6 +104,13 @@ typedef struct gdp_tb_request {
    a host pair pool and genome replays them (gmapdp_dynprog.h). */
 #define GDP_TB_OPS 1
 
+/* With GDP_TB_NOGAPS the walk is traceback_nogaps (dynprog_end.c:648-722):
+   diagonal pairs only, from (r, c) while r > 0 and c > 0, no directions
+   read and no closing skip.  Its start cell is the closed form of
+   find_best_endpoint_to_queryend_nogaps (dynprog_end.c:576-586):
+   r = c = min(rl, gl). */
+#define GDP_TB_NOGAPS 2
+
 /* One traceback output record, in the order the reference pushes them onto
    its pair list (from the end cell back to the start).  gapp = 1 marks a
    gap holder (pairpool.c:374-452) whose genomejump is the skipped length. */
@@ -179,6 +186,15 @@ int64_t gdp_cells(const gdp_problem *prob);
    dynprog_single.c:603), or INT32_MIN when that cell is outside the band. */
 int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
              int16_t *d_scores, int32_t *d_final, void *stream);
+
+/* The kernels a plan needs, read from the host copy of its descriptors:
+   bit 0 single sweeps, bits 1..3 the paired wave classes.  gdp_fill_ex
+   launches only those (gdp_fill launches all, and empty ones return at
+   once). */
+#define GDP_FILL_ALL 0xFu
+uint32_t gdp_plan_kernels(const void *descs, int32_t n);
+int gdp_fill_ex(const void *d_descs, int32_t n, uint32_t kernels, const char *d_qbuf, const char *d_gbuf,
+                uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, void *stream);
 
 /* Best end cell of end problems: for each of m pairs (upper, lower) of
    one-sided fills with keep_scores set, find_best_endpoint_{8,16}

@@ -486,6 +486,20 @@ static void diag_pair(pairbuf_t *pb, int *counters, int r, int c, const char *rs
   }
 }
 
+/* traceback_nogaps (dynprog_end.c:648-722): diagonal pairs from (r, c) while
+   r > 0 and c > 0, scored as in the gapped walks, no closing skip. */
+int orc_traceback_nogaps(int r, int c, const char *rseq, const char *rsequc, const char *gseq, const char *gseq_alt,
+                         int queryoffset, int genomeoffset, int revp, int genestrand, int *counters, int *pairs,
+                         int maxpairs) {
+  pairbuf_t pb = {pairs, 0, maxpairs};
+  while (r > 0 && c > 0) {
+    diag_pair(&pb, counters, r, c, rseq, rsequc, gseq, gseq_alt, queryoffset, genomeoffset, revp, genestrand);
+    r--;
+    c--;
+  }
+  return pb.n;
+}
+
 int orc_traceback(int kind, const unsigned char *dirs, int rl, int gl, int r, int c,
                   const char *rseq, const char *rsequc, const char *gseq, const char *gseq_alt,
                   int queryoffset, int genomeoffset, const orc_genome_t *genome,

@@ -78,6 +78,9 @@ typedef struct {
    dirs layout as produced by the matching fill.  Returns the number of
    pair records written to pairs (capacity maxpairs records).
    counters = {traceback_score, nmatches, nmismatches, nopens, nindels} (added to). */
+int orc_traceback_nogaps(int r, int c, const char *rseq, const char *rsequc, const char *gseq, const char *gseq_alt,
+                         int queryoffset, int genomeoffset, int revp, int genestrand, int *counters, int *pairs,
+                         int maxpairs);
 int orc_traceback(int kind, const unsigned char *dirs, int rl, int gl, int r, int c,
                   const char *rseq, const char *rsequc, const char *gseq, const char *gseq_alt,
                   int queryoffset, int genomeoffset, const orc_genome_t *genome,

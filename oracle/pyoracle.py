@@ -71,6 +71,8 @@ class Oracle(_Common):
         L.orc_traceback.argtypes = [_i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _i, _i, _p, _i,
                                     _u, _u, _i, _i, _i, _p, _p, _i]
         L.orc_traceback.restype = _i
+        L.orc_traceback_nogaps.argtypes = [_i, _i, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p, _i]
+        L.orc_traceback_nogaps.restype = _i
         L.orc_best_endpoint.argtypes = [_i, _p, _p, _i, _i, _i, _i, _i, _i, _p, _p, _p]
         L.orc_segment_right.argtypes = [_p, _p, _u, _u, _u, _i]
         L.orc_segment_left.argtypes = [_p, _p, _u, _u, _u, _i]
@@ -141,6 +143,16 @@ class Oracle(_Common):
         n = self.lib.orc_traceback(kind, _np_ptr(dirs), rl, gl, r, c, q.ptr, quc.ptr, g.ptr, ga.ptr,
                                    queryoffset, genomeoffset, gptr, int(revp), chroffset, chrhigh,
                                    int(watsonp), genestrand, 0, _np_ptr(cnt), _np_ptr(out), maxpairs)
+        return out[:n].copy(), cnt
+
+    def traceback_nogaps(self, r, c, rseq, rsequc, gseq, galt, queryoffset=0, genomeoffset=0, revp=False,
+                         genestrand=0, maxpairs=8192):
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        quc = _Seq(rsequc, revp)
+        out = np.zeros((maxpairs, PAIR_FIELDS), np.int32)
+        cnt = np.zeros(5, np.int32)
+        n = self.lib.orc_traceback_nogaps(r, c, q.ptr, quc.ptr, g.ptr, ga.ptr, queryoffset, genomeoffset, int(revp),
+                                          genestrand, _np_ptr(cnt), _np_ptr(out), maxpairs)
         return out[:n].copy(), cnt
 
     def _genome_struct(self, genome):

@@ -148,3 +148,22 @@ def test_end_extension_pipeline(oracle):
         pairs, counts = b.traceback(np.array([x[0] for x in starts]))
         for j, (_, po, co) in enumerate(starts):
             assert (H.pairs_to_records(pairs[j]) == po).all() and (counts[j, 1:] == co).all(), ("end traceback", j)
+
+
+def test_traceback_nogaps(oracle):
+    # traceback_nogaps (dynprog_end.c:648-722) from min(rl, gl) on both axes
+    specs = [s for s in H.random_specs(21, 200, kinds=(0,))]
+    b = _run(oracle, specs, keep_scores=False)
+    reqs = []
+    for i, s in enumerate(specs):
+        m = min(s.rl, s.gl)
+        rq = H.tb_request(i, s, m, m, genestrand=i % 3)
+        rq["flags"] = 2  # GDP_TB_NOGAPS
+        reqs.append(rq)
+    pairs, counts = b.traceback(np.array(reqs))
+    for i, s in enumerate(specs):
+        m = min(s.rl, s.gl)
+        want, wc = oracle.traceback_nogaps(m, m, s.q, s.q.upper(), s.g, s.galt, 5, 7, s.revp, i % 3)
+        got = H.pairs_to_records(pairs[i])
+        assert counts[i, 0] == len(want) and (got == want).all(), ("nogaps pairs", i)
+        assert (counts[i, 1:4] == wc[:3]).all() and counts[i, 4] == 0 and counts[i, 5] == 0, ("nogaps counters", i)

@@ -124,3 +124,39 @@ def test_cell_counts_brute_force(oracle):
         assert oracle.cells_full(rl, gl, lb, ub) == full
         assert oracle.cells_upper(rl, gl, ub) == ((c >= 1) & (r >= 1) & (r >= c - ub) & (r <= c)).sum()
         assert oracle.cells_lower(rl, gl, lb) == ((c >= 1) & (r >= 1) & (c >= r - lb) & (c <= r)).sum()
+
+
+def test_traceback_nogaps_is_the_diagonal(oracle):
+    """traceback_nogaps (dynprog_end.c:648-722): pairs are exactly the cells
+    (r-k, c-k) while both stay positive, comp '*' on equal upper-case
+    characters, ':' on consistent ones, ' ' otherwise; '*' genome ends it
+    without a pair.  Restated here from the definition, per pair."""
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        rl, gl = int(rng.integers(1, 60)), int(rng.integers(1, 60))
+        q = H.rand_seq(rng, rl, 0.1)
+        q = bytes(ch + 32 if rng.random() < 0.2 else ch for ch in q)
+        g = H.rand_seq(rng, gl, 0.05)
+        ga = bytes(ch if rng.random() < 0.9 else H.ALPH[rng.integers(0, 4)] for ch in g)
+        revp = bool(rng.integers(0, 2))
+        gs = int(rng.integers(0, 3))
+        r, c = min(rl, gl), min(rl, gl)
+        got, cnt = oracle.traceback_nogaps(r, c, q, q.upper(), g, ga, 3, 11, revp, gs)
+        want, score, nm, nmm = [], 0, 0, 0
+        for k in range(min(r, c)):
+            rr, cc = r - k, c - k
+            # revp: the caller passes pointers to the last character and the walk reads [1 - r]
+            c1, c2, c2a = (q[rl - rr], g[gl - cc], ga[gl - cc]) if revp else (q[rr - 1], g[cc - 1], ga[cc - 1])
+            c1uc = ord(chr(c1).upper())
+            comp = ord('*') if c1uc in (c2, c2a) else (
+                ord(':') if oracle.consistent(gs, c1uc, c2) or oracle.consistent(gs, c1uc, c2a) else ord(' '))
+            qc, gc = (-(rr - 1), -(cc - 1)) if revp else (rr - 1, cc - 1)
+            want.append([3 + qc, 11 + gc, c1, comp, c2, c2a, 0, 0])
+            if comp == ord(' '):
+                score, nmm = score - 3, nmm + 1
+            else:
+                score, nm = score + 1, nm + 1
+        want = np.array(want, np.int32).reshape(-1, 8)
+        keep = (want[:, 0] >= 0) & (want[:, 1] >= 0)  # Pairpool_push drops negative coordinates
+        assert (got == want[keep]).all()
+        assert list(cnt[:3]) == [score, nm, nmm]
