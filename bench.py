@@ -7,7 +7,12 @@ launch).  Inputs are resident in HBM before the timed region.  The default
 workload is the 10^6-problem single-gap batch (rl 150, gl 160-200,
 extraband 5-20, int16); --workload c3 selects the uneven 2x10^6 batch.
 
-Prints one JSON line (the fields the driver reads) with two extra objects:
+After the batch, the serving loop runs (unless --no-serve): a 3.1-Gbp random
+genome resident on the device as a genomecomp image, alignment requests of
+the uneven mix, first all at once (saturation throughput), then as Poisson
+arrivals at fractions of that rate, with p50/p99 latency per rate.
+
+Prints one JSON line (the fields the driver reads) with extra objects:
   roofline      -- the fill kernel against the packed-int16 VALU peak, its
                    average duration measured with HIP events on its stream;
   cpu_baseline  -- the reference's own AVX2 DP objects (oracle/_ref, built
@@ -76,6 +81,55 @@ def cpu_baseline(wl, sample_n, threads):
             "sample": "%d problems of the same workload (first %d), fill + traceback, %.1f s" % (sample_n, sample_n, dt)}
 
 
+def serving_bench(args):
+    """Open-loop Poisson load against the request loop over a device genome."""
+    from gmapdp import serve
+    from gmapdp.lib import PROBLEM_DTYPE
+    t0 = time.time()
+    nb = args.serve_genome
+    gc = serve.gen_genomecomp(1005, nb)
+    dg = serve.DeviceGenome(gc, nb)
+    nreq = args.serve_requests
+    reqs, reads = serve.gen_requests(1005, 0, nreq, gc, nb)
+    del gc
+    probs = np.zeros(nreq, PROBLEM_DTYPE)
+    probs["rl"], probs["gl"] = reqs["rl"], reqs["gl"]
+    probs["lband"], probs["uband"] = reqs["extraband"], reqs["gl"] - reqs["rl"] + reqs["extraband"]
+    from gmapdp.batch import cells as cells_of
+    cells = cells_of(probs)
+    log("serving: genome %d bases on the device, %d requests, %.1fs" % (nb, nreq, time.time() - t0))
+    srv = serve.Server(dg, dir_budget_bytes=args.serve_budget << 20, max_batch=args.serve_batch, slots=3,
+                       max_rl=1000, max_gl=1500, pair_capacity=args.serve_batch * 700)
+    try:
+        srv.drive(reqs[:min(nreq, 20000)], 0.0)  # warm-up
+        lat, el, ok = srv.drive(reqs, 0.0)
+        if ok != nreq:
+            raise RuntimeError("serving: %d of %d requests failed" % (nreq - ok, nreq))
+        sat = nreq / el
+        log("serving saturation: %d requests in %.3f s = %.0f req/s, %.1f GCUPS" % (nreq, el, sat, cells.sum() / el / 1e9))
+        points = []
+        for frac in (None, 0.1, 0.25, 0.5, 0.75, 0.9):
+            rate = 1e4 if frac is None else frac * sat
+            m = int(min(nreq, max(2000, rate * args.serve_seconds)))
+            lat, el, ok = srv.drive(reqs[:m], rate, seed=7)
+            pt = {"rate_req_s": round(rate, 1), "fraction_of_saturation": None if frac is None else frac,
+                  "requests": m, "achieved_req_s": round(m / el, 1),
+                  "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                  "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "ok": ok}
+            log("serving", pt)
+            points.append(pt)
+        st = srv.stats()
+    finally:
+        srv.close()
+    half = [p for p in points if p["fraction_of_saturation"] == 0.5][0]
+    return {"genome_bases": nb, "genome_format": "genomecomp (device-resident)",
+            "requests": nreq, "mean_cells_per_request": round(float(cells.mean()), 1),
+            "saturation_req_s": round(sat, 1), "saturation_gcups": round(cells.sum() / (nreq / sat) / 1e9, 2),
+            "p50_ms_at_50pct": half["p50_ms"], "p99_ms_at_50pct": half["p99_ms"], "points": points,
+            "batches": int(st["batches"]), "max_batch": args.serve_batch, "slots": 3,
+            "dir_budget_mib": args.serve_budget, "dir_bytes_peak": int(st["dir_bytes_peak"])}
+
+
 def batch_cells(probs):
     from gmapdp.batch import cells
     return int(cells(probs).sum())
@@ -94,6 +148,12 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="before the timed region, copy MIB MiB once on the device (a known-size kernel that "
                          "calibrates HBM byte counters in a profiled run)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-genome", type=int, default=3_100_000_000, help="bases of the device genome")
+    ap.add_argument("--serve-requests", type=int, default=300_000)
+    ap.add_argument("--serve-batch", type=int, default=16384, help="largest batch the server forms")
+    ap.add_argument("--serve-budget", type=int, default=8192, help="direction budget in MiB")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="target duration of each load point")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (problems are independent; "
@@ -175,6 +235,12 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_fill")
 
+    serving = None
+    if not args.no_serve:
+        del b, reqs_dev
+        torch.cuda.empty_cache()
+        serving = serving_bench(args)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -206,6 +272,7 @@ def main():
                      "kernel": "gdp_fill2_kernel (one fill: every launch of gdp_fill, HIP events on its stream)",
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

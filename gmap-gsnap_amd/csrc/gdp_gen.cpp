@@ -14,9 +14,14 @@
 //      gl = rl + U{0..20}, 3% subs, 1% indels of 1-5, 1% N, ENDQ,
 //      extraband_end 6, int8 when rl or gl < 24; each yields an upper and a
 //      lower fill
+//   5  serving: a random genome packed as genomecomp (with runs of N), and
+//      alignment requests with the length and divergence mix of 3 against
+//      segments of it on either strand, arriving as a Poisson process
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
 #include <thread>
 #include <vector>
 
@@ -255,6 +260,183 @@ int gdp_gen_workload(int config, int variant, uint64_t seed, int64_t first, int6
     });
   for (auto &t : th) t.join();
   return 0;
+}
+
+// Random genome of `nbases` bases written straight into genomecomp words
+// (gdp_genomecomp_words(nbases) of them).  Each 32-base block draws its two
+// code words from its own stream; one block in 2048 is a run of N (flags set,
+// codes 0, as the packer writes N).  The tail of the last block is X and the
+// two trailing words are 0xFFFFFFFF, as in a packed image.
+void gdp_gen_genomecomp(uint64_t seed, uint64_t nbases, uint32_t *out) {
+  const uint64_t nblocks = (nbases + 31) / 32;
+  const int nthreads = 16;
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; t++)
+    th.emplace_back([=]() {
+      for (uint64_t b = nblocks * t / nthreads; b < nblocks * (t + 1) / nthreads; b++) {
+        Rng r(stream_seed(seed ^ 0x67656e6f6d65ull, (int64_t)b));
+        const uint64_t v = r.next(), u = r.next();
+        uint32_t high = (uint32_t)(v >> 32), low = (uint32_t)v, flags = 0;
+        if (u % 2048 == 0) high = low = 0, flags = 0xFFFFFFFFu;
+        for (int k = 0; k < 32; k++) {
+          if (b * 32 + (uint64_t)k < nbases) continue;
+          if (k < 16) low |= 3u << (2 * k);
+          else high |= 3u << (2 * (k - 16));
+          flags |= 1u << k;
+        }
+        out[3 * b] = high;
+        out[3 * b + 1] = low;
+        out[3 * b + 2] = flags;
+      }
+    });
+  for (auto &t : th) t.join();
+  out[3 * nblocks] = 0xFFFFFFFFu;
+  out[3 * nblocks + 1] = 0xFFFFFFFFu;
+}
+
+}  // extern "C"
+
+namespace {
+
+// uncompress_one_char (genome.c:9723-9760) with flag character 'N'.
+char gc_base(const uint32_t *gc, uint64_t nbases, uint64_t pos) {
+  if (pos >= nbases) return 'N';
+  const uint32_t *b = gc + (pos / 32) * 3;
+  const int k = (int)(pos % 32);
+  if ((b[2] >> k) & 1u) return 'N';
+  const uint32_t w = k < 16 ? b[1] : b[0];
+  return kBases[(w >> (2 * (k & 15))) & 3u];
+}
+
+char compl_base(char c) {
+  switch (c) {
+  case 'A': return 'T';
+  case 'C': return 'G';
+  case 'G': return 'C';
+  case 'T': return 'A';
+  default: return c;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Serving requests [first, first+count) over a genomecomp genome of nbases
+// bases taken as one chromosome (chroffset 0, chrhigh nbases - 1).  Lengths,
+// band and divergence follow config 3; the strand is drawn per request and
+// the read is taken from the segment the server will extract for it
+// (the reverse complement on the minus strand).  Read bytes go to
+// reads + (index - first) * 1008; out[k].read points there.
+int gdp_gen_serve_requests(uint64_t seed, int64_t first, int64_t count, const uint32_t *gc, uint64_t nbases,
+                           gdp_align_request *out, char *reads) {
+  if (nbases < 4096) return -1;
+  const int stride = 1008;
+  const int nthreads = 16;
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; t++)
+    th.emplace_back([=]() {
+      std::vector<char> seg(2048), src(2048);
+      for (int64_t k = count * t / nthreads; k < count * (t + 1) / nthreads; k++) {
+        const int64_t idx = first + k;
+        Rng rng(stream_seed(seed, idx));
+        const int rl = (int)std::lround(std::exp(std::log(20.0) + rng.uni() * (std::log(1000.0) - std::log(20.0))));
+        const int gl = (int)std::lround(rl * (1.0 + 0.5 * rng.uni()));
+        const int eb = rng.range(3, 64);
+        const double div = rng.uni() < 0.10 ? 0.30 : 0.02;
+        const bool watson = (rng.next() & 1) != 0;
+        const int64_t goffset = 8 + (int64_t)(rng.uni() * (double)(nbases - (uint64_t)gl - 32));
+        const uint64_t chrhigh = nbases - 1;
+        if (watson) {
+          for (int j = 0; j < gl; j++) seg[j] = gc_base(gc, nbases, (uint64_t)goffset + j);
+        } else {
+          const uint64_t right = chrhigh - (uint64_t)goffset + 1;
+          for (int j = 0; j < gl; j++) seg[j] = compl_base(gc_base(gc, nbases, right - 1 - (uint64_t)j));
+        }
+        const int skip = gl - rl, a = rng.range(0, rl);
+        memcpy(src.data(), seg.data(), a);
+        memcpy(src.data() + a, seg.data() + a + skip, rl - a);
+        char *q = reads + k * stride;
+        int got = mutate(rng, src.data(), rl, q, rl, div, 0.002, 3, 0.0, nullptr);
+        for (int j = got; j < rl; j++) q[j] = kBases[rng.next() & 3];
+        gdp_align_request &r = out[k];
+        memset(&r, 0, sizeof r);
+        r.tag = (uint64_t)idx;
+        r.read = q;
+        r.rl = rl;
+        r.gl = gl;
+        r.roffset = 0;
+        r.goffset = (int32_t)goffset;
+        r.chroffset = 0;
+        r.chrhigh = (uint32_t)chrhigh;
+        r.extraband = eb;
+        r.watsonp = watson ? 1 : 0;
+        r.widebandp = 1;
+        r.jump_late = (int8_t)(rng.next() & 1);
+        r.genestrand = 0;
+        r.mismatchtype = GDP_HIGHQ;
+        r.open = -8;
+        r.extend = -3;
+      }
+    });
+  for (auto &t : th) t.join();
+  return 0;
+}
+
+// Open-loop load: request k arrives at t0 + arrival[k], where the gaps are
+// exponential with mean 1/rate (rate <= 0: all arrive at t0).  A submitter
+// thread hands every request to the server as its arrival time passes; the
+// calling thread polls results and records latency[tag - tag0] = completion
+// time - arrival time.  Returns the number of results with status GDP_OK and
+// stores the time from t0 to the last completion in *elapsed.
+int64_t gdp_serve_drive(gdp_server *s, const gdp_align_request *reqs, int64_t n, double rate, uint64_t seed,
+                        double *latency, double *elapsed) {
+  if (!s || n <= 0 || !reqs || !latency) return -1;
+  std::vector<double> arrival(n);
+  Rng rng(seed ^ 0x61727269ull);
+  double t = 0;
+  for (int64_t k = 0; k < n; k++) {
+    if (rate > 0) t += -std::log(1.0 - rng.uni()) / rate;
+    arrival[k] = t;
+  }
+  const uint64_t tag0 = reqs[0].tag;
+  const double t0 = gdp_server_now() + 0.002;
+  std::thread sub([&]() {
+    int64_t k = 0;
+    while (k < n) {
+      const double now = gdp_server_now() - t0;
+      int64_t e = k;
+      while (e < n && arrival[e] <= now) e++;
+      if (e > k) {
+        for (int64_t j = k; j < e; j += 65536)
+          gdp_server_submit(s, reqs + j, (int32_t)std::min<int64_t>(65536, e - j));
+        k = e;
+      } else {
+        const double wait = arrival[k] - now;
+        if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait * 0.5e6)));
+        else std::this_thread::yield();
+      }
+    }
+  });
+  std::vector<gdp_align_result> res(65536);
+  std::vector<gdp_pair> pairs((size_t)1 << 22);
+  int64_t got = 0, ok = 0;
+  double last = t0;
+  while (got < n) {
+    const int32_t m = gdp_server_poll(s, res.data(), (int32_t)res.size(), pairs.data(), (int64_t)pairs.size(), 1000);
+    if (m < 0) break;
+    for (int32_t i = 0; i < m; i++) {
+      const int64_t k = (int64_t)(res[i].tag - tag0);
+      if (k < 0 || k >= n) continue;
+      latency[k] = res[i].t_done - (t0 + arrival[k]);
+      last = std::max(last, res[i].t_done);
+      if (res[i].status == GDP_OK) ok++;
+    }
+    got += m;
+  }
+  sub.join();
+  if (elapsed) *elapsed = last - t0;
+  return ok;
 }
 
 }  // extern "C"

@@ -53,7 +53,13 @@ struct gdp_dev_problem {
 static_assert(sizeof(gdp_dev_problem) == 96, "gdp_dev_problem must stay 96 bytes");
 static_assert(sizeof(gdp_problem) == 48, "gdp_problem is 48 bytes");
 static_assert(sizeof(gdp_tb_request) == 48, "gdp_tb_request is 48 bytes");
+// Sets the message gdp_last_error() returns on the calling thread.
+void gdp_set_error(const char *msg);
+
 static_assert(sizeof(gdp_pair) == 16, "gdp_pair is 16 bytes");
+static_assert(sizeof(gdp_segment) == 24, "gdp_segment is 24 bytes");
+static_assert(sizeof(gdp_align_request) == 64, "gdp_align_request is 64 bytes");
+static_assert(sizeof(gdp_align_result) == 72, "gdp_align_result is 72 bytes");
 
 // One wave of the paired fill: 64 lanes = 64/G groups of G lanes, each lane
 // carrying two problems in the low and high 16 bits of its registers.  Slot

@@ -229,32 +229,93 @@ __device__ __forceinline__ void push_pair(tb_out &o, int qpos, int gpos, int cdn
   o.n++;
 }
 
+// complCode (COMPLEMENT_LC, complement.h:30) stated as rules: nucleotide and
+// IUPAC letters go to their complements in the same case (U to A), the
+// bracket pairs () <> [] {} swap, control characters, DEL and the characters
+// ! " ? @ and backslash become '?', and every other character stays.
 __device__ __forceinline__ int compl_char(int ch) {
+  if (ch < 32 || ch > 126) return '?';
   switch (ch) {
-  case 'A': return 'T';
-  case 'C': return 'G';
-  case 'G': return 'C';
-  case 'T': return 'A';
-  case 'a': return 't';
-  case 'c': return 'g';
-  case 'g': return 'c';
-  case 't': return 'a';
-  default: return ch;
+  case '!': case '"': case '?': case '@': case '\\': return '?';
+  case '(': return ')';
+  case ')': return '(';
+  case '<': return '>';
+  case '>': return '<';
+  case '[': return ']';
+  case ']': return '[';
+  case '{': return '}';
+  case '}': return '{';
+  default: break;
   }
+  const int lower = ch & 0x20, up = ch & ~0x20;
+  if (up < 'A' || up > 'Z') return ch;
+  int r;
+  switch (up) {
+  case 'A': r = 'T'; break;
+  case 'T': case 'U': r = 'A'; break;
+  case 'C': r = 'G'; break;
+  case 'G': r = 'C'; break;
+  case 'B': r = 'V'; break;
+  case 'V': r = 'B'; break;
+  case 'D': r = 'H'; break;
+  case 'H': r = 'D'; break;
+  case 'K': r = 'M'; break;
+  case 'M': r = 'K'; break;
+  case 'R': r = 'Y'; break;
+  case 'Y': r = 'R'; break;
+  default: r = up; break;
+  }
+  return r | lower;
 }
 
-// get_genomic_nt (pairpool.c:1016-1063) over an ASCII device genome.
-__device__ int genomic_nt(int *alt, const char *genome, uint32_t glen, int genomicpos, uint32_t chroffset,
-                          uint32_t chrhigh, int watsonp) {
-  uint32_t pos = watsonp ? chroffset + (uint32_t)genomicpos : chrhigh - (uint32_t)genomicpos;
-  if (pos < chroffset || pos >= chrhigh) {
-    *alt = '*';
-    return '*';
+// One base of a genome in device memory, as Genome_get_char reads it
+// (genome.c:11003-11019): ASCII bytes indexed by position, or GMAP's
+// genomecomp blocks decoded as uncompress_one_char with flag character 'N'
+// (genome.c:9723-9760): per 32 bases three words {high, low, flags}, bases
+// 0..15 two bits each in low, 16..31 in high, A C G T = 0 1 2 3, a set flag
+// bit reads 'N'.  Positions at or past `glen` read 'N'.
+__device__ __forceinline__ int genome_at(const void *genome, int fmt, uint32_t glen, uint32_t pos) {
+  if (genome == nullptr || pos >= glen) return 'N';
+  if (fmt == GDP_GENOME_ASCII) return ((const char *)genome)[pos];
+  const uint32_t *b = (const uint32_t *)genome + (size_t)(pos >> 5) * 3;
+  const int bit = (int)(pos & 31);
+  if ((b[2] >> bit) & 1u) return 'N';
+  const uint32_t w = bit < 16 ? b[1] : b[0];
+  return (int)((0x54474341u >> (8 * ((w >> (2 * (bit & 15))) & 3u))) & 0xffu);
+}
+
+// Genome_get_segment_right / Genome_get_segment_left (genome.c:11022-11120)
+// with the alternate genome equal to the genome, one wave per segment.
+// Right: bases [coord, coord + length) with those at or past chrhigh = bound
+// as '*'; left: bases [coord - length, coord) with those below chroffset =
+// bound as '*'; revcomp reverses and complements the whole segment
+// (make_complement_inplace, genome.c:331-345).
+__global__ __launch_bounds__(256) void gdp_segments_kernel(const void *__restrict__ genome, int fmt, uint32_t glen,
+                                                           const gdp_segment *__restrict__ segs, int n,
+                                                           char *__restrict__ out) {
+  const int s = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  if (s >= n) return;
+  const gdp_segment sg = segs[s];
+  if (sg.length <= 0) return;
+  char *o = out + sg.out_off;
+  const int64_t len = sg.length, coord = sg.coord, bound = sg.bound;
+  // source index j is out of bounds below `lo` or at/after `hi`
+  int64_t lo = 0, hi = len, start;
+  if (!sg.left) {
+    start = coord;
+    if (coord >= bound) hi = 0;
+    else if (coord + len >= bound) hi = len - (coord + len - bound);
+  } else {
+    start = coord - len;
+    if (coord < bound) lo = len;
+    else if (coord < bound + len) lo = bound + len - coord;
   }
-  int ch = (genome != nullptr && pos < glen) ? genome[pos] : 'N';
-  if (!watsonp) ch = compl_char(ch);
-  *alt = ch;
-  return ch;
+  for (int64_t i = lane; i < len; i += 64) {
+    const int64_t j = sg.revcomp ? len - 1 - i : i;
+    int ch = (j < lo || j >= hi) ? '*' : genome_at(genome, fmt, glen, (uint32_t)(start + j));
+    if (sg.revcomp) ch = compl_char(ch);
+    o[i] = (char)ch;
+  }
 }
 
 // 16-byte window of device memory kept in registers: a walk reads its
@@ -372,7 +433,8 @@ enum { TB_DECIDE, TB_RUN_H, TB_RUN_V, TB_EMIT_H, TB_EMIT_V, TB_TAIL, TB_DONE };
 __global__ __launch_bounds__(256) void gdp_traceback_kernel(
     const gdp_dev_problem *__restrict__ probs, const gdp_tb_request *__restrict__ reqs, int nreq,
     const char *__restrict__ qbuf, const char *__restrict__ gbuf, const uint32_t *__restrict__ dirs,
-    const char *__restrict__ genome, uint32_t glen, gdp_pair *__restrict__ pairs, int32_t *__restrict__ counts) {
+    const void *__restrict__ genome, int gfmt, uint32_t glen, gdp_pair *__restrict__ pairs,
+    int32_t *__restrict__ counts) {
   __shared__ uint32_t consb[3 * 128 * 4];
   for (int j = threadIdx.x; j < 3 * 128 * 4; j += blockDim.x) consb[j] = gdp_tabConsBits[j];
   __syncthreads();
@@ -485,7 +547,7 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
       int ch;
       if (pos < chroffset || pos >= chrhigh) ch = '*';
       else {
-        ch = (genome != nullptr && pos < glen) ? genome[pos] : 'N';
+        ch = genome_at(genome, gfmt, glen, pos);
         if (!watsonp) ch = compl_char(ch);
       }
       TB_PUSH(queryoffset + eq, gpos, ' ', '-', ch, ch);
@@ -714,12 +776,20 @@ extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, con
 }
 
 extern "C" int gdp_launch_traceback(const void *d_descs, const gdp_tb_request *d_reqs, int nreq, const char *d_qbuf,
-                                    const char *d_gbuf, const uint32_t *d_dirs, const char *d_genome, uint32_t glen,
-                                    gdp_pair *d_pairs, int32_t *d_counts, hipStream_t stream) {
+                                    const char *d_gbuf, const uint32_t *d_dirs, const void *d_genome, int gfmt,
+                                    uint32_t glen, gdp_pair *d_pairs, int32_t *d_counts, hipStream_t stream) {
   if (nreq <= 0) return 0;
   hipLaunchKernelGGL(gdp_traceback_kernel, dim3((nreq + 255) / 256), dim3(256), 0, stream,
-                     (const gdp_dev_problem *)d_descs, d_reqs, nreq, d_qbuf, d_gbuf, d_dirs, d_genome, glen, d_pairs,
-                     d_counts);
+                     (const gdp_dev_problem *)d_descs, d_reqs, nreq, d_qbuf, d_gbuf, d_dirs, d_genome, gfmt, glen,
+                     d_pairs, d_counts);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int gdp_launch_segments(const void *d_genome, int fmt, uint32_t glen, const gdp_segment *d_segs, int n,
+                                   char *d_out, hipStream_t stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(gdp_segments_kernel, dim3((n + 3) / 4), dim3(256), 0, stream, d_genome, fmt, glen, d_segs, n,
+                     d_out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -1,0 +1,606 @@
+// gdp_serve.cpp -- continuous batching of alignment requests over a device
+// genome (the gdp_server_* functions of include/gmapdp.h).
+//
+// One server thread owns every HIP call.  It keeps up to `slots` batches in
+// flight, each on a stream of its own with its own pinned staging buffers
+// and device buffers; the direction words of all batches in flight share one
+// device budget handed out first-fit.  Whenever a slot is free and requests
+// wait, the thread takes the oldest requests that fit the slot and the free
+// budget, plans them, and queues on the slot's stream: one upload, the
+// genome-segment extraction, the fills, the tracebacks, one download and an
+// event.  Finished batches are turned into results as their events fire, so
+// new requests enter while earlier batches still run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "gdp_internal.h"
+
+namespace {
+
+inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+struct Req {
+  gdp_align_request r;  // r.read is not used after submit
+  int16_t lband, uband;
+  int8_t width;
+  int8_t status;        // 0, or the negated GDP_E* code the request is refused with
+  std::string read;
+  double t_submit;
+};
+
+struct DoneBatch {
+  std::vector<gdp_align_result> res;
+  std::vector<gdp_pair> pairs;  // res[i].pair_index indexes this vector
+  size_t next = 0;
+};
+
+// First-fit allocator over [0, size) in 256-byte units.
+class Arena {
+ public:
+  void init(int64_t size) {
+    free_.clear();
+    if (size > 0) free_[0] = size;
+  }
+  int64_t largest() const {
+    int64_t m = 0;
+    for (const auto &kv : free_) m = std::max(m, kv.second);
+    return m;
+  }
+  int64_t alloc(int64_t n) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < n) continue;
+      const int64_t off = it->first, len = it->second;
+      free_.erase(it);
+      if (len > n) free_[off + n] = len - n;
+      return off;
+    }
+    return -1;
+  }
+  void release(int64_t off, int64_t n) {
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+
+ private:
+  std::map<int64_t, int64_t> free_;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  uint8_t *h_up = nullptr, *d_up = nullptr, *h_down = nullptr, *d_down = nullptr;
+  char *d_g = nullptr;
+  bool busy = false;
+  std::vector<Req> reqs;
+  std::vector<int64_t> pair_off;
+  int64_t dir_off = 0, dir_bytes = 0;
+  int64_t off_counts = 0, off_pairs = 0, down_bytes = 0;
+};
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct gdp_server {
+  gdp_server_config cfg{};
+  const void *genome = nullptr;
+  int fmt = 0;
+  uint32_t glen = 0;
+  int device = 0;
+  uint8_t *d_dirs = nullptr;
+  Arena arena;
+  std::vector<Slot> slots;
+  int64_t up_cap = 0, g_cap = 0, down_cap = 0, q_cap = 0;
+  std::vector<gdp_problem> probs;
+
+  std::mutex mu;
+  std::condition_variable cv_work;
+  std::deque<Req> pending;
+
+  std::mutex dmu;
+  std::condition_variable cv_done;
+  std::deque<DoneBatch> done;
+
+  std::mutex smu;
+  gdp_server_stats st{};
+  int64_t dir_in_flight = 0;
+
+  std::atomic<bool> stop{false};
+  std::string fatal;
+  std::thread th;
+
+  void run();
+  bool launch(Slot &s);
+  void retire(Slot &s);
+  void fail_all(const char *what, hipError_t e);
+  void post(DoneBatch &&b) {
+    {
+      std::lock_guard<std::mutex> lk(dmu);
+      done.push_back(std::move(b));
+    }
+    cv_done.notify_all();
+  }
+};
+
+static void set_err(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+static void set_err(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  gdp_set_error(buf);
+}
+
+// Result of a request refused before it reaches the device: the values
+// Dynprog_single_gap returns for lengths it will not align
+// (dynprog_single.c:508-520).
+static gdp_align_result refused(const Req &q, double t) {
+  gdp_align_result r;
+  memset(&r, 0, sizeof r);
+  r.tag = q.r.tag;
+  r.status = -(int)q.status;
+  r.counts[1] = -32768;  // NEG_INFINITY_32
+  r.t_submit = q.t_submit;
+  r.t_done = t;
+  return r;
+}
+
+void gdp_server::fail_all(const char *what, hipError_t e) {
+  fatal = std::string(what) + ": " + hipGetErrorString(e);
+  DoneBatch b;
+  const double t = now_s();
+  for (Slot &s : slots) {
+    if (!s.busy) continue;
+    for (Req &q : s.reqs) {
+      q.status = (int8_t)-GDP_EHIP;
+      b.res.push_back(refused(q, t));
+    }
+    s.busy = false;
+  }
+  std::deque<Req> rest;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    rest.swap(pending);
+  }
+  for (Req &q : rest) {
+    q.status = (int8_t)-GDP_EHIP;
+    b.res.push_back(refused(q, t));
+  }
+  post(std::move(b));
+}
+
+// Takes the oldest pending requests that fit the slot and the free direction
+// budget, and queues their work on the slot's stream.  Returns false when
+// nothing could be started.
+bool gdp_server::launch(Slot &s) {
+  std::vector<Req> batch;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    int64_t qb = 0, gb = 0, pb = 0;
+    while (!pending.empty() && (int32_t)batch.size() < cfg.max_batch) {
+      const Req &q = pending.front();
+      const int64_t pairs = (int64_t)q.r.rl + q.r.gl + 2;
+      if (qb + q.r.rl + 16 > q_cap || gb + q.r.gl + 16 > g_cap || pb + pairs > cfg.pair_capacity) break;
+      qb += q.r.rl;
+      gb += q.r.gl;
+      pb += pairs;
+      batch.push_back(std::move(pending.front()));
+      pending.pop_front();
+    }
+  }
+  if (batch.empty()) return false;
+
+  // Plan; if the direction words do not fit the free budget, give back the
+  // younger half and plan again.
+  const int64_t free_bytes = arena.largest();
+  int n = (int)batch.size();
+  int64_t dw = 0, ss = 0;
+  for (;;) {
+    probs.resize(n);
+    int64_t qpos = 0, gpos = 0;
+    for (int i = 0; i < n; i++) {
+      const Req &q = batch[i];
+      gdp_problem &p = probs[i];
+      memset(&p, 0, sizeof p);
+      p.qoff = qpos;
+      p.goff = p.galtoff = gpos;
+      p.rl = q.r.rl;
+      p.gl = q.r.gl;
+      p.lband = q.lband;
+      p.uband = q.uband;
+      p.kind = GDP_KIND_FULL;
+      p.width = q.width;
+      p.mismatchtype = q.r.mismatchtype;
+      p.jump_late = q.r.jump_late ? 1 : 0;
+      p.open = q.r.open;
+      p.extend = q.r.extend;
+      qpos += q.r.rl;
+      gpos += q.r.gl;
+    }
+    if (gdp_plan_ex(probs.data(), n, 0, s.h_up, &dw, &ss) != GDP_OK) {
+      // Requests were validated at submit; a planning failure is a bug.
+      fatal = std::string("gdp_server: planning failed: ") + gdp_last_error();
+      DoneBatch b;
+      const double t = now_s();
+      for (Req &q : batch) {
+        q.status = (int8_t)-GDP_EINVAL;
+        b.res.push_back(refused(q, t));
+      }
+      post(std::move(b));
+      return true;
+    }
+    const int64_t need = align_up(dw * 4, 256);
+    if (need <= free_bytes) break;
+    if (n == 1) {
+      bool idle = true;
+      for (const Slot &o : slots) idle = idle && !o.busy;
+      if (!idle) {  // wait for budget
+        std::lock_guard<std::mutex> lk(mu);
+        pending.push_front(std::move(batch[0]));
+        return false;
+      }
+      DoneBatch b;  // one request larger than the whole budget
+      batch[0].status = (int8_t)-GDP_ENOMEM;
+      b.res.push_back(refused(batch[0], now_s()));
+      post(std::move(b));
+      return true;
+    }
+    const int keep = n / 2;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (int i = n - 1; i >= keep; i--) pending.push_front(std::move(batch[i]));
+    }
+    batch.resize(keep);
+    n = keep;
+  }
+
+  // Staging layout: descriptors | segments | traceback requests | reads.
+  const int64_t plan_bytes = gdp_plan_bytes(n);
+  const int64_t off_segs = align_up(plan_bytes, 16);
+  const int64_t off_tb = off_segs + align_up((int64_t)n * (int64_t)sizeof(gdp_segment), 16);
+  const int64_t off_q = off_tb + (int64_t)n * (int64_t)sizeof(gdp_tb_request);
+  gdp_segment *segs = (gdp_segment *)(s.h_up + off_segs);
+  gdp_tb_request *tbs = (gdp_tb_request *)(s.h_up + off_tb);
+  char *qh = (char *)(s.h_up + off_q);
+  s.pair_off.resize(n);
+  int64_t qpos = 0, gpos = 0, ppos = 0;
+  for (int i = 0; i < n; i++) {
+    const Req &q = batch[i];
+    gdp_segment &sg = segs[i];
+    memset(&sg, 0, sizeof sg);
+    if (q.r.watsonp) {
+      sg.coord = q.r.chroffset + (uint32_t)q.r.goffset;
+      sg.bound = q.r.chrhigh;
+    } else {
+      sg.coord = q.r.chrhigh - (uint32_t)q.r.goffset + 1u;
+      sg.bound = q.r.chroffset;
+      sg.left = 1;
+      sg.revcomp = 1;
+    }
+    sg.length = q.r.gl;
+    sg.out_off = gpos;
+    memcpy(qh + qpos, q.read.data(), (size_t)q.r.rl);
+    gdp_tb_request &t = tbs[i];
+    memset(&t, 0, sizeof t);
+    t.fill = i;
+    t.r = q.r.rl;
+    t.c = q.r.gl;
+    t.queryoffset = q.r.roffset;
+    t.genomeoffset = q.r.goffset;
+    t.chroffset = q.r.chroffset;
+    t.chrhigh = q.r.chrhigh;
+    t.watsonp = q.r.watsonp ? 1 : 0;
+    t.genestrand = q.r.genestrand;
+    t.pair_off = ppos;
+    t.max_pairs = q.r.rl + q.r.gl + 2;
+    s.pair_off[i] = ppos;
+    qpos += q.r.rl;
+    gpos += q.r.gl;
+    ppos += t.max_pairs;
+  }
+  memset(qh + qpos, 0, 16);
+  const int64_t up_bytes = off_q + qpos + 16;
+  const uint32_t kernels = gdp_plan_kernels(s.h_up, n);
+
+  s.dir_bytes = align_up(dw * 4, 256);
+  s.dir_off = arena.alloc(s.dir_bytes);
+  s.off_counts = align_up((int64_t)n * 4, 16);
+  s.off_pairs = align_up(s.off_counts + (int64_t)n * 24, 16);
+  s.down_bytes = s.off_pairs + ppos * (int64_t)sizeof(gdp_pair);
+
+  s.reqs = std::move(batch);
+  s.busy = true;
+  uint8_t *d_descs = s.d_up;
+  uint32_t *dirs = (uint32_t *)(d_dirs + s.dir_off);
+  hipError_t e = hipMemcpyAsync(s.d_up, s.h_up, (size_t)up_bytes, hipMemcpyHostToDevice, s.stream);
+  int rc = e == hipSuccess ? GDP_OK : GDP_EHIP;
+  if (rc == GDP_OK)
+    rc = gdp_genome_segments(genome, fmt, glen, (const gdp_segment *)(s.d_up + off_segs), n, s.d_g, s.stream);
+  if (rc == GDP_OK)
+    rc = gdp_fill_ex(d_descs, n, kernels, (const char *)(s.d_up + off_q), s.d_g, dirs, nullptr,
+                     (int32_t *)s.d_down, s.stream);
+  if (rc == GDP_OK)
+    rc = gdp_traceback_ex(d_descs, (const gdp_tb_request *)(s.d_up + off_tb), n, (const char *)(s.d_up + off_q),
+                          s.d_g, dirs, genome, fmt, glen, (gdp_pair *)(s.d_down + s.off_pairs),
+                          (int32_t *)(s.d_down + s.off_counts), s.stream);
+  if (rc == GDP_OK) {
+    e = hipMemcpyAsync(s.h_down, s.d_down, (size_t)s.down_bytes, hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipEventRecord(s.ev, s.stream);
+    rc = e == hipSuccess ? GDP_OK : GDP_EHIP;
+  }
+  if (rc != GDP_OK) {
+    fail_all("gdp_server launch", e != hipSuccess ? e : hipGetLastError());
+    return true;
+  }
+  {
+    std::lock_guard<std::mutex> lk(smu);
+    st.batches++;
+    dir_in_flight += s.dir_bytes;
+    st.dir_bytes_peak = std::max(st.dir_bytes_peak, dir_in_flight);
+    int inflight = 0;
+    for (const Slot &o : slots) inflight += o.busy ? 1 : 0;
+    st.inflight_peak = std::max(st.inflight_peak, inflight);
+  }
+  return true;
+}
+
+void gdp_server::retire(Slot &s) {
+  const double t = now_s();
+  const int n = (int)s.reqs.size();
+  const int32_t *fin = (const int32_t *)s.h_down;
+  const int32_t *cnt = (const int32_t *)(s.h_down + s.off_counts);
+  const gdp_pair *pr = (const gdp_pair *)(s.h_down + s.off_pairs);
+  DoneBatch b;
+  b.res.resize(n);
+  int64_t total = 0;
+  for (int i = 0; i < n; i++) total += std::max(0, cnt[6 * i]);
+  b.pairs.resize((size_t)total);
+  int64_t at = 0;
+  for (int i = 0; i < n; i++) {
+    const Req &q = s.reqs[i];
+    gdp_align_result &r = b.res[i];
+    memset(&r, 0, sizeof r);
+    r.tag = q.r.tag;
+    r.status = GDP_OK;
+    r.finalscore = fin[i];
+    for (int k = 0; k < 6; k++) r.counts[k] = cnt[6 * i + k];
+    r.width = q.width;
+    r.pair_index = at;
+    r.t_submit = q.t_submit;
+    r.t_done = t;
+    const int np = std::max(0, cnt[6 * i]);
+    if (np) memcpy(b.pairs.data() + at, pr + s.pair_off[i], (size_t)np * sizeof(gdp_pair));
+    at += np;
+  }
+  arena.release(s.dir_off, s.dir_bytes);
+  {
+    std::lock_guard<std::mutex> lk(smu);
+    st.completed += n;
+    dir_in_flight -= s.dir_bytes;
+  }
+  s.reqs.clear();
+  s.busy = false;
+  post(std::move(b));
+}
+
+void gdp_server::run() {
+  (void)hipSetDevice(device);
+  for (;;) {
+    bool progressed = false;
+    for (Slot &s : slots) {
+      if (!s.busy) continue;
+      const hipError_t e = hipEventQuery(s.ev);
+      if (e == hipSuccess) {
+        retire(s);
+        progressed = true;
+      } else if (e != hipErrorNotReady) {
+        fail_all("gdp_server batch", e);
+        progressed = true;
+      }
+    }
+    bool any_busy = false;
+    for (Slot &s : slots) {
+      if (!s.busy && fatal.empty() && !stop.load()) {
+        if (launch(s)) progressed = true;
+      }
+      any_busy = any_busy || s.busy;
+    }
+    if (stop.load() && !any_busy) break;
+    if (progressed) continue;
+    if (any_busy) {
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    } else {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_work.wait_for(lk, std::chrono::milliseconds(5), [&] { return !pending.empty() || stop.load(); });
+    }
+  }
+}
+
+extern "C" double gdp_server_now(void) { return now_s(); }
+
+extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const void *d_genome, int32_t genome_format,
+                                         uint32_t genome_len) {
+  if (!cfg || cfg->max_batch <= 0 || cfg->slots < 1 || cfg->slots > 4 || cfg->max_rl <= 0 ||
+      cfg->max_rl > GDP_MAX_LEN || cfg->max_gl <= 0 || cfg->max_gl > GDP_MAX_LEN || cfg->dir_budget_bytes <= 0 ||
+      cfg->pair_capacity < (int64_t)cfg->max_rl + cfg->max_gl + 2) {
+    set_err("gdp_server_create: bad configuration");
+    return nullptr;
+  }
+  if (genome_format != GDP_GENOME_ASCII && genome_format != GDP_GENOME_COMP) {
+    set_err("gdp_server_create: unknown genome format %d", genome_format);
+    return nullptr;
+  }
+  gdp_server *s = new gdp_server;
+  s->cfg = *cfg;
+  s->genome = d_genome;
+  s->fmt = genome_format;
+  s->glen = genome_len;
+  (void)hipGetDevice(&s->device);
+  const int64_t nb = cfg->max_batch;
+  s->q_cap = nb * (int64_t)cfg->max_rl + 16;
+  s->g_cap = nb * (int64_t)cfg->max_gl + 16;
+  s->up_cap = align_up(gdp_plan_bytes(cfg->max_batch), 16) + align_up(nb * (int64_t)sizeof(gdp_segment), 16) +
+              nb * (int64_t)sizeof(gdp_tb_request) + s->q_cap;
+  s->down_cap = align_up(nb * 4, 16) + align_up(nb * 24, 16) + cfg->pair_capacity * (int64_t)sizeof(gdp_pair);
+  s->arena.init(align_up(cfg->dir_budget_bytes, 256) - 256);
+  hipError_t e = hipMalloc((void **)&s->d_dirs, (size_t)align_up(cfg->dir_budget_bytes, 256) + 64);
+  s->slots.resize(cfg->slots);
+  for (Slot &sl : s->slots) {
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_up, (size_t)s->up_cap, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_down, (size_t)s->down_cap, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void **)&sl.d_up, (size_t)s->up_cap);
+    if (e == hipSuccess) e = hipMalloc((void **)&sl.d_down, (size_t)s->down_cap);
+    if (e == hipSuccess) e = hipMalloc((void **)&sl.d_g, (size_t)s->g_cap);
+    if (e == hipSuccess) e = hipMemset(sl.d_g, 0, (size_t)s->g_cap);
+  }
+  if (e != hipSuccess) {
+    set_err("gdp_server_create: %s", hipGetErrorString(e));
+    s->stop = true;
+    gdp_server_destroy(s);
+    return nullptr;
+  }
+  s->th = std::thread([s] { s->run(); });
+  return s;
+}
+
+extern "C" int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, int32_t n) {
+  if (!s || n < 0 || (n > 0 && !reqs)) {
+    set_err("gdp_server_submit: bad argument");
+    return GDP_EINVAL;
+  }
+  const double t = now_s();
+  std::vector<Req> in(n);
+  std::vector<Req> refused_now;
+  for (int32_t i = 0; i < n; i++) {
+    const gdp_align_request &r = reqs[i];
+    Req &q = in[i];
+    q.r = r;
+    q.r.read = nullptr;
+    q.t_submit = t;
+    q.status = 0;
+    int32_t lb = 0, ub = 0;
+    if (r.rl <= 0 || r.gl <= 0 || r.rl > s->cfg.max_rl || r.gl > s->cfg.max_gl) {
+      q.status = (int8_t)-GDP_ETOOBIG;
+    } else {
+      gdp_compute_bands(&lb, &ub, r.rl, r.gl, r.extraband, r.widebandp);
+      if (lb < 0 || ub < 0 || lb + ub > GDP_MAX_SPAN || r.mismatchtype < 0 || r.mismatchtype > 3 || r.open > 0 ||
+          r.extend > 0 || r.read == nullptr)
+        q.status = (int8_t)-GDP_EINVAL;
+    }
+    if (q.status) continue;
+    q.lband = (int16_t)lb;
+    q.uband = (int16_t)ub;
+    // Dynprog_single_gap's kernel choice (dynprog_single.c:595).
+    const int u8 = gdp_use8p_size(r.mismatchtype);
+    q.width = (r.rl < u8 && r.gl < u8) ? 8 : 16;
+    q.read.assign(r.read, (size_t)r.rl);
+  }
+  DoneBatch b;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (Req &q : in) {
+      if (q.status) b.res.push_back(refused(q, t));
+      else s->pending.push_back(std::move(q));
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->smu);
+    s->st.submitted += n;
+    s->st.completed += (int64_t)b.res.size();
+  }
+  if (!b.res.empty()) s->post(std::move(b));
+  s->cv_work.notify_one();
+  return GDP_OK;
+}
+
+extern "C" int32_t gdp_server_poll(gdp_server *s, gdp_align_result *out, int32_t max, gdp_pair *pairs,
+                                   int64_t pair_cap, int32_t wait_us) {
+  if (!s || max < 0 || (max > 0 && !out)) return GDP_EINVAL;
+  std::unique_lock<std::mutex> lk(s->dmu);
+  if (s->done.empty() && wait_us > 0)
+    s->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [&] { return !s->done.empty(); });
+  int32_t k = 0;
+  int64_t used = 0;
+  while (k < max && !s->done.empty()) {
+    DoneBatch &b = s->done.front();
+    while (k < max && b.next < b.res.size()) {
+      gdp_align_result r = b.res[b.next];
+      const int64_t np = std::max(0, r.counts[0]);
+      if (pairs) {
+        if (used + np > pair_cap) {
+          if (k == 0) {
+            set_err("gdp_server_poll: pair buffer of %lld records too small for a result of %lld",
+                    (long long)pair_cap, (long long)np);
+            return GDP_ENOMEM;
+          }
+          return k;
+        }
+        if (np) memcpy(pairs + used, b.pairs.data() + r.pair_index, (size_t)np * sizeof(gdp_pair));
+        r.pair_index = used;
+        used += np;
+      } else {
+        r.pair_index = -1;
+      }
+      out[k++] = r;
+      b.next++;
+    }
+    if (b.next >= b.res.size()) s->done.pop_front();
+  }
+  return k;
+}
+
+extern "C" int gdp_server_get_stats(gdp_server *s, gdp_server_stats *st) {
+  if (!s || !st) return GDP_EINVAL;
+  std::lock_guard<std::mutex> lk(s->smu);
+  *st = s->st;
+  return s->fatal.empty() ? GDP_OK : GDP_EHIP;
+}
+
+extern "C" void gdp_server_destroy(gdp_server *s) {
+  if (!s) return;
+  s->stop = true;
+  s->cv_work.notify_all();
+  if (s->th.joinable()) s->th.join();
+  for (Slot &sl : s->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.h_up) (void)hipHostFree(sl.h_up);
+    if (sl.h_down) (void)hipHostFree(sl.h_down);
+    if (sl.d_up) (void)hipFree(sl.d_up);
+    if (sl.d_down) (void)hipFree(sl.d_down);
+    if (sl.d_g) (void)hipFree(sl.d_g);
+    if (sl.ev) (void)hipEventDestroy(sl.ev);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  if (s->d_dirs) (void)hipFree(s->d_dirs);
+  delete s;
+}

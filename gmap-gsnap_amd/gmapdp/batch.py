@@ -107,8 +107,10 @@ class DeviceBatch:
                "gdp_export_dense")
         return (dd.cpu().numpy().reshape(C + 1, R + 1), ds.cpu().numpy().reshape(C + 1, R + 1))
 
-    def traceback(self, reqs, genome=None):
-        """Runs the requests; returns (pairs per request, counts[n,6])."""
+    def traceback(self, reqs, genome=None, genome_format=0, genome_len=None):
+        """Runs the requests; returns (pairs per request, counts[n,6]).
+        genome: ASCII bases (genome_format 0) or a genomecomp image of
+        genome_len bases (genome_format 1)."""
         torch = self.torch
         reqs = np.ascontiguousarray(reqs, TBREQ_DTYPE).copy()
         off = 0
@@ -120,11 +122,17 @@ class DeviceBatch:
         d_counts = torch.zeros(max(len(reqs), 1) * 6, dtype=torch.int32, device=self.dev)
         gptr, glen = None, 0
         if genome is not None:
-            d_gen = torch.from_numpy(np.ascontiguousarray(genome, np.uint8)).to(self.dev)
-            gptr, glen = d_gen.data_ptr(), len(genome)
-        _check(self.L, self.L.gdp_traceback(self.d_descs.data_ptr(), d_reqs.data_ptr(), len(reqs), self.d_q.data_ptr(),
-                                            self.d_g.data_ptr(), self.d_dirs.data_ptr(), gptr, glen,
-                                            d_pairs.data_ptr(), d_counts.data_ptr(), self.stream), "gdp_traceback")
+            if genome_format == 0:
+                d_gen = torch.from_numpy(np.ascontiguousarray(genome, np.uint8)).to(self.dev)
+                glen = len(genome)
+            else:
+                d_gen = torch.from_numpy(np.ascontiguousarray(genome, np.uint32).view(np.uint8)).to(self.dev)
+                glen = genome_len
+            gptr = d_gen.data_ptr()
+        _check(self.L, self.L.gdp_traceback_ex(self.d_descs.data_ptr(), d_reqs.data_ptr(), len(reqs),
+                                               self.d_q.data_ptr(), self.d_g.data_ptr(), self.d_dirs.data_ptr(), gptr,
+                                               genome_format, glen, d_pairs.data_ptr(), d_counts.data_ptr(),
+                                               self.stream), "gdp_traceback")
         counts = d_counts.cpu().numpy()[: 6 * len(reqs)].reshape(-1, 6)
         pairs = d_pairs.cpu().numpy().view(PAIR_DTYPE)
         out = []

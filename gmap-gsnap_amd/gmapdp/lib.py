@@ -47,7 +47,23 @@ TBREQ_DTYPE = np.dtype([("fill", "<i4"), ("r", "<i4"), ("c", "<i4"), ("queryoffs
                         ("max_pairs", "<i4"), ("pad2", "<i4")])
 PAIR_DTYPE = np.dtype([("querypos", "<i4"), ("genomepos", "<i4"), ("cdna", "i1"), ("comp", "i1"),
                        ("genome", "i1"), ("genomealt", "i1"), ("gapp", "<i2"), ("genomejump", "<i2")])
+SEGMENT_DTYPE = np.dtype([("coord", "<u4"), ("bound", "<u4"), ("length", "<i4"), ("left", "i1"), ("revcomp", "i1"),
+                          ("pad", "<i2"), ("out_off", "<i8")])
+ALIGN_REQ_DTYPE = np.dtype([("tag", "<u8"), ("read", "<u8"), ("rl", "<i4"), ("gl", "<i4"), ("roffset", "<i4"),
+                            ("goffset", "<i4"), ("chroffset", "<u4"), ("chrhigh", "<u4"), ("extraband", "<i4"),
+                            ("watsonp", "i1"), ("widebandp", "i1"), ("jump_late", "i1"), ("genestrand", "i1"),
+                            ("mismatchtype", "i1"), ("open", "i1"), ("extend", "i1"), ("pad", "i1"),
+                            ("pad2", "<i4", (3,))])
+ALIGN_RES_DTYPE = np.dtype([("tag", "<u8"), ("status", "<i4"), ("finalscore", "<i4"), ("counts", "<i4", (6,)),
+                            ("width", "<i4"), ("pad", "<i4"), ("pair_index", "<i8"), ("t_submit", "<f8"),
+                            ("t_done", "<f8")])
+SERVER_CFG_DTYPE = np.dtype([("dir_budget_bytes", "<i8"), ("pair_capacity", "<i8"), ("max_batch", "<i4"),
+                             ("slots", "<i4"), ("max_rl", "<i4"), ("max_gl", "<i4")])
+SERVER_STATS_DTYPE = np.dtype([("submitted", "<i8"), ("completed", "<i8"), ("batches", "<i8"), ("cells", "<i8"),
+                               ("dir_bytes_peak", "<i8"), ("inflight_peak", "<i4"), ("pad", "<i4")])
 assert PROBLEM_DTYPE.itemsize == 48 and TBREQ_DTYPE.itemsize == 48 and PAIR_DTYPE.itemsize == 16
+assert SEGMENT_DTYPE.itemsize == 24 and ALIGN_REQ_DTYPE.itemsize == 64 and ALIGN_RES_DTYPE.itemsize == 72
+GENOME_ASCII, GENOME_COMP = 0, 1
 
 _lib = None
 
@@ -87,6 +103,16 @@ def load(path=None):
         "gdp_fill": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, vp, vp]),
         "gdp_best_endpoint": (ctypes.c_int, [vp, vp, i32, i32, vp, vp, vp]),
         "gdp_traceback": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp]),
+        "gdp_traceback_ex": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp]),
+        "gdp_genome_segments": (ctypes.c_int, [vp, i32, ctypes.c_uint32, vp, i32, vp, vp]),
+        "gdp_genomecomp_words": (i64, [ctypes.c_uint64]),
+        "gdp_genomecomp_pack": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
+        "gdp_server_create": (vp, [vp, vp, i32, ctypes.c_uint32]),
+        "gdp_server_submit": (ctypes.c_int, [vp, vp, i32]),
+        "gdp_server_poll": (i32, [vp, vp, i32, vp, i64, i32]),
+        "gdp_server_get_stats": (ctypes.c_int, [vp, vp]),
+        "gdp_server_destroy": (None, [vp]),
+        "gdp_server_now": (ctypes.c_double, []),
         "gdp_export_dense": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, vp]),
         "gdp_pairdistance": (ctypes.POINTER(ctypes.c_int16), [ctypes.c_int]),
         "gdp_nt_to_int": (ctypes.POINTER(ctypes.c_int32), []),

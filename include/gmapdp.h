@@ -204,6 +204,50 @@ int gdp_fill_ex(const void *d_descs, int32_t n, uint32_t kernels, const char *d_
 int gdp_best_endpoint(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,
                       const int16_t *d_scores, int32_t *d_out, void *stream);
 
+/* ---- Genome in device memory ------------------------------------------ */
+
+/* Formats of a device genome handed to gdp_traceback_ex and
+   gdp_genome_segments: ASCII bytes indexed by position, or GMAP's genomecomp
+   image.  A genomecomp image holds three 32-bit words per 32 bases,
+   {high, low, flags}: bases 0..15 of the block two bits each in low (base k at
+   bits 2k), bases 16..31 in high, A C G T = 0 1 2 3, and flag bit k set where
+   base k reads as 'N' (uncompress_one_char, genome.c:9723-9760).  Positions
+   at or past the genome length read 'N' in both formats. */
+#define GDP_GENOME_ASCII 0
+#define GDP_GENOME_COMP 1
+
+/* 32-bit words of the genomecomp image of n bases: 3 per started block of 32
+   plus the two trailing words the reference writes (compress-write.c:633-637). */
+int64_t gdp_genomecomp_words(uint64_t nbases);
+
+/* Packs n ASCII bases into a genomecomp image of gdp_genomecomp_words(n)
+   words, as Compress_compress does (compress-write.c:535-640): A C G T in
+   either case by code, N as a flag, X as T with a flag, any other character
+   as a flag; the last block is padded with X and two 0xFFFFFFFF words
+   follow.  Byte-identical to the .genomecomp file gmap_build writes. */
+int gdp_genomecomp_pack(const char *seq, uint64_t nbases, uint32_t *out);
+
+/* One genome segment to extract.  left = 0: Genome_get_segment_right
+   (genome.c:11022-11075) of the bases [coord, coord + length), those at or
+   past bound = chrhigh read '*'; left = 1: Genome_get_segment_left
+   (genome.c:11078-11120) of [coord - length, coord), those below
+   bound = chroffset read '*'.  revcomp reverses and complements the segment.
+   The length bytes are written at d_out + out_off (no terminator). */
+typedef struct gdp_segment {
+  uint32_t coord;
+  uint32_t bound;
+  int32_t length;
+  int8_t left;
+  int8_t revcomp;
+  int16_t pad;
+  int64_t out_off;
+} gdp_segment;
+
+/* Extracts n segments of a device genome (format GDP_GENOME_*, genome_len
+   bases) on the device, one wave per segment. */
+int gdp_genome_segments(const void *d_genome, int32_t format, uint32_t genome_len, const gdp_segment *d_segs,
+                        int32_t n, char *d_out, void *stream);
+
 /* Tracebacks, one GPU thread per request.  d_qbuf, d_gbuf and d_dirs are
    read in aligned 16-byte windows: they must be 16-byte aligned and readable
    for 16 bytes past their last used byte (the dir_words of gdp_plan include
@@ -216,6 +260,12 @@ int gdp_best_endpoint(const void *d_descs, const int32_t *d_ul, int32_t m, int32
 int gdp_traceback(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq, const char *d_qbuf,
                   const char *d_gbuf, const uint32_t *d_dirs, const char *d_genome, uint32_t genome_len,
                   gdp_pair *d_pairs, int32_t *d_counts, void *stream);
+
+/* As gdp_traceback, with the genome skips read from a device genome of either
+   format (GDP_GENOME_ASCII as gdp_traceback, or GDP_GENOME_COMP). */
+int gdp_traceback_ex(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq, const char *d_qbuf,
+                     const char *d_gbuf, const uint32_t *d_dirs, const void *d_genome, int32_t genome_format,
+                     uint32_t genome_len, gdp_pair *d_pairs, int32_t *d_counts, void *stream);
 
 /* Writes one problem's directions (and scores if kept) in the reference's
    dense layout: [c][r] with stride rl+1 for full/upper, [r][c] with stride
@@ -234,6 +284,113 @@ int gdp_export_dense(const void *d_descs, int32_t idx, const uint32_t *d_dirs, c
    (d_fgap may be NULL for one-sided problems).  Cells not computed are 0. */
 int gdp_export_reference(const void *d_descs, int32_t idx, const uint32_t *d_dirs, const int16_t *d_scores,
                          int32_t stride, void *d_score, void *d_nogap, void *d_egap, void *d_fgap, void *stream);
+
+/* ---- Serving: continuous batching of alignment requests -------------- */
+
+/* A request loop over a device-resident genome.  Each request is the DP of
+   Dynprog_single_gap (dynprog_single.c:590-629): bands from
+   Dynprog_compute_bands, the int8 kernel when rl and gl are both below
+   use8p_size[mismatchtype] and the int16 kernel otherwise, the final score
+   matrix[gl][rl], and the traceback from (rl, gl).  The genome segment is
+   taken on the device as the reference takes it (dynprog_single.c:544-550):
+   Genome_get_segment_right(chroffset + goffset, gl, chrhigh) on the plus
+   strand, the reverse complement Genome_get_segment_left(chrhigh - goffset + 1,
+   gl, chroffset) on the minus strand.  (The caller-side shortcuts of
+   Dynprog_single_gap, single_gap_simple for gl == rl and the homopolymer
+   mode, are not part of the request.)
+
+   A server thread takes submitted requests in arrival order into batches as
+   batch slots free up and as the direction words of the batch fit the free
+   part of a fixed device budget; every batch runs on its own stream, so
+   batches overlap on the device, and results are returned as their batch
+   completes.  A request's result does not depend on the batch it ran in. */
+typedef struct gdp_server gdp_server;
+
+typedef struct gdp_server_config {
+  int64_t dir_budget_bytes; /* device memory for the direction words of all batches in flight */
+  int64_t pair_capacity;    /* pair records one batch may produce (sum over requests of rl + gl + 2) */
+  int32_t max_batch;        /* requests per batch */
+  int32_t slots;            /* batches in flight at once, one HIP stream each (1..4) */
+  int32_t max_rl;           /* longest read accepted (<= GDP_MAX_LEN) */
+  int32_t max_gl;           /* longest genome segment accepted (<= GDP_MAX_LEN) */
+} gdp_server_config;
+
+/* One alignment request (64 bytes).  read: rl query bytes, copied by
+   gdp_server_submit.  roffset/goffset/chroffset/chrhigh/watsonp/genestrand
+   have Dynprog_single_gap's meaning; extraband and widebandp feed
+   Dynprog_compute_bands; open/extend/mismatchtype are the fill's scoring. */
+typedef struct gdp_align_request {
+  uint64_t tag;
+  const char *read;
+  int32_t rl;
+  int32_t gl;
+  int32_t roffset;
+  int32_t goffset;
+  uint32_t chroffset;
+  uint32_t chrhigh;
+  int32_t extraband;
+  int8_t watsonp;
+  int8_t widebandp;
+  int8_t jump_late;
+  int8_t genestrand;
+  int8_t mismatchtype;
+  int8_t open;
+  int8_t extend;
+  int8_t pad;
+  int32_t pad2[3];
+} gdp_align_request;
+
+/* One result (72 bytes).  status GDP_OK, or GDP_ETOOBIG for a request the
+   reference also refuses (rl or gl <= 0 or above the server's limits:
+   traceback_score = -32768, NEG_INFINITY_32, no pairs).  counts =
+   {npairs, traceback_score, nmatches, nmismatches, nopens, nindels}; the
+   pairs are in the traceback's push order (Dynprog_single_gap returns them
+   reversed) at pairs[pair_index ...] of the caller's buffer given to
+   gdp_server_poll.  Times are gdp_server_now() seconds. */
+typedef struct gdp_align_result {
+  uint64_t tag;
+  int32_t status;
+  int32_t finalscore;
+  int32_t counts[6];
+  int32_t width;
+  int32_t pad;
+  int64_t pair_index;
+  double t_submit;
+  double t_done;
+} gdp_align_result;
+
+typedef struct gdp_server_stats {
+  int64_t submitted;
+  int64_t completed;
+  int64_t batches;
+  int64_t cells;        /* in-band cells of completed requests */
+  int64_t dir_bytes_peak; /* most direction bytes in flight at once */
+  int32_t inflight_peak;  /* most batches in flight at once */
+  int32_t pad;
+} gdp_server_stats;
+
+/* Starts a server over a device genome (GDP_GENOME_* format, genome_len
+   bases) that the caller keeps alive until gdp_server_destroy.  gdp_init
+   must have run.  Returns NULL on failure (gdp_last_error). */
+gdp_server *gdp_server_create(const gdp_server_config *cfg, const void *d_genome, int32_t genome_format,
+                              uint32_t genome_len);
+
+/* Queues n requests (thread-safe; any number of submitting threads). */
+int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, int32_t n);
+
+/* Moves up to max finished results into out and their pairs into
+   pairs[0 .. pair_cap) (pairs may be NULL to drop them), waiting up to
+   wait_us microseconds for the first.  Returns the number of results. */
+int32_t gdp_server_poll(gdp_server *s, gdp_align_result *out, int32_t max, gdp_pair *pairs, int64_t pair_cap,
+                        int32_t wait_us);
+
+int gdp_server_get_stats(gdp_server *s, gdp_server_stats *st);
+
+/* Finishes the batches in flight and stops the server thread. */
+void gdp_server_destroy(gdp_server *s);
+
+/* Seconds on the server's monotonic clock. */
+double gdp_server_now(void);
 
 #ifdef __cplusplus
 }
