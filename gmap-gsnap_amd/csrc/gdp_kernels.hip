@@ -736,22 +736,22 @@ extern "C" int gdp_launch_export_ref(const void *d_descs, int idx, int width, co
 
 // Persistent grid of the paired fill: as many workgroups as fit on the device
 // at once, each wave taking waves off the descriptor's queue.
-template <int G, int RING, bool GENK>
+template <bool GENK>
 static int p2_launch(const void *d_descs, int n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
                      int32_t *d_final, hipStream_t stream) {
-  typedef p2cfg<G, RING, GENK ? 4 : 1> cfg;
+  typedef p2lds<GENK> L;
   static int cached[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
   if (cached[dev] == 0) {
     int ncu = 0, per = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill2_kernel<G, RING, GENK>, P2_THREADS,
-                                                     cfg::LDS_BYTES) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill2_kernel<GENK>, P2_THREADS, L::LDS_BYTES) !=
+        hipSuccess)
       return -1;
     cached[dev] = ncu * (per > 0 ? per : 1);
   }
-  hipLaunchKernelGGL((gdp_fill2_kernel<G, RING, GENK>), dim3(cached[dev]), dim3(P2_THREADS), cfg::LDS_BYTES, stream,
+  hipLaunchKernelGGL((gdp_fill2_kernel<GENK>), dim3(cached[dev]), dim3(P2_THREADS), L::LDS_BYTES, stream,
                      (const uint8_t *)d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -766,12 +766,11 @@ extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, con
                        d_qbuf, d_gbuf, d_dirs, d_scores, d_final);
     if (hipGetLastError() != hipSuccess) return -1;
   }
-  if ((kernels & 2u) && p2_launch<16, 128, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
-  if ((kernels & 4u) && p2_launch<16, 256, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
-  if ((kernels & 8u) && p2_launch<64, 1024, false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
-  if ((kernels & 2u) && p2_launch<16, 128, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
-  if ((kernels & 4u) && p2_launch<16, 256, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
-  if ((kernels & 8u) && p2_launch<64, 1024, true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  if (kernels & 0xEu) {
+    // all paired classes in one persistent launch, then the general-lookup pass
+    if (p2_launch<false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+    if (p2_launch<true>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;
+  }
   return 0;
 }
 
