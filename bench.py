@@ -98,8 +98,16 @@ def serving_bench(args):
     from gmapdp.batch import cells as cells_of
     cells = cells_of(probs)
     log("serving: genome %d bases on the device, %d requests, %.1fs" % (nb, nreq, time.time() - t0))
+    # lane groups for latency: the autotuner times fills of the longest
+    # problems of a sample with the requests' length mix
+    from gmapdp.tune import autotune
+    from gmapdp.workload import Workload
+    tw = Workload(3, 50_000, genome_len=1 << 22)
+    lane, table = autotune(tw.problems, tw.qbuf.tobytes(), tw.gbuf.tobytes(), objective="latency")
+    del tw
+    log("serving lane groups (latency autotuner): g16_max_span=%d g64_min_rows=%d" % lane)
     srv = serve.Server(dg, dir_budget_bytes=args.serve_budget << 20, max_batch=args.serve_batch, slots=3,
-                       max_rl=1000, max_gl=1500, pair_capacity=args.serve_batch * 700)
+                       max_rl=1000, max_gl=1500, pair_capacity=args.serve_batch * 700, plan=lane)
     try:
         srv.drive(reqs[:min(nreq, 20000)], 0.0)  # warm-up
         lat, el, ok = srv.drive(reqs, 0.0)
@@ -127,6 +135,8 @@ def serving_bench(args):
             "saturation_req_s": round(sat, 1), "saturation_gcups": round(cells.sum() / (nreq / sat) / 1e9, 2),
             "p50_ms_at_50pct": half["p50_ms"], "p99_ms_at_50pct": half["p99_ms"], "points": points,
             "batches": int(st["batches"]), "max_batch": args.serve_batch, "slots": 3,
+            "lane_groups": {"g16_max_span": lane[0], "g64_min_rows": lane[1],
+                            "candidates_ms": [[list(p), round(ms, 3)] for p, ms in table]},
             "dir_budget_mib": args.serve_budget, "dir_bytes_peak": int(st["dir_bytes_peak"])}
 
 
@@ -149,6 +159,9 @@ def main():
                     help="before the timed region, copy MIB MiB once on the device (a known-size kernel that "
                          "calibrates HBM byte counters in a profiled run)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-graph", action="store_true", help="launch fill and traceback per step instead of "
+                    "replaying one captured HIP graph")
+    ap.add_argument("--no-tune", action="store_true", help="default lane groups instead of the autotuner's")
     ap.add_argument("--serve-genome", type=int, default=3_100_000_000, help="bases of the device genome")
     ap.add_argument("--serve-requests", type=int, default=300_000)
     ap.add_argument("--serve-batch", type=int, default=16384, help="largest batch the server forms")
@@ -172,11 +185,26 @@ def main():
     total_cells = batch_cells(probs)
     log("workload %s: %d problems, %.3e cells, generated in %.1fs" % (args.workload, n, total_cells, time.time() - t0))
 
-    b = DeviceBatch(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes())
+    tuning = None
+    params = None
+    if not args.no_tune:
+        from gmapdp.tune import autotune
+        t1 = time.time()
+        params, table = autotune(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes(), sample=min(n, 200_000))
+        tuning = {"objective": "throughput", "sample": min(n, 200_000), "chosen": list(params),
+                  "candidates": [[list(p), round(ms, 3)] for p, ms in table], "seconds": round(time.time() - t1, 1)}
+        log("autotuner: g16_max_span=%d g64_min_rows=%d (%.1fs)" % (params[0], params[1], time.time() - t1))
+    b = DeviceBatch(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes(), params=params)
     reqs = wl.tb_requests()
     stream = torch.cuda.current_stream()
+    graph = None
 
     def step(ev=None, tev=None):
+        if graph is not None:
+            # fill + traceback replayed from one captured graph; the fill and
+            # traceback split is measured on the eager launches below
+            b.graph_launch(graph)
+            return
         if ev is not None:
             ev[0].record(stream)
         b.fill()
@@ -197,16 +225,24 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # fill and traceback durations from HIP events around eager launches
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     tevs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    for k in range(args.steps):
+        step(evs[k], tevs[k])
     torch.cuda.synchronize()
+    fill_ms = float(np.mean([a.elapsed_time(z) for a, z in evs]))
+    tb_ms = float(np.mean([z.elapsed_time(t) for (a, z), t in zip(evs, tevs)]))
+    if not args.no_graph:
+        graph = b.graph(reqs_dev)
+        for _ in range(max(1, args.warmup)):
+            step()
+        torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         step(evs[k], tevs[k])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    fill_ms = float(np.mean([a.elapsed_time(z) for a, z in evs]))
-    tb_ms = float(np.mean([z.elapsed_time(t) for (a, z), t in zip(evs, tevs)]))
     ms_per_step = dt * 1e3 / args.steps
     gcups = total_cells * args.steps / dt / 1e9
     achieved = total_cells * OPS_PER_CELL / (fill_ms * 1e-3) / 1e12
@@ -261,7 +297,10 @@ def main():
         "vs_baseline": None,
         "dtype": "int16",
         "data": "synthetic (seeded generator of the workload)",
-        "config": {"workload": desc, "problems": n, "cells": total_cells, "step": "fill + traceback"},
+        "config": {"workload": desc, "problems": n, "cells": total_cells, "step": "fill + traceback",
+                   "launch": "eager" if graph is None else "one HIP graph per step",
+                   "lane_groups": None if params is None else {"g16_max_span": params[0],
+                                                                 "g64_min_rows": params[1]}},
         "fill_ms": round(fill_ms, 3),
         "traceback_ms": round(tb_ms, 3),
         "fill_gcups": round(total_cells / (fill_ms * 1e-3) / 1e9, 3),
@@ -273,6 +312,7 @@ def main():
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": cpu,
         "serving": serving,
+        "tuning": tuning,
     }
     print(json.dumps(out), flush=True)
 

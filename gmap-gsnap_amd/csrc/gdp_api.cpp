@@ -109,21 +109,29 @@ extern "C" int64_t gdp_plan_bytes(int32_t n) {
 
 // Full-band int16 fills without kept scores are swept in pairs, 2*(64/G) per
 // wave; the rest (one-sided triangles, int8, kept scores, very wide bands)
-// each get a 16-lane group of their own.
-static int pair_lanes(const gdp_problem &p) {
+// each get a 16-lane group of their own.  16-lane groups serve bands up to
+// g16_max_span; wider bands, and queries of at least g64_min_rows rows (a
+// sweep a quarter as long, for callers that wait on single long problems),
+// take 64-lane groups.
+static gdp_plan_params g_default_params = {240, GDP_MAX_LEN + 1, {0, 0}};
+
+static int pair_lanes(const gdp_problem &p, const gdp_plan_params &pp) {
   if (p.kind != GDP_KIND_FULL || p.width != 16 || p.keep_scores) return 0;
   const int W = p.lband + p.uband;
-  if (W <= 240) return 16;
-  if (W <= 960) return 64;
-  return 0;
+  if (W > 960) return 0;
+  if (W <= pp.g16_max_span && p.rl < pp.g64_min_rows) return 16;
+  return 64;
 }
 
 // Wave class: the ring of a class holds at least P step slots.
 static int pair_class(int G, int P) { return G == 64 ? 2 : (P <= 128 ? 0 : 1); }
 
-extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, void *descs_out, int64_t *dir_words,
-                           int64_t *score_slots) {
+extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags, const gdp_plan_params *params,
+                              void *descs_out, int64_t *dir_words, int64_t *score_slots) {
   if (n < 0 || (n > 0 && probs == nullptr) || descs_out == nullptr) return fail(GDP_EINVAL, "gdp_plan: bad arguments");
+  const gdp_plan_params pp = params ? *params : g_default_params;
+  if (pp.g16_max_span < 0 || pp.g16_max_span > 240 || pp.g64_min_rows < 1)
+    return fail(GDP_EINVAL, "gdp_plan: g16_max_span must lie in [0, 240] and g64_min_rows be positive");
   uint8_t *base = (uint8_t *)descs_out;
   gdp_dev_problem *d = (gdp_dev_problem *)descs_out;
   int64_t dw = 0, ss = 0;
@@ -148,7 +156,7 @@ extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, v
     q.open = p.open;
     q.extend = p.extend;
     q.score_off = -1;
-    if (!(flags & GDP_PLAN_SINGLE) && pair_lanes(p)) {
+    if (!(flags & GDP_PLAN_SINGLE) && pair_lanes(p, pp)) {
       d[i] = q;
       paired.push_back(i);
       continue;
@@ -166,7 +174,7 @@ extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, v
   // Waves: problems with the same tie rule and band, in order of size.
   std::stable_sort(paired.begin(), paired.end(), [&](int32_t a, int32_t b) {
     const gdp_problem &x = probs[a], &y = probs[b];
-    const int kx = pair_lanes(x), ky = pair_lanes(y);
+    const int kx = pair_lanes(x, pp), ky = pair_lanes(y, pp);
     if (kx != ky) return kx < ky;
     if ((x.jump_late != 0) != (y.jump_late != 0)) return (x.jump_late != 0) < (y.jump_late != 0);
     if (x.lband != y.lband) return x.lband < y.lband;
@@ -178,11 +186,11 @@ extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, v
   size_t k = 0;
   while (k < paired.size()) {
     const gdp_problem &p0 = probs[paired[k]];
-    const int G = pair_lanes(p0), slots = 2 * (64 / G);
+    const int G = pair_lanes(p0, pp), slots = 2 * (64 / G);
     size_t e = k;
     while (e < paired.size() && e - k < (size_t)slots) {
       const gdp_problem &x = probs[paired[e]];
-      if (pair_lanes(x) != G || (x.jump_late != 0) != (p0.jump_late != 0) || x.lband != p0.lband ||
+      if (pair_lanes(x, pp) != G || (x.jump_late != 0) != (p0.jump_late != 0) || x.lband != p0.lband ||
           x.uband != p0.uband)
         break;
       e++;
@@ -268,9 +276,14 @@ extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, v
   return GDP_OK;
 }
 
+extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, void *descs_out, int64_t *dir_words,
+                           int64_t *score_slots) {
+  return gdp_plan_tuned(probs, n, flags, nullptr, descs_out, dir_words, score_slots);
+}
+
 extern "C" int gdp_plan(const gdp_problem *probs, int32_t n, void *descs_out, int64_t *dir_words,
                         int64_t *score_slots) {
-  return gdp_plan_ex(probs, n, 0, descs_out, dir_words, score_slots);
+  return gdp_plan_tuned(probs, n, 0, nullptr, descs_out, dir_words, score_slots);
 }
 
 extern "C" int64_t gdp_cells(const gdp_problem *p) {
@@ -391,4 +404,60 @@ extern "C" int gdp_export_reference(const void *d_descs, int32_t idx, const uint
                             (hipStream_t)stream))
     return hip_fail("gdp_export_reference launch", hipGetLastError());
   return GDP_OK;
+}
+
+struct gdp_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" gdp_graph *gdp_graph_create(const void *d_descs, int32_t n, uint32_t kernels, const char *d_qbuf,
+                                       const char *d_gbuf, uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final,
+                                       const gdp_tb_request *d_reqs, int32_t nreq, const void *d_genome,
+                                       int32_t genome_format, uint32_t genome_len, gdp_pair *d_pairs,
+                                       int32_t *d_counts) {
+  if (!g_init) {
+    fail(GDP_ENOINIT, "gdp_graph_create: gdp_init has not been called");
+    return nullptr;
+  }
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    hip_fail("gdp_graph_create stream", e);
+    return nullptr;
+  }
+  gdp_graph *g = new gdp_graph;
+  e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  int rc = e == hipSuccess ? GDP_OK : hip_fail("gdp_graph_create capture", e);
+  if (rc == GDP_OK) rc = gdp_fill_ex(d_descs, n, kernels, d_qbuf, d_gbuf, d_dirs, d_scores, d_final, st);
+  if (rc == GDP_OK && nreq > 0)
+    rc = gdp_traceback_ex(d_descs, d_reqs, nreq, d_qbuf, d_gbuf, d_dirs, d_genome, genome_format, genome_len,
+                          d_pairs, d_counts, st);
+  if (e == hipSuccess) {
+    const hipError_t e2 = hipStreamEndCapture(st, &g->graph);
+    if (rc == GDP_OK && e2 != hipSuccess) rc = hip_fail("gdp_graph_create end capture", e2);
+  }
+  if (rc == GDP_OK) {
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = hip_fail("gdp_graph_create instantiate", e);
+  }
+  (void)hipStreamDestroy(st);
+  if (rc != GDP_OK) {
+    gdp_graph_destroy(g);
+    return nullptr;
+  }
+  return g;
+}
+
+extern "C" int gdp_graph_launch(gdp_graph *g, void *stream) {
+  if (!g || !g->exec) return fail(GDP_EINVAL, "gdp_graph_launch: no graph");
+  const hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+  return e == hipSuccess ? GDP_OK : hip_fail("gdp_graph_launch", e);
+}
+
+extern "C" void gdp_graph_destroy(gdp_graph *g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
 }

@@ -419,11 +419,10 @@ int64_t gdp_serve_drive(gdp_server *s, const gdp_align_request *reqs, int64_t n,
     }
   });
   std::vector<gdp_align_result> res(65536);
-  std::vector<gdp_pair> pairs((size_t)1 << 22);
   int64_t got = 0, ok = 0;
   double last = t0;
   while (got < n) {
-    const int32_t m = gdp_server_poll(s, res.data(), (int32_t)res.size(), pairs.data(), (int64_t)pairs.size(), 1000);
+    const int32_t m = gdp_server_poll(s, res.data(), (int32_t)res.size(), 1000);
     if (m < 0) break;
     for (int32_t i = 0; i < m; i++) {
       const int64_t k = (int64_t)(res[i].tag - tag0);

@@ -42,8 +42,8 @@ struct Req {
 };
 
 struct DoneBatch {
-  std::vector<gdp_align_result> res;
-  std::vector<gdp_pair> pairs;  // res[i].pair_index indexes this vector
+  std::vector<gdp_align_result> res;  // pairs point into result buffer `buf`
+  int buf = -1;                       // -1: no buffer (refused requests)
   size_t next = 0;
 };
 
@@ -92,8 +92,9 @@ class Arena {
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr;
-  uint8_t *h_up = nullptr, *d_up = nullptr, *h_down = nullptr, *d_down = nullptr;
+  uint8_t *h_up = nullptr, *d_up = nullptr, *d_down = nullptr;
   char *d_g = nullptr;
+  int buf = -1;  // result buffer of the batch in flight
   bool busy = false;
   std::vector<Req> reqs;
   std::vector<int64_t> pair_off;
@@ -127,6 +128,9 @@ struct gdp_server {
   std::mutex dmu;
   std::condition_variable cv_done;
   std::deque<DoneBatch> done;
+  std::vector<uint8_t *> rbuf;  // pinned result buffers (down_cap bytes each)
+  std::vector<int> rfree;       // buffers neither in flight nor held by the consumer
+  std::vector<int> handed;      // buffers of results the last poll returned
 
   std::mutex smu;
   gdp_server_stats st{};
@@ -184,6 +188,11 @@ void gdp_server::fail_all(const char *what, hipError_t e) {
       b.res.push_back(refused(q, t));
     }
     s.busy = false;
+    if (s.buf >= 0) {
+      std::lock_guard<std::mutex> lk(dmu);
+      rfree.push_back(s.buf);
+      s.buf = -1;
+    }
   }
   std::deque<Req> rest;
   {
@@ -201,6 +210,10 @@ void gdp_server::fail_all(const char *what, hipError_t e) {
 // budget, and queues their work on the slot's stream.  Returns false when
 // nothing could be started.
 bool gdp_server::launch(Slot &s) {
+  {
+    std::lock_guard<std::mutex> lk(dmu);
+    if (rfree.empty()) return false;  // the consumer holds every result buffer
+  }
   std::vector<Req> batch;
   {
     std::lock_guard<std::mutex> lk(mu);
@@ -245,7 +258,8 @@ bool gdp_server::launch(Slot &s) {
       qpos += q.r.rl;
       gpos += q.r.gl;
     }
-    if (gdp_plan_ex(probs.data(), n, 0, s.h_up, &dw, &ss) != GDP_OK) {
+    const gdp_plan_params *pp = (cfg.plan.g16_max_span || cfg.plan.g64_min_rows) ? &cfg.plan : nullptr;
+    if (gdp_plan_tuned(probs.data(), n, 0, pp, s.h_up, &dw, &ss) != GDP_OK) {
       // Requests were validated at submit; a planning failure is a bug.
       fatal = std::string("gdp_server: planning failed: ") + gdp_last_error();
       DoneBatch b;
@@ -336,6 +350,11 @@ bool gdp_server::launch(Slot &s) {
   s.off_pairs = align_up(s.off_counts + (int64_t)n * 24, 16);
   s.down_bytes = s.off_pairs + ppos * (int64_t)sizeof(gdp_pair);
 
+  {
+    std::lock_guard<std::mutex> lk(dmu);
+    s.buf = rfree.back();
+    rfree.pop_back();
+  }
   s.reqs = std::move(batch);
   s.busy = true;
   uint8_t *d_descs = s.d_up;
@@ -352,7 +371,7 @@ bool gdp_server::launch(Slot &s) {
                           s.d_g, dirs, genome, fmt, glen, (gdp_pair *)(s.d_down + s.off_pairs),
                           (int32_t *)(s.d_down + s.off_counts), s.stream);
   if (rc == GDP_OK) {
-    e = hipMemcpyAsync(s.h_down, s.d_down, (size_t)s.down_bytes, hipMemcpyDeviceToHost, s.stream);
+    e = hipMemcpyAsync(rbuf[s.buf], s.d_down, (size_t)s.down_bytes, hipMemcpyDeviceToHost, s.stream);
     if (e == hipSuccess) e = hipEventRecord(s.ev, s.stream);
     rc = e == hipSuccess ? GDP_OK : GDP_EHIP;
   }
@@ -375,15 +394,13 @@ bool gdp_server::launch(Slot &s) {
 void gdp_server::retire(Slot &s) {
   const double t = now_s();
   const int n = (int)s.reqs.size();
-  const int32_t *fin = (const int32_t *)s.h_down;
-  const int32_t *cnt = (const int32_t *)(s.h_down + s.off_counts);
-  const gdp_pair *pr = (const gdp_pair *)(s.h_down + s.off_pairs);
+  const uint8_t *h = rbuf[s.buf];
+  const int32_t *fin = (const int32_t *)h;
+  const int32_t *cnt = (const int32_t *)(h + s.off_counts);
+  const gdp_pair *pr = (const gdp_pair *)(h + s.off_pairs);
   DoneBatch b;
+  b.buf = s.buf;
   b.res.resize(n);
-  int64_t total = 0;
-  for (int i = 0; i < n; i++) total += std::max(0, cnt[6 * i]);
-  b.pairs.resize((size_t)total);
-  int64_t at = 0;
   for (int i = 0; i < n; i++) {
     const Req &q = s.reqs[i];
     gdp_align_result &r = b.res[i];
@@ -393,12 +410,9 @@ void gdp_server::retire(Slot &s) {
     r.finalscore = fin[i];
     for (int k = 0; k < 6; k++) r.counts[k] = cnt[6 * i + k];
     r.width = q.width;
-    r.pair_index = at;
+    r.pairs = cnt[6 * i] > 0 ? pr + s.pair_off[i] : nullptr;
     r.t_submit = q.t_submit;
     r.t_done = t;
-    const int np = std::max(0, cnt[6 * i]);
-    if (np) memcpy(b.pairs.data() + at, pr + s.pair_off[i], (size_t)np * sizeof(gdp_pair));
-    at += np;
   }
   arena.release(s.dir_off, s.dir_bytes);
   {
@@ -407,6 +421,7 @@ void gdp_server::retire(Slot &s) {
     dir_in_flight -= s.dir_bytes;
   }
   s.reqs.clear();
+  s.buf = -1;
   s.busy = false;
   post(std::move(b));
 }
@@ -477,11 +492,19 @@ extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const voi
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_up, (size_t)s->up_cap, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_down, (size_t)s->down_cap, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void **)&sl.d_up, (size_t)s->up_cap);
     if (e == hipSuccess) e = hipMalloc((void **)&sl.d_down, (size_t)s->down_cap);
     if (e == hipSuccess) e = hipMalloc((void **)&sl.d_g, (size_t)s->g_cap);
     if (e == hipSuccess) e = hipMemset(sl.d_g, 0, (size_t)s->g_cap);
+  }
+  const int nbuf = cfg->result_buffers > 0 ? std::max(cfg->result_buffers, cfg->slots) : cfg->slots + 2;
+  for (int k = 0; k < nbuf && e == hipSuccess; k++) {
+    uint8_t *h = nullptr;
+    e = hipHostMalloc((void **)&h, (size_t)s->down_cap, hipHostMallocDefault);
+    if (e == hipSuccess) {
+      s->rbuf.push_back(h);
+      s->rfree.push_back(k);
+    }
   }
   if (e != hipSuccess) {
     set_err("gdp_server_create: %s", hipGetErrorString(e));
@@ -543,39 +566,23 @@ extern "C" int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, i
   return GDP_OK;
 }
 
-extern "C" int32_t gdp_server_poll(gdp_server *s, gdp_align_result *out, int32_t max, gdp_pair *pairs,
-                                   int64_t pair_cap, int32_t wait_us) {
+extern "C" int32_t gdp_server_poll(gdp_server *s, gdp_align_result *out, int32_t max, int32_t wait_us) {
   if (!s || max < 0 || (max > 0 && !out)) return GDP_EINVAL;
   std::unique_lock<std::mutex> lk(s->dmu);
+  // results of the previous call are no longer referenced
+  for (int k : s->handed) s->rfree.push_back(k);
+  s->handed.clear();
   if (s->done.empty() && wait_us > 0)
     s->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [&] { return !s->done.empty(); });
   int32_t k = 0;
-  int64_t used = 0;
   while (k < max && !s->done.empty()) {
     DoneBatch &b = s->done.front();
-    while (k < max && b.next < b.res.size()) {
-      gdp_align_result r = b.res[b.next];
-      const int64_t np = std::max(0, r.counts[0]);
-      if (pairs) {
-        if (used + np > pair_cap) {
-          if (k == 0) {
-            set_err("gdp_server_poll: pair buffer of %lld records too small for a result of %lld",
-                    (long long)pair_cap, (long long)np);
-            return GDP_ENOMEM;
-          }
-          return k;
-        }
-        if (np) memcpy(pairs + used, b.pairs.data() + r.pair_index, (size_t)np * sizeof(gdp_pair));
-        r.pair_index = used;
-        used += np;
-      } else {
-        r.pair_index = -1;
-      }
-      out[k++] = r;
-      b.next++;
-    }
-    if (b.next >= b.res.size()) s->done.pop_front();
+    while (k < max && b.next < b.res.size()) out[k++] = b.res[b.next++];
+    if (b.next < b.res.size()) break;
+    if (b.buf >= 0) s->handed.push_back(b.buf);
+    s->done.pop_front();
   }
+  // a batch returned in part keeps its buffer until a poll after its last result
   return k;
 }
 
@@ -594,13 +601,13 @@ extern "C" void gdp_server_destroy(gdp_server *s) {
   for (Slot &sl : s->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.h_up) (void)hipHostFree(sl.h_up);
-    if (sl.h_down) (void)hipHostFree(sl.h_down);
     if (sl.d_up) (void)hipFree(sl.d_up);
     if (sl.d_down) (void)hipFree(sl.d_down);
     if (sl.d_g) (void)hipFree(sl.d_g);
     if (sl.ev) (void)hipEventDestroy(sl.ev);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
+  for (uint8_t *h : s->rbuf) (void)hipHostFree(h);
   if (s->d_dirs) (void)hipFree(s->d_dirs);
   delete s;
 }

@@ -7,7 +7,7 @@ import ctypes
 
 import numpy as np
 
-from .lib import PAIR_DTYPE, PROBLEM_DTYPE, TBREQ_DTYPE, load
+from .lib import PAIR_DTYPE, PLAN_PARAMS_DTYPE, PROBLEM_DTYPE, TBREQ_DTYPE, load
 
 _initialized_mode = None
 
@@ -40,15 +40,24 @@ def _check(L, rc, what):
 PLAN_SINGLE = 1  # GDP_PLAN_SINGLE: every problem swept on its own
 
 
-def plan(problems, flags=0):
-    """Host planning (no device needed): descriptors and buffer sizes."""
+def plan_params(g16_max_span=240, g64_min_rows=8001):
+    """gdp_plan_params: lane-group choice of the paired sweep."""
+    p = np.zeros(1, PLAN_PARAMS_DTYPE)
+    p["g16_max_span"], p["g64_min_rows"] = g16_max_span, g64_min_rows
+    return p
+
+
+def plan(problems, flags=0, params=None):
+    """Host planning (no device needed): descriptors and buffer sizes.
+    params: (g16_max_span, g64_min_rows) or None for the defaults."""
     L = load()
     problems = np.ascontiguousarray(problems, PROBLEM_DTYPE)
     n = len(problems)
     descs = np.zeros(int(L.gdp_plan_bytes(n)), np.uint8)
     dw, ss = ctypes.c_int64(), ctypes.c_int64()
-    _check(L, L.gdp_plan_ex(problems.ctypes.data, n, flags, descs.ctypes.data, ctypes.byref(dw), ctypes.byref(ss)),
-           "gdp_plan")
+    pp = None if params is None else plan_params(*params)
+    _check(L, L.gdp_plan_tuned(problems.ctypes.data, n, flags, None if pp is None else pp.ctypes.data,
+                               descs.ctypes.data, ctypes.byref(dw), ctypes.byref(ss)), "gdp_plan")
     return descs, dw.value, ss.value
 
 
@@ -61,16 +70,17 @@ def cells(problems):
 class DeviceBatch:
     """Problems, sequences and outputs resident in device memory."""
 
-    def __init__(self, problems, qbuf, gbuf, mode=0, device="cuda", single=False):
+    def __init__(self, problems, qbuf, gbuf, mode=0, device="cuda", single=False, params=None):
         """single=True plans every problem on its own sweep (required for
-        fill_one, the one-launch-per-problem form)."""
+        fill_one, the one-launch-per-problem form); params = (g16_max_span,
+        g64_min_rows) of the paired sweep (None: defaults)."""
         torch = _torch()
         self.torch = torch
         self.L = init(mode)
         self.problems = np.ascontiguousarray(problems, PROBLEM_DTYPE)
         self.n = len(self.problems)
         self.single = single
-        descs, dw, ss = plan(self.problems, PLAN_SINGLE if single else 0)
+        descs, dw, ss = plan(self.problems, PLAN_SINGLE if single else 0, params)
         self.kernels = int(self.L.gdp_plan_kernels(descs.ctypes.data, self.n))
         self.dir_words, self.score_slots = dw, ss
         dev = torch.device(device)
@@ -176,6 +186,24 @@ class DeviceBatch:
                                             self.d_q.data_ptr(), self.d_g.data_ptr(), self.d_dirs.data_ptr(),
                                             gptr, h["glen"], h["d_pairs"].data_ptr(),
                                             h["d_counts"].data_ptr() + first * 24, self.stream), "gdp_traceback")
+
+    def graph(self, h):
+        """Captures fill + traceback (requests of prepare_traceback's handle h)
+        into one HIP graph; returns a handle for graph_launch."""
+        gptr = h["d_genome"].data_ptr() if h["d_genome"] is not None else None
+        g = self.L.gdp_graph_create(self.d_descs.data_ptr(), self.n, self.kernels, self.d_q.data_ptr(),
+                                    self.d_g.data_ptr(), self.d_dirs.data_ptr(), self.d_scores.data_ptr(),
+                                    self.d_final.data_ptr(), h["d_reqs"].data_ptr(), h["n"], gptr, 0, h["glen"],
+                                    h["d_pairs"].data_ptr(), h["d_counts"].data_ptr())
+        if not g:
+            raise RuntimeError("gdp_graph_create failed: %s" % self.L.gdp_last_error().decode())
+        return g
+
+    def graph_launch(self, g):
+        _check(self.L, self.L.gdp_graph_launch(g, self.stream), "gdp_graph_launch")
+
+    def graph_destroy(self, g):
+        self.L.gdp_graph_destroy(g)
 
     def traceback_results(self, h):
         counts = h["d_counts"].cpu().numpy()[: 6 * h["n"]].reshape(-1, 6)

@@ -55,10 +55,12 @@ ALIGN_REQ_DTYPE = np.dtype([("tag", "<u8"), ("read", "<u8"), ("rl", "<i4"), ("gl
                             ("mismatchtype", "i1"), ("open", "i1"), ("extend", "i1"), ("pad", "i1"),
                             ("pad2", "<i4", (3,))])
 ALIGN_RES_DTYPE = np.dtype([("tag", "<u8"), ("status", "<i4"), ("finalscore", "<i4"), ("counts", "<i4", (6,)),
-                            ("width", "<i4"), ("pad", "<i4"), ("pair_index", "<i8"), ("t_submit", "<f8"),
+                            ("width", "<i4"), ("pad", "<i4"), ("pairs", "<u8"), ("t_submit", "<f8"),
                             ("t_done", "<f8")])
+PLAN_PARAMS_DTYPE = np.dtype([("g16_max_span", "<i4"), ("g64_min_rows", "<i4"), ("pad", "<i4", (2,))])
 SERVER_CFG_DTYPE = np.dtype([("dir_budget_bytes", "<i8"), ("pair_capacity", "<i8"), ("max_batch", "<i4"),
-                             ("slots", "<i4"), ("max_rl", "<i4"), ("max_gl", "<i4")])
+                             ("slots", "<i4"), ("max_rl", "<i4"), ("max_gl", "<i4"), ("result_buffers", "<i4"),
+                             ("pad", "<i4"), ("plan", PLAN_PARAMS_DTYPE)])
 SERVER_STATS_DTYPE = np.dtype([("submitted", "<i8"), ("completed", "<i8"), ("batches", "<i8"), ("cells", "<i8"),
                                ("dir_bytes_peak", "<i8"), ("inflight_peak", "<i4"), ("pad", "<i4")])
 assert PROBLEM_DTYPE.itemsize == 48 and TBREQ_DTYPE.itemsize == 48 and PAIR_DTYPE.itemsize == 16
@@ -97,6 +99,11 @@ def load(path=None):
         "gdp_plan": (ctypes.c_int, [vp, i32, vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "gdp_plan_ex": (ctypes.c_int, [vp, i32, i32, vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "gdp_plan_bytes": (i64, [i32]),
+        "gdp_plan_tuned": (ctypes.c_int, [vp, i32, i32, vp, vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+        "gdp_graph_create": (vp, [vp, i32, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, i32, vp, i32, ctypes.c_uint32,
+                                  vp, vp]),
+        "gdp_graph_launch": (ctypes.c_int, [vp, vp]),
+        "gdp_graph_destroy": (None, [vp]),
         "gdp_plan_kernels": (ctypes.c_uint32, [vp, i32]),
         "gdp_fill_ex": (ctypes.c_int, [vp, i32, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]),
         "gdp_cells": (i64, [vp]),
@@ -109,7 +116,7 @@ def load(path=None):
         "gdp_genomecomp_pack": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
         "gdp_server_create": (vp, [vp, vp, i32, ctypes.c_uint32]),
         "gdp_server_submit": (ctypes.c_int, [vp, vp, i32]),
-        "gdp_server_poll": (i32, [vp, vp, i32, vp, i64, i32]),
+        "gdp_server_poll": (i32, [vp, vp, i32, i32]),
         "gdp_server_get_stats": (ctypes.c_int, [vp, vp]),
         "gdp_server_destroy": (None, [vp]),
         "gdp_server_now": (ctypes.c_double, []),

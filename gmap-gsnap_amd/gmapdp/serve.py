@@ -108,13 +108,16 @@ class Server:
     """gdp_server over a DeviceGenome."""
 
     def __init__(self, genome, dir_budget_bytes=1 << 30, max_batch=8192, slots=3, max_rl=1000, max_gl=1500,
-                 pair_capacity=None):
+                 pair_capacity=None, plan=None, result_buffers=0):
         L = init()
         self.L, self.genome = L, genome
         cfg = np.zeros(1, SERVER_CFG_DTYPE)
         cfg["dir_budget_bytes"] = dir_budget_bytes
         cfg["max_batch"], cfg["slots"], cfg["max_rl"], cfg["max_gl"] = max_batch, slots, max_rl, max_gl
         cfg["pair_capacity"] = pair_capacity or max_batch * (max_rl + max_gl + 2) // 2
+        cfg["result_buffers"] = result_buffers
+        if plan is not None:
+            cfg["plan"]["g16_max_span"], cfg["plan"]["g64_min_rows"] = plan
         self.h = L.gdp_server_create(cfg.ctypes.data, genome.ptr, genome.fmt, genome.nbases)
         if not self.h:
             raise RuntimeError("gdp_server_create failed: %s" % L.gdp_last_error().decode())
@@ -123,16 +126,23 @@ class Server:
         reqs = np.ascontiguousarray(reqs, ALIGN_REQ_DTYPE)
         _check(self.L, self.L.gdp_server_submit(self.h, reqs.ctypes.data, len(reqs)), "gdp_server_submit")
 
-    def poll(self, max_results=4096, pair_cap=1 << 20, wait_us=1000):
-        """Returns (results, pairs) of finished requests."""
+    def poll(self, max_results=4096, wait_us=1000):
+        """Returns [(result, pairs)] of finished requests; the pairs are
+        copied out of the server's pinned buffers before the next poll."""
         res = np.zeros(max_results, ALIGN_RES_DTYPE)
-        pairs = np.zeros(pair_cap, PAIR_DTYPE)
-        n = self.L.gdp_server_poll(self.h, res.ctypes.data, max_results, pairs.ctypes.data, pair_cap, wait_us)
+        n = self.L.gdp_server_poll(self.h, res.ctypes.data, max_results, wait_us)
         if n < 0:
             raise RuntimeError("gdp_server_poll failed: %s" % self.L.gdp_last_error().decode())
-        res = res[:n].copy()
-        used = int((res["pair_index"] + res["counts"][:, 0]).max()) if n else 0
-        return res, pairs[:used].copy()
+        out = []
+        for r in res[:n]:
+            m = int(r["counts"][0])
+            if m > 0:
+                buf = (ctypes.c_char * (m * PAIR_DTYPE.itemsize)).from_address(int(r["pairs"]))
+                pairs = np.frombuffer(bytes(buf), PAIR_DTYPE)
+            else:
+                pairs = np.zeros(0, PAIR_DTYPE)
+            out.append((r.copy(), pairs))
+        return out
 
     def collect(self, count, timeout_s=60.0):
         """Polls until `count` results arrived; returns {tag: (result, pairs)}."""
@@ -142,10 +152,8 @@ class Server:
         while len(out) < count:
             if time.time() > t_end:
                 raise TimeoutError("gdp_server: %d of %d results after %.0f s" % (len(out), count, timeout_s))
-            res, pairs = self.poll()
-            for r in res:
-                i, n = int(r["pair_index"]), int(r["counts"][0])
-                out[int(r["tag"])] = (r, pairs[i:i + n] if n > 0 else pairs[:0])
+            for r, pairs in self.poll():
+                out[int(r["tag"])] = (r, pairs)
         return out
 
     def stats(self):

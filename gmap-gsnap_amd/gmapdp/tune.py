@@ -1,0 +1,49 @@
+"""Autotuner of the paired sweep's lane groups over a measured batch.
+
+The plan's two knobs (gdp_plan_params) trade padding against sweep length:
+16-lane groups pack eight problems into a wave with little padding, 64-lane
+groups sweep a long problem in about a quarter of the steps.  The tuner
+times real fills of a seeded sample of the caller's problems on the device
+for each candidate and keeps the best: the whole sample's fill time for
+throughput, or the fill time of a batch of the sample's longest problems
+for latency (the time a request waits on the slowest problem of its batch).
+"""
+import numpy as np
+
+from .batch import DeviceBatch, _torch
+
+SPANS = (240, 112, 48)
+ROWS = (8001, 512, 256, 128)
+
+
+def _time_fill(b, reps):
+    torch = _torch()
+    b.fill()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, z in ev:
+        a.record()
+        b.fill()
+        z.record()
+    torch.cuda.synchronize()
+    return float(np.median([a.elapsed_time(z) for a, z in ev]))
+
+
+def autotune(problems, qbuf, gbuf, sample=200_000, objective="throughput", reps=3, seed=0, longest=64):
+    """Returns ((g16_max_span, g64_min_rows), table) where table lists
+    (params, milliseconds) of every candidate."""
+    rng = np.random.default_rng(seed)
+    n = len(problems)
+    if objective == "latency":
+        idx = np.argsort(problems["rl"].astype(np.int64) * (problems["lband"] + problems["uband"] + 1))[-longest:]
+    else:
+        idx = np.sort(rng.choice(n, size=min(sample, n), replace=False))
+    sub = np.ascontiguousarray(problems[idx])
+    table = []
+    for span in SPANS:
+        for rows in ROWS:
+            b = DeviceBatch(sub, qbuf, gbuf, params=(span, rows))
+            table.append(((span, rows), _time_fill(b, reps)))
+            del b
+    best = min(table, key=lambda t: t[1])[0]
+    return best, table

@@ -175,6 +175,22 @@ int gdp_plan(const gdp_problem *probs, int32_t n, void *descs_out, int64_t *dir_
 int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, void *descs_out, int64_t *dir_words,
                 int64_t *score_slots);
 
+/* Lane-group choice of the paired sweep.  Full-band int16 problems without
+   kept scores and with lband + uband <= g16_max_span (at most 240) are swept
+   by 16-lane groups, eight problems to a wave; those with a wider band (up
+   to GDP_MAX_SPAN) or with rl >= g64_min_rows by 64-lane groups, two to a
+   wave, whose sweep takes about a quarter of the steps (lower latency for a
+   long problem, more padding per cell).  Defaults: {240, GDP_MAX_LEN + 1}. */
+typedef struct gdp_plan_params {
+  int32_t g16_max_span;
+  int32_t g64_min_rows;
+  int32_t pad[2];
+} gdp_plan_params;
+
+/* As gdp_plan_ex with explicit lane-group parameters (NULL = defaults). */
+int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags, const gdp_plan_params *params,
+                   void *descs_out, int64_t *dir_words, int64_t *score_slots);
+
 /* In-band cell count of a problem (the throughput work unit). */
 int64_t gdp_cells(const gdp_problem *prob);
 
@@ -195,6 +211,22 @@ int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_g
 uint32_t gdp_plan_kernels(const void *descs, int32_t n);
 int gdp_fill_ex(const void *d_descs, int32_t n, uint32_t kernels, const char *d_qbuf, const char *d_gbuf,
                 uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, void *stream);
+
+/* A fill followed by a traceback, captured once into a HIP graph and
+   replayed with gdp_graph_launch: the whole batched step in one launch
+   from the host.  Arguments are those of gdp_fill_ex and gdp_traceback_ex;
+   the buffers they name must stay allocated while the graph lives, and
+   their contents may change between replays.  The persistent fill resets
+   its queues at the end of each run, so replays need no host work.  The
+   capture runs on a stream of the library's own; nothing executes until
+   gdp_graph_launch. */
+typedef struct gdp_graph gdp_graph;
+gdp_graph *gdp_graph_create(const void *d_descs, int32_t n, uint32_t kernels, const char *d_qbuf, const char *d_gbuf,
+                            uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, const gdp_tb_request *d_reqs,
+                            int32_t nreq, const void *d_genome, int32_t genome_format, uint32_t genome_len,
+                            gdp_pair *d_pairs, int32_t *d_counts);
+int gdp_graph_launch(gdp_graph *g, void *stream);
+void gdp_graph_destroy(gdp_graph *g);
 
 /* Best end cell of end problems: for each of m pairs (upper, lower) of
    one-sided fills with keep_scores set, find_best_endpoint_{8,16}
@@ -313,6 +345,9 @@ typedef struct gdp_server_config {
   int32_t slots;            /* batches in flight at once, one HIP stream each (1..4) */
   int32_t max_rl;           /* longest read accepted (<= GDP_MAX_LEN) */
   int32_t max_gl;           /* longest genome segment accepted (<= GDP_MAX_LEN) */
+  int32_t result_buffers;   /* pinned result buffers (>= slots; 0 = slots + 2) */
+  int32_t pad;
+  gdp_plan_params plan;     /* lane groups of the batches (all zero = gdp_plan's defaults) */
 } gdp_server_config;
 
 /* One alignment request (64 bytes).  read: rl query bytes, copied by
@@ -345,8 +380,9 @@ typedef struct gdp_align_request {
    traceback_score = -32768, NEG_INFINITY_32, no pairs).  counts =
    {npairs, traceback_score, nmatches, nmismatches, nopens, nindels}; the
    pairs are in the traceback's push order (Dynprog_single_gap returns them
-   reversed) at pairs[pair_index ...] of the caller's buffer given to
-   gdp_server_poll.  Times are gdp_server_now() seconds. */
+   reversed) at `pairs`, in pinned host memory the server owns, valid until
+   the next gdp_server_poll on this server (NULL when npairs is 0).  Times
+   are gdp_server_now() seconds. */
 typedef struct gdp_align_result {
   uint64_t tag;
   int32_t status;
@@ -354,7 +390,7 @@ typedef struct gdp_align_result {
   int32_t counts[6];
   int32_t width;
   int32_t pad;
-  int64_t pair_index;
+  const gdp_pair *pairs;
   double t_submit;
   double t_done;
 } gdp_align_result;
@@ -378,11 +414,13 @@ gdp_server *gdp_server_create(const gdp_server_config *cfg, const void *d_genome
 /* Queues n requests (thread-safe; any number of submitting threads). */
 int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, int32_t n);
 
-/* Moves up to max finished results into out and their pairs into
-   pairs[0 .. pair_cap) (pairs may be NULL to drop them), waiting up to
-   wait_us microseconds for the first.  Returns the number of results. */
-int32_t gdp_server_poll(gdp_server *s, gdp_align_result *out, int32_t max, gdp_pair *pairs, int64_t pair_cap,
-                        int32_t wait_us);
+/* Moves up to max finished results into out, waiting up to wait_us
+   microseconds for the first, and gives the pinned buffers of the results
+   returned by the previous call back to the server (their pairs pointers
+   become invalid).  One thread polls.  Returns the number of results.  When
+   the consumer holds every result buffer, no new batch starts until it
+   polls again. */
+int32_t gdp_server_poll(gdp_server *s, gdp_align_result *out, int32_t max, int32_t wait_us);
 
 int gdp_server_get_stats(gdp_server *s, gdp_server_stats *st);
 

@@ -28,11 +28,12 @@ def _class_specs(seed, n, lb, ub, late, rl_range, alph_q=b"ACGT", genome_n=0.0, 
     return specs
 
 
-def _check(oracle, specs, seed):
+def _check(oracle, specs, seed, params=None):
     from test_gpu_parity import _check_fills, _check_tracebacks, _run
-    b = _run(oracle, specs, keep_scores=False)
+    b = _run(oracle, specs, keep_scores=False, params=params)
     _check_fills(oracle, specs, b)
     _check_tracebacks(oracle, specs, b, seed)
+    return b
 
 
 @pytest.mark.parametrize("late", [False, True])
@@ -60,3 +61,59 @@ def test_paired_mixed_with_single(oracle):
     specs = _class_specs(11, 40, 10, 40, True, (100, 160)) + H.random_specs(12, 120)
     order = np.random.default_rng(3).permutation(len(specs))
     _check(oracle, [specs[k] for k in order], 8)
+
+
+@pytest.mark.parametrize("params", [(0, 8001), (240, 64), (48, 200)])
+def test_paired_lane_group_choice(oracle, params):
+    """64-lane groups on narrow bands (the autotuner's and the serving
+    loop's latency choice) and 16-lane groups up to a smaller span give the
+    same bits as every other sweep."""
+    specs = []
+    for k, (lb, ub) in enumerate([(0, 0), (2, 5), (12, 12), (5, 60), (30, 90), (3, 200)]):
+        specs += _class_specs(300 + k, 21, lb, ub, bool(k % 2), (1, 400))
+    b = _check(oracle, specs, 8, params)
+    d = b.d_descs.cpu().numpy()[: b.n * 96].reshape(b.n, 96)
+    layout, glog = d[:, 58], d[:, 59]  # gdp_dev_problem.layout / .glog
+    rl = np.array([s.rl for s in specs])
+    span = np.array([s.lband + s.uband for s in specs])
+    want64 = (span > params[0]) | (rl >= params[1])
+    assert (layout == 1).all()
+    assert ((glog == 6) == want64).all()
+
+
+def test_graph_replays_equal_eager(oracle):
+    """fill + traceback captured in one HIP graph: every replay, with eager
+    work and synchronisations between replays, gives the eager results."""
+    import torch
+    from gmapdp.batch import DeviceBatch
+    specs = _class_specs(40, 60, 6, 40, True, (20, 300)) + H.random_specs(41, 100)
+    probs, qbuf, gbuf = H.pack_batch(specs, keep_scores=False)
+    b = DeviceBatch(probs, qbuf, gbuf)
+    reqs = np.array([H.tb_request(i, s, s.rl, s.gl) for i, s in enumerate(specs)
+                     if s.kind == 0 and s.uband >= s.gl - s.rl and s.lband >= s.rl - s.gl])
+    off = np.zeros(len(reqs), np.int64)
+    off[1:] = np.cumsum(reqs["max_pairs"].astype(np.int64))[:-1]
+    reqs["pair_off"] = off
+    h = b.prepare_traceback(reqs)
+    b.fill()
+    b.traceback_device(h)
+    torch.cuda.synchronize()
+    # direction words up to the sink that empty slots of partial waves share
+    # (its contents are whatever slot wrote last)
+    hdr = b.d_descs.cpu().numpy()[b.n * 96: b.n * 96 + 256]
+    sink = int(hdr[176:184].view(np.int64)[0])
+    assert 0 < sink <= b.dir_words
+    want = (b.d_dirs[:sink].clone(), b.d_final.clone(), h["d_pairs"].clone(), h["d_counts"].clone())
+    g = b.graph(h)
+    try:
+        for k in range(4):
+            for t in (b.d_dirs, b.d_final, h["d_pairs"], h["d_counts"]):
+                t.zero_()
+            torch.cuda.synchronize()
+            b.graph_launch(g)
+            torch.cuda.synchronize()
+            got = (b.d_dirs[:sink], b.d_final, h["d_pairs"], h["d_counts"])
+            for x, y in zip(got, want):
+                assert torch.equal(x, y), k
+    finally:
+        b.graph_destroy(g)

@@ -12,10 +12,10 @@ import helpers as H
 pytestmark = pytest.mark.gpu
 
 
-def _run(oracle, specs, keep_scores=True):
+def _run(oracle, specs, keep_scores=True, params=None):
     from gmapdp.batch import DeviceBatch
     probs, qbuf, gbuf = H.pack_batch(specs, keep_scores=keep_scores)
-    b = DeviceBatch(probs, qbuf, gbuf)
+    b = DeviceBatch(probs, qbuf, gbuf, params=params)
     b.fill()
     return b
 

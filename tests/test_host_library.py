@@ -155,3 +155,25 @@ def test_workload_parameter_ranges():
     assert (p["kind"][0::2] == 1).all() and (p["kind"][1::2] == 2).all()
     assert ((p["width"] == 8) == ((p["rl"] < 24) | (p["gl"] < 24))).all()
     assert (p["mismatchtype"] == 3).all()
+
+
+def test_plan_lane_group_parameters():
+    from gmapdp.batch import plan
+    specs = H.random_specs(3, 200, kinds=(0,), widths=(16,))
+    probs, _, _ = H.pack_batch(specs)
+    rl = probs["rl"]
+    span = probs["lband"].astype(int) + probs["uband"]
+    for params in (None, (240, 8001), (0, 8001), (64, 100), (240, 1)):
+        descs, dw, ss = plan(probs, 0, params)
+        d = descs[: len(probs) * 96].reshape(len(probs), 96)
+        layout, glog = d[:, 58], d[:, 59]
+        g16, g64 = (240, 8001) if params is None else params
+        paired = span <= 960
+        assert (layout[paired] == 1).all()
+        want64 = paired & ((span > g16) | (rl >= g64))
+        assert ((glog == 6) == want64)[paired].all(), params
+        assert dw > 0
+    with pytest.raises(RuntimeError):
+        plan(probs, 0, (241, 8001))
+    with pytest.raises(RuntimeError):
+        plan(probs, 0, (240, 0))
