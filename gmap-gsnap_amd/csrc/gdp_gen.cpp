@@ -264,9 +264,18 @@ int gdp_gen_workload(int config, int variant, uint64_t seed, int64_t first, int6
 
 // Random genome of `nbases` bases written straight into genomecomp words
 // (gdp_genomecomp_words(nbases) of them).  Each 32-base block draws its two
-// code words from its own stream; one block in 2048 is a run of N (flags set,
-// codes 0, as the packer writes N).  The tail of the last block is X and the
-// two trailing words are 0xFFFFFFFF, as in a packed image.
+// code words from its own stream.  N comes in runs, as assembly gaps do: a
+// quarter of the 2-Mbp chunks hold one run of 1-50 kb (flags set, codes 0,
+// as the packer writes N).  The tail of the last block is X and the two
+// trailing words are 0xFFFFFFFF, as in a packed image.
+static bool n_block(uint64_t seed, uint64_t b) {
+  Rng r(stream_seed(seed ^ 0x4e72756e73ull, (int64_t)(b >> 16)));
+  if (r.next() % 4 != 0) return false;
+  const uint64_t len = 32 + r.next() % 1531, start = r.next() % ((1u << 16) - len);
+  const uint64_t k = b & 0xFFFF;
+  return k >= start && k < start + len;
+}
+
 void gdp_gen_genomecomp(uint64_t seed, uint64_t nbases, uint32_t *out) {
   const uint64_t nblocks = (nbases + 31) / 32;
   const int nthreads = 16;
@@ -275,9 +284,9 @@ void gdp_gen_genomecomp(uint64_t seed, uint64_t nbases, uint32_t *out) {
     th.emplace_back([=]() {
       for (uint64_t b = nblocks * t / nthreads; b < nblocks * (t + 1) / nthreads; b++) {
         Rng r(stream_seed(seed ^ 0x67656e6f6d65ull, (int64_t)b));
-        const uint64_t v = r.next(), u = r.next();
+        const uint64_t v = r.next();
         uint32_t high = (uint32_t)(v >> 32), low = (uint32_t)v, flags = 0;
-        if (u % 2048 == 0) high = low = 0, flags = 0xFFFFFFFFu;
+        if (n_block(seed, b)) high = low = 0, flags = 0xFFFFFFFFu;
         for (int k = 0; k < 32; k++) {
           if (b * 32 + (uint64_t)k < nbases) continue;
           if (k < 16) low |= 3u << (2 * k);
@@ -345,8 +354,17 @@ int gdp_gen_serve_requests(uint64_t seed, int64_t first, int64_t count, const ui
         const int eb = rng.range(3, 64);
         const double div = rng.uni() < 0.10 ? 0.30 : 0.02;
         const bool watson = (rng.next() & 1) != 0;
-        const int64_t goffset = 8 + (int64_t)(rng.uni() * (double)(nbases - (uint64_t)gl - 32));
+        // reads come from sequenced genome: draw again while the segment
+        // touches a run of N (at most 16 draws)
         const uint64_t chrhigh = nbases - 1;
+        int64_t goffset = 0;
+        for (int tries = 0; tries < 16; tries++) {
+          goffset = 8 + (int64_t)(rng.uni() * (double)(nbases - (uint64_t)gl - 32));
+          const uint64_t lo = watson ? (uint64_t)goffset : chrhigh - (uint64_t)goffset + 1 - (uint64_t)gl;
+          bool clean = true;
+          for (uint64_t blk = lo / 32; blk <= (lo + (uint64_t)gl - 1) / 32 && clean; blk++) clean = gc[3 * blk + 2] == 0;
+          if (clean) break;
+        }
         if (watson) {
           for (int j = 0; j < gl; j++) seg[j] = gc_base(gc, nbases, (uint64_t)goffset + j);
         } else {

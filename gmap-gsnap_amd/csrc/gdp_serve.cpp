@@ -91,7 +91,8 @@ class Arena {
 
 struct Slot {
   hipStream_t stream = nullptr;
-  hipEvent_t ev = nullptr;
+  hipEvent_t ev = nullptr, ev0 = nullptr;  // end and start of the batch on the stream
+  double t_launch = 0;
   uint8_t *h_up = nullptr, *d_up = nullptr, *d_down = nullptr;
   char *d_g = nullptr;
   int buf = -1;  // result buffer of the batch in flight
@@ -359,7 +360,9 @@ bool gdp_server::launch(Slot &s) {
   s.busy = true;
   uint8_t *d_descs = s.d_up;
   uint32_t *dirs = (uint32_t *)(d_dirs + s.dir_off);
-  hipError_t e = hipMemcpyAsync(s.d_up, s.h_up, (size_t)up_bytes, hipMemcpyHostToDevice, s.stream);
+  s.t_launch = now_s();
+  hipError_t e = hipEventRecord(s.ev0, s.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(s.d_up, s.h_up, (size_t)up_bytes, hipMemcpyHostToDevice, s.stream);
   int rc = e == hipSuccess ? GDP_OK : GDP_EHIP;
   if (rc == GDP_OK)
     rc = gdp_genome_segments(genome, fmt, glen, (const gdp_segment *)(s.d_up + off_segs), n, s.d_g, s.stream);
@@ -415,10 +418,14 @@ void gdp_server::retire(Slot &s) {
     r.t_done = t;
   }
   arena.release(s.dir_off, s.dir_bytes);
+  float dev_ms = 0;
+  if (hipEventElapsedTime(&dev_ms, s.ev0, s.ev) != hipSuccess) dev_ms = 0;
   {
     std::lock_guard<std::mutex> lk(smu);
     st.completed += n;
     dir_in_flight -= s.dir_bytes;
+    st.batch_seconds += t - s.t_launch;
+    st.device_seconds += dev_ms * 1e-3;
   }
   s.reqs.clear();
   s.buf = -1;
@@ -434,7 +441,10 @@ void gdp_server::run() {
       if (!s.busy) continue;
       const hipError_t e = hipEventQuery(s.ev);
       if (e == hipSuccess) {
+        const double t0 = now_s();
         retire(s);
+        std::lock_guard<std::mutex> lk(smu);
+        st.retire_seconds += now_s() - t0;
         progressed = true;
       } else if (e != hipErrorNotReady) {
         fail_all("gdp_server batch", e);
@@ -444,7 +454,12 @@ void gdp_server::run() {
     bool any_busy = false;
     for (Slot &s : slots) {
       if (!s.busy && fatal.empty() && !stop.load()) {
-        if (launch(s)) progressed = true;
+        const double t0 = now_s();
+        if (launch(s)) {
+          progressed = true;
+          std::lock_guard<std::mutex> lk(smu);
+          st.prep_seconds += now_s() - t0;
+        }
       }
       any_busy = any_busy || s.busy;
     }
@@ -490,7 +505,8 @@ extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const voi
   s->slots.resize(cfg->slots);
   for (Slot &sl : s->slots) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreate(&sl.ev);
+    if (e == hipSuccess) e = hipEventCreate(&sl.ev0);
     if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_up, (size_t)s->up_cap, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void **)&sl.d_up, (size_t)s->up_cap);
     if (e == hipSuccess) e = hipMalloc((void **)&sl.d_down, (size_t)s->down_cap);
@@ -605,6 +621,7 @@ extern "C" void gdp_server_destroy(gdp_server *s) {
     if (sl.d_down) (void)hipFree(sl.d_down);
     if (sl.d_g) (void)hipFree(sl.d_g);
     if (sl.ev) (void)hipEventDestroy(sl.ev);
+    if (sl.ev0) (void)hipEventDestroy(sl.ev0);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
   for (uint8_t *h : s->rbuf) (void)hipHostFree(h);

@@ -399,10 +399,13 @@ typedef struct gdp_server_stats {
   int64_t submitted;
   int64_t completed;
   int64_t batches;
-  int64_t cells;        /* in-band cells of completed requests */
   int64_t dir_bytes_peak; /* most direction bytes in flight at once */
   int32_t inflight_peak;  /* most batches in flight at once */
   int32_t pad;
+  double prep_seconds;    /* server thread: forming, planning and launching batches */
+  double retire_seconds;  /* server thread: turning finished batches into results */
+  double batch_seconds;   /* sum over batches of launch-to-results wall time */
+  double device_seconds;  /* sum over batches of stream time from upload to download */
 } gdp_server_stats;
 
 /* Starts a server over a device genome (GDP_GENOME_* format, genome_len
