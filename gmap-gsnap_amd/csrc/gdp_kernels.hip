@@ -206,29 +206,6 @@ __device__ __forceinline__ int score_at(const int16_t *scores, const gdp_dev_pro
 #define TB_TINDEL (-1)
 #define MICROINTRON_LENGTH 9 /* pairpool.c:18 */
 
-struct tb_out {
-  gdp_pair *pairs;
-  int n, cap;
-};
-
-// Pairpool_push (pairpool.c:179-259): pairs at negative coordinates are dropped.
-__device__ __forceinline__ void push_pair(tb_out &o, int qpos, int gpos, int cdna, int comp, int g, int ga) {
-  if (qpos < 0 || gpos < 0) return;
-  if (o.n < o.cap) {
-    gdp_pair pr;
-    pr.querypos = qpos;
-    pr.genomepos = gpos;
-    pr.cdna = (int8_t)cdna;
-    pr.comp = (int8_t)comp;
-    pr.genome = (int8_t)g;
-    pr.genomealt = (int8_t)ga;
-    pr.gapp = 0;
-    pr.genomejump = 0;
-    o.pairs[o.n] = pr;
-  }
-  o.n++;
-}
-
 // complCode (COMPLEMENT_LC, complement.h:30) stated as rules: nucleotide and
 // IUPAC letters go to their complements in the same case (U to A), the
 // bracket pairs () <> [] {} swap, control characters, DEL and the characters
@@ -318,29 +295,6 @@ __global__ __launch_bounds__(256) void gdp_segments_kernel(const void *__restric
   }
 }
 
-// 16-byte window of device memory kept in registers: a walk reads its
-// directions and characters backwards a few bytes at a time, so one aligned
-// 16-byte load serves several steps.  Buffers read this way need 16 bytes
-// of padding past their last used byte.
-#define WIN_DECL(W)                                                                             \
-  uintptr_t W##_b = ~(uintptr_t)0;                                                              \
-  uint32_t W##_0 = 0, W##_1 = 0, W##_2 = 0, W##_3 = 0
-// Word at addr through window W (one aligned 16-byte load when addr leaves it).
-#define WIN_WORD(W, addr, out)                                                                  \
-  do {                                                                                          \
-    const uintptr_t a_ = (uintptr_t)(addr), b_ = a_ & ~(uintptr_t)15;                           \
-    if (b_ != W##_b) {                                                                          \
-      const uint4 v_ = *(const uint4 *)b_;                                                      \
-      W##_0 = v_.x;                                                                             \
-      W##_1 = v_.y;                                                                             \
-      W##_2 = v_.z;                                                                             \
-      W##_3 = v_.w;                                                                             \
-      W##_b = b_;                                                                               \
-    }                                                                                           \
-    const int k_ = (int)(a_ - b_) >> 2;                                                         \
-    (out) = k_ < 2 ? (k_ == 0 ? W##_0 : W##_1) : (k_ == 2 ? W##_2 : W##_3);                     \
-  } while (0)
-
 // Direction lookup for the walks: the geometry held in registers, the
 // divisions of gdp_step / gdp_p_step turned into shifts.
 struct tb_dirs {
@@ -386,17 +340,24 @@ struct tb_dirs {
   }
 };
 
-#define WIN_BYTE(W, addr, out)                                                                  \
-  do {                                                                                          \
-    uint32_t w__;                                                                               \
-    WIN_WORD(W, addr, w__);                                                                     \
-    (out) = (int)(int8_t)(w__ >> (8 * ((uintptr_t)(addr) & 3)));                                \
-  } while (0)
+// ---- cooperative walk: 16 lanes per traceback ----------------------------
+//
+// The reference walks one cell at a time (dynprog_simd.c:9153-9945); every
+// step of a single-thread walk would wait on one dependent memory read.  A
+// group of TBG lanes instead evaluates TBG steps of the walk's current
+// straight stretch at once, each lane reading its own cell: in a diagonal
+// stretch lane j looks at (r - j, c - j) and, where the walk would pair the
+// cell, forms that pair; in a gap run lane j looks at the j-th cell of the
+// run.  A ballot finds where the stretch ends, all lanes advance the shared
+// state by that many steps, and the pairs are written in walk order
+// (prefix counts of the ballot).  The group's state (r, c, counters, pair
+// count) is identical in all its lanes, so control flow is uniform per group.
+#define TBG 16
 
-// One pair record (at most cap stored; n counts them all).
-#define TB_PUT(qpos, gpos, cdna_, comp_, g_, ga_, gapp_, jump_)                                 \
+// The pair list entry at index idx (stored when idx < cap).
+#define TB_STORE(idx, qpos, gpos, cdna_, comp_, g_, ga_, gapp_, jump_)                        \
   do {                                                                                          \
-    if (n < cap) {                                                                              \
+    if ((idx) < cap) {                                                                          \
       gdp_pair pr_;                                                                             \
       pr_.querypos = (qpos);                                                                    \
       pr_.genomepos = (gpos);                                                                   \
@@ -406,29 +367,15 @@ struct tb_dirs {
       pr_.genomealt = (int8_t)(ga_);                                                            \
       pr_.gapp = (int16_t)(gapp_);                                                              \
       pr_.genomejump = (int16_t)(jump_);                                                        \
-      obuf[n] = pr_;                                                                            \
+      obuf[(idx)] = pr_;                                                                        \
     }                                                                                           \
-    n++;                                                                                        \
-  } while (0)
-// Pairpool_push (pairpool.c:179-259): pairs at negative coordinates are dropped.
-#define TB_PUSH(qpos, gpos, cdna_, comp_, g_, ga_)                                              \
-  do {                                                                                          \
-    const int qp__ = (qpos), gp__ = (gpos);                                                     \
-    if (qp__ >= 0 && gp__ >= 0) TB_PUT(qp__, gp__, cdna_, comp_, g_, ga_, 0, 0);                \
   } while (0)
 
-// The walk visits one cell, or emits one pair of a pending gap, per
-// iteration, so the lanes of a wave run the same short body:
-//   DECIDE  diagonal pair, or the start of a gap run, at (r, c);
-//   RUN_H   one cell of a run to the left: full (dynprog_simd.c:9552-9711)
-//           follows Egap guarded by c > 0, upper (9715-9832) unguarded;
-//   RUN_V   one cell of a run upwards: full follows Fgap guarded by r > 0,
-//           lower (9835-9945) follows its Egap unguarded;
-//   EMIT_H  one pair of Pairpool_add_genomeskip (pairpool.c:1067-1180);
-//   EMIT_V  one pair of Pairpool_add_queryskip (pairpool.c:980-1010);
-//   TAIL    the leftover query or genome skip once r or c reaches 0.
-// Directions and characters are read through 16-byte register windows.
-enum { TB_DECIDE, TB_RUN_H, TB_RUN_V, TB_EMIT_H, TB_EMIT_V, TB_TAIL, TB_DONE };
+enum { TB_DECIDE, TB_RUN_H, TB_RUN_V, TB_TAIL, TB_DONE };
+
+__device__ __forceinline__ uint32_t tb_ballot(bool p, int g0) {
+  return (uint32_t)((__ballot(p) >> g0) & 0xFFFFull);
+}
 
 __global__ __launch_bounds__(256) void gdp_traceback_kernel(
     const gdp_dev_problem *__restrict__ probs, const gdp_tb_request *__restrict__ reqs, int nreq,
@@ -438,8 +385,10 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
   __shared__ uint32_t consb[3 * 128 * 4];
   for (int j = threadIdx.x; j < 3 * 128 * 4; j += blockDim.x) consb[j] = gdp_tabConsBits[j];
   __syncthreads();
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nreq) return;
+  const int lane = (int)(threadIdx.x & 63), g0 = lane & ~(TBG - 1), j = lane & (TBG - 1);
+  const uint32_t below = (1u << j) - 1u;  // group lanes before this one
+  const int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) / TBG);
+  if (k >= nreq) return;  // whole groups leave together
   const gdp_tb_request q = reqs[k];
   const int queryoffset = q.queryoffset, genomeoffset = q.genomeoffset, watsonp = q.watsonp, revp = q.revp;
   const uint32_t chroffset = q.chroffset, chrhigh = q.chrhigh;
@@ -453,7 +402,7 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
   const int kind = pp->kind;
   const int64_t p_qoff = pp->qoff, p_goff = pp->goff, p_galtoff = pp->galtoff;
   if (q.r < 0 || q.r > pp->rl || q.c < 0 || q.c > pp->gl) {  // start cell outside the matrix
-    cnt[0] = -1;
+    if (j == 0) cnt[0] = -1;
     return;
   }
   tb_dirs TD;
@@ -461,106 +410,109 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
   const bool lower = kind == GDP_KIND_LOWER;
   const uint32_t vbit = kind == GDP_KIND_FULL ? GDP_DIR_F : GDP_DIR_E;
 
-  WIN_DECL(wd);
-  WIN_DECL(wq);
-  WIN_DECL(wg);
-  WIN_DECL(wga);
   int n = 0, sc = 0, nm = 0, nmm = 0, nop = 0, nind = 0;
   int r = q.r, c = q.c;
-  int mode = TB_DECIDE, after = TB_DECIDE, dist = 0;
-  int eq = 0, eg = 0, estep = 0, left = 0;  // emission cursor
-  int sk_rr = 0, sk_cc = 0, sk_d = 0, sk_next = 0, sk_kind = 0;  // pending skip
+  int mode = TB_DECIDE, dist = 0;
 
-  for (;;) {
-    if (mode == TB_DONE) break;
-    sk_kind = 0;
+  while (mode != TB_DONE) {
+    int sk_kind = 0, sk_rr = 0, sk_cc = 0, sk_d = 0;
     if (mode == TB_DECIDE) {
       if (r <= 0 || c <= 0) {
         mode = TB_TAIL;
         continue;
       }
-      // characters of a diagonal pair, fetched together with the directions
-      int c1, c2, c2a, t;
-      bool ok;
-      WIN_BYTE(wq, qbuf + p_qoff + r - 1, c1);
-      WIN_BYTE(wg, gbuf + p_goff + c - 1, c2);
-      WIN_BYTE(wga, gbuf + p_galtoff + c - 1, c2a);
-      const uint32_t *a = lower ? TD.where(c, r, t, ok) : TD.where(r, c, t, ok);
-      uint32_t w;
-      WIN_WORD(wd, a, w);
-      const uint32_t d = nogaps ? 0u : TD.nib(w, t, ok);
-      int m;
-      if (kind == GDP_KIND_FULL) m = (d & GDP_DIR_V) ? TB_RUN_V : ((d & GDP_DIR_H) ? TB_RUN_H : TB_DECIDE);
-      else m = (d & (GDP_DIR_H | GDP_DIR_V)) ? (kind == GDP_KIND_UPPER ? TB_RUN_H : TB_RUN_V) : TB_DECIDE;
-      if (m == TB_DECIDE) {
-        if (c2 != '*') {
-          const int c1uc = (c1 >= 'a' && c1 <= 'z') ? c1 - 32 : c1;
-          const int row = (consrow + (c1uc & 127)) << 2;
-          int comp;
-          if (c1uc == c2 || c1uc == c2a) comp = '*';
-          else if (((consb[row + ((c2 & 127) >> 5)] >> (c2 & 31)) & 1) ||
-                   ((consb[row + ((c2a & 127) >> 5)] >> (c2a & 31)) & 1))
-            comp = ':';  // consistent_array (dynprog.c:893-900)
-          else comp = ' ';
-          if (comp == ' ') {
-            sc += TB_MISMATCH;
-            nmm += 1;
-          } else {
-            sc += TB_MATCH;
-            nm += 1;
-          }
-          if (ops) TB_PUT(r, c, 'D', comp, 0, 0, 0, 0);
-          else TB_PUSH(queryoffset + (revp ? 1 - r : r - 1), genomeoffset + (revp ? 1 - c : c - 1), c1, comp, c2, c2a);
+      // Lane j: the cell the walk reaches after j diagonal steps.
+      const int rr = r - j, cc = c - j;
+      const bool inm = rr > 0 && cc > 0;
+      int c1 = 0, c2 = 0, c2a = 0, m = TB_DECIDE;
+      if (inm) {
+        c1 = qbuf[p_qoff + rr - 1];
+        c2 = gbuf[p_goff + cc - 1];
+        c2a = gbuf[p_galtoff + cc - 1];
+        if (!nogaps) {
+          int t;
+          bool ok;
+          const uint32_t *a = lower ? TD.where(cc, rr, t, ok) : TD.where(rr, cc, t, ok);
+          const uint32_t d = TD.nib(*a, t, ok);
+          if (kind == GDP_KIND_FULL) m = (d & GDP_DIR_V) ? TB_RUN_V : ((d & GDP_DIR_H) ? TB_RUN_H : TB_DECIDE);
+          else m = (d & (GDP_DIR_H | GDP_DIR_V)) ? (kind == GDP_KIND_UPPER ? TB_RUN_H : TB_RUN_V) : TB_DECIDE;
         }
-        r--;
-        c--;
-      } else {
-        mode = m;
-        dist = 1;
       }
+      const uint32_t dm = tb_ballot(inm && m == TB_DECIDE, g0);
+      const int J = __builtin_ctz(~dm | 0x10000u);  // diagonal steps taken this round
+      // a diagonal pair (dynprog_simd.c:9245-9275); '*' columns past the
+      // chromosome end push nothing (dynprog_simd.c:9248)
+      const bool paired = j < J && c2 != '*';
+      int comp = ' ';
+      if (paired) {
+        const int c1uc = (c1 >= 'a' && c1 <= 'z') ? c1 - 32 : c1;
+        const int row = (consrow + (c1uc & 127)) << 2;
+        if (c1uc == c2 || c1uc == c2a) comp = '*';
+        else if (((consb[row + ((c2 & 127) >> 5)] >> (c2 & 31)) & 1) ||
+                 ((consb[row + ((c2a & 127) >> 5)] >> (c2a & 31)) & 1))
+          comp = ':';  // consistent_array (dynprog.c:893-900)
+      }
+      const uint32_t mm_ = tb_ballot(paired && comp == ' ', g0), ma = tb_ballot(paired && comp != ' ', g0);
+      nm += __popc(ma);
+      nmm += __popc(mm_);
+      sc += TB_MATCH * __popc(ma) + TB_MISMATCH * __popc(mm_);
+      const int qpos = queryoffset + (revp ? 1 - rr : rr - 1), gpos = genomeoffset + (revp ? 1 - cc : cc - 1);
+      // Pairpool_push drops pairs at negative coordinates (pairpool.c:189-191)
+      const bool emit = paired && (ops || (qpos >= 0 && gpos >= 0));
+      const uint32_t em = tb_ballot(emit, g0);
+      if (emit) {
+        const int idx = n + __popc(em & below);
+        if (ops) TB_STORE(idx, rr, cc, 'D', comp, 0, 0, 0, 0);
+        else TB_STORE(idx, qpos, gpos, c1, comp, c2, c2a, 0, 0);
+      }
+      n += __popc(em);
+      if (J < TBG) {
+        // the walk stops at lane J's cell: outside the matrix, or a gap starts
+        const int mJ = __shfl(m, g0 + J, 64);
+        r -= J;
+        c -= J;
+        if (r > 0 && c > 0) {
+          mode = mJ;
+          dist = 1;
+        }
+      } else {
+        r -= TBG;
+        c -= TBG;
+      }
+      continue;
     } else if (mode == TB_RUN_H || mode == TB_RUN_V) {
+      // Lane j: iteration j of the run loop (dynprog_simd.c:9230-9243):
+      // full follows Egap leftwards / Fgap upwards while c > 0 / r > 0,
+      // upper and lower follow their Egap unguarded.
       const bool h = mode == TB_RUN_H;
+      const int rr = h ? r : r - j, cc = h ? c - j : c;
+      const bool guard = kind != GDP_KIND_FULL || (h ? cc > 0 : rr > 0);
       bool more = false;
-      if (kind != GDP_KIND_FULL || (h ? c > 0 : r > 0)) {
+      if (guard) {
         int t;
         bool ok;
-        const uint32_t *a = lower ? TD.where(c, r, t, ok) : TD.where(r, c, t, ok);
-        uint32_t w;
-        WIN_WORD(wd, a, w);
-        more = (TD.nib(w, t, ok) & (h ? GDP_DIR_E : vbit)) != 0;
-        if (h) c--;
-        else r--;
+        const uint32_t *a = lower ? TD.where(cc, rr, t, ok) : TD.where(rr, cc, t, ok);
+        more = (TD.nib(*a, t, ok) & (h ? GDP_DIR_E : vbit)) != 0;
       }
-      if (more) {
-        dist++;
-      } else {
-        sk_kind = h ? 1 : 2;
-        sk_rr = h ? r : r + dist;
-        sk_cc = h ? c + dist : c;
-        sk_d = dist;
-        sk_next = TB_DECIDE;
+      const uint32_t mr = tb_ballot(more, g0);
+      const int K = __builtin_ctz(~mr | 0x10000u);
+      if (K == TBG) {
+        dist += TBG;
+        if (h) c -= TBG;
+        else r -= TBG;
+        continue;
       }
-    } else if (mode == TB_EMIT_H) {
-      // get_genomic_nt (pairpool.c:1016-1063) over the device genome
-      const int gpos = genomeoffset + eg;
-      const uint32_t pos = watsonp ? chroffset + (uint32_t)gpos : chrhigh - (uint32_t)gpos;
-      int ch;
-      if (pos < chroffset || pos >= chrhigh) ch = '*';
-      else {
-        ch = genome_at(genome, gfmt, glen, pos);
-        if (!watsonp) ch = compl_char(ch);
-      }
-      TB_PUSH(queryoffset + eq, gpos, ' ', '-', ch, ch);
-      eg += estep;
-      if (--left == 0) mode = after;
-    } else if (mode == TB_EMIT_V) {
-      // rsequence[eq] is q(|eq| + 1) in sweep order for both orientations
-      int ch;
-      WIN_BYTE(wq, qbuf + p_qoff + (eq < 0 ? -eq : eq), ch);
-      TB_PUSH(queryoffset + eq, genomeoffset + eg, ch, '-', ' ', ' ');
-      eq += estep;
-      if (--left == 0) mode = after;
-    } else {  // TB_TAIL
+      const int gK = __shfl((int)guard, g0 + K, 64);
+      const int dec = K + gK;
+      if (h) c -= dec;
+      else r -= dec;
+      dist += K;
+      sk_kind = h ? 1 : 2;
+      sk_rr = h ? r : r + dist;
+      sk_cc = h ? c + dist : c;
+      sk_d = dist;
+      mode = TB_DECIDE;
+    } else {  // TB_TAIL: the leftover skip once r or c reaches 0
       mode = TB_DONE;
       if (nogaps) {
         // traceback_nogaps (dynprog_end.c:648-722) stops without a skip
@@ -576,11 +528,11 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
       } else {
         if (r != 0) sk_kind = 2, sk_rr = r, sk_cc = 1, sk_d = r;
       }
-      sk_next = TB_DONE;
     }
     if (sk_kind == 1) {
-      // Pairpool_add_genomeskip with genomesequence NULL: a gap of 9 or more
-      // (MICROINTRON_LENGTH) becomes one gap holder and is not scored.
+      // Pairpool_add_genomeskip (pairpool.c:1067-1180) with genomesequence
+      // NULL: a gap of 9 or more (MICROINTRON_LENGTH) becomes one gap holder
+      // and is not scored; a shorter one pushes its pairs, lane j the j-th.
       int qc = sk_rr - 1, lo = sk_cc - sk_d, hi = sk_cc - 1, stp = -1;
       if (revp) {
         const int tmp = lo;
@@ -589,25 +541,37 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
         hi = -tmp;
         stp = +1;
       }
-      mode = sk_next;
       if (sk_d < MICROINTRON_LENGTH) {
         sc += TB_TOPEN + sk_d * TB_TINDEL;
         nop += 1;
         nind += sk_d;
       }
-      if (ops) {
-        TB_PUT(sk_rr, sk_cc, 'H', 0, 0, 0, 0, sk_d);
-      } else if (sk_d >= MICROINTRON_LENGTH) {
-        TB_PUT(-1, -1, ' ', ' ', ' ', ' ', 1, sk_d);  // Pairpool_push_gapholder (pairpool.c:374-452)
+      if (ops || sk_d >= MICROINTRON_LENGTH) {
+        if (j == 0) {
+          if (ops) TB_STORE(n, sk_rr, sk_cc, 'H', 0, 0, 0, 0, sk_d);
+          else TB_STORE(n, -1, -1, ' ', ' ', ' ', ' ', 1, sk_d);  // Pairpool_push_gapholder (pairpool.c:374-452)
+        }
+        n += 1;
       } else if (sk_d > 0) {
-        eq = qc;
-        eg = revp ? lo : hi;
-        estep = stp;
-        left = sk_d;
-        after = sk_next;
-        mode = TB_EMIT_H;
+        const int gpos = genomeoffset + (revp ? lo : hi) + j * stp;
+        bool emit = false;
+        int ch = 0;
+        if (j < sk_d) {
+          // get_genomic_nt (pairpool.c:1016-1063) over the device genome
+          const uint32_t pos = watsonp ? chroffset + (uint32_t)gpos : chrhigh - (uint32_t)gpos;
+          if (pos < chroffset || pos >= chrhigh) ch = '*';
+          else {
+            ch = genome_at(genome, gfmt, glen, pos);
+            if (!watsonp) ch = compl_char(ch);
+          }
+          emit = queryoffset + qc >= 0 && gpos >= 0;
+        }
+        const uint32_t em = tb_ballot(emit, g0);
+        if (emit) TB_STORE(n + __popc(em & below), queryoffset + qc, gpos, ' ', '-', ch, ch, 0, 0);
+        n += __popc(em);
       }
     } else if (sk_kind == 2) {
+      // Pairpool_add_queryskip (pairpool.c:980-1010), lanes j, j + TBG, ...
       int qc = sk_rr - 1, gc = sk_cc - 1, stp = -1;
       if (revp) {
         qc = -qc;
@@ -617,25 +581,34 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
       sc += TB_QOPEN + sk_d * TB_QINDEL;
       nop += 1;
       nind += sk_d;
-      mode = sk_next;
       if (ops) {
-        TB_PUT(sk_rr, sk_cc, 'V', 0, 0, 0, 0, sk_d);
-      } else if (sk_d > 0) {
-        eq = qc;
-        eg = gc;
-        estep = stp;
-        left = sk_d;
-        after = sk_next;
-        mode = TB_EMIT_V;
+        if (j == 0) TB_STORE(n, sk_rr, sk_cc, 'V', 0, 0, 0, 0, sk_d);
+        n += 1;
+      } else {
+        for (int base = 0; base < sk_d; base += TBG) {
+          const int eq = qc + (base + j) * stp;
+          bool emit = false;
+          int ch = 0;
+          if (base + j < sk_d) {
+            // rsequence[eq] is q(|eq| + 1) in sweep order for both orientations
+            ch = qbuf[p_qoff + (eq < 0 ? -eq : eq)];
+            emit = queryoffset + eq >= 0 && genomeoffset + gc >= 0;
+          }
+          const uint32_t em = tb_ballot(emit, g0);
+          if (emit) TB_STORE(n + __popc(em & below), queryoffset + eq, genomeoffset + gc, ch, '-', ' ', ' ', 0, 0);
+          n += __popc(em);
+        }
       }
     }
   }
-  cnt[0] = n;
-  cnt[1] = sc;
-  cnt[2] = nm;
-  cnt[3] = nmm;
-  cnt[4] = nop;
-  cnt[5] = nind;
+  if (j == 0) {
+    cnt[0] = n;
+    cnt[1] = sc;
+    cnt[2] = nm;
+    cnt[3] = nmm;
+    cnt[4] = nop;
+    cnt[5] = nind;
+  }
 }
 
 // ------------------------------------------------------------ best end cell
@@ -778,7 +751,7 @@ extern "C" int gdp_launch_traceback(const void *d_descs, const gdp_tb_request *d
                                     const char *d_gbuf, const uint32_t *d_dirs, const void *d_genome, int gfmt,
                                     uint32_t glen, gdp_pair *d_pairs, int32_t *d_counts, hipStream_t stream) {
   if (nreq <= 0) return 0;
-  hipLaunchKernelGGL(gdp_traceback_kernel, dim3((nreq + 255) / 256), dim3(256), 0, stream,
+  hipLaunchKernelGGL(gdp_traceback_kernel, dim3((nreq + 256 / TBG - 1) / (256 / TBG)), dim3(256), 0, stream,
                      (const gdp_dev_problem *)d_descs, d_reqs, nreq, d_qbuf, d_gbuf, d_dirs, d_genome, gfmt, glen,
                      d_pairs, d_counts);
   return hipGetLastError() == hipSuccess ? 0 : -1;
