@@ -40,6 +40,8 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     "c2": (2, 1_000_000, "single-gap rl=150 gl=U{160..200} extraband=U{5..20} int16, 1e6 problems"),
     "c3": (3, 2_000_000, "single-gap rl=logU[20,1000] gl=rl*U(1,1.5) extraband=U{3..64} mixed int8/int16, 2e6 problems"),
+    "c4": (4, 2_000_000, "end extensions rl=U{10..150} gl=rl+U{0..20} ENDQ (upper+lower fill, best end cell, "
+                         "traceback) mixed 1:1 with single-gap logU[20,1000] problems, 2e6 + 2e6"),
 }
 
 
@@ -180,28 +182,51 @@ def main():
     config, n_default, desc = WORKLOADS[args.workload]
     n = args.n or n_default
     t0 = time.time()
-    wl = Workload(config, n)
+    if config == 4:
+        from gmapdp.workload import EndMix
+        wl = EndMix(n, n)
+    else:
+        wl = Workload(config, n)
     probs = wl.problems
     total_cells = batch_cells(probs)
-    log("workload %s: %d problems, %.3e cells, generated in %.1fs" % (args.workload, n, total_cells, time.time() - t0))
+    log("workload %s: %d problems, %.3e cells, generated in %.1fs" % (args.workload, len(probs), total_cells,
+                                                                      time.time() - t0))
 
     tuning = None
     params = None
     if not args.no_tune:
         from gmapdp.tune import autotune
         t1 = time.time()
-        params, table = autotune(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes(), sample=min(n, 200_000))
-        tuning = {"objective": "throughput", "sample": min(n, 200_000), "chosen": list(params),
+        params, table = autotune(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes(), sample=min(len(probs), 200_000))
+        tuning = {"objective": "throughput", "sample": min(len(probs), 200_000), "chosen": list(params),
                   "candidates": [[list(p), round(ms, 3)] for p, ms in table], "seconds": round(time.time() - t1, 1)}
         log("autotuner: g16_max_span=%d g64_min_rows=%d (%.1fs)" % (params[0], params[1], time.time() - t1))
     b = DeviceBatch(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes(), params=params)
-    reqs = wl.tb_requests()
     stream = torch.cuda.current_stream()
     graph = None
+    if config == 4:
+        from gmapdp.lib import TBREQ_DTYPE
+        reqs = np.concatenate([wl.single_requests(), wl.end_templates()])
+        off = np.zeros(len(reqs), np.int64)
+        off[1:] = np.cumsum(reqs["max_pairs"].astype(np.int64))[:-1]
+        reqs["pair_off"] = off
+        d_ul = torch.from_numpy(wl.ul.reshape(-1).copy()).to(b.dev)
+        d_tmpl = torch.from_numpy(reqs[wl.n3:].view(np.uint8).copy()).to(b.dev)
+        d_best = torch.zeros(3 * wl.m, dtype=torch.int32, device=b.dev)
+    else:
+        reqs = wl.tb_requests()
+    reqs_dev = b.prepare_traceback(reqs, genome=wl.genome())
+
+    def enqueue_rest(st=None):
+        # everything after the fill: end cells (config 4), then the tracebacks
+        if config == 4:
+            b.best_endpoint_requests(d_ul, wl.m, False, d_best, d_tmpl.data_ptr(),
+                                     reqs_dev["d_reqs"].data_ptr() + wl.n3 * TBREQ_DTYPE.itemsize, stream=st)
+        b.traceback_device(reqs_dev, stream=st)
 
     def step(ev=None, tev=None):
         if graph is not None:
-            # fill + traceback replayed from one captured graph; the fill and
+            # the whole step replayed from one captured graph; the fill and
             # traceback split is measured on the eager launches below
             b.graph_launch(graph)
             return
@@ -210,11 +235,10 @@ def main():
         b.fill()
         if ev is not None:
             ev[1].record(stream)
-        b.traceback_device(reqs_dev)
+        enqueue_rest()
         if tev is not None:
             tev.record(stream)
 
-    reqs_dev = b.prepare_traceback(reqs, genome=wl.genome())
     if args.calib_copy:
         src = torch.ones(args.calib_copy << 18, dtype=torch.int32, device=b.dev)
         dst = torch.empty_like(src)
@@ -234,7 +258,7 @@ def main():
     fill_ms = float(np.mean([a.elapsed_time(z) for a, z in evs]))
     tb_ms = float(np.mean([z.elapsed_time(t) for (a, z), t in zip(evs, tevs)]))
     if not args.no_graph:
-        graph = b.graph(reqs_dev)
+        graph = b.capture(lambda st: (b.fill(stream=st), enqueue_rest(st)))
         for _ in range(max(1, args.warmup)):
             step()
         torch.cuda.synchronize()
@@ -254,7 +278,9 @@ def main():
     if args.percall:
         m = min(args.percall, n)
         sub = DeviceBatch(probs[:m], wl.qbuf.tobytes(), wl.gbuf.tobytes(), single=True)  # one sweep per problem
-        sreq = sub.prepare_traceback(wl.tb_requests()[:m], genome=wl.genome())
+        # (config 4: its single-gap half, problems[:n3])
+        sreq = sub.prepare_traceback((wl.single_requests() if config == 4 else wl.tb_requests())[:m],
+                                     genome=wl.genome())
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for i in range(m):
@@ -297,7 +323,8 @@ def main():
         "vs_baseline": None,
         "dtype": "int16",
         "data": "synthetic (seeded generator of the workload)",
-        "config": {"workload": desc, "problems": n, "cells": total_cells, "step": "fill + traceback",
+        "config": {"workload": desc, "problems": len(probs), "cells": total_cells,
+                   "step": "fill + traceback" if config != 4 else "fill + best end cell + traceback",
                    "launch": "eager" if graph is None else "one HIP graph per step",
                    "lane_groups": None if params is None else {"g16_max_span": params[0],
                                                                  "g64_min_rows": params[1]}},

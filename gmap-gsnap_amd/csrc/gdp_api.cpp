@@ -22,7 +22,7 @@ extern "C" int gdp_launch_traceback(const void *, const gdp_tb_request *, int, c
                                     const uint32_t *, const void *, int, uint32_t, gdp_pair *, int32_t *, hipStream_t);
 extern "C" int gdp_launch_segments(const void *, int, uint32_t, const gdp_segment *, int, char *, hipStream_t);
 extern "C" int gdp_launch_best_endpoint(const void *, const int32_t *, int, int, const int16_t *, int32_t *,
-                                        hipStream_t);
+                                        const gdp_tb_request *, gdp_tb_request *, hipStream_t);
 extern "C" int gdp_launch_export(const void *, int, const uint32_t *, const int16_t *, uint8_t *, int16_t *,
                                  hipStream_t);
 extern "C" int gdp_launch_export_ref(const void *, int, int, const uint32_t *, const int16_t *, int, void *, void *,
@@ -339,14 +339,21 @@ extern "C" int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, cons
   return gdp_fill_ex(d_descs, n, GDP_FILL_ALL, d_qbuf, d_gbuf, d_dirs, d_scores, d_final, stream);
 }
 
-extern "C" int gdp_best_endpoint(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,
-                                 const int16_t *d_scores, int32_t *d_out, void *stream) {
+extern "C" int gdp_best_endpoint_ex(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,
+                                    const int16_t *d_scores, int32_t *d_out, const gdp_tb_request *d_tb_template,
+                                    gdp_tb_request *d_tb_out, void *stream) {
   if (!g_init) return fail(GDP_ENOINIT, "gdp_best_endpoint: gdp_init has not been called");
-  if (m < 0 || (m > 0 && (!d_descs || !d_ul || !d_scores || !d_out)))
+  if (m < 0 || (m > 0 && (!d_descs || !d_ul || !d_scores || !d_out)) || ((d_tb_template == nullptr) != (d_tb_out == nullptr)))
     return fail(GDP_EINVAL, "gdp_best_endpoint: null argument");
-  if (gdp_launch_best_endpoint(d_descs, d_ul, m, queryend_indels, d_scores, d_out, (hipStream_t)stream))
+  if (gdp_launch_best_endpoint(d_descs, d_ul, m, queryend_indels, d_scores, d_out, d_tb_template, d_tb_out,
+                               (hipStream_t)stream))
     return hip_fail("gdp_best_endpoint launch", hipGetLastError());
   return GDP_OK;
+}
+
+extern "C" int gdp_best_endpoint(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,
+                                 const int16_t *d_scores, int32_t *d_out, void *stream) {
+  return gdp_best_endpoint_ex(d_descs, d_ul, m, queryend_indels, d_scores, d_out, nullptr, nullptr, stream);
 }
 
 extern "C" int gdp_traceback_ex(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq, const char *d_qbuf,
@@ -407,9 +414,36 @@ extern "C" int gdp_export_reference(const void *d_descs, int32_t idx, const uint
 }
 
 struct gdp_graph {
+  hipStream_t stream = nullptr;  // the capture stream
+  bool capturing = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
+
+extern "C" int gdp_graph_begin(gdp_graph **out, void **stream) {
+  if (!out || !stream) return fail(GDP_EINVAL, "gdp_graph_begin: null argument");
+  *out = nullptr;
+  *stream = nullptr;
+  gdp_graph *g = new gdp_graph;
+  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    gdp_graph_destroy(g);
+    return hip_fail("gdp_graph_begin", e);
+  }
+  g->capturing = true;
+  *out = g;
+  *stream = g->stream;
+  return GDP_OK;
+}
+
+extern "C" int gdp_graph_end(gdp_graph *g) {
+  if (!g || !g->capturing) return fail(GDP_EINVAL, "gdp_graph_end: no capture in progress");
+  g->capturing = false;
+  hipError_t e = hipStreamEndCapture(g->stream, &g->graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  return e == hipSuccess ? GDP_OK : hip_fail("gdp_graph_end", e);
+}
 
 extern "C" gdp_graph *gdp_graph_create(const void *d_descs, int32_t n, uint32_t kernels, const char *d_qbuf,
                                        const char *d_gbuf, uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final,
@@ -420,29 +454,17 @@ extern "C" gdp_graph *gdp_graph_create(const void *d_descs, int32_t n, uint32_t 
     fail(GDP_ENOINIT, "gdp_graph_create: gdp_init has not been called");
     return nullptr;
   }
-  hipStream_t st = nullptr;
-  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    hip_fail("gdp_graph_create stream", e);
-    return nullptr;
-  }
-  gdp_graph *g = new gdp_graph;
-  e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-  int rc = e == hipSuccess ? GDP_OK : hip_fail("gdp_graph_create capture", e);
-  if (rc == GDP_OK) rc = gdp_fill_ex(d_descs, n, kernels, d_qbuf, d_gbuf, d_dirs, d_scores, d_final, st);
+  gdp_graph *g = nullptr;
+  void *st = nullptr;
+  if (gdp_graph_begin(&g, &st) != GDP_OK) return nullptr;
+  int rc = gdp_fill_ex(d_descs, n, kernels, d_qbuf, d_gbuf, d_dirs, d_scores, d_final, st);
   if (rc == GDP_OK && nreq > 0)
     rc = gdp_traceback_ex(d_descs, d_reqs, nreq, d_qbuf, d_gbuf, d_dirs, d_genome, genome_format, genome_len,
                           d_pairs, d_counts, st);
-  if (e == hipSuccess) {
-    const hipError_t e2 = hipStreamEndCapture(st, &g->graph);
-    if (rc == GDP_OK && e2 != hipSuccess) rc = hip_fail("gdp_graph_create end capture", e2);
-  }
-  if (rc == GDP_OK) {
-    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) rc = hip_fail("gdp_graph_create instantiate", e);
-  }
-  (void)hipStreamDestroy(st);
-  if (rc != GDP_OK) {
+  std::string err = rc == GDP_OK ? "" : g_err;
+  const int rc2 = gdp_graph_end(g);
+  if (rc != GDP_OK || rc2 != GDP_OK) {
+    if (rc != GDP_OK) g_err = err;
     gdp_graph_destroy(g);
     return nullptr;
   }
@@ -457,6 +479,11 @@ extern "C" int gdp_graph_launch(gdp_graph *g, void *stream) {
 
 extern "C" void gdp_graph_destroy(gdp_graph *g) {
   if (!g) return;
+  if (g->capturing) {
+    hipGraph_t tmp = nullptr;
+    if (hipStreamEndCapture(g->stream, &tmp) == hipSuccess && tmp) (void)hipGraphDestroy(tmp);
+  }
+  if (g->stream) (void)hipStreamDestroy(g->stream);
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;

@@ -613,22 +613,23 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
 
 // ------------------------------------------------------------ best end cell
 
-__global__ void gdp_best_endpoint_kernel(const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ ul, int m,
-                                         int queryend_indels, const int16_t *__restrict__ scores,
-                                         int32_t *__restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
+// find_best_endpoint_{8,16} / find_best_endpoint_to_queryend_indels_{8,16}
+// (dynprog_end.c:143-290, 358-508) over the kept scores of pair k.
+__device__ void gdp_best_cell(const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ ul, int k,
+                              int queryend_indels, const int16_t *__restrict__ scores, int &best, int &bestr,
+                              int &bestc) {
   const gdp_dev_problem pu = probs[ul[2 * k]], pl = probs[ul[2 * k + 1]];
   const gdp_geom Gu = gdp_make_geom(pu), Gl = gdp_make_geom(pl);
   if (pu.score_off < 0 || pl.score_off < 0 || pu.kind != GDP_KIND_UPPER || pl.kind != GDP_KIND_LOWER) {
-    out[3 * k] = INT_MIN;  // scores were not kept, or the pair is not (upper, lower)
-    out[3 * k + 1] = -1;
-    out[3 * k + 2] = -1;
+    best = INT_MIN;  // scores were not kept, or the pair is not (upper, lower)
+    bestr = -1;
+    bestc = -1;
     return;
   }
   const int rl = pu.rl, gl = pu.gl, lband = pl.lband, uband = pu.uband, late = pu.jump_late;
-  int best = queryend_indels ? (pu.width == 8 ? -128 : -32768) : 0;
-  int bestr = queryend_indels ? rl : 0, bestc = 0;
+  best = queryend_indels ? (pu.width == 8 ? -128 : -32768) : 0;
+  bestr = queryend_indels ? rl : 0;
+  bestc = 0;
   for (int r = queryend_indels ? rl : 1; r <= rl; r++) {
     int clo = gdp_imax(1, r - lband), chigh = gdp_imin(gl, r + uband), c;
     for (c = clo; c < r; c++) {
@@ -648,9 +649,28 @@ __global__ void gdp_best_endpoint_kernel(const gdp_dev_problem *__restrict__ pro
       }
     }
   }
+}
+
+__global__ void gdp_best_endpoint_kernel(const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ ul, int m,
+                                         int queryend_indels, const int16_t *__restrict__ scores,
+                                         int32_t *__restrict__ out, const gdp_tb_request *__restrict__ tmpl,
+                                         gdp_tb_request *__restrict__ tbout) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  int best, bestr, bestc;
+  gdp_best_cell(probs, ul, k, queryend_indels, scores, best, bestr, bestc);
   out[3 * k] = best;
   out[3 * k + 1] = bestr;
   out[3 * k + 2] = bestc;
+  if (tbout != nullptr) {
+    // the walk from the best cell runs in the triangle that holds it
+    // (dynprog_end.c:1576-1607): upper when bestc >= bestr, lower otherwise
+    gdp_tb_request q = tmpl[k];
+    q.fill = bestc >= bestr ? ul[2 * k] : ul[2 * k + 1];
+    q.r = bestr;
+    q.c = bestc;
+    tbout[k] = q;
+  }
 }
 
 // ------------------------------------------------------------ dense export
@@ -766,10 +786,11 @@ extern "C" int gdp_launch_segments(const void *d_genome, int fmt, uint32_t glen,
 }
 
 extern "C" int gdp_launch_best_endpoint(const void *d_descs, const int32_t *d_ul, int m, int queryend_indels,
-                                        const int16_t *d_scores, int32_t *d_out, hipStream_t stream) {
+                                        const int16_t *d_scores, int32_t *d_out, const gdp_tb_request *d_tmpl,
+                                        gdp_tb_request *d_tbout, hipStream_t stream) {
   if (m <= 0) return 0;
   hipLaunchKernelGGL(gdp_best_endpoint_kernel, dim3((m + 127) / 128), dim3(128), 0, stream,
-                     (const gdp_dev_problem *)d_descs, d_ul, m, queryend_indels, d_scores, d_out);
+                     (const gdp_dev_problem *)d_descs, d_ul, m, queryend_indels, d_scores, d_out, d_tmpl, d_tbout);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -37,8 +37,25 @@ struct Req {
   int16_t lband, uband;
   int8_t width;
   int8_t status;        // 0, or the negated GDP_E* code the request is refused with
-  std::string read;
+  int64_t roff;         // the read's bytes in the read ring (virtual offset)
   double t_submit;
+};
+
+// Read bytes of pending requests, copied in at submit and out at launch, in
+// arrival order.  Offsets grow without bound; offset v lives at v % size,
+// and a read never wraps (the ring skips to the next lap instead).  Bytes
+// below `freed` may be overwritten.
+struct ReadRing {
+  std::vector<char> buf;
+  int64_t next = 0, freed = 0;
+  int64_t place(int32_t len) const {  // offset a read of len bytes would take
+    const int64_t size = (int64_t)buf.size();
+    int64_t v = next;
+    if (v % size + len > size) v += size - v % size;
+    return v;
+  }
+  bool fits(int64_t v, int32_t len) const { return v + len - freed <= (int64_t)buf.size(); }
+  char *at(int64_t v) { return buf.data() + v % (int64_t)buf.size(); }
 };
 
 struct DoneBatch {
@@ -125,6 +142,8 @@ struct gdp_server {
   std::mutex mu;
   std::condition_variable cv_work;
   std::deque<Req> pending;
+  ReadRing ring;
+  std::condition_variable cv_space;
 
   std::mutex dmu;
   std::condition_variable cv_done;
@@ -322,7 +341,7 @@ bool gdp_server::launch(Slot &s) {
     }
     sg.length = q.r.gl;
     sg.out_off = gpos;
-    memcpy(qh + qpos, q.read.data(), (size_t)q.r.rl);
+    memcpy(qh + qpos, ring.at(q.roff), (size_t)q.r.rl);
     gdp_tb_request &t = tbs[i];
     memset(&t, 0, sizeof t);
     t.fill = i;
@@ -342,6 +361,12 @@ bool gdp_server::launch(Slot &s) {
     ppos += t.max_pairs;
   }
   memset(qh + qpos, 0, 16);
+  {
+    // the launched reads are the oldest; their ring space is free again
+    std::lock_guard<std::mutex> lk(mu);
+    ring.freed = pending.empty() ? ring.next : pending.front().roff;
+  }
+  cv_space.notify_all();
   const int64_t up_bytes = off_q + qpos + 16;
   const uint32_t kernels = gdp_plan_kernels(s.h_up, n);
 
@@ -501,6 +526,7 @@ extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const voi
               nb * (int64_t)sizeof(gdp_tb_request) + s->q_cap;
   s->down_cap = align_up(nb * 4, 16) + align_up(nb * 24, 16) + cfg->pair_capacity * (int64_t)sizeof(gdp_pair);
   s->arena.init(align_up(cfg->dir_budget_bytes, 256) - 256);
+  s->ring.buf.resize((size_t)std::max<int64_t>(64 << 20, 4 * nb * (int64_t)cfg->max_rl));
   hipError_t e = hipMalloc((void **)&s->d_dirs, (size_t)align_up(cfg->dir_budget_bytes, 256) + 64);
   s->slots.resize(cfg->slots);
   for (Slot &sl : s->slots) {
@@ -538,15 +564,17 @@ extern "C" int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, i
     return GDP_EINVAL;
   }
   const double t = now_s();
-  std::vector<Req> in(n);
-  std::vector<Req> refused_now;
+  DoneBatch b;
+  int32_t accepted = 0;
+  std::unique_lock<std::mutex> lk(s->mu);
   for (int32_t i = 0; i < n; i++) {
     const gdp_align_request &r = reqs[i];
-    Req &q = in[i];
+    Req q;
     q.r = r;
     q.r.read = nullptr;
     q.t_submit = t;
     q.status = 0;
+    q.roff = 0;
     int32_t lb = 0, ub = 0;
     if (r.rl <= 0 || r.gl <= 0 || r.rl > s->cfg.max_rl || r.gl > s->cfg.max_gl) {
       q.status = (int8_t)-GDP_ETOOBIG;
@@ -556,49 +584,73 @@ extern "C" int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, i
           r.extend > 0 || r.read == nullptr)
         q.status = (int8_t)-GDP_EINVAL;
     }
-    if (q.status) continue;
+    if (q.status) {
+      b.res.push_back(refused(q, t));
+      continue;
+    }
     q.lband = (int16_t)lb;
     q.uband = (int16_t)ub;
     // Dynprog_single_gap's kernel choice (dynprog_single.c:595).
     const int u8 = gdp_use8p_size(r.mismatchtype);
     q.width = (r.rl < u8 && r.gl < u8) ? 8 : 16;
-    q.read.assign(r.read, (size_t)r.rl);
-  }
-  DoneBatch b;
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    for (Req &q : in) {
-      if (q.status) b.res.push_back(refused(q, t));
-      else s->pending.push_back(std::move(q));
+    int64_t v = s->ring.place(r.rl);
+    while (!s->ring.fits(v, r.rl)) {  // the ring is full of waiting reads
+      s->cv_work.notify_one();
+      s->cv_space.wait_for(lk, std::chrono::milliseconds(1));
+      v = s->ring.place(r.rl);
     }
+    memcpy(s->ring.at(v), r.read, (size_t)r.rl);
+    s->ring.next = v + r.rl;
+    q.roff = v;
+    s->pending.push_back(q);
+    accepted++;
   }
+  lk.unlock();
   {
-    std::lock_guard<std::mutex> lk(s->smu);
+    std::lock_guard<std::mutex> lk2(s->smu);
     s->st.submitted += n;
     s->st.completed += (int64_t)b.res.size();
   }
   if (!b.res.empty()) s->post(std::move(b));
-  s->cv_work.notify_one();
+  if (accepted) s->cv_work.notify_one();
   return GDP_OK;
 }
 
 extern "C" int32_t gdp_server_poll(gdp_server *s, gdp_align_result *out, int32_t max, int32_t wait_us) {
   if (!s || max < 0 || (max > 0 && !out)) return GDP_EINVAL;
-  std::unique_lock<std::mutex> lk(s->dmu);
-  // results of the previous call are no longer referenced
-  for (int k : s->handed) s->rfree.push_back(k);
-  s->handed.clear();
-  if (s->done.empty() && wait_us > 0)
-    s->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [&] { return !s->done.empty(); });
-  int32_t k = 0;
-  while (k < max && !s->done.empty()) {
-    DoneBatch &b = s->done.front();
-    while (k < max && b.next < b.res.size()) out[k++] = b.res[b.next++];
-    if (b.next < b.res.size()) break;
-    if (b.buf >= 0) s->handed.push_back(b.buf);
-    s->done.pop_front();
+  std::vector<DoneBatch> took;
+  {
+    std::unique_lock<std::mutex> lk(s->dmu);
+    // results of the previous call are no longer referenced
+    for (int k : s->handed) s->rfree.push_back(k);
+    s->handed.clear();
+    if (s->done.empty() && wait_us > 0)
+      s->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [&] { return !s->done.empty(); });
+    int64_t have = 0;
+    while (have < max && !s->done.empty()) {
+      have += (int64_t)(s->done.front().res.size() - s->done.front().next);
+      took.push_back(std::move(s->done.front()));
+      s->done.pop_front();
+    }
   }
-  // a batch returned in part keeps its buffer until a poll after its last result
+  // copy outside the lock, so the server thread can post meanwhile
+  int32_t k = 0;
+  for (DoneBatch &b : took) {
+    const size_t m = std::min<size_t>((size_t)(max - k), b.res.size() - b.next);
+    if (m) memcpy(out + k, b.res.data() + b.next, m * sizeof(gdp_align_result));
+    k += (int32_t)m;
+    b.next += m;
+  }
+  std::lock_guard<std::mutex> lk(s->dmu);
+  for (size_t i = 0; i < took.size(); i++) {
+    DoneBatch &b = took[i];
+    if (b.next < b.res.size()) {
+      // only the last batch taken can be left in part: it goes back first
+      s->done.push_front(std::move(b));
+    } else if (b.buf >= 0) {
+      s->handed.push_back(b.buf);
+    }
+  }
   return k;
 }
 

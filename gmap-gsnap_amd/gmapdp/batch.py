@@ -97,11 +97,11 @@ class DeviceBatch:
     def stream(self):
         return ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
 
-    def fill(self):
+    def fill(self, stream=None):
         t = self
         _check(t.L, t.L.gdp_fill_ex(t.d_descs.data_ptr(), t.n, t.kernels, t.d_q.data_ptr(), t.d_g.data_ptr(),
-                                    t.d_dirs.data_ptr(), t.d_scores.data_ptr(), t.d_final.data_ptr(), t.stream),
-               "gdp_fill")
+                                    t.d_dirs.data_ptr(), t.d_scores.data_ptr(), t.d_final.data_ptr(),
+                                    t.stream if stream is None else stream), "gdp_fill")
 
     def final_scores(self):
         return self.d_final[: self.n].cpu().numpy()
@@ -178,14 +178,39 @@ class DeviceBatch:
             h["glen"] = len(genome)
         return h
 
-    def traceback_device(self, h, first=0, count=None):
+    def traceback_device(self, h, first=0, count=None, stream=None):
         count = h["n"] - first if count is None else count
         gptr = h["d_genome"].data_ptr() if h["d_genome"] is not None else None
         _check(self.L, self.L.gdp_traceback(self.d_descs.data_ptr(),
                                             h["d_reqs"].data_ptr() + first * TBREQ_DTYPE.itemsize, count,
                                             self.d_q.data_ptr(), self.d_g.data_ptr(), self.d_dirs.data_ptr(),
                                             gptr, h["glen"], h["d_pairs"].data_ptr(),
-                                            h["d_counts"].data_ptr() + first * 24, self.stream), "gdp_traceback")
+                                            h["d_counts"].data_ptr() + first * 24,
+                                            self.stream if stream is None else stream), "gdp_traceback")
+
+    def best_endpoint_requests(self, d_ul, m, queryend_indels, d_out, d_tmpl, d_tbout, stream=None):
+        """gdp_best_endpoint_ex on device buffers: best cells into d_out and
+        traceback requests (templates at address d_tmpl completed) to
+        address d_tbout."""
+        _check(self.L, self.L.gdp_best_endpoint_ex(self.d_descs.data_ptr(), d_ul.data_ptr(), m, int(queryend_indels),
+                                                   self.d_scores.data_ptr(), d_out.data_ptr(), d_tmpl, d_tbout,
+                                                   self.stream if stream is None else stream), "gdp_best_endpoint")
+
+    def capture(self, fn):
+        """Records fn(stream) into one HIP graph (gdp_graph_begin/end)."""
+        g, st = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(self.L, self.L.gdp_graph_begin(ctypes.byref(g), ctypes.byref(st)), "gdp_graph_begin")
+        try:
+            fn(st)
+        except Exception:
+            self.L.gdp_graph_end(g)
+            self.L.gdp_graph_destroy(g)
+            raise
+        rc = self.L.gdp_graph_end(g)
+        if rc != 0:
+            self.L.gdp_graph_destroy(g)
+            _check(self.L, rc, "gdp_graph_end")
+        return g
 
     def graph(self, h):
         """Captures fill + traceback (requests of prepare_traceback's handle h)

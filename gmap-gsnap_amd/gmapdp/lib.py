@@ -105,6 +105,9 @@ def load(path=None):
         "gdp_graph_create": (vp, [vp, i32, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, i32, vp, i32, ctypes.c_uint32,
                                   vp, vp]),
         "gdp_graph_launch": (ctypes.c_int, [vp, vp]),
+        "gdp_graph_begin": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "gdp_graph_end": (ctypes.c_int, [vp]),
+        "gdp_best_endpoint_ex": (ctypes.c_int, [vp, vp, i32, i32, vp, vp, vp, vp, vp]),
         "gdp_graph_destroy": (None, [vp]),
         "gdp_plan_kernels": (ctypes.c_uint32, [vp, i32]),
         "gdp_fill_ex": (ctypes.c_int, [vp, i32, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]),

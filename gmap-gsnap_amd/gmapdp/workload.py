@@ -32,16 +32,17 @@ class Workload:
     """problems (PROBLEM_DTYPE), meta (META_DTYPE), qbuf and gbuf (uint8) of
     one configuration; gbuf starts with the genome."""
 
-    def __init__(self, config, count, genome_len=1 << 24, variant=0, seed=None, first=0):
+    def __init__(self, config, count, genome_len=1 << 24, variant=0, seed=None, first=0, genome_seed=None):
         L = _bind(load())
         self.config = config
         seed = SEEDS[config] if seed is None else seed
+        genome_seed = seed if genome_seed is None else genome_seed
         stride = L.gdp_gen_max_bytes(config)
         nprob = 2 * count if config == 4 else count
         self.genome_len = genome_len
         extra = count * stride if config == 4 else 0
         self.gbuf = np.zeros(genome_len + extra + 8, np.uint8)
-        L.gdp_gen_genome(seed, self.gbuf.ctypes.data, genome_len)
+        L.gdp_gen_genome(genome_seed, self.gbuf.ctypes.data, genome_len)
         self.qbuf = np.zeros(count * stride + 8, np.uint8)
         self.problems = np.zeros(nprob, PROBLEM_DTYPE)
         self.meta = np.zeros(nprob, META_DTYPE)
@@ -75,4 +76,53 @@ class Workload:
         off = np.zeros(len(idx), np.int64)
         off[1:] = np.cumsum(req["max_pairs"].astype(np.int64))[:-1]
         req["pair_off"] = off
+        return req
+
+
+class EndMix:
+    """Config 4: end-extension problems (an upper and a lower fill each,
+    best end cell over both, traceback in the triangle holding it) mixed 1:1
+    with config-3 single-gap problems, over one genome.  problems[:n3] are
+    the single-gap fills, problems[n3:] the (upper, lower) pairs."""
+
+    def __init__(self, n_end, n_single, genome_len=1 << 24):
+        e = Workload(4, n_end, genome_len=genome_len)
+        s = Workload(3, n_single, genome_len=genome_len, genome_seed=SEEDS[4])
+        self.n3, self.m = n_single, n_end
+        self.genome_len = genome_len
+        qs, gs = len(s.qbuf), len(s.gbuf)
+        pe = e.problems.copy()
+        pe["qoff"] += qs
+        pe["goff"] += gs
+        pe["galtoff"] += gs
+        self.problems = np.concatenate([s.problems, pe])
+        self.meta = np.concatenate([s.meta, e.meta])
+        self.qbuf = np.concatenate([s.qbuf, e.qbuf])
+        self.gbuf = np.concatenate([s.gbuf, e.gbuf])
+        self._genome = e.genome().copy()
+        self.ul = np.stack([n_single + np.arange(0, 2 * n_end, 2), n_single + np.arange(1, 2 * n_end, 2)],
+                           1).astype(np.int32)
+
+    def genome(self):
+        return self._genome
+
+    def single_requests(self):
+        p, m = self.problems[: self.n3], self.meta[: self.n3]
+        req = np.zeros(self.n3, TBREQ_DTYPE)
+        req["fill"], req["r"], req["c"] = np.arange(self.n3), p["rl"], p["gl"]
+        for f in ("queryoffset", "genomeoffset", "chroffset", "chrhigh", "watsonp", "revp"):
+            req[f] = m[f]
+        req["max_pairs"] = p["rl"] + p["gl"] + 4
+        req["pair_off"][1:] = np.cumsum(req["max_pairs"].astype(np.int64))[:-1]
+        return req
+
+    def end_templates(self):
+        """Traceback request templates of the end pairs (fill, r, c left to
+        gdp_best_endpoint_ex)."""
+        up = self.ul[:, 0]
+        p, m = self.problems[up], self.meta[up]
+        req = np.zeros(self.m, TBREQ_DTYPE)
+        for f in ("queryoffset", "genomeoffset", "chroffset", "chrhigh", "watsonp", "revp"):
+            req[f] = m[f]
+        req["max_pairs"] = p["rl"] + p["gl"] + 4
         return req

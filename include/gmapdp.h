@@ -221,6 +221,15 @@ int gdp_fill_ex(const void *d_descs, int32_t n, uint32_t kernels, const char *d_
    capture runs on a stream of the library's own; nothing executes until
    gdp_graph_launch. */
 typedef struct gdp_graph gdp_graph;
+
+/* General form: gdp_graph_begin hands out a stream of the library's in
+   capture mode; every gdp_* call enqueued on it (fills, best end cells,
+   tracebacks, segment extraction) is recorded, not run; gdp_graph_end
+   instantiates the graph.  Capture is per thread: enqueue from the thread
+   that began it. */
+int gdp_graph_begin(gdp_graph **g, void **stream);
+int gdp_graph_end(gdp_graph *g);
+
 gdp_graph *gdp_graph_create(const void *d_descs, int32_t n, uint32_t kernels, const char *d_qbuf, const char *d_gbuf,
                             uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, const gdp_tb_request *d_reqs,
                             int32_t nreq, const void *d_genome, int32_t genome_format, uint32_t genome_len,
@@ -235,6 +244,16 @@ void gdp_graph_destroy(gdp_graph *g);
    d_out[3*k] = {finalscore, bestr, bestc}. */
 int gdp_best_endpoint(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,
                       const int16_t *d_scores, int32_t *d_out, void *stream);
+
+/* As gdp_best_endpoint; with d_tb_template and d_tb_out (both or neither)
+   it also writes the traceback request of each pair: d_tb_out[k] =
+   d_tb_template[k] with fill, r and c set to the best cell and the triangle
+   holding it (the upper fill when bestc >= bestr, dynprog_end.c:1576-1607),
+   so the tracebacks can follow on the device without a host round trip.
+   A best cell of (0, 0) gives a request that pushes no pairs. */
+int gdp_best_endpoint_ex(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,
+                         const int16_t *d_scores, int32_t *d_out, const gdp_tb_request *d_tb_template,
+                         gdp_tb_request *d_tb_out, void *stream);
 
 /* ---- Genome in device memory ------------------------------------------ */
 

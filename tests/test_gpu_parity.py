@@ -167,3 +167,64 @@ def test_traceback_nogaps(oracle):
         got = H.pairs_to_records(pairs[i])
         assert counts[i, 0] == len(want) and (got == want).all(), ("nogaps pairs", i)
         assert (counts[i, 1:4] == wc[:3]).all() and counts[i, 4] == 0 and counts[i, 5] == 0, ("nogaps counters", i)
+
+
+def test_end_mix_device_requests(oracle):
+    """Config-4 mix: best end cells turned into traceback requests on the
+    device (gdp_best_endpoint_ex) give the requests and pairs of the host
+    path, for the single-gap half and the end half in one traceback launch,
+    also when the whole step is replayed from a captured graph."""
+    import torch
+    from gmapdp.batch import DeviceBatch
+    from gmapdp.lib import PAIR_DTYPE, TBREQ_DTYPE
+    from gmapdp.workload import EndMix
+    w = EndMix(150, 100, genome_len=1 << 20)
+    p = w.problems
+    b = DeviceBatch(p, w.qbuf.tobytes(), w.gbuf.tobytes())
+    reqs = np.concatenate([w.single_requests(), w.end_templates()])
+    reqs["pair_off"][1:] = np.cumsum(reqs["max_pairs"].astype(np.int64))[:-1]
+    h = b.prepare_traceback(reqs, genome=w.genome())
+    d_ul = torch.from_numpy(w.ul.reshape(-1).copy()).to(b.dev)
+    d_tmpl = torch.from_numpy(reqs[w.n3:].view(np.uint8).copy()).to(b.dev)
+    d_best = torch.zeros(3 * w.m, dtype=torch.int32, device=b.dev)
+
+    def step(st=None):
+        b.fill(stream=st)
+        b.best_endpoint_requests(d_ul, w.m, False, d_best, d_tmpl.data_ptr(),
+                                 h["d_reqs"].data_ptr() + w.n3 * TBREQ_DTYPE.itemsize, stream=st)
+        b.traceback_device(h, stream=st)
+
+    step()
+    torch.cuda.synchronize()
+    best = d_best.cpu().numpy().reshape(-1, 3)
+    got_reqs = h["d_reqs"].cpu().numpy().view(TBREQ_DTYPE)
+    counts = h["d_counts"].cpu().numpy().reshape(-1, 6)
+    pairs = h["d_pairs"].cpu().numpy().view(PAIR_DTYPE)
+    want_best = b.best_endpoint(w.ul)
+    assert (best == want_best).all()
+    for k in range(w.m):
+        br, bc = int(best[k, 1]), int(best[k, 2])
+        r = got_reqs[w.n3 + k]
+        assert (r["r"], r["c"]) == (br, bc)
+        assert r["fill"] == w.ul[k, 0 if bc >= br else 1]
+    # every request's pairs against a one-request-at-a-time host traceback
+    host_reqs = got_reqs.copy()
+    hp, hc = b.traceback(host_reqs, genome=w.genome())
+    for k in range(len(host_reqs)):
+        off, n = int(host_reqs["pair_off"][k]), int(counts[k, 0])
+        assert (counts[k] == hc[k]).all(), k
+        assert (pairs[off:off + n] == hp[k]).all(), k
+    # a captured graph of the whole step replays to the same bits
+    want = (h["d_pairs"].clone(), h["d_counts"].clone(), d_best.clone())
+    g = b.capture(step)
+    try:
+        for rep in range(3):
+            for t in (h["d_pairs"], h["d_counts"], d_best):
+                t.zero_()
+            torch.cuda.synchronize()
+            b.graph_launch(g)
+            torch.cuda.synchronize()
+            for x, y in zip((h["d_pairs"], h["d_counts"], d_best), want):
+                assert torch.equal(x, y), rep
+    finally:
+        b.graph_destroy(g)
