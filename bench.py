@@ -164,6 +164,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch fill and traceback per step instead of "
                     "replaying one captured HIP graph")
     ap.add_argument("--no-tune", action="store_true", help="default lane groups instead of the autotuner's")
+    ap.add_argument("--lane-groups", default="", metavar="SPAN,ROWS",
+                    help="fixed lane groups (g16_max_span,g64_min_rows) instead of the autotuner's")
     ap.add_argument("--serve-genome", type=int, default=3_100_000_000, help="bases of the device genome")
     ap.add_argument("--serve-requests", type=int, default=300_000)
     ap.add_argument("--serve-batch", type=int, default=16384, help="largest batch the server forms")
@@ -194,11 +196,13 @@ def main():
 
     tuning = None
     params = None
-    if not args.no_tune:
+    if args.lane_groups:
+        params = tuple(int(x) for x in args.lane_groups.split(","))
+    elif not args.no_tune:
         from gmapdp.tune import autotune
         t1 = time.time()
-        params, table = autotune(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes(), sample=min(len(probs), 200_000))
-        tuning = {"objective": "throughput", "sample": min(len(probs), 200_000), "chosen": list(params),
+        params, table = autotune(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes(), sample=min(len(probs), 1_000_000))
+        tuning = {"objective": "throughput", "sample": min(len(probs), 1_000_000), "chosen": list(params),
                   "candidates": [[list(p), round(ms, 3)] for p, ms in table], "seconds": round(time.time() - t1, 1)}
         log("autotuner: g16_max_span=%d g64_min_rows=%d (%.1fs)" % (params[0], params[1], time.time() - t1))
     b = DeviceBatch(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes(), params=params)

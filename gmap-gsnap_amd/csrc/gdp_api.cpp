@@ -103,7 +103,7 @@ static int validate(const gdp_problem &p, int i) {
 
 extern "C" int64_t gdp_plan_bytes(int32_t n) {
   if (n < 0) n = 0;
-  return (int64_t)n * (int64_t)(sizeof(gdp_dev_problem) + sizeof(gdp_wave) + sizeof(int32_t)) +
+  return (int64_t)n * (int64_t)(sizeof(gdp_dev_problem) + sizeof(gdp_wave) + 2 * sizeof(int32_t)) +
          (int64_t)sizeof(gdp_pair_hdr);
 }
 
@@ -269,6 +269,11 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags
     off += (int64_t)waves[c].size() * (int64_t)sizeof(int32_t);
   }
   h.sink_off = sink_off;
+  // problems swept alone, for the single-sweep launch to walk
+  h.list1_off = off;
+  int32_t *list1 = (int32_t *)(base + off);
+  for (int32_t i = 0; i < n; i++)
+    if (d[i].layout == 0) list1[h.n1++] = i;
   memcpy(base + hoff, &h, sizeof h);
   dw += 4;  // the traceback reads direction words in aligned 16-byte windows
   if (dir_words) *dir_words = dw;

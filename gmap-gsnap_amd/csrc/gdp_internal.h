@@ -95,7 +95,10 @@ struct gdp_pair_hdr {
   int32_t gen_queue[4][2];  // the same for the general-lookup pass
   int32_t gen_count[4];     // waves in each general-lookup list
   int64_t sink_off;         // direction words written by the empty slots of partial waves
-  int32_t pad[18];
+  int32_t n1;               // problems swept alone (layout 0)
+  int32_t pad1;
+  int64_t list1_off;        // byte offset of their int32 index list
+  int32_t pad[14];
 };
 static_assert(sizeof(gdp_pair_hdr) == 256, "gdp_pair_hdr must stay 256 bytes");
 

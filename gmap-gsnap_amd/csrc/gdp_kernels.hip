@@ -92,84 +92,92 @@ __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_probl
                                                              int16_t *__restrict__ scores, int32_t *__restrict__ final_out) {
   __shared__ gdp_tables_lds tab;
   __shared__ uint32_t ring_all[GDP_GROUPS_PER_BLOCK * GDP_RING];
+  // A plan lists its single-sweep problems after its header; a bare
+  // descriptor array (one problem of a GDP_PLAN_SINGLE plan) has no header.
+  const gdp_pair_hdr *hdr = (const gdp_pair_hdr *)(probs + n);
+  const bool listed = hdr->magic == (GDP_PAIR_MAGIC ^ (uint64_t)n) && hdr->n == n;
+  const int n1 = listed ? hdr->n1 : n;
+  if (n1 == 0) return;
+  const int32_t *list1 = listed ? (const int32_t *)((const uint8_t *)probs + hdr->list1_off) : nullptr;
   load_tables(tab);
   __syncthreads();
 
   const int lane = threadIdx.x & (GDP_GROUP - 1);
   const int grp = threadIdx.x / GDP_GROUP;
-  const int pid = blockIdx.x * GDP_GROUPS_PER_BLOCK + grp;
-  if (pid >= n) return;
   uint32_t *ring = ring_all + grp * GDP_RING;
+  // groups walk the list grid-stride; no barrier follows
+  for (int slot = blockIdx.x * GDP_GROUPS_PER_BLOCK + grp; slot < n1; slot += gridDim.x * GDP_GROUPS_PER_BLOCK) {
+    const int pid = list1 ? list1[slot] : slot;
+    const gdp_dev_problem p = probs[pid];
+    if (p.layout != 0) continue;  // swept by gdp_fill2_kernel
+    const gdp_geom G = gdp_make_geom(p);
+    const gdp_consts K = gdp_make_consts(p);
+    const bool full = G.full;
+    const bool keep = p.score_off >= 0;
 
-  const gdp_dev_problem p = probs[pid];
-  if (p.layout != 0) return;  // swept by gdp_fill2_kernel
-  const gdp_geom G = gdp_make_geom(p);
-  const gdp_consts K = gdp_make_consts(p);
-  const bool full = G.full;
-  const bool keep = p.score_off >= 0;
+    // Final cell (rl, gl) of a full-band problem; a single writer per problem.
+    const bool final_in_band = full && G.R <= G.C + G.lb && G.R >= G.C - G.ub;
+    if (lane == 0 && !final_in_band) final_out[pid] = INT_MIN;
 
-  // Final cell (rl, gl) of a full-band problem; a single writer per problem.
-  const bool final_in_band = full && G.R <= G.C + G.lb && G.R >= G.C - G.ub;
-  if (lane == 0 && !final_in_band) final_out[pid] = INT_MIN;
+    int T = 0, r = lane;
+    int cs = 0, ce = -1;
+    uint32_t prof = 0;
+    if (r <= G.R) {
+      cs = gdp_cstart(G, r);
+      ce = gdp_cend(G, r);
+      prof = row_profile(tab, p, qbuf, gbuf, r);
+    }
+    int Hl = K.INIT_H, El = K.E_INIT, uprev = K.NEG, Hout = K.NEG, Fout = GDP_F_FLOOR;
+    uint32_t word = 0;
+    uint32_t *dout = dirs + p.dir_off;
 
-  int T = 0, r = lane;
-  int cs = 0, ce = -1;
-  uint32_t prof = 0;
-  if (r <= G.R) {
-    cs = gdp_cstart(G, r);
-    ce = gdp_cend(G, r);
-    prof = row_profile(tab, p, qbuf, gbuf, r);
-  }
-  int Hl = K.INIT_H, El = K.E_INIT, uprev = K.NEG, Hout = K.NEG, Fout = GDP_F_FLOOR;
-  uint32_t word = 0;
-  uint32_t *dout = dirs + p.dir_off;
-
-  for (int s = 0; s < G.nsteps; s++) {
-    int c = s - T - lane;
-    if (r <= G.R && c > ce) {  // this lane's row is done: take row r + 16
-      T += G.P;
-      r += GDP_GROUP;
-      if (r <= G.R) {
-        cs = gdp_cstart(G, r);
-        ce = gdp_cend(G, r);
-        prof = row_profile(tab, p, qbuf, gbuf, r);
+    for (int s = 0; s < G.nsteps; s++) {
+      int c = s - T - lane;
+      if (r <= G.R && c > ce) {  // this lane's row is done: take row r + 16
+        T += G.P;
+        r += GDP_GROUP;
+        if (r <= G.R) {
+          cs = gdp_cstart(G, r);
+          ce = gdp_cend(G, r);
+          prof = row_profile(tab, p, qbuf, gbuf, r);
+        }
+        c = s - T - lane;
       }
-      c = s - T - lane;
-    }
-    const bool act = (r <= G.R) && (c >= cs) && (c <= ce);
+      const bool act = (r <= G.R) && (c >= cs) && (c <= ce);
 
-    // Upper neighbour (r-1, c) from the previous step.
-    int Hup = __builtin_amdgcn_update_dpp(Hout, Hout, 0x111, 0xF, 0xF, false);
-    int Fup = __builtin_amdgcn_update_dpp(Fout, Fout, 0x111, 0xF, 0xF, false);
-    if (lane == 0) {
-      uint32_t ent = ring[c & (GDP_RING - 1)];
-      Hup = (int)(int16_t)(ent & 0xffffu);
-      Fup = (int)(int16_t)(ent >> 16) - 128;
-    }
-    const int Hdg = uprev;  // (r-1, c-1), received one step ago
-    uprev = Hup;
+      // Upper neighbour (r-1, c) from the previous step.
+      int Hup = __builtin_amdgcn_update_dpp(Hout, Hout, 0x111, 0xF, 0xF, false);
+      int Fup = __builtin_amdgcn_update_dpp(Fout, Fout, 0x111, 0xF, 0xF, false);
+      if (lane == 0) {
+        uint32_t ent = ring[c & (GDP_RING - 1)];
+        Hup = (int)(int16_t)(ent & 0xffffu);
+        Fup = (int)(int16_t)(ent >> 16) - 128;
+      }
+      const int Hdg = uprev;  // (r-1, c-1), received one step ago
+      uprev = Hup;
 
-    int H = K.NEG, F = GDP_F_FLOOR, E = El;
-    uint32_t nib = 0;
-    if (act) {
-      int x, xa;
-      col_class(tab, p, qbuf, gbuf, c, x, xa);
-      gdp_cell(K, G, r, c, cs, Hl, El, Hdg, Hup, Fup, prof, x, xa, H, F, E, nib);
-      if (full && r == G.R && c == G.C) final_out[pid] = H;
-      if (lane == GDP_GROUP - 1) ring[c & (GDP_RING - 1)] = (uint32_t)(H & 0xffff) | ((uint32_t)(F + 128) << 16);
+      int H = K.NEG, F = GDP_F_FLOOR, E = El;
+      uint32_t nib = 0;
+      if (act) {
+        int x, xa;
+        col_class(tab, p, qbuf, gbuf, c, x, xa);
+        gdp_cell(K, G, r, c, cs, Hl, El, Hdg, Hup, Fup, prof, x, xa, H, F, E, nib);
+        if (full && r == G.R && c == G.C) final_out[pid] = H;
+        if (lane == GDP_GROUP - 1) ring[c & (GDP_RING - 1)] = (uint32_t)(H & 0xffff) | ((uint32_t)(F + 128) << 16);
+      }
+      if (keep) scores[p.score_off + (int64_t)s * GDP_GROUP + lane] = (int16_t)(act ? H : 0);
+      Hout = H;
+      Fout = F;
+      Hl = H;
+      El = E;
+      word |= nib << (4 * (s & 7));
+      if ((s & 7) == 7) {
+        dout[(int64_t)(s >> 3) * GDP_GROUP + lane] = word;
+        word = 0;
+      }
     }
-    if (keep) scores[p.score_off + (int64_t)s * GDP_GROUP + lane] = (int16_t)(act ? H : 0);
-    Hout = H;
-    Fout = F;
-    Hl = H;
-    El = E;
-    word |= nib << (4 * (s & 7));
-    if ((s & 7) == 7) {
-      dout[(int64_t)(s >> 3) * GDP_GROUP + lane] = word;
-      word = 0;
-    }
+    if (G.nsteps & 7) dout[(int64_t)(G.nsteps >> 3) * GDP_GROUP + lane] = word;
   }
-  if (G.nsteps & 7) dout[(int64_t)(G.nsteps >> 3) * GDP_GROUP + lane] = word;
 }
 
 // ------------------------------------------------------------------ lookups
@@ -614,51 +622,63 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
 // ------------------------------------------------------------ best end cell
 
 // find_best_endpoint_{8,16} / find_best_endpoint_to_queryend_indels_{8,16}
-// (dynprog_end.c:143-290, 358-508) over the kept scores of pair k.
-__device__ void gdp_best_cell(const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ ul, int k,
-                              int queryend_indels, const int16_t *__restrict__ scores, int &best, int &bestr,
-                              int &bestc) {
+// (dynprog_end.c:143-290, 358-508): the best score over both triangles in
+// row-major order, starting from score 0 at (0, 0) (or -inf at (rl, 0), last
+// row only, for queryend_indels); a later cell replaces the best when it is
+// greater (>= under jump-late).  The scan keeps the first (last under
+// jump-late) occurrence of the maximum, so TBG lanes can scan disjoint rows
+// (columns of the single row for queryend_indels) and merge by value, then
+// by position.  One group of TBG lanes per (upper, lower) pair.
+__device__ __forceinline__ bool gdp_best_before(int v, int r, int c, int bv, int br, int bc, int late) {
+  // does candidate (v, r, c) replace (bv, br, bc)?
+  if (v != bv) return v > bv;
+  const bool later = r != br ? r > br : c > bc;
+  return late ? later : !later;
+}
+
+__global__ __launch_bounds__(256) void gdp_best_endpoint_kernel(
+    const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ ul, int m, int queryend_indels,
+    const int16_t *__restrict__ scores, int32_t *__restrict__ out, const gdp_tb_request *__restrict__ tmpl,
+    gdp_tb_request *__restrict__ tbout) {
+  const int j = (int)(threadIdx.x & (TBG - 1));
+  const int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) / TBG);
+  if (k >= m) return;  // whole groups leave together
   const gdp_dev_problem pu = probs[ul[2 * k]], pl = probs[ul[2 * k + 1]];
-  const gdp_geom Gu = gdp_make_geom(pu), Gl = gdp_make_geom(pl);
+  int best, bestr, bestc;
   if (pu.score_off < 0 || pl.score_off < 0 || pu.kind != GDP_KIND_UPPER || pl.kind != GDP_KIND_LOWER) {
     best = INT_MIN;  // scores were not kept, or the pair is not (upper, lower)
     bestr = -1;
     bestc = -1;
-    return;
-  }
-  const int rl = pu.rl, gl = pu.gl, lband = pl.lband, uband = pu.uband, late = pu.jump_late;
-  best = queryend_indels ? (pu.width == 8 ? -128 : -32768) : 0;
-  bestr = queryend_indels ? rl : 0;
-  bestc = 0;
-  for (int r = queryend_indels ? rl : 1; r <= rl; r++) {
-    int clo = gdp_imax(1, r - lband), chigh = gdp_imin(gl, r + uband), c;
-    for (c = clo; c < r; c++) {
-      int v = score_at(scores, pl, Gl, c, r);  // lower: rows are genome columns
-      if (gdp_wins(v, best, late)) {
-        bestr = r;
-        bestc = c;
-        best = v;
+  } else {
+    const gdp_geom Gu = gdp_make_geom(pu), Gl = gdp_make_geom(pl);
+    const int rl = pu.rl, gl = pu.gl, lband = pl.lband, uband = pu.uband, late = pu.jump_late;
+    best = queryend_indels ? (pu.width == 8 ? -128 : -32768) : 0;
+    bestr = queryend_indels ? rl : 0;
+    bestc = 0;
+    const int r0 = queryend_indels ? rl : 1;
+    for (int r = r0 + (queryend_indels ? 0 : j); r <= rl; r += queryend_indels ? rl + 1 : TBG) {
+      const int clo = gdp_imax(1, r - lband), chigh = gdp_imin(gl, r + uband);
+      for (int c = clo + (queryend_indels ? j : 0); c <= chigh; c += queryend_indels ? TBG : 1) {
+        // lower triangle left of the diagonal (its rows are genome columns)
+        const int v = c < r ? score_at(scores, pl, Gl, c, r) : score_at(scores, pu, Gu, r, c);
+        if (late ? v >= best : v > best) {
+          best = v;
+          bestr = r;
+          bestc = c;
+        }
       }
     }
-    for (; c <= chigh; c++) {
-      int v = score_at(scores, pu, Gu, r, c);
-      if (gdp_wins(v, best, late)) {
-        bestr = r;
-        bestc = c;
-        best = v;
+    // merge the lanes' results in scan order
+    for (int d = 1; d < TBG; d <<= 1) {
+      const int ov = __shfl_xor(best, d, 64), orr = __shfl_xor(bestr, d, 64), oc = __shfl_xor(bestc, d, 64);
+      if (gdp_best_before(ov, orr, oc, best, bestr, bestc, late)) {
+        best = ov;
+        bestr = orr;
+        bestc = oc;
       }
     }
   }
-}
-
-__global__ void gdp_best_endpoint_kernel(const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ ul, int m,
-                                         int queryend_indels, const int16_t *__restrict__ scores,
-                                         int32_t *__restrict__ out, const gdp_tb_request *__restrict__ tmpl,
-                                         gdp_tb_request *__restrict__ tbout) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  int best, bestr, bestc;
-  gdp_best_cell(probs, ul, k, queryend_indels, scores, best, bestr, bestc);
+  if (j != 0) return;
   out[3 * k] = best;
   out[3 * k + 1] = bestr;
   out[3 * k + 2] = bestc;
@@ -754,7 +774,17 @@ extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, con
                                uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, hipStream_t stream) {
   if (n <= 0) return 0;
   if (kernels & 1u) {
-    int blocks = (n + GDP_GROUPS_PER_BLOCK - 1) / GDP_GROUPS_PER_BLOCK;
+    // resident blocks only; they walk the plan's list of single sweeps
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
+    if (cached[dev] == 0) {
+      int ncu = 0, per = 0;
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill_kernel, GDP_BLOCK, 0) != hipSuccess) return -1;
+      cached[dev] = ncu * (per > 0 ? per : 1);
+    }
+    const int blocks = gdp_imin(cached[dev], (n + GDP_GROUPS_PER_BLOCK - 1) / GDP_GROUPS_PER_BLOCK);
     hipLaunchKernelGGL(gdp_fill_kernel, dim3(blocks), dim3(GDP_BLOCK), 0, stream, (const gdp_dev_problem *)d_descs, n,
                        d_qbuf, d_gbuf, d_dirs, d_scores, d_final);
     if (hipGetLastError() != hipSuccess) return -1;
@@ -789,7 +819,7 @@ extern "C" int gdp_launch_best_endpoint(const void *d_descs, const int32_t *d_ul
                                         const int16_t *d_scores, int32_t *d_out, const gdp_tb_request *d_tmpl,
                                         gdp_tb_request *d_tbout, hipStream_t stream) {
   if (m <= 0) return 0;
-  hipLaunchKernelGGL(gdp_best_endpoint_kernel, dim3((m + 127) / 128), dim3(128), 0, stream,
+  hipLaunchKernelGGL(gdp_best_endpoint_kernel, dim3((m + 256 / TBG - 1) / (256 / TBG)), dim3(256), 0, stream,
                      (const gdp_dev_problem *)d_descs, d_ul, m, queryend_indels, d_scores, d_out, d_tmpl, d_tbout);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

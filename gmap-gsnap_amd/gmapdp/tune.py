@@ -5,15 +5,16 @@ The plan's two knobs (gdp_plan_params) trade padding against sweep length:
 groups sweep a long problem in about a quarter of the steps.  The tuner
 times real fills of a seeded sample of the caller's problems on the device
 for each candidate and keeps the best: the whole sample's fill time for
-throughput, or the fill time of a batch of the sample's longest problems
-for latency (the time a request waits on the slowest problem of its batch).
+throughput (the sample large enough that the slowest wave does not decide
+it), or the fill time of a batch of the longest problems for latency (the
+time a request waits on the slowest problem of its batch).
 """
 import numpy as np
 
 from .batch import DeviceBatch, _torch
 
-SPANS = (240, 112, 48)
-ROWS = (8001, 512, 256, 128)
+SPANS = (240, 112)
+ROWS = {"throughput": (8001, 512, 256), "latency": (8001, 512, 256, 128)}
 
 
 def _time_fill(b, reps):
@@ -29,7 +30,7 @@ def _time_fill(b, reps):
     return float(np.median([a.elapsed_time(z) for a, z in ev]))
 
 
-def autotune(problems, qbuf, gbuf, sample=200_000, objective="throughput", reps=3, seed=0, longest=64):
+def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", reps=2, seed=0, longest=64):
     """Returns ((g16_max_span, g64_min_rows), table) where table lists
     (params, milliseconds) of every candidate."""
     rng = np.random.default_rng(seed)
@@ -41,7 +42,7 @@ def autotune(problems, qbuf, gbuf, sample=200_000, objective="throughput", reps=
     sub = np.ascontiguousarray(problems[idx])
     table = []
     for span in SPANS:
-        for rows in ROWS:
+        for rows in ROWS[objective]:
             b = DeviceBatch(sub, qbuf, gbuf, params=(span, rows))
             table.append(((span, rows), _time_fill(b, reps)))
             del b
