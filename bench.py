@@ -108,8 +108,9 @@ def serving_bench(args):
     lane, table = autotune(tw.problems, tw.qbuf.tobytes(), tw.gbuf.tobytes(), objective="latency")
     del tw
     log("serving lane groups (latency autotuner): g16_max_span=%d g64_min_rows=%d" % lane)
-    srv = serve.Server(dg, dir_budget_bytes=args.serve_budget << 20, max_batch=args.serve_batch, slots=3,
-                       max_rl=1000, max_gl=1500, pair_capacity=args.serve_batch * 700, plan=lane)
+    srv = serve.Server(dg, dir_budget_bytes=args.serve_budget << 20, max_batch=args.serve_batch,
+                       slots=args.serve_slots, max_rl=1000, max_gl=1500, pair_capacity=args.serve_batch * 700,
+                       plan=lane, long_rows=args.serve_long)
     try:
         srv.drive(reqs[:min(nreq, 20000)], 0.0)  # warm-up
         lat, el, ok = srv.drive(reqs, 0.0)
@@ -136,7 +137,8 @@ def serving_bench(args):
             "requests": nreq, "mean_cells_per_request": round(float(cells.mean()), 1),
             "saturation_req_s": round(sat, 1), "saturation_gcups": round(cells.sum() / (nreq / sat) / 1e9, 2),
             "p50_ms_at_50pct": half["p50_ms"], "p99_ms_at_50pct": half["p99_ms"], "points": points,
-            "batches": int(st["batches"]), "max_batch": args.serve_batch, "slots": 3,
+            "batches": int(st["batches"]), "max_batch": args.serve_batch, "slots": args.serve_slots,
+            "long_rows": args.serve_long,
             "lane_groups": {"g16_max_span": lane[0], "g64_min_rows": lane[1],
                             "candidates_ms": [[list(p), round(ms, 3)] for p, ms in table]},
             "dir_budget_mib": args.serve_budget, "dir_bytes_peak": int(st["dir_bytes_peak"])}
@@ -168,7 +170,10 @@ def main():
                     help="fixed lane groups (g16_max_span,g64_min_rows) instead of the autotuner's")
     ap.add_argument("--serve-genome", type=int, default=3_100_000_000, help="bases of the device genome")
     ap.add_argument("--serve-requests", type=int, default=300_000)
-    ap.add_argument("--serve-batch", type=int, default=16384, help="largest batch the server forms")
+    ap.add_argument("--serve-batch", type=int, default=4096, help="largest batch the server forms")
+    ap.add_argument("--serve-slots", type=int, default=2, help="batches in flight (streams)")
+    ap.add_argument("--serve-long", type=int, default=400,
+                    help="reads of at least this many bases are batched apart (0: one class)")
     ap.add_argument("--serve-budget", type=int, default=8192, help="direction budget in MiB")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="target duration of each load point")
     args = ap.parse_args()

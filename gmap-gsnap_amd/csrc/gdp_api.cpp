@@ -319,11 +319,18 @@ extern "C" uint32_t gdp_plan_kernels(const void *descs, int32_t n) {
   memcpy(&h, (const uint8_t *)descs + (int64_t)n * sizeof(gdp_dev_problem), sizeof h);
   if (h.magic != (GDP_PAIR_MAGIC ^ (uint64_t)n)) return GDP_FILL_ALL;
   uint32_t mask = 0;
+  int pmax = 0;
   for (int32_t i = 0; i < n; i++)
     if (d[i].layout == 0) {
       mask |= 1u;
-      break;
+      pmax = gdp_imax(pmax, gdp_make_geom(d[i]).P);
     }
+  if (mask & 1u) {
+    // LDS ring of the single sweep: the next power of two >= its largest P
+    int rlog = 6;
+    while ((1 << rlog) < pmax) rlog++;
+    mask |= (uint32_t)rlog << 8;
+  }
   for (int c = 0; c < GDP_NCLS; c++)
     if (h.nw[c] > 0) mask |= 2u << c;
   return mask;

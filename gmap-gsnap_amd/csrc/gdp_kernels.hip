@@ -89,9 +89,12 @@ __device__ __forceinline__ void col_class(const gdp_tables_lds &t, const gdp_dev
 __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_problem *__restrict__ probs, int n,
                                                              const char *__restrict__ qbuf,
                                                              const char *__restrict__ gbuf, uint32_t *__restrict__ dirs,
-                                                             int16_t *__restrict__ scores, int32_t *__restrict__ final_out) {
+                                                             int16_t *__restrict__ scores, int32_t *__restrict__ final_out,
+                                                             int ringsz) {
+  // ringsz (a power of two, at least the largest P of the plan's single
+  // sweeps) entries of LDS ring per group, sized per launch
   __shared__ gdp_tables_lds tab;
-  __shared__ uint32_t ring_all[GDP_GROUPS_PER_BLOCK * GDP_RING];
+  extern __shared__ uint32_t ring_all[];
   // A plan lists its single-sweep problems after its header; a bare
   // descriptor array (one problem of a GDP_PLAN_SINGLE plan) has no header.
   const gdp_pair_hdr *hdr = (const gdp_pair_hdr *)(probs + n);
@@ -104,7 +107,8 @@ __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_probl
 
   const int lane = threadIdx.x & (GDP_GROUP - 1);
   const int grp = threadIdx.x / GDP_GROUP;
-  uint32_t *ring = ring_all + grp * GDP_RING;
+  uint32_t *ring = ring_all + grp * ringsz;
+  const int rmask = ringsz - 1;
   // groups walk the list grid-stride; no barrier follows
   for (int slot = blockIdx.x * GDP_GROUPS_PER_BLOCK + grp; slot < n1; slot += gridDim.x * GDP_GROUPS_PER_BLOCK) {
     const int pid = list1 ? list1[slot] : slot;
@@ -149,7 +153,7 @@ __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_probl
       int Hup = __builtin_amdgcn_update_dpp(Hout, Hout, 0x111, 0xF, 0xF, false);
       int Fup = __builtin_amdgcn_update_dpp(Fout, Fout, 0x111, 0xF, 0xF, false);
       if (lane == 0) {
-        uint32_t ent = ring[c & (GDP_RING - 1)];
+        uint32_t ent = ring[c & rmask];
         Hup = (int)(int16_t)(ent & 0xffffu);
         Fup = (int)(int16_t)(ent >> 16) - 128;
       }
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_probl
         col_class(tab, p, qbuf, gbuf, c, x, xa);
         gdp_cell(K, G, r, c, cs, Hl, El, Hdg, Hup, Fup, prof, x, xa, H, F, E, nib);
         if (full && r == G.R && c == G.C) final_out[pid] = H;
-        if (lane == GDP_GROUP - 1) ring[c & (GDP_RING - 1)] = (uint32_t)(H & 0xffff) | ((uint32_t)(F + 128) << 16);
+        if (lane == GDP_GROUP - 1) ring[c & rmask] = (uint32_t)(H & 0xffff) | ((uint32_t)(F + 128) << 16);
       }
       if (keep) scores[p.score_off + (int64_t)s * GDP_GROUP + lane] = (int16_t)(act ? H : 0);
       Hout = H;
@@ -774,19 +778,25 @@ extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, con
                                uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, hipStream_t stream) {
   if (n <= 0) return 0;
   if (kernels & 1u) {
-    // resident blocks only; they walk the plan's list of single sweeps
-    static int cached[64] = {0};
+    // resident blocks only; they walk the plan's list of single sweeps with
+    // the ring size the plan asks for (GDP_RING when it does not say)
+    const int rlog = (kernels >> 8) & 15;
+    const int ringsz = rlog ? 1 << rlog : GDP_RING;
+    if (ringsz > GDP_RING) return -1;
+    const size_t lds = (size_t)GDP_GROUPS_PER_BLOCK * ringsz * sizeof(uint32_t);
+    static int cached[64][16] = {{0}};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
-    if (cached[dev] == 0) {
+    const int slot = rlog ? rlog : 10;
+    if (cached[dev][slot] == 0) {
       int ncu = 0, per = 0;
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill_kernel, GDP_BLOCK, 0) != hipSuccess) return -1;
-      cached[dev] = ncu * (per > 0 ? per : 1);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill_kernel, GDP_BLOCK, lds) != hipSuccess) return -1;
+      cached[dev][slot] = ncu * (per > 0 ? per : 1);
     }
-    const int blocks = gdp_imin(cached[dev], (n + GDP_GROUPS_PER_BLOCK - 1) / GDP_GROUPS_PER_BLOCK);
-    hipLaunchKernelGGL(gdp_fill_kernel, dim3(blocks), dim3(GDP_BLOCK), 0, stream, (const gdp_dev_problem *)d_descs, n,
-                       d_qbuf, d_gbuf, d_dirs, d_scores, d_final);
+    const int blocks = gdp_imin(cached[dev][slot], (n + GDP_GROUPS_PER_BLOCK - 1) / GDP_GROUPS_PER_BLOCK);
+    hipLaunchKernelGGL(gdp_fill_kernel, dim3(blocks), dim3(GDP_BLOCK), lds, stream, (const gdp_dev_problem *)d_descs, n,
+                       d_qbuf, d_gbuf, d_dirs, d_scores, d_final, ringsz);
     if (hipGetLastError() != hipSuccess) return -1;
   }
   if (kernels & 0xEu) {

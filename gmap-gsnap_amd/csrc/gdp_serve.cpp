@@ -141,7 +141,7 @@ struct gdp_server {
 
   std::mutex mu;
   std::condition_variable cv_work;
-  std::deque<Req> pending;
+  std::deque<Req> pending[2];  // requests waiting: short queries, long queries
   ReadRing ring;
   std::condition_variable cv_space;
 
@@ -217,7 +217,10 @@ void gdp_server::fail_all(const char *what, hipError_t e) {
   std::deque<Req> rest;
   {
     std::lock_guard<std::mutex> lk(mu);
-    rest.swap(pending);
+    for (auto &pq : pending)
+      for (Req &q : pq) rest.push_back(q);
+    pending[0].clear();
+    pending[1].clear();
   }
   for (Req &q : rest) {
     q.status = (int8_t)-GDP_EHIP;
@@ -235,18 +238,23 @@ bool gdp_server::launch(Slot &s) {
     if (rfree.empty()) return false;  // the consumer holds every result buffer
   }
   std::vector<Req> batch;
+  int cls = 0;
   {
     std::lock_guard<std::mutex> lk(mu);
+    // a batch holds one length class: the one whose oldest request waited longest
+    if (pending[0].empty() || (!pending[1].empty() && pending[1].front().t_submit < pending[0].front().t_submit))
+      cls = 1;
+    std::deque<Req> &pq = pending[cls];
     int64_t qb = 0, gb = 0, pb = 0;
-    while (!pending.empty() && (int32_t)batch.size() < cfg.max_batch) {
-      const Req &q = pending.front();
+    while (!pq.empty() && (int32_t)batch.size() < cfg.max_batch) {
+      const Req &q = pq.front();
       const int64_t pairs = (int64_t)q.r.rl + q.r.gl + 2;
       if (qb + q.r.rl + 16 > q_cap || gb + q.r.gl + 16 > g_cap || pb + pairs > cfg.pair_capacity) break;
       qb += q.r.rl;
       gb += q.r.gl;
       pb += pairs;
-      batch.push_back(std::move(pending.front()));
-      pending.pop_front();
+      batch.push_back(std::move(pq.front()));
+      pq.pop_front();
     }
   }
   if (batch.empty()) return false;
@@ -298,7 +306,7 @@ bool gdp_server::launch(Slot &s) {
       for (const Slot &o : slots) idle = idle && !o.busy;
       if (!idle) {  // wait for budget
         std::lock_guard<std::mutex> lk(mu);
-        pending.push_front(std::move(batch[0]));
+        pending[cls].push_front(std::move(batch[0]));
         return false;
       }
       DoneBatch b;  // one request larger than the whole budget
@@ -310,7 +318,7 @@ bool gdp_server::launch(Slot &s) {
     const int keep = n / 2;
     {
       std::lock_guard<std::mutex> lk(mu);
-      for (int i = n - 1; i >= keep; i--) pending.push_front(std::move(batch[i]));
+      for (int i = n - 1; i >= keep; i--) pending[cls].push_front(std::move(batch[i]));
     }
     batch.resize(keep);
     n = keep;
@@ -364,7 +372,10 @@ bool gdp_server::launch(Slot &s) {
   {
     // the launched reads are the oldest; their ring space is free again
     std::lock_guard<std::mutex> lk(mu);
-    ring.freed = pending.empty() ? ring.next : pending.front().roff;
+    int64_t oldest = ring.next;
+    for (const auto &pq : pending)
+      if (!pq.empty()) oldest = std::min(oldest, pq.front().roff);
+    ring.freed = oldest;
   }
   cv_space.notify_all();
   const int64_t up_bytes = off_q + qpos + 16;
@@ -494,7 +505,8 @@ void gdp_server::run() {
       std::this_thread::sleep_for(std::chrono::microseconds(20));
     } else {
       std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait_for(lk, std::chrono::milliseconds(5), [&] { return !pending.empty() || stop.load(); });
+      cv_work.wait_for(lk, std::chrono::milliseconds(5),
+                       [&] { return !pending[0].empty() || !pending[1].empty() || stop.load(); });
     }
   }
 }
@@ -602,7 +614,7 @@ extern "C" int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, i
     memcpy(s->ring.at(v), r.read, (size_t)r.rl);
     s->ring.next = v + r.rl;
     q.roff = v;
-    s->pending.push_back(q);
+    s->pending[s->cfg.long_rows > 0 && r.rl >= s->cfg.long_rows ? 1 : 0].push_back(q);
     accepted++;
   }
   lk.unlock();

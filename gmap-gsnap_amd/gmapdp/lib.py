@@ -60,7 +60,7 @@ ALIGN_RES_DTYPE = np.dtype([("tag", "<u8"), ("status", "<i4"), ("finalscore", "<
 PLAN_PARAMS_DTYPE = np.dtype([("g16_max_span", "<i4"), ("g64_min_rows", "<i4"), ("pad", "<i4", (2,))])
 SERVER_CFG_DTYPE = np.dtype([("dir_budget_bytes", "<i8"), ("pair_capacity", "<i8"), ("max_batch", "<i4"),
                              ("slots", "<i4"), ("max_rl", "<i4"), ("max_gl", "<i4"), ("result_buffers", "<i4"),
-                             ("pad", "<i4"), ("plan", PLAN_PARAMS_DTYPE)])
+                             ("long_rows", "<i4"), ("plan", PLAN_PARAMS_DTYPE)])
 SERVER_STATS_DTYPE = np.dtype([("submitted", "<i8"), ("completed", "<i8"), ("batches", "<i8"),
                                ("dir_bytes_peak", "<i8"), ("inflight_peak", "<i4"), ("pad", "<i4"),
                                ("prep_seconds", "<f8"), ("retire_seconds", "<f8"), ("batch_seconds", "<f8"),

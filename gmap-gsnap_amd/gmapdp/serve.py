@@ -108,7 +108,7 @@ class Server:
     """gdp_server over a DeviceGenome."""
 
     def __init__(self, genome, dir_budget_bytes=1 << 30, max_batch=8192, slots=3, max_rl=1000, max_gl=1500,
-                 pair_capacity=None, plan=None, result_buffers=0):
+                 pair_capacity=None, plan=None, result_buffers=0, long_rows=0):
         L = init()
         self.L, self.genome = L, genome
         cfg = np.zeros(1, SERVER_CFG_DTYPE)
@@ -116,6 +116,7 @@ class Server:
         cfg["max_batch"], cfg["slots"], cfg["max_rl"], cfg["max_gl"] = max_batch, slots, max_rl, max_gl
         cfg["pair_capacity"] = pair_capacity or max_batch * (max_rl + max_gl + 2) // 2
         cfg["result_buffers"] = result_buffers
+        cfg["long_rows"] = long_rows
         if plan is not None:
             cfg["plan"]["g16_max_span"], cfg["plan"]["g64_min_rows"] = plan
         self.h = L.gdp_server_create(cfg.ctypes.data, genome.ptr, genome.fmt, genome.nbases)

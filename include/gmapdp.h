@@ -204,7 +204,9 @@ int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_g
              int16_t *d_scores, int32_t *d_final, void *stream);
 
 /* The kernels a plan needs, read from the host copy of its descriptors:
-   bit 0 single sweeps, bits 1..3 the paired wave classes.  gdp_fill_ex
+   bit 0 single sweeps, bits 1..3 the paired wave classes, bits 8..11 the
+   log2 of the single sweep's LDS ring (the next power of two >= the largest
+   batch period among its problems; 0 means the largest ring).  gdp_fill_ex
    launches only those (gdp_fill launches all, and empty ones return at
    once). */
 #define GDP_FILL_ALL 0xFu
@@ -365,7 +367,7 @@ typedef struct gdp_server_config {
   int32_t max_rl;           /* longest read accepted (<= GDP_MAX_LEN) */
   int32_t max_gl;           /* longest genome segment accepted (<= GDP_MAX_LEN) */
   int32_t result_buffers;   /* pinned result buffers (>= slots; 0 = slots + 2) */
-  int32_t pad;
+  int32_t long_rows;        /* reads of at least this many bases batch apart from shorter ones (0 = one class) */
   gdp_plan_params plan;     /* lane groups of the batches (all zero = gdp_plan's defaults) */
 } gdp_server_config;
 

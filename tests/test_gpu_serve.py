@@ -156,14 +156,16 @@ def _check_result(O, genome, nbases, req, read, res, pairs):
     assert (res["counts"][1:] == co).all(), tag
 
 
-def test_server_matches_oracle(oracle, small_genome):
+@pytest.mark.parametrize("long_rows", [0, 300])
+def test_server_matches_oracle(oracle, small_genome, long_rows):
+    """long_rows > 0 batches long and short reads apart (two queues)."""
     from gmapdp import serve
     gc, nbases, genome = small_genome
     reqs, reads = serve.gen_requests(1005, 0, 300, gc, nbases)
     reqs["genestrand"] = np.arange(300) % 3
     reqs["roffset"] = np.arange(300) % 7
     dg = serve.DeviceGenome(gc, nbases)
-    with serve.Server(dg, max_batch=64, slots=3) as srv:
+    with serve.Server(dg, max_batch=64, slots=3, long_rows=long_rows, plan=(112, 128)) as srv:
         srv.submit(reqs)
         out = srv.collect(len(reqs))
         st = srv.stats()
