@@ -38,6 +38,9 @@ OPS_PER_CELL = 13  # algorithmic int16 ops per in-band cell (E 4, F 4, H 1+4)
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
+    "c1": (1, 2_000, "single-gap 100-nt reads from 2%-substituted, 0.2%-indel genome spans, extraband 10, "
+                     "int16 HIGHQ, 2000 problems (the CPU-runnable case)"),
+    "c1lowq": (1, 2_000, "as c1 with LOWQ scoring: int8 fills, 2000 problems"),
     "c2": (2, 1_000_000, "single-gap rl=150 gl=U{160..200} extraband=U{5..20} int16, 1e6 problems"),
     "c3": (3, 2_000_000, "single-gap rl=logU[20,1000] gl=rl*U(1,1.5) extraband=U{3..64} mixed int8/int16, 2e6 problems"),
     "c4": (4, 2_000_000, "end extensions rl=U{10..150} gl=rl+U{0..20} ENDQ (upper+lower fill, best end cell, "
@@ -193,7 +196,7 @@ def main():
         from gmapdp.workload import EndMix
         wl = EndMix(n, n)
     else:
-        wl = Workload(config, n)
+        wl = Workload(config, n, variant=1 if args.workload == "c1lowq" else 0)
     probs = wl.problems
     total_cells = batch_cells(probs)
     log("workload %s: %d problems, %.3e cells, generated in %.1fs" % (args.workload, len(probs), total_cells,
@@ -330,7 +333,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int16",
+        "dtype": {"c1lowq": "int8", "c3": "int16+int8", "c4": "int16+int8"}.get(args.workload, "int16"),
         "data": "synthetic (seeded generator of the workload)",
         "config": {"workload": desc, "problems": len(probs), "cells": total_cells,
                    "step": "fill + traceback" if config != 4 else "fill + best end cell + traceback",
