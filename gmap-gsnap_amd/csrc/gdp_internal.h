@@ -217,27 +217,28 @@ GDP_HD int gdp_field4(uint32_t w, int x) { return ((int)(w << (28 - 4 * x))) >> 
 //   bottom band edge .............. 3390-3416
 //   F chain, top band edge ........ 3419-3476
 //   one-sided E cap, diagonal ..... 4478-4626, 7867-7960
-GDP_HD void gdp_cell(const gdp_consts &k, const gdp_geom &G, int r, int c, int cs, int Hl, int El, int Hdg, int Hup,
-                     int Fup, uint32_t prof, int x, int xa, int &H, int &F, int &E, uint32_t &nib) {
+template <bool FULL>
+GDP_HD void gdp_cell_k(const gdp_consts &k, const gdp_geom &G, int r, int c, int cs, int Hl, int El, int Hdg, int Hup,
+                       int Fup, uint32_t prof, int x, int xa, int &H, int &F, int &E, uint32_t &nib) {
   const bool rowstart = c == cs;
   const int hl = rowstart ? k.INIT_H : Hl;
   int el = rowstart ? k.E_INIT : El;
   int diag = (rowstart && (r % G.S) != 0) ? k.INIT_H : Hdg;
   if (r == 0) diag = c == 0 ? 0 : k.NEG;
   const int sc = gdp_imax(gdp_field4(prof, x), gdp_field4(prof, xa));
-  const int capE = (!G.full && r >= c) ? k.NEG : k.POS;
+  const int capE = (!FULL && r >= c) ? k.NEG : k.POS;
   el = gdp_imin(el, capE);
   const int T1 = gdp_clampw(hl + k.o, k.NEG, k.POS);
   int dE = gdp_wins(el, T1, k.late);
   E = gdp_imin(gdp_clampw(gdp_imax(el, T1) + k.e, k.NEG, k.POS), capE);
   int Hd;
-  if (G.full && c == 0) Hd = r == 0 ? 0 : k.NEG;
+  if (FULL && c == 0) Hd = r == 0 ? 0 : k.NEG;
   else Hd = gdp_clampw(diag + sc, k.NEG, k.POS);
   int dH = gdp_wins(E, Hd, k.late);
   int H0 = gdp_imax(Hd, E);
   int dF = 0, dV = 0;
   F = GDP_F_FLOOR;
-  if (G.full) {
+  if (FULL) {
     const int kd = c - r;
     if (kd == -G.lb && c > 0) {
       H0 = Hd;
@@ -268,4 +269,10 @@ GDP_HD void gdp_cell(const gdp_consts &k, const gdp_geom &G, int r, int c, int c
     H = H0;
   }
   nib = (uint32_t)((dV ? GDP_DIR_V : (dH ? GDP_DIR_H : 0)) | (dE ? GDP_DIR_E : 0) | (dF ? GDP_DIR_F : 0));
+}
+
+GDP_HD void gdp_cell(const gdp_consts &k, const gdp_geom &G, int r, int c, int cs, int Hl, int El, int Hdg, int Hup,
+                     int Fup, uint32_t prof, int x, int xa, int &H, int &F, int &E, uint32_t &nib) {
+  if (G.full) gdp_cell_k<true>(k, G, r, c, cs, Hl, El, Hdg, Hup, Fup, prof, x, xa, H, F, E, nib);
+  else gdp_cell_k<false>(k, G, r, c, cs, Hl, El, Hdg, Hup, Fup, prof, x, xa, H, F, E, nib);
 }

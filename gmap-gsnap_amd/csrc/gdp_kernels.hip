@@ -86,13 +86,114 @@ __device__ __forceinline__ void col_class(const gdp_tables_lds &t, const gdp_dev
   }
 }
 
+// Column classes cached per group in LDS for problems with at most
+// GDP_CCAP - 1 columns: byte c = genome class | alternate class << 4.
+#define GDP_CCAP 512
+
+// One problem of the single sweep, the band kind fixed at compile time.
+template <bool FULL>
+__device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_dev_problem &p, int pid,
+                                         const char *__restrict__ qbuf, const char *__restrict__ gbuf,
+                                         uint32_t *__restrict__ dirs, int16_t *__restrict__ scores,
+                                         int32_t *__restrict__ final_out, uint32_t *ring, int rmask, uint8_t *ccls,
+                                         int lane) {
+  const gdp_geom G = gdp_make_geom(p);
+  const gdp_consts K = gdp_make_consts(p);
+  const bool keep = p.score_off >= 0;
+
+  // Final cell (rl, gl) of a full-band problem; a single writer per problem.
+  const bool final_in_band = FULL && G.R <= G.C + G.lb && G.R >= G.C - G.ub;
+  if (FULL && lane == 0 && !final_in_band) final_out[pid] = INT_MIN;
+
+  // the columns' classes, once per problem
+  const bool cached = G.C < GDP_CCAP;
+  if (cached) {
+    for (int c = lane; c <= G.C; c += GDP_GROUP) {
+      int x, xa;
+      col_class(tab, p, qbuf, gbuf, c, x, xa);
+      ccls[c] = (uint8_t)(x | (xa << 4));
+    }
+    // the group's lanes read what the others wrote: one wave, in order
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+
+  int T = 0, r = lane;
+  int cs = 0, ce = -1;
+  uint32_t prof = 0;
+  if (r <= G.R) {
+    cs = gdp_cstart(G, r);
+    ce = gdp_cend(G, r);
+    prof = row_profile(tab, p, qbuf, gbuf, r);
+  }
+  int Hl = K.INIT_H, El = K.E_INIT, uprev = K.NEG, Hout = K.NEG, Fout = GDP_F_FLOOR;
+  uint32_t word = 0;
+  uint32_t *dout = dirs + p.dir_off;
+
+  for (int s = 0; s < G.nsteps; s++) {
+    int c = s - T - lane;
+    if (r <= G.R && c > ce) {  // this lane's row is done: take row r + 16
+      T += G.P;
+      r += GDP_GROUP;
+      if (r <= G.R) {
+        cs = gdp_cstart(G, r);
+        ce = gdp_cend(G, r);
+        prof = row_profile(tab, p, qbuf, gbuf, r);
+      }
+      c = s - T - lane;
+    }
+    const bool act = (r <= G.R) && (c >= cs) && (c <= ce);
+
+    // Upper neighbour (r-1, c) from the previous step.
+    int Hup = __builtin_amdgcn_update_dpp(Hout, Hout, 0x111, 0xF, 0xF, false);
+    int Fup = FULL ? __builtin_amdgcn_update_dpp(Fout, Fout, 0x111, 0xF, 0xF, false) : GDP_F_FLOOR;
+    if (lane == 0) {
+      uint32_t ent = ring[c & rmask];
+      Hup = (int)(int16_t)(ent & 0xffffu);
+      if (FULL) Fup = (int)(int16_t)(ent >> 16) - 128;
+    }
+    const int Hdg = uprev;  // (r-1, c-1), received one step ago
+    uprev = Hup;
+
+    int H = K.NEG, F = GDP_F_FLOOR, E = El;
+    uint32_t nib = 0;
+    if (act) {
+      int x, xa;
+      if (cached) {
+        const int b = ccls[c];
+        x = b & 15;
+        xa = b >> 4;
+      } else {
+        col_class(tab, p, qbuf, gbuf, c, x, xa);
+      }
+      gdp_cell_k<FULL>(K, G, r, c, cs, Hl, El, Hdg, Hup, Fup, prof, x, xa, H, F, E, nib);
+      if (FULL && r == G.R && c == G.C) final_out[pid] = H;
+      if (lane == GDP_GROUP - 1) ring[c & rmask] = (uint32_t)(H & 0xffff) | ((uint32_t)(F + 128) << 16);
+    }
+    if (keep) scores[p.score_off + (int64_t)s * GDP_GROUP + lane] = (int16_t)(act ? H : 0);
+    Hout = H;
+    if (FULL) Fout = F;
+    Hl = H;
+    El = E;
+    word |= nib << (4 * (s & 7));
+    if ((s & 7) == 7) {
+      dout[(int64_t)(s >> 3) * GDP_GROUP + lane] = word;
+      word = 0;
+    }
+  }
+  if (G.nsteps & 7) dout[(int64_t)(G.nsteps >> 3) * GDP_GROUP + lane] = word;
+  // the next problem rewrites this group's column cache
+  __builtin_amdgcn_wave_barrier();
+}
+
 __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_problem *__restrict__ probs, int n,
                                                              const char *__restrict__ qbuf,
                                                              const char *__restrict__ gbuf, uint32_t *__restrict__ dirs,
                                                              int16_t *__restrict__ scores, int32_t *__restrict__ final_out,
                                                              int ringsz) {
-  // ringsz (a power of two, at least the largest P of the plan's single
-  // sweeps) entries of LDS ring per group, sized per launch
+  // Dynamic LDS per group: ringsz (a power of two, at least the largest P of
+  // the plan's single sweeps) ring entries, then GDP_CCAP column classes.
   __shared__ gdp_tables_lds tab;
   extern __shared__ uint32_t ring_all[];
   // A plan lists its single-sweep problems after its header; a bare
@@ -108,79 +209,16 @@ __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_probl
   const int lane = threadIdx.x & (GDP_GROUP - 1);
   const int grp = threadIdx.x / GDP_GROUP;
   uint32_t *ring = ring_all + grp * ringsz;
-  const int rmask = ringsz - 1;
+  uint8_t *ccls = (uint8_t *)(ring_all + GDP_GROUPS_PER_BLOCK * ringsz) + grp * GDP_CCAP;
   // groups walk the list grid-stride; no barrier follows
   for (int slot = blockIdx.x * GDP_GROUPS_PER_BLOCK + grp; slot < n1; slot += gridDim.x * GDP_GROUPS_PER_BLOCK) {
     const int pid = list1 ? list1[slot] : slot;
     const gdp_dev_problem p = probs[pid];
     if (p.layout != 0) continue;  // swept by gdp_fill2_kernel
-    const gdp_geom G = gdp_make_geom(p);
-    const gdp_consts K = gdp_make_consts(p);
-    const bool full = G.full;
-    const bool keep = p.score_off >= 0;
-
-    // Final cell (rl, gl) of a full-band problem; a single writer per problem.
-    const bool final_in_band = full && G.R <= G.C + G.lb && G.R >= G.C - G.ub;
-    if (lane == 0 && !final_in_band) final_out[pid] = INT_MIN;
-
-    int T = 0, r = lane;
-    int cs = 0, ce = -1;
-    uint32_t prof = 0;
-    if (r <= G.R) {
-      cs = gdp_cstart(G, r);
-      ce = gdp_cend(G, r);
-      prof = row_profile(tab, p, qbuf, gbuf, r);
-    }
-    int Hl = K.INIT_H, El = K.E_INIT, uprev = K.NEG, Hout = K.NEG, Fout = GDP_F_FLOOR;
-    uint32_t word = 0;
-    uint32_t *dout = dirs + p.dir_off;
-
-    for (int s = 0; s < G.nsteps; s++) {
-      int c = s - T - lane;
-      if (r <= G.R && c > ce) {  // this lane's row is done: take row r + 16
-        T += G.P;
-        r += GDP_GROUP;
-        if (r <= G.R) {
-          cs = gdp_cstart(G, r);
-          ce = gdp_cend(G, r);
-          prof = row_profile(tab, p, qbuf, gbuf, r);
-        }
-        c = s - T - lane;
-      }
-      const bool act = (r <= G.R) && (c >= cs) && (c <= ce);
-
-      // Upper neighbour (r-1, c) from the previous step.
-      int Hup = __builtin_amdgcn_update_dpp(Hout, Hout, 0x111, 0xF, 0xF, false);
-      int Fup = __builtin_amdgcn_update_dpp(Fout, Fout, 0x111, 0xF, 0xF, false);
-      if (lane == 0) {
-        uint32_t ent = ring[c & rmask];
-        Hup = (int)(int16_t)(ent & 0xffffu);
-        Fup = (int)(int16_t)(ent >> 16) - 128;
-      }
-      const int Hdg = uprev;  // (r-1, c-1), received one step ago
-      uprev = Hup;
-
-      int H = K.NEG, F = GDP_F_FLOOR, E = El;
-      uint32_t nib = 0;
-      if (act) {
-        int x, xa;
-        col_class(tab, p, qbuf, gbuf, c, x, xa);
-        gdp_cell(K, G, r, c, cs, Hl, El, Hdg, Hup, Fup, prof, x, xa, H, F, E, nib);
-        if (full && r == G.R && c == G.C) final_out[pid] = H;
-        if (lane == GDP_GROUP - 1) ring[c & rmask] = (uint32_t)(H & 0xffff) | ((uint32_t)(F + 128) << 16);
-      }
-      if (keep) scores[p.score_off + (int64_t)s * GDP_GROUP + lane] = (int16_t)(act ? H : 0);
-      Hout = H;
-      Fout = F;
-      Hl = H;
-      El = E;
-      word |= nib << (4 * (s & 7));
-      if ((s & 7) == 7) {
-        dout[(int64_t)(s >> 3) * GDP_GROUP + lane] = word;
-        word = 0;
-      }
-    }
-    if (G.nsteps & 7) dout[(int64_t)(G.nsteps >> 3) * GDP_GROUP + lane] = word;
+    if (p.kind == GDP_KIND_FULL)
+      v1_sweep<true>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
+    else
+      v1_sweep<false>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
   }
 }
 
@@ -783,7 +821,7 @@ extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, con
     const int rlog = (kernels >> 8) & 15;
     const int ringsz = rlog ? 1 << rlog : GDP_RING;
     if (ringsz > GDP_RING) return -1;
-    const size_t lds = (size_t)GDP_GROUPS_PER_BLOCK * ringsz * sizeof(uint32_t);
+    const size_t lds = (size_t)GDP_GROUPS_PER_BLOCK * (ringsz * sizeof(uint32_t) + GDP_CCAP);
     static int cached[64][16] = {{0}};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
