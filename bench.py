@@ -111,9 +111,19 @@ def serving_bench(args):
     lane, table = autotune(tw.problems, tw.qbuf.tobytes(), tw.gbuf.tobytes(), objective="latency")
     del tw
     log("serving lane groups (latency autotuner): g16_max_span=%d g64_min_rows=%d" % lane)
-    srv = serve.Server(dg, dir_budget_bytes=args.serve_budget << 20, max_batch=args.serve_batch,
-                       slots=args.serve_slots, max_rl=1000, max_gl=1500, pair_capacity=args.serve_batch * 700,
-                       plan=lane, long_rows=args.serve_long)
+    knob_table = None
+    if args.serve_batch and args.serve_long >= 0:
+        knob = (args.serve_slots, args.serve_batch, args.serve_long)
+    else:
+        # batch size and length split: the highest saturation whose p99 at half
+        # of it meets 5 ms (measured; see gmapdp.tune.tune_server)
+        from gmapdp.tune import tune_server
+        knob, knob_table = tune_server(dg, reqs, lane, dir_budget_bytes=args.serve_budget << 20, max_rl=1000,
+                                       max_gl=1500)
+        log("serving knobs (slots, max_batch, long_rows): %s" % (knob,))
+    slots, max_batch, long_rows = knob
+    srv = serve.Server(dg, dir_budget_bytes=args.serve_budget << 20, max_batch=max_batch, slots=slots,
+                       max_rl=1000, max_gl=1500, pair_capacity=max_batch * 700, plan=lane, long_rows=long_rows)
     try:
         srv.drive(reqs[:min(nreq, 20000)], 0.0)  # warm-up
         lat, el, ok = srv.drive(reqs, 0.0)
@@ -140,8 +150,10 @@ def serving_bench(args):
             "requests": nreq, "mean_cells_per_request": round(float(cells.mean()), 1),
             "saturation_req_s": round(sat, 1), "saturation_gcups": round(cells.sum() / (nreq / sat) / 1e9, 2),
             "p50_ms_at_50pct": half["p50_ms"], "p99_ms_at_50pct": half["p99_ms"], "points": points,
-            "batches": int(st["batches"]), "max_batch": args.serve_batch, "slots": args.serve_slots,
-            "long_rows": args.serve_long,
+            "batches": int(st["batches"]), "max_batch": max_batch, "slots": slots, "long_rows": long_rows,
+            "knob_candidates": None if knob_table is None else
+            [{"slots": k[0], "max_batch": k[1], "long_rows": k[2], "saturation_req_s": round(sat_, 1),
+              "p99_ms_at_50pct": round(p99_, 3)} for k, sat_, p99_ in knob_table],
             "lane_groups": {"g16_max_span": lane[0], "g64_min_rows": lane[1],
                             "candidates_ms": [[list(p), round(ms, 3)] for p, ms in table]},
             "dir_budget_mib": args.serve_budget, "dir_bytes_peak": int(st["dir_bytes_peak"])}
@@ -173,10 +185,11 @@ def main():
                     help="fixed lane groups (g16_max_span,g64_min_rows) instead of the autotuner's")
     ap.add_argument("--serve-genome", type=int, default=3_100_000_000, help="bases of the device genome")
     ap.add_argument("--serve-requests", type=int, default=300_000)
-    ap.add_argument("--serve-batch", type=int, default=4096, help="largest batch the server forms")
+    ap.add_argument("--serve-batch", type=int, default=0,
+                    help="largest batch the server forms (0: measured choice with --serve-long)")
     ap.add_argument("--serve-slots", type=int, default=2, help="batches in flight (streams)")
-    ap.add_argument("--serve-long", type=int, default=400,
-                    help="reads of at least this many bases are batched apart (0: one class)")
+    ap.add_argument("--serve-long", type=int, default=-1,
+                    help="reads of at least this many bases are batched apart (0: one class; -1: measured)")
     ap.add_argument("--serve-budget", type=int, default=8192, help="direction budget in MiB")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="target duration of each load point")
     args = ap.parse_args()
