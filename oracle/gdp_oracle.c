@@ -712,3 +712,123 @@ long long orc_run_batch(const void *problems, int n, const char *qbuf, const cha
   }
   return check;
 }
+
+/* ------------------------------------------------------- scalar standard */
+
+/* Dynprog_standard (dynprog.c:1267-1785), the scalar int32 fill of the
+   reference's builds without SIMD, for the full band with the upperp /
+   lowerp switches and a saturation floor.  Stated cell by cell:
+     band      rows max(1, c - uband) .. min(rl, c + lband) of column c;
+     row 0     H(0, c) = open + c*extend for c <= uband (nogap/Egap HORIZ),
+     column 0  H(r, 0) = open + r*extend for r <= lband (nogap/Fgap VERT),
+               H(0, 0) = 0 (1316-1341);
+     E         from the clamped H and the E of (r, c-1); a left neighbour
+               outside the band reads its row's initial value (1357-1369,
+               1517-1523);
+     F         from the unclamped H and the F of (r-1, c); the top cell of a
+               column reads F = -inf and H = H(0, c) while the band touches
+               row 0, -inf below, and -inf - open + 1 in column 1
+               (1409-1448, 1485-1491);
+     diagonal  the clamped H(r-1, c-1), except the top cell of a column,
+               which reads H(0, c-1) while the band touches row 0 and the
+               unclamped top cell of the previous column below (1411-1448,
+               1573-1579);
+     H         diagonal + score, then E and F in that order may replace it
+               (1527-1570);
+   scores are stored as max(H, saturation) (1574-1575); the early variant
+   (1587-1785) differs only in its strict comparisons.  Out-of-band cells
+   hold 0 and DIAG (the matrices are cleared per fill, dynprog.c:206-224,
+   487-500).  H (int) and dirs are
+   dense [c][r] with stride rl + 1. */
+void orc_fill_std(const char *rseq, const char *gseq, const char *gseq_alt, int rl, int gl, int mt, int open,
+                  int extend, int lband, int uband, int late, int revp, int saturation, int upperp, int lowerp,
+                  int *H, unsigned char *dirs) {
+  const int NEG = -32768;
+  const size_t ld = (size_t)rl + 1;
+  int *Hu = (int *)calloc((size_t)(gl + 1) * ld, sizeof(int)); /* unclamped H of computed cells */
+  int *E = (int *)calloc((size_t)(gl + 1) * ld, sizeof(int));
+  int *F = (int *)calloc((size_t)(gl + 1) * ld, sizeof(int));
+  memset(H, 0, (size_t)(gl + 1) * ld * sizeof(int));
+  memset(dirs, 0, (size_t)(gl + 1) * ld);
+#define AT(M, cc, rr) (M)[(size_t)(cc) * ld + (rr)]
+  for (int c = 1; c <= uband && c <= gl; c++) {
+    AT(H, c, 0) = open + c * extend;
+    AT(dirs, c, 0) = DIR_H | DIR_E;
+  }
+  for (int r = 1; r <= lband && r <= rl; r++) {
+    AT(H, 0, r) = open + r * extend;
+    AT(dirs, 0, r) = DIR_V | DIR_F;
+  }
+  for (int c = 1; c <= gl; c++) {
+    const int rlo = imax(1, c - uband), rhi = imin(rl, c + lband);
+    const int g = seq_at(gseq, revp, c), ga = seq_at(gseq_alt, revp, c);
+    for (int r = rlo; r <= rhi; r++) {
+      const int q = seq_at(rseq, revp, r);
+      /* left neighbour (r, c-1) */
+      int hl, el;
+      if (c - 1 >= 1 && r <= imin(rl, c - 1 + lband)) {
+        hl = AT(H, c - 1, r);
+        el = AT(E, c - 1, r);
+      } else {
+        hl = r <= lband ? open + r * extend : NEG;
+        el = NEG;
+      }
+      /* upper neighbour (r-1, c) and the diagonal */
+      int hu, fu, dg;
+      if (r > rlo) {
+        hu = AT(Hu, c, r - 1);
+        fu = AT(F, c, r - 1);
+        dg = (c - 1 >= 1) ? AT(H, c - 1, r - 1) : ((r - 1 <= lband) ? open + (r - 1) * extend : NEG);
+      } else {
+        fu = NEG;
+        if (c == 1) {
+          hu = NEG - open + 1;
+          dg = 0;
+        } else if (c <= uband) {
+          hu = open + c * extend;
+          dg = open + (c - 1) * extend;
+        } else if (rlo == 1) {
+          hu = NEG;
+          dg = open + (c - 1) * extend;
+        } else {
+          hu = NEG;
+          dg = AT(Hu, c - 1, r - 1);
+        }
+      }
+      unsigned char d = 0;
+      int e = hl + open, f = hu + open;
+      if (upperp && wins(el, e, late)) {
+        e = el + extend;
+        d |= DIR_E;
+      } else {
+        e = e + extend;
+      }
+      if (lowerp && wins(fu, f, late)) {
+        f = fu + extend;
+        d |= DIR_F;
+      } else {
+        f = f + extend;
+      }
+      int sc = pd[mt][q & 127][g & 127];
+      if (pd[mt][q & 127][ga & 127] > sc) sc = pd[mt][q & 127][ga & 127];
+      int h = dg + sc, nd = 0;
+      if (upperp && wins(e, h, late)) {
+        h = e;
+        nd = DIR_H;
+      }
+      if (lowerp && wins(f, h, late)) {
+        h = f;
+        nd = DIR_V;
+      }
+      AT(Hu, c, r) = h;
+      AT(E, c, r) = e;
+      AT(F, c, r) = f;
+      AT(H, c, r) = h < saturation ? saturation : h;
+      AT(dirs, c, r) = (unsigned char)(d | nd);
+    }
+  }
+#undef AT
+  free(Hu);
+  free(E);
+  free(F);
+}

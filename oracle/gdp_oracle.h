@@ -64,6 +64,13 @@ void orc_fill_lower(int width, const char *rseq, const char *gseq, const char *g
                     int rl, int gl, int mismatchtype, int open, int extend,
                     int lband, int jump_late, int revp, short *H, unsigned char *dirs);
 
+/* Dynprog_standard (dynprog.c:1264-1785): the scalar int32 fill of the
+   reference's builds without SIMD, full band, with the upperp/lowerp
+   switches and a saturation floor.  H (int) and dirs dense [c][r]. */
+void orc_fill_std(const char *rseq, const char *gseq, const char *gseq_alt, int rl, int gl, int mismatchtype,
+                  int open, int extend, int lband, int uband, int jump_late, int revp, int saturation, int upperp,
+                  int lowerp, int *H, unsigned char *dirs);
+
 /* Genome model used for genome-skip characters in the tracebacks
    (pairpool.c:1016-1063 get_genomic_nt over Genome_get_char genome.c:11003).
    codes: 2-bit A=0,C=1,G=2,T=3 one byte per base; nflags: nonzero => 'N'. */

@@ -5,7 +5,9 @@ Two libraries share one plain-array interface:
     (gdp_oracle.c), which also travels to the GPU host;
   * ``Reference`` -- oracle/_ref/libgmapref.so, the reference's own DP
     objects built in place by oracle/Makefile (only where the reference tree
-    exists).
+    exists);
+  * ``ReferenceStd`` -- oracle/_ref_std/libgmapref_std.so, the same objects
+    built without SIMD (Dynprog_standard and Dynprog_traceback_std).
 
 Sequences are passed the way the reference receives them: for ``revp`` the
 pointer addresses the *last* character and the kernels read ``s[1-pos]``
@@ -21,6 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libgmapref.so")
+REF_STD_SO = os.path.join(HERE, "_ref_std", "libgmapref_std.so")
 
 KIND_FULL, KIND_UPPER, KIND_LOWER = 0, 1, 2
 PAIR_FIELDS = 8
@@ -74,6 +77,7 @@ class Oracle(_Common):
         L.orc_traceback_nogaps.argtypes = [_i, _i, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p, _i]
         L.orc_traceback_nogaps.restype = _i
         L.orc_best_endpoint.argtypes = [_i, _p, _p, _i, _i, _i, _i, _i, _i, _p, _p, _p]
+        L.orc_fill_std.argtypes = [_p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]
         L.orc_segment_right.argtypes = [_p, _p, _u, _u, _u, _i]
         L.orc_segment_left.argtypes = [_p, _p, _u, _u, _u, _i]
         for f in ("orc_cells_full",):
@@ -125,6 +129,17 @@ class Oracle(_Common):
         D = np.zeros((rl + 1, gl + 1), np.uint8)
         self.lib.orc_fill_lower(width, q.ptr, g.ptr, ga.ptr, rl, gl, mt, open_, ext, lband,
                                 int(late), int(revp), _np_ptr(H), _np_ptr(D))
+        return H, D
+
+    def fill_std(self, rseq, gseq, galt, mt, open_, ext, lband, uband, late, revp, saturation=-32768,
+                 upperp=True, lowerp=True):
+        """Dynprog_standard: int32 H and nibble dirs, both dense [c][r]."""
+        rl, gl = len(rseq), len(gseq)
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        H = np.zeros((gl + 1, rl + 1), np.int32)
+        D = np.zeros((gl + 1, rl + 1), np.uint8)
+        self.lib.orc_fill_std(q.ptr, g.ptr, ga.ptr, rl, gl, mt, open_, ext, lband, uband, int(late), int(revp),
+                              saturation, int(upperp), int(lowerp), _np_ptr(H), _np_ptr(D))
         return H, D
 
     # traceback ----------------------------------------------------------
@@ -272,6 +287,43 @@ class Reference(_Common):
         n = self.lib.refh_traceback(kind, self._last[kind], r, c, q.ptr, quc.ptr, g.ptr, ga.ptr,
                                     queryoffset, genomeoffset, int(revp), chroffset, chrhigh,
                                     int(watsonp), genestrand, 0, _np_ptr(cnt), _np_ptr(out), maxpairs)
+        return out[:n].copy(), cnt
+
+
+class ReferenceStd(_Common):
+    """The reference's scalar objects (oracle/_ref_std/libgmapref_std.so)."""
+
+    def __init__(self, mode: int = 0, path: str = REF_STD_SO, max_rl: int = 1100, max_gl: int = 2200):
+        self.lib = ctypes.CDLL(path, mode=os.RTLD_LAZY)
+        L = self.lib
+        L.refs_init.argtypes = [_i]
+        L.refs_new.restype = _p
+        L.refs_new.argtypes = [_i, _i]
+        L.refs_fill.argtypes = [_p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]
+        L.refs_traceback.argtypes = [_i, _i, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p, _i]
+        L.refs_traceback.restype = _i
+        L.refs_init(mode)
+        self.ws = L.refs_new(max_rl, max_gl)
+
+    def fill_std(self, rseq, gseq, galt, mt, open_, ext, lband, uband, late, revp, saturation=-32768,
+                 upperp=True, lowerp=True):
+        rl, gl = len(rseq), len(gseq)
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        H = np.zeros((gl + 1, rl + 1), np.int32)
+        D = np.zeros((gl + 1, rl + 1), np.uint8)
+        self.lib.refs_fill(self.ws, q.ptr, g.ptr, ga.ptr, rl, gl, mt, open_, ext, lband, uband, int(late),
+                           int(revp), saturation, int(upperp), int(lowerp), _np_ptr(H), _np_ptr(D))
+        return H, D
+
+    def traceback(self, r, c, rseq, rsequc, gseq, galt, queryoffset=0, genomeoffset=0, revp=False,
+                  genestrand=0, maxpairs=8192):
+        """Dynprog_traceback_std over the last fill (chrhigh 0: genome skips give '*')."""
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        quc = _Seq(rsequc, revp)
+        out = np.zeros((maxpairs, PAIR_FIELDS), np.int32)
+        cnt = np.zeros(5, np.int32)
+        n = self.lib.refs_traceback(r, c, q.ptr, quc.ptr, g.ptr, ga.ptr, queryoffset, genomeoffset, int(revp),
+                                    genestrand, _np_ptr(cnt), _np_ptr(out), maxpairs)
         return out[:n].copy(), cnt
 
 

@@ -11,7 +11,7 @@ for p in (os.path.join(ROOT, "gmap-gsnap_amd"), os.path.join(ROOT, "oracle"), os
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
-    config.addinivalue_line("markers", "ref: needs the reference build under oracle/_ref")
+    config.addinivalue_line("markers", "ref: needs the reference build under oracle/_ref or oracle/_ref_std")
 
 
 @pytest.fixture(scope="session")
@@ -28,3 +28,11 @@ def reference():
     if not os.path.exists(pyoracle.REF_SO):
         pytest.skip("reference build oracle/_ref/libgmapref.so not present on this host")
     return pyoracle.Reference()
+
+
+@pytest.fixture(scope="session")
+def reference_std():
+    import pyoracle
+    if not os.path.exists(pyoracle.REF_STD_SO):
+        pytest.skip("reference build oracle/_ref_std/libgmapref_std.so not present on this host")
+    return pyoracle.ReferenceStd()
