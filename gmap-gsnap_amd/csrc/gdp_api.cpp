@@ -28,11 +28,15 @@ extern "C" int gdp_launch_export(const void *, int, const uint32_t *, const int1
 extern "C" int gdp_launch_export_ref(const void *, int, int, const uint32_t *, const int16_t *, int, void *, void *,
                                      void *, void *, hipStream_t);
 extern "C" int gdp_upload_tables(const uint32_t *, const uint32_t *, const int8_t *, const uint8_t *);
+extern "C" int gdp_upload_std_tables(const uint32_t *, const int8_t *);
 
 namespace {
 thread_local std::string g_err;
 std::mutex g_mu;
 bool g_init = false;
+bool g_std_ok = false;  // the installed tables fit the standard fill's score classes
+uint32_t g_SP[4 * 128 * 4];
+int8_t g_SC[128];
 
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char *fmt, ...) {
@@ -63,6 +67,8 @@ static int upload_current_tables() {
   if (e != hipSuccess || ndev == 0) return fail(GDP_EHIP, "gdp_init: no HIP device (%s)", hipGetErrorString(e));
   if (gdp_upload_tables(PQ, PG, NT, gdp_tables_consistent()) != 0)
     return fail(GDP_EHIP, "gdp_init: table upload failed");
+  g_std_ok = gdp_tables_pack_std(g_SP, g_SC) > 0;
+  if (g_std_ok && gdp_upload_std_tables(g_SP, g_SC) != 0) return fail(GDP_EHIP, "gdp_init: table upload failed");
   g_init = true;
   return GDP_OK;
 }
@@ -83,21 +89,34 @@ extern "C" int gdp_init_tables(const int16_t *pd, const int32_t *nt, const uint8
 extern "C" void gdp_host_tables(int mode) {
   std::lock_guard<std::mutex> lk(g_mu);
   gdp_tables_build(mode);
+  g_std_ok = gdp_tables_pack_std(g_SP, g_SC) > 0;
 }
 
 extern "C" int32_t gdp_descriptor_bytes(void) { return (int32_t)sizeof(gdp_dev_problem); }
 
 static int validate(const gdp_problem &p, int i) {
-  if (p.kind < 0 || p.kind > 2) return fail(GDP_EINVAL, "problem %d: bad kind %d", i, p.kind);
-  if (p.width != 8 && p.width != 16) return fail(GDP_EINVAL, "problem %d: width must be 8 or 16", i);
+  if (p.kind < 0 || p.kind > 3) return fail(GDP_EINVAL, "problem %d: bad kind %d", i, p.kind);
+  if (p.kind == GDP_KIND_STD) {
+    if (p.width != 32) return fail(GDP_EINVAL, "problem %d: the standard fill has width 32", i);
+    if (p.std_flags & ~(GDP_STD_UPPER | GDP_STD_LOWER)) return fail(GDP_EINVAL, "problem %d: bad std_flags", i);
+    if (p.saturation < -32768 || p.saturation > 32767)
+      return fail(GDP_EINVAL, "problem %d: saturation %d outside [-32768, 32767]", i, p.saturation);
+    if (!g_std_ok)
+      return fail(GDP_EINVAL, "problem %d: the installed score tables need more than 32 genome classes", i);
+  } else {
+    if (p.width != 8 && p.width != 16) return fail(GDP_EINVAL, "problem %d: width must be 8 or 16", i);
+    if (p.std_flags != 0 || p.saturation != 0)
+      return fail(GDP_EINVAL, "problem %d: std_flags and saturation belong to GDP_KIND_STD", i);
+  }
   if (p.mismatchtype < 0 || p.mismatchtype > 3) return fail(GDP_EINVAL, "problem %d: bad mismatchtype", i);
   if (p.rl < 1 || p.gl < 1 || p.rl > GDP_MAX_LEN || p.gl > GDP_MAX_LEN)
     return fail(GDP_ETOOBIG, "problem %d: lengths %d x %d outside [1, %d]", i, p.rl, p.gl, GDP_MAX_LEN);
   if (p.lband < 0 || p.uband < 0) return fail(GDP_EINVAL, "problem %d: negative band", i);
   if (p.open > 0 || p.open < -127 || p.extend > 0 || p.extend < -127)
     return fail(GDP_EINVAL, "problem %d: gap penalties must lie in [-127, 0]", i);
-  int span = p.kind == GDP_KIND_FULL ? p.lband + p.uband : (p.kind == GDP_KIND_UPPER ? p.uband : p.lband);
-  if (span > GDP_MAX_SPAN) return fail(GDP_ETOOBIG, "problem %d: band span %d > %d", i, span, GDP_MAX_SPAN);
+  int span = p.kind == GDP_KIND_FULL || p.kind == GDP_KIND_STD ? p.lband + p.uband : (p.kind == GDP_KIND_UPPER ? p.uband : p.lband);
+  const int maxspan = p.kind == GDP_KIND_STD ? GDP_MAX_SPAN_STD : GDP_MAX_SPAN;
+  if (span > maxspan) return fail(GDP_ETOOBIG, "problem %d: band span %d > %d", i, span, maxspan);
   return GDP_OK;
 }
 
@@ -155,6 +174,8 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags
     q.jump_late = p.jump_late ? 1 : 0;
     q.open = p.open;
     q.extend = p.extend;
+    q.saturation = p.saturation;
+    q.std_flags = p.std_flags;
     q.score_off = -1;
     if (!(flags & GDP_PLAN_SINGLE) && pair_lanes(p, pp)) {
       d[i] = q;
@@ -293,7 +314,7 @@ extern "C" int gdp_plan(const gdp_problem *probs, int32_t n, void *descs_out, in
 
 extern "C" int64_t gdp_cells(const gdp_problem *p) {
   int64_t n = 0;
-  if (p->kind == GDP_KIND_FULL) {
+  if (p->kind == GDP_KIND_FULL || p->kind == GDP_KIND_STD) {
     for (int c = 1; c <= p->gl; c++) {
       int lo = gdp_imax(1, c - p->uband), hi = gdp_imin(p->rl, c + p->lband);
       if (hi >= lo) n += hi - lo + 1;
@@ -323,7 +344,9 @@ extern "C" uint32_t gdp_plan_kernels(const void *descs, int32_t n) {
   for (int32_t i = 0; i < n; i++)
     if (d[i].layout == 0) {
       mask |= 1u;
-      pmax = gdp_imax(pmax, gdp_make_geom(d[i]).P);
+      // the standard fill's ring entries take two words (H and F in 32 bits)
+      const int P = gdp_make_geom(d[i]).P;
+      pmax = gdp_imax(pmax, d[i].kind == GDP_KIND_STD ? 2 * P : P);
     }
   if (mask & 1u) {
     // LDS ring of the single sweep: the next power of two >= its largest P
@@ -417,6 +440,8 @@ extern "C" int gdp_export_reference(const void *d_descs, int32_t idx, const uint
   gdp_dev_problem p;
   if (hipMemcpy(&p, (const gdp_dev_problem *)d_descs + idx, sizeof p, hipMemcpyDeviceToHost) != hipSuccess)
     return hip_fail("gdp_export_reference descriptor read", hipGetLastError());
+  if (p.kind == GDP_KIND_STD)
+    return fail(GDP_EINVAL, "gdp_export_reference: the standard fill has no SIMD workspace layout");
   gdp_geom G = gdp_make_geom(p);
   if (stride < G.R + 1) return fail(GDP_EINVAL, "gdp_export_reference: stride %d < %d rows", stride, G.R + 1);
   if (gdp_launch_export_ref(d_descs, idx, p.width, d_dirs, d_scores, stride, d_score, d_nogap, d_egap, d_fgap,

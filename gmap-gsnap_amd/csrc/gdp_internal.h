@@ -48,7 +48,10 @@ struct gdp_dev_problem {
   int32_t sR, sC;   // layout 1: padded sweep extent shared by the wave
   int32_t P;        // layout 1: steps between row batches
   int32_t nsteps;   // layout 1: steps of the sweep (multiple of 8)
-  int32_t pad[5];
+  int32_t saturation;  // GDP_KIND_STD: floor of the stored scores
+  int8_t std_flags;    // GDP_KIND_STD: bit 0 upperp, bit 1 lowerp
+  int8_t pad8[3];
+  int32_t pad[3];
 };
 static_assert(sizeof(gdp_dev_problem) == 96, "gdp_dev_problem must stay 96 bytes");
 static_assert(sizeof(gdp_problem) == 48, "gdp_problem is 48 bytes");
@@ -123,10 +126,11 @@ GDP_HD int gdp_imin(int a, int b) { return a < b ? a : b; }
 
 GDP_HD gdp_geom gdp_make_geom(const gdp_dev_problem &p) {
   gdp_geom g;
-  g.full = p.kind == GDP_KIND_FULL;
+  // the scalar standard fill is a full band without emulated row blocks
+  g.full = p.kind == GDP_KIND_FULL || p.kind == GDP_KIND_STD;
   g.R = p.kind == GDP_KIND_LOWER ? p.gl : p.rl;
   g.C = p.kind == GDP_KIND_LOWER ? p.rl : p.gl;
-  g.S = p.width == 8 ? 32 : 16;
+  g.S = p.kind == GDP_KIND_STD ? 1 : (p.width == 8 ? 32 : 16);
   if (g.full) {
     g.lb = p.lband;
     g.ub = p.uband;
@@ -136,8 +140,11 @@ GDP_HD gdp_geom gdp_make_geom(const gdp_dev_problem &p) {
   }
   // A lane's next row starts P+16 steps after its current row.  The largest
   // span of one row (including the emulated out-of-band prefix of its block)
-  // bounds P from below; 16 keeps the handoff through the ring causal.
-  int span = (g.full ? g.lb + g.ub : g.ub) + g.S - GDP_GROUP;
+  // bounds P from below; 16 keeps the handoff through the ring causal.  The
+  // standard fill adds one step so that lane 0 idles at least one step
+  // before each new row and reads the diagonal of its first cell from the
+  // ring under that row's column.
+  int span = (g.full ? g.lb + g.ub : g.ub) + g.S - GDP_GROUP + (p.kind == GDP_KIND_STD ? 1 : 0);
   g.P = gdp_imax(GDP_GROUP, span);
   int nb = (g.R + GDP_GROUP) / GDP_GROUP;
   g.nsteps = (nb - 1) * g.P + (GDP_GROUP - 1) + g.C + 1;

@@ -34,6 +34,13 @@ __device__ int8_t gdp_tabNT[128];
 __device__ uint8_t gdp_tabCons[3 * 128 * 128];
 // The same as bits: bit g of word [(gs*128 + c)*4 + g/32].
 __device__ uint32_t gdp_tabConsBits[3 * 128 * 4];
+// Scalar standard fill (scores by raw character, dynprog.c:1527-1530):
+//   gdp_tabSC[ch]:     class of genome char ch (characters whose
+//                      pairdistance columns agree share a class, <= 32);
+//   gdp_tabSP[mt][q]:  4-bit scores of query char q against the classes,
+//                      eight per word.
+__device__ int8_t gdp_tabSC[128];
+__device__ uint4 gdp_tabSP[4 * 128];
 
 
 // Packed five-field maximum (lower-triangle profiles take the max over the
@@ -49,6 +56,7 @@ struct gdp_tables_lds {
   uint32_t PQ[4 * 128];
   uint32_t PG[4 * 128];
   int8_t NT[128];
+  int8_t SC[128];
 };
 
 __device__ __forceinline__ void load_tables(gdp_tables_lds &t) {
@@ -56,7 +64,10 @@ __device__ __forceinline__ void load_tables(gdp_tables_lds &t) {
     t.PQ[k] = gdp_tabPQ[k];
     t.PG[k] = gdp_tabPG[k];
   }
-  for (int k = threadIdx.x; k < 128; k += blockDim.x) t.NT[k] = gdp_tabNT[k];
+  for (int k = threadIdx.x; k < 128; k += blockDim.x) {
+    t.NT[k] = gdp_tabNT[k];
+    t.SC[k] = gdp_tabSC[k];
+  }
 }
 
 // Row profile: the five scores of row `row` against the column classes.
@@ -187,6 +198,164 @@ __device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_de
   __builtin_amdgcn_wave_barrier();
 }
 
+// ------------------------------------------------ scalar standard fill
+
+// Score of the row's query character against genome class cls.
+__device__ __forceinline__ int std_score(const uint4 &pf, int cls) {
+  const uint32_t w = cls < 16 ? (cls < 8 ? pf.x : pf.y) : (cls < 24 ? pf.z : pf.w);
+  return gdp_field4(w, cls & 7);
+}
+
+// Dynprog_standard (dynprog.c:1267-1785) for one problem: the int32 fill of
+// the reference's non-SIMD builds, full band, with its upperp / lowerp
+// switches and saturation floor, swept like v1_sweep with rows dealt to the
+// 16 lanes (no emulated row blocks: row r computes columns
+// max(0, r - lb) .. min(gl, r + ub)).  Each lane passes the unclamped H and
+// the F of its cell on; stored scores, and the H that E and the diagonal
+// read, are max(H, saturation), except that row 0 and column 0 keep their
+// initial values unclamped.  Rules, cell by cell:
+//   row 0 / column 0 ......... open + k*extend, HORIZ / VERT (1316-1341);
+//   a row's first cell ....... E and H from the left are -inf (1357-1369);
+//   a column's top cell ...... F from -inf, and H above = H(0, c) while the
+//                              band touches row 0, -inf below it and
+//                              -inf - open + 1 in column 1; the diagonal is
+//                              the unclamped top cell of the previous column
+//                              (1409-1448, 1578);
+//   F, E, then H ............. 1485-1491, 1517-1523, 1527-1570 (jump early:
+//                              1672-1757, strict comparisons).
+__device__ __forceinline__ void v1_sweep_std(const gdp_tables_lds &tab, const gdp_dev_problem &p, int pid,
+                                             const char *__restrict__ qbuf, const char *__restrict__ gbuf,
+                                             uint32_t *__restrict__ dirs, int16_t *__restrict__ scores,
+                                             int32_t *__restrict__ final_out, uint32_t *ring, int ringsz,
+                                             uint8_t *ccls, int lane) {
+  const gdp_geom G = gdp_make_geom(p);
+  const int NEG = -32768, o = p.open, e = p.extend, late = p.jump_late ? 1 : 0, sat = p.saturation;
+  const bool up = (p.std_flags & GDP_STD_UPPER) != 0, lo = (p.std_flags & GDP_STD_LOWER) != 0;
+  const bool keep = p.score_off >= 0;
+  const bool final_in_band = G.R <= G.C + G.lb && G.R >= G.C - G.ub;
+  if (lane == 0 && !final_in_band) final_out[pid] = INT_MIN;
+
+  // genome classes of the columns, two bytes each, for up to GDP_CCAP/2 - 1 columns
+  uint16_t *cc = (uint16_t *)ccls;
+  const bool cached = G.C < GDP_CCAP / 2;
+  if (cached) {
+    for (int c = lane + 1; c <= G.C; c += GDP_GROUP)
+      cc[c] = (uint16_t)((uint8_t)tab.SC[gbuf[p.goff + c - 1] & 127] |
+                         ((uint16_t)(uint8_t)tab.SC[gbuf[p.galtoff + c - 1] & 127] << 8));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  // ring entry per column: (H, F) of the last lane's row, 32 bits each
+  uint2 *ring2 = (uint2 *)ring;
+  const int rmask = ringsz / 2 - 1;
+
+  int T = 0, r = lane, cs = 0, ce = -1;
+  uint4 pf = make_uint4(0, 0, 0, 0);
+  if (r <= G.R) {
+    cs = gdp_cstart(G, r);
+    ce = gdp_cend(G, r);
+    if (r > 0) pf = gdp_tabSP[p.mismatchtype * 128 + (qbuf[p.qoff + r - 1] & 127)];
+  }
+  int Hl = NEG, El = NEG, uprev = NEG, Hout = NEG, Fout = NEG;
+  uint32_t word = 0;
+  uint32_t *dout = dirs + p.dir_off;
+
+  for (int s = 0; s < G.nsteps; s++) {
+    int c = s - T - lane;
+    if (r <= G.R && c > ce) {  // this lane's row is done: take row r + 16
+      T += G.P;
+      r += GDP_GROUP;
+      if (r <= G.R) {
+        cs = gdp_cstart(G, r);
+        ce = gdp_cend(G, r);
+        pf = gdp_tabSP[p.mismatchtype * 128 + (qbuf[p.qoff + r - 1] & 127)];
+      }
+      c = s - T - lane;
+    }
+    const bool act = (r <= G.R) && (c >= cs) && (c <= ce);
+
+    // upper neighbour (r-1, c): the previous lane one step ago, or the ring
+    int Hup = __builtin_amdgcn_update_dpp(Hout, Hout, 0x111, 0xF, 0xF, false);
+    int Fup = __builtin_amdgcn_update_dpp(Fout, Fout, 0x111, 0xF, 0xF, false);
+    if (lane == 0) {
+      const uint2 ent = ring2[c & rmask];
+      Hup = (int)ent.x;
+      Fup = (int)ent.y;
+    }
+    const int Hdg = uprev;  // (r-1, c-1), unclamped
+    uprev = Hup;
+
+    int H = NEG, F = NEG, E = NEG;
+    uint32_t nib = 0;
+    if (act) {
+      if (r == 0) {
+        H = c == 0 ? 0 : o + c * e;
+        nib = c == 0 ? 0u : (uint32_t)(GDP_DIR_H | GDP_DIR_E);
+      } else if (c == 0) {
+        H = o + r * e;
+        nib = GDP_DIR_V | GDP_DIR_F;
+      } else {
+        const bool rowstart = c == cs;  // r > lb: the left neighbour lies outside the band
+        const bool top = r == 1 || c == r + G.ub;
+        const int hl = rowstart ? NEG : (c == 1 ? Hl : gdp_imax(Hl, sat));
+        const int el = rowstart ? NEG : El;
+        int hu, fu, dg;
+        if (top) {
+          fu = NEG;
+          hu = c == 1 ? NEG - o + 1 : (r == 1 ? Hup : NEG);
+          dg = c == 1 ? 0 : Hdg;
+        } else {
+          fu = Fup;
+          hu = Hup;
+          dg = c == 1 ? Hdg : gdp_imax(Hdg, sat);
+        }
+        const int e0 = hl + o, f0 = hu + o;
+        const int dE = up && gdp_wins(el, e0, late);
+        const int dF = lo && gdp_wins(fu, f0, late);
+        E = (dE ? el : e0) + e;
+        F = (dF ? fu : f0) + e;
+        int x, xa;
+        if (cached) {
+          const int b = cc[c];
+          x = b & 255;
+          xa = b >> 8;
+        } else {
+          x = tab.SC[gbuf[p.goff + c - 1] & 127];
+          xa = tab.SC[gbuf[p.galtoff + c - 1] & 127];
+        }
+        int h = dg + gdp_imax(std_score(pf, x), std_score(pf, xa));
+        uint32_t nd = 0;
+        if (up && gdp_wins(E, h, late)) {
+          h = E;
+          nd = GDP_DIR_H;
+        }
+        if (lo && gdp_wins(F, h, late)) {
+          h = F;
+          nd = GDP_DIR_V;
+        }
+        H = h;
+        nib = nd | (dE ? GDP_DIR_E : 0u) | (dF ? GDP_DIR_F : 0u);
+        if (r == G.R && c == G.C) final_out[pid] = gdp_imax(h, sat);
+      }
+      if (lane == GDP_GROUP - 1) ring2[c & rmask] = make_uint2((uint32_t)H, (uint32_t)F);
+    }
+    if (keep)
+      scores[p.score_off + (int64_t)s * GDP_GROUP + lane] = (int16_t)((act && r > 0 && c > 0) ? gdp_imax(H, sat) : 0);
+    Hout = H;
+    Fout = F;
+    Hl = H;
+    El = E;
+    word |= nib << (4 * (s & 7));
+    if ((s & 7) == 7) {
+      dout[(int64_t)(s >> 3) * GDP_GROUP + lane] = word;
+      word = 0;
+    }
+  }
+  if (G.nsteps & 7) dout[(int64_t)(G.nsteps >> 3) * GDP_GROUP + lane] = word;
+  __builtin_amdgcn_wave_barrier();
+}
+
 __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_problem *__restrict__ probs, int n,
                                                              const char *__restrict__ qbuf,
                                                              const char *__restrict__ gbuf, uint32_t *__restrict__ dirs,
@@ -217,6 +386,8 @@ __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_probl
     if (p.layout != 0) continue;  // swept by gdp_fill2_kernel
     if (p.kind == GDP_KIND_FULL)
       v1_sweep<true>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
+    else if (p.kind == GDP_KIND_STD)
+      v1_sweep_std(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz, ccls, lane);
     else
       v1_sweep<false>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
   }
@@ -449,7 +620,9 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
   gdp_pair *const obuf = pairs + q.pair_off;
   int32_t *cnt = counts + 6 * (int64_t)k;
   const gdp_dev_problem *pp = probs + q.fill;
-  const int kind = pp->kind;
+  // the standard fill's directions follow the full band's traceback
+  // (Dynprog_traceback_std, dynprog.c:1795-1947, walks them alike)
+  const int kind = pp->kind == GDP_KIND_STD ? GDP_KIND_FULL : pp->kind;
   const int64_t p_qoff = pp->qoff, p_goff = pp->goff, p_galtoff = pp->galtoff;
   if (q.r < 0 || q.r > pp->rl || q.c < 0 || q.c > pp->gl) {  // start cell outside the matrix
     if (j == 0) cnt[0] = -1;
@@ -806,7 +979,10 @@ static int p2_launch(const void *d_descs, int n, const char *d_qbuf, const char 
       return -1;
     cached[dev] = ncu * (per > 0 ? per : 1);
   }
-  hipLaunchKernelGGL((gdp_fill2_kernel<GENK>), dim3(cached[dev]), dim3(P2_THREADS), L::LDS_BYTES, stream,
+  // resident workgroups, but no more than the waves there can be (a wave
+  // holds at least one of the n problems)
+  const int blocks = gdp_imin(cached[dev], (n + P2_THREADS / 64 - 1) / (P2_THREADS / 64));
+  hipLaunchKernelGGL((gdp_fill2_kernel<GENK>), dim3(blocks), dim3(P2_THREADS), L::LDS_BYTES, stream,
                      (const uint8_t *)d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -877,6 +1053,12 @@ extern "C" int gdp_launch_export(const void *d_descs, int idx, const uint32_t *d
   hipLaunchKernelGGL(gdp_export_kernel, dim3(64), dim3(256), 0, stream, (const gdp_dev_problem *)d_descs, idx, d_dirs,
                      d_scores, d_ddirs, d_dscores);
   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int gdp_upload_std_tables(const uint32_t *SP, const int8_t *SC) {
+  if (hipMemcpyToSymbol(HIP_SYMBOL(gdp_tabSP), SP, sizeof(uint32_t) * 4 * 128 * 4) != hipSuccess) return -1;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(gdp_tabSC), SC, 128) != hipSuccess) return -1;
+  return 0;
 }
 
 extern "C" int gdp_upload_tables(const uint32_t *PQ, const uint32_t *PG, const int8_t *NT, const uint8_t *cons) {

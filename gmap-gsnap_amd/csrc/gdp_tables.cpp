@@ -89,6 +89,42 @@ void gdp_tables_pack(uint32_t *PQ, uint32_t *PG, int8_t *NT) {
   for (int ch = 0; ch < 128; ch++) NT[ch] = (int8_t)nt2int[ch];
 }
 
+// Genome characters whose pairdistance columns agree for every query
+// character and mismatch type score alike; Dynprog_standard looks scores up
+// by the raw characters (dynprog.c:1527-1530), so its fill works on these
+// classes.  SC[ch] = class of genome character ch; SP[(t*128 + q)*4 + w]
+// packs the 4-bit scores of query q against classes 8w .. 8w+7.  Returns the
+// class count, or -1 when the tables need more than 32 classes or a score
+// does not fit 4 signed bits.
+int gdp_tables_pack_std(uint32_t *SP, int8_t *SC) {
+  int ncls = 0, rep[128];
+  for (int g = 0; g < 128; g++) {
+    int k = 0;
+    for (; k < ncls; k++) {
+      bool same = true;
+      for (int t = 0; t < 4 && same; t++)
+        for (int q = 0; q < 128 && same; q++) same = pd[t][q][g] == pd[t][q][rep[k]];
+      if (same) break;
+    }
+    if (k == ncls) {
+      if (ncls == 32) return -1;
+      rep[ncls++] = g;
+    }
+    SC[g] = (int8_t)k;
+  }
+  for (int t = 0; t < 4; t++)
+    for (int q = 0; q < 128; q++) {
+      uint32_t w[4] = {0, 0, 0, 0};
+      for (int k = 0; k < ncls; k++) {
+        const int v = pd[t][q][rep[k]];
+        if (v < -8 || v > 7) return -1;
+        w[k >> 3] |= (uint32_t)(v & 15) << (4 * (k & 7));
+      }
+      for (int j = 0; j < 4; j++) SP[(t * 128 + q) * 4 + j] = w[j];
+    }
+  return ncls;
+}
+
 const uint8_t *gdp_tables_consistent(void) { return &cons[0][0][0]; }
 
 void gdp_tables_set(const int16_t *pd_in, const int32_t *nt_in, const uint8_t *cons_in) {

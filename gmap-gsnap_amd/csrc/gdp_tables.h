@@ -7,6 +7,10 @@ void gdp_tables_build(int mode);
 // Packs 4-bit score fields: PQ[mt*128+ch] field x = score(query ch, letter x),
 // PG[mt*128+ch] field x = score(letter x, genome ch); NT = nucleotide class.
 void gdp_tables_pack(uint32_t *PQ, uint32_t *PG, int8_t *NT);
+// Score classes of genome characters for the scalar standard fill:
+// SP[(mt*128 + q)*4 + w] field k = score(q, class 8w + k), SC[ch] = class.
+// Returns the class count, -1 if the tables need more than 32 or wider scores.
+int gdp_tables_pack_std(uint32_t *SP, int8_t *SC);
 // consistent_array[genestrand][c][g] as 3*128*128 bytes.
 const uint8_t *gdp_tables_consistent(void);
 // Installs tables built elsewhere (the host program's Dynprog_init).

@@ -4,7 +4,8 @@ import os
 
 import numpy as np
 
-GDP_FULL, GDP_UPPER, GDP_LOWER = 0, 1, 2
+GDP_FULL, GDP_UPPER, GDP_LOWER, GDP_STD = 0, 1, 2, 3
+STD_UPPER, STD_LOWER = 1, 2
 HIGHQ, MEDQ, LOWQ, ENDQ = 0, 1, 2, 3
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -19,8 +20,8 @@ class Problem(ctypes.Structure):
                 ("rl", ctypes.c_int32), ("gl", ctypes.c_int32), ("lband", ctypes.c_int16),
                 ("uband", ctypes.c_int16), ("kind", ctypes.c_int8), ("width", ctypes.c_int8),
                 ("mismatchtype", ctypes.c_int8), ("jump_late", ctypes.c_int8), ("open", ctypes.c_int8),
-                ("extend", ctypes.c_int8), ("keep_scores", ctypes.c_int8), ("pad", ctypes.c_int8),
-                ("pad2", ctypes.c_int32)]
+                ("extend", ctypes.c_int8), ("keep_scores", ctypes.c_int8), ("std_flags", ctypes.c_int8),
+                ("saturation", ctypes.c_int32)]
 
 
 class TbRequest(ctypes.Structure):
@@ -40,7 +41,7 @@ class Pair(ctypes.Structure):
 PROBLEM_DTYPE = np.dtype([("qoff", "<i8"), ("goff", "<i8"), ("galtoff", "<i8"), ("rl", "<i4"), ("gl", "<i4"),
                           ("lband", "<i2"), ("uband", "<i2"), ("kind", "i1"), ("width", "i1"),
                           ("mismatchtype", "i1"), ("jump_late", "i1"), ("open", "i1"), ("extend", "i1"),
-                          ("keep_scores", "i1"), ("pad", "i1"), ("pad2", "<i4")])
+                          ("keep_scores", "i1"), ("std_flags", "i1"), ("saturation", "<i4")])
 TBREQ_DTYPE = np.dtype([("fill", "<i4"), ("r", "<i4"), ("c", "<i4"), ("queryoffset", "<i4"),
                         ("genomeoffset", "<i4"), ("chroffset", "<u4"), ("chrhigh", "<u4"), ("watsonp", "i1"),
                         ("revp", "i1"), ("genestrand", "i1"), ("flags", "i1"), ("pair_off", "<i8"),

@@ -41,6 +41,11 @@ extern "C" {
 #define GDP_KIND_FULL 0  /* Dynprog_simd_8 / Dynprog_simd_16 */
 #define GDP_KIND_UPPER 1 /* Dynprog_simd_8_upper / Dynprog_simd_16_upper */
 #define GDP_KIND_LOWER 2 /* Dynprog_simd_8_lower / Dynprog_simd_16_lower */
+#define GDP_KIND_STD 3   /* Dynprog_standard (dynprog.c:1267), width 32 */
+
+/* std_flags of a GDP_KIND_STD problem: Dynprog_standard's upperp / lowerp. */
+#define GDP_STD_UPPER 1
+#define GDP_STD_LOWER 2
 
 #define GDP_HIGHQ 0 /* Mismatchtype_T, dynprog.h:53 */
 #define GDP_MEDQ 1
@@ -49,12 +54,21 @@ extern "C" {
 
 /* Largest band span a problem may use (lband + uband, or the one-sided band). */
 #define GDP_MAX_SPAN 960
+/* GDP_KIND_STD carries 32-bit H and F between row batches, twice the ring
+   space per column, so its band span is capped lower. */
+#define GDP_MAX_SPAN_STD 526
 #define GDP_MAX_LEN 8000
 
 /* One DP problem (48 bytes, pad fields zero).  width 8 or 16 selects the reference's int8 or
    int16 kernel; open/extend are the (negative) gap penalties; jump_late is
    the reference's jump_late_p (ties prefer the gap); keep_scores asks the
-   fill to keep every computed score (needed by gdp_best_endpoint). */
+   fill to keep every computed score (needed by gdp_best_endpoint).
+   GDP_KIND_STD is the reference's scalar fill of its non-SIMD builds
+   (int32 scores, the full band, genome characters scored as they are):
+   width must be 32, std_flags holds upperp / lowerp and saturation is the
+   floor of the stored scores (Dynprog_standard's saturation argument,
+   >= -32768; callers pass NEG_INFINITY_INT = -32768).  Its kept scores are
+   those of the cells with r, c >= 1; row 0 and column 0 read 0. */
 typedef struct gdp_problem {
   int64_t qoff;
   int64_t goff;
@@ -70,8 +84,8 @@ typedef struct gdp_problem {
   int8_t open;
   int8_t extend;
   int8_t keep_scores;
-  int8_t pad;
-  int32_t pad2;
+  int8_t std_flags;   /* GDP_KIND_STD: GDP_STD_UPPER | GDP_STD_LOWER; else 0 */
+  int32_t saturation; /* GDP_KIND_STD: score floor; else 0 */
 } gdp_problem;
 
 /* One traceback request: walk problem `fill` from cell (r, c) (query row,

@@ -53,3 +53,39 @@ def traceback_expect(g, i):
         return None
     off = int(g["tb_off"][i])
     return int(g["tb_r"][i]), int(g["tb_c"][i]), g["tb_pairs"][off:off + n], g["tb_counts"][i]
+
+
+STD_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "std_golden.npz")
+
+
+def load_std():
+    """Scalar standard-fill vectors (make_std_golden.py): specs of kind 3."""
+    z = np.load(STD_PATH, allow_pickle=False)
+    g = {k: z[k] for k in z.files}
+    specs = []
+    for i in range(len(g["rl"])):
+        qo, go, gao = int(g["qoff"][i]), int(g["goff"][i]), int(g["galtoff"][i])
+        rl, gl = int(g["rl"][i]), int(g["gl"][i])
+        specs.append(H.Spec(3, 32, bytes(g["qbuf"][qo:qo + rl]), bytes(g["gbuf"][go:go + gl]),
+                            bytes(g["gbuf"][gao:gao + gl]), int(g["mt"][i]), int(g["open"][i]), int(g["ext"][i]),
+                            int(g["lband"][i]), int(g["uband"][i]), bool(g["late"][i]), bool(g["revp"][i]),
+                            saturation=int(g["saturation"][i]), upperp=bool(g["upperp"][i]),
+                            lowerp=bool(g["lowerp"][i])))
+    return g, specs
+
+
+def check_std(g, i, D, Hs):
+    """D: dense [c][r] direction nibbles (every cell); Hs: scores, of which
+    the cells r, c >= 1 are compared (int32)."""
+    assert digest(D.astype(np.uint8)) + digest(Hs[1:, 1:].astype(np.int32)) == str(g["digests"][i]), \
+        ("std golden digest", i)
+    if g["final"][i] != -(1 << 31):
+        s_gl, s_rl = int(g["gl"][i]), int(g["rl"][i])
+        assert int(Hs[s_gl, s_rl]) == int(g["final"][i]), ("std golden corner", i)
+
+
+def std_traceback_expect(g, i):
+    if g["tb_r"][i] < 0:
+        return None
+    off, n = int(g["tb_off"][i]), int(g["tb_n"][i])
+    return int(g["tb_r"][i]), int(g["tb_c"][i]), g["tb_pairs"][off:off + n], g["tb_counts"][i]

@@ -10,12 +10,15 @@ ALPH = np.frombuffer(b"ACGT", np.uint8)
 class Spec:
     """One DP problem as the reference's callers would pass it."""
 
-    def __init__(self, kind, width, q, g, galt, mt, open_, ext, lband, uband, late, revp, start=None):
+    def __init__(self, kind, width, q, g, galt, mt, open_, ext, lband, uband, late, revp, start=None,
+                 saturation=-32768, upperp=True, lowerp=True):
         self.kind, self.width = kind, width
         self.q, self.g, self.galt = q, g, galt
         self.mt, self.open, self.ext = mt, open_, ext
         self.lband, self.uband, self.late, self.revp = lband, uband, late, revp
         self.start = start  # (r, c) traceback start
+        # kind 3 (Dynprog_standard) only
+        self.saturation, self.upperp, self.lowerp = saturation, upperp, lowerp
 
     @property
     def rl(self):
@@ -82,7 +85,29 @@ def random_specs(seed, n, kinds=(0, 1, 2), widths=(8, 16), maxrl8=60, maxrl16=20
     return specs
 
 
+def random_std_specs(seed, n, maxrl=260):
+    """Problems for the scalar standard fill (kind 3): the shapes of
+    random_specs plus its switches -- upperp / lowerp off, a saturation floor
+    above -inf, narrow and skewed bands (some missing the corner cell)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in random_specs(seed, n, kinds=(0,), widths=(16,), maxrl16=maxrl):
+        lb, ub = s.lband, s.uband
+        u = rng.random()
+        if u < 0.15:
+            lb, ub = int(rng.integers(0, 6)), int(rng.integers(0, 6))
+        elif u < 0.25:
+            lb = ub = min(s.rl + s.gl, 263)
+        out.append(Spec(3, 32, s.q, s.g, s.galt, s.mt, s.open, s.ext, lb, ub, s.late, s.revp,
+                        saturation=int(rng.choice([-32768, -32768, -200, -40, 0])),
+                        upperp=bool(rng.random() > 0.15), lowerp=bool(rng.random() > 0.15)))
+    return out
+
+
 def oracle_fill(O, s):
+    if s.kind == 3:
+        return O.fill_std(s.q, s.g, s.galt, s.mt, s.open, s.ext, s.lband, s.uband, s.late, s.revp, s.saturation,
+                          s.upperp, s.lowerp)
     if s.kind == 0:
         return O.fill_full(s.width, s.q, s.g, s.galt, s.mt, s.open, s.ext, s.lband, s.uband, s.late, s.revp)
     if s.kind == 1:
@@ -91,7 +116,7 @@ def oracle_fill(O, s):
 
 
 def band_mask(s):
-    if s.kind == 0:
+    if s.kind in (0, 3):
         return po.band_mask_full(s.rl, s.gl, s.lband, s.uband)
     if s.kind == 1:
         return po.band_mask_upper(s.rl, s.gl, s.uband)
@@ -102,12 +127,12 @@ def computed_mask(s):
     """Cells the fill computes deterministically, as dense [col][row] of the
     sweep (rows = query for full/upper, genome for lower): the band plus the
     out-of-band prefix of each emulated 32/16-row block."""
-    S = 32 if s.width == 8 else 16
+    S = 1 if s.kind == 3 else (32 if s.width == 8 else 16)
     R, C = (s.gl, s.rl) if s.kind == 2 else (s.rl, s.gl)
     r = np.arange(R + 1)[None, :]
     c = np.arange(C + 1)[:, None]
     blk = (r // S) * S
-    if s.kind == 0:
+    if s.kind in (0, 3):
         cs = np.maximum(0, blk - s.lband)
         ce = np.minimum(C, r + s.uband)
     else:
@@ -129,6 +154,9 @@ def pack_batch(specs, keep_scores=False):
         p["rl"], p["gl"], p["lband"], p["uband"] = s.rl, s.gl, s.lband, s.uband
         p["kind"], p["width"], p["mismatchtype"], p["jump_late"] = s.kind, s.width, s.mt, int(s.late)
         p["open"], p["extend"], p["keep_scores"] = s.open, s.ext, int(keep_scores)
+        if s.kind == 3:
+            p["std_flags"] = int(s.upperp) | (int(s.lowerp) << 1)
+            p["saturation"] = s.saturation
         qparts.append(q)
         gparts.append(g + ga)
         qo += len(q)

@@ -113,6 +113,32 @@ def test_plan_validates_and_sizes(L):
         plan(bad)
 
 
+def test_plan_validates_standard_fills(L):
+    """Kind 3 (Dynprog_standard): width 32, its own flags and floor, a lower
+    span cap; the SIMD kinds must leave those fields zero."""
+    from gmapdp.batch import plan
+    L.gdp_host_tables(0)
+    specs = H.random_std_specs(12, 20)
+    probs, _, _ = H.pack_batch(specs, keep_scores=True)
+    descs, dw, ss = plan(probs)
+    assert dw > 0 and ss > 0
+    assert L.gdp_plan_kernels(descs.ctypes.data, len(probs)) & 1  # single sweeps only
+    cases = [("width", 16, "width 32"), ("std_flags", 4, "std_flags"), ("saturation", -40000, "saturation")]
+    for field, val, msg in cases:
+        bad = probs.copy()
+        bad[field][2] = val
+        with pytest.raises(RuntimeError, match=msg):
+            plan(bad)
+    bad = probs.copy()
+    bad["lband"][0], bad["uband"][0] = 264, 264
+    with pytest.raises(RuntimeError, match="span"):
+        plan(bad)
+    simd, _, _ = H.pack_batch(H.random_specs(13, 4))
+    simd["saturation"][1] = -5
+    with pytest.raises(RuntimeError, match="GDP_KIND_STD"):
+        plan(simd)
+
+
 def test_cells_match_oracle(L, oracle):
     from gmapdp.batch import cells
     specs = H.random_specs(10, 60)

@@ -21,6 +21,57 @@ def test_oracle_matches_reference_golden_vectors(oracle):
     assert ntb > 300
 
 
+def test_oracle_matches_std_golden_vectors(oracle):
+    """Dynprog_standard / Dynprog_traceback_std vectors of the reference's
+    scalar build (tests/golden/make_std_golden.py)."""
+    g, specs = golden_io.load_std()
+    ntb = 0
+    for i, s in enumerate(specs):
+        Hs, D = H.oracle_fill(oracle, s)
+        golden_io.check_std(g, i, D, Hs)
+        exp = golden_io.std_traceback_expect(g, i)
+        if exp is not None:
+            r, c, pairs, counts = exp
+            got, cnt = oracle.traceback(0, D, r, c, s.q, s.q.upper(), s.g, s.galt, 5, 7, s.revp)
+            assert got.shape == pairs.shape and (got == pairs).all(), ("std golden pairs", i)
+            assert (cnt == counts).all(), ("std golden counters", i)
+            ntb += 1
+    assert ntb > 150
+
+
+def test_std_without_gaps_is_the_diagonal_sum(oracle):
+    """With upperp and lowerp off no gap can win, and each in-band cell is a
+    running sum of pair scores along its diagonal, started from the initial
+    row or column value where the diagonal meets row 1 or column 1.  Stored
+    cells are floored at the saturation; the sum carries the floored value
+    forward except along the band's top edge (c - r = uband), whose chain
+    runs unclamped (dynprog.c:1578)."""
+    rng = np.random.default_rng(8)
+    pd = [oracle.pairdistance(mt).astype(np.int64) for mt in range(4)]
+    for _ in range(40):
+        rl, gl = int(rng.integers(1, 60)), int(rng.integers(1, 60))
+        q, g = H.rand_seq(rng, rl, 0.05), H.rand_seq(rng, gl, 0.05)
+        lb, ub = int(rng.integers(0, 70)), int(rng.integers(0, 70))
+        mt, sat = int(rng.integers(0, 4)), int(rng.choice([-32768, -5]))
+        Hs, D = oracle.fill_std(q, g, g, mt, -7, -2, lb, ub, bool(rng.integers(0, 2)), False, sat, False, False)
+        assert not (D[1:, 1:] & 3).any()
+        for c in range(1, gl + 1):
+            for r in range(max(1, c - ub), min(rl, c + lb) + 1):
+                k = min(r, c) - 1
+                r0, c0 = r - k, c - k
+                if r0 == 1 and c0 == 1:
+                    acc = 0
+                elif r0 == 1:
+                    acc = -7 - 2 * (c0 - 1)  # H(0, c0 - 1)
+                else:
+                    acc = -7 - 2 * (r0 - 1)  # H(r0 - 1, 0)
+                for j in range(k + 1):
+                    if j > 0 and c - r != ub:
+                        acc = max(acc, sat)
+                    acc += int(pd[mt][q[r0 + j - 1], g[c0 + j - 1]])
+                assert Hs[c, r] == max(acc, sat), (r, c, lb, ub)
+
+
 def test_traceback_score_is_sum_of_pair_scores(oracle):
     # traceback_score = MATCH*nmatches + MISMATCH*nmismatches + opens*(-3) + indels*(-1) (scores.h:4-9)
     for s in H.random_specs(5, 150, kinds=(0,)):

@@ -15,23 +15,7 @@ pytestmark = pytest.mark.ref
 
 
 def _std_cases(seed, n):
-    """Random problems of all shapes plus the switches the full fill takes:
-    upperp / lowerp off, a saturation floor above -inf, narrow and skewed
-    bands (including bands that miss the corner cell)."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for s in H.random_specs(seed, n, kinds=(0,), widths=(16,), maxrl16=260):
-        lb, ub = s.lband, s.uband
-        u = rng.random()
-        if u < 0.15:
-            lb, ub = int(rng.integers(0, 6)), int(rng.integers(0, 6))
-        elif u < 0.25:
-            lb, ub = s.rl + s.gl, s.rl + s.gl
-        sat = int(rng.choice([-32768, -32768, -200, -40, 0]))
-        upperp = bool(rng.random() > 0.15)
-        lowerp = bool(rng.random() > 0.15)
-        out.append((s, lb, ub, sat, upperp, lowerp))
-    return out
+    return [(s, s.lband, s.uband, s.saturation, s.upperp, s.lowerp) for s in H.random_std_specs(seed, n)]
 
 
 @pytest.mark.parametrize("seed", [11, 12, 13])
