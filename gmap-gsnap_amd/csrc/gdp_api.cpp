@@ -294,7 +294,9 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags
   h.list1_off = off;
   int32_t *list1 = (int32_t *)(base + off);
   for (int32_t i = 0; i < n; i++)
-    if (d[i].layout == 0) list1[h.n1++] = i;
+    if (d[i].layout == 0 && d[i].kind != GDP_KIND_STD) list1[h.n1++] = i;
+  for (int32_t i = 0; i < n; i++)
+    if (d[i].kind == GDP_KIND_STD) list1[h.n1 + h.n1std++] = i;
   memcpy(base + hoff, &h, sizeof h);
   dw += 4;  // the traceback reads direction words in aligned 16-byte windows
   if (dir_words) *dir_words = dw;
@@ -340,20 +342,28 @@ extern "C" uint32_t gdp_plan_kernels(const void *descs, int32_t n) {
   memcpy(&h, (const uint8_t *)descs + (int64_t)n * sizeof(gdp_dev_problem), sizeof h);
   if (h.magic != (GDP_PAIR_MAGIC ^ (uint64_t)n)) return GDP_FILL_ALL;
   uint32_t mask = 0;
-  int pmax = 0;
-  for (int32_t i = 0; i < n; i++)
-    if (d[i].layout == 0) {
+  int pmax = 0, pstd = 0;
+  for (int32_t i = 0; i < n; i++) {
+    if (d[i].layout != 0) continue;
+    const int P = gdp_make_geom(d[i]).P;
+    if (d[i].kind == GDP_KIND_STD) {
+      mask |= GDP_FILL_STD;
+      pstd = gdp_imax(pstd, 2 * P);  // ring entries of two words (H and F in 32 bits)
+    } else {
       mask |= 1u;
-      // the standard fill's ring entries take two words (H and F in 32 bits)
-      const int P = gdp_make_geom(d[i]).P;
-      pmax = gdp_imax(pmax, d[i].kind == GDP_KIND_STD ? 2 * P : P);
+      pmax = gdp_imax(pmax, P);
     }
-  if (mask & 1u) {
-    // LDS ring of the single sweep: the next power of two >= its largest P
-    int rlog = 6;
-    while ((1 << rlog) < pmax) rlog++;
-    mask |= (uint32_t)rlog << 8;
   }
+  // LDS rings of the single sweeps: the next power of two >= the largest P
+  // (in words), bits 8..11 for the SIMD-rule sweep and 12..15 for the
+  // standard fill
+  auto rlog = [](int need) {
+    int k = 6;
+    while ((1 << k) < need) k++;
+    return (uint32_t)k;
+  };
+  if (mask & 1u) mask |= rlog(pmax) << 8;
+  if (mask & GDP_FILL_STD) mask |= rlog(pstd) << 12;
   for (int c = 0; c < GDP_NCLS; c++)
     if (h.nw[c] > 0) mask |= 2u << c;
   return mask;

@@ -182,8 +182,9 @@ void **run_fill(Dynprog_T dynprog, int kind, int width, char *rseq, char *gseq, 
   uint8_t *dg = sl.g.need(2 * (size_t)gl + 16);
   hipcheck(hipMemcpy(dg, g.data(), gl, hipMemcpyHostToDevice), "upload genome");
   hipcheck(hipMemcpy(dg + gl, ga.data(), gl, hipMemcpyHostToDevice), "upload genome alt");
-  if (gdp_fill(sl.desc.p, 1, (const char *)sl.q.p, (const char *)sl.g.p, sl.dirs.need(dw), sl.scores.need(ss),
-               sl.fin.need(1), nullptr) != 0)
+  // only the launch this problem needs (the plan's kernel mask)
+  if (gdp_fill_ex(sl.desc.p, 1, gdp_plan_kernels(desc.data(), 1), (const char *)sl.q.p, (const char *)sl.g.p,
+                  sl.dirs.need(dw), sl.scores.need(ss), sl.fin.need(1), nullptr) != 0)
     die("gdp_fill");
 
   // Reference layout: outer index (columns of the reference's tables) and the

@@ -98,8 +98,8 @@ struct gdp_pair_hdr {
   int32_t gen_queue[4][2];  // the same for the general-lookup pass
   int32_t gen_count[4];     // waves in each general-lookup list
   int64_t sink_off;         // direction words written by the empty slots of partial waves
-  int32_t n1;               // problems swept alone (layout 0)
-  int32_t pad1;
+  int32_t n1;               // problems swept alone (layout 0) by the SIMD-rule single sweep
+  int32_t n1std;            // standard fills, listed after them
   int64_t list1_off;        // byte offset of their int32 index list
   int32_t pad[14];
 };

@@ -108,6 +108,7 @@ __device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_de
                                          uint32_t *__restrict__ dirs, int16_t *__restrict__ scores,
                                          int32_t *__restrict__ final_out, uint32_t *ring, int rmask, uint8_t *ccls,
                                          int lane) {
+  __builtin_assume(p.kind != GDP_KIND_STD);  // swept by v1_sweep_std
   const gdp_geom G = gdp_make_geom(p);
   const gdp_consts K = gdp_make_consts(p);
   const bool keep = p.score_off >= 0;
@@ -200,10 +201,12 @@ __device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_de
 
 // ------------------------------------------------ scalar standard fill
 
-// Score of the row's query character against genome class cls.
-__device__ __forceinline__ int std_score(const uint4 &pf, int cls) {
-  const uint32_t w = cls < 16 ? (cls < 8 ? pf.x : pf.y) : (cls < 24 ? pf.z : pf.w);
-  return gdp_field4(w, cls & 7);
+// Score of the row's query character against genome class cls: the row's
+// 32 four-bit scores as two 64-bit halves (a register pair each; selecting
+// a word of a 4-word vector by index would go through scratch memory).
+__device__ __forceinline__ int std_score(uint64_t lo, uint64_t hi, int cls) {
+  const uint64_t v = cls < 16 ? lo : hi;
+  return ((int)(uint32_t)(v >> (4 * (cls & 15))) << 28) >> 28;
 }
 
 // Dynprog_standard (dynprog.c:1267-1785) for one problem: the int32 fill of
@@ -251,11 +254,15 @@ __device__ __forceinline__ void v1_sweep_std(const gdp_tables_lds &tab, const gd
   const int rmask = ringsz / 2 - 1;
 
   int T = 0, r = lane, cs = 0, ce = -1;
-  uint4 pf = make_uint4(0, 0, 0, 0);
+  uint64_t plo = 0, phi = 0;  // the row's scores against classes 0-15 and 16-31
   if (r <= G.R) {
     cs = gdp_cstart(G, r);
     ce = gdp_cend(G, r);
-    if (r > 0) pf = gdp_tabSP[p.mismatchtype * 128 + (qbuf[p.qoff + r - 1] & 127)];
+    if (r > 0) {
+      const uint4 pf = gdp_tabSP[p.mismatchtype * 128 + (qbuf[p.qoff + r - 1] & 127)];
+      plo = pf.x | ((uint64_t)pf.y << 32);
+      phi = pf.z | ((uint64_t)pf.w << 32);
+    }
   }
   int Hl = NEG, El = NEG, uprev = NEG, Hout = NEG, Fout = NEG;
   uint32_t word = 0;
@@ -269,7 +276,9 @@ __device__ __forceinline__ void v1_sweep_std(const gdp_tables_lds &tab, const gd
       if (r <= G.R) {
         cs = gdp_cstart(G, r);
         ce = gdp_cend(G, r);
-        pf = gdp_tabSP[p.mismatchtype * 128 + (qbuf[p.qoff + r - 1] & 127)];
+        const uint4 pf = gdp_tabSP[p.mismatchtype * 128 + (qbuf[p.qoff + r - 1] & 127)];
+        plo = pf.x | ((uint64_t)pf.y << 32);
+        phi = pf.z | ((uint64_t)pf.w << 32);
       }
       c = s - T - lane;
     }
@@ -324,7 +333,7 @@ __device__ __forceinline__ void v1_sweep_std(const gdp_tables_lds &tab, const gd
           x = tab.SC[gbuf[p.goff + c - 1] & 127];
           xa = tab.SC[gbuf[p.galtoff + c - 1] & 127];
         }
-        int h = dg + gdp_imax(std_score(pf, x), std_score(pf, xa));
+        int h = dg + gdp_imax(std_score(plo, phi, x), std_score(plo, phi, xa));
         uint32_t nd = 0;
         if (up && gdp_wins(E, h, late)) {
           h = E;
@@ -356,22 +365,28 @@ __device__ __forceinline__ void v1_sweep_std(const gdp_tables_lds &tab, const gd
   __builtin_amdgcn_wave_barrier();
 }
 
-__global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_problem *__restrict__ probs, int n,
-                                                             const char *__restrict__ qbuf,
-                                                             const char *__restrict__ gbuf, uint32_t *__restrict__ dirs,
-                                                             int16_t *__restrict__ scores, int32_t *__restrict__ final_out,
-                                                             int ringsz) {
-  // Dynamic LDS per group: ringsz (a power of two, at least the largest P of
-  // the plan's single sweeps) ring entries, then GDP_CCAP column classes.
+// The single-sweep launches.  STD = false sweeps the SIMD-rule problems
+// (gdp_fill_kernel), STD = true the standard fills (gdp_fill_std_kernel); the
+// two are separate kernels so that neither carries the other's registers.
+// Dynamic LDS per group: ringsz ring words (a power of two, at least the
+// largest batch period of the plan's problems of that launch, two words per
+// column for the standard fill), then GDP_CCAP bytes of column classes.
+template <bool STD>
+__device__ __forceinline__ void single_sweeps(const gdp_dev_problem *__restrict__ probs, int n,
+                                              const char *__restrict__ qbuf, const char *__restrict__ gbuf,
+                                              uint32_t *__restrict__ dirs, int16_t *__restrict__ scores,
+                                              int32_t *__restrict__ final_out, int ringsz) {
   __shared__ gdp_tables_lds tab;
   extern __shared__ uint32_t ring_all[];
-  // A plan lists its single-sweep problems after its header; a bare
-  // descriptor array (one problem of a GDP_PLAN_SINGLE plan) has no header.
+  // A plan lists its single-sweep problems after its header (the SIMD-rule
+  // ones, then the standard fills); a bare descriptor array (one problem of a
+  // GDP_PLAN_SINGLE plan) has no header and is walked whole.
   const gdp_pair_hdr *hdr = (const gdp_pair_hdr *)(probs + n);
   const bool listed = hdr->magic == (GDP_PAIR_MAGIC ^ (uint64_t)n) && hdr->n == n;
-  const int n1 = listed ? hdr->n1 : n;
+  const int n1 = listed ? (STD ? hdr->n1std : hdr->n1) : n;
   if (n1 == 0) return;
-  const int32_t *list1 = listed ? (const int32_t *)((const uint8_t *)probs + hdr->list1_off) : nullptr;
+  const int32_t *list1 =
+      listed ? (const int32_t *)((const uint8_t *)probs + hdr->list1_off) + (STD ? hdr->n1 : 0) : nullptr;
   load_tables(tab);
   __syncthreads();
 
@@ -383,14 +398,31 @@ __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_probl
   for (int slot = blockIdx.x * GDP_GROUPS_PER_BLOCK + grp; slot < n1; slot += gridDim.x * GDP_GROUPS_PER_BLOCK) {
     const int pid = list1 ? list1[slot] : slot;
     const gdp_dev_problem p = probs[pid];
-    if (p.layout != 0) continue;  // swept by gdp_fill2_kernel
-    if (p.kind == GDP_KIND_FULL)
-      v1_sweep<true>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
-    else if (p.kind == GDP_KIND_STD)
+    if (p.layout != 0 || (p.kind == GDP_KIND_STD) != STD) continue;  // another launch sweeps it
+    if (STD)
       v1_sweep_std(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz, ccls, lane);
+    else if (p.kind == GDP_KIND_FULL)
+      v1_sweep<true>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
     else
       v1_sweep<false>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
   }
+}
+
+__global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_problem *__restrict__ probs, int n,
+                                                             const char *__restrict__ qbuf,
+                                                             const char *__restrict__ gbuf, uint32_t *__restrict__ dirs,
+                                                             int16_t *__restrict__ scores, int32_t *__restrict__ final_out,
+                                                             int ringsz) {
+  single_sweeps<false>(probs, n, qbuf, gbuf, dirs, scores, final_out, ringsz);
+}
+
+__global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_std_kernel(const gdp_dev_problem *__restrict__ probs, int n,
+                                                                 const char *__restrict__ qbuf,
+                                                                 const char *__restrict__ gbuf,
+                                                                 uint32_t *__restrict__ dirs,
+                                                                 int16_t *__restrict__ scores,
+                                                                 int32_t *__restrict__ final_out, int ringsz) {
+  single_sweeps<true>(probs, n, qbuf, gbuf, dirs, scores, final_out, ringsz);
 }
 
 // ------------------------------------------------------------------ lookups
@@ -987,32 +1019,35 @@ static int p2_launch(const void *d_descs, int n, const char *d_qbuf, const char 
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// kernels: bit 0 single sweeps, bits 1..3 the paired classes (gdp_plan_kernels).
+// kernels: bit 0 single sweeps, bits 1..3 the paired classes, bit 4 standard
+// fills (gdp_plan_kernels).
 extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, const char *d_qbuf, const char *d_gbuf,
                                uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, hipStream_t stream) {
   if (n <= 0) return 0;
-  if (kernels & 1u) {
-    // resident blocks only; they walk the plan's list of single sweeps with
-    // the ring size the plan asks for (GDP_RING when it does not say)
-    const int rlog = (kernels >> 8) & 15;
+  // resident blocks only; they walk the plan's list of single sweeps with
+  // the ring size the plan asks for (GDP_RING when it does not say)
+  auto single = [&](bool std_fill, int rlog) -> int {
     const int ringsz = rlog ? 1 << rlog : GDP_RING;
     if (ringsz > GDP_RING) return -1;
     const size_t lds = (size_t)GDP_GROUPS_PER_BLOCK * (ringsz * sizeof(uint32_t) + GDP_CCAP);
-    static int cached[64][16] = {{0}};
+    static int cached[64][2][16] = {{{0}}};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
     const int slot = rlog ? rlog : 10;
-    if (cached[dev][slot] == 0) {
+    auto kern = std_fill ? gdp_fill_std_kernel : gdp_fill_kernel;
+    if (cached[dev][std_fill][slot] == 0) {
       int ncu = 0, per = 0;
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill_kernel, GDP_BLOCK, lds) != hipSuccess) return -1;
-      cached[dev][slot] = ncu * (per > 0 ? per : 1);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, GDP_BLOCK, lds) != hipSuccess) return -1;
+      cached[dev][std_fill][slot] = ncu * (per > 0 ? per : 1);
     }
-    const int blocks = gdp_imin(cached[dev][slot], (n + GDP_GROUPS_PER_BLOCK - 1) / GDP_GROUPS_PER_BLOCK);
-    hipLaunchKernelGGL(gdp_fill_kernel, dim3(blocks), dim3(GDP_BLOCK), lds, stream, (const gdp_dev_problem *)d_descs, n,
+    const int blocks = gdp_imin(cached[dev][std_fill][slot], (n + GDP_GROUPS_PER_BLOCK - 1) / GDP_GROUPS_PER_BLOCK);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(GDP_BLOCK), lds, stream, (const gdp_dev_problem *)d_descs, n,
                        d_qbuf, d_gbuf, d_dirs, d_scores, d_final, ringsz);
-    if (hipGetLastError() != hipSuccess) return -1;
-  }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+  };
+  if ((kernels & 1u) && single(false, (kernels >> 8) & 15)) return -1;
+  if ((kernels & GDP_FILL_STD) && single(true, (kernels >> 12) & 15)) return -1;
   if (kernels & 0xEu) {
     // all paired classes in one persistent launch, then the general-lookup pass
     if (p2_launch<false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;

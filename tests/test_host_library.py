@@ -122,7 +122,9 @@ def test_plan_validates_standard_fills(L):
     probs, _, _ = H.pack_batch(specs, keep_scores=True)
     descs, dw, ss = plan(probs)
     assert dw > 0 and ss > 0
-    assert L.gdp_plan_kernels(descs.ctypes.data, len(probs)) & 1  # single sweeps only
+    k = L.gdp_plan_kernels(descs.ctypes.data, len(probs))
+    assert k & 0x10 and not k & 0xF  # the standard-fill launch only
+    assert 6 <= (k >> 12) & 15 <= 10  # its ring: two words per column
     cases = [("width", 16, "width 32"), ("std_flags", 4, "std_flags"), ("saturation", -40000, "saturation")]
     for field, val, msg in cases:
         bad = probs.copy()
