@@ -112,6 +112,10 @@ static int validate(const gdp_problem &p, int i) {
   if (p.rl < 1 || p.gl < 1 || p.rl > GDP_MAX_LEN || p.gl > GDP_MAX_LEN)
     return fail(GDP_ETOOBIG, "problem %d: lengths %d x %d outside [1, %d]", i, p.rl, p.gl, GDP_MAX_LEN);
   if (p.lband < 0 || p.uband < 0) return fail(GDP_EINVAL, "problem %d: negative band", i);
+  if (p.keep_scores < 0 || p.keep_scores > 3) return fail(GDP_EINVAL, "problem %d: keep_scores must be 0-3", i);
+  if (p.keep_scores >= 2 && p.kind != GDP_KIND_UPPER && p.kind != GDP_KIND_LOWER)
+    return fail(GDP_EINVAL, "problem %d: keep_scores %d (best end cell) needs an upper or lower fill", i,
+                p.keep_scores);
   if (p.open > 0 || p.open < -127 || p.extend > 0 || p.extend < -127)
     return fail(GDP_EINVAL, "problem %d: gap penalties must lie in [-127, 0]", i);
   int span = p.kind == GDP_KIND_FULL || p.kind == GDP_KIND_STD ? p.lband + p.uband : (p.kind == GDP_KIND_UPPER ? p.uband : p.lband);
@@ -176,6 +180,7 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags
     q.extend = p.extend;
     q.saturation = p.saturation;
     q.std_flags = p.std_flags;
+    q.keep_mode = p.keep_scores;
     q.score_off = -1;
     if (!(flags & GDP_PLAN_SINGLE) && pair_lanes(p, pp)) {
       d[i] = q;
@@ -185,9 +190,12 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags
     gdp_geom G = gdp_make_geom(q);
     q.dir_off = dw;
     dw += gdp_dir_words(G);
-    if (p.keep_scores) {
+    if (p.keep_scores == 1) {
       q.score_off = ss;
       ss += (int64_t)G.nsteps * GDP_GROUP;
+    } else if (p.keep_scores >= 2) {
+      q.score_off = ss;  // the best end cell's record: score, r, c
+      ss += GDP_BEST_SLOTS;
     }
     d[i] = q;
   }

@@ -50,7 +50,8 @@ struct gdp_dev_problem {
   int32_t nsteps;   // layout 1: steps of the sweep (multiple of 8)
   int32_t saturation;  // GDP_KIND_STD: floor of the stored scores
   int8_t std_flags;    // GDP_KIND_STD: bit 0 upperp, bit 1 lowerp
-  int8_t pad8[3];
+  int8_t keep_mode;    // gdp_problem.keep_scores: 1 every score, 2/3 the best end cell only
+  int8_t pad8[2];
   int32_t pad[3];
 };
 static_assert(sizeof(gdp_dev_problem) == 96, "gdp_dev_problem must stay 96 bytes");

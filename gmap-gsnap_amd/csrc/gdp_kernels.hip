@@ -70,6 +70,16 @@ __device__ __forceinline__ void load_tables(gdp_tables_lds &t) {
   }
 }
 
+// Does candidate (v, r, c) replace the best (bv, br, bc) of an end scan?  The
+// reference's scan (dynprog_end.c:143-290, 358-508) runs in row-major order
+// and keeps the first maximum (the last under jump-late), so any partition
+// of the cells can be scanned apart and merged by value, then by position.
+__device__ __forceinline__ bool gdp_best_before(int v, int r, int c, int bv, int br, int bc, int late) {
+  if (v != bv) return v > bv;
+  const bool later = r != br ? r > br : c > bc;
+  return late ? later : !later;
+}
+
 // Row profile: the five scores of row `row` against the column classes.
 __device__ __forceinline__ uint32_t row_profile(const gdp_tables_lds &t, const gdp_dev_problem &p, const char *qbuf,
                                                 const char *gbuf, int row) {
@@ -111,7 +121,37 @@ __device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_de
   __builtin_assume(p.kind != GDP_KIND_STD);  // swept by v1_sweep_std
   const gdp_geom G = gdp_make_geom(p);
   const gdp_consts K = gdp_make_consts(p);
-  const bool keep = p.score_off >= 0;
+  const bool keep = p.score_off >= 0 && p.keep_mode == GDP_KEEP_ALL;
+  // GDP_KEEP_BEST(_QUERYEND): this triangle's part of the end scan.  A lane
+  // visits a sweep row's cells in column order, which is scan order within
+  // the row (the lower fill's sweep row is one genome column, its columns
+  // query rows), so within a row the scan's own rule applies (> or >=); the
+  // row's best (rbv at sweep column rbc) joins the lane's best (query row br,
+  // genome column bc) by value and position when the lane moves on, and the
+  // group's lanes merge at the end.  A row's eligible sweep columns are its
+  // computed columns from emin on: c >= r for the upper fill, c > r for the
+  // lower, and for queryend_indels only query row rl (the upper fill's row
+  // rl, the lower fill's last column).
+  const int bmode = FULL ? 0 : p.keep_mode;
+  const bool upper = p.kind == GDP_KIND_UPPER;
+  const int qrl = upper ? G.R : G.C;  // query length
+  int bv = bmode == GDP_KEEP_BEST_QUERYEND ? K.NEG : 0, br = bmode == GDP_KEEP_BEST_QUERYEND ? qrl : 0, bc = 0;
+  int rbv = INT_MIN, rbc = 0, emin = 1 << 30;
+  auto best_row = [&](int row) {  // eligible columns of sweep row `row`
+    emin = upper ? row : row + 1;
+    if (bmode == GDP_KEEP_BEST_QUERYEND) emin = upper ? (row == qrl ? row : 1 << 30) : gdp_imax(emin, qrl);
+    if (row < 1) emin = 1 << 30;
+    rbv = INT_MIN;
+  };
+  auto best_merge = [&](int row) {  // the row's best into the lane's
+    if (rbv == INT_MIN) return;
+    const int qr = upper ? row : rbc, qc = upper ? rbc : row;
+    if (gdp_best_before(rbv, qr, qc, bv, br, bc, K.late)) {
+      bv = rbv;
+      br = qr;
+      bc = qc;
+    }
+  };
 
   // Final cell (rl, gl) of a full-band problem; a single writer per problem.
   const bool final_in_band = FULL && G.R <= G.C + G.lb && G.R >= G.C - G.ub;
@@ -139,6 +179,7 @@ __device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_de
     ce = gdp_cend(G, r);
     prof = row_profile(tab, p, qbuf, gbuf, r);
   }
+  if (!FULL && bmode >= GDP_KEEP_BEST) best_row(r);
   int Hl = K.INIT_H, El = K.E_INIT, uprev = K.NEG, Hout = K.NEG, Fout = GDP_F_FLOOR;
   uint32_t word = 0;
   uint32_t *dout = dirs + p.dir_off;
@@ -146,6 +187,10 @@ __device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_de
   for (int s = 0; s < G.nsteps; s++) {
     int c = s - T - lane;
     if (r <= G.R && c > ce) {  // this lane's row is done: take row r + 16
+      if (!FULL && bmode >= GDP_KEEP_BEST) {
+        best_merge(r);
+        best_row(r + GDP_GROUP);
+      }
       T += G.P;
       r += GDP_GROUP;
       if (r <= G.R) {
@@ -181,6 +226,11 @@ __device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_de
       }
       gdp_cell_k<FULL>(K, G, r, c, cs, Hl, El, Hdg, Hup, Fup, prof, x, xa, H, F, E, nib);
       if (FULL && r == G.R && c == G.C) final_out[pid] = H;
+      // the scan's rule within the row: > (>= under jump-late)
+      if (!FULL && bmode >= GDP_KEEP_BEST && c >= emin && H + K.late > rbv) {
+        rbv = H;
+        rbc = c;
+      }
       if (lane == GDP_GROUP - 1) ring[c & rmask] = (uint32_t)(H & 0xffff) | ((uint32_t)(F + 128) << 16);
     }
     if (keep) scores[p.score_off + (int64_t)s * GDP_GROUP + lane] = (int16_t)(act ? H : 0);
@@ -195,6 +245,18 @@ __device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_de
     }
   }
   if (G.nsteps & 7) dout[(int64_t)(G.nsteps >> 3) * GDP_GROUP + lane] = word;
+  if (!FULL && bmode >= GDP_KEEP_BEST) {
+    if (r <= G.R) best_merge(r);
+    for (int d = 1; d < GDP_GROUP; d <<= 1) {
+      const int ov = __shfl_xor(bv, d, 64), orr = __shfl_xor(br, d, 64), oc = __shfl_xor(bc, d, 64);
+      if (gdp_best_before(ov, orr, oc, bv, br, bc, K.late)) {
+        bv = ov;
+        br = orr;
+        bc = oc;
+      }
+    }
+    if (lane < GDP_BEST_SLOTS) scores[p.score_off + lane] = (int16_t)(lane == 0 ? bv : lane == 1 ? br : lane == 2 ? bc : 0);
+  }
   // the next problem rewrites this group's column cache
   __builtin_amdgcn_wave_barrier();
 }
@@ -408,7 +470,9 @@ __device__ __forceinline__ void single_sweeps(const gdp_dev_problem *__restrict_
   }
 }
 
-__global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_kernel(const gdp_dev_problem *__restrict__ probs, int n,
+// Five waves per SIMD: the register budget (96) that holds the sweep without
+// spills.
+__global__ __launch_bounds__(GDP_BLOCK, 5) void gdp_fill_kernel(const gdp_dev_problem *__restrict__ probs, int n,
                                                              const char *__restrict__ qbuf,
                                                              const char *__restrict__ gbuf, uint32_t *__restrict__ dirs,
                                                              int16_t *__restrict__ scores, int32_t *__restrict__ final_out,
@@ -876,13 +940,6 @@ __global__ __launch_bounds__(256) void gdp_traceback_kernel(
 // jump-late) occurrence of the maximum, so TBG lanes can scan disjoint rows
 // (columns of the single row for queryend_indels) and merge by value, then
 // by position.  One group of TBG lanes per (upper, lower) pair.
-__device__ __forceinline__ bool gdp_best_before(int v, int r, int c, int bv, int br, int bc, int late) {
-  // does candidate (v, r, c) replace (bv, br, bc)?
-  if (v != bv) return v > bv;
-  const bool later = r != br ? r > br : c > bc;
-  return late ? later : !later;
-}
-
 __global__ __launch_bounds__(256) void gdp_best_endpoint_kernel(
     const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ ul, int m, int queryend_indels,
     const int16_t *__restrict__ scores, int32_t *__restrict__ out, const gdp_tb_request *__restrict__ tmpl,
@@ -892,10 +949,23 @@ __global__ __launch_bounds__(256) void gdp_best_endpoint_kernel(
   if (k >= m) return;  // whole groups leave together
   const gdp_dev_problem pu = probs[ul[2 * k]], pl = probs[ul[2 * k + 1]];
   int best, bestr, bestc;
-  if (pu.score_off < 0 || pl.score_off < 0 || pu.kind != GDP_KIND_UPPER || pl.kind != GDP_KIND_LOWER) {
-    best = INT_MIN;  // scores were not kept, or the pair is not (upper, lower)
+  const int want = queryend_indels ? GDP_KEEP_BEST_QUERYEND : GDP_KEEP_BEST;
+  if (pu.score_off < 0 || pl.score_off < 0 || pu.kind != GDP_KIND_UPPER || pl.kind != GDP_KIND_LOWER ||
+      pu.keep_mode != pl.keep_mode || (pu.keep_mode != GDP_KEEP_ALL && pu.keep_mode != want)) {
+    best = INT_MIN;  // scores were not kept (or kept for the other scan), or the pair is not (upper, lower)
     bestr = -1;
     bestc = -1;
+  } else if (pu.keep_mode == want) {
+    // the fills scanned their triangles: merge the two records
+    const int16_t *a = scores + pu.score_off, *b = scores + pl.score_off;
+    best = a[0];
+    bestr = a[1];
+    bestc = a[2];
+    if (gdp_best_before(b[0], b[1], b[2], best, bestr, bestc, pu.jump_late)) {
+      best = b[0];
+      bestr = b[1];
+      bestc = b[2];
+    }
   } else {
     const gdp_geom Gu = gdp_make_geom(pu), Gl = gdp_make_geom(pl);
     const int rl = pu.rl, gl = pu.gl, lband = pl.lband, uband = pu.uband, late = pu.jump_late;
@@ -952,7 +1022,8 @@ __global__ void gdp_export_kernel(const gdp_dev_problem *__restrict__ probs, int
     const int col = (int)(k / (G.R + 1)), row = (int)(k % (G.R + 1));
     const bool comp = col >= gdp_cstart(G, row) && col <= gdp_cend(G, row);
     ddirs[k] = (uint8_t)(comp ? nib_at(dirs, p, G, row, col) : 0);
-    if (dscores != nullptr) dscores[k] = (int16_t)((comp && p.score_off >= 0) ? score_at(scores, p, G, row, col) : 0);
+    if (dscores != nullptr)
+      dscores[k] = (int16_t)((comp && p.score_off >= 0 && p.keep_mode == GDP_KEEP_ALL) ? score_at(scores, p, G, row, col) : 0);
   }
 }
 
@@ -969,7 +1040,7 @@ __global__ void gdp_export_ref_kernel(const gdp_dev_problem *__restrict__ probs,
     const int col = (int)(k / stride), row = (int)(k % stride);
     const bool comp = row <= G.R && col >= gdp_cstart(G, row) && col <= gdp_cend(G, row);
     const uint32_t nib = comp ? nib_at(dirs, p, G, row, col) : 0;
-    sc[k] = (T)((comp && p.score_off >= 0) ? score_at(scores, p, G, row, col) : 0);
+    sc[k] = (T)((comp && p.score_off >= 0 && p.keep_mode == GDP_KEEP_ALL) ? score_at(scores, p, G, row, col) : 0);
     nogap[k] = (T)((nib & GDP_DIR_V) ? -2 : ((nib & GDP_DIR_H) ? -1 : 0));
     egap[k] = (T)((nib & GDP_DIR_E) ? -1 : 0);
     if (fgap != nullptr) fgap[k] = (T)((nib & GDP_DIR_F) ? -2 : 0);

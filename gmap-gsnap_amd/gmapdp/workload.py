@@ -83,10 +83,16 @@ class EndMix:
     """Config 4: end-extension problems (an upper and a lower fill each,
     best end cell over both, traceback in the triangle holding it) mixed 1:1
     with config-3 single-gap problems, over one genome.  problems[:n3] are
-    the single-gap fills, problems[n3:] the (upper, lower) pairs."""
+    the single-gap fills, problems[n3:] the (upper, lower) pairs.
 
-    def __init__(self, n_end, n_single, genome_len=1 << 24):
+    keep: keep_scores of the end fills -- 2 (GDP_KEEP_BEST, the default: each
+    fill hands its triangle's best end cell to gdp_best_endpoint, for the
+    BEST_LOCAL / QUERYEND_GAP scan), 3 (the same for queryend_indels), or 1
+    (every score kept and scanned afterwards)."""
+
+    def __init__(self, n_end, n_single, genome_len=1 << 24, keep=2):
         e = Workload(4, n_end, genome_len=genome_len)
+        e.problems["keep_scores"] = keep
         s = Workload(3, n_single, genome_len=genome_len, genome_seed=SEEDS[4])
         self.n3, self.m = n_single, n_end
         self.genome_len = genome_len

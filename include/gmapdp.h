@@ -52,6 +52,13 @@ extern "C" {
 #define GDP_LOWQ 2
 #define GDP_ENDQ 3
 
+/* keep_scores values (gdp_problem) */
+#define GDP_KEEP_NONE 0
+#define GDP_KEEP_ALL 1
+#define GDP_KEEP_BEST 2          /* best end cell, find_best_endpoint_{8,16} */
+#define GDP_KEEP_BEST_QUERYEND 3 /* best end cell, ..._to_queryend_indels_{8,16} */
+#define GDP_BEST_SLOTS 4         /* int16 score slots of a best-cell record: score, r, c, 0 */
+
 /* Largest band span a problem may use (lband + uband, or the one-sided band). */
 #define GDP_MAX_SPAN 960
 /* GDP_KIND_STD carries 32-bit H and F between row batches, twice the ring
@@ -61,8 +68,15 @@ extern "C" {
 
 /* One DP problem (48 bytes, pad fields zero).  width 8 or 16 selects the reference's int8 or
    int16 kernel; open/extend are the (negative) gap penalties; jump_late is
-   the reference's jump_late_p (ties prefer the gap); keep_scores asks the
-   fill to keep every computed score (needed by gdp_best_endpoint).
+   the reference's jump_late_p (ties prefer the gap); keep_scores = 1 asks
+   the fill to keep every computed score (gdp_best_endpoint can scan them,
+   and gdp_export_* return them).  For the one-sided fills of end problems,
+   keep_scores = GDP_KEEP_BEST (or GDP_KEEP_BEST_QUERYEND) keeps only the
+   fill's part of gdp_best_endpoint's scan, computed during the fill: the
+   best cell of its triangle (the upper fill's cells with c >= r, the lower
+   fill's with c < r) in row-major order under the problem's tie rule, from
+   score 0 at (0, 0) (or, for queryend_indels, -inf at (rl, 0) over row rl
+   only).  It takes GDP_BEST_SLOTS score slots instead of one per cell.
    GDP_KIND_STD is the reference's scalar fill of its non-SIMD builds
    (int32 scores, the full band, genome characters scored as they are):
    width must be 32, std_flags holds upperp / lowerp and saturation is the
@@ -259,7 +273,11 @@ void gdp_graph_destroy(gdp_graph *g);
    one-sided fills with keep_scores set, find_best_endpoint_{8,16}
    (dynprog_end.c:143-290) when queryend_indels = 0, or
    find_best_endpoint_to_queryend_indels_{8,16} (358-508) otherwise.
-   d_out[3*k] = {finalscore, bestr, bestc}. */
+   d_out[3*k] = {finalscore, bestr, bestc}.  Fills kept with GDP_KEEP_ALL are
+   scanned; fills kept with GDP_KEEP_BEST (queryend_indels = 0) or
+   GDP_KEEP_BEST_QUERYEND (queryend_indels = 1) hand over their triangle's
+   best cell and the pair's two are merged.  A pair whose fills kept nothing,
+   or kept the other mode, gets finalscore INT32_MIN and cell (-1, -1). */
 int gdp_best_endpoint(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,
                       const int16_t *d_scores, int32_t *d_out, void *stream);
 

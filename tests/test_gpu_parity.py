@@ -150,6 +150,46 @@ def test_end_extension_pipeline(oracle):
             assert (H.pairs_to_records(pairs[j]) == po).all() and (counts[j, 1:] == co).all(), ("end traceback", j)
 
 
+@pytest.mark.parametrize("qi", [False, True])
+def test_end_best_cell_kept_during_the_fill(oracle, qi):
+    """keep_scores 2 / 3: each one-sided fill keeps its triangle's best end
+    cell; the merged pair equals the scan of fully kept scores and the
+    oracle's find_best_endpoint, and the fills' directions are unchanged."""
+    from gmapdp.batch import DeviceBatch
+    from gmapdp.workload import Workload
+    w = Workload(4, 400, genome_len=1 << 20)
+    ul = np.stack([np.arange(0, len(w.problems), 2), np.arange(1, len(w.problems), 2)], 1).astype(np.int32)
+    full = DeviceBatch(w.problems, w.qbuf.tobytes(), w.gbuf.tobytes())
+    full.fill()
+    want = full.best_endpoint(ul, queryend_indels=qi)
+    p = w.problems.copy()
+    p["keep_scores"] = 3 if qi else 2
+    b = DeviceBatch(p, w.qbuf.tobytes(), w.gbuf.tobytes())
+    b.fill()
+    got = b.best_endpoint(ul, queryend_indels=qi)
+    assert (got == want).all(), np.argwhere((got != want).any(1))[:5]
+    other = b.best_endpoint(ul, queryend_indels=not qi)  # kept for the other scan
+    assert (other[:, 0] == np.iinfo(np.int32).min).all()
+    for k in range(0, len(ul), 37):
+        pu = p[ul[k, 0]]
+        revp = bool(w.meta["revp"][ul[k, 0]])
+        q = bytes(w.qbuf[pu["qoff"]: pu["qoff"] + pu["rl"]])
+        g = bytes(w.gbuf[pu["goff"]: pu["goff"] + pu["gl"]])
+        if revp:
+            q, g = q[::-1], g[::-1]
+        su = H.Spec(1, int(pu["width"]), q, g, g, 3, int(pu["open"]), int(pu["extend"]), int(pu["lband"]),
+                    int(pu["uband"]), bool(pu["jump_late"]), revp)
+        sl = H.Spec(2, su.width, q, g, g, 3, su.open, su.ext, su.lband, su.uband, su.late, revp)
+        Hu, Du = H.oracle_fill(oracle, su)
+        Hl, Dl = H.oracle_fill(oracle, sl)
+        assert tuple(got[k]) == oracle.best_endpoint(qi, Hu, Hl, su.width, su.rl, su.gl, su.lband, su.uband,
+                                                     su.late), ("best end cell", k)
+        for i, s_, D in ((ul[k, 0], su, Du), (ul[k, 1], sl, Dl)):
+            Dg, _ = b.export(int(i))
+            B = H.band_mask(s_)
+            assert (Dg[B] == D[B]).all(), ("directions", k)
+
+
 def test_traceback_nogaps(oracle):
     # traceback_nogaps (dynprog_end.c:648-722) from min(rl, gl) on both axes
     specs = [s for s in H.random_specs(21, 200, kinds=(0,))]

@@ -111,6 +111,13 @@ def test_plan_validates_and_sizes(L):
     bad["open"][1] = 5
     with pytest.raises(RuntimeError, match="penalties"):
         plan(bad)
+    bad = probs.copy()
+    bad["kind"][4], bad["keep_scores"][4] = 0, 2  # the best end cell belongs to one-sided fills
+    with pytest.raises(RuntimeError, match="upper or lower"):
+        plan(bad)
+    one = probs.copy()
+    one["kind"], one["keep_scores"] = 1, 2
+    assert plan(one)[2] == 4 * len(one)  # four score slots per best-cell record
 
 
 def test_plan_validates_standard_fills(L):
