@@ -300,11 +300,27 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags
   h.sink_off = sink_off;
   // problems swept alone, for the single-sweep launch to walk
   h.list1_off = off;
+  // The four 16-lane groups of a wave take consecutive list entries and run
+  // their sweeps in lockstep until the longest one ends; the list is sorted
+  // by kind, width and sweep length (longest first) so that they match.
   int32_t *list1 = (int32_t *)(base + off);
   for (int32_t i = 0; i < n; i++)
     if (d[i].layout == 0 && d[i].kind != GDP_KIND_STD) list1[h.n1++] = i;
   for (int32_t i = 0; i < n; i++)
     if (d[i].kind == GDP_KIND_STD) list1[h.n1 + h.n1std++] = i;
+  {
+    std::vector<int32_t> steps(n, 0);
+    for (int32_t i = 0; i < n; i++)
+      if (d[i].layout == 0) steps[i] = gdp_make_geom(d[i]).nsteps;
+    auto order = [&](int32_t a, int32_t b) {
+      if (d[a].kind != d[b].kind) return d[a].kind < d[b].kind;
+      if (d[a].width != d[b].width) return d[a].width < d[b].width;
+      if (steps[a] != steps[b]) return steps[a] > steps[b];
+      return a < b;
+    };
+    std::sort(list1, list1 + h.n1, order);
+    std::sort(list1 + h.n1, list1 + h.n1 + h.n1std, order);
+  }
   memcpy(base + hoff, &h, sizeof h);
   dw += 4;  // the traceback reads direction words in aligned 16-byte windows
   if (dir_words) *dir_words = dw;

@@ -156,7 +156,7 @@ GDP_HD gdp_geom gdp_make_geom(const gdp_dev_problem &p) {
 // where the row's emulated block starts (dynprog_simd.c:3221-3222); the
 // one-sided triangles start at the block's first row (7853).
 GDP_HD int gdp_cstart(const gdp_geom &g, int r) {
-  int blk = (r / g.S) * g.S;
+  int blk = r & -g.S;  // S is 1, 16 or 32
   return g.full ? gdp_imax(0, blk - g.lb) : blk;
 }
 GDP_HD int gdp_cend(const gdp_geom &g, int r) { return gdp_imin(g.C, r + g.ub); }

@@ -80,16 +80,24 @@ __device__ __forceinline__ bool gdp_best_before(int v, int r, int c, int bv, int
   return late ? later : !later;
 }
 
-// Row profile: the five scores of row `row` against the column classes.
-__device__ __forceinline__ uint32_t row_profile(const gdp_tables_lds &t, const gdp_dev_problem &p, const char *qbuf,
-                                                const char *gbuf, int row) {
+// The characters that score row `row`: the query character, or for the
+// lower triangle (rows are genome positions) the genome character and its
+// alternate (g | ga << 8).  Rows past the last read nothing.
+__device__ __forceinline__ uint32_t row_chars(const gdp_dev_problem &p, const gdp_geom &G, const char *qbuf,
+                                              const char *gbuf, int row) {
+  if (row < 1 || row > G.R) return 0;
+  if (p.kind != GDP_KIND_LOWER) return (uint32_t)(qbuf[p.qoff + row - 1] & 127);
+  return (uint32_t)(gbuf[p.goff + row - 1] & 127) | ((uint32_t)(gbuf[p.galtoff + row - 1] & 127) << 8);
+}
+
+// Row profile: the five scores of row `row` (characters `ch` from row_chars)
+// against the column classes.
+__device__ __forceinline__ uint32_t row_profile(const gdp_tables_lds &t, const gdp_dev_problem &p, uint32_t ch,
+                                                int row) {
   const int mt = p.mismatchtype;
-  if (p.kind != GDP_KIND_LOWER) {
-    int ch = row == 0 ? 'N' : (qbuf[p.qoff + row - 1] & 127);
-    return t.PQ[mt * 128 + ch];
-  }
+  if (p.kind != GDP_KIND_LOWER) return t.PQ[mt * 128 + (row == 0 ? 'N' : (int)ch)];
   if (row == 0) return t.PG[mt * 128 + (p.width == 8 ? 4 : 'N')];
-  int g = gbuf[p.goff + row - 1] & 127, ga = gbuf[p.galtoff + row - 1] & 127;
+  const int g = ch & 127, ga = (ch >> 8) & 127;
   uint32_t a = t.PG[mt * 128 + g];
   return ga == g ? a : fieldmax(a, t.PG[mt * 128 + ga]);
 }
@@ -177,8 +185,11 @@ __device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_de
   if (r <= G.R) {
     cs = gdp_cstart(G, r);
     ce = gdp_cend(G, r);
-    prof = row_profile(tab, p, qbuf, gbuf, r);
+    prof = row_profile(tab, p, row_chars(p, G, qbuf, gbuf, r), r);
   }
+  // the characters of the lane's next row, loaded a row ahead so that a row
+  // switch waits on no global load
+  uint32_t nxt = row_chars(p, G, qbuf, gbuf, r + GDP_GROUP);
   if (!FULL && bmode >= GDP_KEEP_BEST) best_row(r);
   int Hl = K.INIT_H, El = K.E_INIT, uprev = K.NEG, Hout = K.NEG, Fout = GDP_F_FLOOR;
   uint32_t word = 0;
@@ -196,7 +207,8 @@ __device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_de
       if (r <= G.R) {
         cs = gdp_cstart(G, r);
         ce = gdp_cend(G, r);
-        prof = row_profile(tab, p, qbuf, gbuf, r);
+        prof = row_profile(tab, p, nxt, r);
+        nxt = row_chars(p, G, qbuf, gbuf, r + GDP_GROUP);
       }
       c = s - T - lane;
     }
