@@ -120,13 +120,15 @@ __device__ __forceinline__ void col_class(const gdp_tables_lds &t, const gdp_dev
 #define GDP_CCAP 512
 
 // One problem of the single sweep, the band kind fixed at compile time.
-template <bool FULL>
-__device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_dev_problem &p, int pid,
+template <int KIND>
+__device__ __forceinline__ void v1_sweep(const gdp_tables_lds &tab, const gdp_dev_problem &pin, int pid,
                                          const char *__restrict__ qbuf, const char *__restrict__ gbuf,
                                          uint32_t *__restrict__ dirs, int16_t *__restrict__ scores,
                                          int32_t *__restrict__ final_out, uint32_t *ring, int rmask, uint8_t *ccls,
                                          int lane) {
-  __builtin_assume(p.kind != GDP_KIND_STD);  // swept by v1_sweep_std
+  constexpr bool FULL = KIND == GDP_KIND_FULL;
+  gdp_dev_problem p = pin;
+  p.kind = KIND;  // known at compile time from here on
   const gdp_geom G = gdp_make_geom(p);
   const gdp_consts K = gdp_make_consts(p);
   const bool keep = p.score_off >= 0 && p.keep_mode == GDP_KEEP_ALL;
@@ -473,12 +475,16 @@ __device__ __forceinline__ void single_sweeps(const gdp_dev_problem *__restrict_
     const int pid = list1 ? list1[slot] : slot;
     const gdp_dev_problem p = probs[pid];
     if (p.layout != 0 || (p.kind == GDP_KIND_STD) != STD) continue;  // another launch sweeps it
+    // one loop per band kind; the plan's list keeps each kind together, so
+    // the groups of a wave run the same one
     if (STD)
       v1_sweep_std(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz, ccls, lane);
     else if (p.kind == GDP_KIND_FULL)
-      v1_sweep<true>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
+      v1_sweep<GDP_KIND_FULL>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
+    else if (p.kind == GDP_KIND_UPPER)
+      v1_sweep<GDP_KIND_UPPER>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
     else
-      v1_sweep<false>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
+      v1_sweep<GDP_KIND_LOWER>(tab, p, pid, qbuf, gbuf, dirs, scores, final_out, ring, ringsz - 1, ccls, lane);
   }
 }
 
