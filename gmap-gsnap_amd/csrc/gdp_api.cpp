@@ -126,7 +126,7 @@ static int validate(const gdp_problem &p, int i) {
 
 extern "C" int64_t gdp_plan_bytes(int32_t n) {
   if (n < 0) n = 0;
-  return (int64_t)n * (int64_t)(sizeof(gdp_dev_problem) + sizeof(gdp_wave) + 2 * sizeof(int32_t)) +
+  return (int64_t)n * (int64_t)(sizeof(gdp_dev_problem) + sizeof(gdp_wave) + 3 * sizeof(int32_t)) +
          (int64_t)sizeof(gdp_pair_hdr);
 }
 
@@ -189,7 +189,17 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags
     }
     gdp_geom G = gdp_make_geom(q);
     q.dir_off = dw;
-    dw += gdp_dir_words(G);
+    const int nstride = (p.kind == GDP_KIND_UPPER || p.kind == GDP_KIND_LOWER) && p.keep_scores != GDP_KEEP_ALL &&
+                                !(flags & GDP_PLAN_SINGLE)
+                            ? gdp_narrow_stride(G.ub)
+                            : 0;
+    if (nstride) {  // one lane sweeps its in-band cells: (R + 1) rows of nstride nibbles
+      q.layout = 2;
+      q.P = nstride;
+      dw += (int64_t)(G.R + 1) * (nstride / 8);
+    } else {
+      dw += gdp_dir_words(G);
+    }
     if (p.keep_scores == 1) {
       q.score_off = ss;
       ss += (int64_t)G.nsteps * GDP_GROUP;
@@ -320,6 +330,20 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags
     };
     std::sort(list1, list1 + h.n1, order);
     std::sort(list1 + h.n1, list1 + h.n1 + h.n1std, order);
+    off += (int64_t)(h.n1 + h.n1std) * (int64_t)sizeof(int32_t);
+    // narrow sweeps: a wave takes 64 consecutive entries, one per lane, and
+    // runs until its longest one ends; sorted by row stride, then rows
+    h.list2_off = off;
+    int32_t *list2 = (int32_t *)(base + off);
+    for (int32_t i = 0; i < n; i++)
+      if (d[i].layout == 2) list2[h.n2++] = i;
+    std::sort(list2, list2 + h.n2, [&](int32_t a, int32_t b) {
+      if (d[a].P != d[b].P) return d[a].P > d[b].P;
+      const int ra = d[a].kind == GDP_KIND_LOWER ? d[a].gl : d[a].rl;
+      const int rb = d[b].kind == GDP_KIND_LOWER ? d[b].gl : d[b].rl;
+      if (ra != rb) return ra > rb;
+      return a < b;
+    });
   }
   memcpy(base + hoff, &h, sizeof h);
   dw += 4;  // the traceback reads direction words in aligned 16-byte windows
@@ -368,6 +392,7 @@ extern "C" uint32_t gdp_plan_kernels(const void *descs, int32_t n) {
   uint32_t mask = 0;
   int pmax = 0, pstd = 0;
   for (int32_t i = 0; i < n; i++) {
+    if (d[i].layout == 2) mask |= GDP_FILL_NARROW;
     if (d[i].layout != 0) continue;
     const int P = gdp_make_geom(d[i]).P;
     if (d[i].kind == GDP_KIND_STD) {

@@ -25,6 +25,12 @@
 // clamped here to keep it in 16 bits (+128 bias) in the LDS ring.
 #define GDP_F_FLOOR (-32896)
 
+#if defined(__HIPCC__)
+#define GDP_HD __host__ __device__ __forceinline__
+#else
+#define GDP_HD inline
+#endif
+
 // Device copy of one problem (96 bytes).  Row/column roles: for the full band
 // and the upper triangle rows are query positions r = 0..rl and columns are
 // genome positions c = 0..gl; the lower triangle is swept transposed (rows are
@@ -34,7 +40,10 @@
 // layout 0: the problem is swept alone by a 16-lane group (gdp_fill_kernel);
 // layout 1: it is one half of a lane pair swept together with problems of the
 // same band and tie rule (gdp_fill2_kernel), over the padded sweep sR x sC
-// with batch period P and 2^glog lanes per group.
+// with batch period P and 2^glog lanes per group;
+// layout 2: a one-sided fill with a band under 32 columns, swept by one lane
+// (gdp_fill_narrow_kernel) over its in-band cells only; P is its row stride
+// in direction nibbles (8, 16 or 32).
 struct gdp_dev_problem {
   int64_t qoff;     // q(r) = qbuf[qoff + r - 1], r = 1..rl (already in sweep order)
   int64_t goff;     // g(c) = gbuf[goff + c - 1]
@@ -102,9 +111,16 @@ struct gdp_pair_hdr {
   int32_t n1;               // problems swept alone (layout 0) by the SIMD-rule single sweep
   int32_t n1std;            // standard fills, listed after them
   int64_t list1_off;        // byte offset of their int32 index list
-  int32_t pad[14];
+  int32_t n2;               // narrow one-sided sweeps (layout 2)
+  int32_t pad2;
+  int64_t list2_off;        // byte offset of their int32 index list
+  int32_t pad[10];
 };
 static_assert(sizeof(gdp_pair_hdr) == 256, "gdp_pair_hdr must stay 256 bytes");
+
+// Row stride (direction nibbles) of a narrow sweep over band width w
+// (uband of an upper fill, lband of a lower one), 0 when w + 1 > 32.
+GDP_HD int gdp_narrow_stride(int w) { return w < 8 ? 8 : (w < 16 ? 16 : (w < 32 ? 32 : 0)); }
 
 // Sweep geometry shared by the fill, traceback, export and scan kernels.
 struct gdp_geom {
@@ -116,11 +132,6 @@ struct gdp_geom {
   int nsteps;      // steps the sweep takes
 };
 
-#if defined(__HIPCC__)
-#define GDP_HD __host__ __device__ __forceinline__
-#else
-#define GDP_HD inline
-#endif
 
 GDP_HD int gdp_imax(int a, int b) { return a > b ? a : b; }
 GDP_HD int gdp_imin(int a, int b) { return a < b ? a : b; }
@@ -210,7 +221,7 @@ GDP_HD gdp_consts gdp_make_consts(const gdp_dev_problem &p) {
   return k;
 }
 
-GDP_HD int gdp_clampw(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+GDP_HD int gdp_clampw(int x, int lo, int hi) { return gdp_imin(gdp_imax(x, lo), hi); }
 GDP_HD int gdp_wins(int a, int b, int late) { return late ? (a >= b) : (a > b); }
 GDP_HD int gdp_field4(uint32_t w, int x) { return ((int)(w << (28 - 4 * x))) >> 28; }
 

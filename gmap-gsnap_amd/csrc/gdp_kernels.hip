@@ -86,8 +86,8 @@ __device__ __forceinline__ bool gdp_best_before(int v, int r, int c, int bv, int
 __device__ __forceinline__ uint32_t row_chars(const gdp_dev_problem &p, const gdp_geom &G, const char *qbuf,
                                               const char *gbuf, int row) {
   if (row < 1 || row > G.R) return 0;
-  if (p.kind != GDP_KIND_LOWER) return (uint32_t)(qbuf[p.qoff + row - 1] & 127);
-  return (uint32_t)(gbuf[p.goff + row - 1] & 127) | ((uint32_t)(gbuf[p.galtoff + row - 1] & 127) << 8);
+  if (p.kind != GDP_KIND_LOWER) return (uint32_t)(uint8_t)qbuf[p.qoff + row - 1];
+  return (uint32_t)(uint8_t)gbuf[p.goff + row - 1] | ((uint32_t)(uint8_t)gbuf[p.galtoff + row - 1] << 8);
 }
 
 // Row profile: the five scores of row `row` (characters `ch` from row_chars)
@@ -95,7 +95,7 @@ __device__ __forceinline__ uint32_t row_chars(const gdp_dev_problem &p, const gd
 __device__ __forceinline__ uint32_t row_profile(const gdp_tables_lds &t, const gdp_dev_problem &p, uint32_t ch,
                                                 int row) {
   const int mt = p.mismatchtype;
-  if (p.kind != GDP_KIND_LOWER) return t.PQ[mt * 128 + (row == 0 ? 'N' : (int)ch)];
+  if (p.kind != GDP_KIND_LOWER) return t.PQ[mt * 128 + (row == 0 ? 'N' : (int)(ch & 127))];
   if (row == 0) return t.PG[mt * 128 + (p.width == 8 ? 4 : 'N')];
   const int g = ch & 127, ga = (ch >> 8) & 127;
   uint32_t a = t.PG[mt * 128 + g];
@@ -507,12 +507,150 @@ __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_std_kernel(const gdp_dev_p
   single_sweeps<true>(probs, n, qbuf, gbuf, dirs, scores, final_out, ringsz);
 }
 
+// ------------------------------------------------------------ narrow sweep
+
+// One-sided fills (layout 2) with a band under 32 columns, one problem per
+// lane.  The upper fill's E is -inf on and below the diagonal and its
+// diagonal cells take the diagonal (dynprog_simd.c:4478-4626, 7867-7960; the
+// lower fill the same, transposed), so in-band cells never read the
+// emulated out-of-band prefix of a row block: a lane computes row r's
+// in-band cells c = r .. r + ub only, left to right, with the same cell
+// rule (gdp_cell_k) and the block-start seeds of the row's block.
+// Per lane, in LDS laid out lane-minor (no bank conflicts): the row above,
+// where cell r + k reads the diagonal H(r-1, r-1+k) from slot k and
+// overwrites it, and a ring of column class bytes (genome class |
+// alternate << 4) that gains one column per row.  Directions go out as P
+// nibbles per row, one word per 8 cells.
+#define GDP_NARROW_BW 32    // widest row slot (cells) of a narrow sweep
+#define GDP_NARROW_RING 64  // column ring per lane
+template <int BW>
+__device__ __forceinline__ void narrow_sweep(const gdp_tables_lds &tab, const gdp_dev_problem &p,
+                                             const char *__restrict__ qbuf, const char *__restrict__ gbuf,
+                                             uint32_t *__restrict__ dirs, int16_t *__restrict__ scores,
+                                             int16_t *hrow, uint8_t *cring, int lane) {
+  const gdp_geom G = gdp_make_geom(p);
+  const gdp_consts K = gdp_make_consts(p);
+  const bool upper = p.kind == GDP_KIND_UPPER;
+  const int B = G.ub, R = G.R, C = G.C, nw = p.P / 8;
+  uint32_t *dout = dirs + p.dir_off;
+  // the characters of column col (the upper fill's columns are genome
+  // positions, the lower fill's query positions), then their class byte
+  auto col_raw = [&](int col) -> uint32_t {
+    if (col < 1 || col > C) return 0u;
+    if (upper) return (uint32_t)(uint8_t)gbuf[p.goff + col - 1] | ((uint32_t)(uint8_t)gbuf[p.galtoff + col - 1] << 8);
+    const uint32_t q = (uint8_t)qbuf[p.qoff + col - 1];
+    return q | (q << 8);
+  };
+  auto col_cls = [&](int col, uint32_t raw) -> uint8_t {
+    if (col == 0) return (uint8_t)(4u | (4u << 4));
+    return (uint8_t)((uint32_t)tab.NT[raw & 127] | ((uint32_t)tab.NT[(raw >> 8) & 127] << 4));
+  };
+#pragma unroll 1
+  for (int c = 0; c < BW; c++) cring[(c & (GDP_NARROW_RING - 1)) * 64 + lane] = col_cls(c, col_raw(c));
+
+  // best end cell (GDP_KEEP_BEST / _QUERYEND): the upper fill is visited in
+  // scan order; the lower fill's rows are genome columns, so each row's best
+  // joins the lane's by value and position
+  const int bmode = p.keep_mode;
+  const int qrl = upper ? R : C;
+  int bv = bmode == GDP_KEEP_BEST_QUERYEND ? K.NEG : 0, br = bmode == GDP_KEEP_BEST_QUERYEND ? qrl : 0, bc = 0;
+
+  uint32_t ch = row_chars(p, G, qbuf, gbuf, 0), chn = row_chars(p, G, qbuf, gbuf, 1);
+  for (int r = 0; r <= R; r++) {
+    const uint32_t prof = row_profile(tab, p, ch, r);
+    ch = chn;
+    chn = row_chars(p, G, qbuf, gbuf, r + 2);
+    const uint32_t nextraw = col_raw(r + BW);  // joins the ring after this row
+    const int cs = r & -G.S;
+    const int nact = gdp_imin(B, C - r) + 1;  // in-band cells of this row
+    // cells eligible for the end scan: the upper fill's (c >= r, row >= 1;
+    // queryend_indels: row rl), the lower fill's with query row c > r
+    // (queryend_indels: c = rl)
+    const bool rowok = r >= 1 && bmode >= GDP_KEEP_BEST && (!upper || bmode == GDP_KEEP_BEST || r == R);
+    int Hl = K.INIT_H, El = K.E_INIT, rbv = INT_MIN, rbk = 0;
+#pragma unroll 1
+    for (int kk = 0; kk < BW / 8; kk++) {
+      uint32_t word = 0;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int k = 8 * kk + j, c = r + k;
+        const bool act = k < nact;
+        const uint32_t cb = cring[(c & (GDP_NARROW_RING - 1)) * 64 + lane];
+        const int hd = hrow[k * 64 + lane];
+        int H, F, E;
+        uint32_t nib;
+        gdp_cell_k<false>(K, G, r, c, cs, Hl, El, hd, 0, 0, prof, (int)(cb & 15u), (int)(cb >> 4), H, F, E, nib);
+        Hl = H;
+        El = E;
+        hrow[k * 64 + lane] = (int16_t)(act ? H : hd);
+        word |= (act ? nib : 0u) << (4 * j);
+        const bool el = act && rowok && (upper || (k >= 1 && (bmode == GDP_KEEP_BEST || c == C)));
+        const bool upd = el && H + K.late > rbv;
+        rbv = upd ? H : rbv;
+        rbk = upd ? k : rbk;
+      }
+      if (kk < nw) dout[(int64_t)r * nw + kk] = word;
+    }
+    if (rbv != INT_MIN) {
+      const int qr = upper ? r : r + rbk, qc = upper ? r + rbk : r;
+      if (upper ? (rbv + K.late > bv) : gdp_best_before(rbv, qr, qc, bv, br, bc, K.late)) {
+        bv = rbv;
+        br = qr;
+        bc = qc;
+      }
+    }
+    cring[((r + BW) & (GDP_NARROW_RING - 1)) * 64 + lane] = col_cls(r + BW, nextraw);
+  }
+  if (bmode >= GDP_KEEP_BEST) {
+    scores[p.score_off] = (int16_t)bv;
+    scores[p.score_off + 1] = (int16_t)br;
+    scores[p.score_off + 2] = (int16_t)bc;
+    scores[p.score_off + 3] = 0;
+  }
+}
+
+// One lane per problem of the plan's narrow list; a wave takes 64
+// consecutive entries (sorted by row stride and rows) and runs the loop of
+// the widest stride among them.
+__global__ __launch_bounds__(GDP_BLOCK, 4) void gdp_fill_narrow_kernel(const gdp_dev_problem *__restrict__ probs, int n,
+                                                                    const char *__restrict__ qbuf,
+                                                                    const char *__restrict__ gbuf,
+                                                                    uint32_t *__restrict__ dirs,
+                                                                    int16_t *__restrict__ scores) {
+  __shared__ gdp_tables_lds tab;
+  __shared__ int16_t hrow_all[GDP_BLOCK / 64][GDP_NARROW_BW * 64];
+  __shared__ uint8_t ring_all[GDP_BLOCK / 64][GDP_NARROW_RING * 64];
+  const gdp_pair_hdr *hdr = (const gdp_pair_hdr *)(probs + n);
+  if (hdr->magic != (GDP_PAIR_MAGIC ^ (uint64_t)n) || hdr->n != n || hdr->n2 == 0) return;
+  const int n2 = hdr->n2;
+  const int32_t *list2 = (const int32_t *)((const uint8_t *)probs + hdr->list2_off);
+  load_tables(tab);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nwave = (n2 + 63) / 64;
+  for (int w = blockIdx.x * (GDP_BLOCK / 64) + wv; w < nwave; w += gridDim.x * (GDP_BLOCK / 64)) {
+    const int slot = w * 64 + lane;
+    if (slot >= n2) continue;  // lanes past the list's end
+    const gdp_dev_problem p = probs[list2[slot]];
+    // the widest stride among the wave's active lanes picks the loop
+    const int bw = __ballot(p.P == 32) ? 32 : (__ballot(p.P == 16) ? 16 : 8);
+    if (bw == 32) narrow_sweep<32>(tab, p, qbuf, gbuf, dirs, scores, hrow_all[wv], ring_all[wv], lane);
+    else if (bw == 16) narrow_sweep<16>(tab, p, qbuf, gbuf, dirs, scores, hrow_all[wv], ring_all[wv], lane);
+    else narrow_sweep<8>(tab, p, qbuf, gbuf, dirs, scores, hrow_all[wv], ring_all[wv], lane);
+  }
+}
+
 // ------------------------------------------------------------------ lookups
 
 __device__ __forceinline__ uint32_t nib_at(const uint32_t *dirs, const gdp_dev_problem &p, const gdp_geom &G, int row,
                                            int col) {
   if (row < 0 || row > G.R || col < 0 || col > G.C) return 0;
   if (col < gdp_cstart(G, row) || col > gdp_cend(G, row)) return 0;  // not computed: DIAG
+  if (p.layout == 2) {  // narrow sweep: in-band cells only, P nibbles per row
+    if (col < row) return 0;
+    const int64_t idx = (int64_t)row * p.P + (col - row);
+    return (dirs[p.dir_off + (idx >> 3)] >> (4 * (idx & 7))) & 15;
+  }
   if (p.layout == 1) {  // paired sweep: byte 0 Egap, 1 HORIZ, 2 Fgap, 3 VERT; bit = step & 7
     const int g = 1 << p.glog;
     const int t = gdp_p_step(g, p.P, p.lband, row, col);
@@ -640,7 +778,7 @@ struct tb_dirs {
     layout = p.layout;
     glog = p.layout == 1 ? p.glog : 4;
     gmask = (1 << glog) - 1;
-    P = p.layout == 1 ? p.P : G.P;
+    P = p.layout != 0 ? p.P : G.P;
     lb = G.lb;
     ub = G.ub;
     R = G.R;
@@ -655,6 +793,12 @@ struct tb_dirs {
     const int cs = full ? (blk - lb > 0 ? blk - lb : 0) : blk;
     const int ce = row + ub < C ? row + ub : C;
     ok = (unsigned)row <= (unsigned)R && col >= cs && col <= ce && col >= 0;
+    if (layout == 2) {  // narrow sweep: in-band cells, P nibbles per row
+      ok = ok && col >= row;
+      const int idx = row * P + (col - row);
+      t = idx & 7;
+      return ok ? base + (idx >> 3) : base;
+    }
     const int b = row >> glog, i = row & gmask;
     if (layout == 1) {
       const int c0 = (b << glog) - lb;
@@ -1109,7 +1253,7 @@ static int p2_launch(const void *d_descs, int n, const char *d_qbuf, const char 
 }
 
 // kernels: bit 0 single sweeps, bits 1..3 the paired classes, bit 4 standard
-// fills (gdp_plan_kernels).
+// fills, bit 5 narrow sweeps (gdp_plan_kernels).
 extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, const char *d_qbuf, const char *d_gbuf,
                                uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, hipStream_t stream) {
   if (n <= 0) return 0;
@@ -1137,6 +1281,23 @@ extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, con
   };
   if ((kernels & 1u) && single(false, (kernels >> 8) & 15)) return -1;
   if ((kernels & GDP_FILL_STD) && single(true, (kernels >> 12) & 15)) return -1;
+  if (kernels & GDP_FILL_NARROW) {
+    static int cached_n[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
+    if (cached_n[dev] == 0) {
+      int ncu = 0, per = 0;
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill_narrow_kernel, GDP_BLOCK, 0) != hipSuccess)
+        return -1;
+      cached_n[dev] = ncu * (per > 0 ? per : 1);
+    }
+    // resident blocks, but no more than the list's waves need
+    const int blocks = gdp_imin(cached_n[dev], (n + GDP_BLOCK - 1) / GDP_BLOCK);
+    hipLaunchKernelGGL(gdp_fill_narrow_kernel, dim3(blocks), dim3(GDP_BLOCK), 0, stream,
+                       (const gdp_dev_problem *)d_descs, n, d_qbuf, d_gbuf, d_dirs, d_scores);
+    if (hipGetLastError() != hipSuccess) return -1;
+  }
   if (kernels & 0xEu) {
     // all paired classes in one persistent launch, then the general-lookup pass
     if (p2_launch<false>(d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final, stream)) return -1;

@@ -82,6 +82,9 @@ class DeviceBatch:
         self.single = single
         descs, dw, ss = plan(self.problems, PLAN_SINGLE if single else 0, params)
         self.kernels = int(self.L.gdp_plan_kernels(descs.ctypes.data, self.n))
+        # the sweep the plan chose per problem (gdp_dev_problem byte 58): 0
+        # single, 1 paired, 2 narrow one-sided
+        self.layout = descs[: self.n * 96].reshape(self.n, 96)[:, 58].copy() if self.n else np.zeros(0, np.uint8)
         self.dir_words, self.score_slots = dw, ss
         dev = torch.device(device)
         self.d_descs = torch.from_numpy(descs).to(dev)

@@ -233,13 +233,16 @@ int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_g
 
 /* The kernels a plan needs, read from the host copy of its descriptors:
    bit 0 single sweeps, bits 1..3 the paired wave classes, bit 4 standard
-   fills (GDP_KIND_STD), bits 8..11 the log2 of the single sweep's LDS ring
+   fills (GDP_KIND_STD), bit 5 narrow one-sided sweeps (one-sided fills with
+   a band under 32 columns that keep at most their best end cell, one per
+   lane), bits 8..11 the log2 of the single sweep's LDS ring
    and bits 12..15 that of the standard fill's (the next power of two >= the
    largest batch period among their problems, in ring words; 0 means the
    largest ring).  gdp_fill_ex launches only those (gdp_fill launches all,
    and empty ones return at once). */
 #define GDP_FILL_STD 0x10u
-#define GDP_FILL_ALL 0x1Fu
+#define GDP_FILL_NARROW 0x20u
+#define GDP_FILL_ALL 0x3Fu
 uint32_t gdp_plan_kernels(const void *descs, int32_t n);
 int gdp_fill_ex(const void *d_descs, int32_t n, uint32_t kernels, const char *d_qbuf, const char *d_gbuf,
                 uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, void *stream);

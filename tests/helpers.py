@@ -123,11 +123,12 @@ def band_mask(s):
     return po.band_mask_lower(s.rl, s.gl, s.lband)
 
 
-def computed_mask(s):
+def computed_mask(s, layout=0):
     """Cells the fill computes deterministically, as dense [col][row] of the
     sweep (rows = query for full/upper, genome for lower): the band plus the
-    out-of-band prefix of each emulated 32/16-row block."""
-    S = 1 if s.kind == 3 else (32 if s.width == 8 else 16)
+    out-of-band prefix of each emulated 32/16-row block; a narrow one-sided
+    sweep (layout 2) computes the band's cells from the diagonal on only."""
+    S = 1 if (s.kind == 3 or layout == 2) else (32 if s.width == 8 else 16)
     R, C = (s.gl, s.rl) if s.kind == 2 else (s.rl, s.gl)
     r = np.arange(R + 1)[None, :]
     c = np.arange(C + 1)[:, None]

@@ -26,7 +26,7 @@ def _check_fills(oracle, specs, b):
         Ho, Do = H.oracle_fill(oracle, s)
         Dg, Sg = b.export(i)
         B = H.band_mask(s)
-        M = H.computed_mask(s)
+        M = H.computed_mask(s, b.layout[i])
         assert (Dg[B] == Do[B]).all(), ("in-band directions differ", i, s.kind, s.width, s.rl, s.gl)
         assert (Dg[M] == Do[M]).all(), ("out-of-band block directions differ", i, s.kind, s.width)
         if b.problems["keep_scores"][i]:
@@ -59,7 +59,7 @@ def _check_tracebacks(oracle, specs, b, seed):
     for k, (i, r, c) in enumerate(starts):
         s = specs[i]
         Ho, Do = H.oracle_fill(oracle, s)
-        Do = np.where(H.computed_mask(s), Do, 0).astype(np.uint8)
+        Do = np.where(H.computed_mask(s, b.layout[i]), Do, 0).astype(np.uint8)
         po, co = oracle.traceback(s.kind, Do, r, c, s.q, s.q.upper(), s.g, s.galt, 5, 7, s.revp)
         got = H.pairs_to_records(pairs[k])
         assert counts[k, 0] == len(po), ("pair count", i, counts[k, 0], len(po))
@@ -140,7 +140,8 @@ def test_end_extension_pipeline(oracle):
                 continue
             kind = 1 if bc >= br else 2
             s = su if kind == 1 else sl
-            D = np.where(H.computed_mask(s), Du if kind == 1 else Dl, 0).astype(np.uint8)
+            D = np.where(H.computed_mask(s, b.layout[ul[k, 0 if kind == 1 else 1]]), Du if kind == 1 else Dl,
+                         0).astype(np.uint8)
             qo, go = int(m["queryoffset"][ul[k, 0]]), int(m["genomeoffset"][ul[k, 0]])
             po, co = oracle.traceback(kind, D, br, bc, s.q, s.q.upper(), s.g, s.galt, qo, go, revp)
             req = H.tb_request(int(ul[k, 0 if kind == 1 else 1]), s, br, bc, qoff=qo, goff=go)
