@@ -116,7 +116,7 @@ def serving_bench(args):
         knob = (args.serve_slots, args.serve_batch, args.serve_long)
     else:
         # batch size and length split: the highest saturation whose p99 at half
-        # of it meets 5 ms (measured; see gmapdp.tune.tune_server)
+        # of it stays within 90 % of 5 ms (measured; see gmapdp.tune.tune_server)
         from gmapdp.tune import tune_server
         knob, knob_table = tune_server(dg, reqs, lane, dir_budget_bytes=args.serve_budget << 20, max_rl=1000,
                                        max_gl=1500)

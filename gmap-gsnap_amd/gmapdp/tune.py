@@ -53,11 +53,13 @@ def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", rep
 SERVER_KNOBS = ((2, 4096, 400), (2, 4096, 300), (2, 2048, 400), (2, 2048, 300))
 
 
-def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, seconds=1.0, **server_kw):
+def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, seconds=1.0, margin=0.9, **server_kw):
     """Picks (slots, max_batch, long_rows) for the request server by measurement:
     for each candidate, the saturation throughput (all requests at once) and
     p99 latency of Poisson arrivals at half of it.  Keeps the candidate with
-    the highest saturation whose p99 meets the target, else the lowest p99.
+    the highest saturation whose p99 is within margin * target (p99 of a
+    short run moves by several percent from run to run), else within the
+    target, else the one with the lowest p99.
     Returns (knob, table of (knob, saturation req/s, p99 ms))."""
     from .serve import Server
     table = []
@@ -70,6 +72,6 @@ def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, secon
             m = int(min(len(reqs), max(2000, 0.5 * sat * seconds)))
             lat, el, ok = srv.drive(reqs[:m], 0.5 * sat, seed=11)
             table.append(((slots, mb, lr), sat, float(np.percentile(lat, 99)) * 1e3))
-    good = [t for t in table if t[2] <= target_p99_ms]
+    good = [t for t in table if t[2] <= margin * target_p99_ms] or [t for t in table if t[2] <= target_p99_ms]
     best = max(good, key=lambda t: t[1]) if good else min(table, key=lambda t: t[2])
     return best[0], table
