@@ -193,10 +193,9 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags
                                 !(flags & GDP_PLAN_SINGLE)
                             ? gdp_narrow_stride(G.ub)
                             : 0;
-    if (nstride) {  // one lane sweeps its in-band cells: (R + 1) rows of nstride nibbles
+    if (nstride) {  // one lane sweeps its in-band cells; words allocated per wave below
       q.layout = 2;
       q.P = nstride;
-      dw += (int64_t)(G.R + 1) * (nstride / 8);
     } else {
       dw += gdp_dir_words(G);
     }
@@ -337,13 +336,29 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags
     int32_t *list2 = (int32_t *)(base + off);
     for (int32_t i = 0; i < n; i++)
       if (d[i].layout == 2) list2[h.n2++] = i;
+    auto rows = [&](int32_t a) { return d[a].kind == GDP_KIND_LOWER ? d[a].gl : d[a].rl; };
     std::sort(list2, list2 + h.n2, [&](int32_t a, int32_t b) {
       if (d[a].P != d[b].P) return d[a].P > d[b].P;
-      const int ra = d[a].kind == GDP_KIND_LOWER ? d[a].gl : d[a].rl;
-      const int rb = d[b].kind == GDP_KIND_LOWER ? d[b].gl : d[b].rl;
-      if (ra != rb) return ra > rb;
+      if (rows(a) != rows(b)) return rows(a) > rows(b);
       return a < b;
     });
+    // Each 64 consecutive entries are one wave's lanes.  Their direction
+    // words are interleaved so that the wave's stores are coalesced: word kk
+    // of row r of lane l at base + (r * nw + kk) * 64 + l, with nw the
+    // wave's largest row stride in words (P becomes nw).
+    for (int32_t w0 = 0; w0 < h.n2; w0 += 64) {
+      const int32_t w1 = std::min(h.n2, w0 + 64);
+      int nw = 1, rmax = 0;
+      for (int32_t j = w0; j < w1; j++) {
+        nw = std::max(nw, d[list2[j]].P / 8);
+        rmax = std::max(rmax, rows(list2[j]));
+      }
+      for (int32_t j = w0; j < w1; j++) {
+        d[list2[j]].P = nw;
+        d[list2[j]].dir_off = dw + (j - w0);
+      }
+      dw += (int64_t)(rmax + 1) * nw * 64;
+    }
   }
   memcpy(base + hoff, &h, sizeof h);
   dw += 4;  // the traceback reads direction words in aligned 16-byte windows

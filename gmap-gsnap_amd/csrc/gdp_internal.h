@@ -42,8 +42,9 @@
 // same band and tie rule (gdp_fill2_kernel), over the padded sweep sR x sC
 // with batch period P and 2^glog lanes per group;
 // layout 2: a one-sided fill with a band under 32 columns, swept by one lane
-// (gdp_fill_narrow_kernel) over its in-band cells only; P is its row stride
-// in direction nibbles (8, 16 or 32).
+// (gdp_fill_narrow_kernel) over its in-band cells only; the 64 problems of a
+// wave interleave their direction words: word kk (cells 8kk .. 8kk+7 right
+// of the diagonal) of row r at dir_off + (r * P + kk) * 64, P words per row.
 struct gdp_dev_problem {
   int64_t qoff;     // q(r) = qbuf[qoff + r - 1], r = 1..rl (already in sweep order)
   int64_t goff;     // g(c) = gbuf[goff + c - 1]

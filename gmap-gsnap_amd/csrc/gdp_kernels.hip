@@ -515,7 +515,9 @@ __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_std_kernel(const gdp_dev_p
 // lower fill the same, transposed), so in-band cells never read the
 // emulated out-of-band prefix of a row block: a lane computes row r's
 // in-band cells c = r .. r + ub only, left to right, with the same cell
-// rule (gdp_cell_k) and the block-start seeds of the row's block.
+// rule as gdp_cell_k restricted to those cells (the diagonal cell's E is
+// -inf and its H the diagonal's; right of it E and H as in the full rule
+// without F), so the block-start seeds never enter.
 // Per lane, in LDS laid out lane-minor (no bank conflicts): the row above,
 // where cell r + k reads the diagonal H(r-1, r-1+k) from slot k and
 // overwrites it, and a ring of column class bytes (genome class |
@@ -531,7 +533,7 @@ __device__ __forceinline__ void narrow_sweep(const gdp_tables_lds &tab, const gd
   const gdp_geom G = gdp_make_geom(p);
   const gdp_consts K = gdp_make_consts(p);
   const bool upper = p.kind == GDP_KIND_UPPER;
-  const int B = G.ub, R = G.R, C = G.C, nw = p.P / 8;
+  const int B = G.ub, R = G.R, C = G.C, nw = p.P;  // nw = BW / 8 for every lane of the wave
   uint32_t *dout = dirs + p.dir_off;
   // the characters of column col (the upper fill's columns are genome
   // positions, the lower fill's query positions), then their class byte
@@ -561,13 +563,19 @@ __device__ __forceinline__ void narrow_sweep(const gdp_tables_lds &tab, const gd
     ch = chn;
     chn = row_chars(p, G, qbuf, gbuf, r + 2);
     const uint32_t nextraw = col_raw(r + BW);  // joins the ring after this row
-    const int cs = r & -G.S;
     const int nact = gdp_imin(B, C - r) + 1;  // in-band cells of this row
-    // cells eligible for the end scan: the upper fill's (c >= r, row >= 1;
-    // queryend_indels: row rl), the lower fill's with query row c > r
-    // (queryend_indels: c = rl)
-    const bool rowok = r >= 1 && bmode >= GDP_KEEP_BEST && (!upper || bmode == GDP_KEEP_BEST || r == R);
-    int Hl = K.INIT_H, El = K.E_INIT, rbv = INT_MIN, rbk = 0;
+    // the end scan's cells of this row are k in [kmin, kmax]: the upper
+    // fill's from the diagonal on (queryend_indels: row rl only), the lower
+    // fill's right of it (queryend_indels: query row rl only)
+    int kmin = upper ? 0 : 1, kmax = nact - 1;
+    if (bmode == GDP_KEEP_BEST_QUERYEND) {
+      if (upper && r != R) kmax = -1;
+      if (!upper) kmin = gdp_imax(kmin, C - r);
+    }
+    if (r < 1 || bmode < GDP_KEEP_BEST) kmax = -1;
+    const unsigned kspan = kmax >= kmin ? (unsigned)(kmax - kmin) : 0u;
+    if (kmax < kmin) kmin = 1 << 20;
+    int Hl = K.NEG, El = K.NEG, rbv = INT_MIN, rbk = 0;
 #pragma unroll 1
     for (int kk = 0; kk < BW / 8; kk++) {
       uint32_t word = 0;
@@ -577,19 +585,38 @@ __device__ __forceinline__ void narrow_sweep(const gdp_tables_lds &tab, const gd
         const bool act = k < nact;
         const uint32_t cb = cring[(c & (GDP_NARROW_RING - 1)) * 64 + lane];
         const int hd = hrow[k * 64 + lane];
-        int H, F, E;
-        uint32_t nib;
-        gdp_cell_k<false>(K, G, r, c, cs, Hl, El, hd, 0, 0, prof, (int)(cb & 15u), (int)(cb >> 4), H, F, E, nib);
+        // score of the cell: the better of the two genome alternatives
+        // (4-bit fields of the row profile, one per class)
+        const int sc = gdp_imax(__builtin_amdgcn_sbfe((int)prof, (int)((cb & 15u) << 2), 4),
+                                __builtin_amdgcn_sbfe((int)prof, (int)((cb >> 4) << 2), 4));
+        int H, E;
+        uint32_t nib = 0;
+        if (j == 0 && kk == 0) {
+          // the diagonal cell c = r: E is -inf there and H takes the
+          // diagonal (row 0: H(0, 0) = 0 + score)
+          E = K.NEG;
+          H = gdp_clampw((r == 0 ? 0 : hd) + sc, K.NEG, K.POS);
+        } else {
+          // E from the left cell, then H = max(diagonal + score, E)
+          // (gdp_cell_k's one-sided rule right of the diagonal; row 0 reads
+          // a diagonal of -inf)
+          const int T1 = gdp_imax(Hl + K.o, K.NEG);
+          const bool dE = El + K.late > T1;
+          E = gdp_imax(gdp_imax(El, T1) + K.e, K.NEG);
+          const int Hd = gdp_clampw((r == 0 ? K.NEG : hd) + sc, K.NEG, K.POS);
+          const bool dH = E + K.late > Hd;
+          H = gdp_imax(Hd, E);
+          nib = (dH ? (uint32_t)GDP_DIR_H : 0u) | (dE ? (uint32_t)GDP_DIR_E : 0u);
+        }
         Hl = H;
         El = E;
         hrow[k * 64 + lane] = (int16_t)(act ? H : hd);
         word |= (act ? nib : 0u) << (4 * j);
-        const bool el = act && rowok && (upper || (k >= 1 && (bmode == GDP_KEEP_BEST || c == C)));
-        const bool upd = el && H + K.late > rbv;
+        const bool upd = (unsigned)(k - kmin) <= kspan && H + K.late > rbv;
         rbv = upd ? H : rbv;
         rbk = upd ? k : rbk;
       }
-      if (kk < nw) dout[(int64_t)r * nw + kk] = word;
+      dout[((int64_t)r * nw + kk) * 64] = word;  // lanes of the wave: consecutive words
     }
     if (rbv != INT_MIN) {
       const int qr = upper ? r : r + rbk, qc = upper ? r + rbk : r;
@@ -632,8 +659,8 @@ __global__ __launch_bounds__(GDP_BLOCK, 4) void gdp_fill_narrow_kernel(const gdp
     const int slot = w * 64 + lane;
     if (slot >= n2) continue;  // lanes past the list's end
     const gdp_dev_problem p = probs[list2[slot]];
-    // the widest stride among the wave's active lanes picks the loop
-    const int bw = __ballot(p.P == 32) ? 32 : (__ballot(p.P == 16) ? 16 : 8);
+    // the wave's row stride (the plan gives all its lanes the widest) picks the loop
+    const int bw = 8 * __builtin_amdgcn_readfirstlane(p.P);
     if (bw == 32) narrow_sweep<32>(tab, p, qbuf, gbuf, dirs, scores, hrow_all[wv], ring_all[wv], lane);
     else if (bw == 16) narrow_sweep<16>(tab, p, qbuf, gbuf, dirs, scores, hrow_all[wv], ring_all[wv], lane);
     else narrow_sweep<8>(tab, p, qbuf, gbuf, dirs, scores, hrow_all[wv], ring_all[wv], lane);
@@ -646,10 +673,10 @@ __device__ __forceinline__ uint32_t nib_at(const uint32_t *dirs, const gdp_dev_p
                                            int col) {
   if (row < 0 || row > G.R || col < 0 || col > G.C) return 0;
   if (col < gdp_cstart(G, row) || col > gdp_cend(G, row)) return 0;  // not computed: DIAG
-  if (p.layout == 2) {  // narrow sweep: in-band cells only, P nibbles per row
+  if (p.layout == 2) {  // narrow sweep: in-band cells only, interleaved words
     if (col < row) return 0;
-    const int64_t idx = (int64_t)row * p.P + (col - row);
-    return (dirs[p.dir_off + (idx >> 3)] >> (4 * (idx & 7))) & 15;
+    const int k = col - row;
+    return (dirs[p.dir_off + ((int64_t)row * p.P + (k >> 3)) * 64] >> (4 * (k & 7))) & 15;
   }
   if (p.layout == 1) {  // paired sweep: byte 0 Egap, 1 HORIZ, 2 Fgap, 3 VERT; bit = step & 7
     const int g = 1 << p.glog;
@@ -793,11 +820,11 @@ struct tb_dirs {
     const int cs = full ? (blk - lb > 0 ? blk - lb : 0) : blk;
     const int ce = row + ub < C ? row + ub : C;
     ok = (unsigned)row <= (unsigned)R && col >= cs && col <= ce && col >= 0;
-    if (layout == 2) {  // narrow sweep: in-band cells, P nibbles per row
+    if (layout == 2) {  // narrow sweep: in-band cells, interleaved words
       ok = ok && col >= row;
-      const int idx = row * P + (col - row);
-      t = idx & 7;
-      return ok ? base + (idx >> 3) : base;
+      const int k = col - row;
+      t = k & 7;
+      return ok ? base + ((int64_t)row * P + (k >> 3)) * 64 : base;
     }
     const int b = row >> glog, i = row & gmask;
     if (layout == 1) {
