@@ -151,6 +151,8 @@ def serving_bench(args):
             "saturation_req_s": round(sat, 1), "saturation_gcups": round(cells.sum() / (nreq / sat) / 1e9, 2),
             "p50_ms_at_50pct": half["p50_ms"], "p99_ms_at_50pct": half["p99_ms"], "points": points,
             "batches": int(st["batches"]), "max_batch": max_batch, "slots": slots, "long_rows": long_rows,
+            "per_batch_ms": {k: round(float(st[k + "_seconds"]) / max(int(st["batches"]), 1) * 1e3, 3)
+                             for k in ("prep", "retire", "batch", "device")},
             "knob_candidates": None if knob_table is None else
             [{"slots": k[0], "max_batch": k[1], "long_rows": k[2], "saturation_req_s": round(sat_, 1),
               "p99_ms_at_50pct": round(p99_, 3)} for k, sat_, p99_ in knob_table],
@@ -184,7 +186,7 @@ def main():
     ap.add_argument("--lane-groups", default="", metavar="SPAN,ROWS",
                     help="fixed lane groups (g16_max_span,g64_min_rows) instead of the autotuner's")
     ap.add_argument("--serve-genome", type=int, default=3_100_000_000, help="bases of the device genome")
-    ap.add_argument("--serve-requests", type=int, default=300_000)
+    ap.add_argument("--serve-requests", type=int, default=600_000)
     ap.add_argument("--serve-batch", type=int, default=0,
                     help="largest batch the server forms (0: measured choice with --serve-long)")
     ap.add_argument("--serve-slots", type=int, default=2, help="batches in flight (streams)")

@@ -50,7 +50,12 @@ def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", rep
     return best, table
 
 
-SERVER_KNOBS = ((2, 4096, 400), (2, 4096, 300), (2, 2048, 400), (2, 2048, 300))
+# (slots, max_batch, long_rows) candidates of the request server: batches in
+# flight, requests per batch, and the query length from which reads are
+# batched apart.  At half load a request waits for a batch slot and then for
+# its batch, whose time the longest reads set; a third slot lets short batches
+# run beside a long one.
+SERVER_KNOBS = ((2, 4096, 300), (3, 4096, 300), (3, 2048, 200), (2, 2048, 300))
 
 
 def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, seconds=1.0, margin=0.9, **server_kw):
