@@ -136,7 +136,7 @@ class DeviceBatch:
         gptr, glen = None, 0
         if genome is not None:
             if genome_format == 0:
-                d_gen = torch.from_numpy(np.ascontiguousarray(genome, np.uint8)).to(self.dev)
+                d_gen = torch.from_numpy(np.array(genome, np.uint8, copy=True)).to(self.dev)
                 glen = len(genome)
             else:
                 d_gen = torch.from_numpy(np.ascontiguousarray(genome, np.uint32).view(np.uint8)).to(self.dev)

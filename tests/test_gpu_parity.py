@@ -92,10 +92,16 @@ def test_wide_bands_and_long_queries(oracle):
     _check_tracebacks(oracle, specs, b, 7)
 
 
-def test_edge_shapes(oracle):
+@pytest.mark.parametrize("keep", [True, False])
+def test_edge_shapes(oracle, keep):
+    # keep=False sends the one-sided fills with bands under 32 columns to the
+    # narrow sweep (one problem per lane) and full-band int16 to the paired one
     specs = []
     for rl, gl, lb, ub in [(1, 1, 0, 0), (1, 5, 0, 4), (5, 1, 4, 0), (2, 40, 0, 38), (40, 2, 38, 0),
-                           (33, 33, 0, 0), (64, 64, 1, 1), (31, 95, 3, 67), (16, 16, 15, 15)]:
+                           (33, 33, 0, 0), (64, 64, 1, 1), (31, 95, 3, 67), (16, 16, 15, 15),
+                           # one-sided bands at the narrow sweep's row widths (8, 16, 32 cells)
+                           (40, 40, 7, 7), (40, 40, 8, 8), (40, 48, 15, 15), (40, 56, 16, 16), (50, 81, 31, 31),
+                           (50, 82, 32, 32)]:
         for width in (8, 16):
             for kind in (0, 1, 2):
                 rng = np.random.default_rng(rl * 1000 + gl)
@@ -103,7 +109,9 @@ def test_edge_shapes(oracle):
                 q = H.rand_seq(rng, rl)
                 for late in (False, True):
                     specs.append(H.Spec(kind, width, q, g, g, 3, -10, -2, lb, ub, late, late))
-    b = _run(oracle, specs)
+    b = _run(oracle, specs, keep_scores=keep)
+    if not keep:
+        assert (b.layout == 2).any() and (b.layout == 1).any()
     _check_fills(oracle, specs, b)
     _check_tracebacks(oracle, specs, b, 3)
 
