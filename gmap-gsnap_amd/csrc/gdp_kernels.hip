@@ -883,7 +883,9 @@ __device__ __forceinline__ uint32_t tb_ballot(bool p, int g0) {
   return (uint32_t)((__ballot(p) >> g0) & 0xFFFFull);
 }
 
-__global__ __launch_bounds__(256) void gdp_traceback_kernel(
+// Eight waves per SIMD (64 registers): the walks are latency-bound, and
+// more of them in flight hide more of each round's memory wait.
+__global__ __launch_bounds__(256, 8) void gdp_traceback_kernel(
     const gdp_dev_problem *__restrict__ probs, const gdp_tb_request *__restrict__ reqs, int nreq,
     const char *__restrict__ qbuf, const char *__restrict__ gbuf, const uint32_t *__restrict__ dirs,
     const void *__restrict__ genome, int gfmt, uint32_t glen, gdp_pair *__restrict__ pairs,
