@@ -135,11 +135,20 @@ def serving_bench(args):
         for frac in (None, 0.1, 0.25, 0.5, 0.75, 0.9):
             rate = 1e4 if frac is None else frac * sat
             m = int(min(nreq, max(2000, rate * args.serve_seconds)))
-            lat, el, ok = srv.drive(reqs[:m], rate, seed=7)
+            # the half-load point is run three times (different arrival
+            # seeds) and reports the run with the median p99, all three listed:
+            # a single run moves with the shared host's load
+            runs = []
+            for rep in range(3 if frac == 0.5 else 1):
+                lat, el, ok = srv.drive(reqs[:m], rate, seed=7 + rep)
+                runs.append((float(np.percentile(lat, 99)) * 1e3, float(np.percentile(lat, 50)) * 1e3, el, ok))
+            runs_sorted = sorted(runs)
+            p99, p50, el, ok = runs_sorted[len(runs) // 2]
             pt = {"rate_req_s": round(rate, 1), "fraction_of_saturation": None if frac is None else frac,
                   "requests": m, "achieved_req_s": round(m / el, 1),
-                  "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
-                  "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "ok": ok}
+                  "p50_ms": round(p50, 3), "p99_ms": round(p99, 3), "ok": ok}
+            if len(runs) > 1:
+                pt["p99_ms_runs"] = [round(r[0], 3) for r in runs]
             log("serving", pt)
             points.append(pt)
         st = srv.stats()

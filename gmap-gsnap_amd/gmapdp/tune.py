@@ -75,8 +75,11 @@ def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, secon
             lat, el, ok = srv.drive(reqs, 0.0)
             sat = len(reqs) / el
             m = int(min(len(reqs), max(2000, 0.5 * sat * seconds)))
-            lat, el, ok = srv.drive(reqs[:m], 0.5 * sat, seed=11)
-            table.append(((slots, mb, lr), sat, float(np.percentile(lat, 99)) * 1e3))
+            p99 = []
+            for rep in range(2):  # the larger of two runs: a slow stretch of the host shows in either
+                lat, el, ok = srv.drive(reqs[:m], 0.5 * sat, seed=11 + rep)
+                p99.append(float(np.percentile(lat, 99)) * 1e3)
+            table.append(((slots, mb, lr), sat, max(p99)))
     good = [t for t in table if t[2] <= margin * target_p99_ms] or [t for t in table if t[2] <= target_p99_ms]
     best = max(good, key=lambda t: t[1]) if good else min(table, key=lambda t: t[2])
     return best[0], table
