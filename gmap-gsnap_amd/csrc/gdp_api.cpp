@@ -23,6 +23,8 @@ extern "C" int gdp_launch_traceback(const void *, const gdp_tb_request *, int, c
 extern "C" int gdp_launch_segments(const void *, int, uint32_t, const gdp_segment *, int, char *, hipStream_t);
 extern "C" int gdp_launch_best_endpoint(const void *, const int32_t *, int, int, const int16_t *, int32_t *,
                                         const gdp_tb_request *, gdp_tb_request *, hipStream_t);
+extern "C" int gdp_launch_bridge_cdna(const void *, const int32_t *, int, const int16_t *, int32_t *,
+                                      const gdp_tb_request *, gdp_tb_request *, hipStream_t);
 extern "C" int gdp_launch_export(const void *, int, const uint32_t *, const int16_t *, uint8_t *, int16_t *,
                                  hipStream_t);
 extern "C" int gdp_launch_export_ref(const void *, int, int, const uint32_t *, const int16_t *, int, void *, void *,
@@ -463,6 +465,18 @@ extern "C" int gdp_best_endpoint_ex(const void *d_descs, const int32_t *d_ul, in
 extern "C" int gdp_best_endpoint(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,
                                  const int16_t *d_scores, int32_t *d_out, void *stream) {
   return gdp_best_endpoint_ex(d_descs, d_ul, m, queryend_indels, d_scores, d_out, nullptr, nullptr, stream);
+}
+
+extern "C" int gdp_bridge_cdna(const void *d_descs, const int32_t *d_quads, int32_t m, const int16_t *d_scores,
+                               int32_t *d_out, const gdp_tb_request *d_tb_template, gdp_tb_request *d_tb_out,
+                               void *stream) {
+  if (!g_init) return fail(GDP_ENOINIT, "gdp_bridge_cdna: gdp_init has not been called");
+  if (m < 0 || (m > 0 && (!d_descs || !d_quads || !d_scores || !d_out)) ||
+      ((d_tb_template == nullptr) != (d_tb_out == nullptr)))
+    return fail(GDP_EINVAL, "gdp_bridge_cdna: null argument");
+  if (gdp_launch_bridge_cdna(d_descs, d_quads, m, d_scores, d_out, d_tb_template, d_tb_out, (hipStream_t)stream))
+    return hip_fail("gdp_bridge_cdna launch", hipGetLastError());
+  return GDP_OK;
 }
 
 extern "C" int gdp_traceback_ex(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq, const char *d_qbuf,

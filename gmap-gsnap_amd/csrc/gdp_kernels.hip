@@ -1201,6 +1201,156 @@ __global__ __launch_bounds__(256) void gdp_best_endpoint_kernel(
   }
 }
 
+// ------------------------------------------------------------- cDNA bridge
+
+#define GDP_BRIDGE_ROWS 1024  // rows of one right-end column held in LDS
+
+// Of two candidates of the bridge, the one the reference's sequential scan
+// keeps: the higher score; on a tie the later one in scan order under
+// jump-late (>=), the earlier one otherwise (>).  key is the scan position.
+__device__ __forceinline__ bool bridge_wins(int v, int64_t key, int bv, int64_t bkey, int late) {
+  return v > bv || (v == bv && (late ? key > bkey : key < bkey));
+}
+
+// Score of cell (r, c) of one end: rows below the diagonal (r >= c) come
+// from the lower fill, which is swept transposed, the others from the upper
+// one (dynprog_cdna.c:417-418, 454-455).
+__device__ __forceinline__ int bridge_score(const int16_t *scores, const gdp_dev_problem &U, const gdp_geom &GU,
+                                            const gdp_dev_problem &Lo, const gdp_geom &GL, int r, int c) {
+  return r < c ? score_at(scores, U, GU, r, c) : score_at(scores, Lo, GL, c, r);
+}
+
+// bridge_cdna_gap_{8,16}_ud (dynprog_cdna.c:123-381, 386-644), one wave per
+// gap.  The reference scans cL ascending, cR descending from gl-cL, rL and rR
+// ascending over each end's rows: from max(1, c-uband) up the upper triangle
+// to the diagonal, then down the lower one to min(c+lband, rl-1), the right
+// end's rows also below rightoffset-leftoffset-rL.  A genomic insertion
+// between the ends costs 0 when empty and open otherwise (the loop resets its
+// penalty each step, 496-497).  For one (cL, cR, rL) the best rR is a prefix
+// maximum of the right column over rows up to min(top, lim-1), so the wave
+// builds that prefix table per cR in LDS (ties to the later row under
+// jump-late) and each lane walks its own left columns; candidates merge by
+// value, then scan position, which makes the result independent of the order
+// the lanes visit them.  Upper rows beyond rl (reachable when rl < gl-1) are
+// rows the reference's fill never wrote; they are not candidates here.
+__global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
+    const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ quads, int m,
+    const int16_t *__restrict__ scores, int32_t *__restrict__ out, const gdp_tb_request *__restrict__ tmpl,
+    gdp_tb_request *__restrict__ tbout) {
+  __shared__ int pmv[GDP_BRIDGE_ROWS];
+  __shared__ int pmr[GDP_BRIDGE_ROWS];
+  const int k = blockIdx.x, lane = threadIdx.x;
+  if (k >= m) return;
+  const int32_t *q = quads + 5 * (int64_t)k;
+  const int iLu = q[0], iLl = q[1], iRu = q[2], iRl = q[3], D = q[4];
+  const gdp_dev_problem Lu = probs[iLu], Ll = probs[iLl], Ru = probs[iRu], Rl = probs[iRl];
+  const int gl = Lu.gl, rlL = Lu.rl, rlR = Ru.rl;
+  const int ubL = Lu.uband, lbL = Ll.lband, ubR = Ru.uband, lbR = Rl.lband;
+  const int late = Lu.jump_late, open = Lu.open;
+  const int init = Lu.width == 8 ? -128 : -32768;
+  const bool ok = Lu.kind == GDP_KIND_UPPER && Ll.kind == GDP_KIND_LOWER && Ru.kind == GDP_KIND_UPPER &&
+                  Rl.kind == GDP_KIND_LOWER && Lu.keep_mode == GDP_KEEP_ALL && Ll.keep_mode == GDP_KEEP_ALL &&
+                  Ru.keep_mode == GDP_KEEP_ALL && Rl.keep_mode == GDP_KEEP_ALL && Lu.score_off >= 0 &&
+                  Ll.score_off >= 0 && Ru.score_off >= 0 && Rl.score_off >= 0 && Ll.gl == gl && Ru.gl == gl &&
+                  Rl.gl == gl && Ll.rl == rlL && Rl.rl == rlR && Ll.width == Lu.width && Ru.width == Lu.width &&
+                  Rl.width == Lu.width && ubR + lbR + 1 <= GDP_BRIDGE_ROWS;
+  int bv = INT_MIN;
+  int64_t bkey = -1;
+  const int64_t RR = (int64_t)gdp_imax(rlL, rlR) + 2;  // radix of the row fields of a scan position
+  if (ok) {
+    const gdp_geom GLu = gdp_make_geom(Lu), GLl = gdp_make_geom(Ll), GRu = gdp_make_geom(Ru), GRl = gdp_make_geom(Rl);
+    for (int cR = 0; cR < gl; cR++) {
+      const int rloR = gdp_imax(1, cR - ubR), rhighR = gdp_imin(rlR - 1, cR + lbR);
+      const int rtopR = gdp_imax(gdp_imin(cR - 1, rlR), rhighR);
+      const int nR = rtopR - rloR + 1;
+      if (nR <= 0) continue;
+      __syncthreads();  // the previous column's table has been read
+      int cv = INT_MIN, cr = -1;
+      for (int base = 0; base < nR; base += 64) {
+        const int j = base + lane;
+        int v = INT_MIN, r = -1;
+        if (j < nR) {
+          r = rloR + j;
+          v = bridge_score(scores, Ru, GRu, Rl, GRl, r, cR);
+        }
+        // inclusive prefix maximum over the lanes, then over the earlier chunks
+        for (int d = 1; d < 64; d <<= 1) {
+          const int ov = __shfl_up(v, d, 64), orr = __shfl_up(r, d, 64);
+          if (lane >= d && !(late ? v >= ov : v > ov)) {
+            v = ov;
+            r = orr;
+          }
+        }
+        if (!(late ? v >= cv : v > cv)) {
+          v = cv;
+          r = cr;
+        }
+        if (j < nR) {
+          pmv[j] = v;
+          pmr[j] = r;
+        }
+        cv = __shfl(v, 63, 64);
+        cr = __shfl(r, 63, 64);
+      }
+      __syncthreads();
+      const int cLmax = gdp_imin(gl - 1, gl - cR);
+      for (int cL = 1 + lane; cL <= cLmax; cL += 64) {
+        const int pen = cR == gl - cL ? 0 : open;
+        const int rloL = gdp_imax(1, cL - ubL), rhighL = gdp_imin(rlL - 1, cL + lbL);
+        const int rtopL = gdp_imax(gdp_imin(cL - 1, rlL), rhighL);
+        const int64_t kc = ((int64_t)cL * (gl + 1) + (gl - cR)) * RR;
+        for (int rL = rloL; rL <= rtopL; rL++) {
+          const int t = gdp_imin(rtopR, D - rL - 1);
+          if (t < rloR) break;  // the bound only falls as rL grows
+          const int v = bridge_score(scores, Lu, GLu, Ll, GLl, rL, cL) + pmv[t - rloR] + pen;
+          const int64_t key = (kc + rL) * RR + pmr[t - rloR];
+          if (bridge_wins(v, key, bv, bkey, late)) {
+            bv = v;
+            bkey = key;
+          }
+        }
+      }
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+      const int ov = __shfl_xor(bv, d, 64);
+      const int64_t okey = __shfl_xor(bkey, d, 64);
+      if (bridge_wins(ov, okey, bv, bkey, late)) {
+        bv = ov;
+        bkey = okey;
+      }
+    }
+  }
+  if (lane != 0) return;
+  const bool has = ok && (late ? bv >= init : bv > init);
+  int cL = -1, cR = -1, rL = -1, rR = -1;
+  if (has) {
+    rR = (int)(bkey % RR);
+    rL = (int)((bkey / RR) % RR);
+    const int64_t kc = bkey / RR / RR;
+    cR = gl - (int)(kc % (gl + 1));
+    cL = (int)(kc / (gl + 1));
+  }
+  int32_t *o = out + 5 * (int64_t)k;
+  o[0] = ok ? (has ? bv : init) : INT_MIN;
+  o[1] = cL;
+  o[2] = cR;
+  o[3] = rL;
+  o[4] = rR;
+  if (tbout != nullptr) {
+    // Dynprog_cdna_gap walks the right end first, each in the triangle that
+    // holds its best cell (dynprog_cdna.c:995-1013, 1045-1060)
+    gdp_tb_request a = tmpl[2 * (int64_t)k], b = tmpl[2 * (int64_t)k + 1];
+    a.fill = has && cR < rR ? iRl : iRu;
+    a.r = has ? rR : 0;
+    a.c = has ? cR : 0;
+    b.fill = has && cL < rL ? iLl : iLu;
+    b.r = has ? rL : 0;
+    b.c = has ? cL : 0;
+    tbout[2 * (int64_t)k] = a;
+    tbout[2 * (int64_t)k + 1] = b;
+  }
+}
+
 // ------------------------------------------------------------ dense export
 
 __global__ void gdp_export_kernel(const gdp_dev_problem *__restrict__ probs, int idx, const uint32_t *__restrict__ dirs,
@@ -1359,6 +1509,15 @@ extern "C" int gdp_launch_best_endpoint(const void *d_descs, const int32_t *d_ul
   if (m <= 0) return 0;
   hipLaunchKernelGGL(gdp_best_endpoint_kernel, dim3((m + 256 / TBG - 1) / (256 / TBG)), dim3(256), 0, stream,
                      (const gdp_dev_problem *)d_descs, d_ul, m, queryend_indels, d_scores, d_out, d_tmpl, d_tbout);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int gdp_launch_bridge_cdna(const void *d_descs, const int32_t *d_quads, int m, const int16_t *d_scores,
+                                      int32_t *d_out, const gdp_tb_request *d_tmpl, gdp_tb_request *d_tbout,
+                                      hipStream_t stream) {
+  if (m <= 0) return 0;
+  hipLaunchKernelGGL(gdp_bridge_cdna_kernel, dim3(m), dim3(64), 0, stream, (const gdp_dev_problem *)d_descs, d_quads,
+                     m, d_scores, d_out, d_tmpl, d_tbout);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

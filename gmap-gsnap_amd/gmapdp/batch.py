@@ -165,6 +165,31 @@ class DeviceBatch:
                "gdp_best_endpoint")
         return d_out.cpu().numpy()[: 3 * m].reshape(-1, 3)
 
+    def bridge_cdna(self, quads, tb_template=None):
+        """gdp_bridge_cdna over rows {left upper, left lower, right upper,
+        right lower, rightoffset - leftoffset}: (m, 5) rows {bestscore,
+        bestcL, bestcR, bestrL, bestrR}; with tb_template (2m requests) also
+        the completed traceback requests (right end, then left end)."""
+        torch = self.torch
+        quads = np.ascontiguousarray(quads, np.int32).reshape(-1, 5)
+        m = len(quads)
+        d_q = torch.from_numpy(quads.reshape(-1).copy()).to(self.dev)
+        d_out = torch.zeros(max(m, 1) * 5, dtype=torch.int32, device=self.dev)
+        d_tmpl = d_tb = None
+        if tb_template is not None:
+            tmpl = np.ascontiguousarray(tb_template, TBREQ_DTYPE)
+            assert len(tmpl) == 2 * m
+            d_tmpl = torch.from_numpy(tmpl.view(np.uint8).copy()).to(self.dev)
+            d_tb = torch.zeros_like(d_tmpl)
+        _check(self.L, self.L.gdp_bridge_cdna(self.d_descs.data_ptr(), d_q.data_ptr(), m, self.d_scores.data_ptr(),
+                                              d_out.data_ptr(), d_tmpl.data_ptr() if d_tmpl is not None else None,
+                                              d_tb.data_ptr() if d_tb is not None else None, self.stream),
+               "gdp_bridge_cdna")
+        out = d_out.cpu().numpy()[: 5 * m].reshape(-1, 5)
+        if tb_template is None:
+            return out
+        return out, d_tb.cpu().numpy().view(TBREQ_DTYPE)
+
     # -- device-resident traceback (timed paths) ---------------------------
     def prepare_traceback(self, reqs, genome=None):
         """Uploads requests and allocates outputs once; returns a handle."""

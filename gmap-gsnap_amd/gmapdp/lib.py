@@ -115,6 +115,7 @@ def load(path=None):
         "gdp_cells": (i64, [vp]),
         "gdp_fill": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, vp, vp]),
         "gdp_best_endpoint": (ctypes.c_int, [vp, vp, i32, i32, vp, vp, vp]),
+        "gdp_bridge_cdna": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, vp]),
         "gdp_traceback": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp]),
         "gdp_traceback_ex": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp]),
         "gdp_genome_segments": (ctypes.c_int, [vp, i32, ctypes.c_uint32, vp, i32, vp, vp]),

@@ -294,6 +294,32 @@ int gdp_best_endpoint_ex(const void *d_descs, const int32_t *d_ul, int32_t m, in
                          const int16_t *d_scores, int32_t *d_out, const gdp_tb_request *d_tb_template,
                          gdp_tb_request *d_tb_out, void *stream);
 
+/* cDNA gap bridge: the step of Dynprog_cdna_gap (dynprog_cdna.c:786-1180)
+   that joins a left and a right end across one genomic gap,
+   bridge_cdna_gap_{8,16}_ud (dynprog_cdna.c:123-381, 386-644; called at
+   977 and 1106), batched.  For each of m gaps d_quads[5k .. 5k+4] =
+   {left upper, left lower, right upper, right lower, rightoffset -
+   leftoffset}: indices of four one-sided fills of one width over the gap's
+   gl genome columns, kept with GDP_KEEP_ALL -- the left pair over
+   rsequenceL forward (jump_late_p), the right pair over rev_rsequenceR read
+   backwards against the reversed genome (revp, !jump_late_p) -- and the
+   two ends' query offsets' difference, which keeps the ends' rows apart.
+   Open penalty and tie rule come from the left upper fill.
+   d_out[5k .. 5k+4] = {bestscore, bestcL, bestcR, bestrL, bestrR}; when no
+   join reaches the initial score (-128 / -32768) bestscore is that score and
+   the cells are -1 (the reference leaves them unset); gaps whose fills do
+   not fit (kinds, widths, lengths, kept scores, or a right band wider than
+   1023 rows) get INT32_MIN.  Rows of an upper triangle beyond its rl, which
+   the reference reads from its workspace only when rl < gl-1, are not
+   candidates.  With d_tb_template and d_tb_out (both or neither)
+   d_tb_out[2k] is the right end's traceback request and d_tb_out[2k+1] the
+   left end's (d_tb_template's entries with fill, r and c set: the upper
+   fill when bestc >= bestr, dynprog_cdna.c:995-1060); cells of -1 give
+   requests at (0, 0) that push no pairs.  Indices must name problems of
+   d_descs. */
+int gdp_bridge_cdna(const void *d_descs, const int32_t *d_quads, int32_t m, const int16_t *d_scores,
+                    int32_t *d_out, const gdp_tb_request *d_tb_template, gdp_tb_request *d_tb_out, void *stream);
+
 /* ---- Genome in device memory ------------------------------------------ */
 
 /* Formats of a device genome handed to gdp_traceback_ex and

@@ -603,6 +603,64 @@ void orc_best_endpoint(int queryend_indels, const short *Hupper, const short *Hl
   *finalscore = best;
 }
 
+/* ------------------------------------------------------------ cDNA bridge */
+
+/* bridge_cdna_gap_{8,16}_ud (dynprog_cdna.c:386-644; the int8 form 123-381
+   differs only in its initial score).  Candidates are visited in the
+   reference's order -- cL ascending, cR descending from gl-cL, rL
+   ascending, rR ascending -- and a
+   candidate replaces the best under >= (jump late) or > (jump early).
+   Each end's rows run from max(1, c-uband) up the upper triangle to the
+   diagonal (no cap at rl-1 there, 417-418), then down the lower triangle to
+   min(c+lband, rl-1) (407-414); the right end's rows also stay below
+   rightoffset-leftoffset-rL (422-423).  A genomic insertion of
+   k = gl-cL-cR columns costs 0 when k = 0 and open otherwise: the loop body
+   resets the penalty to open-extend (496-497) before the step adds extend
+   back, so it never grows with k.  An upper row beyond rl, reachable
+   only when rl < gl-1, reads rows the fill never wrote in the reference
+   (stale workspace); here such candidates are skipped. */
+static inline int bridge_score(const short *U, const short *Lo, int gl, int rl, int r, int c, int upper) {
+  return upper ? U[(size_t)c * (rl + 1) + r] : Lo[(size_t)r * (gl + 1) + c];
+}
+
+void orc_bridge_cdna(int width, const short *LU, const short *LL, const short *RU, const short *RL,
+                     int gl, int rlL, int rlR, int lbandL, int ubandL, int lbandR, int ubandR,
+                     int open, int extend, int leftoffset, int rightoffset, int late, int *out) {
+  int best = width == 8 ? -128 : -32768;
+  int bcL = -1, bcR = -1, brL = -1, brR = -1;
+  for (int cL = 1; cL < gl; cL++) {
+    for (int cR = gl - cL; cR >= 0; cR--) {
+      const int k = gl - cL - cR; /* genomic columns left between the two ends */
+      const int pen = k == 0 ? 0 : open;
+      const int rloL = imax(1, cL - ubandL), rhighL = imin(rlL - 1, cL + lbandL);
+      const int rloR = imax(1, cR - ubandR), rhighR = imin(rlR - 1, cR + lbandR);
+      for (int rL = rloL; rL < cL || rL <= rhighL; rL++) {
+        const int upL = rL < cL;
+        if (upL && rL > rlL) continue;
+        const int sL = bridge_score(LU, LL, gl, rlL, rL, cL, upL);
+        const int lim = rightoffset - leftoffset - rL;
+        for (int rR = rloR; (rR < cR || rR <= rhighR) && rR < lim; rR++) {
+          const int upR = rR < cR;
+          if (upR && rR > rlR) continue;
+          const int v = sL + bridge_score(RU, RL, gl, rlR, rR, cR, upR) + pen;
+          if (wins(v, best, late)) {
+            best = v;
+            bcL = cL;
+            bcR = cR;
+            brL = rL;
+            brR = rR;
+          }
+        }
+      }
+    }
+  }
+  out[0] = best;
+  out[1] = bcL;
+  out[2] = bcR;
+  out[3] = brL;
+  out[4] = brR;
+}
+
 /* ---------------------------------------------------------- genome segment */
 
 static void revcomp_inplace(char *s, unsigned int n) {

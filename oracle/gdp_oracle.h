@@ -101,6 +101,17 @@ void orc_best_endpoint(int queryend_indels, const short *Hupper, const short *Hl
                        int width, int rl, int gl, int lband, int uband, int jump_late,
                        int *finalscore, int *bestr, int *bestc);
 
+/* bridge_cdna_gap_{8,16}_ud (dynprog_cdna.c:123-381, 386-644): the best join of
+   a left and a right pair of one-sided fills over one genomic gap of gl
+   columns.  LU/LL are the left fills (rsequenceL forward), RU/RL the right
+   ones (rev_rsequenceR backward); upper [c][r] stride rl+1, lower [r][c]
+   stride gl+1, with rlL / rlR rows.  out = {bestscore, bestcL, bestcR,
+   bestrL, bestrR}; the cells stay -1 when no candidate reaches the
+   initial -128 / -32768 (the reference leaves them unset then). */
+void orc_bridge_cdna(int width, const short *LU, const short *LL, const short *RU, const short *RL,
+                     int gl, int rlL, int rlR, int lbandL, int ubandL, int lbandR, int ubandR,
+                     int open, int extend, int leftoffset, int rightoffset, int jump_late, int *out);
+
 /* Genome_get_segment_right / _left (genome.c:11022-11120) over orc_genome_t,
    without SNP alternates (segmentalt == segment). out has length+1 bytes. */
 void orc_segment_right(char *out, const orc_genome_t *genome, unsigned int left,

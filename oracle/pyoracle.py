@@ -56,6 +56,17 @@ class _Common:
         return q, g, ga
 
 
+def _bridge_cdna_call(fn, width, LU, LL, RU, RL, gl, rlL, rlR, lbandL, ubandL, lbandR, ubandR, open_, ext,
+                      leftoffset, rightoffset, late):
+    arrs = [np.ascontiguousarray(a, np.int16) for a in (LU, LL, RU, RL)]
+    assert arrs[0].size == (gl + 1) * (rlL + 1) and arrs[1].size == (rlL + 1) * (gl + 1)
+    assert arrs[2].size == (gl + 1) * (rlR + 1) and arrs[3].size == (rlR + 1) * (gl + 1)
+    out = np.zeros(5, np.int32)
+    fn(int(width), *[_np_ptr(a) for a in arrs], gl, rlL, rlR, lbandL, ubandL, lbandR, ubandR, open_, ext,
+       leftoffset, rightoffset, int(late), _np_ptr(out))
+    return tuple(int(x) for x in out)
+
+
 class Oracle(_Common):
     """CPU restatement (gdp_oracle.c)."""
 
@@ -78,6 +89,7 @@ class Oracle(_Common):
         L.orc_traceback_nogaps.restype = _i
         L.orc_best_endpoint.argtypes = [_i, _p, _p, _i, _i, _i, _i, _i, _i, _p, _p, _p]
         L.orc_fill_std.argtypes = [_p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]
+        L.orc_bridge_cdna.argtypes = [_i, _p, _p, _p, _p] + [_i] * 12 + [_p]
         L.orc_segment_right.argtypes = [_p, _p, _u, _u, _u, _i]
         L.orc_segment_left.argtypes = [_p, _p, _u, _u, _u, _i]
         for f in ("orc_cells_full",):
@@ -191,6 +203,10 @@ class Oracle(_Common):
                                    ctypes.byref(bc))
         return fs.value, br.value, bc.value
 
+    def bridge_cdna(self, *args):
+        """orc_bridge_cdna: (bestscore, bestcL, bestcR, bestrL, bestrR)."""
+        return _bridge_cdna_call(self.lib.orc_bridge_cdna, *args)
+
     def segment_right(self, genome, left, length, chrhigh, revcomp):
         out = ctypes.create_string_buffer(length + 1)
         self.lib.orc_segment_right(out, self._genome_struct(genome), left, length, chrhigh, int(revcomp))
@@ -229,6 +245,7 @@ class Reference(_Common):
         L.refh_traceback.argtypes = [_i, _i, _i, _i, _p, _p, _p, _p, _i, _i, _i, _u, _u, _i, _i, _i,
                                      _p, _p, _i]
         L.refh_traceback.restype = _i
+        L.refc_bridge_cdna.argtypes = [_i, _p, _p, _p, _p] + [_i] * 12 + [_p]
         L.refh_init(mode)
         self.ws_single = L.refh_new(max_rl, max_gl, 0)
         self.ws_double = L.refh_new(max_rl, max_gl, 1)
@@ -274,6 +291,10 @@ class Reference(_Common):
                                  lband, int(late), int(revp), _np_ptr(H), _np_ptr(D), garbage)
         self._last[KIND_LOWER] = width
         return H, D
+
+    def bridge_cdna(self, *args):
+        """The reference's bridge_cdna_gap_{8,16}_ud, same arguments as Oracle.bridge_cdna."""
+        return _bridge_cdna_call(self.lib.refc_bridge_cdna, *args)
 
     def traceback(self, kind, dirs, r, c, rseq, rsequc, gseq, galt, queryoffset=0, genomeoffset=0,
                   revp=False, chroffset=0, chrhigh=0, watsonp=True, genestrand=0, genome=None,

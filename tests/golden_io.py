@@ -89,3 +89,23 @@ def std_traceback_expect(g, i):
         return None
     off, n = int(g["tb_off"][i]), int(g["tb_n"][i])
     return int(g["tb_r"][i]), int(g["tb_c"][i]), g["tb_pairs"][off:off + n], g["tb_counts"][i]
+
+
+BRIDGE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bridge_golden.npz")
+BRIDGE_FIELDS = ("width", "gl", "rlL", "rlR", "lbandL", "ubandL", "lbandR", "ubandR", "open", "ext", "leftoffset",
+                 "rightoffset", "late")
+
+
+def load_bridge():
+    """cDNA gap bridge vectors (make_bridge_golden.py): a list of
+    (args for Oracle.bridge_cdna, the reference's 5-tuple)."""
+    z = np.load(BRIDGE_PATH, allow_pickle=False)
+    tabs, offs, out = z["tables"], z["table_off"], z["out"]
+    cases = []
+    for i in range(len(out)):
+        f = {k: int(z[k][i]) for k in BRIDGE_FIELDS}
+        A = [tabs[offs[4 * i + j]: offs[4 * i + j + 1]] for j in range(4)]
+        args = (f["width"], *A, f["gl"], f["rlL"], f["rlR"], f["lbandL"], f["ubandL"], f["lbandR"], f["ubandR"],
+                f["open"], f["ext"], f["leftoffset"], f["rightoffset"], f["late"])
+        cases.append((args, tuple(int(x) for x in out[i])))
+    return cases
