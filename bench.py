@@ -45,6 +45,8 @@ WORKLOADS = {
     "c3": (3, 2_000_000, "single-gap rl=logU[20,1000] gl=rl*U(1,1.5) extraband=U{3..64} mixed int8/int16, 2e6 problems"),
     "c4": (4, 2_000_000, "end extensions rl=U{10..150} gl=rl+U{0..20} ENDQ (upper+lower fill, best end cell, "
                          "traceback) mixed 1:1 with single-gap logU[20,1000] problems, 2e6 + 2e6"),
+    "cdna": ("cdna", 200_000, "cDNA gaps gl=U{16..160}, both ends rl=gl+8, extraband 14, open/extend -10/-7 "
+                              "(four one-sided fills keeping every score, the bridge, two end tracebacks), 2e5 gaps"),
 }
 
 
@@ -219,6 +221,10 @@ def main():
     if config == 4:
         from gmapdp.workload import EndMix
         wl = EndMix(n, n)
+    elif config == "cdna":
+        from gmapdp.workload import CdnaGaps
+        wl = CdnaGaps(n)
+        args.percall, args.no_cpu = 0, True  # neither form covers the bridge
     else:
         wl = Workload(config, n, variant=1 if args.workload == "c1lowq" else 0)
     probs = wl.problems
@@ -249,15 +255,23 @@ def main():
         d_ul = torch.from_numpy(wl.ul.reshape(-1).copy()).to(b.dev)
         d_tmpl = torch.from_numpy(reqs[wl.n3:].view(np.uint8).copy()).to(b.dev)
         d_best = torch.zeros(3 * wl.m, dtype=torch.int32, device=b.dev)
+    elif config == "cdna":
+        reqs = wl.templates()
+        d_quads = torch.from_numpy(wl.quads.reshape(-1).copy()).to(b.dev)
+        d_tmpl = torch.from_numpy(reqs.view(np.uint8).copy()).to(b.dev)
+        d_best = torch.zeros(5 * wl.n, dtype=torch.int32, device=b.dev)
     else:
         reqs = wl.tb_requests()
     reqs_dev = b.prepare_traceback(reqs, genome=wl.genome())
 
     def enqueue_rest(st=None):
-        # everything after the fill: end cells (config 4), then the tracebacks
+        # everything after the fill: end cells (config 4) or the cDNA bridge,
+        # then the tracebacks
         if config == 4:
             b.best_endpoint_requests(d_ul, wl.m, False, d_best, d_tmpl.data_ptr(),
                                      reqs_dev["d_reqs"].data_ptr() + wl.n3 * TBREQ_DTYPE.itemsize, stream=st)
+        elif config == "cdna":
+            b.bridge_requests(d_quads, wl.n, d_best, d_tmpl.data_ptr(), reqs_dev["d_reqs"].data_ptr(), stream=st)
         b.traceback_device(reqs_dev, stream=st)
 
     def step(ev=None, tev=None):
@@ -309,6 +323,18 @@ def main():
     log("fill %.2f ms, traceback %.2f ms, step %.2f ms, %.1f GCUPS end-to-end, fill-only %.1f GCUPS"
         % (fill_ms, tb_ms, ms_per_step, gcups, total_cells / (fill_ms * 1e-3) / 1e9))
 
+    bridge_ms = None
+    if config == "cdna":
+        # the bridge alone, HIP events on the stream it runs on (torch's current one)
+        bev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        bev[0].record(stream)
+        for _ in range(args.steps):
+            b.bridge_requests(d_quads, wl.n, d_best, d_tmpl.data_ptr(), reqs_dev["d_reqs"].data_ptr())
+        bev[1].record(stream)
+        torch.cuda.synchronize()
+        bridge_ms = bev[0].elapsed_time(bev[1]) / args.steps
+        log("cDNA bridge: %.3f ms for %d gaps" % (bridge_ms, wl.n))
+
     # Plain one-launch-per-problem form on the same inputs.
     percall = None
     if args.percall:
@@ -357,10 +383,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"c1lowq": "int8", "c3": "int16+int8", "c4": "int16+int8"}.get(args.workload, "int16"),
+        "dtype": {"c1lowq": "int8", "c3": "int16+int8", "c4": "int16+int8", "cdna": "int16+int8"}.get(
+            args.workload, "int16"),
         "data": "synthetic (seeded generator of the workload)",
         "config": {"workload": desc, "problems": len(probs), "cells": total_cells,
-                   "step": "fill + traceback" if config != 4 else "fill + best end cell + traceback",
+                   "step": {4: "fill + best end cell + traceback", "cdna": "fill + bridge + traceback"}.get(
+                       config, "fill + traceback"),
                    "launch": "eager" if graph is None else "one HIP graph per step",
                    "lane_groups": None if params is None else {"g16_max_span": params[0],
                                                                  "g64_min_rows": params[1]}},
@@ -371,9 +399,12 @@ def main():
         "batched_vs_percall": None if percall is None else round(gcups / percall, 1),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
                      "frac": round(achieved / PEAK_INT16_TOPS, 4), "traffic": traffic,
-                     "kernel": "gdp_fill2_kernel (one fill: every launch of gdp_fill, HIP events on its stream)",
+                     "kernel": ("gdp_fill_kernel (one fill of one-sided problems keeping every score)"
+                                if config == "cdna" else
+                                "gdp_fill2_kernel (one fill: every launch of gdp_fill, HIP events on its stream)"),
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": cpu,
+        "bridge_ms": None if bridge_ms is None else round(bridge_ms, 3),
         "serving": serving,
         "tuning": tuning,
     }

@@ -1203,7 +1203,11 @@ __global__ __launch_bounds__(256) void gdp_best_endpoint_kernel(
 
 // ------------------------------------------------------------- cDNA bridge
 
-#define GDP_BRIDGE_ROWS 1024  // rows of one right-end column held in LDS
+#define GDP_BRIDGE_ROWS 1024  // rows of one end's column held in LDS
+#define GDP_BRIDGE_LDS 8192   // words of LDS: right prefix tables and one left column
+
+// A score (int16 range) and its row in one word: the score in the high half.
+__device__ __forceinline__ int bridge_pack(int v, int r) { return (int)(((uint32_t)v << 16) | (uint32_t)(r & 0xffff)); }
 
 // Of two candidates of the bridge, the one the reference's sequential scan
 // keeps: the higher score; on a tie the later one in scan order under
@@ -1224,21 +1228,27 @@ __device__ __forceinline__ int bridge_score(const int16_t *scores, const gdp_dev
 // gap.  The reference scans cL ascending, cR descending from gl-cL, rL and rR
 // ascending over each end's rows: from max(1, c-uband) up the upper triangle
 // to the diagonal, then down the lower one to min(c+lband, rl-1), the right
-// end's rows also below rightoffset-leftoffset-rL.  A genomic insertion
-// between the ends costs 0 when empty and open otherwise (the loop resets its
-// penalty each step, 496-497).  For one (cL, cR, rL) the best rR is a prefix
-// maximum of the right column over rows up to min(top, lim-1), so the wave
-// builds that prefix table per cR in LDS (ties to the later row under
-// jump-late) and each lane walks its own left columns; candidates merge by
-// value, then scan position, which makes the result independent of the order
-// the lanes visit them.  Upper rows beyond rl (reachable when rl < gl-1) are
-// rows the reference's fill never wrote; they are not candidates here.
+// end's rows also below D - rL with D = rightoffset-leftoffset.  A genomic
+// insertion between the ends costs 0 when empty and open otherwise (the loop
+// resets its penalty each step, 496-497).  Candidates merge by value, then
+// scan position, which makes the result independent of the order the lanes
+// visit them.  Upper rows beyond rl (reachable when rl < gl-1) are rows the
+// reference's fill never wrote; they are not candidates here.
+//
+// For one (cL, cR) the rows rL <= D-1-top(cR) see the whole right column, so
+// their best join is the left column's prefix maximum up to that row plus
+// the right column's maximum (ties to the later row under jump-late, as the
+// scan keeps it): O(1) per (cL, cR) from the column maxima (LDS, once per gap)
+// and the prefix table of the current left column (LDS, one wave scan per
+// cL), with lanes over cR.  The few rows above that bound see a prefix of the
+// right column that grows as rL falls, walked once per (cL, cR).  Gaps wider
+// than the LDS tables take the direct form: per cR a prefix table of the
+// right column, lanes over cL and every left row.
 __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
     const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ quads, int m,
     const int16_t *__restrict__ scores, int32_t *__restrict__ out, const gdp_tb_request *__restrict__ tmpl,
     gdp_tb_request *__restrict__ tbout) {
-  __shared__ int pmv[GDP_BRIDGE_ROWS];
-  __shared__ int pmr[GDP_BRIDGE_ROWS];
+  __shared__ int lds[GDP_BRIDGE_LDS];
   const int k = blockIdx.x, lane = threadIdx.x;
   if (k >= m) return;
   const int32_t *q = quads + 5 * (int64_t)k;
@@ -1248,17 +1258,121 @@ __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
   const int ubL = Lu.uband, lbL = Ll.lband, ubR = Ru.uband, lbR = Rl.lband;
   const int late = Lu.jump_late, open = Lu.open;
   const int init = Lu.width == 8 ? -128 : -32768;
-  const bool ok = Lu.kind == GDP_KIND_UPPER && Ll.kind == GDP_KIND_LOWER && Ru.kind == GDP_KIND_UPPER &&
+  const bool ok0 = Lu.kind == GDP_KIND_UPPER && Ll.kind == GDP_KIND_LOWER && Ru.kind == GDP_KIND_UPPER &&
                   Rl.kind == GDP_KIND_LOWER && Lu.keep_mode == GDP_KEEP_ALL && Ll.keep_mode == GDP_KEEP_ALL &&
                   Ru.keep_mode == GDP_KEEP_ALL && Rl.keep_mode == GDP_KEEP_ALL && Lu.score_off >= 0 &&
                   Ll.score_off >= 0 && Ru.score_off >= 0 && Rl.score_off >= 0 && Ll.gl == gl && Ru.gl == gl &&
                   Rl.gl == gl && Ll.rl == rlL && Rl.rl == rlR && Ll.width == Lu.width && Ru.width == Lu.width &&
-                  Rl.width == Lu.width && ubR + lbR + 1 <= GDP_BRIDGE_ROWS;
+                  Rl.width == Lu.width;
+  // the table form needs its offsets in LDS; the direct form one right column
+  const bool fast = ok0 && gl + 1 <= GDP_BRIDGE_LDS && ubL + lbL + 1 <= GDP_BRIDGE_ROWS;
   int bv = INT_MIN;
   int64_t bkey = -1;
   const int64_t RR = (int64_t)gdp_imax(rlL, rlR) + 2;  // radix of the row fields of a scan position
-  if (ok) {
+  // the table form when the right end's prefix tables, their offsets and one
+  // left column fit in LDS (a uniform decision: every input is the gap's)
+  int nRsum = 0;
+  for (int c0 = 0; fast && c0 < gl; c0 += 64) {
+    const int c = c0 + lane;
+    const int n = c < gl ? gdp_imax(0, gdp_imax(gdp_imin(c - 1, rlR), gdp_imin(rlR - 1, c + lbR)) -
+                                           gdp_imax(1, c - ubR) + 1)
+                         : 0;
+    int x = n;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(x, d, 64);
+      if (lane >= d) x += o;
+    }
+    if (fast && c <= gl) lds[c] = nRsum + x - n;  // offset of column c's table (column gl: the end)
+    nRsum += __shfl(x, 63, 64);
+  }
+  if (fast && lane == 0) lds[gl] = nRsum;
+  const int nLmax = ubL + lbL + 1;
+  const bool table = fast && (gl + 1) + nRsum + 2 * nLmax <= GDP_BRIDGE_LDS;
+  const bool ok = ok0 && (table || ubR + lbR + 1 <= GDP_BRIDGE_ROWS);
+  if (ok && table) {
     const gdp_geom GLu = gdp_make_geom(Lu), GLl = gdp_make_geom(Ll), GRu = gdp_make_geom(Ru), GRl = gdp_make_geom(Rl);
+    const int *offR = lds;
+    int *rpm = lds + gl + 1, *lp = rpm + nRsum, *lraw = lp + nLmax;
+    __syncthreads();  // offsets written
+    // prefix maxima of every right column, ties to the later row under jump-late
+    for (int cR = lane; cR < gl; cR += 64) {
+      const int rloR = gdp_imax(1, cR - ubR);
+      int v0 = INT_MIN, r0 = 0, w = offR[cR];
+      for (int r = rloR; w < offR[cR + 1]; r++, w++) {
+        const int v = bridge_score(scores, Ru, GRu, Rl, GRl, r, cR);
+        if (late ? v >= v0 : v > v0) {
+          v0 = v;
+          r0 = r;
+        }
+        rpm[w] = bridge_pack(v0, r0);
+      }
+    }
+    for (int cL = 1; cL < gl; cL++) {
+      const int rloL = gdp_imax(1, cL - ubL), rtopL = gdp_imax(gdp_imin(cL - 1, rlL), gdp_imin(rlL - 1, cL + lbL));
+      const int nL = rtopL - rloL + 1;
+      if (nL <= 0) continue;
+      __syncthreads();  // right tables written; the previous left column has been read
+      int cv = INT_MIN, cr = -1;
+      for (int base = 0; base < nL; base += 64) {
+        const int j = base + lane;
+        int v = INT_MIN, r = -1;
+        if (j < nL) {
+          r = rloL + j;
+          v = bridge_score(scores, Lu, GLu, Ll, GLl, r, cL);
+          lraw[j] = v;
+        }
+        for (int d = 1; d < 64; d <<= 1) {
+          const int ov = __shfl_up(v, d, 64), orr = __shfl_up(r, d, 64);
+          if (lane >= d && !(late ? v >= ov : v > ov)) {
+            v = ov;
+            r = orr;
+          }
+        }
+        if (!(late ? v >= cv : v > cv)) {
+          v = cv;
+          r = cr;
+        }
+        if (j < nL) lp[j] = bridge_pack(v, r);
+        cv = __shfl(v, 63, 64);
+        cr = __shfl(r, 63, 64);
+      }
+      __syncthreads();
+      const int64_t kc0 = (int64_t)cL * (gl + 1);
+      for (int cR = lane; cR <= gl - cL; cR += 64) {
+        const int o = offR[cR], nR = offR[cR + 1] - o;
+        if (nR <= 0) continue;
+        const int rloR = gdp_imax(1, cR - ubR), rtopR = rloR + nR - 1;
+        const int pen = cR == gl - cL ? 0 : open;
+        const int64_t kc = (kc0 + (gl - cR)) * RR;
+        const int B = D - 1 - rtopR;  // rows up to B see the whole right column
+        const int ub = gdp_imin(rtopL, B);
+        if (ub >= rloL) {
+          const int x = lp[ub - rloL], y = rpm[o + nR - 1];
+          const int v = (x >> 16) + (y >> 16) + pen;
+          const int64_t key = (kc + (x & 0xffff)) * RR + (y & 0xffff);
+          if (bridge_wins(v, key, bv, bkey, late)) {
+            bv = v;
+            bkey = key;
+          }
+        }
+        // rows above B see the right rows below D - rL only
+        for (int rL = gdp_imax(rloL, B + 1); rL <= rtopL; rL++) {
+          const int t = D - rL - 1;
+          if (t < rloR) break;  // the bound only falls as rL grows
+          const int y = rpm[o + t - rloR];
+          const int v = lraw[rL - rloL] + (y >> 16) + pen;
+          const int64_t key = (kc + rL) * RR + (y & 0xffff);
+          if (bridge_wins(v, key, bv, bkey, late)) {
+            bv = v;
+            bkey = key;
+          }
+        }
+      }
+    }
+  } else if (ok) {
+    __syncthreads();  // the table form's offsets are no longer read
+    const gdp_geom GLu = gdp_make_geom(Lu), GLl = gdp_make_geom(Ll), GRu = gdp_make_geom(Ru), GRl = gdp_make_geom(Rl);
+    int *pmv = lds, *pmr = lds + GDP_BRIDGE_ROWS;
     for (int cR = 0; cR < gl; cR++) {
       const int rloR = gdp_imax(1, cR - ubR), rhighR = gdp_imin(rlR - 1, cR + lbR);
       const int rtopR = gdp_imax(gdp_imin(cR - 1, rlR), rhighR);
@@ -1311,13 +1425,13 @@ __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
         }
       }
     }
-    for (int d = 32; d >= 1; d >>= 1) {
-      const int ov = __shfl_xor(bv, d, 64);
-      const int64_t okey = __shfl_xor(bkey, d, 64);
-      if (bridge_wins(ov, okey, bv, bkey, late)) {
-        bv = ov;
-        bkey = okey;
-      }
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    const int ov = __shfl_xor(bv, d, 64);
+    const int64_t okey = __shfl_xor(bkey, d, 64);
+    if (bridge_wins(ov, okey, bv, bkey, late)) {
+      bv = ov;
+      bkey = okey;
     }
   }
   if (lane != 0) return;

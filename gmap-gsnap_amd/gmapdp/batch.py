@@ -190,6 +190,14 @@ class DeviceBatch:
             return out
         return out, d_tb.cpu().numpy().view(TBREQ_DTYPE)
 
+    def bridge_requests(self, d_quads, m, d_out, d_tmpl, d_tbout, stream=None):
+        """gdp_bridge_cdna on device buffers: joins into d_out and both ends'
+        traceback requests (templates at address d_tmpl completed) to address
+        d_tbout."""
+        _check(self.L, self.L.gdp_bridge_cdna(self.d_descs.data_ptr(), d_quads.data_ptr(), m, self.d_scores.data_ptr(),
+                                              d_out.data_ptr(), d_tmpl, d_tbout,
+                                              self.stream if stream is None else stream), "gdp_bridge_cdna")
+
     # -- device-resident traceback (timed paths) ---------------------------
     def prepare_traceback(self, reqs, genome=None):
         """Uploads requests and allocates outputs once; returns a handle."""

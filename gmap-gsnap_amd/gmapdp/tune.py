@@ -54,8 +54,12 @@ def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", rep
 # flight, requests per batch, and the query length from which reads are
 # batched apart.  At half load a request waits for a batch slot and then for
 # its batch, whose time the longest reads set; a third slot lets short batches
-# run beside a long one.
-SERVER_KNOBS = ((2, 4096, 300), (3, 4096, 300), (3, 2048, 200), (2, 2048, 300))
+# run beside a long one.  Measured on the configuration-5 mix (3 runs each at
+# half load): 2 slots of 2048 with reads of 500 rows and more apart kept p99
+# near 4.3 ms at the highest saturation; 4096-request batches and 4 slots
+# stayed above 5 ms; (3, 1024, 200) trades a third of the saturation for
+# p99 near 4.2 ms.
+SERVER_KNOBS = ((2, 2048, 500), (2, 2048, 300), (3, 2048, 200), (3, 1024, 200))
 
 
 def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, seconds=1.0, margin=0.9, **server_kw):

@@ -10,7 +10,7 @@ META_DTYPE = np.dtype([("queryoffset", "<i4"), ("genomeoffset", "<i4"), ("chroff
                        ("watsonp", "i1"), ("revp", "i1"), ("pad", "i1", (2,)), ("pair", "<i4")])
 assert META_DTYPE.itemsize == 24
 
-SEEDS = {1: 1001, 2: 1002, 3: 1003, 4: 1004, 5: 1005}
+SEEDS = {1: 1001, 2: 1002, 3: 1003, 4: 1004, 5: 1005, "cdna": 1006}
 
 
 def _bind(L):
@@ -131,4 +131,102 @@ class EndMix:
         for f in ("queryoffset", "genomeoffset", "chroffset", "chrhigh", "watsonp", "revp"):
             req[f] = m[f]
         req["max_pairs"] = p["rl"] + p["gl"] + 4
+        return req
+
+
+class CdnaGaps:
+    """cDNA gaps as stage 3 hands them to Dynprog_cdna_gap
+    (stage3.c:9307-9322): a genomic gap of gl bases, both ends' query length
+    rl = gl + 8 (extramaterial_paired, gmap.c:283), extraband_paired 14
+    (gmap.c:292), the cDNA penalties -10/-7 (dynprog_cdna.c:31-37) under
+    HIGHQ scoring, and the width by the rule of dynprog_cdna.c:906-913 (int8
+    when gl or both ends are under use8p_size).  The cDNA between the two
+    anchors is the genomic gap with 3 % substitutions and 10-60 bases of
+    extra material at a random point.
+
+    problems[4k .. 4k+3]: the gap's left upper, left lower, right upper and
+    right lower fills, kept with every score (the right end runs over the
+    reversed cDNA and genome, with the other tie rule); quads: the rows of
+    gdp_bridge_cdna; genome(): the gaps' genomic bases end to end, gap k at
+    goffset[k]."""
+
+    OPEN, EXTEND, EXTRABAND, EXTRAMATERIAL = -10, -7, 14, 8
+
+    def __init__(self, n, seed=None, gl_range=(16, 160), jump_late=False):
+        L = load()
+        L.gdp_host_tables(0)  # GMAP's score tables on the host, for use8p_size
+        rng = np.random.default_rng(SEEDS["cdna"] if seed is None else seed)
+        use8 = int(L.gdp_use8p_size(0))
+        acgt = np.frombuffer(b"ACGT", np.uint8)
+        gls = rng.integers(gl_range[0], gl_range[1] + 1, n).astype(np.int64)
+        rls = gls + self.EXTRAMATERIAL
+        self.goffset = np.zeros(n, np.int64)
+        self.goffset[1:] = np.cumsum(gls)[:-1]
+        self._genome = acgt[rng.integers(0, 4, int(gls.sum()))]
+        cuts = rng.integers(0, gls + 1)
+        ins = rng.integers(10, 61, n)  # extra cDNA bases: the gap's query span is gl + ins
+        extra = acgt[rng.integers(0, 4, int(ins.sum()))]
+        eoff = np.zeros(n + 1, np.int64)
+        eoff[1:] = np.cumsum(ins)
+        # per gap: left query, right query (reversed cDNA); genome, reversed genome
+        qbuf = np.empty(int(2 * rls.sum()) + 8, np.uint8)
+        gbuf = np.empty(int(2 * gls.sum()) + 8, np.uint8)
+        qbuf[-8:] = 0
+        gbuf[-8:] = 0
+        qo = np.zeros(n, np.int64)
+        qo[1:] = np.cumsum(2 * rls)[:-1]
+        go = 2 * self.goffset
+        for k in range(n):
+            gl, rl, c = int(gls[k]), int(rls[k]), int(cuts[k])
+            g = self._genome[self.goffset[k]: self.goffset[k] + gl]
+            cdna = np.concatenate([g[:c], extra[eoff[k]: eoff[k + 1]], g[c:]])
+            sub = rng.random(len(cdna)) < 0.03
+            cdna[sub] = acgt[rng.integers(0, 4, int(sub.sum()))]
+            q0 = int(qo[k])
+            qbuf[q0: q0 + rl] = cdna[:rl]
+            qbuf[q0 + rl: q0 + 2 * rl] = cdna[::-1][:rl]
+            g0 = int(go[k])
+            gbuf[g0: g0 + gl] = g
+            gbuf[g0 + gl: g0 + 2 * gl] = g[::-1]
+        probs = np.zeros(4 * n, PROBLEM_DTYPE)
+        late = int(jump_late)
+        for j in range(4):
+            p = probs[j::4]
+            right = j >= 2
+            p["qoff"] = qo + (rls if right else 0)
+            p["goff"] = go + (gls if right else 0)
+            p["galtoff"] = p["goff"]
+            p["rl"], p["gl"] = rls, gls
+            # Dynprog_compute_bands with widebandp, rl > gl (dynprog.c:1245-1261)
+            p["lband"], p["uband"] = rls - gls + self.EXTRABAND, self.EXTRABAND
+            p["kind"] = 1 + (j & 1)
+            p["width"] = np.where((gls < use8) | (rls < use8), 8, 16)
+            p["jump_late"] = 1 - late if right else late
+            p["open"], p["extend"], p["keep_scores"] = self.OPEN, self.EXTEND, 1
+        self.problems = probs
+        self.qbuf, self.gbuf = qbuf, gbuf
+        # the left end starts at query offset 0 (roffsetL) of the gap's gl + ins
+        # cDNA bases, the right end at its last base (rev_roffsetR); the bridge
+        # keeps the ends' rows below rev_roffsetR - roffsetL
+        self.rev_roffset = gls + ins - 1
+        self.quads = np.stack([4 * np.arange(n) + j for j in range(4)] + [self.rev_roffset], 1).astype(np.int32)
+        self.n = n
+
+    def genome(self):
+        return self._genome
+
+    def templates(self):
+        """Traceback request templates, right end then left end per gap
+        (fill, r and c are set by gdp_bridge_cdna); pair_off laid out."""
+        p = self.problems
+        req = np.zeros(2 * self.n, TBREQ_DTYPE)
+        rl, gl = p["rl"][0::4], p["gl"][0::4]
+        req["queryoffset"][0::2] = self.rev_roffset  # walked with revp
+        req["queryoffset"][1::2] = 0  # roffsetL
+        req["genomeoffset"][0::2] = self.goffset + gl - 1  # rev_goffset
+        req["genomeoffset"][1::2] = self.goffset
+        req["revp"][0::2] = 1
+        req["chroffset"], req["chrhigh"], req["watsonp"] = 0, len(self._genome), 1
+        req["max_pairs"] = np.repeat(rl + gl + 4, 2)
+        req["pair_off"][1:] = np.cumsum(req["max_pairs"].astype(np.int64))[:-1]
         return req

@@ -308,8 +308,10 @@ int gdp_best_endpoint_ex(const void *d_descs, const int32_t *d_ul, int32_t m, in
    d_out[5k .. 5k+4] = {bestscore, bestcL, bestcR, bestrL, bestrR}; when no
    join reaches the initial score (-128 / -32768) bestscore is that score and
    the cells are -1 (the reference leaves them unset); gaps whose fills do
-   not fit (kinds, widths, lengths, kept scores, or a right band wider than
-   1023 rows) get INT32_MIN.  Rows of an upper triangle beyond its rl, which
+   not fit (kinds, widths, lengths, kept scores; or a right band of more
+   than 1023 rows in a gap too large for the kernel's 8192-word LDS tables:
+   gl + 1 words, one per in-band right-end cell, and twice the left band)
+   get INT32_MIN.  Rows of an upper triangle beyond its rl, which
    the reference reads from its workspace only when rl < gl-1, are not
    candidates.  With d_tb_template and d_tb_out (both or neither)
    d_tb_out[2k] is the right end's traceback request and d_tb_out[2k+1] the

@@ -27,6 +27,8 @@ def _gaps(rng, n):
         ext = 4 + t % 5
         lbL, ubL = _bands(rlL, gl, ext)
         lbR, ubR = _bands(rlR, gl, ext)
+        if t % 11 == 5:
+            lbL = ubL = 520  # a left band past the kernel's LDS column: the direct form
         g = H.rand_seq(rng, gl)
         qL = H.mutate(rng, g, 0.04, 0.02)[:rlL].ljust(rlL, b"G")
         qR = H.mutate(rng, g[::-1], 0.04, 0.02)[:rlR].ljust(rlR, b"T")
@@ -45,7 +47,7 @@ def _gaps(rng, n):
 def test_bridge_and_end_tracebacks(oracle):
     from gmapdp.batch import DeviceBatch
     rng = np.random.default_rng(2024)
-    gaps = _gaps(rng, 70)
+    gaps = _gaps(rng, 90)
     specs = [s for sp, _ in gaps for s in sp]
     probs, qbuf, gbuf = H.pack_batch(specs, keep_scores=True)
     b = DeviceBatch(probs, qbuf, gbuf)
