@@ -1294,22 +1294,39 @@ __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
     const int *offR = lds;
     int *rpm = lds + gl + 1, *lp = rpm + nRsum, *lraw = lp + nLmax;
     __syncthreads();  // offsets written
-    // prefix maxima of every right column, ties to the later row under jump-late
+    // prefix maxima of every right column, ties to the later row under
+    // jump-late; eight rows' loads are issued before the first is used
     for (int cR = lane; cR < gl; cR += 64) {
-      const int rloR = gdp_imax(1, cR - ubR);
-      int v0 = INT_MIN, r0 = 0, w = offR[cR];
-      for (int r = rloR; w < offR[cR + 1]; r++, w++) {
-        const int v = bridge_score(scores, Ru, GRu, Rl, GRl, r, cR);
-        if (late ? v >= v0 : v > v0) {
-          v0 = v;
-          r0 = r;
+      const int rloR = gdp_imax(1, cR - ubR), w0 = offR[cR], n = offR[cR + 1] - w0;
+      int v0 = INT_MIN, r0 = 0;
+      for (int i0 = 0; i0 < n; i0 += 8) {
+        int vals[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          vals[u] = i0 + u < n ? bridge_score(scores, Ru, GRu, Rl, GRl, rloR + i0 + u, cR) : 0;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          if (i0 + u < n) {
+            if (late ? vals[u] >= v0 : vals[u] > v0) {
+              v0 = vals[u];
+              r0 = rloR + i0 + u;
+            }
+            rpm[w0 + i0 + u] = bridge_pack(v0, r0);
+          }
         }
-        rpm[w] = bridge_pack(v0, r0);
       }
     }
+    // the first 64 rows of left column c, one per lane (0 past its rows)
+    auto left_rows = [&](int c) {
+      const int lo = gdp_imax(1, c - ubL), top = gdp_imax(gdp_imin(c - 1, rlL), gdp_imin(rlL - 1, c + lbL));
+      return lane <= top - lo ? bridge_score(scores, Lu, GLu, Ll, GLl, lo + lane, c) : 0;
+    };
+    int pre = gl > 1 ? left_rows(1) : 0;
     for (int cL = 1; cL < gl; cL++) {
       const int rloL = gdp_imax(1, cL - ubL), rtopL = gdp_imax(gdp_imin(cL - 1, rlL), gdp_imin(rlL - 1, cL + lbL));
       const int nL = rtopL - rloL + 1;
+      const int cur = pre;
+      if (cL + 1 < gl) pre = left_rows(cL + 1);  // the next column's loads run under this one's work
       if (nL <= 0) continue;
       __syncthreads();  // right tables written; the previous left column has been read
       int cv = INT_MIN, cr = -1;
@@ -1318,7 +1335,7 @@ __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
         int v = INT_MIN, r = -1;
         if (j < nL) {
           r = rloL + j;
-          v = bridge_score(scores, Lu, GLu, Ll, GLl, r, cL);
+          v = base == 0 ? cur : bridge_score(scores, Lu, GLu, Ll, GLl, r, cL);
           lraw[j] = v;
         }
         for (int d = 1; d < 64; d <<= 1) {
