@@ -368,7 +368,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        sample = args.cpu_sample or (400000 if args.workload == "c2" else 40000)
+        sample = args.cpu_sample or (700000 if args.workload == "c2" else 40000)
         cpu = cpu_baseline(wl, min(sample, n), threads)
         log("cpu baseline:", cpu)
 
