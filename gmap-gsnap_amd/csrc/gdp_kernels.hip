@@ -1238,12 +1238,13 @@ __device__ __forceinline__ int bridge_score(const int16_t *scores, const gdp_dev
 // For one (cL, cR) the rows rL <= D-1-top(cR) see the whole right column, so
 // their best join is the left column's prefix maximum up to that row plus
 // the right column's maximum (ties to the later row under jump-late, as the
-// scan keeps it): O(1) per (cL, cR) from the column maxima (LDS, once per gap)
-// and the prefix table of the current left column (LDS, one wave scan per
-// cL), with lanes over cR.  The few rows above that bound see a prefix of the
-// right column that grows as rL falls, walked once per (cL, cR).  Gaps wider
-// than the LDS tables take the direct form: per cR a prefix table of the
-// right column, lanes over cL and every left row.
+// scan keeps it): O(1) per (cL, cR) from the right columns' prefix tables
+// (LDS, built once per gap) and the current left column's (LDS, one wave scan
+// per cL), with lanes over cR.  Each row above that bound takes one lookup in
+// the right column's table at its own bound; those rows are counted per cR
+// and shared evenly over the lanes.  Gaps wider than the LDS tables take the
+// direct form: per cR a prefix table of the right column, lanes over cL and
+// every left row.
 __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
     const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ quads, int m,
     const int16_t *__restrict__ scores, int32_t *__restrict__ out, const gdp_tb_request *__restrict__ tmpl,
@@ -1287,12 +1288,12 @@ __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
   }
   if (fast && lane == 0) lds[gl] = nRsum;
   const int nLmax = ubL + lbL + 1;
-  const bool table = fast && (gl + 1) + nRsum + 2 * nLmax <= GDP_BRIDGE_LDS;
+  const bool table = fast && 2 * (gl + 1) + nRsum + 2 * nLmax <= GDP_BRIDGE_LDS;
   const bool ok = ok0 && (table || ubR + lbR + 1 <= GDP_BRIDGE_ROWS);
   if (ok && table) {
     const gdp_geom GLu = gdp_make_geom(Lu), GLl = gdp_make_geom(Ll), GRu = gdp_make_geom(Ru), GRl = gdp_make_geom(Rl);
     const int *offR = lds;
-    int *rpm = lds + gl + 1, *lp = rpm + nRsum, *lraw = lp + nLmax;
+    int *rpm = lds + gl + 1, *lp = rpm + nRsum, *lraw = lp + nLmax, *bpre = lraw + nLmax;
     __syncthreads();  // offsets written
     // prefix maxima of every right column, ties to the later row under
     // jump-late; eight rows' loads are issued before the first is used
@@ -1355,34 +1356,59 @@ __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
       }
       __syncthreads();
       const int64_t kc0 = (int64_t)cL * (gl + 1);
-      for (int cR = lane; cR <= gl - cL; cR += 64) {
-        const int o = offR[cR], nR = offR[cR + 1] - o;
-        if (nR <= 0) continue;
-        const int rloR = gdp_imax(1, cR - ubR), rtopR = rloR + nR - 1;
-        const int pen = cR == gl - cL ? 0 : open;
-        const int64_t kc = (kc0 + (gl - cR)) * RR;
-        const int B = D - 1 - rtopR;  // rows up to B see the whole right column
-        const int ub = gdp_imin(rtopL, B);
-        if (ub >= rloL) {
-          const int x = lp[ub - rloL], y = rpm[o + nR - 1];
-          const int v = (x >> 16) + (y >> 16) + pen;
-          const int64_t key = (kc + (x & 0xffff)) * RR + (y & 0xffff);
-          if (bridge_wins(v, key, bv, bkey, late)) {
-            bv = v;
-            bkey = key;
+      const int ncr = gl - cL + 1;  // right columns cR = 0 .. gl-cL
+      // lanes over cR: the rows up to B = D-1-top(cR) see the whole right
+      // column; the rows above B (each with its own bound) are counted into
+      // bpre so that the lanes can then share them evenly
+      int btot = 0;
+      for (int c0 = 0; c0 < ncr; c0 += 64) {
+        const int cR = c0 + lane;
+        int nb = 0;
+        if (cR < ncr) {
+          const int o = offR[cR], nR = offR[cR + 1] - o;
+          if (nR > 0) {
+            const int rloR = gdp_imax(1, cR - ubR), rtopR = rloR + nR - 1;
+            const int B = D - 1 - rtopR;
+            const int ub = gdp_imin(rtopL, B);
+            if (ub >= rloL) {
+              const int x = lp[ub - rloL], y = rpm[o + nR - 1];
+              const int v = (x >> 16) + (y >> 16) + (cR == gl - cL ? 0 : open);
+              const int64_t key = ((kc0 + (gl - cR)) * RR + (x & 0xffff)) * RR + (y & 0xffff);
+              if (bridge_wins(v, key, bv, bkey, late)) {
+                bv = v;
+                bkey = key;
+              }
+            }
+            // rows above B whose bound D-1-rL still reaches the column's first row
+            nb = gdp_imax(0, gdp_imin(rtopL, D - 1 - rloR) - gdp_imax(rloL, B + 1) + 1);
           }
         }
-        // rows above B see the right rows below D - rL only
-        for (int rL = gdp_imax(rloL, B + 1); rL <= rtopL; rL++) {
-          const int t = D - rL - 1;
-          if (t < rloR) break;  // the bound only falls as rL grows
-          const int y = rpm[o + t - rloR];
-          const int v = lraw[rL - rloL] + (y >> 16) + pen;
-          const int64_t key = (kc + rL) * RR + (y & 0xffff);
-          if (bridge_wins(v, key, bv, bkey, late)) {
-            bv = v;
-            bkey = key;
-          }
+        int x = nb;
+        for (int d = 1; d < 64; d <<= 1) {
+          const int o2 = __shfl_up(x, d, 64);
+          if (lane >= d) x += o2;
+        }
+        if (cR < ncr) bpre[cR] = btot + x - nb;
+        btot += __shfl(x, 63, 64);
+      }
+      __syncthreads();
+      // the bounded rows, item f in column cR = the last with bpre[cR] <= f
+      for (int f = lane; f < btot; f += 64) {
+        int lo = 0, hi = ncr - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (bpre[mid] <= f) lo = mid;
+          else hi = mid - 1;
+        }
+        const int cR = lo, o = offR[cR], nR = offR[cR + 1] - o;
+        const int rloR = gdp_imax(1, cR - ubR), rtopR = rloR + nR - 1;
+        const int rL = gdp_imax(rloL, D - rtopR) + (f - bpre[cR]);
+        const int y = rpm[o + (D - rL - 1) - rloR];  // right rows up to D-1-rL
+        const int v = lraw[rL - rloL] + (y >> 16) + (cR == gl - cL ? 0 : open);
+        const int64_t key = ((kc0 + (gl - cR)) * RR + rL) * RR + (y & 0xffff);
+        if (bridge_wins(v, key, bv, bkey, late)) {
+          bv = v;
+          bkey = key;
         }
       }
     }
