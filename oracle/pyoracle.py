@@ -246,6 +246,9 @@ class Reference(_Common):
                                      _p, _p, _i]
         L.refh_traceback.restype = _i
         L.refc_bridge_cdna.argtypes = [_i, _p, _p, _p, _p] + [_i] * 12 + [_p]
+        L.refe_best_endpoint.argtypes = [_i, _i, _p, _p, _i, _i, _i, _i, _i, _p]
+        L.refe_traceback_nogaps.argtypes = [_i, _i, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p, _i]
+        L.refe_traceback_nogaps.restype = _i
         L.refh_init(mode)
         self.ws_single = L.refh_new(max_rl, max_gl, 0)
         self.ws_double = L.refh_new(max_rl, max_gl, 1)
@@ -291,6 +294,27 @@ class Reference(_Common):
                                  lband, int(late), int(revp), _np_ptr(H), _np_ptr(D), garbage)
         self._last[KIND_LOWER] = width
         return H, D
+
+    def best_endpoint(self, queryend_indels, Hupper, Hlower, width, rl, gl, lband, uband, late):
+        """The reference's find_best_endpoint{,_to_queryend_indels}_{8,16}, as Oracle.best_endpoint."""
+        Hu = np.ascontiguousarray(Hupper, np.int16)
+        Hl = np.ascontiguousarray(Hlower, np.int16)
+        assert Hu.size == (gl + 1) * (rl + 1) and Hl.size == (rl + 1) * (gl + 1)
+        out = np.zeros(3, np.int32)
+        self.lib.refe_best_endpoint(int(queryend_indels), width, _np_ptr(Hu), _np_ptr(Hl), rl, gl, lband, uband,
+                                    int(late), _np_ptr(out))
+        return tuple(int(x) for x in out)
+
+    def traceback_nogaps(self, r, c, rseq, rsequc, gseq, galt, queryoffset=0, genomeoffset=0, revp=False,
+                         genestrand=0, maxpairs=8192):
+        """The reference's traceback_nogaps, as Oracle.traceback_nogaps."""
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        quc = _Seq(rsequc, revp)
+        out = np.zeros((maxpairs, PAIR_FIELDS), np.int32)
+        cnt = np.zeros(5, np.int32)
+        n = self.lib.refe_traceback_nogaps(r, c, q.ptr, quc.ptr, g.ptr, ga.ptr, queryoffset, genomeoffset,
+                                           int(revp), genestrand, _np_ptr(cnt), _np_ptr(out), maxpairs)
+        return out[:n].copy(), cnt
 
     def bridge_cdna(self, *args):
         """The reference's bridge_cdna_gap_{8,16}_ud, same arguments as Oracle.bridge_cdna."""

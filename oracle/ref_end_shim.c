@@ -1,0 +1,80 @@
+/*
+ * ref_end_shim.c -- reaches the reference's end-cell scans and nogaps walk.
+ *
+ * TEST INFRASTRUCTURE ONLY.  find_best_endpoint_{8,16},
+ * find_best_endpoint_to_queryend_indels_{8,16} and traceback_nogaps are
+ * static functions of the reference's dynprog_end.c, so this file compiles
+ * that source in place (an #include of the file as it lies in the reference
+ * tree; the Makefile puts that tree on the include path) and exposes them
+ * over plain arrays in oracle/_ref/libgmapref.so.  The end-gap callers
+ * themselves are never called: they read the genome through
+ * Genome_get_segment_*, which stay unresolved (lazy binding).
+ */
+#include "dynprog_end.c"
+
+static Pairpool_T end_pool = NULL;
+
+/* Row tables of the reference's matrix[c][r] (upper, stride rl+1) and
+   matrix[r][c] (lower, stride gl+1) shapes over dense int16 arrays; int8
+   copies for width 8 (freed by the caller through *keep). */
+static void *rows_of(const short *a, int n, int stride, int width, void **keep) {
+  void **t = (void **)malloc(sizeof(void *) * (size_t)n);
+  if (width == 8) {
+    Score8_T *b = (Score8_T *)malloc((size_t)n * stride);
+    for (size_t k = 0; k < (size_t)n * stride; k++) b[k] = (Score8_T)a[k];
+    for (int i = 0; i < n; i++) t[i] = b + (size_t)i * stride;
+    *keep = b;
+  } else {
+    for (int i = 0; i < n; i++) t[i] = (void *)(a + (size_t)i * stride);
+    *keep = NULL;
+  }
+  return t;
+}
+
+/* out = {finalscore, bestr, bestc} of the scan the end-gap callers run
+   (dynprog_end.c:143-290, 358-508). */
+void refe_best_endpoint(int queryend_indels, int width, const short *Hupper, const short *Hlower, int rl, int gl,
+                        int lband, int uband, int jump_late, int *out) {
+  void *ku, *kl;
+  void *mu = rows_of(Hupper, gl + 1, rl + 1, width, &ku), *ml = rows_of(Hlower, rl + 1, gl + 1, width, &kl);
+  int fs = 0, br = -1, bc = -1;
+  bool late = jump_late != 0;
+  if (width == 8) {
+    if (queryend_indels)
+      find_best_endpoint_to_queryend_indels_8(&fs, &br, &bc, (Score8_T **)mu, (Score8_T **)ml, rl, gl, lband, uband,
+                                              late);
+    else
+      find_best_endpoint_8(&fs, &br, &bc, (Score8_T **)mu, (Score8_T **)ml, rl, gl, lband, uband, late);
+  } else {
+    if (queryend_indels)
+      find_best_endpoint_to_queryend_indels_16(&fs, &br, &bc, (Score16_T **)mu, (Score16_T **)ml, rl, gl, lband,
+                                               uband, late);
+    else
+      find_best_endpoint_16(&fs, &br, &bc, (Score16_T **)mu, (Score16_T **)ml, rl, gl, lband, uband, late);
+  }
+  free(ku);
+  free(kl);
+  free(mu);
+  free(ml);
+  out[0] = fs;
+  out[1] = br;
+  out[2] = bc;
+}
+
+int refh_flatten(void *list, int *out, int maxpairs);
+
+/* traceback_nogaps (dynprog_end.c:648-722) from (r, c); pairs in push order
+   as refh_flatten writes them, counters += {score, nmatches, nmismatches}.
+   Returns the number of pairs. */
+int refe_traceback_nogaps(int r, int c, char *rseq, char *rsequc, char *gseq, char *galt, int queryoffset,
+                          int genomeoffset, int revp, int genestrand, int *counters, int *out, int maxpairs) {
+  int sc = 0, nm = 0, nmm = 0;
+  if (end_pool == NULL) end_pool = Pairpool_new();
+  Pairpool_reset(end_pool);
+  List_T pairs = traceback_nogaps(NULL, &sc, &nm, &nmm, r, c, rseq, rsequc, gseq, galt, queryoffset, genomeoffset,
+                                  end_pool, genestrand, revp != 0, 0);
+  counters[0] += sc;
+  counters[1] += nm;
+  counters[2] += nmm;
+  return refh_flatten(pairs, out, maxpairs);
+}
