@@ -19,7 +19,8 @@
 extern "C" int gdp_launch_fill(const void *, int, uint32_t, const char *, const char *, uint32_t *, int16_t *,
                                int32_t *, hipStream_t);
 extern "C" int gdp_launch_traceback(const void *, const gdp_tb_request *, int, const char *, const char *,
-                                    const uint32_t *, const void *, int, uint32_t, gdp_pair *, int32_t *, hipStream_t);
+                                    const uint32_t *, const void *, int, uint32_t, gdp_pair *, int32_t *, int32_t *,
+                                    hipStream_t);
 extern "C" int gdp_launch_segments(const void *, int, uint32_t, const gdp_segment *, int, char *, hipStream_t);
 extern "C" int gdp_launch_best_endpoint(const void *, const int32_t *, int, int, const int16_t *, int32_t *,
                                         const gdp_tb_request *, gdp_tb_request *, hipStream_t);
@@ -479,18 +480,26 @@ extern "C" int gdp_bridge_cdna(const void *d_descs, const int32_t *d_quads, int3
   return GDP_OK;
 }
 
-extern "C" int gdp_traceback_ex(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq, const char *d_qbuf,
-                                const char *d_gbuf, const uint32_t *d_dirs, const void *d_genome, int32_t genome_format,
-                                uint32_t genome_len, gdp_pair *d_pairs, int32_t *d_counts, void *stream) {
+extern "C" int gdp_traceback_ends(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq,
+                                  const char *d_qbuf, const char *d_gbuf, const uint32_t *d_dirs, const void *d_genome,
+                                  int32_t genome_format, uint32_t genome_len, gdp_pair *d_pairs, int32_t *d_counts,
+                                  int32_t *d_ends, void *stream) {
   if (!g_init) return fail(GDP_ENOINIT, "gdp_traceback: gdp_init has not been called");
   if (nreq < 0 || (nreq > 0 && (!d_descs || !d_reqs || !d_qbuf || !d_gbuf || !d_dirs || !d_pairs || !d_counts)))
     return fail(GDP_EINVAL, "gdp_traceback: null argument");
   if (genome_format != GDP_GENOME_ASCII && genome_format != GDP_GENOME_COMP)
     return fail(GDP_EINVAL, "gdp_traceback: unknown genome format %d", genome_format);
   if (gdp_launch_traceback(d_descs, d_reqs, nreq, d_qbuf, d_gbuf, d_dirs, d_genome, genome_format, genome_len, d_pairs,
-                           d_counts, (hipStream_t)stream))
+                           d_counts, d_ends, (hipStream_t)stream))
     return hip_fail("gdp_traceback launch", hipGetLastError());
   return GDP_OK;
+}
+
+extern "C" int gdp_traceback_ex(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq, const char *d_qbuf,
+                                const char *d_gbuf, const uint32_t *d_dirs, const void *d_genome, int32_t genome_format,
+                                uint32_t genome_len, gdp_pair *d_pairs, int32_t *d_counts, void *stream) {
+  return gdp_traceback_ends(d_descs, d_reqs, nreq, d_qbuf, d_gbuf, d_dirs, d_genome, genome_format, genome_len, d_pairs,
+                            d_counts, nullptr, stream);
 }
 
 extern "C" int gdp_traceback(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq, const char *d_qbuf,

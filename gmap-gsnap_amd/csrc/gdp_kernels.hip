@@ -889,7 +889,7 @@ __global__ __launch_bounds__(256, 8) void gdp_traceback_kernel(
     const gdp_dev_problem *__restrict__ probs, const gdp_tb_request *__restrict__ reqs, int nreq,
     const char *__restrict__ qbuf, const char *__restrict__ gbuf, const uint32_t *__restrict__ dirs,
     const void *__restrict__ genome, int gfmt, uint32_t glen, gdp_pair *__restrict__ pairs,
-    int32_t *__restrict__ counts) {
+    int32_t *__restrict__ counts, int32_t *__restrict__ ends) {
   __shared__ uint32_t consb[3 * 128 * 4];
   for (int j = threadIdx.x; j < 3 * 128 * 4; j += blockDim.x) consb[j] = gdp_tabConsBits[j];
   __syncthreads();
@@ -903,6 +903,11 @@ __global__ __launch_bounds__(256, 8) void gdp_traceback_kernel(
   const int consrow = q.genestrand * 128;
   const bool ops = (q.flags & GDP_TB_OPS) != 0;
   const bool nogaps = (q.flags & GDP_TB_NOGAPS) != 0;
+  // traceback_local_{8,16}_{upper,lower} (dynprog_end.c:727-1135): the walk
+  // stops at genome column endc, its diagonal pairs skip no '*' column and
+  // its genome skips read the fill's own genome sequence
+  const bool local = (q.flags & GDP_TB_LOCAL) != 0;
+  const int endc = local ? q.endc : 0;
   const int cap = q.max_pairs;
   gdp_pair *const obuf = pairs + q.pair_off;
   int32_t *cnt = counts + 6 * (int64_t)k;
@@ -911,7 +916,8 @@ __global__ __launch_bounds__(256, 8) void gdp_traceback_kernel(
   // (Dynprog_traceback_std, dynprog.c:1795-1947, walks them alike)
   const int kind = pp->kind == GDP_KIND_STD ? GDP_KIND_FULL : pp->kind;
   const int64_t p_qoff = pp->qoff, p_goff = pp->goff, p_galtoff = pp->galtoff;
-  if (q.r < 0 || q.r > pp->rl || q.c < 0 || q.c > pp->gl) {  // start cell outside the matrix
+  if (q.r < 0 || q.r > pp->rl || q.c < 0 || q.c > pp->gl ||  // start cell outside the matrix
+      (local && kind != GDP_KIND_UPPER && kind != GDP_KIND_LOWER)) {  // local walks are one-sided
     if (j == 0) cnt[0] = -1;
     return;
   }
@@ -927,13 +933,13 @@ __global__ __launch_bounds__(256, 8) void gdp_traceback_kernel(
   while (mode != TB_DONE) {
     int sk_kind = 0, sk_rr = 0, sk_cc = 0, sk_d = 0;
     if (mode == TB_DECIDE) {
-      if (r <= 0 || c <= 0) {
+      if (r <= 0 || c <= endc) {
         mode = TB_TAIL;
         continue;
       }
       // Lane j: the cell the walk reaches after j diagonal steps.
       const int rr = r - j, cc = c - j;
-      const bool inm = rr > 0 && cc > 0;
+      const bool inm = rr > 0 && cc > endc;
       int c1 = 0, c2 = 0, c2a = 0, m = TB_DECIDE;
       if (inm) {
         c1 = qbuf[p_qoff + rr - 1];
@@ -952,7 +958,7 @@ __global__ __launch_bounds__(256, 8) void gdp_traceback_kernel(
       const int J = __builtin_ctz(~dm | 0x10000u);  // diagonal steps taken this round
       // a diagonal pair (dynprog_simd.c:9245-9275); '*' columns past the
       // chromosome end push nothing (dynprog_simd.c:9248)
-      const bool paired = j < J && c2 != '*';
+      const bool paired = j < J && (local || c2 != '*');
       int comp = ' ';
       if (paired) {
         const int c1uc = (c1 >= 'a' && c1 <= 'z') ? c1 - 32 : c1;
@@ -981,7 +987,7 @@ __global__ __launch_bounds__(256, 8) void gdp_traceback_kernel(
         const int mJ = __shfl(m, g0 + J, 64);
         r -= J;
         c -= J;
-        if (r > 0 && c > 0) {
+        if (r > 0 && c > endc) {
           mode = mJ;
           dist = 1;
         }
@@ -996,7 +1002,8 @@ __global__ __launch_bounds__(256, 8) void gdp_traceback_kernel(
       // upper and lower follow their Egap unguarded.
       const bool h = mode == TB_RUN_H;
       const int rr = h ? r : r - j, cc = h ? c - j : c;
-      const bool guard = kind != GDP_KIND_FULL || (h ? cc > 0 : rr > 0);
+      // (the local upper walk stops its run at endc, dynprog_end.c:960)
+      const bool guard = local ? (!h || cc > endc) : (kind != GDP_KIND_FULL || (h ? cc > 0 : rr > 0));
       bool more = false;
       if (guard) {
         int t;
@@ -1034,9 +1041,14 @@ __global__ __launch_bounds__(256, 8) void gdp_traceback_kernel(
           sk_kind = 1, sk_rr = 1, sk_cc = c, sk_d = c;
         }
       } else if (kind == GDP_KIND_UPPER) {
-        if (c != 0) sk_kind = 1, sk_rr = 1, sk_cc = c, sk_d = c;
+        if (c != endc) sk_kind = 1, sk_rr = 1, sk_cc = c, sk_d = c - endc;
       } else {
-        if (r != 0) sk_kind = 2, sk_rr = r, sk_cc = 1, sk_d = r;
+        if (r != 0) sk_kind = 2, sk_rr = r, sk_cc = endc + 1, sk_d = r;
+      }
+      if (ends != nullptr && j == 0) {
+        // where the walk stopped: traceback_local leaves it in *r, *c
+        ends[2 * (int64_t)k] = r;
+        ends[2 * (int64_t)k + 1] = c;
       }
     }
     if (sk_kind == 1) {
@@ -1069,7 +1081,12 @@ __global__ __launch_bounds__(256, 8) void gdp_traceback_kernel(
         if (j < sk_d) {
           // get_genomic_nt (pairpool.c:1016-1063) over the device genome
           const uint32_t pos = watsonp ? chroffset + (uint32_t)gpos : chrhigh - (uint32_t)gpos;
-          if (pos < chroffset || pos >= chrhigh) ch = '*';
+          if (local) {
+            // Pairpool_add_genomeskip with the walk's genomesequence
+            // (pairpool.c:1142-1149): the fill's genome in sweep order
+            const int gc = (revp ? lo : hi) + j * stp;
+            ch = gbuf[p_goff + (gc < 0 ? -gc : gc)];
+          } else if (pos < chroffset || pos >= chrhigh) ch = '*';
           else {
             ch = genome_at(genome, gfmt, glen, pos);
             if (!watsonp) ch = compl_char(ch);
@@ -1644,11 +1661,12 @@ extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, con
 
 extern "C" int gdp_launch_traceback(const void *d_descs, const gdp_tb_request *d_reqs, int nreq, const char *d_qbuf,
                                     const char *d_gbuf, const uint32_t *d_dirs, const void *d_genome, int gfmt,
-                                    uint32_t glen, gdp_pair *d_pairs, int32_t *d_counts, hipStream_t stream) {
+                                    uint32_t glen, gdp_pair *d_pairs, int32_t *d_counts, int32_t *d_ends,
+                                    hipStream_t stream) {
   if (nreq <= 0) return 0;
   hipLaunchKernelGGL(gdp_traceback_kernel, dim3((nreq + 256 / TBG - 1) / (256 / TBG)), dim3(256), 0, stream,
                      (const gdp_dev_problem *)d_descs, d_reqs, nreq, d_qbuf, d_gbuf, d_dirs, d_genome, gfmt, glen,
-                     d_pairs, d_counts);
+                     d_pairs, d_counts, d_ends);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -120,10 +120,11 @@ class DeviceBatch:
                "gdp_export_dense")
         return (dd.cpu().numpy().reshape(C + 1, R + 1), ds.cpu().numpy().reshape(C + 1, R + 1))
 
-    def traceback(self, reqs, genome=None, genome_format=0, genome_len=None):
-        """Runs the requests; returns (pairs per request, counts[n,6]).
-        genome: ASCII bases (genome_format 0) or a genomecomp image of
-        genome_len bases (genome_format 1)."""
+    def traceback(self, reqs, genome=None, genome_format=0, genome_len=None, ends=False):
+        """Runs the requests; returns (pairs per request, counts[n,6]), and
+        with ends=True also the cells [n,2] where the walks stopped
+        (gdp_traceback_ends).  genome: ASCII bases (genome_format 0) or a
+        genomecomp image of genome_len bases (genome_format 1)."""
         torch = self.torch
         reqs = np.ascontiguousarray(reqs, TBREQ_DTYPE).copy()
         off = 0
@@ -142,16 +143,19 @@ class DeviceBatch:
                 d_gen = torch.from_numpy(np.ascontiguousarray(genome, np.uint32).view(np.uint8)).to(self.dev)
                 glen = genome_len
             gptr = d_gen.data_ptr()
-        _check(self.L, self.L.gdp_traceback_ex(self.d_descs.data_ptr(), d_reqs.data_ptr(), len(reqs),
-                                               self.d_q.data_ptr(), self.d_g.data_ptr(), self.d_dirs.data_ptr(), gptr,
-                                               genome_format, glen, d_pairs.data_ptr(), d_counts.data_ptr(),
-                                               self.stream), "gdp_traceback")
+        d_ends = torch.full((max(len(reqs), 1) * 2,), -1, dtype=torch.int32, device=self.dev) if ends else None
+        _check(self.L, self.L.gdp_traceback_ends(self.d_descs.data_ptr(), d_reqs.data_ptr(), len(reqs),
+                                                 self.d_q.data_ptr(), self.d_g.data_ptr(), self.d_dirs.data_ptr(),
+                                                 gptr, genome_format, glen, d_pairs.data_ptr(), d_counts.data_ptr(),
+                                                 d_ends.data_ptr() if ends else None, self.stream), "gdp_traceback")
         counts = d_counts.cpu().numpy()[: 6 * len(reqs)].reshape(-1, 6)
         pairs = d_pairs.cpu().numpy().view(PAIR_DTYPE)
         out = []
         for k in range(len(reqs)):
             n = min(int(counts[k, 0]), int(reqs["max_pairs"][k]))
             out.append(pairs[int(reqs["pair_off"][k]): int(reqs["pair_off"][k]) + n].copy())
+        if ends:
+            return out, counts, d_ends.cpu().numpy()[: 2 * len(reqs)].reshape(-1, 2)
         return out, counts
 
     def best_endpoint(self, ul, queryend_indels=False):

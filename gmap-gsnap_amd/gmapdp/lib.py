@@ -29,7 +29,7 @@ class TbRequest(ctypes.Structure):
                 ("queryoffset", ctypes.c_int32), ("genomeoffset", ctypes.c_int32),
                 ("chroffset", ctypes.c_uint32), ("chrhigh", ctypes.c_uint32), ("watsonp", ctypes.c_int8),
                 ("revp", ctypes.c_int8), ("genestrand", ctypes.c_int8), ("flags", ctypes.c_int8),
-                ("pair_off", ctypes.c_int64), ("max_pairs", ctypes.c_int32), ("pad2", ctypes.c_int32)]
+                ("pair_off", ctypes.c_int64), ("max_pairs", ctypes.c_int32), ("endc", ctypes.c_int32)]
 
 
 class Pair(ctypes.Structure):
@@ -45,7 +45,7 @@ PROBLEM_DTYPE = np.dtype([("qoff", "<i8"), ("goff", "<i8"), ("galtoff", "<i8"), 
 TBREQ_DTYPE = np.dtype([("fill", "<i4"), ("r", "<i4"), ("c", "<i4"), ("queryoffset", "<i4"),
                         ("genomeoffset", "<i4"), ("chroffset", "<u4"), ("chrhigh", "<u4"), ("watsonp", "i1"),
                         ("revp", "i1"), ("genestrand", "i1"), ("flags", "i1"), ("pair_off", "<i8"),
-                        ("max_pairs", "<i4"), ("pad2", "<i4")])
+                        ("max_pairs", "<i4"), ("endc", "<i4")])
 PAIR_DTYPE = np.dtype([("querypos", "<i4"), ("genomepos", "<i4"), ("cdna", "i1"), ("comp", "i1"),
                        ("genome", "i1"), ("genomealt", "i1"), ("gapp", "<i2"), ("genomejump", "<i2")])
 SEGMENT_DTYPE = np.dtype([("coord", "<u4"), ("bound", "<u4"), ("length", "<i4"), ("left", "i1"), ("revcomp", "i1"),
@@ -118,6 +118,7 @@ def load(path=None):
         "gdp_bridge_cdna": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, vp]),
         "gdp_traceback": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp]),
         "gdp_traceback_ex": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp]),
+        "gdp_traceback_ends": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp, vp]),
         "gdp_genome_segments": (ctypes.c_int, [vp, i32, ctypes.c_uint32, vp, i32, vp, vp]),
         "gdp_genomecomp_words": (i64, [ctypes.c_uint64]),
         "gdp_genomecomp_pack": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),

@@ -121,7 +121,7 @@ typedef struct gdp_tb_request {
   int8_t flags; /* GDP_TB_OPS: emit pair-list operations instead of pairs */
   int64_t pair_off;
   int32_t max_pairs;
-  int32_t pad2;
+  int32_t endc; /* GDP_TB_LOCAL: the genome column the walk stops at */
 } gdp_tb_request;
 
 /* With GDP_TB_OPS the walk records, instead of pairs, the calls the
@@ -138,6 +138,17 @@ typedef struct gdp_tb_request {
    find_best_endpoint_to_queryend_nogaps (dynprog_end.c:576-586):
    r = c = min(rl, gl). */
 #define GDP_TB_NOGAPS 2
+
+/* With GDP_TB_LOCAL an upper or lower walk is traceback_local_{8,16}_upper /
+   _lower (dynprog_end.c:727-1135), the walk of the splice-junction end
+   variants: it stops at genome column endc instead of 0 (the upper walk's
+   gap runs stop there too), its closing skip is c - endc genome columns
+   (upper) or r query rows at column endc + 1 (lower), its diagonal pairs
+   skip no '*' column, and its genome skips read the fill's own genome
+   sequence rather than the device genome.  gdp_traceback_ends reports the
+   cell where the walk stopped, from which the caller's next walk starts.
+   A full-band fill with this flag gets counts[0] = -1. */
+#define GDP_TB_LOCAL 4
 
 /* One traceback output record, in the order the reference pushes them onto
    its pair list (from the end cell back to the start).  gapp = 1 marks a
@@ -384,6 +395,15 @@ int gdp_traceback(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nre
 int gdp_traceback_ex(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq, const char *d_qbuf,
                      const char *d_gbuf, const uint32_t *d_dirs, const void *d_genome, int32_t genome_format,
                      uint32_t genome_len, gdp_pair *d_pairs, int32_t *d_counts, void *stream);
+
+/* As gdp_traceback_ex; d_ends (2 int32 per request, or NULL) receives the
+   cell (r, c) where each walk stopped before its closing skip -- what
+   traceback_local_{8,16}_{upper,lower} (dynprog_end.c:727-1135) leave in
+   *r and *c, from which Dynprog_end5/3_splicejunction (1652-1922,
+   2248-2745) start their second walk past the gap holder. */
+int gdp_traceback_ends(const void *d_descs, const gdp_tb_request *d_reqs, int32_t nreq, const char *d_qbuf,
+                       const char *d_gbuf, const uint32_t *d_dirs, const void *d_genome, int32_t genome_format,
+                       uint32_t genome_len, gdp_pair *d_pairs, int32_t *d_counts, int32_t *d_ends, void *stream);
 
 /* Writes one problem's directions (and scores if kept) in the reference's
    dense layout: [c][r] with stride rl+1 for full/upper, [r][c] with stride

@@ -580,6 +580,87 @@ int orc_traceback(int kind, const unsigned char *dirs, int rl, int gl, int r, in
   return pb.n;
 }
 
+/* traceback_local_{8,16}_upper / _lower (dynprog_end.c:932-1135; the int8
+   forms 727-931 are the same walk): the one-sided walk of the splice-junction
+   end variants.  It stops at genome column endc instead of 0 -- the upper
+   walk's gap runs stop there too (960) --, pushes every diagonal pair (no
+   '*' test), reads its genome-skip characters from the fill's own genome
+   sequence (Pairpool_add_genomeskip with genomesequence, pairpool.c:1142-1149,
+   genomealt = genome), and closes with a genome skip of c - endc columns at
+   row 1 (upper, 1019-1031) or a query skip of r rows at column endc + 1
+   (lower, 1121-1133).  end_rc receives where the walk stopped. */
+int orc_traceback_local(int kind, const unsigned char *dirs, int rl, int gl, int r, int c, int endc,
+                        const char *rseq, const char *rsequc, const char *gseq, const char *gseq_alt,
+                        int queryoffset, int genomeoffset, int revp, int genestrand,
+                        int *counters, int *pairs, int maxpairs, int *end_rc) {
+  pairbuf_t pb = {pairs, 0, maxpairs};
+  const int ldc = rl + 1, ldr = gl + 1;
+#define DCR(cc, rr) dirs[(size_t)(cc) * ldc + (rr)]
+#define DRC(rr, cc) dirs[(size_t)(rr) * ldr + (cc)]
+  while (r > 0 && c > endc) {
+    const int d = kind == ORC_KIND_UPPER ? DCR(c, r) : DRC(r, c);
+    if (d & (DIR_H | DIR_V)) {
+      int dist = 1;
+      if (kind == ORC_KIND_UPPER) {
+        while (c > endc && (DCR(c--, r) & DIR_E)) dist++;
+        /* the genome skip of (r, c + dist, dist), characters from gseq */
+        int qc = r - 1, left = c + dist - dist, right = c + dist - 1, step = -1;
+        if (revp) { int t = left; qc = -qc; left = -right; right = -t; step = +1; }
+        if (dist < MICROINTRON_LENGTH) {
+          int gc = revp ? left : right;
+          for (int j = 0; j < dist; j++, gc += step)
+            push_pair(&pb, queryoffset + qc, genomeoffset + gc, ' ', '-', gseq[gc], gseq[gc]);
+          counters[0] += TB_TOPEN + dist * TB_TINDEL; counters[3] += 1; counters[4] += dist;
+        } else {
+          push_gapholder(&pb, dist);
+        }
+      } else {
+        while (DRC(r--, c) & DIR_E) dist++;
+        queryskip(&pb, r + dist, c, dist, rseq, queryoffset, genomeoffset, revp);
+        counters[0] += TB_QOPEN + dist * TB_QINDEL; counters[3] += 1; counters[4] += dist;
+      }
+    } else {
+      int qc = r - 1, gc = c - 1;
+      if (revp) { qc = -qc; gc = -gc; }
+      const int c1 = rseq[qc], c1uc = rsequc[qc], c2 = gseq[gc], c2a = gseq_alt[gc];
+      if (c1uc == c2 || c1uc == c2a) {
+        counters[0] += TB_MATCH; counters[1] += 1;
+        push_pair(&pb, queryoffset + qc, genomeoffset + gc, c1, '*', c2, c2a);
+      } else if (orc_consistent(genestrand, c1uc, c2) || orc_consistent(genestrand, c1uc, c2a)) {
+        counters[0] += TB_MATCH; counters[1] += 1;
+        push_pair(&pb, queryoffset + qc, genomeoffset + gc, c1, ':', c2, c2a);
+      } else {
+        counters[0] += TB_MISMATCH; counters[2] += 1;
+        push_pair(&pb, queryoffset + qc, genomeoffset + gc, c1, ' ', c2, c2a);
+      }
+      r--; c--;
+    }
+  }
+  end_rc[0] = r;
+  end_rc[1] = c;
+  if (kind == ORC_KIND_UPPER) {
+    if (c != endc) {
+      const int dist = c - endc;
+      int qc = 0, left = c - dist, right = c - 1, step = -1;
+      if (revp) { int t = left; qc = -qc; left = -right; right = -t; step = +1; }
+      if (dist < MICROINTRON_LENGTH) {
+        int gc = revp ? left : right;
+        for (int j = 0; j < dist; j++, gc += step)
+          push_pair(&pb, queryoffset + qc, genomeoffset + gc, ' ', '-', gseq[gc], gseq[gc]);
+        counters[0] += TB_TOPEN + dist * TB_TINDEL; counters[3] += 1; counters[4] += dist;
+      } else {
+        push_gapholder(&pb, dist);
+      }
+    }
+  } else if (r != 0) {
+    queryskip(&pb, r, endc + 1, r, rseq, queryoffset, genomeoffset, revp);
+    counters[0] += TB_QOPEN + r * TB_QINDEL; counters[3] += 1; counters[4] += r;
+  }
+#undef DCR
+#undef DRC
+  return pb.n;
+}
+
 /* ------------------------------------------------------ best end-point scan */
 
 void orc_best_endpoint(int queryend_indels, const short *Hupper, const short *Hlower,

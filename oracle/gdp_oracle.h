@@ -94,6 +94,15 @@ int orc_traceback(int kind, const unsigned char *dirs, int rl, int gl, int r, in
                   int revp, unsigned int chroffset, unsigned int chrhigh, int watsonp,
                   int genestrand, int dynprogindex, int *counters, int *pairs, int maxpairs);
 
+/* traceback_local_{8,16}_{upper,lower} (dynprog_end.c:727-1135): kind 1
+   (upper, dirs [c][r]) or 2 (lower, dirs [r][c]) walk stopping at genome
+   column endc; end_rc[2] = the cell where it stopped.  Returns the number of
+   pairs (push order). */
+int orc_traceback_local(int kind, const unsigned char *dirs, int rl, int gl, int r, int c, int endc,
+                        const char *rseq, const char *rsequc, const char *gseq, const char *gseq_alt,
+                        int queryoffset, int genomeoffset, int revp, int genestrand,
+                        int *counters, int *pairs, int maxpairs, int *end_rc);
+
 /* find_best_endpoint_{8,16} (dynprog_end.c:143-290) when queryend_indels==0,
    find_best_endpoint_to_queryend_indels_{8,16} (dynprog_end.c:358-508) otherwise.
    Hupper is [c][r] stride rl+1, Hlower is [r][c] stride gl+1. */

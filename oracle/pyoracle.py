@@ -90,6 +90,8 @@ class Oracle(_Common):
         L.orc_best_endpoint.argtypes = [_i, _p, _p, _i, _i, _i, _i, _i, _i, _p, _p, _p]
         L.orc_fill_std.argtypes = [_p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]
         L.orc_bridge_cdna.argtypes = [_i, _p, _p, _p, _p] + [_i] * 12 + [_p]
+        L.orc_traceback_local.argtypes = [_i, _p, _i, _i, _i, _i, _i, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p, _i, _p]
+        L.orc_traceback_local.restype = _i
         L.orc_segment_right.argtypes = [_p, _p, _u, _u, _u, _i]
         L.orc_segment_left.argtypes = [_p, _p, _u, _u, _u, _i]
         for f in ("orc_cells_full",):
@@ -203,6 +205,21 @@ class Oracle(_Common):
                                    ctypes.byref(bc))
         return fs.value, br.value, bc.value
 
+    def traceback_local(self, kind, dirs, r, c, endc, rseq, rsequc, gseq, galt, queryoffset=0, genomeoffset=0,
+                        revp=False, genestrand=0, maxpairs=8192):
+        """orc_traceback_local: (pair records, counters[5], (r, c) where the walk stopped)."""
+        rl, gl = len(rseq), len(gseq)
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        quc = _Seq(rsequc, revp)
+        dirs = np.ascontiguousarray(dirs, np.uint8)
+        out = np.zeros((maxpairs, PAIR_FIELDS), np.int32)
+        cnt = np.zeros(5, np.int32)
+        end = np.zeros(2, np.int32)
+        n = self.lib.orc_traceback_local(kind, _np_ptr(dirs), rl, gl, r, c, endc, q.ptr, quc.ptr, g.ptr, ga.ptr,
+                                         queryoffset, genomeoffset, int(revp), genestrand, _np_ptr(cnt),
+                                         _np_ptr(out), maxpairs, _np_ptr(end))
+        return out[:n].copy(), cnt, (int(end[0]), int(end[1]))
+
     def bridge_cdna(self, *args):
         """orc_bridge_cdna: (bestscore, bestcL, bestcR, bestrL, bestrR)."""
         return _bridge_cdna_call(self.lib.orc_bridge_cdna, *args)
@@ -249,6 +266,8 @@ class Reference(_Common):
         L.refe_best_endpoint.argtypes = [_i, _i, _p, _p, _i, _i, _i, _i, _i, _p]
         L.refe_traceback_nogaps.argtypes = [_i, _i, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p, _i]
         L.refe_traceback_nogaps.restype = _i
+        L.refe_traceback_local.argtypes = [_i, _i, _i, _i, _i, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p, _i, _p]
+        L.refe_traceback_local.restype = _i
         L.refh_init(mode)
         self.ws_single = L.refh_new(max_rl, max_gl, 0)
         self.ws_double = L.refh_new(max_rl, max_gl, 1)
@@ -315,6 +334,19 @@ class Reference(_Common):
         n = self.lib.refe_traceback_nogaps(r, c, q.ptr, quc.ptr, g.ptr, ga.ptr, queryoffset, genomeoffset,
                                            int(revp), genestrand, _np_ptr(cnt), _np_ptr(out), maxpairs)
         return out[:n].copy(), cnt
+
+    def traceback_local(self, kind, r, c, endc, rseq, rsequc, gseq, galt, queryoffset=0, genomeoffset=0,
+                        revp=False, genestrand=0, maxpairs=8192):
+        """The reference's traceback_local_* over the *last* fill of this kind (1 upper, 2 lower)."""
+        q, g, ga = self._seqs(rseq, gseq, galt, revp)
+        quc = _Seq(rsequc, revp)
+        out = np.zeros((maxpairs, PAIR_FIELDS), np.int32)
+        cnt = np.zeros(5, np.int32)
+        end = np.zeros(2, np.int32)
+        n = self.lib.refe_traceback_local(kind, self._last[kind], r, c, endc, q.ptr, quc.ptr, g.ptr, ga.ptr,
+                                          queryoffset, genomeoffset, int(revp), genestrand, _np_ptr(cnt),
+                                          _np_ptr(out), maxpairs, _np_ptr(end))
+        return out[:n].copy(), cnt, (int(end[0]), int(end[1]))
 
     def bridge_cdna(self, *args):
         """The reference's bridge_cdna_gap_{8,16}_ud, same arguments as Oracle.bridge_cdna."""

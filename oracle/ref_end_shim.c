@@ -78,3 +78,47 @@ int refe_traceback_nogaps(int r, int c, char *rseq, char *rsequc, char *gseq, ch
   counters[2] += nmm;
   return refh_flatten(pairs, out, maxpairs);
 }
+
+void refh_last_tables(int kind, void ***nogap, void ***egap);
+
+/* traceback_local_{8,16}_{upper,lower} (dynprog_end.c:727-1135) over the
+   last upper (kind 1) or lower (kind 2) fill of ref_shim.c, from (r, c) to
+   genome column endc; pairs in push order, counters += {score, nmatches,
+   nmismatches, nopens, nindels}, end_rc = where the walk stopped. */
+int refe_traceback_local(int kind, int width, int r, int c, int endc, char *rseq, char *rsequc, char *gseq,
+                         char *galt, int queryoffset, int genomeoffset, int revp, int genestrand, int *counters,
+                         int *out, int maxpairs, int *end_rc) {
+  int sc = 0, nm = 0, nmm = 0, no = 0, ni = 0;
+  void **nogap, **egap;
+  List_T pairs;
+  if (end_pool == NULL) end_pool = Pairpool_new();
+  Pairpool_reset(end_pool);
+  refh_last_tables(kind, &nogap, &egap);
+  if (kind == 1) {
+    if (width == 8)
+      pairs = traceback_local_8_upper(NULL, &sc, &nm, &nmm, &no, &ni, (Direction8_T **)nogap, (Direction8_T **)egap,
+                                      &r, &c, endc, rseq, rsequc, gseq, galt, queryoffset, genomeoffset, NULL, NULL,
+                                      end_pool, revp != 0, 0, 0, true, genestrand, 0);
+    else
+      pairs = traceback_local_16_upper(NULL, &sc, &nm, &nmm, &no, &ni, (Direction16_T **)nogap,
+                                       (Direction16_T **)egap, &r, &c, endc, rseq, rsequc, gseq, galt, queryoffset,
+                                       genomeoffset, NULL, NULL, end_pool, revp != 0, 0, 0, true, genestrand, 0);
+  } else {
+    if (width == 8)
+      pairs = traceback_local_8_lower(NULL, &sc, &nm, &nmm, &no, &ni, (Direction8_T **)nogap, (Direction8_T **)egap,
+                                      &r, &c, endc, rseq, rsequc, gseq, galt, queryoffset, genomeoffset, end_pool,
+                                      genestrand, revp != 0, 0);
+    else
+      pairs = traceback_local_16_lower(NULL, &sc, &nm, &nmm, &no, &ni, (Direction16_T **)nogap,
+                                       (Direction16_T **)egap, &r, &c, endc, rseq, rsequc, gseq, galt, queryoffset,
+                                       genomeoffset, end_pool, genestrand, revp != 0, 0);
+  }
+  counters[0] += sc;
+  counters[1] += nm;
+  counters[2] += nmm;
+  counters[3] += no;
+  counters[4] += ni;
+  end_rc[0] = r;
+  end_rc[1] = c;
+  return refh_flatten(pairs, out, maxpairs);
+}

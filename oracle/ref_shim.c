@@ -206,6 +206,13 @@ int refh_traceback(int kind, int width, int r, int c, char *rseq, char *rsequc, 
   return n;
 }
 
+/* The direction tables of the last fill of `kind` (1 upper, 2 lower), for
+   the walks of ref_end_shim.c. */
+void refh_last_tables(int kind, void ***nogap, void ***egap) {
+  *nogap = last_nogap[kind];
+  *egap = last_Egap[kind];
+}
+
 /* ------------------------------------------------- batch timing (baseline) */
 
 typedef struct {

@@ -1,0 +1,54 @@
+"""Local end walks on the GPU (GDP_TB_LOCAL through gdp_traceback_ends:
+traceback_local_{8,16}_{upper,lower}, dynprog_end.c:727-1135) against the
+oracle's restatement, which tests/test_oracle_end.py pins against the
+reference's own functions: pairs, counters and the cell where each walk
+stopped."""
+import numpy as np
+import pytest
+
+import helpers as H
+from test_oracle_end import _local_cases
+
+pytestmark = pytest.mark.gpu
+
+GDP_TB_LOCAL = 4
+
+
+def test_local_walks(oracle):
+    from gmapdp.batch import DeviceBatch
+    rng = np.random.default_rng(34)
+    specs, walks = [], []
+    for t, kind, width, q, g, band, revp, late, r, c, endc in _local_cases(rng, 1200):
+        lb, ub = (0, band) if kind == 1 else (band, 0)
+        if kind == 1 and not (r <= c <= r + ub):
+            continue
+        if kind == 2 and not (c <= r <= c + lb):
+            continue
+        walks.append((len(specs), r, c, endc))
+        specs.append(H.Spec(kind, width, q, g, g, 3, -10, -2, lb, ub, late, revp))
+    assert len(walks) > 300
+    probs, qbuf, gbuf = H.pack_batch(specs)
+    b = DeviceBatch(probs, qbuf, gbuf)
+    b.fill()
+    reqs = []
+    for i, r, c, endc in walks:
+        q = H.tb_request(i, specs[i], r, c, qoff=3, goff=11)
+        q["flags"], q["endc"] = GDP_TB_LOCAL, endc
+        reqs.append(q)
+    pairs, counts, ends = b.traceback(np.array(reqs), ends=True)
+    for j, (i, r, c, endc) in enumerate(walks):
+        s = specs[i]
+        _, D = H.oracle_fill(oracle, s)
+        D = np.where(H.computed_mask(s, b.layout[i]), D, 0).astype(np.uint8)
+        po, co, eo = oracle.traceback_local(s.kind, D, r, c, endc, s.q, s.q.upper(), s.g, s.galt, 3, 11, s.revp)
+        assert (H.pairs_to_records(pairs[j]) == po).all(), ("local walk pairs", j, s.kind, r, c, endc)
+        assert (counts[j, 1:] == co).all() and tuple(ends[j]) == eo, ("local walk counters", j)
+    # a full-band fill refuses the flag
+    full = [H.Spec(0, 16, s.q, s.g, s.g, 3, -10, -2, 3, 3, False, False) for s in specs[:1]]
+    probs, qbuf, gbuf = H.pack_batch(full)
+    bf = DeviceBatch(probs, qbuf, gbuf)
+    bf.fill()
+    q = H.tb_request(0, full[0], full[0].rl, full[0].gl)
+    q["flags"] = GDP_TB_LOCAL
+    _, cf = bf.traceback(np.array([q]))
+    assert cf[0, 0] == -1

@@ -46,3 +46,45 @@ def test_traceback_nogaps_matches_reference(oracle, reference):
         po, co = oracle.traceback_nogaps(r, c, q, q.upper(), g, ga, qo, go, revp, gs)
         pr, cr = reference.traceback_nogaps(r, c, q, q.upper(), g, ga, qo, go, revp, gs)
         assert (po == pr).all() and (co[:3] == cr[:3]).all() and (co[3:] == 0).all(), ("nogaps", t)
+
+
+def _local_cases(rng, n):
+    """One-sided fills of end-like shapes and local walks from random cells to
+    random stop columns, both orientations and widths."""
+    for t in range(n):
+        rl = int(rng.integers(3, 45))
+        gl = rl + int(rng.integers(0, 15))
+        g = H.rand_seq(rng, gl, 0.02)
+        q = H.mutate(rng, g, 0.08, 0.06)[:rl].ljust(rl, b"A")
+        if t % 5 == 0:
+            q = q.lower()
+        kind = 1 + t % 2
+        width = 8 if t % 3 == 0 else 16
+        band = int(rng.integers(0, 12)) + (gl - rl if kind == 1 else 0)
+        revp, late = bool(t % 4 >= 2), bool(t % 7 % 2)
+        r, c = int(rng.integers(0, rl + 1)), int(rng.integers(0, gl + 1))
+        endc = int(rng.integers(0, max(1, min(c, gl) + 3)))
+        yield t, kind, width, q, g, band, revp, late, r, c, endc
+
+
+def test_traceback_local_matches_reference(oracle, reference):
+    rng = np.random.default_rng(33)
+    walks = 0
+    for t, kind, width, q, g, band, revp, late, r, c, endc in _local_cases(rng, 1500):
+        lb, ub = (0, band) if kind == 1 else (band, 0)
+        s = H.Spec(kind, width, q, g, g, 3, -10, -2, lb, ub, late, revp)
+        Ho, Do = H.oracle_fill(oracle, s)
+        if kind == 1:
+            reference.fill_upper(width, q, g, g, 3, -10, -2, ub, late, revp)
+        else:
+            reference.fill_lower(width, q, g, g, 3, -10, -2, lb, late, revp)
+        # walks stay inside the band's computed cells from a cell in the band
+        if kind == 1 and not (r <= c <= r + ub):
+            continue
+        if kind == 2 and not (c <= r <= c + lb):
+            continue
+        po, co, eo = oracle.traceback_local(kind, Do, r, c, endc, q, q.upper(), g, g, 3, 11, revp)
+        pr, cr, er = reference.traceback_local(kind, r, c, endc, q, q.upper(), g, g, 3, 11, revp)
+        assert (po == pr).all() and (co == cr).all() and eo == er, ("local walk", t, kind, r, c, endc)
+        walks += 1
+    assert walks > 300
