@@ -268,6 +268,8 @@ class Reference(_Common):
         L.refe_traceback_nogaps.restype = _i
         L.refe_traceback_local.argtypes = [_i, _i, _i, _i, _i, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p, _i, _p]
         L.refe_traceback_local.restype = _i
+        L.refe_end5_splicejunction.argtypes = [_p, _p, _p, _p, _p] + [_i] * 8 + [ctypes.c_double, _i, _p, _p, _i]
+        L.refe_end5_splicejunction.restype = _i
         L.refh_init(mode)
         self.ws_single = L.refh_new(max_rl, max_gl, 0)
         self.ws_double = L.refh_new(max_rl, max_gl, 1)
@@ -347,6 +349,24 @@ class Reference(_Common):
                                           queryoffset, genomeoffset, int(revp), genestrand, _np_ptr(cnt),
                                           _np_ptr(out), maxpairs, _np_ptr(end))
         return out[:n].copy(), cnt, (int(end[0]), int(end[1]))
+
+    def end5_splicejunction(self, rev_rseq, rev_gseq, rev_galt, rev_roffset, anchor, far, genestrand, late,
+                            extraband_end, defect_rate, contlength, maxpairs=8192):
+        """The reference's Dynprog_end5_splicejunction; strings in the caller's
+        orientation (read backwards from their last base).  Returns None for a
+        NULL list, else (pair records, {traceback_score, missscore, nmatches,
+        nmismatches, nopens, nindels})."""
+        q, g, ga = self._seqs(rev_rseq, rev_gseq, rev_galt, True)
+        quc = _Seq(rev_rseq.upper(), True)
+        out = np.zeros((maxpairs, PAIR_FIELDS), np.int32)
+        res = np.zeros(6, np.int32)
+        n = self.lib.refe_end5_splicejunction(self.ws_double, q.ptr, quc.ptr, g.ptr, ga.ptr, len(rev_rseq),
+                                              len(rev_gseq), rev_roffset, anchor, far, genestrand, int(late),
+                                              extraband_end, float(defect_rate), contlength, _np_ptr(res),
+                                              _np_ptr(out), maxpairs)
+        if n < 0:
+            return None
+        return out[:n].copy(), res
 
     def bridge_cdna(self, *args):
         """The reference's bridge_cdna_gap_{8,16}_ud, same arguments as Oracle.bridge_cdna."""

@@ -122,3 +122,30 @@ int refe_traceback_local(int kind, int width, int r, int c, int endc, char *rseq
   end_rc[1] = c;
   return refh_flatten(pairs, out, maxpairs);
 }
+
+/* Dynprog_end5_splicejunction (dynprog_end.c:1652-1922) on workspace ws
+   (a doublep Dynprog_T of ref_shim.c), sequences as the caller passes them
+   (read backwards from the pointers).  out: pairs in push order of the
+   returned list; res = {traceback_score, missscore, nmatches, nmismatches,
+   nopens, nindels}.  Returns the number of pairs, or -1 for a NULL list. */
+int refe_end5_splicejunction(void *ws, char *rev_rseq, char *rev_rsequc, char *rev_gseq, char *rev_galt, int rlength,
+                             int glength, int rev_roffset, int rev_goffset_anchor, int rev_goffset_far,
+                             int genestrand, int jump_late, int extraband_end, double defect_rate, int contlength,
+                             int *res, int *out, int maxpairs) {
+  int dpi = 1, ts = 0, miss = 0, nm = 0, nmm = 0, no = 0, ni = 0;
+  if (end_pool == NULL) end_pool = Pairpool_new();
+  Pairpool_reset(end_pool);
+  List_T pairs = Dynprog_end5_splicejunction(&dpi, &ts, &miss, &nm, &nmm, &no, &ni, (Dynprog_T)ws, rev_rseq,
+                                             rev_rsequc, rev_gseq, rev_galt, rlength, glength, rev_roffset,
+                                             rev_goffset_anchor, rev_goffset_far, 0, 0, true, genestrand,
+                                             jump_late != 0, NULL, NULL, end_pool, extraband_end, defect_rate,
+                                             contlength);
+  res[0] = ts;
+  res[1] = miss;
+  res[2] = nm;
+  res[3] = nmm;
+  res[4] = no;
+  res[5] = ni;
+  if (pairs == NULL) return -1;
+  return refh_flatten(pairs, out, maxpairs);
+}

@@ -52,3 +52,22 @@ def test_local_walks(oracle):
     q["flags"] = GDP_TB_LOCAL
     _, cf = bf.traceback(np.array([q]))
     assert cf[0, 0] == -1
+
+
+def test_end5_splicejunction(oracle):
+    """Dynprog_end5_splicejunction composed from the batched kernels
+    (gmapdp.ends) against the oracle's composition, which
+    tests/test_oracle_end.py pins against the reference's own caller."""
+    from gmapdp.ends import end5_splicejunction
+    rng = np.random.default_rng(36)
+    ends = H.random_splice_ends(rng, 400)
+    got = end5_splicejunction(ends)
+    walked = 0
+    for k, e in enumerate(ends):
+        want = H.oracle_end5_splicejunction(oracle, e)
+        assert (got[k] is None) == (want is None), ("null", k)
+        if want is None:
+            continue
+        assert (H.pairs_to_records(got[k][0]) == want[0]).all() and (got[k][1] == want[1]).all(), ("end", k)
+        walked += 1
+    assert walked > 200

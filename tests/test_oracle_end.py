@@ -88,3 +88,22 @@ def test_traceback_local_matches_reference(oracle, reference):
         assert (po == pr).all() and (co == cr).all() and eo == er, ("local walk", t, kind, r, c, endc)
         walks += 1
     assert walks > 300
+
+
+def test_end5_splicejunction_matches_reference(oracle, reference):
+    """The oracle's composition of Dynprog_end5_splicejunction against the
+    reference's own caller."""
+    rng = np.random.default_rng(35)
+    ends = H.random_splice_ends(rng, 300)
+    got = 0
+    for k, e in enumerate(ends):
+        want = reference.end5_splicejunction(e["rev_rseq"], e["rev_gseq"], e["rev_galt"], e["rev_roffset"],
+                                             e["anchor"], e["far"], e["genestrand"], e["jump_late"],
+                                             e["extraband"], e["defect_rate"], e["contlength"])
+        mine = H.oracle_end5_splicejunction(oracle, e)
+        assert (want is None) == (mine is None), ("null", k)
+        if want is None:
+            continue
+        assert (mine[0] == want[0]).all() and (mine[1] == want[1]).all(), ("splice end", k)
+        got += 1
+    assert 150 < got < 300
