@@ -1,5 +1,6 @@
 """Splice-junction end alignment composed from the batched kernels:
-Dynprog_end5_splicejunction (dynprog_end.c:1652-1922) for many ends at once.
+Dynprog_end5_splicejunction (dynprog_end.c:1652-1922) and
+Dynprog_end3_splicejunction (2248-2745) for many ends at once.
 
 Per end the reference fills the upper and lower triangles of the reversed
 read against the reversed genome (ENDQ scores, end penalties by defect
@@ -8,7 +9,9 @@ shorter than use8p_size, the opposite tie rule), takes the best cell of the
 last row (find_best_endpoint_to_queryend_indels), and when that score is
 not negative walks back twice: a local walk to the continuation column,
 a known gap holder, and a local walk from where the first stopped to the
-anchor.  It then trims indel pairs from the 5' end.  Here the fills keep
+anchor.  It then trims indel pairs from the read's end.  The 3' form is
+the same on the forward strings with the caller's own tie rule, its gap
+holder spanning far - anchor, and its list returned in the trimmed order.  Here the fills keep
 only their best last-row cell (GDP_KEEP_BEST_QUERYEND), one launch merges
 the triangles, and each set of walks is one traceback launch
 (GDP_TB_LOCAL); the host joins the two walks.
@@ -46,9 +49,22 @@ def end5_splicejunction(ends, max_rlength=1000, max_glength=2000):
     the caller's orientation, read backwards from the last base), rev_roffset,
     anchor, far (rev_goffset_anchor / rev_goffset_far), genestrand,
     jump_late, extraband, defect_rate, contlength.  Returns per end None (the
-    reference's NULL list) or (pairs in the reference's push order as
-    gdp_pair records, {traceback_score, missscore, nmatches, nmismatches,
-    nopens, nindels})."""
+    reference's NULL list) or (the returned list's pairs from its last
+    element to its head -- the push order -- as gdp_pair records,
+    {traceback_score, missscore, nmatches, nmismatches, nopens, nindels})."""
+    return _splicejunction(ends, True, max_rlength, max_glength)
+
+
+def end3_splicejunction(ends, max_rlength=1000, max_glength=2000):
+    """As end5_splicejunction for Dynprog_end3_splicejunction: the same keys
+    with the strings forward (rsequence, gsequence) and rev_roffset, anchor,
+    far standing for roffset, goffset_anchor, goffset_far.  The pairs come
+    from the returned list's last element to its head: the push order
+    reversed, since that caller returns its trimmed list unreversed."""
+    return _splicejunction(ends, False, max_rlength, max_glength)
+
+
+def _splicejunction(ends, five, max_rlength, max_glength):
     L = load()
     use8 = int(L.gdp_use8p_size(ENDQ))
     out = [None] * len(ends)
@@ -62,14 +78,16 @@ def end5_splicejunction(ends, max_rlength=1000, max_glength=2000):
         open_, ext = end_penalties(e["defect_rate"])
         lb, ub = wide_bands(rl, gl, e["extraband"])
         width = 8 if (rl < use8 or gl < use8) else 16
-        # sweep order: the reversed strings forward
-        q, g, ga = e["rev_rseq"][::-1], e["rev_gseq"][::-1], e["rev_galt"][::-1]
+        # sweep order: the 5' form's reversed strings forward
+        q, g, ga = e["rev_rseq"], e["rev_gseq"], e["rev_galt"]
+        if five:
+            q, g, ga = q[::-1], g[::-1], ga[::-1]
         for kind in (GDP_KIND_UPPER, GDP_KIND_LOWER):
             p = np.zeros(1, PROBLEM_DTYPE)
             p["qoff"], p["goff"], p["galtoff"] = qo, go, go + gl
             p["rl"], p["gl"], p["lband"], p["uband"] = rl, gl, lb, ub
             p["kind"], p["width"], p["mismatchtype"] = kind, width, ENDQ
-            p["jump_late"] = 0 if e["jump_late"] else 1  # the fills take the opposite tie rule
+            p["jump_late"] = int(bool(e["jump_late"]) != five)  # the 5' fills take the opposite tie rule
             p["open"], p["extend"], p["keep_scores"] = open_, ext, GDP_KEEP_BEST_QUERYEND
             probs.append(p)
         qparts.append(q)
@@ -91,7 +109,7 @@ def end5_splicejunction(ends, max_rlength=1000, max_glength=2000):
             req[j]["fill"] = 2 * i + (0 if c >= r else 1)  # the triangle holding the cell
             req[j]["r"], req[j]["c"] = r, c
             req[j]["queryoffset"], req[j]["genomeoffset"] = e["rev_roffset"], goff_of(e)
-            req[j]["revp"], req[j]["genestrand"], req[j]["watsonp"] = 1, e["genestrand"], 1
+            req[j]["revp"], req[j]["genestrand"], req[j]["watsonp"] = int(five), e["genestrand"], 1
             req[j]["flags"], req[j]["endc"] = GDP_TB_LOCAL, endc_of(e)
             req[j]["max_pairs"] = len(e["rev_rseq"]) + len(e["rev_gseq"]) + 4
         return req
@@ -107,14 +125,15 @@ def end5_splicejunction(ends, max_rlength=1000, max_glength=2000):
         gap = np.zeros(1, PAIR_DTYPE)  # Pairpool_push_gapholder, knownp (pairpool.c:374-452)
         gap["querypos"] = gap["genomepos"] = -1
         gap["cdna"] = gap["comp"] = gap["genome"] = gap["genomealt"] = ord(" ")
-        gap["gapp"], gap["genomejump"] = 1, e["anchor"] - e["far"]
+        gap["gapp"], gap["genomejump"] = 1, (e["anchor"] - e["far"]) if five else (e["far"] - e["anchor"])
         pairs = np.concatenate([p1[j], gap, p2[j]])
         cnt = c1[j, 1:] + c2[j, 1:]
-        # the reference reverses the list and drops indel pairs from its 5' end,
-        # which are the first pushed (1905-1910)
+        # the reference reverses the list and drops indel pairs from the read's
+        # end, which are the first pushed (1905-1910, 2485-2490); the 3' form
+        # returns that trimmed list as it is
         k = 0
         while k < len(pairs) and pairs["comp"][k] == INDEL_COMP:
             k += 1
-        out[live[i]] = (pairs[k:], np.array([cnt[0], cnt[0] - len(e["rev_rseq"]) * FULLMATCH, cnt[1], cnt[2],
+        out[live[i]] = (pairs[k:] if five else pairs[k:][::-1].copy(), np.array([cnt[0], cnt[0] - len(e["rev_rseq"]) * FULLMATCH, cnt[1], cnt[2],
                                              cnt[3], cnt[4]], np.int32))
     return out

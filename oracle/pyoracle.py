@@ -270,6 +270,8 @@ class Reference(_Common):
         L.refe_traceback_local.restype = _i
         L.refe_end5_splicejunction.argtypes = [_p, _p, _p, _p, _p] + [_i] * 8 + [ctypes.c_double, _i, _p, _p, _i]
         L.refe_end5_splicejunction.restype = _i
+        L.refe_end3_splicejunction.argtypes = L.refe_end5_splicejunction.argtypes
+        L.refe_end3_splicejunction.restype = _i
         L.refh_init(mode)
         self.ws_single = L.refh_new(max_rl, max_gl, 0)
         self.ws_double = L.refh_new(max_rl, max_gl, 1)
@@ -351,16 +353,19 @@ class Reference(_Common):
         return out[:n].copy(), cnt, (int(end[0]), int(end[1]))
 
     def end5_splicejunction(self, rev_rseq, rev_gseq, rev_galt, rev_roffset, anchor, far, genestrand, late,
-                            extraband_end, defect_rate, contlength, maxpairs=8192):
-        """The reference's Dynprog_end5_splicejunction; strings in the caller's
-        orientation (read backwards from their last base).  Returns None for a
-        NULL list, else (pair records, {traceback_score, missscore, nmatches,
+                            extraband_end, defect_rate, contlength, maxpairs=8192, five=True):
+        """The reference's Dynprog_end5_splicejunction (five) or
+        Dynprog_end3_splicejunction; strings in the caller's orientation (the
+        5' form reads them backwards from their last base).  Returns None for
+        a NULL list, else (the returned list's pair records from its last
+        element to its head, {traceback_score, missscore, nmatches,
         nmismatches, nopens, nindels})."""
-        q, g, ga = self._seqs(rev_rseq, rev_gseq, rev_galt, True)
-        quc = _Seq(rev_rseq.upper(), True)
+        q, g, ga = self._seqs(rev_rseq, rev_gseq, rev_galt, five)
+        quc = _Seq(rev_rseq.upper(), five)
         out = np.zeros((maxpairs, PAIR_FIELDS), np.int32)
         res = np.zeros(6, np.int32)
-        n = self.lib.refe_end5_splicejunction(self.ws_double, q.ptr, quc.ptr, g.ptr, ga.ptr, len(rev_rseq),
+        fn = self.lib.refe_end5_splicejunction if five else self.lib.refe_end3_splicejunction
+        n = fn(self.ws_double, q.ptr, quc.ptr, g.ptr, ga.ptr, len(rev_rseq),
                                               len(rev_gseq), rev_roffset, anchor, far, genestrand, int(late),
                                               extraband_end, float(defect_rate), contlength, _np_ptr(res),
                                               _np_ptr(out), maxpairs)

@@ -149,3 +149,26 @@ int refe_end5_splicejunction(void *ws, char *rev_rseq, char *rev_rsequc, char *r
   if (pairs == NULL) return -1;
   return refh_flatten(pairs, out, maxpairs);
 }
+
+/* Dynprog_end3_splicejunction (dynprog_end.c:2248-2745): as
+   refe_end5_splicejunction, sequences forward from the pointers. */
+int refe_end3_splicejunction(void *ws, char *rseq, char *rsequc, char *gseq, char *galt, int rlength, int glength,
+                             int roffset, int goffset_anchor, int goffset_far, int genestrand, int jump_late,
+                             int extraband_end, double defect_rate, int contlength, int *res, int *out,
+                             int maxpairs) {
+  int dpi = 1, ts = 0, miss = 0, nm = 0, nmm = 0, no = 0, ni = 0;
+  if (end_pool == NULL) end_pool = Pairpool_new();
+  Pairpool_reset(end_pool);
+  List_T pairs = Dynprog_end3_splicejunction(&dpi, &ts, &miss, &nm, &nmm, &no, &ni, (Dynprog_T)ws, rseq, rsequc,
+                                             gseq, galt, rlength, glength, roffset, goffset_anchor, goffset_far, 0,
+                                             0, true, genestrand, jump_late != 0, NULL, NULL, end_pool,
+                                             extraband_end, defect_rate, contlength);
+  res[0] = ts;
+  res[1] = miss;
+  res[2] = nm;
+  res[3] = nmm;
+  res[4] = no;
+  res[5] = ni;
+  if (pairs == NULL) return -1;
+  return refh_flatten(pairs, out, maxpairs);
+}

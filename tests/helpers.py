@@ -189,7 +189,7 @@ def tb_request(i, s, r, c, qoff=5, goff=7, chroffset=0, chrhigh=0, watsonp=True,
     return req[0]
 
 
-def oracle_end5_splicejunction(O, e):
+def oracle_end5_splicejunction(O, e, five=True):
     """Dynprog_end5_splicejunction (dynprog_end.c:1652-1922) composed from the
     oracle's fills, last-row best cell and local walks; e as
     gmapdp.ends.end5_splicejunction takes it.  None or (pair records,
@@ -201,9 +201,9 @@ def oracle_end5_splicejunction(O, e):
     lb, ub = wide_bands(rl, gl, e["extraband"])
     u8 = O.use8p(3)
     width = 8 if (rl < u8 or gl < u8) else 16
-    late = not e["jump_late"]
-    su = Spec(1, width, q, g, ga, 3, open_, ext, lb, ub, late, True)
-    sl = Spec(2, width, q, g, ga, 3, open_, ext, lb, ub, late, True)
+    late = bool(e["jump_late"]) != five  # the 5' form fills with the opposite tie rule
+    su = Spec(1, width, q, g, ga, 3, open_, ext, lb, ub, late, five)
+    sl = Spec(2, width, q, g, ga, 3, open_, ext, lb, ub, late, five)
     Hu, Du = oracle_fill(O, su)
     Hl, Dl = oracle_fill(O, sl)
     fs, br, bc = O.best_endpoint(True, Hu, Hl, width, rl, gl, lb, ub, late)
@@ -214,15 +214,16 @@ def oracle_end5_splicejunction(O, e):
     for endc, goff in ((e["contlength"], e["far"]), (0, e["anchor"])):
         kind = 1 if c >= r else 2
         p, cnt, (r, c) = O.traceback_local(kind, Du if kind == 1 else Dl, r, c, endc, q, q.upper(), g, ga,
-                                           e["rev_roffset"], goff, True, e["genestrand"])
+                                           e["rev_roffset"], goff, five, e["genestrand"])
         walk.append((p, cnt))
-    gap = np.array([[-1, -1, 32, 32, 32, 32, 1, e["anchor"] - e["far"]]], np.int32)
+    gap = np.array([[-1, -1, 32, 32, 32, 32, 1, (e["anchor"] - e["far"]) * (1 if five else -1)]], np.int32)
     pairs = np.concatenate([walk[0][0], gap, walk[1][0]])
     cnt = walk[0][1] + walk[1][1]
     k = 0
     while k < len(pairs) and pairs[k, 3] == ord("-"):
         k += 1
-    return pairs[k:], np.array([cnt[0], cnt[0] - rl * 3, cnt[1], cnt[2], cnt[3], cnt[4]], np.int32)
+    pairs = pairs[k:] if five else pairs[k:][::-1].copy()  # the 3' form returns its trimmed list as it is
+    return pairs, np.array([cnt[0], cnt[0] - rl * 3, cnt[1], cnt[2], cnt[3], cnt[4]], np.int32)
 
 
 def random_splice_ends(rng, n):
@@ -239,7 +240,8 @@ def random_splice_ends(rng, n):
         ga = bytes(x if rng.random() > 0.05 else ord("R") for x in g)
         anchor = int(rng.integers(200, 5000))
         out.append({"rev_rseq": q, "rev_gseq": g, "rev_galt": ga, "rev_roffset": int(rng.integers(rl, rl + 300)),
-                    "anchor": anchor, "far": anchor - int(rng.integers(20, 400)), "genestrand": 0,
+                    "anchor": anchor, "far": anchor + int(rng.integers(20, 400)) * (-1 if t % 2 else 1),
+                    "genestrand": 0,
                     "jump_late": bool(t % 2), "extraband": int(rng.integers(3, 10)),
                     "defect_rate": float(rng.choice([0.001, 0.01, 0.05])),
                     "contlength": int(rng.integers(0, min(rl, gl) + 2))})

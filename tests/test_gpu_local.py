@@ -54,17 +54,18 @@ def test_local_walks(oracle):
     assert cf[0, 0] == -1
 
 
-def test_end5_splicejunction(oracle):
-    """Dynprog_end5_splicejunction composed from the batched kernels
+@pytest.mark.parametrize("five", [True, False])
+def test_splicejunction(oracle, five):
+    """Dynprog_end5/3_splicejunction composed from the batched kernels
     (gmapdp.ends) against the oracle's composition, which
-    tests/test_oracle_end.py pins against the reference's own caller."""
-    from gmapdp.ends import end5_splicejunction
-    rng = np.random.default_rng(36)
+    tests/test_oracle_end.py pins against the reference's own callers."""
+    from gmapdp.ends import end3_splicejunction, end5_splicejunction
+    rng = np.random.default_rng(36 if five else 38)
     ends = H.random_splice_ends(rng, 400)
-    got = end5_splicejunction(ends)
+    got = (end5_splicejunction if five else end3_splicejunction)(ends)
     walked = 0
     for k, e in enumerate(ends):
-        want = H.oracle_end5_splicejunction(oracle, e)
+        want = H.oracle_end5_splicejunction(oracle, e, five)
         assert (got[k] is None) == (want is None), ("null", k)
         if want is None:
             continue

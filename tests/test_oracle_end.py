@@ -4,6 +4,7 @@ oracle/ref_end_shim.c): find_best_endpoint_{8,16} (143-290),
 find_best_endpoint_to_queryend_indels_{8,16} (358-508) and
 traceback_nogaps (648-722)."""
 import numpy as np
+import pytest
 
 import helpers as H
 
@@ -90,17 +91,18 @@ def test_traceback_local_matches_reference(oracle, reference):
     assert walks > 300
 
 
-def test_end5_splicejunction_matches_reference(oracle, reference):
-    """The oracle's composition of Dynprog_end5_splicejunction against the
-    reference's own caller."""
-    rng = np.random.default_rng(35)
+@pytest.mark.parametrize("five", [True, False])
+def test_splicejunction_matches_reference(oracle, reference, five):
+    """The oracle's composition of Dynprog_end5_splicejunction (five) and
+    Dynprog_end3_splicejunction against the reference's own callers."""
+    rng = np.random.default_rng(35 if five else 37)
     ends = H.random_splice_ends(rng, 300)
     got = 0
     for k, e in enumerate(ends):
         want = reference.end5_splicejunction(e["rev_rseq"], e["rev_gseq"], e["rev_galt"], e["rev_roffset"],
                                              e["anchor"], e["far"], e["genestrand"], e["jump_late"],
-                                             e["extraband"], e["defect_rate"], e["contlength"])
-        mine = H.oracle_end5_splicejunction(oracle, e)
+                                             e["extraband"], e["defect_rate"], e["contlength"], five=five)
+        mine = H.oracle_end5_splicejunction(oracle, e, five)
         assert (want is None) == (mine is None), ("null", k)
         if want is None:
             continue
