@@ -137,15 +137,15 @@ def serving_bench(args):
         for frac in (None, 0.1, 0.25, 0.5, 0.75, 0.9):
             rate = 1e4 if frac is None else frac * sat
             m = int(min(nreq, max(2000, rate * args.serve_seconds)))
-            # the half-load point is run three times (different arrival
-            # seeds) and reports the run with the median p99, all three listed:
+            # the half-load point is run five times (different arrival
+            # seeds) and reports the run with the median p99, all five listed:
             # a single run moves with the shared host's load
             runs = []
             if frac == 0.5:
                 # one unmeasured run at this rate first: the first run after a
                 # rate change has measured up to 50 % slower at p99
                 srv.drive(reqs[:m // 2], rate, seed=99)
-            for rep in range(3 if frac == 0.5 else 1):
+            for rep in range(5 if frac == 0.5 else 1):
                 lat, el, ok = srv.drive(reqs[:m], rate, seed=7 + rep)
                 runs.append((float(np.percentile(lat, 99)) * 1e3, float(np.percentile(lat, 50)) * 1e3, el, ok))
             runs_sorted = sorted(runs)
