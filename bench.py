@@ -211,8 +211,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="target duration of each load point")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (problems are independent; "
-              "see DESIGN.md for how batches would shard).  Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (problems are independent, so a batch "
+              "would split by length bin across GPUs, each with its own copy of the genome).  Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
