@@ -137,3 +137,45 @@ def _splicejunction(ends, five, max_rlength, max_glength):
         out[live[i]] = (pairs[k:] if five else pairs[k:][::-1].copy(), np.array([cnt[0], cnt[0] - len(e["rev_rseq"]) * FULLMATCH, cnt[1], cnt[2],
                                              cnt[3], cnt[4]], np.int32))
     return out
+
+
+def _random_ends(n, seed=1007):
+    """Splice-junction ends of the end-extension mix: read ends of 10-150
+    bases against their genome window plus 0-20 bases (3 % substitutions,
+    1 % indels), extraband 6, continuation columns up to the read length."""
+    rng = np.random.default_rng(seed)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    ends = []
+    for k in range(n):
+        rl = int(rng.integers(10, 151))
+        gl = rl + int(rng.integers(0, 21))
+        g = acgt[rng.integers(0, 4, gl)]
+        q = g[:rl].copy()
+        sub = rng.random(rl) < 0.03
+        q[sub] = acgt[rng.integers(0, 4, int(sub.sum()))]
+        anchor = int(rng.integers(1000, 1 << 30))
+        ends.append({"rev_rseq": q.tobytes(), "rev_gseq": g.tobytes(), "rev_galt": g.tobytes(),
+                     "rev_roffset": rl + 100, "anchor": anchor, "far": anchor - int(rng.integers(50, 5000)),
+                     "genestrand": 0, "jump_late": bool(k % 2), "extraband": 6, "defect_rate": 0.001,
+                     "contlength": int(rng.integers(0, rl // 2 + 1))})
+    return ends
+
+
+if __name__ == "__main__":
+    # python3 -m gmapdp.ends [N]: times end5_splicejunction over N ends of the
+    # end-extension mix (after one untimed call of the same size).
+    import sys
+    import time
+
+    import torch
+
+    n = int(sys.argv[1]) if len(sys.argv) > 1 else 200_000
+    ends = _random_ends(n)
+    end5_splicejunction(ends)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = end5_splicejunction(ends)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    walked = sum(r is not None for r in res)
+    print("end5_splicejunction: %d ends (%d walked) in %.3f s = %.0f ends/s" % (n, walked, dt, n / dt), flush=True)
