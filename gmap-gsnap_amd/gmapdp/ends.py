@@ -68,74 +68,81 @@ def _splicejunction(ends, five, max_rlength, max_glength):
     L = load()
     use8 = int(L.gdp_use8p_size(ENDQ))
     out = [None] * len(ends)
-    live = []
-    probs, qparts, gparts = [], [], []
-    qo = go = 0
-    for k, e in enumerate(ends):
-        rl, gl = len(e["rev_rseq"]), len(e["rev_gseq"])
-        if rl <= 0 or rl > max_rlength or gl <= 0 or gl > max_glength:
-            continue  # dynprog_end.c:1702-1717
-        open_, ext = end_penalties(e["defect_rate"])
-        lb, ub = wide_bands(rl, gl, e["extraband"])
-        width = 8 if (rl < use8 or gl < use8) else 16
-        # sweep order: the 5' form's reversed strings forward
-        q, g, ga = e["rev_rseq"], e["rev_gseq"], e["rev_galt"]
-        if five:
-            q, g, ga = q[::-1], g[::-1], ga[::-1]
-        for kind in (GDP_KIND_UPPER, GDP_KIND_LOWER):
-            p = np.zeros(1, PROBLEM_DTYPE)
-            p["qoff"], p["goff"], p["galtoff"] = qo, go, go + gl
-            p["rl"], p["gl"], p["lband"], p["uband"] = rl, gl, lb, ub
-            p["kind"], p["width"], p["mismatchtype"] = kind, width, ENDQ
-            p["jump_late"] = int(bool(e["jump_late"]) != five)  # the 5' fills take the opposite tie rule
-            p["open"], p["extend"], p["keep_scores"] = open_, ext, GDP_KEEP_BEST_QUERYEND
-            probs.append(p)
-        qparts.append(q)
-        gparts.append(g + ga)
-        qo += rl
-        go += 2 * gl
-        live.append(k)
-    if not live:
+    rl_all = np.array([len(e["rev_rseq"]) for e in ends], np.int64)
+    gl_all = np.array([len(e["rev_gseq"]) for e in ends], np.int64)
+    # dynprog_end.c:1702-1717: empty or oversized ends give NULL
+    live = np.nonzero((rl_all > 0) & (rl_all <= max_rlength) & (gl_all > 0) & (gl_all <= max_glength))[0]
+    if len(live) == 0:
         return out
-    b = DeviceBatch(np.concatenate(probs), b"".join(qparts), b"".join(gparts))
+    E = [ends[k] for k in live]
+    rl, gl = rl_all[live], gl_all[live]
+    defect = np.array([e["defect_rate"] for e in E])
+    extra = np.array([e["extraband"] for e in E], np.int64)
+    late = np.array([bool(e["jump_late"]) for e in E])
+    roff = np.array([e["rev_roffset"] for e in E], np.int64)
+    anchor = np.array([e["anchor"] for e in E], np.int64)
+    far = np.array([e["far"] for e in E], np.int64)
+    endc1 = np.array([e["contlength"] for e in E], np.int64)
+    gs = np.array([e["genestrand"] for e in E], np.int64)
+    # sweep order: the 5' form's reversed strings forward
+    if five:
+        qbuf = b"".join(e["rev_rseq"][::-1] for e in E)
+        gbuf = b"".join(e["rev_gseq"][::-1] + e["rev_galt"][::-1] for e in E)
+    else:
+        qbuf = b"".join(e["rev_rseq"] for e in E)
+        gbuf = b"".join(e["rev_gseq"] + e["rev_galt"] for e in E)
+    qoff = np.concatenate([[0], np.cumsum(rl)[:-1]])
+    goff = np.concatenate([[0], np.cumsum(2 * gl)[:-1]])
+    # end penalties by defect rate (dynprog_end.c:88-94), bands widened to the far corner
+    open_ = np.where(defect < 0.003, -10, np.where(defect < 0.014, -8, -6))
+    lb = np.where(gl >= rl, extra, rl - gl + extra)
+    ub = np.where(gl >= rl, gl - rl + extra, extra)
+    probs = np.zeros(2 * len(live), PROBLEM_DTYPE)
+    for j, kind in enumerate((GDP_KIND_UPPER, GDP_KIND_LOWER)):
+        p = probs[j::2]
+        p["qoff"], p["goff"], p["galtoff"] = qoff, goff, goff + gl
+        p["rl"], p["gl"], p["lband"], p["uband"] = rl, gl, lb, ub
+        p["kind"], p["mismatchtype"] = kind, ENDQ
+        p["width"] = np.where((rl < use8) | (gl < use8), 8, 16)
+        p["jump_late"] = late != five  # the 5' fills take the opposite tie rule
+        p["open"], p["extend"], p["keep_scores"] = open_, -2, GDP_KEEP_BEST_QUERYEND
+    b = DeviceBatch(probs, qbuf, gbuf)
     b.fill()
     ul = np.stack([np.arange(0, 2 * len(live), 2), np.arange(1, 2 * len(live), 2)], 1).astype(np.int32)
     best = b.best_endpoint(ul, queryend_indels=True)
+    sel = np.nonzero(best[:, 0] >= 0)[0]  # finalscore < 0: no splice (1797-1800)
+    if len(sel) == 0:
+        return out
 
-    def requests(sel, cells, endc_of, goff_of):
+    def requests(r, c, endc, goffs):
         req = np.zeros(len(sel), TBREQ_DTYPE)
-        for j, (i, (r, c)) in enumerate(zip(sel, cells)):
-            e = ends[live[i]]
-            req[j]["fill"] = 2 * i + (0 if c >= r else 1)  # the triangle holding the cell
-            req[j]["r"], req[j]["c"] = r, c
-            req[j]["queryoffset"], req[j]["genomeoffset"] = e["rev_roffset"], goff_of(e)
-            req[j]["revp"], req[j]["genestrand"], req[j]["watsonp"] = int(five), e["genestrand"], 1
-            req[j]["flags"], req[j]["endc"] = GDP_TB_LOCAL, endc_of(e)
-            req[j]["max_pairs"] = len(e["rev_rseq"]) + len(e["rev_gseq"]) + 4
+        req["fill"] = 2 * sel + (c < r)  # the triangle holding the cell
+        req["r"], req["c"] = r, c
+        req["queryoffset"], req["genomeoffset"] = roff[sel], goffs
+        req["revp"], req["genestrand"], req["watsonp"] = int(five), gs[sel], 1
+        req["flags"], req["endc"] = GDP_TB_LOCAL, endc
+        req["max_pairs"] = rl[sel] + gl[sel] + 4
         return req
 
-    sel = [i for i in range(len(live)) if best[i, 0] >= 0]  # finalscore < 0: no splice (1797-1800)
-    if not sel:
-        return out
-    p1, c1, stop = b.traceback(requests(sel, [(best[i, 1], best[i, 2]) for i in sel], lambda e: e["contlength"],
-                                        lambda e: e["far"]), ends=True)
-    p2, c2 = b.traceback(requests(sel, [tuple(x) for x in stop], lambda e: 0, lambda e: e["anchor"]))
+    p1, c1, stop = b.traceback(requests(best[sel, 1], best[sel, 2], endc1[sel], far[sel]), ends=True)
+    p2, c2 = b.traceback(requests(stop[:, 0], stop[:, 1], 0, anchor[sel]))
+    cnt = c1[:, 1:] + c2[:, 1:]
+    jump = (anchor - far) if five else (far - anchor)
     for j, i in enumerate(sel):
-        e = ends[live[i]]
         gap = np.zeros(1, PAIR_DTYPE)  # Pairpool_push_gapholder, knownp (pairpool.c:374-452)
         gap["querypos"] = gap["genomepos"] = -1
         gap["cdna"] = gap["comp"] = gap["genome"] = gap["genomealt"] = ord(" ")
-        gap["gapp"], gap["genomejump"] = 1, (e["anchor"] - e["far"]) if five else (e["far"] - e["anchor"])
+        gap["gapp"], gap["genomejump"] = 1, jump[i]
         pairs = np.concatenate([p1[j], gap, p2[j]])
-        cnt = c1[j, 1:] + c2[j, 1:]
         # the reference reverses the list and drops indel pairs from the read's
         # end, which are the first pushed (1905-1910, 2485-2490); the 3' form
         # returns that trimmed list as it is
         k = 0
         while k < len(pairs) and pairs["comp"][k] == INDEL_COMP:
             k += 1
-        out[live[i]] = (pairs[k:] if five else pairs[k:][::-1].copy(), np.array([cnt[0], cnt[0] - len(e["rev_rseq"]) * FULLMATCH, cnt[1], cnt[2],
-                                             cnt[3], cnt[4]], np.int32))
+        out[live[i]] = (pairs[k:] if five else pairs[k:][::-1].copy(),
+                        np.array([cnt[j, 0], cnt[j, 0] - rl[i] * FULLMATCH, cnt[j, 1], cnt[j, 2], cnt[j, 3],
+                                  cnt[j, 4]], np.int32))
     return out
 
 
