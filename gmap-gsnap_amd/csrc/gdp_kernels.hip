@@ -1222,6 +1222,7 @@ __global__ __launch_bounds__(256) void gdp_best_endpoint_kernel(
 
 #define GDP_BRIDGE_ROWS 1024  // rows of one end's column held in LDS
 #define GDP_BRIDGE_LDS 8192   // words of LDS: right prefix tables and one left column
+#define GDP_BRIDGE_LDS_SMALL 4096  // the launch for gaps whose tables fit in half of it
 
 // A score (int16 range) and its row in one word: the score in the high half.
 __device__ __forceinline__ int bridge_pack(int v, int r) { return (int)(((uint32_t)v << 16) | (uint32_t)(r & 0xffff)); }
@@ -1262,11 +1263,15 @@ __device__ __forceinline__ int bridge_score(const int16_t *scores, const gdp_dev
 // and shared evenly over the lanes.  Gaps wider than the LDS tables take the
 // direct form: per cR a prefix table of the right column, lanes over cL and
 // every left row.
+// Two launches share the gaps: LDSW = GDP_BRIDGE_LDS_SMALL takes the gaps
+// whose tables fit in that many words (twice the blocks per CU), the full
+// size takes all others, including those that fit neither form.
+template <int LDSW>
 __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
     const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ quads, int m,
     const int16_t *__restrict__ scores, int32_t *__restrict__ out, const gdp_tb_request *__restrict__ tmpl,
     gdp_tb_request *__restrict__ tbout) {
-  __shared__ int lds[GDP_BRIDGE_LDS];
+  __shared__ int lds[LDSW];
   const int k = blockIdx.x, lane = threadIdx.x;
   if (k >= m) return;
   const int32_t *q = quads + 5 * (int64_t)k;
@@ -1283,7 +1288,7 @@ __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
                   Rl.gl == gl && Ll.rl == rlL && Rl.rl == rlR && Ll.width == Lu.width && Ru.width == Lu.width &&
                   Rl.width == Lu.width;
   // the table form needs its offsets in LDS; the direct form one right column
-  const bool fast = ok0 && gl + 1 <= GDP_BRIDGE_LDS && ubL + lbL + 1 <= GDP_BRIDGE_ROWS;
+  const bool fast = ok0 && gl + 1 <= LDSW && ubL + lbL + 1 <= GDP_BRIDGE_ROWS;
   int bv = INT_MIN;
   int64_t bkey = -1;
   const int64_t RR = (int64_t)gdp_imax(rlL, rlR) + 2;  // radix of the row fields of a scan position
@@ -1305,7 +1310,10 @@ __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
   }
   if (fast && lane == 0) lds[gl] = nRsum;
   const int nLmax = ubL + lbL + 1;
-  const bool table = fast && 2 * (gl + 1) + nRsum + 2 * nLmax <= GDP_BRIDGE_LDS;
+  const bool table = fast && 2 * (gl + 1) + nRsum + 2 * nLmax <= LDSW;
+  // the same test at the small size, in both launches (nRsum is the same)
+  const bool small = fast && 2 * (gl + 1) + nRsum + 2 * nLmax <= GDP_BRIDGE_LDS_SMALL;
+  if (small != (LDSW == GDP_BRIDGE_LDS_SMALL)) return;  // the other launch's gap
   const bool ok = ok0 && (table || ubR + lbR + 1 <= GDP_BRIDGE_ROWS);
   if (ok && table) {
     const gdp_geom GLu = gdp_make_geom(Lu), GLl = gdp_make_geom(Ll), GRu = gdp_make_geom(Ru), GRl = gdp_make_geom(Rl);
@@ -1691,8 +1699,10 @@ extern "C" int gdp_launch_bridge_cdna(const void *d_descs, const int32_t *d_quad
                                       int32_t *d_out, const gdp_tb_request *d_tmpl, gdp_tb_request *d_tbout,
                                       hipStream_t stream) {
   if (m <= 0) return 0;
-  hipLaunchKernelGGL(gdp_bridge_cdna_kernel, dim3(m), dim3(64), 0, stream, (const gdp_dev_problem *)d_descs, d_quads,
-                     m, d_scores, d_out, d_tmpl, d_tbout);
+  hipLaunchKernelGGL(gdp_bridge_cdna_kernel<GDP_BRIDGE_LDS_SMALL>, dim3(m), dim3(64), 0, stream,
+                     (const gdp_dev_problem *)d_descs, d_quads, m, d_scores, d_out, d_tmpl, d_tbout);
+  hipLaunchKernelGGL(gdp_bridge_cdna_kernel<GDP_BRIDGE_LDS>, dim3(m), dim3(64), 0, stream,
+                     (const gdp_dev_problem *)d_descs, d_quads, m, d_scores, d_out, d_tmpl, d_tbout);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -22,9 +22,13 @@ def _gaps(rng, n):
         short = t % 7 == 3  # ends shorter than the gap: upper rows beyond rl are not candidates
         rlL = int(rng.integers(1, gl)) if short else int(rng.integers(gl - 1, gl + 30))
         rlR = int(rng.integers(1, gl)) if short else int(rng.integers(gl - 1, gl + 30))
+        if t % 13 == 7:
+            # tables between 4096 and 8192 LDS words: the full-size launch's table form
+            gl = int(rng.integers(110, 150))
+            rlL, rlR = gl + int(rng.integers(5, 15)), gl + int(rng.integers(5, 15))
         width = 8 if t % 3 == 0 else 16
         late = bool(t % 2)
-        ext = 4 + t % 5
+        ext = 20 if t % 13 == 7 else 4 + t % 5
         lbL, ubL = _bands(rlL, gl, ext)
         lbR, ubR = _bands(rlR, gl, ext)
         if t % 11 == 5:
