@@ -4,8 +4,9 @@
 A step is one pass of the hot path over one batch: every problem of the
 workload is filled (one batched launch) and traced back (one batched
 launch).  Inputs are resident in HBM before the timed region.  The default
-workload is the 10^6-problem single-gap batch (rl 150, gl 160-200,
-extraband 5-20, int16); --workload c3 selects the uneven 2x10^6 batch.
+workload is the uneven 2x10^6-problem single-gap batch (rl log-uniform
+20-1000, gl = rl x U(1, 1.5), extraband 3-64, int8 and int16), on which the
+targets are quoted; --workload c2 selects the 10^6-problem batch of rl 150.
 
 After the batch, the serving loop runs (unless --no-serve): a 3.1-Gbp random
 genome resident on the device as a genomecomp image, alignment requests of
@@ -17,7 +18,14 @@ Prints one JSON line (the fields the driver reads) with extra objects:
                    average duration measured with HIP events on its stream;
   cpu_baseline  -- the reference's own AVX2 DP objects (oracle/_ref, built
                    from the reference sources) or, where absent, the CPU
-                   restatement, timed on a bounded sample on the host cores.
+                   restatement, timed on a bounded sample on the host cores
+                   (fill + traceback, and fill alone);
+  percall_gcups -- the one-launch-per-problem GPU form (gdp_align_each: a
+                   C loop of one fill launch, one traceback launch and a
+                   stream synchronize per problem) on the first problems;
+  e2e_gcups     -- the step with its PCIe transfers: inputs uploaded from
+                   pinned host memory, fill and traceback, counts and pair
+                   records downloaded.
 """
 import argparse
 import concurrent.futures as cf
@@ -55,7 +63,8 @@ def log(*a):
 
 
 def cpu_baseline(wl, sample_n, threads):
-    """Times the CPU DP (fill + traceback) on a bounded sample."""
+    """Times the CPU DP on a bounded sample: fill + traceback, then the same
+    sample filled only."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     probs = wl.problems[:sample_n]
@@ -69,7 +78,8 @@ def cpu_baseline(wl, sample_n, threads):
                                      ctypes.c_int, ctypes.c_int]
         L.refh_run_batch.restype = ctypes.c_longlong
         maxrl, maxgl = int(probs["rl"].max()) + 8, int(probs["gl"].max()) + 8
-        run = lambda p: L.refh_run_batch(p.ctypes.data, len(p), qbuf.ctypes.data, gbuf.ctypes.data, 1, maxrl, maxgl)
+        run = lambda p, tb=1: L.refh_run_batch(p.ctypes.data, len(p), qbuf.ctypes.data, gbuf.ctypes.data, tb, maxrl,
+                                               maxgl)
         kind = "reference"
     else:
         L = ctypes.CDLL(pyoracle.ORACLE_SO)
@@ -77,15 +87,23 @@ def cpu_baseline(wl, sample_n, threads):
         L.orc_init(0)
         L.orc_run_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.orc_run_batch.restype = ctypes.c_longlong
-        run = lambda p: L.orc_run_batch(p.ctypes.data, len(p), qbuf.ctypes.data, gbuf.ctypes.data, 1)
+        run = lambda p, tb=1: L.orc_run_batch(p.ctypes.data, len(p), qbuf.ctypes.data, gbuf.ctypes.data, tb)
         kind = "port"
     parts = np.array_split(np.ascontiguousarray(probs), threads)
-    t0 = time.perf_counter()
-    with cf.ThreadPoolExecutor(threads) as ex:
-        list(ex.map(run, parts))
-    dt = time.perf_counter() - t0
+
+    def timed(tb):
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda p: run(p, tb), parts))
+        return time.perf_counter() - t0
+
+    dt = timed(1)
+    dtf = timed(0)
     return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": kind,
-            "sample": "%d problems of the same workload (first %d), fill + traceback, %.1f s" % (sample_n, sample_n, dt)}
+            "host_cores": os.cpu_count(), "fill_only_value": round(cells / dtf / 1e9, 4),
+            "sample": "%d problems of the same workload (the first %d), fill + traceback in %.1f s, fill alone "
+                      "in %.1f s, %d threads (the host share of one GPU), one workspace and pair pool each"
+                      % (sample_n, sample_n, dt, dtf, threads)}
 
 
 def serving_bench(args):
@@ -186,11 +204,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=0, help="override the problem count")
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--percall", type=int, default=2000, help="problems timed one launch each (0 = skip)")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the PCIe-inclusive step")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="before the timed region, copy MIB MiB once on the device (a known-size kernel that "
                          "calibrates HBM byte counters in a profiled run)")
@@ -340,7 +359,41 @@ def main():
         bridge_ms = bev[0].elapsed_time(bev[1]) / args.steps
         log("cDNA bridge: %.3f ms for %d gaps" % (bridge_ms, wl.n))
 
-    # Plain one-launch-per-problem form on the same inputs.
+    # PCIe-inclusive step: inputs from pinned host memory, the step, counts and
+    # pair records back (the pair buffer is downloaded whole: rl + gl + 4
+    # records per request, of which the walk fills its count).
+    e2e = None
+    if not args.no_e2e and config not in (4, "cdna"):
+        h_in = [torch.empty(t.numel(), dtype=t.dtype, pin_memory=True) for t in (b.d_descs, b.d_q, b.d_g,
+                                                                                  reqs_dev["d_reqs"])]
+        for h, t in zip(h_in, (b.d_descs, b.d_q, b.d_g, reqs_dev["d_reqs"])):
+            h.copy_(t)
+        h_cnt = torch.empty(reqs_dev["d_counts"].numel(), dtype=torch.int32, pin_memory=True)
+        h_pairs = torch.empty(reqs_dev["d_pairs"].numel(), dtype=torch.uint8, pin_memory=True)
+        torch.cuda.synchronize()
+        reps = 2
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            for h, t in zip(h_in, (b.d_descs, b.d_q, b.d_g, reqs_dev["d_reqs"])):
+                t.copy_(h, non_blocking=True)
+            step()
+            h_cnt.copy_(reqs_dev["d_counts"], non_blocking=True)
+            h_pairs.copy_(reqs_dev["d_pairs"], non_blocking=True)
+        torch.cuda.synchronize()
+        dte = (time.perf_counter() - t1) / reps
+        up = sum(h.numel() * h.element_size() for h in h_in)
+        down = h_cnt.numel() * 4 + h_pairs.numel()
+        e2e = {"gcups": round(total_cells / dte / 1e9, 3), "ms_per_step": round(dte * 1e3, 3),
+               "upload_bytes": int(up), "download_bytes": int(down),
+               "method": "descriptors, sequences and traceback requests uploaded from pinned memory, the step, "
+                         "counts and the whole pair buffer downloaded to pinned memory, one synchronize"}
+        log("PCIe-inclusive step: %.2f ms, %.1f GCUPS (%.2f GB up, %.2f GB down)" % (dte * 1e3, e2e["gcups"],
+                                                                                    up / 1e9, down / 1e9))
+        del h_in, h_cnt, h_pairs
+
+    # Plain one-launch-per-problem form on the same inputs: a C loop
+    # (gdp_align_each) of one fill launch of the problem's own kernel, one
+    # traceback launch and a stream synchronize per problem.
     percall = None
     if args.percall:
         m = min(args.percall, n)
@@ -348,15 +401,15 @@ def main():
         # (config 4: its single-gap half, problems[:n3])
         sreq = sub.prepare_traceback((wl.single_requests() if config == 4 else wl.tb_requests())[:m],
                                      genome=wl.genome())
+        sub.align_each(sreq)  # untimed: first launches of each kernel
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        for i in range(m):
-            sub.fill_one(i)
-            sub.traceback_one(sreq, i)
-            torch.cuda.synchronize()
+        sub.align_each(sreq)
+        torch.cuda.synchronize()
         dtp = time.perf_counter() - t1
         percall = batch_cells(probs[:m]) / dtp / 1e9
-        log("per-call form: %d problems, %.3f GCUPS" % (m, percall))
+        log("per-call form: %d problems, %.3f GCUPS (%.1f us per problem)" % (m, percall, dtp / m * 1e6))
+        del sub, sreq
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
@@ -372,6 +425,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        # the host share of one GPU on these machines is 16 cores
         threads = min(16, os.cpu_count() or 1)
         sample = args.cpu_sample or (700000 if args.workload == "c2" else 40000)
         cpu = cpu_baseline(wl, min(sample, n), threads)
@@ -401,6 +455,10 @@ def main():
         "traceback_ms": round(tb_ms, 3),
         "fill_gcups": round(total_cells / (fill_ms * 1e-3) / 1e9, 3),
         "percall_gcups": None if percall is None else round(percall, 4),
+        "percall_method": None if percall is None else
+        "gdp_align_each over the first %d problems: per problem one fill launch of its own kernel, one traceback "
+        "launch, one stream synchronize (a C loop)" % min(args.percall, n),
+        "e2e": e2e,
         "batched_vs_percall": None if percall is None else round(gcups / percall, 1),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
                      "frac": round(achieved / PEAK_INT16_TOPS, 4), "traffic": traffic,

@@ -97,6 +97,24 @@ extern "C" void gdp_host_tables(int mode) {
 
 extern "C" int32_t gdp_descriptor_bytes(void) { return (int32_t)sizeof(gdp_dev_problem); }
 
+// Bands wider than the matrix narrowed to one past it.  No cell changes: a
+// lower band past rl already holds every row of every column (r - lband < 0
+// for every row, so each row starts at column 0 and the bottom edge
+// r = c + lband lies below the matrix), and an upper band past gl never
+// reaches its top edge r = c - uband; a one-sided band past its triangle's
+// length covers the triangle.  Each band stays past the matrix, so every
+// comparison of a cell with a band edge comes out as before.
+static void narrow_bands(const gdp_problem &p, int &lb, int &ub) {
+  if (p.kind == GDP_KIND_FULL || p.kind == GDP_KIND_STD) {
+    lb = gdp_imin(lb, p.rl + 1);
+    ub = gdp_imin(ub, p.gl + 1);
+  } else if (p.kind == GDP_KIND_UPPER) {
+    ub = gdp_imin(ub, p.gl + 1);
+  } else if (p.kind == GDP_KIND_LOWER) {
+    lb = gdp_imin(lb, p.rl + 1);
+  }
+}
+
 static int validate(const gdp_problem &p, int i) {
   if (p.kind < 0 || p.kind > 3) return fail(GDP_EINVAL, "problem %d: bad kind %d", i, p.kind);
   if (p.kind == GDP_KIND_STD) {
@@ -121,7 +139,9 @@ static int validate(const gdp_problem &p, int i) {
                 p.keep_scores);
   if (p.open > 0 || p.open < -127 || p.extend > 0 || p.extend < -127)
     return fail(GDP_EINVAL, "problem %d: gap penalties must lie in [-127, 0]", i);
-  int span = p.kind == GDP_KIND_FULL || p.kind == GDP_KIND_STD ? p.lband + p.uband : (p.kind == GDP_KIND_UPPER ? p.uband : p.lband);
+  int lb = p.lband, ub = p.uband;
+  narrow_bands(p, lb, ub);
+  int span = p.kind == GDP_KIND_FULL || p.kind == GDP_KIND_STD ? lb + ub : (p.kind == GDP_KIND_UPPER ? ub : lb);
   const int maxspan = p.kind == GDP_KIND_STD ? GDP_MAX_SPAN_STD : GDP_MAX_SPAN;
   if (span > maxspan) return fail(GDP_ETOOBIG, "problem %d: band span %d > %d", i, span, maxspan);
   return GDP_OK;
@@ -152,9 +172,21 @@ static int pair_lanes(const gdp_problem &p, const gdp_plan_params &pp) {
 // Wave class: the ring of a class holds at least P step slots.
 static int pair_class(int G, int P) { return G == 64 ? 2 : (P <= 128 ? 0 : 1); }
 
-extern "C" int gdp_plan_tuned(const gdp_problem *probs, int32_t n, int32_t flags, const gdp_plan_params *params,
+extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t flags, const gdp_plan_params *params,
                               void *descs_out, int64_t *dir_words, int64_t *score_slots) {
-  if (n < 0 || (n > 0 && probs == nullptr) || descs_out == nullptr) return fail(GDP_EINVAL, "gdp_plan: bad arguments");
+  if (n < 0 || (n > 0 && probs_in == nullptr) || descs_out == nullptr)
+    return fail(GDP_EINVAL, "gdp_plan: bad arguments");
+  // the problems with their bands narrowed to the matrix: waves group
+  // problems by the bands they are swept with
+  std::vector<gdp_problem> narrowed(probs_in, probs_in + n);
+  for (gdp_problem &p : narrowed) {
+    if (p.lband < 0 || p.uband < 0 || p.rl < 1 || p.gl < 1) continue;  // validate refuses it
+    int lb = p.lband, ub = p.uband;
+    narrow_bands(p, lb, ub);
+    p.lband = (int16_t)lb;
+    p.uband = (int16_t)ub;
+  }
+  const gdp_problem *probs = narrowed.data();
   const gdp_plan_params pp = params ? *params : g_default_params;
   if (pp.g16_max_span < 0 || pp.g16_max_span > 240 || pp.g64_min_rows < 1)
     return fail(GDP_EINVAL, "gdp_plan: g16_max_span must lie in [0, 240] and g64_min_rows be positive");
@@ -408,7 +440,7 @@ extern "C" uint32_t gdp_plan_kernels(const void *descs, int32_t n) {
   memcpy(&h, (const uint8_t *)descs + (int64_t)n * sizeof(gdp_dev_problem), sizeof h);
   if (h.magic != (GDP_PAIR_MAGIC ^ (uint64_t)n)) return GDP_FILL_ALL;
   uint32_t mask = 0;
-  int pmax = 0, pstd = 0;
+  int pmax = 0, pstd = 0, pwide = 0;
   for (int32_t i = 0; i < n; i++) {
     if (d[i].layout == 2) mask |= GDP_FILL_NARROW;
     if (d[i].layout != 0) continue;
@@ -416,6 +448,9 @@ extern "C" uint32_t gdp_plan_kernels(const void *descs, int32_t n) {
     if (d[i].kind == GDP_KIND_STD) {
       mask |= GDP_FILL_STD;
       pstd = gdp_imax(pstd, 2 * P);  // ring entries of two words (H and F in 32 bits)
+    } else if (P > GDP_RING) {
+      mask |= GDP_FILL_WIDE;
+      pwide = gdp_imax(pwide, P);
     } else {
       mask |= 1u;
       pmax = gdp_imax(pmax, P);
@@ -431,6 +466,7 @@ extern "C" uint32_t gdp_plan_kernels(const void *descs, int32_t n) {
   };
   if (mask & 1u) mask |= rlog(pmax) << 8;
   if (mask & GDP_FILL_STD) mask |= rlog(pstd) << 12;
+  if (mask & GDP_FILL_WIDE) mask |= rlog(pwide) << 16;
   for (int c = 0; c < GDP_NCLS; c++)
     if (h.nw[c] > 0) mask |= 2u << c;
   return mask;
@@ -449,6 +485,45 @@ extern "C" int gdp_fill_ex(const void *d_descs, int32_t n, uint32_t kernels, con
 extern "C" int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
                         int16_t *d_scores, int32_t *d_final, void *stream) {
   return gdp_fill_ex(d_descs, n, GDP_FILL_ALL, d_qbuf, d_gbuf, d_dirs, d_scores, d_final, stream);
+}
+
+// The launch mask of one descriptor filled by itself (a GDP_PLAN_SINGLE
+// problem): its own kernel, with the ring its batch period needs.
+static uint32_t kernels_of_one(const gdp_dev_problem &d) {
+  const int P = gdp_make_geom(d).P;
+  const int need = d.kind == GDP_KIND_STD ? 2 * P : P;
+  int k = 6;
+  while ((1 << k) < need) k++;
+  if (d.kind == GDP_KIND_STD) return GDP_FILL_STD | ((uint32_t)k << 12);
+  return need > GDP_RING ? (GDP_FILL_WIDE | ((uint32_t)k << 16)) : (1u | ((uint32_t)k << 8));
+}
+
+extern "C" int gdp_align_each(const void *descs, const void *d_descs, int32_t n, const char *d_qbuf,
+                              const char *d_gbuf, uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final,
+                              const gdp_tb_request *d_reqs, const void *d_genome, int32_t genome_format,
+                              uint32_t genome_len, gdp_pair *d_pairs, int32_t *d_counts, void *stream) {
+  if (!g_init) return fail(GDP_ENOINIT, "gdp_align_each: gdp_init has not been called");
+  if (n < 0 || (n > 0 && (!descs || !d_descs || !d_qbuf || !d_gbuf || !d_dirs || !d_final || !d_reqs || !d_pairs ||
+                          !d_counts)))
+    return fail(GDP_EINVAL, "gdp_align_each: null argument");
+  const gdp_dev_problem *h = (const gdp_dev_problem *)descs;
+  for (int32_t k = 0; k < n; k++)
+    if (h[k].layout != 0)
+      return fail(GDP_EINVAL, "gdp_align_each: problem %d was not planned with GDP_PLAN_SINGLE", k);
+  hipStream_t st = (hipStream_t)stream;
+  for (int32_t k = 0; k < n; k++) {
+    const uint8_t *dk = (const uint8_t *)d_descs + (int64_t)k * sizeof(gdp_dev_problem);
+    if (gdp_launch_fill(dk, 1, kernels_of_one(h[k]), d_qbuf, d_gbuf, d_dirs, d_scores, d_final + k, st))
+      return hip_fail("gdp_align_each fill launch", hipGetLastError());
+    // the request names its problem by index in the whole array; the walk
+    // reads the array, so it sees problem k's descriptor
+    if (gdp_launch_traceback(d_descs, d_reqs + k, 1, d_qbuf, d_gbuf, d_dirs, d_genome, genome_format, genome_len,
+                             d_pairs, d_counts + 6 * (int64_t)k, nullptr, st))
+      return hip_fail("gdp_align_each traceback launch", hipGetLastError());
+    const hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail("gdp_align_each", e);
+  }
+  return GDP_OK;
 }
 
 extern "C" int gdp_best_endpoint_ex(const void *d_descs, const int32_t *d_ul, int32_t m, int32_t queryend_indels,

@@ -17,6 +17,7 @@
 //   5  serving: a random genome packed as genomecomp (with runs of N), and
 //      alignment requests with the length and divergence mix of 3 against
 //      segments of it on either strand, arriving as a Poisson process
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -419,6 +420,7 @@ int64_t gdp_serve_drive(gdp_server *s, const gdp_align_request *reqs, int64_t n,
   }
   const uint64_t tag0 = reqs[0].tag;
   const double t0 = gdp_server_now() + 0.002;
+  std::atomic<bool> sub_done{false};
   std::thread sub([&]() {
     int64_t k = 0;
     while (k < n) {
@@ -435,6 +437,7 @@ int64_t gdp_serve_drive(gdp_server *s, const gdp_align_request *reqs, int64_t n,
         else std::this_thread::yield();
       }
     }
+    sub_done = true;
   });
   std::vector<gdp_align_result> res(65536);
   int64_t got = 0, ok = 0;
@@ -442,6 +445,14 @@ int64_t gdp_serve_drive(gdp_server *s, const gdp_align_request *reqs, int64_t n,
   while (got < n) {
     const int32_t m = gdp_server_poll(s, res.data(), (int32_t)res.size(), 1000);
     if (m < 0) break;
+    if (m == 0 && sub_done.load()) {
+      // a server stopped on a failure refuses what is left; once nothing
+      // more arrives, stop waiting
+      gdp_server_stats st;
+      if (gdp_server_get_stats(s, &st) != GDP_OK && gdp_server_poll(s, res.data(), 0, 0) == 0 &&
+          st.completed >= st.submitted)
+        break;
+    }
     for (int32_t i = 0; i < m; i++) {
       const int64_t k = (int64_t)(res[i].tag - tag0);
       if (k < 0 || k >= n) continue;

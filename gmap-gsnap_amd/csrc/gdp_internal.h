@@ -10,8 +10,12 @@
 // later through a DPP row shift, and the last lane hands row r+15 to lane 0
 // through an LDS ring indexed by column.
 #define GDP_GROUP 16
-// Ring entries per group (power of two, > batch period - 15).
+// Ring entries per group (power of two, > batch period - 15).  Problems
+// whose batch period exceeds GDP_RING (bands wider than about 1000 columns)
+// take the wide launch of the single sweep, with rings of up to
+// GDP_RING_WIDE entries.
 #define GDP_RING 1024
+#define GDP_RING_WIDE 8192
 
 // Packed direction nibble, one per cell, same bits as the test oracle:
 // bit0 nogap==HORIZ, bit1 nogap==VERT, bit2 Egap!=DIAG, bit3 Fgap!=DIAG.

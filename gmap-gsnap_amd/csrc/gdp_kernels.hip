@@ -446,12 +446,16 @@ __device__ __forceinline__ void v1_sweep_std(const gdp_tables_lds &tab, const gd
 // two are separate kernels so that neither carries the other's registers.
 // Dynamic LDS per group: ringsz ring words (a power of two, at least the
 // largest batch period of the plan's problems of that launch, two words per
-// column for the standard fill), then GDP_CCAP bytes of column classes.
+// column for the standard fill), then GDP_CCAP bytes of column classes.  A
+// launch sweeps the problems whose ring need (batch period, twice that for
+// the standard fill) lies in (ring_lo, ringsz]: the wide launch, one wave per
+// workgroup with rings of up to GDP_RING_WIDE words, takes those past
+// GDP_RING.
 template <bool STD>
 __device__ __forceinline__ void single_sweeps(const gdp_dev_problem *__restrict__ probs, int n,
                                               const char *__restrict__ qbuf, const char *__restrict__ gbuf,
                                               uint32_t *__restrict__ dirs, int16_t *__restrict__ scores,
-                                              int32_t *__restrict__ final_out, int ringsz) {
+                                              int32_t *__restrict__ final_out, int ringsz, int ring_lo) {
   __shared__ gdp_tables_lds tab;
   extern __shared__ uint32_t ring_all[];
   // A plan lists its single-sweep problems after its header (the SIMD-rule
@@ -468,13 +472,16 @@ __device__ __forceinline__ void single_sweeps(const gdp_dev_problem *__restrict_
 
   const int lane = threadIdx.x & (GDP_GROUP - 1);
   const int grp = threadIdx.x / GDP_GROUP;
+  const int groups = blockDim.x / GDP_GROUP;
   uint32_t *ring = ring_all + grp * ringsz;
-  uint8_t *ccls = (uint8_t *)(ring_all + GDP_GROUPS_PER_BLOCK * ringsz) + grp * GDP_CCAP;
+  uint8_t *ccls = (uint8_t *)(ring_all + groups * ringsz) + grp * GDP_CCAP;
   // groups walk the list grid-stride; no barrier follows
-  for (int slot = blockIdx.x * GDP_GROUPS_PER_BLOCK + grp; slot < n1; slot += gridDim.x * GDP_GROUPS_PER_BLOCK) {
+  for (int slot = blockIdx.x * groups + grp; slot < n1; slot += gridDim.x * groups) {
     const int pid = list1 ? list1[slot] : slot;
     const gdp_dev_problem p = probs[pid];
     if (p.layout != 0 || (p.kind == GDP_KIND_STD) != STD) continue;  // another launch sweeps it
+    const int need = (STD ? 2 : 1) * gdp_make_geom(p).P;
+    if (need <= ring_lo || need > ringsz) continue;  // the other ring size's launch sweeps it
     // one loop per band kind; the plan's list keeps each kind together, so
     // the groups of a wave run the same one
     if (STD)
@@ -494,8 +501,8 @@ __global__ __launch_bounds__(GDP_BLOCK, 5) void gdp_fill_kernel(const gdp_dev_pr
                                                              const char *__restrict__ qbuf,
                                                              const char *__restrict__ gbuf, uint32_t *__restrict__ dirs,
                                                              int16_t *__restrict__ scores, int32_t *__restrict__ final_out,
-                                                             int ringsz) {
-  single_sweeps<false>(probs, n, qbuf, gbuf, dirs, scores, final_out, ringsz);
+                                                             int ringsz, int ring_lo) {
+  single_sweeps<false>(probs, n, qbuf, gbuf, dirs, scores, final_out, ringsz, ring_lo);
 }
 
 __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_std_kernel(const gdp_dev_problem *__restrict__ probs, int n,
@@ -503,8 +510,9 @@ __global__ __launch_bounds__(GDP_BLOCK) void gdp_fill_std_kernel(const gdp_dev_p
                                                                  const char *__restrict__ gbuf,
                                                                  uint32_t *__restrict__ dirs,
                                                                  int16_t *__restrict__ scores,
-                                                                 int32_t *__restrict__ final_out, int ringsz) {
-  single_sweeps<true>(probs, n, qbuf, gbuf, dirs, scores, final_out, ringsz);
+                                                                 int32_t *__restrict__ final_out, int ringsz,
+                                                                 int ring_lo) {
+  single_sweeps<true>(probs, n, qbuf, gbuf, dirs, scores, final_out, ringsz, ring_lo);
 }
 
 // ------------------------------------------------------------ narrow sweep
@@ -917,7 +925,10 @@ __global__ __launch_bounds__(256, 8) void gdp_traceback_kernel(
   const int kind = pp->kind == GDP_KIND_STD ? GDP_KIND_FULL : pp->kind;
   const int64_t p_qoff = pp->qoff, p_goff = pp->goff, p_galtoff = pp->galtoff;
   if (q.r < 0 || q.r > pp->rl || q.c < 0 || q.c > pp->gl ||  // start cell outside the matrix
-      (local && kind != GDP_KIND_UPPER && kind != GDP_KIND_LOWER)) {  // local walks are one-sided
+      (local && kind != GDP_KIND_UPPER && kind != GDP_KIND_LOWER) ||  // local walks are one-sided
+      (local && endc < 0)) {  // a stop column left of the matrix: column -1 is no cell
+    // (a stop column right of the start is the reference's case too: the
+    // walk stops at once and its closing skip is empty)
     if (j == 0) cnt[0] = -1;
     return;
   }
@@ -1619,29 +1630,36 @@ extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, con
                                uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, hipStream_t stream) {
   if (n <= 0) return 0;
   // resident blocks only; they walk the plan's list of single sweeps with
-  // the ring size the plan asks for (GDP_RING when it does not say)
-  auto single = [&](bool std_fill, int rlog) -> int {
-    const int ringsz = rlog ? 1 << rlog : GDP_RING;
-    if (ringsz > GDP_RING) return -1;
-    const size_t lds = (size_t)GDP_GROUPS_PER_BLOCK * (ringsz * sizeof(uint32_t) + GDP_CCAP);
-    static int cached[64][2][16] = {{{0}}};
+  // the ring size the plan asks for (GDP_RING, or GDP_RING_WIDE for the
+  // wide launch, when it does not say).  The wide launch runs one wave of
+  // four groups per workgroup, so that four rings of GDP_RING_WIDE words fit
+  // the LDS of a workgroup.
+  auto single = [&](bool std_fill, int rlog, bool wide) -> int {
+    const int cap = wide ? GDP_RING_WIDE : GDP_RING;
+    const int ringsz = rlog ? 1 << rlog : cap;
+    if (ringsz > cap || (wide && std_fill)) return -1;
+    const int threads = wide ? 64 : GDP_BLOCK, groups = threads / GDP_GROUP;
+    const size_t lds = (size_t)groups * (ringsz * sizeof(uint32_t) + GDP_CCAP);
+    static int cached[64][2][2][16] = {{{{0}}}};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
-    const int slot = rlog ? rlog : 10;
+    const int slot = rlog ? rlog : (wide ? 13 : 10);
     auto kern = std_fill ? gdp_fill_std_kernel : gdp_fill_kernel;
-    if (cached[dev][std_fill][slot] == 0) {
+    int &res = cached[dev][std_fill][wide][slot];
+    if (res == 0) {
       int ncu = 0, per = 0;
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, GDP_BLOCK, lds) != hipSuccess) return -1;
-      cached[dev][std_fill][slot] = ncu * (per > 0 ? per : 1);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, lds) != hipSuccess) return -1;
+      res = ncu * (per > 0 ? per : 1);
     }
-    const int blocks = gdp_imin(cached[dev][std_fill][slot], (n + GDP_GROUPS_PER_BLOCK - 1) / GDP_GROUPS_PER_BLOCK);
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(GDP_BLOCK), lds, stream, (const gdp_dev_problem *)d_descs, n,
-                       d_qbuf, d_gbuf, d_dirs, d_scores, d_final, ringsz);
+    const int blocks = gdp_imin(res, (n + groups - 1) / groups);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, stream, (const gdp_dev_problem *)d_descs, n,
+                       d_qbuf, d_gbuf, d_dirs, d_scores, d_final, ringsz, wide ? GDP_RING : 0);
     return hipGetLastError() == hipSuccess ? 0 : -1;
   };
-  if ((kernels & 1u) && single(false, (kernels >> 8) & 15)) return -1;
-  if ((kernels & GDP_FILL_STD) && single(true, (kernels >> 12) & 15)) return -1;
+  if ((kernels & 1u) && single(false, (kernels >> 8) & 15, false)) return -1;
+  if ((kernels & GDP_FILL_WIDE) && single(false, (kernels >> 16) & 15, true)) return -1;
+  if ((kernels & GDP_FILL_STD) && single(true, (kernels >> 12) & 15, false)) return -1;
   if (kernels & GDP_FILL_NARROW) {
     static int cached_n[64] = {0};
     int dev = 0;

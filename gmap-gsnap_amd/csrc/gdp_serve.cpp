@@ -157,13 +157,16 @@ struct gdp_server {
   int64_t dir_in_flight = 0;
 
   std::atomic<bool> stop{false};
-  std::string fatal;
+  // Set once a HIP call of the server thread failed: from then on no batch
+  // starts, pending and newly submitted requests are refused with GDP_EHIP.
+  std::atomic<bool> failed{false};
+  std::string fatal;  // what failed (under smu)
   std::thread th;
 
   void run();
   bool launch(Slot &s);
   void retire(Slot &s);
-  void fail_all(const char *what, hipError_t e);
+  void fail_all(const std::string &what);
   void post(DoneBatch &&b) {
     {
       std::lock_guard<std::mutex> lk(dmu);
@@ -197,8 +200,12 @@ static gdp_align_result refused(const Req &q, double t) {
   return r;
 }
 
-void gdp_server::fail_all(const char *what, hipError_t e) {
-  fatal = std::string(what) + ": " + hipGetErrorString(e);
+void gdp_server::fail_all(const std::string &what) {
+  {
+    std::lock_guard<std::mutex> lk(smu);
+    if (fatal.empty()) fatal = what;
+  }
+  failed = true;
   DoneBatch b;
   const double t = now_s();
   for (Slot &s : slots) {
@@ -221,10 +228,18 @@ void gdp_server::fail_all(const char *what, hipError_t e) {
       for (Req &q : pq) rest.push_back(q);
     pending[0].clear();
     pending[1].clear();
+    // no read waits in the ring any more: a submitter blocked on ring space
+    // wakes up and is refused
+    ring.freed = ring.next;
   }
+  cv_space.notify_all();
   for (Req &q : rest) {
     q.status = (int8_t)-GDP_EHIP;
     b.res.push_back(refused(q, t));
+  }
+  {
+    std::lock_guard<std::mutex> lk(smu);
+    st.completed += (int64_t)b.res.size();
   }
   post(std::move(b));
 }
@@ -288,15 +303,21 @@ bool gdp_server::launch(Slot &s) {
     }
     const gdp_plan_params *pp = (cfg.plan.g16_max_span || cfg.plan.g64_min_rows) ? &cfg.plan : nullptr;
     if (gdp_plan_tuned(probs.data(), n, 0, pp, s.h_up, &dw, &ss) != GDP_OK) {
-      // Requests were validated at submit; a planning failure is a bug.
-      fatal = std::string("gdp_server: planning failed: ") + gdp_last_error();
+      // Requests and the plan parameters were validated before; a planning
+      // failure is a bug: refuse this batch and stop like a HIP failure.
+      const std::string why = std::string("gdp_server: planning failed: ") + gdp_last_error();
       DoneBatch b;
       const double t = now_s();
       for (Req &q : batch) {
         q.status = (int8_t)-GDP_EINVAL;
         b.res.push_back(refused(q, t));
       }
+      {
+        std::lock_guard<std::mutex> lk(smu);
+        st.completed += (int64_t)b.res.size();
+      }
       post(std::move(b));
+      fail_all(why);
       return true;
     }
     const int64_t need = align_up(dw * 4, 256);
@@ -415,7 +436,7 @@ bool gdp_server::launch(Slot &s) {
     rc = e == hipSuccess ? GDP_OK : GDP_EHIP;
   }
   if (rc != GDP_OK) {
-    fail_all("gdp_server launch", e != hipSuccess ? e : hipGetLastError());
+    fail_all(std::string("gdp_server launch: ") + hipGetErrorString(e != hipSuccess ? e : hipGetLastError()));
     return true;
   }
   {
@@ -483,13 +504,13 @@ void gdp_server::run() {
         st.retire_seconds += now_s() - t0;
         progressed = true;
       } else if (e != hipErrorNotReady) {
-        fail_all("gdp_server batch", e);
+        fail_all(std::string("gdp_server batch: ") + hipGetErrorString(e));
         progressed = true;
       }
     }
     bool any_busy = false;
     for (Slot &s : slots) {
-      if (!s.busy && fatal.empty() && !stop.load()) {
+      if (!s.busy && !failed.load() && !stop.load()) {
         const double t0 = now_s();
         if (launch(s)) {
           progressed = true;
@@ -519,6 +540,12 @@ extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const voi
       cfg->max_rl > GDP_MAX_LEN || cfg->max_gl <= 0 || cfg->max_gl > GDP_MAX_LEN || cfg->dir_budget_bytes <= 0 ||
       cfg->pair_capacity < (int64_t)cfg->max_rl + cfg->max_gl + 2) {
     set_err("gdp_server_create: bad configuration");
+    return nullptr;
+  }
+  // lane groups: all zero for gdp_plan's defaults, else what gdp_plan_tuned accepts
+  if ((cfg->plan.g16_max_span != 0 || cfg->plan.g64_min_rows != 0) &&
+      (cfg->plan.g16_max_span < 0 || cfg->plan.g16_max_span > 240 || cfg->plan.g64_min_rows < 1)) {
+    set_err("gdp_server_create: plan.g16_max_span must lie in [0, 240] and plan.g64_min_rows be positive");
     return nullptr;
   }
   if (genome_format != GDP_GENOME_ASCII && genome_format != GDP_GENOME_COMP) {
@@ -588,7 +615,9 @@ extern "C" int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, i
     q.status = 0;
     q.roff = 0;
     int32_t lb = 0, ub = 0;
-    if (r.rl <= 0 || r.gl <= 0 || r.rl > s->cfg.max_rl || r.gl > s->cfg.max_gl) {
+    if (s->failed.load()) {
+      q.status = (int8_t)-GDP_EHIP;  // the server stopped on an error
+    } else if (r.rl <= 0 || r.gl <= 0 || r.rl > s->cfg.max_rl || r.gl > s->cfg.max_gl) {
       q.status = (int8_t)-GDP_ETOOBIG;
     } else {
       gdp_compute_bands(&lb, &ub, r.rl, r.gl, r.extraband, r.widebandp);
@@ -606,10 +635,15 @@ extern "C" int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, i
     const int u8 = gdp_use8p_size(r.mismatchtype);
     q.width = (r.rl < u8 && r.gl < u8) ? 8 : 16;
     int64_t v = s->ring.place(r.rl);
-    while (!s->ring.fits(v, r.rl)) {  // the ring is full of waiting reads
+    while (!s->ring.fits(v, r.rl) && !s->failed.load()) {  // the ring is full of waiting reads
       s->cv_work.notify_one();
       s->cv_space.wait_for(lk, std::chrono::milliseconds(1));
       v = s->ring.place(r.rl);
+    }
+    if (s->failed.load()) {  // failed while waiting for ring space
+      q.status = (int8_t)-GDP_EHIP;
+      b.res.push_back(refused(q, t));
+      continue;
     }
     memcpy(s->ring.at(v), r.read, (size_t)r.rl);
     s->ring.next = v + r.rl;
@@ -625,7 +659,7 @@ extern "C" int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, i
   }
   if (!b.res.empty()) s->post(std::move(b));
   if (accepted) s->cv_work.notify_one();
-  return GDP_OK;
+  return s->failed.load() ? GDP_EHIP : GDP_OK;
 }
 
 extern "C" int32_t gdp_server_poll(gdp_server *s, gdp_align_result *out, int32_t max, int32_t wait_us) {
@@ -670,7 +704,11 @@ extern "C" int gdp_server_get_stats(gdp_server *s, gdp_server_stats *st) {
   if (!s || !st) return GDP_EINVAL;
   std::lock_guard<std::mutex> lk(s->smu);
   *st = s->st;
-  return s->fatal.empty() ? GDP_OK : GDP_EHIP;
+  if (!s->fatal.empty()) {
+    gdp_set_error(s->fatal.c_str());
+    return GDP_EHIP;
+  }
+  return GDP_OK;
 }
 
 extern "C" void gdp_server_destroy(gdp_server *s) {

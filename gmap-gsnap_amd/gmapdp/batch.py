@@ -285,3 +285,19 @@ class DeviceBatch:
 
     def traceback_one(self, h, i):
         self.traceback_device(h, i, 1)
+
+    def align_each(self, h, stream=None):
+        """gdp_align_each: every problem filled and walked on its own, with a
+        stream synchronize after each (request k of handle h walks problem
+        k).  Needs a batch planned with single=True."""
+        if not self.single:
+            raise RuntimeError("align_each needs a batch planned with single=True")
+        if h["n"] != self.n:
+            raise ValueError("align_each needs one request per problem")
+        descs = self.d_descs.cpu().numpy()
+        gptr = h["d_genome"].data_ptr() if h["d_genome"] is not None else None
+        _check(self.L, self.L.gdp_align_each(descs.ctypes.data, self.d_descs.data_ptr(), self.n, self.d_q.data_ptr(),
+                                             self.d_g.data_ptr(), self.d_dirs.data_ptr(), self.d_scores.data_ptr(),
+                                             self.d_final.data_ptr(), h["d_reqs"].data_ptr(), gptr, 0, h["glen"],
+                                             h["d_pairs"].data_ptr(), h["d_counts"].data_ptr(),
+                                             self.stream if stream is None else stream), "gdp_align_each")

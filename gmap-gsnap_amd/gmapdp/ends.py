@@ -19,7 +19,7 @@ the triangles, and each set of walks is one traceback launch
 import numpy as np
 
 from .batch import DeviceBatch
-from .lib import PAIR_DTYPE, PROBLEM_DTYPE, TBREQ_DTYPE, load
+from .lib import PROBLEM_DTYPE, SJ_PAIR_DTYPE, TBREQ_DTYPE, load
 
 GDP_KIND_UPPER, GDP_KIND_LOWER = 1, 2
 GDP_KEEP_BEST_QUERYEND = 3
@@ -51,7 +51,9 @@ def end5_splicejunction(ends, max_rlength=1000, max_glength=2000):
     jump_late, extraband, defect_rate, contlength.  Returns per end None (the
     reference's NULL list) or (the returned list's pairs from its last
     element to its head -- the push order -- as gdp_pair records,
-    {traceback_score, missscore, nmatches, nmismatches, nopens, nindels})."""
+    {traceback_score, missscore, nmatches, nmismatches, nopens, nindels}).
+    The records are SJ_PAIR_DTYPE: the known-site gap holder has gapp 2 and
+    its 32-bit genomejump is the intron's length."""
     return _splicejunction(ends, True, max_rlength, max_glength)
 
 
@@ -128,12 +130,19 @@ def _splicejunction(ends, five, max_rlength, max_glength):
     p2, c2 = b.traceback(requests(stop[:, 0], stop[:, 1], 0, anchor[sel]))
     cnt = c1[:, 1:] + c2[:, 1:]
     jump = (anchor - far) if five else (far - anchor)
+
+    def widen(p):
+        w = np.zeros(len(p), SJ_PAIR_DTYPE)
+        for f in ("querypos", "genomepos", "cdna", "comp", "genome", "genomealt", "gapp", "genomejump"):
+            w[f] = p[f]
+        return w
+
     for j, i in enumerate(sel):
-        gap = np.zeros(1, PAIR_DTYPE)  # Pairpool_push_gapholder, knownp (pairpool.c:374-452)
+        gap = np.zeros(1, SJ_PAIR_DTYPE)  # Pairpool_push_gapholder, knownp (pairpool.c:374-452)
         gap["querypos"] = gap["genomepos"] = -1
         gap["cdna"] = gap["comp"] = gap["genome"] = gap["genomealt"] = ord(" ")
-        gap["gapp"], gap["genomejump"] = 1, jump[i]
-        pairs = np.concatenate([p1[j], gap, p2[j]])
+        gap["gapp"], gap["genomejump"] = 2, jump[i]
+        pairs = np.concatenate([widen(p1[j]), gap, widen(p2[j])])
         # the reference reverses the list and drops indel pairs from the read's
         # end, which are the first pushed (1905-1910, 2485-2490); the 3' form
         # returns that trimmed list as it is

@@ -48,6 +48,12 @@ TBREQ_DTYPE = np.dtype([("fill", "<i4"), ("r", "<i4"), ("c", "<i4"), ("queryoffs
                         ("max_pairs", "<i4"), ("endc", "<i4")])
 PAIR_DTYPE = np.dtype([("querypos", "<i4"), ("genomepos", "<i4"), ("cdna", "i1"), ("comp", "i1"),
                        ("genome", "i1"), ("genomealt", "i1"), ("gapp", "<i2"), ("genomejump", "<i2")])
+# Pair records of the splice-junction end callers: a known-site gap holder
+# (gapp 2, Pairpool_push_gapholder with knownp, pairpool.c:374-452) spans an
+# intron, so genomejump is 32 bits here (Pair.genomejump is an int, pairdef.h).
+SJ_PAIR_DTYPE = np.dtype([("querypos", "<i4"), ("genomepos", "<i4"), ("cdna", "i1"), ("comp", "i1"),
+                          ("genome", "i1"), ("genomealt", "i1"), ("gapp", "<i2"), ("pad", "<i2"),
+                          ("genomejump", "<i4")])
 SEGMENT_DTYPE = np.dtype([("coord", "<u4"), ("bound", "<u4"), ("length", "<i4"), ("left", "i1"), ("revcomp", "i1"),
                           ("pad", "<i2"), ("out_off", "<i8")])
 ALIGN_REQ_DTYPE = np.dtype([("tag", "<u8"), ("read", "<u8"), ("rl", "<i4"), ("gl", "<i4"), ("roffset", "<i4"),
@@ -112,6 +118,8 @@ def load(path=None):
         "gdp_graph_destroy": (None, [vp]),
         "gdp_plan_kernels": (ctypes.c_uint32, [vp, i32]),
         "gdp_fill_ex": (ctypes.c_int, [vp, i32, ctypes.c_uint32, vp, vp, vp, vp, vp, vp]),
+        "gdp_align_each": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp,
+                                          vp]),
         "gdp_cells": (i64, [vp]),
         "gdp_fill": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, vp, vp]),
         "gdp_best_endpoint": (ctypes.c_int, [vp, vp, i32, i32, vp, vp, vp]),

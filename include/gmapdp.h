@@ -59,8 +59,12 @@ extern "C" {
 #define GDP_KEEP_BEST_QUERYEND 3 /* best end cell, ..._to_queryend_indels_{8,16} */
 #define GDP_BEST_SLOTS 4         /* int16 score slots of a best-cell record: score, r, c, 0 */
 
-/* Largest band span a problem may use (lband + uband, or the one-sided band). */
-#define GDP_MAX_SPAN 960
+/* Largest band span a problem may use (lband + uband, or the one-sided band),
+   after bands wider than the matrix are narrowed to one past it (lband to
+   rl + 1 and uband to gl + 1), which changes no cell.  Full-band int16 problems with spans
+   up to 960 are swept in pairs; wider ones one to a 16-lane group with a
+   ring of up to 8192 entries. */
+#define GDP_MAX_SPAN 8176
 /* GDP_KIND_STD carries 32-bit H and F between row batches, twice the ring
    space per column, so its band span is capped lower. */
 #define GDP_MAX_SPAN_STD 526
@@ -147,7 +151,9 @@ typedef struct gdp_tb_request {
    skip no '*' column, and its genome skips read the fill's own genome
    sequence rather than the device genome.  gdp_traceback_ends reports the
    cell where the walk stopped, from which the caller's next walk starts.
-   A full-band fill with this flag gets counts[0] = -1. */
+   A full-band fill with this flag, or a negative stop column, gets
+   counts[0] = -1 and no pairs; a stop column at or right of the start
+   column stops the walk at once, as the reference's walk does. */
 #define GDP_TB_LOCAL 4
 
 /* One traceback output record, in the order the reference pushes them onto
@@ -246,17 +252,34 @@ int gdp_fill(const void *d_descs, int32_t n, const char *d_qbuf, const char *d_g
    bit 0 single sweeps, bits 1..3 the paired wave classes, bit 4 standard
    fills (GDP_KIND_STD), bit 5 narrow one-sided sweeps (one-sided fills with
    a band under 32 columns that keep at most their best end cell, one per
-   lane), bits 8..11 the log2 of the single sweep's LDS ring
-   and bits 12..15 that of the standard fill's (the next power of two >= the
+   lane), bit 6 the wide single sweep (batch periods past 1024 steps: one
+   wave per workgroup, rings of up to 8192 entries), bits 8..11 the log2 of
+   the single sweep's LDS ring, bits 12..15 that of the standard fill's and
+   bits 16..19 that of the wide sweep's (the next power of two >= the
    largest batch period among their problems, in ring words; 0 means the
    largest ring).  gdp_fill_ex launches only those (gdp_fill launches all,
    and empty ones return at once). */
 #define GDP_FILL_STD 0x10u
 #define GDP_FILL_NARROW 0x20u
-#define GDP_FILL_ALL 0x3Fu
+#define GDP_FILL_WIDE 0x40u
+#define GDP_FILL_ALL 0x7Fu
 uint32_t gdp_plan_kernels(const void *descs, int32_t n);
 int gdp_fill_ex(const void *d_descs, int32_t n, uint32_t kernels, const char *d_qbuf, const char *d_gbuf,
                 uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, void *stream);
+
+/* The one-launch-per-problem form, problem after problem: for k = 0..n-1,
+   problem k is filled alone (one launch of its own kernel, as
+   Dynprog_simd_{8,16} fills one problem per call), then request k is walked
+   (one traceback launch), and the stream is synchronized before the next
+   problem, as a caller that needs each result before it goes on would.
+   descs is the host copy of a GDP_PLAN_SINGLE plan (it gives each problem's
+   kernel); d_descs its device copy; d_reqs[k] the request of problem k.
+   No counterpart in the reference (the per-call baseline of the batched
+   form). */
+int gdp_align_each(const void *descs, const void *d_descs, int32_t n, const char *d_qbuf, const char *d_gbuf,
+                   uint32_t *d_dirs, int16_t *d_scores, int32_t *d_final, const gdp_tb_request *d_reqs,
+                   const void *d_genome, int32_t genome_format, uint32_t genome_len, gdp_pair *d_pairs,
+                   int32_t *d_counts, void *stream);
 
 /* A fill followed by a traceback, captured once into a HIP graph and
    replayed with gdp_graph_launch: the whole batched step in one launch
@@ -520,7 +543,11 @@ typedef struct gdp_server_stats {
 gdp_server *gdp_server_create(const gdp_server_config *cfg, const void *d_genome, int32_t genome_format,
                               uint32_t genome_len);
 
-/* Queues n requests (thread-safe; any number of submitting threads). */
+/* Queues n requests (thread-safe; any number of submitting threads).  Each
+   request gets exactly one result.  Once a HIP call of the server has failed,
+   no batch starts again: the requests still waiting and every request
+   submitted later get results with status GDP_EHIP, and submit returns
+   GDP_EHIP. */
 int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, int32_t n);
 
 /* Moves up to max finished results into out, waiting up to wait_us
@@ -531,6 +558,8 @@ int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, int32_t n);
    polls again. */
 int32_t gdp_server_poll(gdp_server *s, gdp_align_result *out, int32_t max, int32_t wait_us);
 
+/* Copies the statistics; returns GDP_EHIP (message in gdp_last_error) once
+   the server has stopped on a failure. */
 int gdp_server_get_stats(gdp_server *s, gdp_server_stats *st);
 
 /* Finishes the batches in flight and stops the server thread. */

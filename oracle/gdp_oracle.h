@@ -16,7 +16,10 @@
  *                    bit3 Fgap!=DIAG (VERT excludes HORIZ in bit0/1).
  *   pair record    : 8 x int32 {querypos, genomepos, cdna, comp, genome,
  *                    genomealt, gapp, genomejump}, in push order
- *                    (the reference pushes from the end cell back to (0,0)).
+ *                    (the reference pushes from the end cell back to (0,0));
+ *                    gapp is 1 for a gap holder and 2 for a known-site gap
+ *                    holder (Pairpool_push_gapholder with knownp,
+ *                    pairpool.c:374-452).
  */
 #ifndef GDP_ORACLE_H
 #define GDP_ORACLE_H

@@ -151,7 +151,7 @@ void refh_fill_lower(void *ws, int width, char *rseq, char *gseq, char *galt, in
 /* Runs the reference traceback over the tables of the last fill of `kind`
    (0 full, 1 upper, 2 lower) and flattens the pair list into push order:
    8 ints per pair {querypos, genomepos, cdna, comp, genome, genomealt, gapp,
-   genomejump}.  counters[5] receive {score, nmatches, nmismatches, nopens,
+   genomejump}, gapp 2 for a known-site gap holder (pair->knowngapp).  counters[5] receive {score, nmatches, nmismatches, nopens,
    nindels}.  Returns the number of pairs. */
 int refh_traceback(int kind, int width, int r, int c, char *rseq, char *rsequc, char *gseq, char *galt,
                    int queryoffset, int genomeoffset, int revp, unsigned int chroffset,
@@ -201,7 +201,7 @@ int refh_traceback(int kind, int width, int r, int c, char *rseq, char *rsequc, 
     Pair_T pr = (Pair_T)p->first;
     int *o = out + (size_t)k * 8;
     o[0] = pr->querypos; o[1] = (int)pr->genomepos; o[2] = pr->cdna; o[3] = pr->comp;
-    o[4] = pr->genome; o[5] = pr->genomealt; o[6] = pr->gapp ? 1 : 0; o[7] = pr->genomejump;
+    o[4] = pr->genome; o[5] = pr->genomealt; o[6] = pr->gapp ? (pr->knowngapp ? 2 : 1) : 0; o[7] = pr->genomejump;
   }
   return n;
 }
@@ -276,7 +276,7 @@ int refh_flatten(void *list, int *out, int maxpairs) {
     Pair_T pr = (Pair_T)p->first;
     int *o = out + (size_t)k * 8;
     o[0] = pr->querypos; o[1] = (int)pr->genomepos; o[2] = pr->cdna; o[3] = pr->comp;
-    o[4] = pr->genome; o[5] = pr->genomealt; o[6] = pr->gapp ? 1 : 0; o[7] = pr->genomejump;
+    o[4] = pr->genome; o[5] = pr->genomealt; o[6] = pr->gapp ? (pr->knowngapp ? 2 : 1) : 0; o[7] = pr->genomejump;
   }
   return n;
 }

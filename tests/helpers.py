@@ -85,6 +85,40 @@ def random_specs(seed, n, kinds=(0, 1, 2), widths=(8, 16), maxrl8=60, maxrl16=20
     return specs
 
 
+def wide_specs(seed, n):
+    """Problems whose band spans more than 1000 columns (the wide single
+    sweep): short reads against long genome windows and the reverse, with
+    Dynprog_compute_bands' widened bands, both one-sided kinds, and bands
+    past the matrix's edges (narrowed by the planner, no cell changes)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for t in range(n):
+        shape = t % 6
+        eb = int(rng.integers(3, 30))
+        if shape in (0, 2):  # a short read against a long window
+            rl, gl = int(rng.integers(10, 80)), int(rng.integers(1100, 2600))
+        elif shape in (1, 3):  # a long read against a short window
+            rl, gl = int(rng.integers(1100, 2000)), int(rng.integers(10, 60))
+        else:  # bands past both edges of a small matrix
+            rl, gl = int(rng.integers(20, 200)), int(rng.integers(20, 200))
+        g = rand_seq(rng, gl, 0.01)
+        src = g if gl >= rl else g + rand_seq(rng, rl)
+        q = mutate(rng, src, 0.05, 0.02)[:rl].ljust(rl, b"A")
+        lb, ub = (eb, gl - rl + eb) if gl >= rl else (rl - gl + eb, eb)
+        if shape in (0, 1):
+            kind, width = 0, 16
+        elif shape == 2:
+            kind, width = 1, int(rng.choice([8, 16]))
+        elif shape == 3:
+            kind, width = 2, int(rng.choice([8, 16]))
+        else:
+            kind, width = 0, 16
+            lb, ub = rl + int(rng.integers(1, 40)), gl + int(rng.integers(1, 900))
+        late, revp = bool(t % 2), bool((t // 2) % 2)
+        specs.append(Spec(kind, width, q, g, g, int(rng.integers(0, 4)), -8, -3, lb, ub, late, revp))
+    return specs
+
+
 def random_std_specs(seed, n, maxrl=260):
     """Problems for the scalar standard fill (kind 3): the shapes of
     random_specs plus its switches -- upperp / lowerp off, a saturation floor
@@ -216,7 +250,8 @@ def oracle_end5_splicejunction(O, e, five=True):
         p, cnt, (r, c) = O.traceback_local(kind, Du if kind == 1 else Dl, r, c, endc, q, q.upper(), g, ga,
                                            e["rev_roffset"], goff, five, e["genestrand"])
         walk.append((p, cnt))
-    gap = np.array([[-1, -1, 32, 32, 32, 32, 1, (e["anchor"] - e["far"]) * (1 if five else -1)]], np.int32)
+    # the known-site gap holder (pairpool.c:374-452, knownp): gapp 2
+    gap = np.array([[-1, -1, 32, 32, 32, 32, 2, (e["anchor"] - e["far"]) * (1 if five else -1)]], np.int32)
     pairs = np.concatenate([walk[0][0], gap, walk[1][0]])
     cnt = walk[0][1] + walk[1][1]
     k = 0
@@ -226,9 +261,11 @@ def oracle_end5_splicejunction(O, e, five=True):
     return pairs, np.array([cnt[0], cnt[0] - rl * 3, cnt[1], cnt[2], cnt[3], cnt[4]], np.int32)
 
 
-def random_splice_ends(rng, n):
+def random_splice_ends(rng, n, long_jumps=False):
     """Splice-junction end cases: a read end against a genome window that
-    holds its continuation past the far column."""
+    holds its continuation past the far column.  long_jumps draws the
+    anchor-to-far distance (the intron the known gap holder spans) from
+    32768-500000 instead of 20-400."""
     out = []
     for t in range(n):
         rl = int(rng.integers(8, 70))
@@ -238,9 +275,10 @@ def random_splice_ends(rng, n):
         if t % 6 == 0:
             q = rand_seq(rng, rl)  # no match: a negative best score, no walk
         ga = bytes(x if rng.random() > 0.05 else ord("R") for x in g)
-        anchor = int(rng.integers(200, 5000))
+        anchor = int(rng.integers(600000, 2000000) if long_jumps else rng.integers(200, 5000))
+        jump = int(rng.integers(32768, 500001) if long_jumps else rng.integers(20, 400))
         out.append({"rev_rseq": q, "rev_gseq": g, "rev_galt": ga, "rev_roffset": int(rng.integers(rl, rl + 300)),
-                    "anchor": anchor, "far": anchor + int(rng.integers(20, 400)) * (-1 if t % 2 else 1),
+                    "anchor": anchor, "far": anchor + jump * (-1 if t % 2 else 1),
                     "genestrand": 0,
                     "jump_late": bool(t % 2), "extraband": int(rng.integers(3, 10)),
                     "defect_rate": float(rng.choice([0.001, 0.01, 0.05])),

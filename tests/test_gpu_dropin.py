@@ -123,3 +123,39 @@ def test_dropin_matches_oracle(oracle, libs):
         assert [x.value for x in cnt] == list(wcnt), ("drop-in counters", i, s.kind, W)
         ntb += 1
     assert ntb > 50
+
+
+def test_dropin_wide_bands(oracle, libs):
+    """GMAP's end callers chop the genome window to the workspace's
+    max_glength (at least 2000, dynprog.c:608-623) and widen the band to the
+    far corner, so a short read end meets a band of about 2000 columns: the
+    drop-in fills it on the GPU (wide single sweep) like any other."""
+    ref, mine, _, _, pool = libs
+    ws = ref.refh_new(100, 2100, 0)
+    rng = np.random.default_rng(5)
+    for t in range(6):
+        rl, gl = int(rng.integers(30, 60)), int(rng.integers(1900, 2001))
+        g = H.rand_seq(rng, gl)
+        q = H.mutate(rng, g[-rl - 5:], 0.03, 0.01)[:rl].ljust(rl, b"C")
+        eb = 6
+        s = H.Spec(0, 16, q, g, g, 3, -10, -2, eb, gl - rl + eb, bool(t % 2), bool(t % 3 == 0))
+        qq, gg, gga = po._Seq(s.q, s.revp), po._Seq(s.g, s.revp), po._Seq(s.galt, s.revp)
+        quc = po._Seq(s.q.upper(), s.revp)
+        m, no, eg, fg = _fill(mine, s, ws, qq, gg, gga)
+        Hm = _read(m, s.gl, s.rl, 16)
+        Dm = _enc(_read(no, s.gl, s.rl, 16), _read(eg, s.gl, s.rl, 16), _read(fg, s.gl, s.rl, 16))
+        Ho, Do = H.oracle_fill(oracle, s)
+        B = H.band_mask(s)
+        assert (Hm[B] == Ho[B]).all() and (Dm[B] == Do[B]).all(), ("wide drop-in fill", t)
+        ref.refh_pool_reset(c_p(pool))
+        cnt = [c_i(0) for _ in range(5)]
+        args = [c_p(None)] + [ctypes.byref(x) for x in cnt] + [c_p(no), c_p(eg), c_p(fg)]
+        args += [c_i(s.rl), c_i(s.gl), c_p(qq.ptr), c_p(quc.ptr), c_p(gg.ptr), c_p(gga.ptr), c_i(5), c_i(7)]
+        args += [c_p(None), c_p(None), c_p(pool), c_b(s.revp), c_u(0), c_u(0), c_b(True), c_i(0), c_i(0)]
+        lst = mine.Dynprog_traceback_16(*args)
+        out = np.zeros((s.rl + s.gl + 8, 8), np.int32)
+        n = ref.refh_flatten(c_p(lst), out.ctypes.data, len(out))
+        Dmask = np.where(H.computed_mask(s), Do, 0).astype(np.uint8)
+        want, wcnt = oracle.traceback(0, Dmask, s.rl, s.gl, s.q, s.q.upper(), s.g, s.galt, 5, 7, s.revp)
+        assert n == len(want) and (out[:n] == want).all(), ("wide drop-in pairs", t)
+        assert [x.value for x in cnt] == list(wcnt), ("wide drop-in counters", t)

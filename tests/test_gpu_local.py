@@ -43,6 +43,12 @@ def test_local_walks(oracle):
         po, co, eo = oracle.traceback_local(s.kind, D, r, c, endc, s.q, s.q.upper(), s.g, s.galt, 3, 11, s.revp)
         assert (H.pairs_to_records(pairs[j]) == po).all(), ("local walk pairs", j, s.kind, r, c, endc)
         assert (counts[j, 1:] == co).all() and tuple(ends[j]) == eo, ("local walk counters", j)
+    # a negative stop column is refused (column -1 is no cell)
+    i, r, c, _ = walks[0]
+    q = H.tb_request(i, specs[i], r, c)
+    q["flags"], q["endc"] = GDP_TB_LOCAL, -1
+    _, cneg = b.traceback(np.array([q]))
+    assert cneg[0, 0] == -1
     # a full-band fill refuses the flag
     full = [H.Spec(0, 16, s.q, s.g, s.g, 3, -10, -2, 3, 3, False, False) for s in specs[:1]]
     probs, qbuf, gbuf = H.pack_batch(full)
@@ -55,13 +61,15 @@ def test_local_walks(oracle):
 
 
 @pytest.mark.parametrize("five", [True, False])
-def test_splicejunction(oracle, five):
+@pytest.mark.parametrize("long_jumps", [False, True])
+def test_splicejunction(oracle, five, long_jumps):
     """Dynprog_end5/3_splicejunction composed from the batched kernels
     (gmapdp.ends) against the oracle's composition, which
-    tests/test_oracle_end.py pins against the reference's own callers."""
+    tests/test_oracle_end.py pins against the reference's own callers; with
+    long_jumps the known gap holders span 32768-500000 bases."""
     from gmapdp.ends import end3_splicejunction, end5_splicejunction
-    rng = np.random.default_rng(36 if five else 38)
-    ends = H.random_splice_ends(rng, 400)
+    rng = np.random.default_rng((36 if five else 38) + (100 if long_jumps else 0))
+    ends = H.random_splice_ends(rng, 400, long_jumps)
     got = (end5_splicejunction if five else end3_splicejunction)(ends)
     walked = 0
     for k, e in enumerate(ends):

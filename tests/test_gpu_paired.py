@@ -75,7 +75,8 @@ def test_paired_lane_group_choice(oracle, params):
     d = b.d_descs.cpu().numpy()[: b.n * 96].reshape(b.n, 96)
     layout, glog = d[:, 58], d[:, 59]  # gdp_dev_problem.layout / .glog
     rl = np.array([s.rl for s in specs])
-    span = np.array([s.lband + s.uband for s in specs])
+    # the span the planner sweeps: bands narrowed to one past the matrix
+    span = np.array([min(s.lband, s.rl + 1) + min(s.uband, s.gl + 1) for s in specs])
     want64 = (span > params[0]) | (rl >= params[1])
     assert (layout == 1).all()
     assert ((glog == 6) == want64).all()

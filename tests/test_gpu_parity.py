@@ -277,3 +277,16 @@ def test_end_mix_device_requests(oracle):
                 assert torch.equal(x, y), rep
     finally:
         b.graph_destroy(g)
+
+
+@pytest.mark.parametrize("keep", [True, False])
+def test_wide_bands(oracle, keep):
+    """Bands spanning 1000-2700 columns (Dynprog_compute_bands with
+    widebandp over a short read and a long window, or the reverse) take the
+    wide single sweep; bands past the matrix's edges are narrowed by the
+    planner.  Cells, final scores and walks against the oracle."""
+    specs = H.wide_specs(91 if keep else 92, 36)
+    b = _run(oracle, specs, keep_scores=keep)
+    assert b.kernels & 0x40, "no problem took the wide single sweep"
+    _check_fills(oracle, specs, b)
+    _check_tracebacks(oracle, specs, b, 93)

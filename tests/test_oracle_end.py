@@ -92,11 +92,13 @@ def test_traceback_local_matches_reference(oracle, reference):
 
 
 @pytest.mark.parametrize("five", [True, False])
-def test_splicejunction_matches_reference(oracle, reference, five):
+@pytest.mark.parametrize("long_jumps", [False, True])
+def test_splicejunction_matches_reference(oracle, reference, five, long_jumps):
     """The oracle's composition of Dynprog_end5_splicejunction (five) and
-    Dynprog_end3_splicejunction against the reference's own callers."""
-    rng = np.random.default_rng(35 if five else 37)
-    ends = H.random_splice_ends(rng, 300)
+    Dynprog_end3_splicejunction against the reference's own callers; with
+    long_jumps the known gap holder spans 32768-500000 bases, past int16."""
+    rng = np.random.default_rng((35 if five else 37) + (100 if long_jumps else 0))
+    ends = H.random_splice_ends(rng, 300, long_jumps)
     got = 0
     for k, e in enumerate(ends):
         want = reference.end5_splicejunction(e["rev_rseq"], e["rev_gseq"], e["rev_galt"], e["rev_roffset"],
