@@ -1,9 +1,13 @@
-// gdp_dynprog.cpp -- the reference's dynprog_simd.h entry points over the GPU
+// gdp_dynprog.hip -- the reference's dynprog_simd.h entry points over the GPU
 // core (include/gmapdp_dynprog.h).  One call = one problem: the sequences go
 // to the device, one fill launch runs, and the result comes back into the
 // caller's Dynprog_T workspace in the reference's layout.  Tracebacks walk
 // the device copy of the last fill of the same workspace and replay the
 // walk's pair-pool calls on the caller's own Pairpool.
+//
+// Device buffers: each workspace's fills get buffers of their own, allocated
+// (and grown) on first use.  Dynprog_simd_16_noalloc instead carves them out
+// of `scratch`, a fixed array in this library's device image, one per GPU.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -32,6 +36,13 @@ List_T Pairpool_add_genomeskip(bool *add_dashes_p, List_T pairs, int r, int c, i
                                bool watsonp, int dynprogindex);
 int gdp_init_tables(const int16_t *pd, const int32_t *nt, const uint8_t *cons);
 }
+void gdp_set_error(const char *msg);  // libgmapdp: the message gdp_last_error() returns
+
+// Device buffers of Dynprog_simd_16_noalloc: one array per GPU, kept until
+// the process ends (GDP_NOALLOC_SCRATCH_BYTES, include/gmapdp_dynprog.h).
+// Host code uses the address hipGetSymbolAddress gives on each device, never
+// the array's name.
+static __device__ __align__(256) uint8_t scratch[GDP_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -72,17 +83,30 @@ void hipcheck(hipError_t e, const char *what) {
   }
 }
 
+// A device buffer: allocated and grown on demand, or (fixed) a part of the
+// scratch array that the allocation-free fill sized before use.
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  bool fixed = false;
   T *need(size_t want) {
     if (want > n) {
+      if (fixed) {  // sized by scratch_layout for this fill; never reached
+        fprintf(stderr, "gmapdp drop-in: scratch part of %zu elements asked for %zu\n", n, want);
+        abort();
+      }
       if (p) hipcheck(hipFree(p), "hipFree");
       hipcheck(hipMalloc((void **)&p, want * sizeof(T)), "hipMalloc");
       n = want;
     }
     return p;
+  }
+  void place(uint8_t *base, size_t &off, size_t count) {  // the next part of the scratch array
+    p = (T *)(base + off);
+    n = count;
+    fixed = true;
+    off += (count * sizeof(T) + 255) & ~(size_t)255;
   }
 };
 
@@ -106,6 +130,12 @@ struct Workspace {
 std::mutex g_mu;
 std::unordered_map<const void *, Workspace *> g_ws;
 bool g_tables = false;
+
+// The allocation-free fill's state per device: the scratch address on that
+// device and the slot of its last fill (its buffers point into scratch).
+constexpr int kMaxDevices = 64;
+uint8_t *g_scratch_ptr[kMaxDevices] = {nullptr};
+Slot g_static_slot[kMaxDevices];
 
 // The kernels use the host program's own tables (built by its Dynprog_init).
 void ensure_tables() {
@@ -137,8 +167,23 @@ Slot &slot_for_table(void **nogap, int kind) {
   std::lock_guard<std::mutex> lk(g_mu);
   for (auto &kv : g_ws)
     if (kv.second->slot[kind].nogap_table == nogap) return kv.second->slot[kind];
+  int dev = 0;
+  if (kind == GDP_KIND_FULL && hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kMaxDevices &&
+      g_static_slot[dev].nogap_table == nogap)
+    return g_static_slot[dev];
   fprintf(stderr, "gmapdp drop-in: traceback on direction tables this library did not produce\n");
   abort();
+}
+
+// A workspace's tables name the last fill written into them: forget any
+// other slot that handed out the same tables.
+void forget_table(void **nogap, const Slot *keep) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto &kv : g_ws)
+    for (Slot &s : kv.second->slot)
+      if (&s != keep && s.nogap_table == nogap) s.nogap_table = nullptr;
+  for (Slot &s : g_static_slot)
+    if (&s != keep && s.nogap_table == nogap) s.nogap_table = nullptr;
 }
 
 // Sequence in sweep order: reversed reads use s[1 - pos] (dynprog_end.c:1415).
@@ -149,15 +194,18 @@ void sweep_copy(std::vector<char> &out, const char *s, int n, bool revp) {
 
 // Runs one fill and copies it into the workspace spaces.  Returns the score
 // table; fills the direction tables.
-void **run_fill(Dynprog_T dynprog, int kind, int width, char *rseq, char *gseq, char *galt, int rl, int gl,
-                int mt, int open, int extend, int lband, int uband, bool late, bool revp, void ***nogap, void ***egap,
-                void ***fgap) {
-  RefDynprog *ws = (RefDynprog *)dynprog;
-  Slot &sl = slot_of(dynprog, kind);
-  std::vector<char> q, g, ga;
-  sweep_copy(q, rseq, rl, revp);
-  sweep_copy(g, gseq, gl, revp);
-  sweep_copy(ga, galt, gl, revp);
+// The reference layout of a fill's tables: the inner stride rounded up to
+// the SIMD width, as dynprog_simd.c:3044, 6618, 8626 round it.
+size_t ref_table_bytes(int kind, int width, int rl, int gl, int *stride_out) {
+  const int S = width == 8 ? 32 : 16;
+  const int inner = kind == GDP_KIND_LOWER ? gl : rl, outer = kind == GDP_KIND_LOWER ? rl : gl;
+  const int stride = ((inner + S) / S) * S;
+  if (stride_out) *stride_out = stride;
+  return (size_t)(outer + 1) * stride * (width / 8);
+}
+
+gdp_problem make_problem(int kind, int width, int rl, int gl, int mt, int open, int extend, int lband, int uband,
+                         bool late) {
   gdp_problem p;
   memset(&p, 0, sizeof p);
   p.qoff = 0;
@@ -174,6 +222,18 @@ void **run_fill(Dynprog_T dynprog, int kind, int width, char *rseq, char *gseq, 
   p.open = (int8_t)open;
   p.extend = (int8_t)extend;
   p.keep_scores = 1;
+  return p;
+}
+
+void **run_fill(Slot &sl, Dynprog_T dynprog, int kind, int width, char *rseq, char *gseq, char *galt, int rl,
+                int gl, int mt, int open, int extend, int lband, int uband, bool late, bool revp, void ***nogap,
+                void ***egap, void ***fgap) {
+  RefDynprog *ws = (RefDynprog *)dynprog;
+  std::vector<char> q, g, ga;
+  sweep_copy(q, rseq, rl, revp);
+  sweep_copy(g, gseq, gl, revp);
+  sweep_copy(ga, galt, gl, revp);
+  const gdp_problem p = make_problem(kind, width, rl, gl, mt, open, extend, lband, uband, late);
   std::vector<uint8_t> desc((size_t)gdp_plan_bytes(1));
   int64_t dw = 0, ss = 0;
   if (gdp_plan(&p, 1, desc.data(), &dw, &ss) != 0) die("gdp_plan");
@@ -189,10 +249,9 @@ void **run_fill(Dynprog_T dynprog, int kind, int width, char *rseq, char *gseq, 
 
   // Reference layout: outer index (columns of the reference's tables) and the
   // SIMD-rounded inner stride (dynprog_simd.c:3044, 6618, 8626).
-  const int S = width == 8 ? 32 : 16;
-  const int inner = kind == GDP_KIND_LOWER ? gl : rl, outer = kind == GDP_KIND_LOWER ? rl : gl;
-  const int stride = ((inner + S) / S) * S;
-  const size_t elems = (size_t)(outer + 1) * stride, bytes = elems * (width / 8);
+  const int outer = kind == GDP_KIND_LOWER ? rl : gl;
+  int stride = 0;
+  const size_t bytes = ref_table_bytes(kind, width, rl, gl, &stride);
   void *e[4];
   for (int k = 0; k < 4; k++) e[k] = sl.exp[k].need(bytes);
   if (gdp_export_reference(sl.desc.p, 0, sl.dirs.p, sl.scores.p, stride, e[0], e[1], e[2],
@@ -234,6 +293,7 @@ void **run_fill(Dynprog_T dynprog, int kind, int width, char *rseq, char *gseq, 
     char *base = (char *)space[k];
     for (int o = 0; o <= outer; o++) ptrs[k][o] = base + (size_t)o * stride * (width / 8);
   }
+  forget_table(ptrs[1], &sl);
   sl.nogap_table = ptrs[1];
   sl.rl = rl;
   sl.gl = gl;
@@ -273,7 +333,7 @@ List_T run_traceback(int kind, List_T pairs, int *traceback_score, int *nmatches
   rq.genestrand = (int8_t)genestrand;
   rq.flags = GDP_TB_OPS;
   rq.pair_off = 0;
-  rq.max_pairs = sl.rl + sl.gl + 4;
+  rq.max_pairs = sl.rl + sl.gl + 4;  // the scratch layout reserves as many
   hipcheck(hipMemcpy(sl.req.need(1), &rq, sizeof rq, hipMemcpyHostToDevice), "upload request");
   if (gdp_traceback(sl.desc.p, sl.req.p, 1, (const char *)sl.q.p, (const char *)sl.g.p, sl.dirs.p, nullptr, 0,
                     sl.ops.need(rq.max_pairs), sl.counts.need(6), nullptr) != 0)
@@ -319,7 +379,7 @@ Score8_T **Dynprog_simd_8(Direction8_T ***directions_nogap, Direction8_T ***dire
                           Direction8_T ***directions_Fgap, Dynprog_T dynprog, char *rsequence, char *gsequence,
                           char *gsequence_alt, int rlength, int glength, Mismatchtype_T mismatchtype, int open,
                           int extend, int lband, int uband, bool jump_late_p, bool revp) {
-  return (Score8_T **)run_fill(dynprog, GDP_KIND_FULL, 8, rsequence, gsequence, gsequence_alt, rlength, glength,
+  return (Score8_T **)run_fill(slot_of(dynprog, GDP_KIND_FULL), dynprog, GDP_KIND_FULL, 8, rsequence, gsequence, gsequence_alt, rlength, glength,
                                mismatchtype, open, extend, lband, uband, jump_late_p, revp,
                                (void ***)directions_nogap, (void ***)directions_Egap, (void ***)directions_Fgap);
 }
@@ -328,7 +388,75 @@ Score16_T **Dynprog_simd_16(Direction16_T ***directions_nogap, Direction16_T ***
                             Direction16_T ***directions_Fgap, Dynprog_T dynprog, char *rsequence, char *gsequence,
                             char *gsequence_alt, int rlength, int glength, Mismatchtype_T mismatchtype, int open,
                             int extend, int lband, int uband, bool jump_late_p, bool revp) {
-  return (Score16_T **)run_fill(dynprog, GDP_KIND_FULL, 16, rsequence, gsequence, gsequence_alt, rlength, glength,
+  return (Score16_T **)run_fill(slot_of(dynprog, GDP_KIND_FULL), dynprog, GDP_KIND_FULL, 16, rsequence, gsequence, gsequence_alt, rlength, glength,
+                                mismatchtype, open, extend, lband, uband, jump_late_p, revp,
+                                (void ***)directions_nogap, (void ***)directions_Egap, (void ***)directions_Fgap);
+}
+
+Score16_T **Dynprog_simd_16_noalloc(Direction16_T ***directions_nogap, Direction16_T ***directions_Egap,
+                                    Direction16_T ***directions_Fgap, Dynprog_T dynprog, char *rsequence,
+                                    char *gsequence, char *gsequence_alt, int rlength, int glength,
+                                    Mismatchtype_T mismatchtype, int open, int extend, int lband, int uband,
+                                    bool jump_late_p, bool revp) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    gdp_set_error("Dynprog_simd_16_noalloc: no current HIP device");
+    return nullptr;
+  }
+  uint8_t *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ensure_tables();
+    if (g_scratch_ptr[dev] == nullptr) {
+      // once per device, with that device current: the array's address on it
+      void *ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || ptr == nullptr) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "Dynprog_simd_16_noalloc: hipGetSymbolAddress(scratch) failed: %s",
+                 hipGetErrorString(e));
+        gdp_set_error(msg);
+        return nullptr;
+      }
+      g_scratch_ptr[dev] = (uint8_t *)ptr;
+    }
+    base = g_scratch_ptr[dev];
+  }
+  // The fill's buffer sizes, from the same plan run_fill makes.
+  const gdp_problem p = make_problem(GDP_KIND_FULL, 16, rlength, glength, mismatchtype, open, extend, lband, uband,
+                                     jump_late_p);
+  if (rlength < 1 || glength < 1 || rlength > GDP_MAX_LEN || glength > GDP_MAX_LEN) {
+    gdp_set_error("Dynprog_simd_16_noalloc: lengths outside [1, GDP_MAX_LEN]");
+    return nullptr;
+  }
+  std::vector<uint8_t> desc((size_t)gdp_plan_bytes(1));
+  int64_t dw = 0, ss = 0;
+  if (gdp_plan(&p, 1, desc.data(), &dw, &ss) != 0) return nullptr;  // gdp_plan set the message
+  const size_t table = ref_table_bytes(GDP_KIND_FULL, 16, rlength, glength, nullptr);
+  const size_t pairs = (size_t)rlength + glength + 4;
+  Slot &sl = g_static_slot[dev];
+  size_t off = 0;
+  sl.desc.place(base, off, desc.size());
+  sl.q.place(base, off, (size_t)rlength + 16);
+  sl.g.place(base, off, 2 * (size_t)glength + 16);
+  sl.dirs.place(base, off, (size_t)dw);
+  sl.scores.place(base, off, (size_t)(ss > 0 ? ss : 1));
+  sl.fin.place(base, off, 1);
+  for (int k = 0; k < 4; k++) sl.exp[k].place(base, off, table);
+  sl.counts.place(base, off, 6);
+  sl.ops.place(base, off, pairs);
+  sl.req.place(base, off, 1);
+  if (off > GDP_NOALLOC_SCRATCH_BYTES) {
+    char msg[200];
+    snprintf(msg, sizeof msg,
+             "Dynprog_simd_16_noalloc: a %d x %d fill with band %d+%d needs %zu bytes of device buffers, more than "
+             "the %u of the scratch array",
+             rlength, glength, lband, uband, off, GDP_NOALLOC_SCRATCH_BYTES);
+    gdp_set_error(msg);
+    sl.nogap_table = nullptr;
+    return nullptr;
+  }
+  return (Score16_T **)run_fill(sl, dynprog, GDP_KIND_FULL, 16, rsequence, gsequence, gsequence_alt, rlength, glength,
                                 mismatchtype, open, extend, lband, uband, jump_late_p, revp,
                                 (void ***)directions_nogap, (void ***)directions_Egap, (void ***)directions_Fgap);
 }
@@ -337,7 +465,7 @@ Score8_T **Dynprog_simd_8_upper(Direction8_T ***directions_nogap, Direction8_T *
                                 char *rsequence, char *gsequence, char *gsequence_alt, int rlength, int glength,
                                 Mismatchtype_T mismatchtype, int open, int extend, int uband, bool jump_late_p,
                                 bool revp) {
-  return (Score8_T **)run_fill(dynprog, GDP_KIND_UPPER, 8, rsequence, gsequence, gsequence_alt, rlength, glength,
+  return (Score8_T **)run_fill(slot_of(dynprog, GDP_KIND_UPPER), dynprog, GDP_KIND_UPPER, 8, rsequence, gsequence, gsequence_alt, rlength, glength,
                                mismatchtype, open, extend, 0, uband, jump_late_p, revp, (void ***)directions_nogap,
                                (void ***)directions_Egap, nullptr);
 }
@@ -346,7 +474,7 @@ Score16_T **Dynprog_simd_16_upper(Direction16_T ***directions_nogap, Direction16
                                   Dynprog_T dynprog, char *rsequence, char *gsequence, char *gsequence_alt,
                                   int rlength, int glength, Mismatchtype_T mismatchtype, int open, int extend,
                                   int uband, bool jump_late_p, bool revp) {
-  return (Score16_T **)run_fill(dynprog, GDP_KIND_UPPER, 16, rsequence, gsequence, gsequence_alt, rlength, glength,
+  return (Score16_T **)run_fill(slot_of(dynprog, GDP_KIND_UPPER), dynprog, GDP_KIND_UPPER, 16, rsequence, gsequence, gsequence_alt, rlength, glength,
                                 mismatchtype, open, extend, 0, uband, jump_late_p, revp, (void ***)directions_nogap,
                                 (void ***)directions_Egap, nullptr);
 }
@@ -355,7 +483,7 @@ Score8_T **Dynprog_simd_8_lower(Direction8_T ***directions_nogap, Direction8_T *
                                 char *rsequence, char *gsequence, char *gsequence_alt, int rlength, int glength,
                                 Mismatchtype_T mismatchtype, int open, int extend, int lband, bool jump_late_p,
                                 bool revp) {
-  return (Score8_T **)run_fill(dynprog, GDP_KIND_LOWER, 8, rsequence, gsequence, gsequence_alt, rlength, glength,
+  return (Score8_T **)run_fill(slot_of(dynprog, GDP_KIND_LOWER), dynprog, GDP_KIND_LOWER, 8, rsequence, gsequence, gsequence_alt, rlength, glength,
                                mismatchtype, open, extend, lband, 0, jump_late_p, revp, (void ***)directions_nogap,
                                (void ***)directions_Egap, nullptr);
 }
@@ -364,7 +492,7 @@ Score16_T **Dynprog_simd_16_lower(Direction16_T ***directions_nogap, Direction16
                                   Dynprog_T dynprog, char *rsequence, char *gsequence, char *gsequence_alt,
                                   int rlength, int glength, Mismatchtype_T mismatchtype, int open, int extend,
                                   int lband, bool jump_late_p, bool revp) {
-  return (Score16_T **)run_fill(dynprog, GDP_KIND_LOWER, 16, rsequence, gsequence, gsequence_alt, rlength, glength,
+  return (Score16_T **)run_fill(slot_of(dynprog, GDP_KIND_LOWER), dynprog, GDP_KIND_LOWER, 16, rsequence, gsequence, gsequence_alt, rlength, glength,
                                 mismatchtype, open, extend, lband, 0, jump_late_p, revp, (void ***)directions_nogap,
                                 (void ***)directions_Egap, nullptr);
 }

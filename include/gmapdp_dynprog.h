@@ -44,6 +44,29 @@ typedef struct gdp_ref_pairpool *Pairpool_T;
 typedef struct gdp_ref_genome *Genome_T;
 typedef unsigned int Univcoord_T;
 
+/* Dynprog_simd_16 without device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Same arguments, result and tables as Dynprog_simd_16, and
+   Dynprog_traceback_16 walks its tables as it walks Dynprog_simd_16's; but
+   the fill's device buffers (descriptor, sequences, direction words, scores,
+   the exported tables and the traceback's requests and pairs) are parts of
+   one fixed array of GDP_NOALLOC_SCRATCH_BYTES in the library's device
+   image, one per GPU, held until the process ends; the call makes no
+   hipMalloc or hipFree.  A problem whose buffers do not fit (for example rl
+   and gl of 1000 with any band fit) gets NULL, with the reason in
+   gdp_last_error() (include/gmapdp.h).  Every call on one GPU shares that
+   GPU's array: a fill's tables may be walked until the next call of this
+   function on the same GPU, and calls that overlap on one GPU (threads
+   sharing a device) race; serialize them.  Nothing in the array needs to
+   start at zero (every byte read is written by the same call first). */
+#define GDP_NOALLOC_SCRATCH_BYTES (16u << 20)
+Score16_T **Dynprog_simd_16_noalloc(Direction16_T ***directions_nogap, Direction16_T ***directions_Egap,
+                                    Direction16_T ***directions_Fgap, Dynprog_T dynprog, char *rsequence,
+                                    char *gsequence, char *gsequence_alt, int rlength, int glength,
+                                    Mismatchtype_T mismatchtype, int open, int extend, int lband, int uband,
+                                    bool jump_late_p, bool revp);
+
 #ifdef __cplusplus
 }
 #endif
@@ -120,6 +143,29 @@ List_T Dynprog_traceback_16_lower(List_T pairs, int *traceback_score, int *nmatc
                                   int r, int c, char *rsequence, char *rsequenceuc, char *gsequence,
                                   char *gsequence_alt, int queryoffset, int genomeoffset, Pairpool_T pairpool,
                                   int genestrand, bool revp, int dynprogindex);
+
+/* Dynprog_simd_16 without device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Same arguments, result and tables as Dynprog_simd_16, and
+   Dynprog_traceback_16 walks its tables as it walks Dynprog_simd_16's; but
+   the fill's device buffers (descriptor, sequences, direction words, scores,
+   the exported tables and the traceback's requests and pairs) are parts of
+   one fixed array of GDP_NOALLOC_SCRATCH_BYTES in the library's device
+   image, one per GPU, held until the process ends; the call makes no
+   hipMalloc or hipFree.  A problem whose buffers do not fit (for example rl
+   and gl of 1000 with any band fit) gets NULL, with the reason in
+   gdp_last_error() (include/gmapdp.h).  Every call on one GPU shares that
+   GPU's array: a fill's tables may be walked until the next call of this
+   function on the same GPU, and calls that overlap on one GPU (threads
+   sharing a device) race; serialize them.  Nothing in the array needs to
+   start at zero (every byte read is written by the same call first). */
+#define GDP_NOALLOC_SCRATCH_BYTES (16u << 20)
+Score16_T **Dynprog_simd_16_noalloc(Direction16_T ***directions_nogap, Direction16_T ***directions_Egap,
+                                    Direction16_T ***directions_Fgap, Dynprog_T dynprog, char *rsequence,
+                                    char *gsequence, char *gsequence_alt, int rlength, int glength,
+                                    Mismatchtype_T mismatchtype, int open, int extend, int lband, int uband,
+                                    bool jump_late_p, bool revp);
 
 #ifdef __cplusplus
 }
