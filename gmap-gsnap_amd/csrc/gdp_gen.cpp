@@ -333,7 +333,9 @@ char compl_base(char c) {
 extern "C" {
 
 // Serving requests [first, first+count) over a genomecomp genome of nbases
-// bases taken as one chromosome (chroffset 0, chrhigh nbases - 1).  Lengths,
+// bases cut into chromosomes of 2^27 bases (one chromosome below 2^29
+// bases): each request's chroffset and chrhigh are its chromosome's first
+// and last base and goffset its window's offset in it.  Lengths,
 // band and divergence follow config 3; the strand is drawn per request and
 // the read is taken from the segment the server will extract for it
 // (the reverse complement on the minus strand).  Read bytes go to
@@ -341,6 +343,9 @@ extern "C" {
 int gdp_gen_serve_requests(uint64_t seed, int64_t first, int64_t count, const uint32_t *gc, uint64_t nbases,
                            gdp_align_request *out, char *reads) {
   if (nbases < 4096) return -1;
+  // chromosomes of 2^27 bases (134 Mbp, about a human chromosome's mean),
+  // or one chromosome for a genome of fewer than four of them
+  const uint64_t kChrom = nbases >= (4ull << 27) ? (1ull << 27) : nbases;
   const int stride = 1008;
   const int nthreads = 16;
   std::vector<std::thread> th;
@@ -355,19 +360,33 @@ int gdp_gen_serve_requests(uint64_t seed, int64_t first, int64_t count, const ui
         const int eb = rng.range(3, 64);
         const double div = rng.uni() < 0.10 ? 0.30 : 0.02;
         const bool watson = (rng.next() & 1) != 0;
-        // reads come from sequenced genome: draw again while the segment
-        // touches a run of N (at most 16 draws)
-        const uint64_t chrhigh = nbases - 1;
+        // The genome is cut into chromosomes of kChrom bases (the last one
+        // shorter); a request picks a position uniformly over the genome and
+        // aligns within that position's chromosome: chroffset is the
+        // chromosome's first base, chrhigh its last, goffset the offset of
+        // the window in it (GMAP's convention: an offset within one
+        // chromosome, below 2^31 however long the genome).  Reads come from
+        // sequenced genome: draw again while the segment touches a run of N
+        // (at most 16 draws).
+        uint64_t chroffset = 0, chrhigh = 0;
         int64_t goffset = 0;
         for (int tries = 0; tries < 16; tries++) {
-          goffset = 8 + (int64_t)(rng.uni() * (double)(nbases - (uint64_t)gl - 32));
-          const uint64_t lo = watson ? (uint64_t)goffset : chrhigh - (uint64_t)goffset + 1 - (uint64_t)gl;
+          uint64_t chrom = (uint64_t)(rng.uni() * (double)nbases) / kChrom;
+          chroffset = chrom * kChrom;
+          uint64_t chrlen = std::min<uint64_t>(kChrom, nbases - chroffset);
+          if (chrlen < (uint64_t)gl + 64) {  // a stub at the genome's end: take the chromosome before it
+            chroffset -= kChrom;
+            chrlen = kChrom;
+          }
+          chrhigh = chroffset + chrlen - 1;
+          goffset = 8 + (int64_t)(rng.uni() * (double)(chrlen - (uint64_t)gl - 32));
+          const uint64_t lo = watson ? chroffset + (uint64_t)goffset : chrhigh - (uint64_t)goffset + 1 - (uint64_t)gl;
           bool clean = true;
           for (uint64_t blk = lo / 32; blk <= (lo + (uint64_t)gl - 1) / 32 && clean; blk++) clean = gc[3 * blk + 2] == 0;
           if (clean) break;
         }
         if (watson) {
-          for (int j = 0; j < gl; j++) seg[j] = gc_base(gc, nbases, (uint64_t)goffset + j);
+          for (int j = 0; j < gl; j++) seg[j] = gc_base(gc, nbases, chroffset + (uint64_t)goffset + j);
         } else {
           const uint64_t right = chrhigh - (uint64_t)goffset + 1;
           for (int j = 0; j < gl; j++) seg[j] = compl_base(gc_base(gc, nbases, right - 1 - (uint64_t)j));
@@ -386,7 +405,7 @@ int gdp_gen_serve_requests(uint64_t seed, int64_t first, int64_t count, const ui
         r.gl = gl;
         r.roffset = 0;
         r.goffset = (int32_t)goffset;
-        r.chroffset = 0;
+        r.chroffset = (uint32_t)chroffset;
         r.chrhigh = (uint32_t)chrhigh;
         r.extraband = eb;
         r.watsonp = watson ? 1 : 0;

@@ -409,6 +409,9 @@ static int complement_char(int ch) {
 
 static int genome_char(const orc_genome_t *genome, unsigned int pos) {
   if (genome == NULL || pos >= genome->length) return 'N';
+  const unsigned int count = genome->count ? genome->count : genome->length;
+  if (pos < genome->base || pos - genome->base >= count) return '?';  /* outside the arrays given */
+  pos -= genome->base;
   if (genome->nflags && genome->nflags[pos]) return 'N';
   return "ACGT"[genome->codes[pos] & 3];
 }

@@ -76,11 +76,16 @@ void orc_fill_std(const char *rseq, const char *gseq, const char *gseq_alt, int 
 
 /* Genome model used for genome-skip characters in the tracebacks
    (pairpool.c:1016-1063 get_genomic_nt over Genome_get_char genome.c:11003).
-   codes: 2-bit A=0,C=1,G=2,T=3 one byte per base; nflags: nonzero => 'N'. */
+   codes: 2-bit A=0,C=1,G=2,T=3 one byte per base; nflags: nonzero => 'N'.
+   length is the genome's length (positions at or past it read 'N'); the
+   arrays hold positions base .. base + count - 1 (count 0: all of them from
+   base 0), and any other position reads '?', which no check accepts. */
 typedef struct {
   const unsigned char *codes;
   const unsigned char *nflags;
   unsigned int length;
+  unsigned int base;
+  unsigned int count;
 } orc_genome_t;
 
 /* Tracebacks: Dynprog_traceback_{8,16} (dynprog_simd.c:9153-9312, 9552-9711),

@@ -56,6 +56,21 @@ def decode(gc, nbases):
     return codes, nflags
 
 
+def decode_window(gc, nbases, start, count):
+    """The oracle's genome model of positions start .. start + count - 1 of
+    a genome of nbases bases (clipped to the genome): (codes, nflags, base,
+    nbases) for pyoracle, without decoding the rest."""
+    start = max(0, min(int(start), nbases))
+    end = max(start, min(int(start) + int(count), nbases))
+    pos = np.arange(start, end, dtype=np.int64)
+    blk = (pos // 32) * 3
+    k = (pos % 32).astype(np.uint32)
+    word = np.where(k < 16, gc[blk + 1], gc[blk])
+    codes = ((word >> (2 * (k & 15))) & 3).astype(np.uint8)
+    nflags = ((gc[blk + 2] >> k) & 1).astype(np.uint8)
+    return codes, nflags, start, nbases
+
+
 def to_ascii(codes, nflags):
     ch = np.frombuffer(b"ACGT", np.uint8)[codes]
     return np.where(nflags != 0, ord("N"), ch).astype(np.uint8)

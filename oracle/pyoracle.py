@@ -185,14 +185,20 @@ class Oracle(_Common):
         return out[:n].copy(), cnt
 
     def _genome_struct(self, genome):
-        codes, nflags = genome
+        """genome: (codes, nflags) of a whole genome, or (codes, nflags,
+        base, length) of positions base .. base + len(codes) - 1 of a genome
+        of `length` bases (genomecomp.decode_window)."""
+        if len(genome) == 2:
+            (codes, nflags), base, length = genome, 0, len(genome[0])
+        else:
+            codes, nflags, base, length = genome
 
         class G(ctypes.Structure):
-            _fields_ = [("codes", _p), ("nflags", _p), ("length", _u)]
+            _fields_ = [("codes", _p), ("nflags", _p), ("length", _u), ("base", _u), ("count", _u)]
 
         codes = np.ascontiguousarray(codes, np.uint8)
         nflags = np.ascontiguousarray(nflags, np.uint8)
-        st = G(codes.ctypes.data, nflags.ctypes.data, len(codes))
+        st = G(codes.ctypes.data, nflags.ctypes.data, int(length), int(base), len(codes))
         self._genome_keep = (codes, nflags, st)
         return ctypes.addressof(st)
 
@@ -262,6 +268,8 @@ class Reference(_Common):
         L.refh_traceback.argtypes = [_i, _i, _i, _i, _p, _p, _p, _p, _i, _i, _i, _u, _u, _i, _i, _i,
                                      _p, _p, _i]
         L.refh_traceback.restype = _i
+        L.refh_traceback_genome.argtypes = [_p, _u] + list(L.refh_traceback.argtypes)
+        L.refh_traceback_genome.restype = _i
         L.refc_bridge_cdna.argtypes = [_i, _p, _p, _p, _p] + [_i] * 12 + [_p]
         L.refe_best_endpoint.argtypes = [_i, _i, _p, _p, _i, _i, _i, _i, _i, _p]
         L.refe_traceback_nogaps.argtypes = [_i, _i, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p, _i]
@@ -379,13 +387,24 @@ class Reference(_Common):
 
     def traceback(self, kind, dirs, r, c, rseq, rsequc, gseq, galt, queryoffset=0, genomeoffset=0,
                   revp=False, chroffset=0, chrhigh=0, watsonp=True, genestrand=0, genome=None,
-                  maxpairs=8192):
-        """Traceback over the *last* fill of this kind (dirs is ignored)."""
-        assert genome is None, "the reference harness has no genome; use chrhigh=0"
+                  maxpairs=8192, genomecomp=None):
+        """Traceback over the *last* fill of this kind (dirs is ignored).
+        genomecomp = (image, nbases): genome skips read that genomecomp
+        image through the reference's Genome_get_char; without it every
+        genome-skip character is '*' (chrhigh = 0)."""
+        assert genome is None, "the reference reads a genome as genomecomp=(image, nbases)"
         q, g, ga = self._seqs(rseq, gseq, galt, revp)
         quc = _Seq(rsequc, revp)
         out = np.zeros((maxpairs, PAIR_FIELDS), np.int32)
         cnt = np.zeros(5, np.int32)
+        if genomecomp is not None:
+            image = np.ascontiguousarray(genomecomp[0], np.uint32)
+            n = self.lib.refh_traceback_genome(_np_ptr(image), ctypes.c_uint(int(genomecomp[1])), kind,
+                                               self._last[kind], r, c, q.ptr, quc.ptr, g.ptr, ga.ptr, queryoffset,
+                                               genomeoffset, int(revp), ctypes.c_uint(chroffset),
+                                               ctypes.c_uint(chrhigh), int(watsonp), genestrand, 0, _np_ptr(cnt),
+                                               _np_ptr(out), maxpairs)
+            return out[:n].copy(), cnt
         n = self.lib.refh_traceback(kind, self._last[kind], r, c, q.ptr, quc.ptr, g.ptr, ga.ptr,
                                     queryoffset, genomeoffset, int(revp), chroffset, chrhigh,
                                     int(watsonp), genestrand, 0, _np_ptr(cnt), _np_ptr(out), maxpairs)

@@ -6,14 +6,17 @@
  * except.c, assert.c) read in place from the reference tree, into
  * oracle/_ref/libgmapref.so.  It exposes the same plain-array interface as
  * gdp_oracle.h so tests can compare the CPU restatement with the reference
- * cell for cell.  Genome_get_char is never reached: callers pass chrhigh = 0,
- * which makes every genome-skip character '*' (pairpool.c:1037-1040).
+ * cell for cell.  refh_traceback passes no genome (callers pass chrhigh = 0,
+ * which makes every genome-skip character '*', pairpool.c:1037-1040);
+ * refh_traceback_genome passes a Genome_T over a genomecomp image, read by
+ * the reference's own Genome_get_char (genome.c, compiled in place too).
  */
 #include <stdlib.h>
 #include <string.h>
 
 #include "dynprog.h"
 #include "dynprog_simd.h"
+#include "genome.h"
 #include "list.h"
 #include "pairdef.h"
 #include "pairpool.h"
@@ -153,10 +156,10 @@ void refh_fill_lower(void *ws, int width, char *rseq, char *gseq, char *galt, in
    8 ints per pair {querypos, genomepos, cdna, comp, genome, genomealt, gapp,
    genomejump}, gapp 2 for a known-site gap holder (pair->knowngapp).  counters[5] receive {score, nmatches, nmismatches, nopens,
    nindels}.  Returns the number of pairs. */
-int refh_traceback(int kind, int width, int r, int c, char *rseq, char *rsequc, char *gseq, char *galt,
-                   int queryoffset, int genomeoffset, int revp, unsigned int chroffset,
-                   unsigned int chrhigh, int watsonp, int genestrand, int dynprogindex,
-                   int *counters, int *out, int maxpairs) {
+static int traceback_with(Genome_T genome, int kind, int width, int r, int c, char *rseq, char *rsequc, char *gseq,
+                          char *galt, int queryoffset, int genomeoffset, int revp, unsigned int chroffset,
+                          unsigned int chrhigh, int watsonp, int genestrand, int dynprogindex, int *counters, int *out,
+                          int maxpairs) {
   List_T pairs = NULL, p;
   int sc = 0, nm = 0, nmm = 0, no = 0, ni = 0;
   Pairpool_reset(pool);
@@ -164,23 +167,23 @@ int refh_traceback(int kind, int width, int r, int c, char *rseq, char *rsequc, 
     if (width == 8)
       pairs = Dynprog_traceback_8(NULL, &sc, &nm, &nmm, &no, &ni, (Direction8_T **)last_nogap[0],
                                   (Direction8_T **)last_Egap[0], (Direction8_T **)last_Fgap[0], r, c, rseq, rsequc,
-                                  gseq, galt, queryoffset, genomeoffset, NULL, NULL, pool, revp ? true : false,
+                                  gseq, galt, queryoffset, genomeoffset, genome, genome, pool, revp ? true : false,
                                   chroffset, chrhigh, watsonp ? true : false, genestrand, dynprogindex);
     else
       pairs = Dynprog_traceback_16(NULL, &sc, &nm, &nmm, &no, &ni, (Direction16_T **)last_nogap[0],
                                    (Direction16_T **)last_Egap[0], (Direction16_T **)last_Fgap[0], r, c, rseq, rsequc,
-                                   gseq, galt, queryoffset, genomeoffset, NULL, NULL, pool, revp ? true : false,
+                                   gseq, galt, queryoffset, genomeoffset, genome, genome, pool, revp ? true : false,
                                    chroffset, chrhigh, watsonp ? true : false, genestrand, dynprogindex);
   } else if (kind == 1) {
     if (width == 8)
       pairs = Dynprog_traceback_8_upper(NULL, &sc, &nm, &nmm, &no, &ni, (Direction8_T **)last_nogap[1],
                                         (Direction8_T **)last_Egap[1], r, c, rseq, rsequc, gseq, galt, queryoffset,
-                                        genomeoffset, NULL, NULL, pool, revp ? true : false, chroffset, chrhigh,
+                                        genomeoffset, genome, genome, pool, revp ? true : false, chroffset, chrhigh,
                                         watsonp ? true : false, genestrand, dynprogindex);
     else
       pairs = Dynprog_traceback_16_upper(NULL, &sc, &nm, &nmm, &no, &ni, (Direction16_T **)last_nogap[1],
                                          (Direction16_T **)last_Egap[1], r, c, rseq, rsequc, gseq, galt, queryoffset,
-                                         genomeoffset, NULL, NULL, pool, revp ? true : false, chroffset, chrhigh,
+                                         genomeoffset, genome, genome, pool, revp ? true : false, chroffset, chrhigh,
                                          watsonp ? true : false, genestrand, dynprogindex);
   } else {
     if (width == 8)
@@ -204,6 +207,31 @@ int refh_traceback(int kind, int width, int r, int c, char *rseq, char *rsequc, 
     o[4] = pr->genome; o[5] = pr->genomealt; o[6] = pr->gapp ? (pr->knowngapp ? 2 : 1) : 0; o[7] = pr->genomejump;
   }
   return n;
+}
+
+int refh_traceback(int kind, int width, int r, int c, char *rseq, char *rsequc, char *gseq, char *galt,
+                   int queryoffset, int genomeoffset, int revp, unsigned int chroffset,
+                   unsigned int chrhigh, int watsonp, int genestrand, int dynprogindex,
+                   int *counters, int *out, int maxpairs) {
+  return traceback_with(NULL, kind, width, r, c, rseq, rsequc, gseq, galt, queryoffset, genomeoffset, revp,
+                        chroffset, chrhigh, watsonp, genestrand, dynprogindex, counters, out, maxpairs);
+}
+
+/* As refh_traceback, with genome skips read from a genome: a Genome_T whose
+   blocks are the genomecomp image `blocks` of `length` bases (the genome and
+   its alternate are the same image), through the reference's own
+   Pairpool_add_genomeskip -> get_genomic_nt -> Genome_get_char
+   (pairpool.c:1016-1180, genome.c:11003-11017). */
+int refh_traceback_genome(const unsigned int *blocks, unsigned int length, int kind, int width, int r, int c,
+                          char *rseq, char *rsequc, char *gseq, char *galt, int queryoffset, int genomeoffset,
+                          int revp, unsigned int chroffset, unsigned int chrhigh, int watsonp, int genestrand,
+                          int dynprogindex, int *counters, int *out, int maxpairs) {
+  struct Genome_T g;
+  memset(&g, 0, sizeof g);
+  g.blocks = (Genomecomp_T *)blocks;
+  g.genomelength = length;
+  return traceback_with(&g, kind, width, r, c, rseq, rsequc, gseq, galt, queryoffset, genomeoffset, revp, chroffset,
+                        chrhigh, watsonp, genestrand, dynprogindex, counters, out, maxpairs);
 }
 
 /* The direction tables of the last fill of `kind` (1 upper, 2 lower), for

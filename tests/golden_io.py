@@ -109,3 +109,47 @@ def load_bridge():
                 f["open"], f["ext"], f["leftoffset"], f["rightoffset"], f["late"])
         cases.append((args, tuple(int(x) for x in out[i])))
     return cases
+
+
+SKIP_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "genome_skip_golden.npz")
+
+
+def skip_genome_image():
+    """The genome of the genome-skip vectors: the reference's test
+    chromosome with N runs (helpers.skip_genome), as ASCII and genomecomp."""
+    import gzip
+    import genomecomp as GC
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "chr17test.fa.gz")
+    fa = gzip.open(path).read().decode()
+    seq = H.skip_genome("".join(line.strip() for line in fa.splitlines() if not line.startswith(">")).encode())
+    return seq, GC.pack(seq)
+
+
+def load_genome_skips():
+    """tests/golden/genome_skip_golden.npz (make_genome_skip_golden.py):
+    (seq, genomecomp image, [(Spec, (genomeoffset, chroffset, chrhigh,
+    watsonp), pair records, counters)])."""
+    z = np.load(SKIP_PATH, allow_pickle=False)
+    g = {k: z[k] for k in z.files}
+    seq, gc = skip_genome_image()
+    assert len(seq) == int(g["genome_bases"])
+    assert hashlib.sha256(gc.tobytes()).digest() == g["genome_sha256"].tobytes(), "genome image differs"
+    cases = []
+    for i in range(len(g["kind"])):
+        qo, go, rl, gl = int(g["qoff"][i]), int(g["goff_buf"][i]), int(g["rl"][i]), int(g["gl"][i])
+        q, gg = bytes(g["qbuf"][qo:qo + rl]), bytes(g["gbuf"][go:go + gl])
+        s = H.Spec(int(g["kind"][i]), int(g["width"][i]), q, gg, gg, int(g["mt"][i]), int(g["open"][i]),
+                   int(g["ext"][i]), int(g["lband"][i]), int(g["uband"][i]), bool(g["late"][i]), bool(g["revp"][i]))
+        po_, n = int(g["pair_off"][i]), int(g["npairs"][i])
+        cases.append((s, (int(g["goff"][i]), int(g["chroffset"][i]), int(g["chrhigh"][i]), bool(g["watsonp"][i])),
+                      g["pairs"][po_:po_ + n], g["counters"][i]))
+    return seq, gc, cases
+
+
+def skip_kinds(pairs):
+    """Counts in one walk's records: skipped bases read from the genome
+    (an A/C/G/T/N character), read as outside the chromosome ('*'), and gap
+    holders."""
+    sk = pairs[(pairs[:, 2] == ord(" ")) & (pairs[:, 3] == ord("-"))]
+    star = int((sk[:, 4] == ord("*")).sum())
+    return len(sk) - star, star, int((pairs[:, 6] == 1).sum())

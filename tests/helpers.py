@@ -284,3 +284,57 @@ def random_splice_ends(rng, n, long_jumps=False):
                     "defect_rate": float(rng.choice([0.001, 0.01, 0.05])),
                     "contlength": int(rng.integers(0, min(rl, gl) + 2))})
     return out
+
+
+def skip_genome(seq):
+    """The reference's test chromosome with N runs put in (40 bases in every
+    seventh kilobase), so that genome skips also read flagged bases."""
+    a = np.frombuffer(bytes(seq), np.uint8).copy()
+    pos = np.arange(len(a))
+    a[((pos // 1000) % 7 == 3) & (pos % 1000 < 40)] = ord("N")
+    return a.tobytes()
+
+
+def genome_skip_cases(rng, n, nbases):
+    """Walks that end in a genome skip read from the genome
+    (Pairpool_add_genomeskip, pairpool.c:1067-1180): the read is the tail of
+    its genome window, so the walk from (rl, gl) reaches row 0 at column L,
+    1-8 (bases pushed one by one through get_genomic_nt) or 9-30 (a gap
+    holder).  Full-band and upper fills, both widths, both strands, revp,
+    and chromosome bounds placed so that some skipped bases fall outside
+    them.  Returns [(Spec, (genomeoffset, chroffset, chrhigh, watsonp))]."""
+    out = []
+    for t in range(n):
+        L = int(rng.integers(1, 9)) if t % 5 < 3 else int(rng.integers(9, 31))
+        rl = int(rng.integers(3, 60))
+        gl = rl + L
+        g = rand_seq(rng, gl)
+        q = g[L:]
+        if t % 4 == 1:  # a mismatch or two inside the read
+            q = bytes(ch if rng.random() > 0.04 else ALPH[rng.integers(0, 4)] for ch in q)
+        kind = 0 if t % 3 else 1
+        width = 8 if t % 2 else 16
+        eb = int(rng.integers(1, 6))
+        lb, ub = eb, L + eb
+        revp = t % 7 == 2
+        late = bool((t // 2) % 2)
+        s = Spec(kind, width, q, g, g, int(rng.integers(0, 3)), -8, -3, lb, ub, late, revp)
+        watsonp = bool(rng.integers(0, 2))
+        u = rng.random()
+        if u < 0.5:  # inside the chromosome
+            chroffset = int(rng.integers(0, nbases // 2))
+            chrhigh = int(rng.integers(chroffset + 200, nbases))
+            goff = int(rng.integers(40, chrhigh - chroffset - 40)) if watsonp else \
+                int(rng.integers(40, chrhigh - chroffset - 40))
+        elif u < 0.75:  # the skip crosses chrhigh (plus) or falls past it (minus)
+            chroffset = int(rng.integers(0, nbases // 2))
+            chrhigh = int(rng.integers(chroffset + 100, nbases))
+            goff = (chrhigh - chroffset) - int(rng.integers(0, L + 1)) if watsonp else int(rng.integers(-L, 2))
+        else:  # the skip crosses chroffset
+            chroffset = int(rng.integers(100, nbases // 2))
+            chrhigh = int(rng.integers(chroffset + 100, nbases))
+            goff = -int(rng.integers(0, L + 1)) if watsonp else (chrhigh - chroffset) - int(rng.integers(-L, 2))
+        if revp:
+            goff += gl  # the reversed walk's coordinates run down from genomeoffset
+        out.append((s, (goff, chroffset, chrhigh, watsonp)))
+    return out

@@ -245,3 +245,37 @@ def test_server_refuses_what_the_reference_refuses(small_genome):
         r, p = out[k]
         assert r["status"] == -2 and r["counts"][1] == -32768 and r["counts"][0] == 0 and len(p) == 0
     assert out[3][0]["status"] == 0
+
+
+def _window(gc, nbases, r):
+    """The oracle's genome model of the bases a request can read: its
+    segment and the genome skips of its walk, with a margin."""
+    gl, chroffset, chrhigh, goffset = int(r["gl"]), int(r["chroffset"]), int(r["chrhigh"]), int(r["goffset"])
+    start = chroffset + goffset if r["watsonp"] else chrhigh - goffset - gl
+    return GC.decode_window(gc, nbases, start - 64, gl + 128)
+
+
+def test_server_on_the_whole_genome(oracle):
+    """Config 5's genome: 3.1 Gbp of genomecomp on the device, cut into
+    chromosomes of 2^27 bases, so that requests carry chromosome offsets past
+    2^31.  300 requests served together, each checked against the oracle's
+    Dynprog_single_gap DP over the bases it reads (decoded window by
+    window)."""
+    from gmapdp import serve
+    nbases = 3_100_000_000
+    gc = serve.gen_genomecomp(1005, nbases)
+    reqs, reads = serve.gen_requests(1005, 0, 300, gc, nbases)
+    reqs["roffset"] = np.arange(300) % 11
+    high = reqs["chroffset"].astype(np.int64) >= 1 << 31
+    assert high.sum() >= 50, "too few requests past 2^31"
+    dg = serve.DeviceGenome(gc, nbases)
+    with serve.Server(dg, max_batch=128, slots=2) as srv:
+        srv.submit(reqs)
+        out = srv.collect(len(reqs))
+    del dg
+    skipped = 0
+    for r in reqs:
+        res, pairs = out[int(r["tag"])]
+        _check_result(oracle, _window(gc, nbases, r), nbases, r, _read(reads, r), res, pairs)
+        skipped += int((pairs["cdna"] == ord(" ")).sum())
+    assert skipped > 0, "no walk crossed a genome gap"

@@ -103,8 +103,15 @@ def test_plan_validates_and_sizes(L):
     bad["width"][3] = 12
     with pytest.raises(RuntimeError, match="width"):
         plan(bad)
+    # a band past the matrix is narrowed to it (no cell changes), so 700 +
+    # 700 around a small problem plans; spans past GDP_MAX_SPAN after
+    # narrowing do not
+    wide = probs.copy()
+    wide["lband"][0], wide["uband"][0], wide["kind"][0] = 700, 700, 0
+    plan(wide)
     bad = probs.copy()
-    bad["lband"][0], bad["uband"][0], bad["kind"][0] = 700, 700, 0
+    bad["lband"][0], bad["uband"][0], bad["kind"][0] = 6000, 6000, 0
+    bad["rl"][0], bad["gl"][0] = 5000, 5000
     with pytest.raises(RuntimeError, match="span"):
         plan(bad)
     bad = probs.copy()
@@ -140,6 +147,7 @@ def test_plan_validates_standard_fills(L):
             plan(bad)
     bad = probs.copy()
     bad["lband"][0], bad["uband"][0] = 264, 264
+    bad["rl"][0], bad["gl"][0] = 400, 400  # a span past 526 after narrowing to the matrix
     with pytest.raises(RuntimeError, match="span"):
         plan(bad)
     simd, _, _ = H.pack_batch(H.random_specs(13, 4))
@@ -197,7 +205,8 @@ def test_plan_lane_group_parameters():
     specs = H.random_specs(3, 200, kinds=(0,), widths=(16,))
     probs, _, _ = H.pack_batch(specs)
     rl = probs["rl"]
-    span = probs["lband"].astype(int) + probs["uband"]
+    # the span the planner sweeps: bands narrowed to one past the matrix
+    span = np.minimum(probs["lband"].astype(int), rl + 1) + np.minimum(probs["uband"].astype(int), probs["gl"] + 1)
     for params in (None, (240, 8001), (0, 8001), (64, 100), (240, 1)):
         descs, dw, ss = plan(probs, 0, params)
         d = descs[: len(probs) * 96].reshape(len(probs), 96)
