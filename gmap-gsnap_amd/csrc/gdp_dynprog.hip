@@ -326,8 +326,12 @@ List_T run_traceback(int kind, List_T pairs, int *traceback_score, int *nmatches
   rq.c = c;
   rq.queryoffset = queryoffset;
   rq.genomeoffset = genomeoffset;
-  rq.chroffset = chroffset;
-  rq.chrhigh = chrhigh;
+  // The walk records its pair-pool calls (GDP_TB_OPS): genome skips come
+  // back as calls that the host's Pairpool_add_genomeskip replays with its
+  // own genome and bounds, so the device never reads these (nor truncates a
+  // 64-bit Univcoord_T of a large-genome build).
+  rq.chroffset = 0;
+  rq.chrhigh = 0;
   rq.watsonp = watsonp ? 1 : 0;
   rq.revp = revp ? 1 : 0;
   rq.genestrand = (int8_t)genestrand;

@@ -42,7 +42,14 @@ typedef struct gdp_ref_dynprog *Dynprog_T;
 typedef struct gdp_ref_list *List_T;
 typedef struct gdp_ref_pairpool *Pairpool_T;
 typedef struct gdp_ref_genome *Genome_T;
+/* Univcoord_T is 64 bits in GMAP's large-genome builds (gmapl, LARGE_GENOMES,
+   univcoord.h:10-12): libgmapdp_dynprog_large.so is this library built with
+   GDP_LARGE_GENOMES for them. */
+#if defined(LARGE_GENOMES) || defined(GDP_LARGE_GENOMES)
+typedef unsigned long long Univcoord_T;
+#else
 typedef unsigned int Univcoord_T;
+#endif
 
 /* Dynprog_simd_16 without device allocation.  No counterpart in the
    reference: it is for callers that cannot allocate device memory (code

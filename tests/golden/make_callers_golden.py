@@ -1,0 +1,49 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/callers_golden.npz: calls of the reference's own
+callers Dynprog_single_gap (dynprog_single.c:428-675), Dynprog_end5_gap
+(dynprog_end.c:1293-1650) and Dynprog_end3_gap (1923-2243) over the
+reference's SIMD fills, with the genome-skip test genome (the reference's
+test chromosome with N runs) cut into two chromosomes.
+
+Run in a container where the reference tree is mounted, after
+`make -C oracle ref`, which builds oracle/_ref/libgmapref_callers.so from
+the reference sources in place.  The cases are helpers.caller_cases: 2000
+single gaps of the single-gap generator (HIGHQ int16 and LOWQ int8 fills,
+plus about 2000-column bands and ungapped windows) and 1000 end gaps of the
+end-extension generator under all four end modes.  Stored: the calls'
+arguments and reads, and the returned pair lists (8 ints per pair, from the
+list's head) and counters {dynprogindex, traceback_score, nmatches,
+nmismatches, nopens, nindels}; npairs -1 is a NULL list.
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "gmap-gsnap_amd")]
+
+import callers_run  # noqa: E402
+import golden_io  # noqa: E402
+import helpers as H  # noqa: E402
+
+OUT = os.path.join(HERE, "callers_golden.npz")
+SEED = 1001
+
+
+def main():
+    seq, _ = golden_io.skip_genome_image()
+    cases = H.caller_cases(np.random.default_rng(SEED), seq, 2000, 1000)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath = os.path.join(d, "cases.npz"), os.path.join(d, "out.npz")
+        callers_run.save_cases(cpath, cases)
+        callers_run.run("ref", cpath, opath)
+        z, o = np.load(cpath), np.load(opath)
+        np.savez_compressed(OUT, **{k: z[k] for k in z.files}, **{"out_" + k: o[k] for k in o.files})
+    print("wrote %s: %d calls" % (OUT, len(cases)))
+
+
+if __name__ == "__main__":
+    main()

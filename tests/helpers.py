@@ -338,3 +338,89 @@ def genome_skip_cases(rng, n, nbases):
             goff += gl  # the reversed walk's coordinates run down from genomeoffset
         out.append((s, (goff, chroffset, chrhigh, watsonp)))
     return out
+
+
+_COMPL = bytes.maketrans(b"ACGTNacgtn", b"TGCANtgcan")
+
+
+def revcomp(b):
+    return bytes(b).translate(_COMPL)[::-1]
+
+
+def caller_cases(rng, genome_seq, n_single, n_end, chromosomes=2):
+    """Calls of the reference's own callers over a genome: Dynprog_single_gap
+    with the single-gap generator (100-nt reads from 150-nt windows, 2 %
+    substitutions, 0.2 % indels of 1-3, extraband 10, widened bands, HIGHQ
+    and LOWQ defect rates, so int16 and int8 fills), a few windows of about
+    2000 bases against short reads (bands of about 2000 columns), and
+    windows as long as the read (the ungapped shortcut); Dynprog_end5_gap and
+    Dynprog_end3_gap with the end-extension generator (reads of 10-150
+    against windows 0-20 longer, 3 % substitutions, 1 % indels of 1-5, 1 % N
+    in the read, extraband 6) under all four end modes, a few windows past
+    the workspace's 2000 columns (chopped by the caller).  The genome is cut
+    into `chromosomes` equal chromosomes.  Each case is a dict of the call's
+    arguments; the read is taken from the segment the caller extracts."""
+    nb = len(genome_seq)
+    clen = nb // chromosomes
+    cases = []
+
+    def chrom():
+        k = int(rng.integers(0, chromosomes))
+        return k * clen, k * clen + clen - 1  # chroffset, chrhigh
+
+    def seg_right(left, length, chrhigh):
+        s = bytearray(genome_seq[left:left + length])
+        for j in range(len(s)):
+            if left + j >= chrhigh:
+                s[j] = ord("*")
+        return bytes(s)
+
+    for t in range(n_single + n_end):
+        single = t < n_single
+        watsonp = bool(rng.integers(0, 2))
+        chroffset, chrhigh = chrom()
+        if single:
+            u = rng.random()
+            if u < 0.06:
+                rl, gl = int(rng.integers(30, 60)), int(rng.integers(1900, 2001))
+            elif u < 0.12:
+                rl = int(rng.integers(20, 120))
+                gl = rl
+            else:
+                rl, gl = 100, int(rng.integers(100, 151))
+        else:
+            rl = int(rng.integers(10, 151))
+            gl = rl + int(rng.integers(0, 21)) if rng.random() > 0.05 else int(rng.integers(2001, 2100))
+        five = (not single) and t % 2 == 0
+        goffset = int(rng.integers(10 + (gl if five else 0), clen - gl - 10))
+        # the segment the caller extracts (dynprog_single.c:544-550,
+        # dynprog_end.c:1393-1399 for the 5' end, 1990-1996 for the 3' end)
+        if five:
+            left = chroffset + goffset + 1 - gl if watsonp else chrhigh - goffset
+        else:
+            left = chroffset + goffset if watsonp else chrhigh - goffset + 1 - gl
+        seg = genome_seq[left:left + gl]
+        if not watsonp:
+            seg = revcomp(seg)
+        if single:
+            if gl == rl:
+                q = mutate(rng, seg, 0.02, 0.0)[:rl]
+            else:
+                a = int(rng.integers(0, rl + 1))
+                src = seg[:a] + seg[a + gl - rl:]
+                q = mutate(rng, src, 0.02, 0.002, 3)[:rl]
+        else:
+            # the 5' end reads its segment backwards from the last base: the
+            # read is anchored at the segment's end; the 3' read at its start
+            src = seg[gl - rl:] if five else seg[:rl]
+            q = mutate(rng, src, 0.03, 0.01, 5)[:rl]
+            q = bytes(ord("N") if rng.random() < 0.01 else ch for ch in q)
+        q = q.ljust(rl, b"A")
+        case = {"single": single, "five": five, "rseq": q, "rl": rl, "gl": gl,
+                "roffset": int(rng.integers(0, 50)), "goffset": goffset, "chroffset": chroffset,
+                "chrhigh": chrhigh, "watsonp": watsonp, "genestrand": 0, "jump_late": bool(rng.integers(0, 2)),
+                "extraband": 10 if single else 6, "widebandp": True,
+                "defect_rate": float(rng.choice([0.001, 0.02] if single else [0.001, 0.005, 0.02])),
+                "endalign": int(rng.choice([3, 0, 1, 2], p=[0.4, 0.4, 0.1, 0.1])) if not single else 0}
+        cases.append(case)
+    return cases
