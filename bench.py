@@ -131,6 +131,10 @@ def serving_bench(args):
     lane, table = autotune(tw.problems, tw.qbuf.tobytes(), tw.gbuf.tobytes(), objective="latency")
     del tw
     log("serving lane groups (latency autotuner): g16_max_span=%d g64_min_rows=%d" % lane)
+    if args.serve_mix > 0:
+        # requests rarely share a band: mixed paired waves hold neighbours in band
+        lane = tuple(lane) + (args.serve_mix,)
+        log("serving mixed waves: bands within %d columns share a wave" % args.serve_mix)
     knob_table = None
     if args.serve_batch and args.serve_long >= 0:
         knob = (args.serve_slots, args.serve_batch, args.serve_long)
@@ -219,6 +223,12 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default lane groups instead of the autotuner's")
     ap.add_argument("--lane-groups", default="", metavar="SPAN,ROWS",
                     help="fixed lane groups (g16_max_span,g64_min_rows) instead of the autotuner's")
+    ap.add_argument("--mix-span", type=int, default=16,
+                    help="band spread allowed in mixed paired waves of the batch (0: one band per wave); "
+                         "16 measured best on c3 (fill 77.0 ms against 85.1 ms with one band per wave)")
+    ap.add_argument("--serve-mix", type=int, default=0,
+                    help="band spread allowed in the server's mixed waves (0: one band per wave; serving "
+                         "saturation measured the same at 0, 24 and 64)")
     ap.add_argument("--serve-genome", type=int, default=3_100_000_000, help="bases of the device genome")
     ap.add_argument("--serve-requests", type=int, default=600_000)
     ap.add_argument("--serve-batch", type=int, default=0,
@@ -267,6 +277,8 @@ def main():
         tuning = {"objective": "throughput", "sample": min(len(probs), 1_000_000), "chosen": list(params),
                   "candidates": [[list(p), round(ms, 3)] for p, ms in table], "seconds": round(time.time() - t1, 1)}
         log("autotuner: g16_max_span=%d g64_min_rows=%d (%.1fs)" % (params[0], params[1], time.time() - t1))
+    if args.mix_span:
+        params = tuple(params or (240, 8001)) + (args.mix_span,)
     b = DeviceBatch(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes(), params=params)
     stream = torch.cuda.current_stream()
     graph = None
@@ -450,7 +462,8 @@ def main():
                        config, "fill + traceback"),
                    "launch": "eager" if graph is None else "one HIP graph per step",
                    "lane_groups": None if params is None else {"g16_max_span": params[0],
-                                                                 "g64_min_rows": params[1]}},
+                                                                 "g64_min_rows": params[1]},
+                   "mix_span": args.mix_span},
         "fill_ms": round(fill_ms, 3),
         "traceback_ms": round(tb_ms, 3),
         "fill_gcups": round(total_cells / (fill_ms * 1e-3) / 1e9, 3),

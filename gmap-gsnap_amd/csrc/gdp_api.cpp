@@ -159,7 +159,7 @@ extern "C" int64_t gdp_plan_bytes(int32_t n) {
 // g16_max_span; wider bands, and queries of at least g64_min_rows rows (a
 // sweep a quarter as long, for callers that wait on single long problems),
 // take 64-lane groups.
-static gdp_plan_params g_default_params = {240, GDP_MAX_LEN + 1, {0, 0}};
+static gdp_plan_params g_default_params = {240, GDP_MAX_LEN + 1, 0, 0};
 
 static int pair_lanes(const gdp_problem &p, const gdp_plan_params &pp) {
   if (p.kind != GDP_KIND_FULL || p.width != 16 || p.keep_scores) return 0;
@@ -188,8 +188,9 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
   }
   const gdp_problem *probs = narrowed.data();
   const gdp_plan_params pp = params ? *params : g_default_params;
-  if (pp.g16_max_span < 0 || pp.g16_max_span > 240 || pp.g64_min_rows < 1)
-    return fail(GDP_EINVAL, "gdp_plan: g16_max_span must lie in [0, 240] and g64_min_rows be positive");
+  if (pp.g16_max_span < 0 || pp.g16_max_span > 240 || pp.g64_min_rows < 1 || pp.mix_span < 0)
+    return fail(GDP_EINVAL,
+                "gdp_plan: g16_max_span must lie in [0, 240], g64_min_rows be positive and mix_span not negative");
   uint8_t *base = (uint8_t *)descs_out;
   gdp_dev_problem *d = (gdp_dev_problem *)descs_out;
   int64_t dw = 0, ss = 0;
@@ -244,7 +245,10 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
     d[i] = q;
   }
 
-  // Waves: problems with the same tie rule and band, in order of size.
+  // Waves: problems with the same tie rule and band, in order of size; with
+  // pp.mix_span > 0 the problems left over from full waves of one band go
+  // to mixed waves, whose problems' lower and upper bands each lie within
+  // mix_span of the wave's largest (the sweep is laid out for those).
   std::stable_sort(paired.begin(), paired.end(), [&](int32_t a, int32_t b) {
     const gdp_problem &x = probs[a], &y = probs[b];
     const int kx = pair_lanes(x, pp), ky = pair_lanes(y, pp);
@@ -256,6 +260,65 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
     return x.gl < y.gl;
   });
   std::vector<gdp_wave> waves[GDP_NCLS];
+  auto emit = [&](const int32_t *ix, int cnt, int G) {
+    gdp_wave w;
+    memset(&w, 0, sizeof w);
+    for (int s = 0; s < 8; s++) {
+      w.prob[s] = -1;
+      w.fin_t[s] = INT32_MAX;
+    }
+    int R = 0, C = 0, lb = 0, ub = 0, lbmin = INT32_MAX, ubmin = INT32_MAX;
+    for (int j = 0; j < cnt; j++) {
+      const gdp_problem &x = probs[ix[j]];
+      R = gdp_imax(R, x.rl);
+      C = gdp_imax(C, x.gl);
+      lb = gdp_imax(lb, x.lband);
+      ub = gdp_imax(ub, x.uband);
+      lbmin = gdp_imin(lbmin, x.lband);
+      ubmin = gdp_imin(ubmin, x.uband);
+    }
+    // A half whose lower band is below the wave's starts its rows and meets
+    // its bottom edge up to lb - lbmin steps later than the sweep's own band
+    // would; the period keeps those events inside their batch.
+    const int P = gdp_imax(gdp_p_period(G, lb, ub), (2 * G + 8 + (lb - lbmin) + 7) & ~7);
+    w.R = R;
+    w.C = C;
+    w.P = P;
+    w.nb = (R + G) / G;
+    w.nsteps = gdp_p_nsteps(G, P, R);
+    w.lb = (int16_t)lb;
+    w.ub = (int16_t)ub;
+    w.lbmin = (int16_t)lbmin;
+    w.ubmin = (int16_t)ubmin;
+    w.mixed = (lbmin != lb || ubmin != ub) ? 1 : 0;
+    w.late = probs[ix[0]].jump_late ? 1 : 0;
+    w.glog = G == 16 ? 4 : 6;
+    std::vector<std::pair<int32_t, int8_t>> fin;
+    for (int s = 0; s < cnt; s++) {
+      const int idx = ix[s];
+      const gdp_problem &x = probs[idx];
+      w.prob[s] = idx;
+      gdp_dev_problem &q = d[idx];
+      q.layout = 1;
+      q.glog = (int8_t)w.glog;
+      q.sR = R;
+      q.sC = C;
+      q.P = P;
+      q.nsteps = w.nsteps;
+      q.slb = (int16_t)lb;
+      q.dir_off = dw;
+      dw += (int64_t)(w.nsteps / 8) * G;
+      if (x.rl <= x.gl + x.lband && x.rl >= x.gl - x.uband)
+        fin.push_back({gdp_p_step(G, P, lb, x.rl, x.gl), (int8_t)s});
+    }
+    std::sort(fin.begin(), fin.end());
+    for (size_t j = 0; j < fin.size(); j++) {
+      w.fin_t[j] = fin[j].first;
+      w.fin_slot[j] = fin[j].second;
+    }
+    waves[pair_class(G, P)].push_back(w);
+  };
+  std::vector<int32_t> rest;  // problems left for mixed waves
   size_t k = 0;
   while (k < paired.size()) {
     const gdp_problem &p0 = probs[paired[k]];
@@ -268,50 +331,51 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
         break;
       e++;
     }
-    gdp_wave w;
-    memset(&w, 0, sizeof w);
-    for (int s = 0; s < 8; s++) {
-      w.prob[s] = -1;
-      w.fin_t[s] = INT32_MAX;
+    if (pp.mix_span > 0 && e - k < (size_t)slots) {
+      for (size_t j = k; j < e; j++) rest.push_back(paired[j]);
+    } else {
+      emit(paired.data() + k, (int)(e - k), G);
     }
-    int R = 0, C = 0;
-    for (size_t j = k; j < e; j++) {
-      R = gdp_imax(R, probs[paired[j]].rl);
-      C = gdp_imax(C, probs[paired[j]].gl);
-    }
-    const int lb = p0.lband, ub = p0.uband, P = gdp_p_period(G, lb, ub);
-    w.R = R;
-    w.C = C;
-    w.P = P;
-    w.nb = (R + G) / G;
-    w.nsteps = gdp_p_nsteps(G, P, R);
-    w.lb = (int16_t)lb;
-    w.ub = (int16_t)ub;
-    w.late = p0.jump_late ? 1 : 0;
-    w.glog = G == 16 ? 4 : 6;
-    std::vector<std::pair<int32_t, int8_t>> fin;
-    for (size_t j = k; j < e; j++) {
-      const int s = (int)(j - k), idx = paired[j];
-      const gdp_problem &x = probs[idx];
-      w.prob[s] = idx;
-      gdp_dev_problem &q = d[idx];
-      q.layout = 1;
-      q.glog = (int8_t)w.glog;
-      q.sR = R;
-      q.sC = C;
-      q.P = P;
-      q.nsteps = w.nsteps;
-      q.dir_off = dw;
-      dw += (int64_t)(w.nsteps / 8) * G;
-      if (x.rl <= x.gl + lb && x.rl >= x.gl - ub) fin.push_back({gdp_p_step(G, P, lb, x.rl, x.gl), (int8_t)s});
-    }
-    std::sort(fin.begin(), fin.end());
-    for (size_t j = 0; j < fin.size(); j++) {
-      w.fin_t[j] = fin[j].first;
-      w.fin_slot[j] = fin[j].second;
-    }
-    waves[pair_class(G, P)].push_back(w);
     k = e;
+  }
+  if (!rest.empty()) {
+    // neighbours in band width and lower band share a wave
+    std::stable_sort(rest.begin(), rest.end(), [&](int32_t a, int32_t b) {
+      const gdp_problem &x = probs[a], &y = probs[b];
+      const int kx = pair_lanes(x, pp), ky = pair_lanes(y, pp);
+      if (kx != ky) return kx < ky;
+      if ((x.jump_late != 0) != (y.jump_late != 0)) return (x.jump_late != 0) < (y.jump_late != 0);
+      const int wx = x.lband + x.uband, wy = y.lband + y.uband;
+      if (wx != wy) return wx < wy;
+      if (x.lband != y.lband) return x.lband < y.lband;
+      return x.rl < y.rl;
+    });
+    size_t a = 0;
+    while (a < rest.size()) {
+      const gdp_problem &p0 = probs[rest[a]];
+      const int G = pair_lanes(p0, pp), slots = 2 * (64 / G);
+      // the sweep's span must fit the class's ring: 240 for 16-lane groups,
+      // 960 for 64-lane groups
+      const int maxspan = G == 16 ? 240 : 960;  // (and 2G + 8 + mix_span stays below the ring)
+      int lb = p0.lband, ub = p0.uband, lbmin = lb, ubmin = ub;
+      size_t e = a + 1;
+      while (e < rest.size() && e - a < (size_t)slots) {
+        const gdp_problem &x = probs[rest[e]];
+        if (pair_lanes(x, pp) != G || (x.jump_late != 0) != (p0.jump_late != 0)) break;
+        const int nlb = gdp_imax(lb, x.lband), nub = gdp_imax(ub, x.uband);
+        const int nlbmin = gdp_imin(lbmin, x.lband), nubmin = gdp_imin(ubmin, x.uband);
+        if (nlb - nlbmin > pp.mix_span || nub - nubmin > pp.mix_span || nlb + nub > maxspan ||
+            2 * G + 8 + (nlb - nlbmin) > (G == 16 ? 256 : 1024))
+          break;
+        lb = nlb;
+        ub = nub;
+        lbmin = nlbmin;
+        ubmin = nubmin;
+        e++;
+      }
+      emit(rest.data() + a, (int)(e - a), G);
+      a = e;
+    }
   }
   // Empty slots of partial waves store into one shared sink.
   int64_t sink_words = 0;

@@ -65,7 +65,8 @@ struct gdp_dev_problem {
   int32_t saturation;  // GDP_KIND_STD: floor of the stored scores
   int8_t std_flags;    // GDP_KIND_STD: bit 0 upperp, bit 1 lowerp
   int8_t keep_mode;    // gdp_problem.keep_scores: 1 every score, 2/3 the best end cell only
-  int8_t pad8[2];
+  int16_t slb;         // layout 1: the lower band of the wave's sweep (its largest lband), which fixes
+                       // the step of each cell (gdp_p_step); the problem's own band is lband/uband
   int32_t pad[3];
 };
 static_assert(sizeof(gdp_dev_problem) == 96, "gdp_dev_problem must stay 96 bytes");
@@ -81,8 +82,11 @@ static_assert(sizeof(gdp_align_result) == 72, "gdp_align_result is 72 bytes");
 
 // One wave of the paired fill: 64 lanes = 64/G groups of G lanes, each lane
 // carrying two problems in the low and high 16 bits of its registers.  Slot
-// 2*g + h is group g, half h.  Every problem of a wave has the same band and
-// tie rule; rows and columns are padded to the largest problem.
+// 2*g + h is group g, half h.  Every problem of a wave has the same tie rule;
+// rows and columns are padded to the largest problem.  The sweep is laid out
+// for the largest lower and upper bands of the wave (lb, ub); in a mixed wave
+// (mixed = 1) the problems' own bands differ, lie within [lbmin, lb] and
+// [ubmin, ub], and each problem's band edges are applied in its own half.
 struct gdp_wave {
   int32_t prob[8];      // problem index per slot, -1 = empty
   int32_t fin_t[8];     // steps at which final cells (rl, gl) are computed, ascending, INT32_MAX pads
@@ -90,8 +94,9 @@ struct gdp_wave {
   int32_t R, C;         // padded sweep: rows 0..R, columns 0..C
   int32_t P, nb, nsteps;
   int16_t lb, ub;
-  int8_t late, glog, pad0[2];
-  int32_t pad[7];
+  int8_t late, glog, mixed, pad0;
+  int16_t lbmin, ubmin;
+  int32_t pad[6];
 };
 static_assert(sizeof(gdp_wave) == 128, "gdp_wave must stay 128 bytes");
 

@@ -688,7 +688,7 @@ __device__ __forceinline__ uint32_t nib_at(const uint32_t *dirs, const gdp_dev_p
   }
   if (p.layout == 1) {  // paired sweep: byte 0 Egap, 1 HORIZ, 2 Fgap, 3 VERT; bit = step & 7
     const int g = 1 << p.glog;
-    const int t = gdp_p_step(g, p.P, p.lband, row, col);
+    const int t = gdp_p_step(g, p.P, p.slb, row, col);  // the wave's sweep layout
     const uint32_t w = dirs[p.dir_off + (int64_t)(t >> 3) * g + (row & (g - 1))] >> (t & 7);
     return ((w >> 24) & 1 ? GDP_DIR_V : ((w >> 8) & 1 ? GDP_DIR_H : 0)) | (w & 1 ? GDP_DIR_E : 0) |
            ((w >> 16) & 1 ? GDP_DIR_F : 0);
@@ -807,7 +807,7 @@ __global__ __launch_bounds__(256) void gdp_segments_kernel(const void *__restric
 // divisions of gdp_step / gdp_p_step turned into shifts.
 struct tb_dirs {
   const uint32_t *base;
-  int layout, glog, gmask, P, lb, ub, R, C, full, smask;
+  int layout, glog, gmask, P, lb, ub, R, C, full, smask, slb;
   __device__ __forceinline__ void init(const uint32_t *dirs, const gdp_dev_problem &p, const gdp_geom &G) {
     base = dirs + p.dir_off;
     layout = p.layout;
@@ -820,6 +820,7 @@ struct tb_dirs {
     C = G.C;
     full = G.full;
     smask = ~(G.S - 1);
+    slb = p.layout == 1 ? p.slb : G.lb;  // the paired sweep's layout follows the wave's band
   }
   // Word holding sweep cell (row, col) and the cell's step within it;
   // ok = false outside the computed cells (direction DIAG there).
@@ -836,7 +837,7 @@ struct tb_dirs {
     }
     const int b = row >> glog, i = row & gmask;
     if (layout == 1) {
-      const int c0 = (b << glog) - lb;
+      const int c0 = (b << glog) - slb;
       t = b * P + i + col - (c0 > 0 ? c0 : 0);
       return ok ? base + ((t >> 3) << glog) + i : base;
     }

@@ -301,7 +301,7 @@ bool gdp_server::launch(Slot &s) {
       qpos += q.r.rl;
       gpos += q.r.gl;
     }
-    const gdp_plan_params *pp = (cfg.plan.g16_max_span || cfg.plan.g64_min_rows) ? &cfg.plan : nullptr;
+    const gdp_plan_params *pp = (cfg.plan.g16_max_span || cfg.plan.g64_min_rows || cfg.plan.mix_span) ? &cfg.plan : nullptr;
     if (gdp_plan_tuned(probs.data(), n, 0, pp, s.h_up, &dw, &ss) != GDP_OK) {
       // Requests and the plan parameters were validated before; a planning
       // failure is a bug: refuse this batch and stop like a HIP failure.
@@ -543,9 +543,11 @@ extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const voi
     return nullptr;
   }
   // lane groups: all zero for gdp_plan's defaults, else what gdp_plan_tuned accepts
-  if ((cfg->plan.g16_max_span != 0 || cfg->plan.g64_min_rows != 0) &&
-      (cfg->plan.g16_max_span < 0 || cfg->plan.g16_max_span > 240 || cfg->plan.g64_min_rows < 1)) {
-    set_err("gdp_server_create: plan.g16_max_span must lie in [0, 240] and plan.g64_min_rows be positive");
+  if ((cfg->plan.g16_max_span != 0 || cfg->plan.g64_min_rows != 0 || cfg->plan.mix_span != 0) &&
+      (cfg->plan.g16_max_span < 0 || cfg->plan.g16_max_span > 240 || cfg->plan.g64_min_rows < 1 ||
+       cfg->plan.mix_span < 0)) {
+    set_err("gdp_server_create: plan.g16_max_span must lie in [0, 240], plan.g64_min_rows be positive and "
+            "plan.mix_span not negative");
     return nullptr;
   }
   if (genome_format != GDP_GENOME_ASCII && genome_format != GDP_GENOME_COMP) {

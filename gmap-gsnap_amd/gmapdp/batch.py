@@ -40,16 +40,17 @@ def _check(L, rc, what):
 PLAN_SINGLE = 1  # GDP_PLAN_SINGLE: every problem swept on its own
 
 
-def plan_params(g16_max_span=240, g64_min_rows=8001):
-    """gdp_plan_params: lane-group choice of the paired sweep."""
+def plan_params(g16_max_span=240, g64_min_rows=8001, mix_span=0):
+    """gdp_plan_params: lane-group choice of the paired sweep and the band
+    spread allowed in mixed waves (0: one band per wave)."""
     p = np.zeros(1, PLAN_PARAMS_DTYPE)
-    p["g16_max_span"], p["g64_min_rows"] = g16_max_span, g64_min_rows
+    p["g16_max_span"], p["g64_min_rows"], p["mix_span"] = g16_max_span, g64_min_rows, mix_span
     return p
 
 
 def plan(problems, flags=0, params=None):
     """Host planning (no device needed): descriptors and buffer sizes.
-    params: (g16_max_span, g64_min_rows) or None for the defaults."""
+    params: (g16_max_span, g64_min_rows[, mix_span]) or None for the defaults."""
     L = load()
     problems = np.ascontiguousarray(problems, PROBLEM_DTYPE)
     n = len(problems)

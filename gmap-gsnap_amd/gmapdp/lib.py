@@ -64,7 +64,7 @@ ALIGN_REQ_DTYPE = np.dtype([("tag", "<u8"), ("read", "<u8"), ("rl", "<i4"), ("gl
 ALIGN_RES_DTYPE = np.dtype([("tag", "<u8"), ("status", "<i4"), ("finalscore", "<i4"), ("counts", "<i4", (6,)),
                             ("width", "<i4"), ("pad", "<i4"), ("pairs", "<u8"), ("t_submit", "<f8"),
                             ("t_done", "<f8")])
-PLAN_PARAMS_DTYPE = np.dtype([("g16_max_span", "<i4"), ("g64_min_rows", "<i4"), ("pad", "<i4", (2,))])
+PLAN_PARAMS_DTYPE = np.dtype([("g16_max_span", "<i4"), ("g64_min_rows", "<i4"), ("mix_span", "<i4"), ("pad", "<i4")])
 SERVER_CFG_DTYPE = np.dtype([("dir_budget_bytes", "<i8"), ("pair_capacity", "<i8"), ("max_batch", "<i4"),
                              ("slots", "<i4"), ("max_rl", "<i4"), ("max_gl", "<i4"), ("result_buffers", "<i4"),
                              ("long_rows", "<i4"), ("plan", PLAN_PARAMS_DTYPE)])

@@ -229,7 +229,13 @@ int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, void *descs_
 typedef struct gdp_plan_params {
   int32_t g16_max_span;
   int32_t g64_min_rows;
-  int32_t pad[2];
+  /* mix_span > 0: the problems that do not fill a wave of their own band
+     share mixed waves, in which each problem's lower and upper band lies
+     within mix_span of the wave's largest; the sweep is laid out for the
+     largest and every problem keeps its own band edges.  0: a wave holds
+     problems of one band only. */
+  int32_t mix_span;
+  int32_t pad;
 } gdp_plan_params;
 
 /* As gdp_plan_ex with explicit lane-group parameters (NULL = defaults). */

@@ -118,3 +118,48 @@ def test_graph_replays_equal_eager(oracle):
                 assert torch.equal(x, y), k
     finally:
         b.graph_destroy(g)
+
+
+def _mixed_specs(seed, n, late, rl_range, lb_range, ub_extra, alph_q=b"ACGT", genome_n=0.0, alt=0.0):
+    """Problems each with a band of its own: lower bands from lb_range and
+    upper bands gl - rl + U(ub_extra), as Dynprog_compute_bands widens them,
+    so that few share a band and most land in mixed waves."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(n):
+        lb = int(rng.integers(*lb_range))
+        s = _class_specs(seed * 1000 + k, 1, lb, lb + int(rng.integers(*ub_extra)), late, rl_range, alph_q, genome_n,
+                         alt)[0]
+        specs.append(s)
+    return specs
+
+
+@pytest.mark.parametrize("params", [(240, 8001, 16), (112, 256, 24), (0, 8001, 40), (240, 1, 8)])
+def test_mixed_band_waves(oracle, params):
+    """Waves whose problems have different bands (gdp_plan_params.mix_span):
+    the sweep is laid out for the wave's largest bands and each half applies
+    its own problem's row starts, block prefix, bottom edge and top edge.
+    Both tie rules, narrow bands (bottom edges near column 0), 16- and
+    64-lane groups, and the general lookup."""
+    specs = []
+    specs += _mixed_specs(51, 48, False, (1, 120), (0, 40), (0, 60))
+    specs += _mixed_specs(52, 48, True, (20, 300), (3, 64), (0, 120))
+    specs += _mixed_specs(53, 24, True, (1, 40), (0, 6), (0, 8))
+    specs += _mixed_specs(54, 20, False, (30, 160), (5, 30), (0, 40), alph_q=b"ACGTNRY", genome_n=0.05, alt=0.1)
+    b = _check(oracle, specs, 9, params)
+    d = b.d_descs.cpu().numpy()[: b.n * 96].reshape(b.n, 96)
+    layout = d[:, 58]
+    slb = d[:, 82].astype(np.int16) | (d[:, 83].astype(np.int16) << 8)
+    lband = np.array([min(s.lband, s.rl + 1) for s in specs])
+    paired = layout == 1
+    assert paired.sum() > 100
+    assert (slb[paired] > lband[paired]).sum() > 20, "no wave mixed lower bands"
+    assert (slb[paired] >= lband[paired]).all()
+
+
+def test_mixed_band_waves_end_on_uniform_ones(oracle):
+    """With mix_span the problems that fill waves of their own band keep
+    them; the rest go to mixed waves; the fills stay exact either way."""
+    specs = _class_specs(61, 20, 9, 30, False, (50, 150)) + _mixed_specs(62, 30, False, (40, 150), (5, 15), (0, 40))
+    _check(oracle, specs, 10, (240, 8001, 12))
+
