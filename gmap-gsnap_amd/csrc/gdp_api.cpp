@@ -26,6 +26,9 @@ extern "C" int gdp_launch_best_endpoint(const void *, const int32_t *, int, int,
                                         const gdp_tb_request *, gdp_tb_request *, hipStream_t);
 extern "C" int gdp_launch_bridge_cdna(const void *, const int32_t *, int, const int16_t *, int32_t *,
                                       const gdp_tb_request *, gdp_tb_request *, hipStream_t);
+extern "C" int gdp_launch_bridge_intron(const void *, const gdp_intron_gap *, int, const int16_t *, const char *,
+                                        const double *, int32_t *, double *, const gdp_tb_request *, gdp_tb_request *,
+                                        hipStream_t);
 extern "C" int gdp_launch_export(const void *, int, const uint32_t *, const int16_t *, uint8_t *, int16_t *,
                                  hipStream_t);
 extern "C" int gdp_launch_export_ref(const void *, int, int, const uint32_t *, const int16_t *, int, void *, void *,
@@ -616,6 +619,19 @@ extern "C" int gdp_bridge_cdna(const void *d_descs, const int32_t *d_quads, int3
     return fail(GDP_EINVAL, "gdp_bridge_cdna: null argument");
   if (gdp_launch_bridge_cdna(d_descs, d_quads, m, d_scores, d_out, d_tb_template, d_tb_out, (hipStream_t)stream))
     return hip_fail("gdp_bridge_cdna launch", hipGetLastError());
+  return GDP_OK;
+}
+
+extern "C" int gdp_bridge_intron(const void *d_descs, const gdp_intron_gap *d_gaps, int32_t m, const int16_t *d_scores,
+                                 const char *d_gbuf, const double *d_probs, int32_t *d_out, double *d_out_probs,
+                                 const gdp_tb_request *d_tb_template, gdp_tb_request *d_tb_out, void *stream) {
+  if (!g_init) return fail(GDP_ENOINIT, "gdp_bridge_intron: gdp_init has not been called");
+  if (m < 0 || (m > 0 && (!d_descs || !d_gaps || !d_scores || !d_gbuf || !d_probs || !d_out || !d_out_probs)) ||
+      ((d_tb_template == nullptr) != (d_tb_out == nullptr)))
+    return fail(GDP_EINVAL, "gdp_bridge_intron: null argument");
+  if (gdp_launch_bridge_intron(d_descs, d_gaps, m, d_scores, d_gbuf, d_probs, d_out, d_out_probs, d_tb_template,
+                               d_tb_out, (hipStream_t)stream))
+    return hip_fail("gdp_bridge_intron launch", hipGetLastError());
   return GDP_OK;
 }
 

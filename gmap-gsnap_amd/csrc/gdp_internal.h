@@ -76,6 +76,7 @@ static_assert(sizeof(gdp_tb_request) == 48, "gdp_tb_request is 48 bytes");
 void gdp_set_error(const char *msg);
 
 static_assert(sizeof(gdp_pair) == 16, "gdp_pair is 16 bytes");
+static_assert(sizeof(gdp_intron_gap) == 32, "gdp_intron_gap is 32 bytes");
 static_assert(sizeof(gdp_segment) == 24, "gdp_segment is 24 bytes");
 static_assert(sizeof(gdp_align_request) == 64, "gdp_align_request is 64 bytes");
 static_assert(sizeof(gdp_align_result) == 72, "gdp_align_result is 72 bytes");

@@ -1545,6 +1545,261 @@ __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
   }
 }
 
+// ----------------------------------------------------------- intron bridge
+
+#define GDP_INTRON_SITES 8192        // LDS bytes of one end's dinucleotide codes (glength + 1 entries)
+#define GDP_INTRON_SITES_SMALL 1024  // the launch for gaps of both ends under this many columns
+
+// Dinucleotide codes (intron.h:10-21): a left site from the two bases after
+// the left exon, a right site from the two before the right exon, either
+// allele supplying each base (dynprog_genome.c:1791-1843).  Their AND is the
+// intron type (intron.h:26-32).
+__device__ __forceinline__ bool gdp_either(int x, int xa, int b) { return x == b || xa == b; }
+
+__device__ __forceinline__ int gdp_left_site(int l1, int l1a, int l2, int l2a) {
+  if (gdp_either(l1, l1a, 'G') && gdp_either(l2, l2a, 'T')) return 0x21;  // GT
+  if (gdp_either(l1, l1a, 'G') && gdp_either(l2, l2a, 'C')) return 0x10;  // GC
+  if (gdp_either(l1, l1a, 'A') && gdp_either(l2, l2a, 'T')) return 0x08;  // AT
+  if (gdp_either(l1, l1a, 'C') && gdp_either(l2, l2a, 'T')) return 0x06;  // CT
+  return 0;
+}
+
+__device__ __forceinline__ int gdp_right_site(int r2, int r2a, int r1, int r1a) {
+  if (gdp_either(r2, r2a, 'A') && gdp_either(r1, r1a, 'G')) return 0x30;  // AG
+  if (gdp_either(r2, r2a, 'A') && gdp_either(r1, r1a, 'C')) return 0x0C;  // AC
+  if (gdp_either(r2, r2a, 'G') && gdp_either(r1, r1a, 'C')) return 0x02;  // GC
+  if (gdp_either(r2, r2a, 'A') && gdp_either(r1, r1a, 'T')) return 0x01;  // AT
+  return 0;
+}
+
+// The reward of an intron type (intron_score_setup, dynprog_genome.c:143-186,
+// with its build's constants, 76-115): GT-AG / CT-AC 14 preliminary, 16
+// final; GC-AG and its antisense 8 / 10; AT-AC and its antisense 4 / 8; the
+// sense table (cdna_direction > 0) holds the forward types, the antisense
+// one (< 0) the reverse ones, the either table both -- with the reference's
+// preliminary forward GT-AG at 16 and final reverse CT-AC at 14.
+__device__ __forceinline__ int gdp_intron_reward(int dir, int finalp, int type) {
+  const bool fwd = type == 0x20 || type == 0x10 || type == 0x08;
+  const bool rev = type == 0x04 || type == 0x02 || type == 0x01;
+  if ((dir > 0 && !fwd) || (dir < 0 && !rev) || (!fwd && !rev)) return 0;
+  if (type == 0x20 || type == 0x04) {
+    if (dir == 0) return type == 0x20 ? 16 : 14;
+    return finalp ? 16 : 14;
+  }
+  if (type == 0x10 || type == 0x02) return finalp ? 10 : 8;
+  return finalp ? 8 : 4;
+}
+
+// Of two bridge states, whether (v, p, row) comes first: the higher score,
+// then the higher probability sum, then the earlier row of the scan (row -1
+// is the initial state, which precedes every candidate).
+__device__ __forceinline__ bool gdp_intron_before(int v, double p, int row, int bv, double bp, int brow) {
+  return v > bv || (v == bv && (p > bp || (p == bp && row < brow)));
+}
+
+// bridge_intron_gap_{8,16}_site_level (dynprog_genome.c:866-1386,
+// 1741-2261) as bridge_intron_gap_{8,16}_ud runs it (1387-1488, 2262-2363),
+// one wave per gap.  The reference scans rL = 1 .. rlength-1 with rR =
+// rlength-rL: the diagonal pair (cL, cR) = (rL, rR); the left end on its
+// diagonal with cR from max(1, rR-lbandR) below the right diagonal, then
+// past it up to (not including) min(rR+ubandR, glengthR-1); the right end on
+// its diagonal with cL likewise; every cR below span - cL.  A candidate
+// replaces the best on a higher score, or an equal score with a higher
+// probability sum, so the result is the scan's first candidate of the
+// highest (score, probability sum): lanes take whole rows, keep their
+// rows' first best, and merge by (score, sum, row).  The dinucleotide track
+// (diagonal pairs with a positive reward, the first of the highest sum above
+// 0) merges by (sum, row).  The dinucleotide codes of both ends sit in LDS.
+// Two launches share the gaps: SITES = GDP_INTRON_SITES_SMALL takes those
+// whose ends both have fewer columns (eight times the blocks per CU), the
+// full size all others, including those that do not fit.
+template <int SITES>
+__global__ __launch_bounds__(64) void gdp_bridge_intron_kernel(
+    const gdp_dev_problem *__restrict__ probs, const gdp_intron_gap *__restrict__ gaps, int m,
+    const int16_t *__restrict__ scores, const char *__restrict__ gbuf, const double *__restrict__ prob,
+    int32_t *__restrict__ out, double *__restrict__ outp, const gdp_tb_request *__restrict__ tmpl,
+    gdp_tb_request *__restrict__ tbout) {
+  __shared__ uint8_t ldi[SITES], rdi[SITES];
+  __shared__ int rew[64];
+  const int k = blockIdx.x, lane = threadIdx.x;
+  if (k >= m) return;
+  const gdp_intron_gap q = gaps[k];
+  const gdp_dev_problem Lu = probs[q.fills[0]], Ll = probs[q.fills[1]], Ru = probs[q.fills[2]],
+                        Rl = probs[q.fills[3]];
+  const int rl = Lu.rl, glL = Lu.gl, glR = Ru.gl;
+  const bool ok = Lu.kind == GDP_KIND_UPPER && Ll.kind == GDP_KIND_LOWER && Ru.kind == GDP_KIND_UPPER &&
+                  Rl.kind == GDP_KIND_LOWER && Lu.keep_mode == GDP_KEEP_ALL && Ll.keep_mode == GDP_KEEP_ALL &&
+                  Ru.keep_mode == GDP_KEEP_ALL && Rl.keep_mode == GDP_KEEP_ALL && Lu.score_off >= 0 &&
+                  Ll.score_off >= 0 && Ru.score_off >= 0 && Rl.score_off >= 0 && Ll.width == Lu.width &&
+                  Ru.width == Lu.width && Rl.width == Lu.width && Ll.rl == rl && Ru.rl == rl && Rl.rl == rl &&
+                  Ll.gl == glL && Rl.gl == glR && glL >= rl + 1 && glR >= rl + 1 && glL < GDP_INTRON_SITES &&
+                  glR < GDP_INTRON_SITES;
+  const bool small = ok && glL < GDP_INTRON_SITES_SMALL && glR < GDP_INTRON_SITES_SMALL;
+  if (small != (SITES == GDP_INTRON_SITES_SMALL)) return;  // the other launch's gap
+  if (!ok) {
+    if (lane == 0) {
+      int32_t *o = out + 5 * (int64_t)k;
+      o[0] = INT_MIN;
+      o[1] = o[2] = o[3] = o[4] = -1;
+      outp[2 * (int64_t)k] = outp[2 * (int64_t)k + 1] = 0.0;
+      if (tbout != nullptr) {
+        gdp_tb_request a = tmpl[2 * (int64_t)k], b = tmpl[2 * (int64_t)k + 1];
+        a.fill = q.fills[2];
+        b.fill = q.fills[0];
+        a.r = a.c = b.r = b.c = 0;
+        tbout[2 * (int64_t)k] = a;
+        tbout[2 * (int64_t)k + 1] = b;
+      }
+    }
+    return;
+  }
+  // column c of a fill reads g[c-1]: the left site after column c is bytes
+  // c, c+1 of the left fill's genome; the right fill runs backwards, so the
+  // site before its column c is bytes c+1, c of its genome
+  const char *gL = gbuf + Lu.goff, *gLa = gbuf + Lu.galtoff, *gR = gbuf + Ru.goff, *gRa = gbuf + Ru.galtoff;
+  for (int c = lane; c <= glL; c += 64) ldi[c] = c < glL - 1 ? gdp_left_site(gL[c], gLa[c], gL[c + 1], gLa[c + 1]) : 0;
+  for (int c = lane; c <= glR; c += 64)
+    rdi[c] = c < glR - 1 ? gdp_right_site(gR[c + 1], gRa[c + 1], gR[c], gRa[c]) : 0;
+  rew[lane] = gdp_intron_reward(q.cdna_direction, q.finalp, lane);
+  __syncthreads();
+
+  const gdp_geom GLu = gdp_make_geom(Lu), GLl = gdp_make_geom(Ll), GRu = gdp_make_geom(Ru), GRl = gdp_make_geom(Rl);
+  const int lbL = Ll.lband, ubL = Lu.uband, lbR = Rl.lband, ubR = Ru.uband, D = q.span;
+  const double *pL = prob + q.prob_off, *pR = pL + glL;
+  const int neg = Lu.width == 8 ? -128 : -32768;
+  int bv = neg, brow = -1, bcL = -1, bcR = -1;
+  double bp = 0.0;
+  double dp = 0.0;
+  int dv = 0, drow = -1;
+  for (int rL = 1 + lane; rL < rl; rL += 64) {
+    const int rR = rl - rL;
+    const int cloL = gdp_imax(1, rL - lbL), chighL = gdp_imin(rL + ubL, glL - 1);
+    const int cloR = gdp_imax(1, rR - lbR), chighR = gdp_imin(rR + ubR, glR - 1);
+    const int sLd = score_at(scores, Lu, GLu, rL, rL), sRd = score_at(scores, Ru, GRu, rR, rR);
+    const int siteL = ldi[rL], siteR = rdi[rR];
+    const double pLd = pL[rL], pRd = pR[rR];
+    // the diagonal pair, also the dinucleotide track's candidate
+    {
+      const int r = rew[siteL & siteR];
+      const int v = sLd + r + sRd;
+      const double p = pLd + pRd;
+      if (v > bv || (v == bv && p > bp)) {
+        bv = v;
+        bp = p;
+        brow = rL;
+        bcL = rL;
+        bcR = rR;
+      }
+      if (r > 0 && p > dp) {
+        dp = p;
+        dv = v;
+        drow = rL;
+      }
+    }
+    // left end on its diagonal, right end off it
+    int cR = cloR;
+    for (; cR < rR && cR < D - rL; cR++) {
+      const int v = sLd + rew[siteL & rdi[cR]] + score_at(scores, Rl, GRl, cR, rR);
+      const double p = pLd + pR[cR];
+      if (v > bv || (v == bv && p > bp)) {
+        bv = v;
+        bp = p;
+        brow = rL;
+        bcL = rL;
+        bcR = cR;
+      }
+    }
+    for (cR++; cR < chighR && cR < D - rL; cR++) {
+      const int v = sLd + rew[siteL & rdi[cR]] + score_at(scores, Ru, GRu, rR, cR);
+      const double p = pLd + pR[cR];
+      if (v > bv || (v == bv && p > bp)) {
+        bv = v;
+        bp = p;
+        brow = rL;
+        bcL = rL;
+        bcR = cR;
+      }
+    }
+    // right end on its diagonal, left end off it
+    int cL = cloL;
+    for (; cL < rL && cL < D - rR; cL++) {
+      const int v = score_at(scores, Ll, GLl, cL, rL) + rew[ldi[cL] & siteR] + sRd;
+      const double p = pL[cL] + pRd;
+      if (v > bv || (v == bv && p > bp)) {
+        bv = v;
+        bp = p;
+        brow = rL;
+        bcL = cL;
+        bcR = rR;
+      }
+    }
+    for (cL++; cL < chighL && cL < D - rR; cL++) {
+      const int v = score_at(scores, Lu, GLu, rL, cL) + rew[ldi[cL] & siteR] + sRd;
+      const double p = pL[cL] + pRd;
+      if (v > bv || (v == bv && p > bp)) {
+        bv = v;
+        bp = p;
+        brow = rL;
+        bcL = cL;
+        bcR = rR;
+      }
+    }
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    const int ov = __shfl_xor(bv, d, 64), orow = __shfl_xor(brow, d, 64), ocL = __shfl_xor(bcL, d, 64),
+              ocR = __shfl_xor(bcR, d, 64);
+    const double op = __shfl_xor(bp, d, 64);
+    if (gdp_intron_before(ov, op, orow, bv, bp, brow)) {
+      bv = ov;
+      bp = op;
+      brow = orow;
+      bcL = ocL;
+      bcR = ocR;
+    }
+    const double odp = __shfl_xor(dp, d, 64);
+    const int odv = __shfl_xor(dv, d, 64), odrow = __shfl_xor(drow, d, 64);
+    if (odp > dp || (odp == dp && odrow < drow)) {
+      dp = odp;
+      dv = odv;
+      drow = odrow;
+    }
+  }
+  if (lane != 0) return;
+  int brL = brow, brR = brow >= 0 ? rl - brow : -1;
+  // the canonical-dinucleotide candidate takes over unless the best's
+  // probability sum exceeds 2 * PROB_CEILING (0.85), none was found, or it
+  // scores below 0 or more than 9 under the best (2210-2237)
+  if (!(bp > 2 * 0.85) && dp != 0.0 && !(dv < 0 || dv < bv - 9)) {
+    bv = dv;
+    brL = bcL = drow;
+    brR = bcR = rl - drow;
+  }
+  int fin = bv;
+  if (bv >= 0 && q.halfp) fin = bv - rew[ldi[bcL] & rdi[bcR]] / 2;  // 2248-2255
+  int32_t *o = out + 5 * (int64_t)k;
+  o[0] = fin;
+  o[1] = brL;
+  o[2] = brR;
+  o[3] = bcL;
+  o[4] = bcR;
+  outp[2 * (int64_t)k] = fin >= 0 ? pL[bcL] : 0.0;
+  outp[2 * (int64_t)k + 1] = fin >= 0 ? pR[bcR] : 0.0;
+  if (tbout != nullptr) {
+    // Dynprog_genome_gap walks the right end first, each in the triangle that
+    // holds its cell (3575-3633, 3718-3774)
+    const bool has = fin >= 0;
+    gdp_tb_request a = tmpl[2 * (int64_t)k], b = tmpl[2 * (int64_t)k + 1];
+    a.fill = has && bcR < brR ? q.fills[3] : q.fills[2];
+    a.r = has ? brR : 0;
+    a.c = has ? bcR : 0;
+    b.fill = has && bcL < brL ? q.fills[1] : q.fills[0];
+    b.r = has ? brL : 0;
+    b.c = has ? bcL : 0;
+    tbout[2 * (int64_t)k] = a;
+    tbout[2 * (int64_t)k + 1] = b;
+  }
+}
+
 // ------------------------------------------------------------ dense export
 
 __global__ void gdp_export_kernel(const gdp_dev_problem *__restrict__ probs, int idx, const uint32_t *__restrict__ dirs,
@@ -1722,6 +1977,20 @@ extern "C" int gdp_launch_bridge_cdna(const void *d_descs, const int32_t *d_quad
                      (const gdp_dev_problem *)d_descs, d_quads, m, d_scores, d_out, d_tmpl, d_tbout);
   hipLaunchKernelGGL(gdp_bridge_cdna_kernel<GDP_BRIDGE_LDS>, dim3(m), dim3(64), 0, stream,
                      (const gdp_dev_problem *)d_descs, d_quads, m, d_scores, d_out, d_tmpl, d_tbout);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int gdp_launch_bridge_intron(const void *d_descs, const gdp_intron_gap *d_gaps, int m,
+                                        const int16_t *d_scores, const char *d_gbuf, const double *d_probs,
+                                        int32_t *d_out, double *d_outp, const gdp_tb_request *d_tmpl,
+                                        gdp_tb_request *d_tbout, hipStream_t stream) {
+  if (m <= 0) return 0;
+  hipLaunchKernelGGL(gdp_bridge_intron_kernel<GDP_INTRON_SITES_SMALL>, dim3(m), dim3(64), 0, stream,
+                     (const gdp_dev_problem *)d_descs, d_gaps, m, d_scores, d_gbuf, d_probs, d_out, d_outp, d_tmpl,
+                     d_tbout);
+  hipLaunchKernelGGL(gdp_bridge_intron_kernel<GDP_INTRON_SITES>, dim3(m), dim3(64), 0, stream,
+                     (const gdp_dev_problem *)d_descs, d_gaps, m, d_scores, d_gbuf, d_probs, d_out, d_outp, d_tmpl,
+                     d_tbout);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

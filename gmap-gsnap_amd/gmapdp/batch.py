@@ -7,7 +7,7 @@ import ctypes
 
 import numpy as np
 
-from .lib import PAIR_DTYPE, PLAN_PARAMS_DTYPE, PROBLEM_DTYPE, TBREQ_DTYPE, load
+from .lib import INTRON_GAP_DTYPE, PAIR_DTYPE, PLAN_PARAMS_DTYPE, PROBLEM_DTYPE, TBREQ_DTYPE, load
 
 _initialized_mode = None
 
@@ -194,6 +194,38 @@ class DeviceBatch:
         if tb_template is None:
             return out
         return out, d_tb.cpu().numpy().view(TBREQ_DTYPE)
+
+    def bridge_intron(self, gaps, site_probs, tb_template=None):
+        """gdp_bridge_intron over INTRON_GAP_DTYPE rows, with the per-column
+        splice-site probabilities site_probs (float64, indexed by each gap's
+        prob_off): ((m, 5) rows {finalscore, bestrL, bestrR, bestcL, bestcR},
+        (m, 2) rows {left_prob, right_prob}); with tb_template (2m requests)
+        also the completed traceback requests (right end, then left end)."""
+        torch = self.torch
+        gaps = np.ascontiguousarray(gaps, INTRON_GAP_DTYPE)
+        m = len(gaps)
+        d_gaps = torch.from_numpy(gaps.view(np.uint8).copy()).to(self.dev)
+        probs = np.ascontiguousarray(site_probs, np.float64)
+        d_p = torch.from_numpy(probs.copy() if probs.size else np.zeros(1)).to(self.dev)
+        d_out = torch.zeros(max(m, 1) * 5, dtype=torch.int32, device=self.dev)
+        d_outp = torch.zeros(max(m, 1) * 2, dtype=torch.float64, device=self.dev)
+        d_tmpl = d_tb = None
+        if tb_template is not None:
+            tmpl = np.ascontiguousarray(tb_template, TBREQ_DTYPE)
+            assert len(tmpl) == 2 * m
+            d_tmpl = torch.from_numpy(tmpl.view(np.uint8).copy()).to(self.dev)
+            d_tb = torch.zeros_like(d_tmpl)
+        _check(self.L, self.L.gdp_bridge_intron(self.d_descs.data_ptr(), d_gaps.data_ptr(), m,
+                                                self.d_scores.data_ptr(), self.d_g.data_ptr(), d_p.data_ptr(),
+                                                d_out.data_ptr(), d_outp.data_ptr(),
+                                                d_tmpl.data_ptr() if d_tmpl is not None else None,
+                                                d_tb.data_ptr() if d_tb is not None else None, self.stream),
+               "gdp_bridge_intron")
+        out = d_out.cpu().numpy()[: 5 * m].reshape(-1, 5)
+        outp = d_outp.cpu().numpy()[: 2 * m].reshape(-1, 2)
+        if tb_template is None:
+            return out, outp
+        return out, outp, d_tb.cpu().numpy().view(TBREQ_DTYPE)
 
     def bridge_requests(self, d_quads, m, d_out, d_tmpl, d_tbout, stream=None):
         """gdp_bridge_cdna on device buffers: joins into d_out and both ends'

@@ -64,6 +64,8 @@ ALIGN_REQ_DTYPE = np.dtype([("tag", "<u8"), ("read", "<u8"), ("rl", "<i4"), ("gl
 ALIGN_RES_DTYPE = np.dtype([("tag", "<u8"), ("status", "<i4"), ("finalscore", "<i4"), ("counts", "<i4", (6,)),
                             ("width", "<i4"), ("pad", "<i4"), ("pairs", "<u8"), ("t_submit", "<f8"),
                             ("t_done", "<f8")])
+INTRON_GAP_DTYPE = np.dtype([("fills", "<i4", (4,)), ("span", "<i4"), ("cdna_direction", "i1"), ("finalp", "i1"),
+                             ("halfp", "i1"), ("pad", "i1"), ("prob_off", "<i8")])
 PLAN_PARAMS_DTYPE = np.dtype([("g16_max_span", "<i4"), ("g64_min_rows", "<i4"), ("mix_span", "<i4"), ("pad", "<i4")])
 SERVER_CFG_DTYPE = np.dtype([("dir_budget_bytes", "<i8"), ("pair_capacity", "<i8"), ("max_batch", "<i4"),
                              ("slots", "<i4"), ("max_rl", "<i4"), ("max_gl", "<i4"), ("result_buffers", "<i4"),
@@ -124,6 +126,7 @@ def load(path=None):
         "gdp_fill": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, vp, vp]),
         "gdp_best_endpoint": (ctypes.c_int, [vp, vp, i32, i32, vp, vp, vp]),
         "gdp_bridge_cdna": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, vp]),
+        "gdp_bridge_intron": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
         "gdp_traceback": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp]),
         "gdp_traceback_ex": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp]),
         "gdp_traceback_ends": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp, vp]),

@@ -362,6 +362,55 @@ int gdp_best_endpoint_ex(const void *d_descs, const int32_t *d_ul, int32_t m, in
 int gdp_bridge_cdna(const void *d_descs, const int32_t *d_quads, int32_t m, const int16_t *d_scores,
                     int32_t *d_out, const gdp_tb_request *d_tb_template, gdp_tb_request *d_tb_out, void *stream);
 
+/* One intron gap of gdp_bridge_intron (32 bytes, pad zero). */
+typedef struct gdp_intron_gap {
+  int32_t fills[4];       /* left upper, left lower, right upper, right lower */
+  int32_t span;           /* rightoffset - leftoffset (rev_goffsetR - goffsetL) */
+  int8_t cdna_direction;  /* > 0 sense, < 0 antisense, 0 either */
+  int8_t finalp;          /* the final intron rewards instead of the preliminary ones */
+  int8_t halfp;           /* half the chosen sites' reward off the score */
+  int8_t pad;
+  int64_t prob_off;       /* d_probs index of the left end's first column; the right end's follow */
+} gdp_intron_gap;
+
+/* Intron gap bridge: the step of Dynprog_genome_gap (dynprog_genome.c:
+   3287-3770) that picks where an intron joins a left and a right exon end,
+   bridge_intron_gap_{8,16}_ud (1387-1488, 2262-2363; called at 3548, 3690) on
+   its site-level path (bridge_intron_gap_{8,16}_site_level, 866-1386,
+   1741-2261: novel splicing, no splicing IIT), batched.  For each of m gaps,
+   d_gaps[k].fills name four one-sided fills of one width and rlength rows,
+   kept with GDP_KEEP_ALL: the left pair over rsequence forward against the
+   left segment (glengthL columns, jump_late_p), the right pair over
+   rev_rsequence against the right segment read backwards (glengthR columns,
+   stored reversed, !jump_late_p).  Candidates pair row rL of the left end
+   with row rlength-rL of the right end, at least one end on its diagonal.
+   A candidate scores left + intron reward + right: the reward of the
+   canonical dinucleotides (GT-AG, GC-AG, AT-AC and their antisense forms,
+   either allele) read from the fills' genome bytes in d_gbuf after the left
+   exon's column and before the right exon's, under cdna_direction and
+   finalp (intron_score_setup, 143-186).  Ties on score go to the higher sum
+   of the two columns' splice-site probabilities, read from d_probs (the
+   caller's per-column site model values, left_probabilities and
+   right_probabilities of 1845-1933: glengthL doubles from prob_off, then
+   glengthR; entries glength-1 are not read); the first candidate of the
+   reference's scan keeps an exact tie.  A diagonal pair with a canonical
+   intron and a higher probability sum replaces the best as the reference's
+   dinucleotide rule decides (2210-2237).  d_out[5k .. 5k+4] = {finalscore,
+   bestrL, bestrR, bestcL, bestcR} (halfp applied; cells -1 when no candidate
+   beats the initial -128 / -32768); d_out_probs[2k, 2k+1] = the chosen
+   columns' probabilities when finalscore >= 0, else 0.0 (get_splicesite_probs,
+   332-401).  Gaps whose fills do not fit (kinds, widths, row counts,
+   kept scores, glengthL or glengthR not above rlength -- the reference
+   reads past its probability arrays there -- or above GDP_MAX_LEN) get
+   INT32_MIN.  With d_tb_template and d_tb_out (both or neither), d_tb_out[2k]
+   is the right end's traceback request and d_tb_out[2k+1] the left end's,
+   from the template with fill, r and c set: the upper fill when bestc >= bestr
+   (3575-3633, 3718-3774); a gap with finalscore < 0 gets requests at (0, 0), which push
+   no pairs. */
+int gdp_bridge_intron(const void *d_descs, const gdp_intron_gap *d_gaps, int32_t m, const int16_t *d_scores,
+                      const char *d_gbuf, const double *d_probs, int32_t *d_out, double *d_out_probs,
+                      const gdp_tb_request *d_tb_template, gdp_tb_request *d_tb_out, void *stream);
+
 /* ---- Genome in device memory ------------------------------------------ */
 
 /* Formats of a device genome handed to gdp_traceback_ex and

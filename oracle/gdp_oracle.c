@@ -745,6 +745,149 @@ void orc_bridge_cdna(int width, const short *LU, const short *LL, const short *R
   out[4] = brR;
 }
 
+/* ---------------------------------------------------------- intron bridge */
+
+/* Dinucleotide codes of intron.h:10-21 and the intron types their AND gives
+   (intron.h:26-32): a left site reads the two bases after the left exon, a
+   right site the two before the right exon; either allele may supply each
+   base (dynprog_genome.c:1791-1843). */
+static int left_site(int l1, int l1a, int l2, int l2a) {
+#define EITHER(x, xa, b) ((x) == (b) || (xa) == (b))
+  if (EITHER(l1, l1a, 'G') && EITHER(l2, l2a, 'T')) return 0x21;
+  if (EITHER(l1, l1a, 'G') && EITHER(l2, l2a, 'C')) return 0x10;
+  if (EITHER(l1, l1a, 'A') && EITHER(l2, l2a, 'T')) return 0x08;
+  if (EITHER(l1, l1a, 'C') && EITHER(l2, l2a, 'T')) return 0x06;
+  return 0;
+}
+
+static int right_site(int r2, int r2a, int r1, int r1a) {
+  if (EITHER(r2, r2a, 'A') && EITHER(r1, r1a, 'G')) return 0x30;
+  if (EITHER(r2, r2a, 'A') && EITHER(r1, r1a, 'C')) return 0x0C;
+  if (EITHER(r2, r2a, 'G') && EITHER(r1, r1a, 'C')) return 0x02;
+  if (EITHER(r2, r2a, 'A') && EITHER(r1, r1a, 'T')) return 0x01;
+  return 0;
+#undef EITHER
+}
+
+/* intron_score_setup (dynprog_genome.c:143-186) with the constants of its
+   build (76-115: USE_SCOREI, no USE_WEAK_SCOREI): per intron type the
+   reward of the sense (cdna_direction > 0), antisense (< 0) or either (0)
+   table, final or preliminary.  The either tables keep the reference's
+   values as set, including the preliminary GT-AG forward reward of 16 and
+   the final CT-AC reward of 14. */
+static int intron_reward(int cdna_direction, int finalp, int type) {
+  const int fwd = type == 0x20 || type == 0x10 || type == 0x08;
+  const int rev = type == 0x04 || type == 0x02 || type == 0x01;
+  if (cdna_direction > 0 && !fwd) return 0;
+  if (cdna_direction < 0 && !rev) return 0;
+  if (!fwd && !rev) return 0;
+  if (cdna_direction == 0) {
+    static const int either_final[0x21] = {[0x20] = 16, [0x10] = 10, [0x08] = 8, [0x04] = 14, [0x02] = 10, [0x01] = 8};
+    static const int either_prelim[0x21] = {[0x20] = 16, [0x10] = 8, [0x08] = 4, [0x04] = 14, [0x02] = 8, [0x01] = 4};
+    return finalp ? either_final[type] : either_prelim[type];
+  }
+  const int canon = type == 0x20 || type == 0x04, gcag = type == 0x10 || type == 0x02;
+  if (canon) return finalp ? 16 : 14;
+  if (gcag) return finalp ? 10 : 8;
+  return finalp ? 8 : 4; /* AT-AC */
+}
+
+void orc_bridge_intron(int width, const short *LU, const short *LL, const short *RU, const short *RL,
+                       const char *gL, const char *gLalt, const char *gR, const char *gRalt, const double *probL,
+                       const double *probR, int rlength, int glengthL, int glengthR, int lbandL, int ubandL,
+                       int lbandR, int ubandR, int leftoffset, int rightoffset, int cdna_direction, int finalp,
+                       int halfp, int *out, double *prob_out) {
+  int *ldi = (int *)calloc((size_t)glengthL + 1, sizeof(int));
+  int *rdi = (int *)calloc((size_t)glengthR + 1, sizeof(int));
+  for (int c = 0; c < glengthL - 1; c++) ldi[c] = left_site(gL[c], gLalt[c], gL[c + 1], gLalt[c + 1]);
+  /* the right fill reads its segment backwards: column c is gR[c-1], so the
+     base before the exon at column c is gR[c] and the one before it gR[c+1] */
+  for (int c = 0; c < glengthR - 1; c++) rdi[c] = right_site(gR[c + 1], gRalt[c + 1], gR[c], gRalt[c]);
+
+  const int neg = width == 8 ? -128 : -32768;
+  const int D = rightoffset - leftoffset;
+  int best = neg, brL = -1, brR = -1, bcL = -1, bcR = -1;
+  double bestprob = 0.0;
+  int dbest = neg, drL = -1, drR = -1, dcL = -1, dcR = -1;
+  double dprob = 0.0;
+#define LUS(c, r) LU[(size_t)(c) * (rlength + 1) + (r)]
+#define LLS(r, c) LL[(size_t)(r) * (glengthL + 1) + (c)]
+#define RUS(c, r) RU[(size_t)(c) * (rlength + 1) + (r)]
+#define RLS(r, c) RL[(size_t)(r) * (glengthR + 1) + (c)]
+  /* one candidate: replaces the best on a higher score, or on an equal score
+     with a higher probability sum (1976-1996) */
+#define CANDIDATE(sL, sR)                                                                   \
+  do {                                                                                      \
+    const int v_ = (sL) + intron_reward(cdna_direction, finalp, ldi[cL] & rdi[cR]) + (sR); \
+    const double p_ = probL[cL] + probR[cR];                                                \
+    if (v_ > best || (v_ == best && p_ > bestprob)) {                                       \
+      best = v_;                                                                            \
+      brL = rL;                                                                             \
+      brR = rR;                                                                             \
+      bcL = cL;                                                                             \
+      bcR = cR;                                                                             \
+      bestprob = p_;                                                                        \
+    }                                                                                       \
+  } while (0)
+
+  for (int rL = 1, rR = rlength - 1; rL < rlength; rL++, rR--) {
+    const int cloL = imax(1, rL - lbandL), chighL = imin(rL + ubandL, glengthL - 1);
+    const int cloR = imax(1, rR - lbandR), chighR = imin(rR + ubandR, glengthR - 1);
+    int cL, cR;
+    /* both ends on their diagonals; the only candidates of the dinucleotide
+       track (1958-2012) */
+    cL = rL;
+    cR = rR;
+    CANDIDATE(LUS(cL, rL), RUS(cR, rR));
+    if (intron_reward(cdna_direction, finalp, ldi[cL] & rdi[cR]) > 0 && probL[cL] + probR[cR] > dprob) {
+      dbest = LUS(cL, rL) + intron_reward(cdna_direction, finalp, ldi[cL] & rdi[cR]) + RUS(cR, rR);
+      dcL = cL;
+      dcR = cR;
+      drL = rL;
+      drR = rR;
+      dprob = probL[cL] + probR[cR];
+    }
+    /* left end on its diagonal, right end off it: lower triangle up to the
+       diagonal, then the upper one past it (2014-2111) */
+    cL = rL;
+    for (cR = cloR; cR < rR && cR < D - cL; cR++) CANDIDATE(LUS(cL, rL), RLS(rR, cR));
+    for (cR++; cR < chighR && cR < D - cL; cR++) CANDIDATE(LUS(cL, rL), RUS(cR, rR));
+    /* right end on its diagonal, left end off it (2113-2206) */
+    cR = rR;
+    for (cL = cloL; cL < rL && cL < D - cR; cL++) CANDIDATE(LLS(rL, cL), RUS(cR, rR));
+    for (cL++; cL < chighL && cL < D - cR; cL++) CANDIDATE(LUS(cL, rL), RUS(cR, rR));
+  }
+#undef CANDIDATE
+#undef LUS
+#undef LLS
+#undef RUS
+#undef RLS
+
+  /* the canonical-dinucleotide candidate takes over unless the best already
+     has a probability sum above 2 * PROB_CEILING, none was found, or it
+     scores below 0 or more than 9 under the best (2210-2237) */
+  if (!(bestprob > 2 * 0.85) && dprob != 0.0 && !(dbest < 0 || dbest < best - 9)) {
+    best = dbest;
+    brL = drL;
+    brR = drR;
+    bcL = dcL;
+    bcR = dcR;
+  }
+  int final = best;
+  if (best >= 0 && halfp) final = best - intron_reward(cdna_direction, finalp, ldi[bcL] & rdi[bcR]) / 2;
+  out[0] = final;
+  out[1] = brL;
+  out[2] = brR;
+  out[3] = bcL;
+  out[4] = bcR;
+  /* get_splicesite_probs at the chosen sites (2343-2348, 332-401); halfp
+     takes half the chosen sites' reward off a non-negative score (2248-2255) */
+  prob_out[0] = final >= 0 ? probL[bcL] : 0.0;
+  prob_out[1] = final >= 0 ? probR[bcR] : 0.0;
+  free(ldi);
+  free(rdi);
+}
+
 /* ---------------------------------------------------------- genome segment */
 
 static void revcomp_inplace(char *s, unsigned int n) {

@@ -129,6 +129,26 @@ void orc_bridge_cdna(int width, const short *LU, const short *LL, const short *R
                      int gl, int rlL, int rlR, int lbandL, int ubandL, int lbandR, int ubandR,
                      int open, int extend, int leftoffset, int rightoffset, int jump_late, int *out);
 
+/* bridge_intron_gap_{8,16}_ud (dynprog_genome.c:1387-1488, 2262-2363) on its
+   site-level path, bridge_intron_gap_{8,16}_site_level (866-1386,
+   1741-2261): the best intron between a left pair of one-sided fills
+   (rsequence forward over the left segment) and a right pair (rev_rsequence
+   against the right segment read backwards), the ends' rows summing to
+   rlength.  LU/LL and RU/RL as orc_bridge_cdna, all with rlength rows; gL,
+   gLalt and gR, gRalt are each end's genome in fill order (column c reads
+   g[c-1]: the right segment reversed).  probL / probR hold each column's
+   splice-site probability (the site-level loops' left_probabilities and
+   right_probabilities, 1845-1933); entries up to glength-2 are read, and
+   glengthL, glengthR must exceed rlength (the reference reads past its
+   arrays otherwise).  out = {finalscore, bestrL, bestrR, bestcL, bestcR}
+   (cells -1 when no candidate beats -128 / -32768); prob_out = {left_prob,
+   right_prob}, 0.0 when finalscore < 0. */
+void orc_bridge_intron(int width, const short *LU, const short *LL, const short *RU, const short *RL,
+                       const char *gL, const char *gLalt, const char *gR, const char *gRalt, const double *probL,
+                       const double *probR, int rlength, int glengthL, int glengthR, int lbandL, int ubandL,
+                       int lbandR, int ubandR, int leftoffset, int rightoffset, int cdna_direction, int finalp,
+                       int halfp, int *out, double *prob_out);
+
 /* Genome_get_segment_right / _left (genome.c:11022-11120) over orc_genome_t,
    without SNP alternates (segmentalt == segment). out has length+1 bytes. */
 void orc_segment_right(char *out, const orc_genome_t *genome, unsigned int left,

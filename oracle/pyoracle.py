@@ -67,6 +67,16 @@ def _bridge_cdna_call(fn, width, LU, LL, RU, RL, gl, rlL, rlR, lbandL, ubandL, l
     return tuple(int(x) for x in out)
 
 
+def _bridge_intron_args(width, LU, LL, RU, RL, gL, gLalt, gR, gRalt, rlength, lbandL, ubandL, lbandR, ubandR):
+    gl_L, gl_R = len(gL), len(gR)
+    arrs = [np.ascontiguousarray(a, np.int16) for a in (LU, LL, RU, RL)]
+    assert arrs[0].size == (gl_L + 1) * (rlength + 1) and arrs[1].size == (rlength + 1) * (gl_L + 1)
+    assert arrs[2].size == (gl_R + 1) * (rlength + 1) and arrs[3].size == (rlength + 1) * (gl_R + 1)
+    assert len(gLalt) == gl_L and len(gRalt) == gl_R
+    bufs = [ctypes.create_string_buffer(bytes(x), len(x) + 1) for x in (gL, gLalt, gR, gRalt)]
+    return arrs, bufs, gl_L, gl_R
+
+
 class Oracle(_Common):
     """CPU restatement (gdp_oracle.c)."""
 
@@ -90,6 +100,7 @@ class Oracle(_Common):
         L.orc_best_endpoint.argtypes = [_i, _p, _p, _i, _i, _i, _i, _i, _i, _p, _p, _p]
         L.orc_fill_std.argtypes = [_p, _p, _p, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _p, _p]
         L.orc_bridge_cdna.argtypes = [_i, _p, _p, _p, _p] + [_i] * 12 + [_p]
+        L.orc_bridge_intron.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p] + [_i] * 12 + [_p, _p]
         L.orc_traceback_local.argtypes = [_i, _p, _i, _i, _i, _i, _i, _p, _p, _p, _p, _i, _i, _i, _i, _p, _p, _i, _p]
         L.orc_traceback_local.restype = _i
         L.orc_segment_right.argtypes = [_p, _p, _u, _u, _u, _i]
@@ -230,6 +241,23 @@ class Oracle(_Common):
         """orc_bridge_cdna: (bestscore, bestcL, bestcR, bestrL, bestrR)."""
         return _bridge_cdna_call(self.lib.orc_bridge_cdna, *args)
 
+    def bridge_intron(self, width, LU, LL, RU, RL, gL, gLalt, gR, gRalt, probL, probR, rlength, lbandL, ubandL,
+                      lbandR, ubandR, leftoffset, rightoffset, cdna_direction, finalp, halfp):
+        """orc_bridge_intron.  gL / gR are each end's genome in fill order
+        (the right one reversed); probL / probR per column.  Returns
+        ((finalscore, bestrL, bestrR, bestcL, bestcR), (left_prob, right_prob))."""
+        arrs, bufs, gl_L, gl_R = _bridge_intron_args(width, LU, LL, RU, RL, gL, gLalt, gR, gRalt, rlength,
+                                                     lbandL, ubandL, lbandR, ubandR)
+        pL = np.ascontiguousarray(probL, np.float64)
+        pR = np.ascontiguousarray(probR, np.float64)
+        assert pL.size >= gl_L and pR.size >= gl_R
+        out = np.zeros(5, np.int32)
+        pr = np.zeros(2, np.float64)
+        self.lib.orc_bridge_intron(int(width), *[_np_ptr(a) for a in arrs], *bufs, _np_ptr(pL), _np_ptr(pR),
+                                   rlength, gl_L, gl_R, lbandL, ubandL, lbandR, ubandR, leftoffset, rightoffset,
+                                   int(cdna_direction), int(finalp), int(halfp), _np_ptr(out), _np_ptr(pr))
+        return tuple(int(x) for x in out), (float(pr[0]), float(pr[1]))
+
     def segment_right(self, genome, left, length, chrhigh, revcomp):
         out = ctypes.create_string_buffer(length + 1)
         self.lib.orc_segment_right(out, self._genome_struct(genome), left, length, chrhigh, int(revcomp))
@@ -280,6 +308,10 @@ class Reference(_Common):
         L.refe_end5_splicejunction.restype = _i
         L.refe_end3_splicejunction.argtypes = L.refe_end5_splicejunction.argtypes
         L.refe_end3_splicejunction.restype = _i
+        L.refg_setup.argtypes = [_p, _u]
+        L.refg_site_probs.argtypes = [_p, _p, _i, _i, _u, _u, _u, _u, _i, _i]
+        L.refg_bridge_intron.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p, _p] + [_i] * 9 + [_u] * 4 + [_i] * 3 + \
+            [_p, _p]
         L.refh_init(mode)
         self.ws_single = L.refh_new(max_rl, max_gl, 0)
         self.ws_double = L.refh_new(max_rl, max_gl, 1)
@@ -384,6 +416,37 @@ class Reference(_Common):
     def bridge_cdna(self, *args):
         """The reference's bridge_cdna_gap_{8,16}_ud, same arguments as Oracle.bridge_cdna."""
         return _bridge_cdna_call(self.lib.refc_bridge_cdna, *args)
+
+    def genome_setup(self, image, nbases):
+        """Dynprog_genome_setup without a splicing IIT, and the genomecomp
+        image the splice-site models read (kept alive here)."""
+        self._genome_image = np.ascontiguousarray(image, np.uint32)
+        self.lib.refg_setup(_np_ptr(self._genome_image), ctypes.c_uint(int(nbases)))
+
+    def site_probs(self, glengthL, glengthR, leftoffset, rightoffset, chroffset, chrhigh, cdna_direction, watsonp):
+        """The per-column splice-site probabilities the site-level bridge
+        computes (Maxent_hr over the genome of genome_setup)."""
+        left = np.zeros(glengthL, np.float64)
+        right = np.zeros(glengthR, np.float64)
+        self.lib.refg_site_probs(_np_ptr(left), _np_ptr(right), glengthL, glengthR, leftoffset, rightoffset,
+                                 chroffset, chrhigh, int(cdna_direction), int(watsonp))
+        return left, right
+
+    def bridge_intron(self, width, LU, LL, RU, RL, gL, gLalt, gRseg, gRsegalt, rlength, lbandL, ubandL, lbandR,
+                      ubandR, leftoffset, rightoffset, chroffset, chrhigh, cdna_direction, watsonp, finalp, halfp,
+                      late):
+        """The reference's bridge_intron_gap_{8,16}_ud.  gRseg is the right
+        segment in genome order (the bridge reads it backwards).  Returns as
+        Oracle.bridge_intron."""
+        arrs, bufs, gl_L, gl_R = _bridge_intron_args(width, LU, LL, RU, RL, gL, gLalt, gRseg, gRsegalt, rlength,
+                                                     lbandL, ubandL, lbandR, ubandR)
+        out = np.zeros(5, np.int32)
+        pr = np.zeros(2, np.float64)
+        self.lib.refg_bridge_intron(int(width), *[_np_ptr(a) for a in arrs], *bufs, rlength, gl_L, gl_R,
+                                    int(cdna_direction), int(watsonp), lbandL, ubandL, lbandR, ubandR, leftoffset,
+                                    rightoffset, chroffset, chrhigh, int(halfp), int(finalp), int(late),
+                                    _np_ptr(out), _np_ptr(pr))
+        return tuple(int(x) for x in out), (float(pr[0]), float(pr[1]))
 
     def traceback(self, kind, dirs, r, c, rseq, rsequc, gseq, galt, queryoffset=0, genomeoffset=0,
                   revp=False, chroffset=0, chrhigh=0, watsonp=True, genestrand=0, genome=None,

@@ -153,3 +153,37 @@ def skip_kinds(pairs):
     sk = pairs[(pairs[:, 2] == ord(" ")) & (pairs[:, 3] == ord("-"))]
     star = int((sk[:, 4] == ord("*")).sum())
     return len(sk) - star, star, int((pairs[:, 6] == 1).sum())
+
+
+INTRON_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "intron_bridge_golden.npz")
+INTRON_FIELDS = ("width", "mt", "open", "ext", "rlength", "glengthL", "glengthR", "lbandL", "ubandL", "lbandR",
+                 "ubandR", "leftoffset", "rightoffset", "chroffset", "chrhigh", "watsonp", "cdna_direction",
+                 "finalp", "halfp", "late")
+
+
+def load_intron_bridge():
+    """Intron bridge vectors (make_intron_bridge_golden.py): a list of
+    (case dict as helpers.intron_gap_cases makes it, plus "probL"/"probR",
+    the reference's (finalscore, bestrL, bestrR, bestcL, bestcR) and its
+    (left_prob, right_prob))."""
+    z = np.load(INTRON_PATH, allow_pickle=False)
+    g = {k: z[k] for k in z.files}
+    out = []
+    qo = go = po_ = 0
+    for i in range(len(g["out"])):
+        c = {k: int(g[k][i]) for k in INTRON_FIELDS}
+        for k in ("watsonp", "finalp", "halfp", "late"):
+            c[k] = bool(c[k])
+        rl, glL, glR = c["rlength"], c["glengthL"], c["glengthR"]
+        c["q"] = bytes(g["qbuf"][qo:qo + rl])
+        c["gL"] = bytes(g["gbuf"][go:go + glL])
+        c["gRseg"] = bytes(g["gbuf"][go + glL:go + glL + glR])
+        c["gLalt"] = bytes(g["gbuf"][go + glL + glR:go + 2 * glL + glR])
+        c["gRsegalt"] = bytes(g["gbuf"][go + 2 * glL + glR:go + 2 * glL + 2 * glR])
+        c["probL"] = g["site_probs"][po_:po_ + glL]
+        c["probR"] = g["site_probs"][po_ + glL:po_ + glL + glR]
+        qo += rl
+        go += 2 * (glL + glR)
+        po_ += glL + glR
+        out.append((c, tuple(int(x) for x in g["out"][i]), tuple(float(x) for x in g["out_probs"][i])))
+    return out

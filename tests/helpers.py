@@ -1,4 +1,6 @@
 """Shared generators and comparison helpers for the tests."""
+import os
+
 import numpy as np
 
 import pyoracle as po
@@ -424,3 +426,128 @@ def caller_cases(rng, genome_seq, n_single, n_end, chromosomes=2):
                 "endalign": int(rng.choice([3, 0, 1, 2], p=[0.4, 0.4, 0.1, 0.1])) if not single else 0}
         cases.append(case)
     return cases
+
+
+def intron_genome():
+    """The intron-bridge genome: the reference's test chromosome followed by
+    6000 bases of a 12-base repeat holding GT, AG, CT and AC, where equal
+    splice-site contexts give equal probabilities (ties the bridge breaks by
+    scan order).  ASCII and genomecomp."""
+    import gzip
+    import genomecomp as GC
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "chr17test.fa.gz")
+    fa = gzip.open(path).read().decode()
+    seq = "".join(line.strip() for line in fa.splitlines() if not line.startswith(">")).encode()
+    seq += b"GTAAGCTACAGT" * 500
+    return seq, GC.pack(seq)
+
+
+_SITES = {1: (b"GT", b"AG"), -1: (b"CT", b"AC"), 0: (b"GT", b"AG")}
+
+
+def intron_gap_cases(rng, n, genome_seq, repeat_start=200000, long_reads=False):
+    """Intron gaps set up as Dynprog_genome_gap does (dynprog_genome.c:
+    3287-3547, called from stage3.c:9567 with glengthL = glengthR = queryjump +
+    extramaterial_paired): a read made of the end of a left exon and the start
+    of a right exon (2 % substitutions, 0.5 % indels), the left segment from
+    goffsetL forward and the right segment ending at rev_goffsetR, both
+    longer than the read, on either strand (the minus strand's coordinate x
+    reads the complement of base chrhigh - x).  Exon ends sit before the
+    canonical dinucleotides of cdna_direction (+1 GT..AG, -1 CT..AC, 0 either)
+    in most cases; introns of 1-20 bases, and segments that overlap, make
+    the bridge's rightoffset-leftoffset bound bite; some gaps lie in the repeat after
+    repeat_start; one read in ten has 60 % substitutions (joins that score
+    below 0); one gap in four has SNP alleles on 5 % of its segments' bases.
+    long_reads draws reads of 1000-1499 bases instead.  Widths follow the caller's use8p rule (3504-3508);
+    penalties PAIRED_*_{HIGHQ,MEDQ,LOWQ} (dynprog.h:69-75).  Returns dicts
+    with the bridge's arguments and the read."""
+    nb = len(genome_seq)
+    G = np.frombuffer(genome_seq, np.uint8)
+    comp = np.frombuffer(bytes.maketrans(b"ACGTN", b"TGCAN"), np.uint8)
+    use8p = {0: 41, 1: 63, 2: 127}
+    pen = {0: (-8, -3), 1: (-7, -2), 2: (-6, -1)}
+    out = []
+    for t in range(n):
+        watsonp = bool(t % 2 == 0) if t < 8 else bool(rng.integers(0, 2))
+        cdna_direction = [1, -1, 0][t % 3]
+        u = rng.random()
+        rlength = int(rng.integers(2, 40)) if u < 0.3 else (int(rng.integers(40, 200)) if u < 0.93 else
+                                                            int(rng.integers(200, 520)))
+        if long_reads:
+            rlength = int(rng.integers(1000, 1500))
+        extra = int(rng.integers(1, 26))
+        glength = rlength + extra
+        intron = int(rng.integers(1, 21)) if t % 6 == 5 else int(rng.integers(30, 3000))
+        if t % 12 == 11:
+            intron = -int(rng.integers(0, 12))  # segments that overlap (rightoffset-leftoffset < rlength)
+        in_repeat = t % 9 == 4
+        chroffset, chrhigh = (int(rng.integers(0, 500)), nb - int(rng.integers(64, 400)))
+        span = chrhigh - chroffset
+
+        def S(x0, length):
+            if watsonp:
+                return G[chroffset + x0: chroffset + x0 + length].tobytes()
+            idx = chrhigh - np.arange(x0, x0 + length)
+            return comp[G[idx]].tobytes()
+
+        a = int(rng.integers(1, rlength)) if rlength > 1 else 1
+        b = rlength - a
+        if in_repeat:
+            intron = min(intron, 300)
+        # room for the segments, the dinucleotide search (200 bases each side)
+        # and the splice models' context
+        lo = 200 + glength
+        hi = span - intron - glength - 400
+        if in_repeat and watsonp:
+            lo = repeat_start - chroffset + 200 + glength
+        elif in_repeat:  # the minus strand meets the repeat at its start
+            hi = chrhigh - repeat_start - intron - glength - 400
+        e1 = int(rng.integers(lo, hi))  # last base of the left exon
+        if rng.random() < 0.8:
+            want = _SITES[cdna_direction][0]
+            for d in range(200):
+                if S(e1 + 1 + d, 2) == want:
+                    e1 += d
+                    break
+        i2 = e1 + 1 + intron  # first base of the right exon
+        if rng.random() < 0.8 and intron >= 4:
+            want = _SITES[cdna_direction][1]
+            for d in range(200):
+                if S(i2 - 2 + d, 2) == want:
+                    i2 += d
+                    break
+        goffsetL = e1 + 1 - a
+        rev_goffsetR = i2 + b - 1
+        gL = S(goffsetL, glength)
+        gRseg = S(rev_goffsetR + 1 - glength, glength)
+        src = S(goffsetL, a) + S(rev_goffsetR + 1 - b, b)
+        sub = 0.6 if t % 10 == 7 else 0.02  # some reads too poor for a positive join
+        q = mutate(rng, src, sub, 0.005, 3)[:rlength].ljust(rlength, b"C")
+        gLalt, gRsegalt = gL, gRseg
+        if t % 4 == 3:  # SNP alleles: the fills score them, the sites read either allele
+            gLalt = bytes(ch if rng.random() > 0.05 else int(ALPH[rng.integers(0, 4)]) for ch in gL)
+            gRsegalt = bytes(ch if rng.random() > 0.05 else int(ALPH[rng.integers(0, 4)]) for ch in gRseg)
+        mt = int(rng.integers(0, 3))
+        width = 8 if rlength < use8p[mt] or glength < use8p[mt] else 16
+        eb = int(rng.integers(2, 13))
+        lb, ub = eb, glength - rlength + eb  # Dynprog_compute_bands, widebandp (dynprog.c:1245-1261)
+        out.append({"width": width, "mt": mt, "open": pen[mt][0], "ext": pen[mt][1], "q": q, "gL": gL,
+                    "gRseg": gRseg, "gLalt": gLalt, "gRsegalt": gRsegalt, "rlength": rlength, "glengthL": glength, "glengthR": glength, "lbandL": lb,
+                    "ubandL": ub, "lbandR": lb, "ubandR": ub, "leftoffset": goffsetL, "rightoffset": rev_goffsetR,
+                    "chroffset": chroffset, "chrhigh": chrhigh, "watsonp": watsonp,
+                    "cdna_direction": cdna_direction, "finalp": bool(rng.integers(0, 2)),
+                    "halfp": bool(t % 4 == 1), "late": bool(rng.integers(0, 2))})
+    return out
+
+
+def intron_gap_specs(c):
+    """The four one-sided fills of an intron gap (dynprog_genome.c:3510-3547):
+    left upper and lower over the read forward against the left segment,
+    right upper and lower over the read backwards against the right segment
+    read backwards, with the other tie rule."""
+    q, gL, gLa, gRf, gRfa = c["q"], c["gL"], c["gLalt"], c["gRseg"][::-1], c["gRsegalt"][::-1]
+    w, mt, op, ex = c["width"], c["mt"], c["open"], c["ext"]
+    return [Spec(1, w, q, gL, gLa, mt, op, ex, c["lbandL"], c["ubandL"], c["late"], False),
+            Spec(2, w, q, gL, gLa, mt, op, ex, c["lbandL"], c["ubandL"], c["late"], False),
+            Spec(1, w, q[::-1], gRf, gRfa, mt, op, ex, c["lbandR"], c["ubandR"], not c["late"], False),
+            Spec(2, w, q[::-1], gRf, gRfa, mt, op, ex, c["lbandR"], c["ubandR"], not c["late"], False)]
