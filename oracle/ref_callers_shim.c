@@ -2,8 +2,9 @@
  * ref_callers_shim.c -- the reference's own DP callers over plain arrays.
  *
  * TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles the reference's
- * dynprog_single.c and dynprog_end.c in place together with dynprog.c,
- * pairpool.c, genome.c and their helpers, twice:
+ * dynprog_single.c, dynprog_end.c and dynprog_genome.c (through
+ * ref_genome_shim.c) in place together with dynprog.c, pairpool.c, genome.c,
+ * maxent_hr.c, pair.c and their helpers, twice:
  *   _ref/libgmapref_callers.so  with the reference's dynprog_simd.o (AVX2);
  *   _ref/libgmapref_callers_gpu.so with libgmapdp_dynprog.so in place of
  *                               dynprog_simd.o (the drop-in, on the GPU).
@@ -17,6 +18,7 @@
 
 #include "dynprog.h"
 #include "dynprog_end.h"
+#include "dynprog_genome.h"
 #include "dynprog_single.h"
 #include "genome.h"
 #include "list.h"
@@ -27,13 +29,14 @@ static Pairpool_T pool = NULL;
 static struct Genome_T genome_s;
 static Genome_T genome = NULL;
 
-/* Tables, caller settings (no user penalties, no homopolymer mode, no known
-   splice sites) and one genome: the genomecomp image `blocks` of `length`
+/* Tables, caller settings (no user penalties, no homopolymer mode, novel
+   splicing without a splicing IIT) and one genome: the genomecomp image `blocks` of `length`
    bases, kept by the caller while calls run. */
 void refc_init(int mode, const unsigned int *blocks, unsigned int length) {
   Dynprog_init((Mode_T)mode);
   Dynprog_single_setup(/*user_open*/ 0, /*user_extend*/ 0, /*user_dynprog_p*/ false, /*homopolymerp*/ false);
   Dynprog_end_setup(NULL, NULL, NULL, 0, NULL, NULL, NULL, NULL, 0, 0, false);
+  Dynprog_genome_setup(/*novelsplicingp*/ true, /*splicing_iit*/ NULL, NULL, -1, -1, 0, 0, false);
   if (pool == NULL) pool = Pairpool_new();
   memset(&genome_s, 0, sizeof genome_s);
   genome_s.blocks = (Genomecomp_T *)blocks;
@@ -100,5 +103,31 @@ int refc_end_gap(int five, void *ws, char *rseq, char *rsequc, int rl, int gl, i
                              genome, genome, pool, extraband, defect_rate, (Endalign_T)endalign,
                              /*require_pos_score_p*/ false);
   counters[0] = idx; counters[1] = sc; counters[2] = nm; counters[3] = nmm; counters[4] = no; counters[5] = ni;
+  return pairs == NULL ? -1 : flatten(pairs, out, maxpairs);
+}
+
+/* Dynprog_genome_gap (dynprog_genome.c:3287-3770) with two workspaces, as
+   stage 3 calls it (stage3.c:9567-9576): rseq points at the gap's first read
+   base.  counters = {dynprogindex after the call, new_leftgenomepos,
+   new_rightgenomepos, traceback_score, nmatches, nmismatches, nopens,
+   nindels, exonhead, introntype}; probs = {left_prob, right_prob}.  Returns
+   the number of pairs of the returned list (its gap holder carries the
+   intron), -1 for NULL. */
+int refc_genome_gap(void *wsL, void *wsR, char *rseq, char *rsequc, int rl, int glL, int glR, int roffset,
+                    int goffsetL, int rev_goffsetR, unsigned int chroffset, unsigned int chrhigh, int cdna_direction,
+                    int watsonp, int genestrand, int jump_late, int extraband, double defect_rate, int maxpeelback,
+                    int halfp, int finalp, int *counters, double *probs, int *out, int maxpairs) {
+  int idx = 1, nl = 0, nr = 0, sc = 0, nm = 0, nmm = 0, no = 0, ni = 0, eh = 0, it = 0;
+  double lp = 0.0, rp = 0.0;
+  Pairpool_reset(pool);
+  List_T pairs = Dynprog_genome_gap(&idx, &nl, &nr, &lp, &rp, &sc, &nm, &nmm, &no, &ni, &eh, &it, (Dynprog_T)wsL,
+                                    (Dynprog_T)wsR, rseq, rsequc, rl, glL, glR, roffset, goffsetL, rev_goffsetR,
+                                    /*chrnum*/ 1, chroffset, chrhigh, cdna_direction, watsonp ? true : false,
+                                    genestrand, jump_late ? true : false, genome, genome, pool, extraband,
+                                    defect_rate, maxpeelback, halfp ? true : false, finalp ? true : false);
+  counters[0] = idx; counters[1] = nl; counters[2] = nr; counters[3] = sc; counters[4] = nm;
+  counters[5] = nmm; counters[6] = no; counters[7] = ni; counters[8] = eh; counters[9] = it;
+  probs[0] = lp;
+  probs[1] = rp;
   return pairs == NULL ? -1 : flatten(pairs, out, maxpairs);
 }

@@ -4,7 +4,9 @@ Dynprog_end5_gap, Dynprog_end3_gap, through oracle/ref_callers_shim.c) from
 one of their two builds, over the genome-skip test genome, and writes the
 returned pair lists and counters.
 
-    python3 tests/callers_run.py {ref|gpu|ref_large|gpu_large} CASES.npz OUT.npz
+    python3 tests/callers_run.py {ref|gpu|ref_large|gpu_large} CASES.npz OUT.npz [genome]
+
+With `genome` the cases are Dynprog_genome_gap calls (save_genome_cases).
 
 ref: oracle/_ref/libgmapref_callers.so (the reference's SIMD fills);
 gpu: oracle/_ref/libgmapref_callers_gpu.so (the same callers over the
@@ -81,6 +83,61 @@ def run(which, cases_path, out_path):
                         pairs=np.concatenate(pairs) if pairs else np.zeros((0, 8), np.int32))
 
 
+GENOME_KEYS = ("rl", "glL", "glR", "roffset", "goffsetL", "rev_goffsetR", "chroffset", "chrhigh", "cdna_direction",
+               "watsonp", "genestrand", "jump_late", "extraband", "maxpeelback", "halfp", "finalp")
+
+
+def run_genome(which, cases_path, out_path):
+    """Dynprog_genome_gap calls (helpers.genome_gap_calls) through refc_genome_gap."""
+    import golden_io
+    L = ctypes.CDLL(callers_lib(which), mode=os.RTLD_LAZY)
+    _p, _i, _u, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_double
+    L.refc_init.argtypes = [_i, _p, _u]
+    L.refc_new.argtypes = [_i] * 5
+    L.refc_new.restype = _p
+    L.refc_genome_gap.argtypes = [_p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _u, _u, _i, _i, _i, _i, _i, _d, _i, _i, _i,
+                                  _p, _p, _p, _i]
+    seq, gc = golden_io.skip_genome_image()
+    L.refc_init(0, gc.ctypes.data, len(seq))
+    wsL = L.refc_new(60, 50, 20, 0, 1)
+    wsR = L.refc_new(60, 50, 20, 0, 1)
+    z = np.load(cases_path, allow_pickle=False)
+    c = {k: z[k] for k in z.files}
+    n = len(c["rl"])
+    npairs = np.zeros(n, np.int64)
+    counters = np.zeros((n, 10), np.int32)
+    probs = np.zeros((n, 2), np.float64)
+    pairs = []
+    out = np.zeros((MAXPAIRS, 8), np.int32)
+    for k in range(n):
+        q = bytes(c["rbuf"][c["roff"][k]:c["roff"][k] + c["rl"][k]]) + b"\0"
+        qb, qub = ctypes.create_string_buffer(q, len(q)), ctypes.create_string_buffer(q.upper(), len(q))
+        cnt = np.zeros(10, np.int32)
+        pr = np.zeros(2, np.float64)
+        a = [int(c[key][k]) for key in GENOME_KEYS]
+        # GENOME_KEYS in the call's order, defect_rate between extraband and maxpeelback
+        m = L.refc_genome_gap(wsL, wsR, qb, qub, *a[:13], float(c["defect_rate"][k]), *a[13:], cnt.ctypes.data,
+                              pr.ctypes.data, out.ctypes.data, MAXPAIRS)
+        npairs[k] = m
+        counters[k] = cnt
+        probs[k] = pr
+        if m > 0:
+            pairs.append(out[:m].copy())
+        if k % max(1, n // 4) == 0:
+            print("callers_run %s genome: %d of %d" % (which, k, n), flush=True)
+    np.savez_compressed(out_path, npairs=npairs, counters=counters, probs=probs,
+                        pairs=np.concatenate(pairs) if pairs else np.zeros((0, 8), np.int32))
+
+
+def save_genome_cases(path, cases):
+    """The cases of helpers.genome_gap_calls as arrays."""
+    arr = {k: np.array([int(c[k]) for c in cases], np.int64) for k in GENOME_KEYS}
+    arr["defect_rate"] = np.array([c["defect_rate"] for c in cases], np.float64)
+    arr["roff"] = np.concatenate([[0], np.cumsum(arr["rl"])[:-1]]).astype(np.int64)
+    arr["rbuf"] = np.frombuffer(b"".join(c["rseq"] for c in cases), np.uint8)
+    np.savez_compressed(path, **arr)
+
+
 def save_cases(path, cases):
     """The cases of helpers.caller_cases as arrays."""
     keys = ("single", "five", "rl", "gl", "roffset", "goffset", "chroffset", "chrhigh", "watsonp", "genestrand",
@@ -93,4 +150,4 @@ def save_cases(path, cases):
 
 
 if __name__ == "__main__":
-    run(sys.argv[1], sys.argv[2], sys.argv[3])
+    (run_genome if sys.argv[4:5] == ["genome"] else run)(sys.argv[1], sys.argv[2], sys.argv[3])

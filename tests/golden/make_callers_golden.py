@@ -3,7 +3,14 @@
 callers Dynprog_single_gap (dynprog_single.c:428-675), Dynprog_end5_gap
 (dynprog_end.c:1293-1650) and Dynprog_end3_gap (1923-2243) over the
 reference's SIMD fills, with the genome-skip test genome (the reference's
-test chromosome with N runs) cut into two chromosomes.
+test chromosome with N runs) cut into two chromosomes; and
+tests/golden/callers_genome_golden.npz: 600 calls of Dynprog_genome_gap
+(dynprog_genome.c:3287-3770) from helpers.genome_gap_calls (seed 1002) over
+the same genome, with their pair lists (the gap holder carries the intron),
+counters {dynprogindex, new_leftgenomepos, new_rightgenomepos,
+traceback_score, nmatches, nmismatches, nopens, nindels, exonhead,
+introntype} and splice-site probabilities {left_prob, right_prob}.  With
+the argument `genome` only the second file is written.
 
 Run in a container where the reference tree is mounted, after
 `make -C oracle ref`, which builds oracle/_ref/libgmapref_callers.so from
@@ -30,11 +37,14 @@ import golden_io  # noqa: E402
 import helpers as H  # noqa: E402
 
 OUT = os.path.join(HERE, "callers_golden.npz")
-SEED = 1001
+OUT_GENOME = os.path.join(HERE, "callers_genome_golden.npz")
+SEED, SEED_GENOME, N_GENOME = 1001, 1002, 600
 
 
 def main():
     seq, _ = golden_io.skip_genome_image()
+    if sys.argv[1:] == ["genome"]:
+        return main_genome(seq)
     cases = H.caller_cases(np.random.default_rng(SEED), seq, 2000, 1000)
     with tempfile.TemporaryDirectory() as d:
         cpath, opath = os.path.join(d, "cases.npz"), os.path.join(d, "out.npz")
@@ -43,6 +53,18 @@ def main():
         z, o = np.load(cpath), np.load(opath)
         np.savez_compressed(OUT, **{k: z[k] for k in z.files}, **{"out_" + k: o[k] for k in o.files})
     print("wrote %s: %d calls" % (OUT, len(cases)))
+    main_genome(seq)
+
+
+def main_genome(seq):
+    gcases = H.genome_gap_calls(np.random.default_rng(SEED_GENOME), seq, N_GENOME)
+    with tempfile.TemporaryDirectory() as d:
+        cpath, opath = os.path.join(d, "cases.npz"), os.path.join(d, "out.npz")
+        callers_run.save_genome_cases(cpath, gcases)
+        callers_run.run_genome("ref", cpath, opath)
+        z, o = np.load(cpath), np.load(opath)
+        np.savez_compressed(OUT_GENOME, **{k: z[k] for k in z.files}, **{"out_" + k: o[k] for k in o.files})
+    print("wrote %s: %d calls" % (OUT_GENOME, len(gcases)))
 
 
 if __name__ == "__main__":

@@ -551,3 +551,65 @@ def intron_gap_specs(c):
             Spec(2, w, q, gL, gLa, mt, op, ex, c["lbandL"], c["ubandL"], c["late"], False),
             Spec(1, w, q[::-1], gRf, gRfa, mt, op, ex, c["lbandR"], c["ubandR"], not c["late"], False),
             Spec(2, w, q[::-1], gRf, gRfa, mt, op, ex, c["lbandR"], c["ubandR"], not c["late"], False)]
+
+
+def genome_gap_calls(rng, genome_seq, n, chromosomes=2):
+    """Calls of Dynprog_genome_gap as stage 3 makes them (stage3.c:9538-9576:
+    glengthL = glengthR = queryjump + extramaterial_paired, goffsetL =
+    genomedp5, rev_goffsetR = genomedp3): a read of 10-300 bases made of the
+    end of a left exon and the start of a right exon across an intron of
+    30-4000 bases (1 % substitutions, 0.2 % indels of 1-2 bases, each exon
+    part keeping its anchor; one read in twelve unrelated to the genome), exon ends before the canonical dinucleotides of
+    cdna_direction in most calls, on either strand of one of `chromosomes`
+    equal chromosomes, extramaterial 6-30, extraband 3-10, all three defect
+    classes (so both widths and, for preliminary calls under DEFECT_MEDQ,
+    genome_gap_simple), finalp and halfp both ways; a few reads past the
+    workspace's 500 rows (chopped by the caller).  Dicts of the call's
+    arguments."""
+    nb = len(genome_seq)
+    clen = nb // chromosomes
+    G = np.frombuffer(genome_seq, np.uint8)
+    comp = np.frombuffer(bytes.maketrans(b"ACGTN", b"TGCAN"), np.uint8)
+    out = []
+    for t in range(n):
+        k = int(rng.integers(0, chromosomes))
+        chroffset, chrhigh = k * clen, k * clen + clen - 1
+        watsonp = bool(rng.integers(0, 2))
+        cdna_direction = [1, -1, 0][t % 3]
+        rl = int(rng.integers(10, 300)) if t % 40 else int(rng.integers(501, 520))
+        extra = int(rng.integers(6, 31))
+        gl = rl + extra
+        intron = int(rng.integers(30, 4000))
+
+        def S(x0, length):
+            if watsonp:
+                return G[chroffset + x0: chroffset + x0 + length].tobytes()
+            return comp[G[chrhigh - np.arange(x0, x0 + length)]].tobytes()
+
+        a = int(rng.integers(1, rl))
+        b = rl - a
+        e1 = int(rng.integers(gl + 300, clen - intron - gl - 700))
+        if rng.random() < 0.8:
+            for d in range(200):
+                if S(e1 + 1 + d, 2) == _SITES[cdna_direction][0]:
+                    e1 += d
+                    break
+        i2 = e1 + 1 + intron
+        if rng.random() < 0.8:
+            for d in range(200):
+                if S(i2 - 2 + d, 2) == _SITES[cdna_direction][1]:
+                    i2 += d
+                    break
+        goffsetL, rev_goffsetR = e1 + 1 - a, i2 + b - 1
+        # each exon part keeps its anchor: the left from the read's start,
+        # the right at its end (the gap's two ends are aligned bases)
+        left = mutate(rng, S(goffsetL, a), 0.01, 0.002, 2)[:a].ljust(a, b"A")
+        right = mutate(rng, S(rev_goffsetR + 1 - b, b), 0.01, 0.002, 2)[-b:].rjust(b, b"A") if b else b""
+        q = left + right if t % 12 != 5 else rand_seq(rng, rl)
+        out.append({"rseq": q, "rl": rl, "glL": gl, "glR": gl, "roffset": int(rng.integers(0, 50)),
+                    "goffsetL": goffsetL, "rev_goffsetR": rev_goffsetR, "chroffset": chroffset, "chrhigh": chrhigh,
+                    "cdna_direction": cdna_direction, "watsonp": watsonp, "genestrand": 0,
+                    "jump_late": bool(rng.integers(0, 2)), "extraband": int(rng.integers(3, 11)),
+                    "defect_rate": float(rng.choice([0.001, 0.005, 0.02])), "maxpeelback": 20,
+                    "halfp": bool(t % 5 == 2), "finalp": bool(rng.integers(0, 2))})
+    return out

@@ -4,7 +4,7 @@ exists: the harness that tests/test_gpu_callers.py drives over the drop-in
 gives, over the reference's SIMD fills, the committed results."""
 import pytest
 
-from test_gpu_callers import _compare, _run
+from test_gpu_callers import _compare, _compare_genome, _run
 
 pytestmark = pytest.mark.ref
 
@@ -20,3 +20,13 @@ def test_large_genome_reference_callers_reproduce_golden(reference, tmp_path):
     z, o = _run("ref_large", tmp_path)
     _compare(z, o)
 
+
+
+def test_reference_genome_gap_reproduces_golden(reference, tmp_path):
+    z, o = _run("ref", tmp_path, genome=True)
+    _compare_genome(z, o)
+
+
+def test_large_genome_reference_genome_gap_reproduces_golden(reference, tmp_path):
+    z, o = _run("ref_large", tmp_path, genome=True)
+    _compare_genome(z, o)
