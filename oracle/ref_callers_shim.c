@@ -2,9 +2,10 @@
  * ref_callers_shim.c -- the reference's own DP callers over plain arrays.
  *
  * TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles the reference's
- * dynprog_single.c, dynprog_end.c and dynprog_genome.c (through
- * ref_genome_shim.c) in place together with dynprog.c, pairpool.c, genome.c,
- * maxent_hr.c, pair.c and their helpers, twice:
+ * dynprog_single.c, dynprog_end.c, dynprog_cdna.c and dynprog_genome.c
+ * (the last two through ref_cdna_shim.c and ref_genome_shim.c) in place
+ * together with dynprog.c, pairpool.c, genome.c, maxent_hr.c, pair.c and
+ * their helpers, twice:
  *   _ref/libgmapref_callers.so  with the reference's dynprog_simd.o (AVX2);
  *   _ref/libgmapref_callers_gpu.so with libgmapdp_dynprog.so in place of
  *                               dynprog_simd.o (the drop-in, on the GPU).
@@ -17,6 +18,7 @@
 #include <string.h>
 
 #include "dynprog.h"
+#include "dynprog_cdna.h"
 #include "dynprog_end.h"
 #include "dynprog_genome.h"
 #include "dynprog_single.h"
@@ -129,5 +131,27 @@ int refc_genome_gap(void *wsL, void *wsR, char *rseq, char *rsequc, int rl, int 
   counters[5] = nmm; counters[6] = no; counters[7] = ni; counters[8] = eh; counters[9] = it;
   probs[0] = lp;
   probs[1] = rp;
+  return pairs == NULL ? -1 : flatten(pairs, out, maxpairs);
+}
+
+/* Dynprog_cdna_gap (dynprog_cdna.c:786-1180) with two workspaces, as stage 3
+   calls it (stage3.c:9310-9321): the left end reads query[qdp5 ...] forward
+   and the right end query[qdp3] backwards, rlengthL and rlengthR bases, over
+   the glength genome columns from goffset.  counters = {dynprogindex after
+   the call, traceback_score, incompletep}.  Returns the number of pairs of
+   the returned list, -1 for NULL. */
+int refc_cdna_gap(void *wsL, void *wsR, char *query, char *queryuc, int qdp5, int qdp3, int rlengthL, int rlengthR,
+                  int glength, int goffset, unsigned int chroffset, unsigned int chrhigh, int watsonp, int genestrand,
+                  int jump_late, int extraband, double defect_rate, int *counters, int *out, int maxpairs) {
+  int idx = 1, sc = 0;
+  bool incomplete = false;
+  Pairpool_reset(pool);
+  List_T pairs = Dynprog_cdna_gap(&idx, &sc, &incomplete, (Dynprog_T)wsL, (Dynprog_T)wsR, &query[qdp5],
+                                  &queryuc[qdp5], &query[qdp3], &queryuc[qdp3], rlengthL, rlengthR, glength, qdp5,
+                                  qdp3, goffset, chroffset, chrhigh, watsonp ? true : false, genestrand,
+                                  jump_late ? true : false, genome, genome, pool, extraband, defect_rate);
+  counters[0] = idx;
+  counters[1] = sc;
+  counters[2] = incomplete ? 1 : 0;
   return pairs == NULL ? -1 : flatten(pairs, out, maxpairs);
 }

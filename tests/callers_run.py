@@ -6,7 +6,8 @@ returned pair lists and counters.
 
     python3 tests/callers_run.py {ref|gpu|ref_large|gpu_large} CASES.npz OUT.npz [genome]
 
-With `genome` the cases are Dynprog_genome_gap calls (save_genome_cases).
+With `genome` the cases are Dynprog_genome_gap calls (save_genome_cases),
+with `cdna` Dynprog_cdna_gap calls (save_cdna_cases).
 
 ref: oracle/_ref/libgmapref_callers.so (the reference's SIMD fills);
 gpu: oracle/_ref/libgmapref_callers_gpu.so (the same callers over the
@@ -129,6 +130,57 @@ def run_genome(which, cases_path, out_path):
                         pairs=np.concatenate(pairs) if pairs else np.zeros((0, 8), np.int32))
 
 
+CDNA_KEYS = ("qdp5", "qdp3", "rlengthL", "rlengthR", "glength", "goffset", "chroffset", "chrhigh", "watsonp",
+             "genestrand", "jump_late", "extraband")
+
+
+def run_cdna(which, cases_path, out_path):
+    """Dynprog_cdna_gap calls (helpers.cdna_gap_calls) through refc_cdna_gap."""
+    import golden_io
+    L = ctypes.CDLL(callers_lib(which), mode=os.RTLD_LAZY)
+    _p, _i, _u, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_double
+    L.refc_init.argtypes = [_i, _p, _u]
+    L.refc_new.argtypes = [_i] * 5
+    L.refc_new.restype = _p
+    L.refc_cdna_gap.argtypes = [_p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _u, _u, _i, _i, _i, _i, _d, _p, _p, _i]
+    seq, gc = golden_io.skip_genome_image()
+    L.refc_init(0, gc.ctypes.data, len(seq))
+    wsL = L.refc_new(60, 50, 20, 0, 1)
+    wsR = L.refc_new(60, 50, 20, 0, 1)
+    z = np.load(cases_path, allow_pickle=False)
+    c = {k: z[k] for k in z.files}
+    n = len(c["qlen"])
+    npairs = np.zeros(n, np.int64)
+    counters = np.zeros((n, 3), np.int32)
+    pairs = []
+    out = np.zeros((MAXPAIRS, 8), np.int32)
+    for k in range(n):
+        q = bytes(c["qbuf"][c["qoff"][k]:c["qoff"][k] + c["qlen"][k]]) + b"\0"
+        qb, qub = ctypes.create_string_buffer(q, len(q)), ctypes.create_string_buffer(q.upper(), len(q))
+        cnt = np.zeros(3, np.int32)
+        a = [int(c[key][k]) for key in CDNA_KEYS]
+        m = L.refc_cdna_gap(wsL, wsR, qb, qub, *a, float(c["defect_rate"][k]), cnt.ctypes.data, out.ctypes.data,
+                            MAXPAIRS)
+        npairs[k] = m
+        counters[k] = cnt
+        if m > 0:
+            pairs.append(out[:m].copy())
+        if k % max(1, n // 4) == 0:
+            print("callers_run %s cdna: %d of %d" % (which, k, n), flush=True)
+    np.savez_compressed(out_path, npairs=npairs, counters=counters,
+                        pairs=np.concatenate(pairs) if pairs else np.zeros((0, 8), np.int32))
+
+
+def save_cdna_cases(path, cases):
+    """The cases of helpers.cdna_gap_calls as arrays."""
+    arr = {k: np.array([int(c[k]) for c in cases], np.int64) for k in CDNA_KEYS}
+    arr["defect_rate"] = np.array([c["defect_rate"] for c in cases], np.float64)
+    arr["qlen"] = np.array([len(c["query"]) for c in cases], np.int64)
+    arr["qoff"] = np.concatenate([[0], np.cumsum(arr["qlen"])[:-1]]).astype(np.int64)
+    arr["qbuf"] = np.frombuffer(b"".join(c["query"] for c in cases), np.uint8)
+    np.savez_compressed(path, **arr)
+
+
 def save_genome_cases(path, cases):
     """The cases of helpers.genome_gap_calls as arrays."""
     arr = {k: np.array([int(c[k]) for c in cases], np.int64) for k in GENOME_KEYS}
@@ -150,4 +202,5 @@ def save_cases(path, cases):
 
 
 if __name__ == "__main__":
-    (run_genome if sys.argv[4:5] == ["genome"] else run)(sys.argv[1], sys.argv[2], sys.argv[3])
+    mode = sys.argv[4] if len(sys.argv) > 4 else ""
+    {"genome": run_genome, "cdna": run_cdna}.get(mode, run)(sys.argv[1], sys.argv[2], sys.argv[3])

@@ -9,8 +9,12 @@ tests/golden/callers_genome_golden.npz: 600 calls of Dynprog_genome_gap
 the same genome, with their pair lists (the gap holder carries the intron),
 counters {dynprogindex, new_leftgenomepos, new_rightgenomepos,
 traceback_score, nmatches, nmismatches, nopens, nindels, exonhead,
-introntype} and splice-site probabilities {left_prob, right_prob}.  With
-the argument `genome` only the second file is written.
+introntype} and splice-site probabilities {left_prob, right_prob}; and
+tests/golden/callers_cdna_golden.npz: 600 calls of Dynprog_cdna_gap
+(dynprog_cdna.c:786-1180) from helpers.cdna_gap_calls (seed 1003), with
+their pair lists (the gap holder between the ends marks the cDNA insertion)
+and counters {dynprogindex, traceback_score, incompletep}.  With the
+argument `genome` or `cdna` only that file is written.
 
 Run in a container where the reference tree is mounted, after
 `make -C oracle ref`, which builds oracle/_ref/libgmapref_callers.so from
@@ -38,13 +42,14 @@ import helpers as H  # noqa: E402
 
 OUT = os.path.join(HERE, "callers_golden.npz")
 OUT_GENOME = os.path.join(HERE, "callers_genome_golden.npz")
-SEED, SEED_GENOME, N_GENOME = 1001, 1002, 600
+OUT_CDNA = os.path.join(HERE, "callers_cdna_golden.npz")
+SEED, SEED_GENOME, SEED_CDNA, N_EXTRA = 1001, 1002, 1003, 600
 
 
 def main():
     seq, _ = golden_io.skip_genome_image()
-    if sys.argv[1:] == ["genome"]:
-        return main_genome(seq)
+    if sys.argv[1:] in (["genome"], ["cdna"]):
+        return main_extra(seq, sys.argv[1])
     cases = H.caller_cases(np.random.default_rng(SEED), seq, 2000, 1000)
     with tempfile.TemporaryDirectory() as d:
         cpath, opath = os.path.join(d, "cases.npz"), os.path.join(d, "out.npz")
@@ -53,18 +58,24 @@ def main():
         z, o = np.load(cpath), np.load(opath)
         np.savez_compressed(OUT, **{k: z[k] for k in z.files}, **{"out_" + k: o[k] for k in o.files})
     print("wrote %s: %d calls" % (OUT, len(cases)))
-    main_genome(seq)
+    main_extra(seq, "genome")
+    main_extra(seq, "cdna")
 
 
-def main_genome(seq):
-    gcases = H.genome_gap_calls(np.random.default_rng(SEED_GENOME), seq, N_GENOME)
+def main_extra(seq, kind):
+    if kind == "genome":
+        cases = H.genome_gap_calls(np.random.default_rng(SEED_GENOME), seq, N_EXTRA)
+        save, run, out = callers_run.save_genome_cases, callers_run.run_genome, OUT_GENOME
+    else:
+        cases = H.cdna_gap_calls(np.random.default_rng(SEED_CDNA), seq, N_EXTRA)
+        save, run, out = callers_run.save_cdna_cases, callers_run.run_cdna, OUT_CDNA
     with tempfile.TemporaryDirectory() as d:
         cpath, opath = os.path.join(d, "cases.npz"), os.path.join(d, "out.npz")
-        callers_run.save_genome_cases(cpath, gcases)
-        callers_run.run_genome("ref", cpath, opath)
+        save(cpath, cases)
+        run("ref", cpath, opath)
         z, o = np.load(cpath), np.load(opath)
-        np.savez_compressed(OUT_GENOME, **{k: z[k] for k in z.files}, **{"out_" + k: o[k] for k in o.files})
-    print("wrote %s: %d calls" % (OUT_GENOME, len(gcases)))
+        np.savez_compressed(out, **{k: z[k] for k in z.files}, **{"out_" + k: o[k] for k in o.files})
+    print("wrote %s: %d calls" % (out, len(cases)))
 
 
 if __name__ == "__main__":

@@ -613,3 +613,47 @@ def genome_gap_calls(rng, genome_seq, n, chromosomes=2):
                     "defect_rate": float(rng.choice([0.001, 0.005, 0.02])), "maxpeelback": 20,
                     "halfp": bool(t % 5 == 2), "finalp": bool(rng.integers(0, 2))})
     return out
+
+
+def cdna_gap_calls(rng, genome_seq, n, chromosomes=2, flank=260):
+    """Calls of Dynprog_cdna_gap as stage 3 makes them (stage3.c:9274-9321: a
+    query gap longer than its genome gap by more than MININTRONLEN, both ends
+    read rlengthL = rlengthR = genomejump + extramaterial_paired bases): a
+    genome gap of 2-150 columns on either strand, and a query holding the
+    genome gap's bases with an insertion of 10-60 random bases at a random
+    point (1 % substitutions), between `flank` bases of the genome on each
+    side (the ends read past the query gap into them).  Extramaterial 4-30,
+    extraband 3-10, all three defect classes; a few genome gaps past the
+    workspace's 2000 columns.  Dicts of the call's arguments; query holds the
+    whole read, qdp5 / qdp3 the query gap's first and last bases."""
+    nb = len(genome_seq)
+    clen = nb // chromosomes
+    G = np.frombuffer(genome_seq, np.uint8)
+    comp = np.frombuffer(bytes.maketrans(b"ACGTN", b"TGCAN"), np.uint8)
+    out = []
+    for t in range(n):
+        k = int(rng.integers(0, chromosomes))
+        chroffset, chrhigh = k * clen, k * clen + clen - 1
+        watsonp = bool(rng.integers(0, 2))
+        glength = int(rng.integers(2, 151)) if t % 50 else int(rng.integers(2001, 2050))
+        ins = int(rng.integers(10, 61))
+        extra = int(rng.integers(4, 31))
+        goffset = int(rng.integers(flank + 10, clen - glength - flank - 10))
+
+        def S(x0, length):
+            if watsonp:
+                return G[chroffset + x0: chroffset + x0 + length].tobytes()
+            return comp[G[chrhigh - np.arange(x0, x0 + length)]].tobytes()
+
+        a = int(rng.integers(0, glength + 1))
+        seg = S(goffset, glength)
+        mid = mutate(rng, seg[:a], 0.01, 0.0) + rand_seq(rng, ins) + mutate(rng, seg[a:], 0.01, 0.0)
+        query = S(goffset - flank, flank) + mid + S(goffset + glength, flank)
+        qdp5 = flank
+        qdp3 = flank + len(mid) - 1
+        rlen = min(glength + extra, flank)
+        out.append({"query": query, "qdp5": qdp5, "qdp3": qdp3, "rlengthL": rlen, "rlengthR": rlen,
+                    "glength": glength, "goffset": goffset, "chroffset": chroffset, "chrhigh": chrhigh,
+                    "watsonp": watsonp, "genestrand": 0, "jump_late": bool(rng.integers(0, 2)),
+                    "extraband": int(rng.integers(3, 11)), "defect_rate": float(rng.choice([0.001, 0.005, 0.02]))})
+    return out

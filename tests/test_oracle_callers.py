@@ -23,10 +23,20 @@ def test_large_genome_reference_callers_reproduce_golden(reference, tmp_path):
 
 
 def test_reference_genome_gap_reproduces_golden(reference, tmp_path):
-    z, o = _run("ref", tmp_path, genome=True)
+    z, o = _run("ref", tmp_path, "genome")
     _compare_genome(z, o)
 
 
 def test_large_genome_reference_genome_gap_reproduces_golden(reference, tmp_path):
-    z, o = _run("ref_large", tmp_path, genome=True)
+    z, o = _run("ref_large", tmp_path, "genome")
     _compare_genome(z, o)
+
+
+def test_reference_cdna_gap_reproduces_golden(reference, tmp_path):
+    z, o = _run("ref", tmp_path, "cdna")
+    _compare(z, o)
+
+
+def test_large_genome_reference_cdna_gap_reproduces_golden(reference, tmp_path):
+    z, o = _run("ref_large", tmp_path, "cdna")
+    _compare(z, o)
