@@ -436,14 +436,23 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
     // runs until its longest one ends; sorted by row stride, then rows
     h.list2_off = off;
     int32_t *list2 = (int32_t *)(base + off);
-    for (int32_t i = 0; i < n; i++)
-      if (d[i].layout == 2) list2[h.n2++] = i;
     auto rows = [&](int32_t a) { return d[a].kind == GDP_KIND_LOWER ? d[a].gl : d[a].rl; };
-    std::sort(list2, list2 + h.n2, [&](int32_t a, int32_t b) {
-      if (d[a].P != d[b].P) return d[a].P > d[b].P;
-      if (rows(a) != rows(b)) return rows(a) > rows(b);
-      return a < b;
-    });
+    // a counting sort: strides 32, 16, 8, then rows from GDP_MAX_LEN down,
+    // indices ascending within a bucket
+    {
+      const int nb = 3 * (GDP_MAX_LEN + 1);
+      auto bucket = [&](int32_t a) {
+        const int sc = d[a].P == 32 ? 0 : (d[a].P == 16 ? 1 : 2);
+        return sc * (GDP_MAX_LEN + 1) + (GDP_MAX_LEN - gdp_imin(rows(a), GDP_MAX_LEN));
+      };
+      std::vector<int32_t> start(nb + 1, 0);
+      for (int32_t i = 0; i < n; i++)
+        if (d[i].layout == 2) start[bucket(i) + 1]++;
+      for (int b = 0; b < nb; b++) start[b + 1] += start[b];
+      h.n2 = start[nb];
+      for (int32_t i = 0; i < n; i++)
+        if (d[i].layout == 2) list2[start[bucket(i)]++] = i;
+    }
     // Each 64 consecutive entries are one wave's lanes.  Their direction
     // words are interleaved so that the wave's stores are coalesced: word kk
     // of row r of lane l at base + (r * nw + kk) * 64 + l, with nw the
@@ -467,6 +476,11 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
   if (dir_words) *dir_words = dw;
   if (score_slots) *score_slots = ss;
   return GDP_OK;
+}
+
+int64_t gdp_plan_used_bytes(const void *descs, int32_t n) {
+  const gdp_pair_hdr *h = (const gdp_pair_hdr *)((const uint8_t *)descs + (int64_t)n * sizeof(gdp_dev_problem));
+  return h->list2_off + (int64_t)h->n2 * (int64_t)sizeof(int32_t);
 }
 
 extern "C" int gdp_plan_ex(const gdp_problem *probs, int32_t n, int32_t flags, void *descs_out, int64_t *dir_words,

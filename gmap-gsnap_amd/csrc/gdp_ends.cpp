@@ -1,0 +1,271 @@
+// gdp_ends.cpp -- Dynprog_end5_splicejunction / Dynprog_end3_splicejunction
+// (reference dynprog_end.c:1652-1922, 2248-2745) for a batch of ends, with
+// every step on the device: the two one-sided fills per end keeping their
+// best last-row cell, the merge of the two cells, both local walks, and the
+// join of the walks around the known-site gap holder.  The host builds the
+// problems, the plan and the walk templates, and sizes the output from the
+// per-end lengths the device reports.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gdp_internal.h"
+
+extern "C" int gdp_launch_sj(int stage, int five, int m, const int32_t *best, const int32_t *stops,
+                             const gdp_tb_request *tmpl, gdp_tb_request *req, const gdp_pair *p1, const gdp_pair *p2,
+                             const gdp_tb_request *req1, const gdp_tb_request *req2, const int32_t *c1,
+                             const int32_t *c2, int32_t *len, int32_t *trim, const int64_t *off, const int32_t *aux,
+                             gdp_sj_pair *out, gdp_sj_result *res, hipStream_t stream);
+
+static_assert(sizeof(gdp_sj_end) == 56, "gdp_sj_end is 56 bytes");
+static_assert(sizeof(gdp_sj_pair) == 20, "gdp_sj_pair is 20 bytes");
+static_assert(sizeof(gdp_sj_result) == 40, "gdp_sj_result is 40 bytes");
+
+namespace {
+
+constexpr int kEndq = GDP_ENDQ;
+constexpr int kEndExtend = -2;  // END_EXTEND (dynprog_end.c:88-94)
+
+// END_OPEN by defect rate (dynprog_end.c:88-94, 1688-1700; DEFECT_HIGHQ
+// 0.003 and DEFECT_MEDQ 0.014, dynprog.h:58-59).
+int end_open(double defect_rate) { return defect_rate < 0.003 ? -10 : (defect_rate < 0.014 ? -8 : -6); }
+
+// A device buffer that grows to the largest size asked of it.
+struct Grow {
+  void *p = nullptr;
+  size_t cap = 0;
+  bool ensure(size_t bytes) {
+    if (bytes <= cap) return true;
+    if (p != nullptr) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, bytes) != hipSuccess) return false;
+    cap = bytes;
+    return true;
+  }
+  void release() {
+    if (p != nullptr) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// A pinned host buffer that grows likewise (the results are copied back
+// into it at full PCIe rate).
+struct PinnedGrow {
+  void *p = nullptr;
+  size_t cap = 0;
+  bool ensure(size_t bytes) {
+    if (bytes <= cap) return true;
+    if (p != nullptr) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return false;
+    cap = bytes;
+    return true;
+  }
+  void release() {
+    if (p != nullptr) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+int err(int code, const std::string &msg) {
+  gdp_set_error(msg.c_str());
+  return code;
+}
+
+}  // namespace
+
+struct gdp_sj_batch {
+  Grow descs, q, g, dirs, scores, fin, ul, best, tmpl1, tmpl2, req1, req2, p1, p2, c1, c2, stops, len, trim, off, aux,
+      out, res;
+  std::vector<gdp_problem> probs;
+  std::vector<uint8_t> hdescs;
+  std::vector<int32_t> hul, haux, hlen, live;
+  std::vector<gdp_tb_request> ht1, ht2;
+  std::vector<int64_t> hoff;
+  std::vector<gdp_sj_result> results;
+  std::vector<gdp_sj_result> lres;
+  PinnedGrow pairs;
+  int64_t npairs = 0;
+  ~gdp_sj_batch() {
+    for (Grow *b : {&descs, &q, &g, &dirs, &scores, &fin, &ul, &best, &tmpl1, &tmpl2, &req1, &req2, &p1, &p2, &c1, &c2,
+                    &stops, &len, &trim, &off, &aux, &out, &res})
+      b->release();
+    pairs.release();
+  }
+};
+
+extern "C" gdp_sj_batch *gdp_sj_create(void) { return new gdp_sj_batch(); }
+extern "C" void gdp_sj_destroy(gdp_sj_batch *b) { delete b; }
+extern "C" int64_t gdp_sj_npairs(const gdp_sj_batch *b) { return b ? b->npairs : 0; }
+extern "C" const gdp_sj_result *gdp_sj_results(const gdp_sj_batch *b) { return b ? b->results.data() : nullptr; }
+extern "C" const gdp_sj_pair *gdp_sj_pairs(const gdp_sj_batch *b) {
+  return b ? static_cast<const gdp_sj_pair *>(b->pairs.p) : nullptr;
+}
+
+extern "C" int gdp_sj_run(gdp_sj_batch *b, int32_t five, const gdp_sj_end *ends, int32_t n, const char *qbuf,
+                          int64_t qbytes, const char *gbuf, int64_t gbytes, int32_t max_rlength, int32_t max_glength,
+                          void *stream_) {
+  if (b == nullptr || n < 0 || (n > 0 && (ends == nullptr || qbuf == nullptr || gbuf == nullptr)) || qbytes < 0 ||
+      gbytes < 0)
+    return err(GDP_EINVAL, "gdp_sj_run: null or negative argument");
+  hipStream_t stream = (hipStream_t)stream_;
+  const int use8 = gdp_use8p_size(kEndq);
+  b->npairs = 0;
+  b->results.assign((size_t)n, gdp_sj_result{0, -1, {0, 0, 0, 0, 0, 0}, 0});
+  // ends that are empty or past the workspace give NULL (dynprog_end.c:1702-1717)
+  b->live.clear();
+  for (int32_t k = 0; k < n; k++) {
+    const gdp_sj_end &e = ends[k];
+    if (e.rl <= 0 || e.rl > max_rlength || e.gl <= 0 || e.gl > max_glength) continue;
+    if (e.qoff < 0 || e.goff < 0 || e.qoff + e.rl > qbytes || e.goff + 2 * (int64_t)e.gl > gbytes)
+      return err(GDP_EINVAL, "gdp_sj_run: end " + std::to_string(k) + " reads past its buffers");
+    if (e.contlength < 0) return err(GDP_EINVAL, "gdp_sj_run: end " + std::to_string(k) + " has contlength < 0");
+    b->live.push_back(k);
+  }
+  const int m = (int)b->live.size();
+  if (m == 0) return GDP_OK;
+
+  // two one-sided fills per end (1719-1779, 2300-2360)
+  b->probs.assign(2 * (size_t)m, gdp_problem{});
+  b->hul.resize(2 * (size_t)m);
+  b->haux.resize(2 * (size_t)m);
+  b->ht1.assign((size_t)m, gdp_tb_request{});
+  b->ht2.assign((size_t)m, gdp_tb_request{});
+  int64_t pair_cap = 0;
+  for (int j = 0; j < m; j++) {
+    const gdp_sj_end &e = ends[b->live[j]];
+    const int lb = e.gl >= e.rl ? e.extraband : e.rl - e.gl + e.extraband;  // Dynprog_compute_bands, widebandp
+    const int ub = e.gl >= e.rl ? e.gl - e.rl + e.extraband : e.extraband;
+    for (int t = 0; t < 2; t++) {
+      gdp_problem &p = b->probs[2 * (size_t)j + t];
+      p.qoff = e.qoff;
+      p.goff = e.goff;
+      p.galtoff = e.goff + e.gl;
+      p.rl = e.rl;
+      p.gl = e.gl;
+      p.lband = (int16_t)lb;
+      p.uband = (int16_t)ub;
+      p.kind = t == 0 ? GDP_KIND_UPPER : GDP_KIND_LOWER;
+      p.width = (e.rl < use8 || e.gl < use8) ? 8 : 16;
+      p.mismatchtype = kEndq;
+      p.jump_late = five ? !e.jump_late : (e.jump_late != 0);  // the 5' fills take the opposite tie rule
+      p.open = (int8_t)end_open(e.defect_rate);
+      p.extend = kEndExtend;
+      p.keep_scores = GDP_KEEP_BEST_QUERYEND;
+    }
+    b->hul[2 * (size_t)j] = 2 * j;
+    b->hul[2 * (size_t)j + 1] = 2 * j + 1;
+    b->haux[2 * (size_t)j] = e.rl;
+    b->haux[2 * (size_t)j + 1] = five ? e.anchor - e.far : e.far - e.anchor;
+    const int maxp = e.rl + e.gl + 4;
+    for (int w = 0; w < 2; w++) {
+      gdp_tb_request &q = w == 0 ? b->ht1[j] : b->ht2[j];
+      q.queryoffset = e.roffset;
+      q.genomeoffset = w == 0 ? e.far : e.anchor;
+      q.watsonp = 1;
+      q.revp = five ? 1 : 0;
+      q.genestrand = e.genestrand;
+      q.flags = GDP_TB_LOCAL;
+      q.pair_off = pair_cap;
+      q.max_pairs = maxp;
+      q.endc = w == 0 ? e.contlength : 0;
+    }
+    pair_cap += maxp;
+  }
+  const int np = 2 * m;
+  b->hdescs.resize((size_t)gdp_plan_bytes(np));
+  int64_t dw = 0, ss = 0;
+  int rc = gdp_plan_ex(b->probs.data(), np, 0, b->hdescs.data(), &dw, &ss);
+  if (rc != GDP_OK) return rc;
+  const uint32_t kernels = gdp_plan_kernels(b->hdescs.data(), np);
+
+  const size_t tb = sizeof(gdp_tb_request) * (size_t)m;
+  const int64_t used = gdp_plan_used_bytes(b->hdescs.data(), np);
+  if (!b->descs.ensure((size_t)used) || !b->q.ensure((size_t)qbytes + 16) || !b->g.ensure((size_t)gbytes + 16) ||
+      !b->dirs.ensure(sizeof(uint32_t) * (size_t)(dw > 0 ? dw : 1)) ||
+      !b->scores.ensure(sizeof(int16_t) * (size_t)(ss > 0 ? ss : 1)) || !b->fin.ensure(sizeof(int32_t) * np) ||
+      !b->ul.ensure(sizeof(int32_t) * np) || !b->best.ensure(sizeof(int32_t) * 3 * (size_t)m) ||
+      !b->tmpl1.ensure(tb) || !b->tmpl2.ensure(tb) || !b->req1.ensure(tb) || !b->req2.ensure(tb) ||
+      !b->p1.ensure(sizeof(gdp_pair) * (size_t)pair_cap) || !b->p2.ensure(sizeof(gdp_pair) * (size_t)pair_cap) ||
+      !b->c1.ensure(sizeof(int32_t) * 6 * (size_t)m) || !b->c2.ensure(sizeof(int32_t) * 6 * (size_t)m) ||
+      !b->stops.ensure(sizeof(int32_t) * 2 * (size_t)m) || !b->len.ensure(sizeof(int32_t) * (size_t)m) ||
+      !b->trim.ensure(sizeof(int32_t) * (size_t)m) || !b->off.ensure(sizeof(int64_t) * (size_t)m) ||
+      !b->aux.ensure(sizeof(int32_t) * np) || !b->res.ensure(sizeof(gdp_sj_result) * (size_t)m))
+    return err(GDP_ENOMEM, "gdp_sj_run: device allocation failed");
+  auto up = [&](Grow &d, const void *h, size_t bytes) {
+    return hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, stream) == hipSuccess;
+  };
+  if (!up(b->descs, b->hdescs.data(), (size_t)used) || !up(b->q, qbuf, (size_t)qbytes) ||
+      !up(b->g, gbuf, (size_t)gbytes) || !up(b->ul, b->hul.data(), sizeof(int32_t) * np) ||
+      !up(b->aux, b->haux.data(), sizeof(int32_t) * np) || !up(b->tmpl1, b->ht1.data(), tb) ||
+      !up(b->tmpl2, b->ht2.data(), tb))
+    return err(GDP_EHIP, "gdp_sj_run: upload failed");
+  // the walks' stop cells start at (-1, -1), as an unwalked end leaves them
+  if (hipMemsetAsync(b->stops.p, 0xff, sizeof(int32_t) * 2 * (size_t)m, stream) != hipSuccess)
+    return err(GDP_EHIP, "gdp_sj_run: clear failed");
+
+  const char *dq = b->q.as<char>(), *dg = b->g.as<char>();
+  if ((rc = gdp_fill_ex(b->descs.p, np, kernels, dq, dg, b->dirs.as<uint32_t>(), b->scores.as<int16_t>(),
+                        b->fin.as<int32_t>(), stream)) != GDP_OK)
+    return rc;
+  // the best cell of the last row (find_best_endpoint_to_queryend_indels)
+  if ((rc = gdp_best_endpoint(b->descs.p, b->ul.as<int32_t>(), m, 1, b->scores.as<int16_t>(), b->best.as<int32_t>(),
+                              stream)) != GDP_OK)
+    return rc;
+  if (gdp_launch_sj(0, five, m, b->best.as<int32_t>(), nullptr, b->tmpl1.as<gdp_tb_request>(),
+                    b->req1.as<gdp_tb_request>(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                    nullptr, nullptr, nullptr, nullptr, nullptr, stream))
+    return err(GDP_EHIP, "gdp_sj_run: first requests failed");
+  if ((rc = gdp_traceback_ends(b->descs.p, b->req1.as<gdp_tb_request>(), m, dq, dg, b->dirs.as<uint32_t>(), nullptr, 0,
+                               0, b->p1.as<gdp_pair>(), b->c1.as<int32_t>(), b->stops.as<int32_t>(), stream)) !=
+      GDP_OK)
+    return rc;
+  if (gdp_launch_sj(1, five, m, b->best.as<int32_t>(), b->stops.as<int32_t>(), b->tmpl2.as<gdp_tb_request>(),
+                    b->req2.as<gdp_tb_request>(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                    nullptr, nullptr, nullptr, nullptr, nullptr, stream))
+    return err(GDP_EHIP, "gdp_sj_run: second requests failed");
+  if ((rc = gdp_traceback_ex(b->descs.p, b->req2.as<gdp_tb_request>(), m, dq, dg, b->dirs.as<uint32_t>(), nullptr, 0, 0,
+                             b->p2.as<gdp_pair>(), b->c2.as<int32_t>(), stream)) != GDP_OK)
+    return rc;
+  if (gdp_launch_sj(2, five, m, b->best.as<int32_t>(), nullptr, nullptr, nullptr, b->p1.as<gdp_pair>(), nullptr,
+                    b->req1.as<gdp_tb_request>(), nullptr, b->c1.as<int32_t>(), b->c2.as<int32_t>(),
+                    b->len.as<int32_t>(), b->trim.as<int32_t>(), nullptr, nullptr, nullptr, nullptr, stream))
+    return err(GDP_EHIP, "gdp_sj_run: lengths failed");
+  // output offsets from the lengths
+  b->hlen.resize((size_t)m);
+  if (hipMemcpyAsync(b->hlen.data(), b->len.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    return err(GDP_EHIP, "gdp_sj_run: lengths copy failed");
+  b->hoff.resize((size_t)m);
+  int64_t total = 0;
+  for (int j = 0; j < m; j++) {
+    b->hoff[j] = total;
+    total += b->hlen[j] > 0 ? b->hlen[j] : 0;
+  }
+  if (!b->out.ensure(sizeof(gdp_sj_pair) * (size_t)(total > 0 ? total : 1)) || !up(b->off, b->hoff.data(), 8 * (size_t)m))
+    return err(GDP_ENOMEM, "gdp_sj_run: output allocation failed");
+  if (gdp_launch_sj(3, five, m, nullptr, nullptr, nullptr, nullptr, b->p1.as<gdp_pair>(), b->p2.as<gdp_pair>(),
+                    b->req1.as<gdp_tb_request>(), b->req2.as<gdp_tb_request>(), b->c1.as<int32_t>(),
+                    b->c2.as<int32_t>(), b->len.as<int32_t>(), b->trim.as<int32_t>(), b->off.as<int64_t>(),
+                    b->aux.as<int32_t>(), b->out.as<gdp_sj_pair>(), b->res.as<gdp_sj_result>(), stream))
+    return err(GDP_EHIP, "gdp_sj_run: join failed");
+  b->lres.resize((size_t)m);
+  if (!b->pairs.ensure(sizeof(gdp_sj_pair) * (size_t)(total > 0 ? total : 1)))
+    return err(GDP_ENOMEM, "gdp_sj_run: pinned host allocation failed");
+  if (hipMemcpyAsync(b->lres.data(), b->res.p, sizeof(gdp_sj_result) * m, hipMemcpyDeviceToHost, stream) !=
+          hipSuccess ||
+      (total > 0 && hipMemcpyAsync(b->pairs.p, b->out.p, sizeof(gdp_sj_pair) * (size_t)total, hipMemcpyDeviceToHost,
+                                   stream) != hipSuccess) ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    return err(GDP_EHIP, "gdp_sj_run: result copy failed");
+  for (int j = 0; j < m; j++) b->results[b->live[j]] = b->lres[j];
+  b->npairs = total;
+  return GDP_OK;
+}

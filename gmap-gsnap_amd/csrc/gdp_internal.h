@@ -74,6 +74,10 @@ static_assert(sizeof(gdp_problem) == 48, "gdp_problem is 48 bytes");
 static_assert(sizeof(gdp_tb_request) == 48, "gdp_tb_request is 48 bytes");
 // Sets the message gdp_last_error() returns on the calling thread.
 void gdp_set_error(const char *msg);
+// Bytes of a plan of n problems that the kernels read: the descriptors, the
+// header, the wave lists and the index lists (gdp_plan_bytes reserves room
+// for one wave per problem).
+int64_t gdp_plan_used_bytes(const void *descs, int32_t n);
 
 static_assert(sizeof(gdp_pair) == 16, "gdp_pair is 16 bytes");
 static_assert(sizeof(gdp_intron_gap) == 32, "gdp_intron_gap is 32 bytes");

@@ -1800,6 +1800,141 @@ __global__ __launch_bounds__(64) void gdp_bridge_intron_kernel(
   }
 }
 
+// --------------------------------------------------- splice-junction ends
+
+// The first walk of each end of Dynprog_end5_splicejunction /
+// Dynprog_end3_splicejunction (dynprog_end.c:1797-1830, 2385-2420): from its
+// best last-row cell, in the triangle that holds it (fill 2k upper, 2k+1
+// lower); an end whose best scores below 0 is not walked (a request at
+// (0, 0), which pushes nothing).
+__global__ void gdp_sj_req1_kernel(const int32_t *__restrict__ best, int m, const gdp_tb_request *__restrict__ tmpl,
+                                   gdp_tb_request *__restrict__ req) {
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= m) return;
+  gdp_tb_request q = tmpl[k];
+  const bool go = best[3 * (int64_t)k] >= 0;
+  const int r = go ? best[3 * (int64_t)k + 1] : 0, c = go ? best[3 * (int64_t)k + 2] : 0;
+  q.fill = 2 * k + (c < r ? 1 : 0);
+  q.r = r;
+  q.c = c;
+  req[k] = q;
+}
+
+// The second walk: from the cell where the first one stopped.
+__global__ void gdp_sj_req2_kernel(const int32_t *__restrict__ best, const int32_t *__restrict__ stops, int m,
+                                   const gdp_tb_request *__restrict__ tmpl, gdp_tb_request *__restrict__ req) {
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= m) return;
+  gdp_tb_request q = tmpl[k];
+  int r = stops[2 * (int64_t)k], c = stops[2 * (int64_t)k + 1];
+  if (best[3 * (int64_t)k] < 0 || r < 0 || c < 0) r = c = 0;
+  q.fill = 2 * k + (c < r ? 1 : 0);
+  q.r = r;
+  q.c = c;
+  req[k] = q;
+}
+
+// Length of each end's returned list: the first walk's pairs, the gap
+// holder, the second walk's pairs, less the indel pairs at the read's end
+// (the first pushed, which the reference drops after reversing the list,
+// 1905-1910, 2485-2490); -1 for an end that is not walked.
+__global__ void gdp_sj_len_kernel(const int32_t *__restrict__ best, int m, const gdp_pair *__restrict__ p1,
+                                  const gdp_tb_request *__restrict__ req1, const int32_t *__restrict__ c1,
+                                  const int32_t *__restrict__ c2, int32_t *__restrict__ len,
+                                  int32_t *__restrict__ trim) {
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= m) return;
+  if (best[3 * (int64_t)k] < 0) {
+    len[k] = -1;
+    trim[k] = 0;
+    return;
+  }
+  const int n1 = gdp_imax(0, c1[6 * (int64_t)k]), n2 = gdp_imax(0, c2[6 * (int64_t)k]);
+  const gdp_pair *a = p1 + req1[k].pair_off;
+  int t = 0;
+  while (t < n1 && a[t].comp == '-') t++;
+  len[k] = n1 + 1 + n2 - t;
+  trim[k] = t;
+}
+
+// Writes each walked end's list at its offset, one wave per end: the 5'
+// form in push order, the 3' form reversed; the gap holder between the walks
+// is Pairpool_push_gapholder with knownp (pairpool.c:374-452).  aux[2k] = the
+// read's length (missscore = traceback_score - rl * FULLMATCH), aux[2k+1] =
+// the gap holder's genomejump.
+__global__ __launch_bounds__(256) void gdp_sj_join_kernel(
+    int five, int m, const gdp_pair *__restrict__ p1, const gdp_pair *__restrict__ p2,
+    const gdp_tb_request *__restrict__ req1, const gdp_tb_request *__restrict__ req2, const int32_t *__restrict__ c1,
+    const int32_t *__restrict__ c2, const int32_t *__restrict__ len, const int32_t *__restrict__ trim,
+    const int64_t *__restrict__ off, const int32_t *__restrict__ aux, gdp_sj_pair *__restrict__ out,
+    gdp_sj_result *__restrict__ res) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (k >= m) return;
+  const int L = len[k];
+  const int64_t o = off[k];
+  if (lane == 0) {
+    gdp_sj_result r;
+    r.pair_off = o;
+    r.npairs = L;
+    r.pad = 0;
+    for (int i = 0; i < 6; i++) r.counters[i] = 0;
+    if (L >= 0) {
+      const int32_t *a = c1 + 6 * (int64_t)k, *b = c2 + 6 * (int64_t)k;
+      const int sc = a[1] + b[1];
+      r.counters[0] = sc;
+      r.counters[1] = sc - aux[2 * (int64_t)k] * 3;  // FULLMATCH (dynprog.h:42)
+      for (int i = 2; i < 6; i++) r.counters[i] = a[i] + b[i];
+    }
+    res[k] = r;
+  }
+  if (L <= 0) return;
+  const int n1 = gdp_imax(0, c1[6 * (int64_t)k]), n2 = gdp_imax(0, c2[6 * (int64_t)k]), N = n1 + 1 + n2;
+  const int t = trim[k];
+  const gdp_pair *a = p1 + req1[k].pair_off, *b = p2 + req2[k].pair_off;
+  const int jump = aux[2 * (int64_t)k + 1];
+  for (int j = lane; j < L; j += 64) {
+    const int i = five ? t + j : N - 1 - j;
+    gdp_sj_pair w;
+    w.pad = 0;
+    if (i == n1) {
+      w.querypos = w.genomepos = -1;
+      w.cdna = w.comp = w.genome = w.genomealt = ' ';
+      w.gapp = 2;
+      w.genomejump = jump;
+    } else {
+      const gdp_pair &s = i < n1 ? a[i] : b[i - n1 - 1];
+      w.querypos = s.querypos;
+      w.genomepos = s.genomepos;
+      w.cdna = s.cdna;
+      w.comp = s.comp;
+      w.genome = s.genome;
+      w.genomealt = s.genomealt;
+      w.gapp = s.gapp;
+      w.genomejump = s.genomejump;
+    }
+    out[o + j] = w;
+  }
+}
+
+extern "C" int gdp_launch_sj(int stage, int five, int m, const int32_t *best, const int32_t *stops,
+                             const gdp_tb_request *tmpl, gdp_tb_request *req, const gdp_pair *p1, const gdp_pair *p2,
+                             const gdp_tb_request *req1, const gdp_tb_request *req2, const int32_t *c1,
+                             const int32_t *c2, int32_t *len, int32_t *trim, const int64_t *off, const int32_t *aux,
+                             gdp_sj_pair *out, gdp_sj_result *res, hipStream_t stream) {
+  if (m <= 0) return 0;
+  const dim3 g((m + 255) / 256), b(256);
+  switch (stage) {
+  case 0: hipLaunchKernelGGL(gdp_sj_req1_kernel, g, b, 0, stream, best, m, tmpl, req); break;
+  case 1: hipLaunchKernelGGL(gdp_sj_req2_kernel, g, b, 0, stream, best, stops, m, tmpl, req); break;
+  case 2: hipLaunchKernelGGL(gdp_sj_len_kernel, g, b, 0, stream, best, m, p1, req1, c1, c2, len, trim); break;
+  default:
+    hipLaunchKernelGGL(gdp_sj_join_kernel, dim3((m + 3) / 4), b, 0, stream, five, m, p1, p2, req1, req2, c1, c2, len,
+                       trim, off, aux, out, res);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // ------------------------------------------------------------ dense export
 
 __global__ void gdp_export_kernel(const gdp_dev_problem *__restrict__ probs, int idx, const uint32_t *__restrict__ dirs,

@@ -11,22 +11,16 @@ not negative walks back twice: a local walk to the continuation column,
 a known gap holder, and a local walk from where the first stopped to the
 anchor.  It then trims indel pairs from the read's end.  The 3' form is
 the same on the forward strings with the caller's own tie rule, its gap
-holder spanning far - anchor, and its list returned in the trimmed order.  Here the fills keep
-only their best last-row cell (GDP_KEEP_BEST_QUERYEND), one launch merges
-the triangles, and each set of walks is one traceback launch
-(GDP_TB_LOCAL); the host joins the two walks.
+holder spanning far - anchor, and its list returned in the trimmed order.
+Here the whole caller runs in the library (gdp_sj_run): the fills keep only
+their best last-row cell (GDP_KEEP_BEST_QUERYEND), one launch merges the
+triangles, each set of walks is one traceback launch (GDP_TB_LOCAL), and a
+join kernel writes each end's list around its gap holder.
 """
 import numpy as np
 
-from .batch import DeviceBatch
-from .lib import PROBLEM_DTYPE, SJ_PAIR_DTYPE, TBREQ_DTYPE, load
+from .lib import SJ_END_DTYPE, SJ_PAIR_DTYPE, SJ_RESULT_DTYPE
 
-GDP_KIND_UPPER, GDP_KIND_LOWER = 1, 2
-GDP_KEEP_BEST_QUERYEND = 3
-GDP_TB_LOCAL = 4
-ENDQ = 3
-FULLMATCH = 3  # dynprog.h:42
-INDEL_COMP = ord("-")  # comp.h:9
 
 
 def end_penalties(defect_rate):
@@ -66,92 +60,83 @@ def end3_splicejunction(ends, max_rlength=1000, max_glength=2000):
     return _splicejunction(ends, False, max_rlength, max_glength)
 
 
-def _splicejunction(ends, five, max_rlength, max_glength):
-    L = load()
-    use8 = int(L.gdp_use8p_size(ENDQ))
-    out = [None] * len(ends)
-    rl_all = np.array([len(e["rev_rseq"]) for e in ends], np.int64)
-    gl_all = np.array([len(e["rev_gseq"]) for e in ends], np.int64)
-    # dynprog_end.c:1702-1717: empty or oversized ends give NULL
-    live = np.nonzero((rl_all > 0) & (rl_all <= max_rlength) & (gl_all > 0) & (gl_all <= max_glength))[0]
-    if len(live) == 0:
-        return out
-    E = [ends[k] for k in live]
-    rl, gl = rl_all[live], gl_all[live]
-    defect = np.array([e["defect_rate"] for e in E])
-    extra = np.array([e["extraband"] for e in E], np.int64)
-    late = np.array([bool(e["jump_late"]) for e in E])
-    roff = np.array([e["rev_roffset"] for e in E], np.int64)
-    anchor = np.array([e["anchor"] for e in E], np.int64)
-    far = np.array([e["far"] for e in E], np.int64)
-    endc1 = np.array([e["contlength"] for e in E], np.int64)
-    gs = np.array([e["genestrand"] for e in E], np.int64)
-    # sweep order: the 5' form's reversed strings forward
+def pack_ends(ends, five):
+    """The ends as gdp_sj_end rows and the query / genome byte buffers in
+    sweep order (the 5' form's reversed strings forward)."""
+    n = len(ends)
+    arr = np.zeros(n, SJ_END_DTYPE)
+    rl = np.array([len(e["rev_rseq"]) for e in ends], np.int64)
+    gl = np.array([len(e["rev_gseq"]) for e in ends], np.int64)
     if five:
-        qbuf = b"".join(e["rev_rseq"][::-1] for e in E)
-        gbuf = b"".join(e["rev_gseq"][::-1] + e["rev_galt"][::-1] for e in E)
+        qbuf = b"".join(e["rev_rseq"][::-1] for e in ends)
+        gbuf = b"".join(e["rev_gseq"][::-1] + e["rev_galt"][::-1] for e in ends)
     else:
-        qbuf = b"".join(e["rev_rseq"] for e in E)
-        gbuf = b"".join(e["rev_gseq"] + e["rev_galt"] for e in E)
-    qoff = np.concatenate([[0], np.cumsum(rl)[:-1]])
-    goff = np.concatenate([[0], np.cumsum(2 * gl)[:-1]])
-    # end penalties by defect rate (dynprog_end.c:88-94), bands widened to the far corner
-    open_ = np.where(defect < 0.003, -10, np.where(defect < 0.014, -8, -6))
-    lb = np.where(gl >= rl, extra, rl - gl + extra)
-    ub = np.where(gl >= rl, gl - rl + extra, extra)
-    probs = np.zeros(2 * len(live), PROBLEM_DTYPE)
-    for j, kind in enumerate((GDP_KIND_UPPER, GDP_KIND_LOWER)):
-        p = probs[j::2]
-        p["qoff"], p["goff"], p["galtoff"] = qoff, goff, goff + gl
-        p["rl"], p["gl"], p["lband"], p["uband"] = rl, gl, lb, ub
-        p["kind"], p["mismatchtype"] = kind, ENDQ
-        p["width"] = np.where((rl < use8) | (gl < use8), 8, 16)
-        p["jump_late"] = late != five  # the 5' fills take the opposite tie rule
-        p["open"], p["extend"], p["keep_scores"] = open_, -2, GDP_KEEP_BEST_QUERYEND
-    b = DeviceBatch(probs, qbuf, gbuf)
-    b.fill()
-    ul = np.stack([np.arange(0, 2 * len(live), 2), np.arange(1, 2 * len(live), 2)], 1).astype(np.int32)
-    best = b.best_endpoint(ul, queryend_indels=True)
-    sel = np.nonzero(best[:, 0] >= 0)[0]  # finalscore < 0: no splice (1797-1800)
-    if len(sel) == 0:
-        return out
+        qbuf = b"".join(e["rev_rseq"] for e in ends)
+        gbuf = b"".join(e["rev_gseq"] + e["rev_galt"] for e in ends)
+    arr["qoff"] = np.concatenate([[0], np.cumsum(rl)[:-1]]) if n else 0
+    arr["goff"] = np.concatenate([[0], np.cumsum(2 * gl)[:-1]]) if n else 0
+    arr["rl"], arr["gl"] = rl, gl
+    for f, key in (("roffset", "rev_roffset"), ("anchor", "anchor"), ("far", "far"), ("contlength", "contlength"),
+                   ("extraband", "extraband"), ("genestrand", "genestrand"), ("defect_rate", "defect_rate")):
+        arr[f] = [e[key] for e in ends]
+    arr["jump_late"] = [bool(e["jump_late"]) for e in ends]
+    return arr, qbuf, gbuf
 
-    def requests(r, c, endc, goffs):
-        req = np.zeros(len(sel), TBREQ_DTYPE)
-        req["fill"] = 2 * sel + (c < r)  # the triangle holding the cell
-        req["r"], req["c"] = r, c
-        req["queryoffset"], req["genomeoffset"] = roff[sel], goffs
-        req["revp"], req["genestrand"], req["watsonp"] = int(five), gs[sel], 1
-        req["flags"], req["endc"] = GDP_TB_LOCAL, endc
-        req["max_pairs"] = rl[sel] + gl[sel] + 4
-        return req
 
-    p1, c1, stop = b.traceback(requests(best[sel, 1], best[sel, 2], endc1[sel], far[sel]), ends=True)
-    p2, c2 = b.traceback(requests(stop[:, 0], stop[:, 1], 0, anchor[sel]))
-    cnt = c1[:, 1:] + c2[:, 1:]
-    jump = (anchor - far) if five else (far - anchor)
+class SpliceBatch:
+    """gdp_sj_run through one gdp_sj_batch (device buffers reused across
+    runs): packed ends in, per-end results and the flat pair array out."""
 
-    def widen(p):
-        w = np.zeros(len(p), SJ_PAIR_DTYPE)
-        for f in ("querypos", "genomepos", "cdna", "comp", "genome", "genomealt", "gapp", "genomejump"):
-            w[f] = p[f]
-        return w
+    def __init__(self):
+        from .batch import init
+        self.L = init()
+        self.h = self.L.gdp_sj_create()
 
-    for j, i in enumerate(sel):
-        gap = np.zeros(1, SJ_PAIR_DTYPE)  # Pairpool_push_gapholder, knownp (pairpool.c:374-452)
-        gap["querypos"] = gap["genomepos"] = -1
-        gap["cdna"] = gap["comp"] = gap["genome"] = gap["genomealt"] = ord(" ")
-        gap["gapp"], gap["genomejump"] = 2, jump[i]
-        pairs = np.concatenate([widen(p1[j]), gap, widen(p2[j])])
-        # the reference reverses the list and drops indel pairs from the read's
-        # end, which are the first pushed (1905-1910, 2485-2490); the 3' form
-        # returns that trimmed list as it is
-        k = 0
-        while k < len(pairs) and pairs["comp"][k] == INDEL_COMP:
-            k += 1
-        out[live[i]] = (pairs[k:] if five else pairs[k:][::-1].copy(),
-                        np.array([cnt[j, 0], cnt[j, 0] - rl[i] * FULLMATCH, cnt[j, 1], cnt[j, 2], cnt[j, 3],
-                                  cnt[j, 4]], np.int32))
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.gdp_sj_destroy(self.h)
+            self.h = None
+
+    def run(self, five, arr, qbuf, gbuf, max_rlength=1000, max_glength=2000, stream=None, copy=True):
+        """(per-end SJ_RESULT_DTYPE rows, SJ_PAIR_DTYPE pairs); with
+        copy=False the pairs are a view of the batch's pinned buffer, valid
+        until its next run."""
+        import ctypes
+        arr = np.ascontiguousarray(arr, SJ_END_DTYPE)
+        q = np.frombuffer(bytes(qbuf) or b"\0", np.uint8)
+        g = np.frombuffer(bytes(gbuf) or b"\0", np.uint8)
+        rc = self.L.gdp_sj_run(self.h, int(five), arr.ctypes.data, len(arr), q.ctypes.data, len(qbuf), g.ctypes.data,
+                               len(gbuf), max_rlength, max_glength, stream)
+        if rc != 0:
+            raise RuntimeError("gdp_sj_run failed (%d): %s" % (rc, self.L.gdp_last_error().decode()))
+        n, npairs = len(arr), int(self.L.gdp_sj_npairs(self.h))
+        res = np.ctypeslib.as_array(ctypes.cast(self.L.gdp_sj_results(self.h), ctypes.POINTER(ctypes.c_uint8)),
+                                    (max(n, 1) * SJ_RESULT_DTYPE.itemsize,))[: n * SJ_RESULT_DTYPE.itemsize]
+        res = res.view(SJ_RESULT_DTYPE).copy() if n else np.zeros(0, SJ_RESULT_DTYPE)
+        if npairs:
+            pp = np.ctypeslib.as_array(ctypes.cast(self.L.gdp_sj_pairs(self.h), ctypes.POINTER(ctypes.c_uint8)),
+                                       (npairs * SJ_PAIR_DTYPE.itemsize,))
+            pairs = pp.view(SJ_PAIR_DTYPE)
+            if copy:
+                pairs = pairs.copy()
+        else:
+            pairs = np.zeros(0, SJ_PAIR_DTYPE)
+        return res, pairs
+
+
+_batch = None
+
+
+def _splicejunction(ends, five, max_rlength, max_glength):
+    global _batch
+    if _batch is None:
+        _batch = SpliceBatch()
+    arr, qbuf, gbuf = pack_ends(ends, five)
+    res, pairs = _batch.run(five, arr, qbuf, gbuf, max_rlength, max_glength)
+    out = [None] * len(ends)
+    for k in np.nonzero(res["npairs"] >= 0)[0]:
+        o, m = int(res["pair_off"][k]), int(res["npairs"][k])
+        out[k] = (pairs[o:o + m], res["counters"][k].astype(np.int32))
     return out
 
 
@@ -178,20 +163,30 @@ def _random_ends(n, seed=1007):
 
 
 if __name__ == "__main__":
-    # python3 -m gmapdp.ends [N]: times end5_splicejunction over N ends of the
-    # end-extension mix (after one untimed call of the same size).
+    # python3 -m gmapdp.ends [N]: times Dynprog_end5_splicejunction over N ends
+    # of the end-extension mix: the library call on packed ends (host buffers
+    # in and out, so the copies over PCIe are inside), then the Python API
+    # with its packing of the dicts and the per-end views (each after one
+    # untimed run of the same size).
     import sys
     import time
 
-    import torch
-
     n = int(sys.argv[1]) if len(sys.argv) > 1 else 200_000
     ends = _random_ends(n)
-    end5_splicejunction(ends)
-    torch.cuda.synchronize()
+    arr, qbuf, gbuf = pack_ends(ends, True)
+    sb = SpliceBatch()
+    sb.run(True, arr, qbuf, gbuf)
+    reps = 5
     t0 = time.perf_counter()
-    res = end5_splicejunction(ends)
-    torch.cuda.synchronize()
+    for _ in range(reps):
+        res, pairs = sb.run(True, arr, qbuf, gbuf, copy=False)
+    dt = (time.perf_counter() - t0) / reps
+    walked = int((res["npairs"] >= 0).sum())
+    print("gdp_sj_run (5'): %d ends (%d walked, %d pairs) in %.4f s = %.0f ends/s" % (n, walked, len(pairs), dt, n / dt),
+          flush=True)
+    end5_splicejunction(ends)
+    t0 = time.perf_counter()
+    out = end5_splicejunction(ends)
     dt = time.perf_counter() - t0
-    walked = sum(r is not None for r in res)
-    print("end5_splicejunction: %d ends (%d walked) in %.3f s = %.0f ends/s" % (n, walked, dt, n / dt), flush=True)
+    print("end5_splicejunction (dicts in, per-end lists out): %d ends in %.3f s = %.0f ends/s" % (n, dt, n / dt),
+          flush=True)

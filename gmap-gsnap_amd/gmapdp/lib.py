@@ -54,6 +54,10 @@ PAIR_DTYPE = np.dtype([("querypos", "<i4"), ("genomepos", "<i4"), ("cdna", "i1")
 SJ_PAIR_DTYPE = np.dtype([("querypos", "<i4"), ("genomepos", "<i4"), ("cdna", "i1"), ("comp", "i1"),
                           ("genome", "i1"), ("genomealt", "i1"), ("gapp", "<i2"), ("pad", "<i2"),
                           ("genomejump", "<i4")])
+SJ_END_DTYPE = np.dtype([("qoff", "<i8"), ("goff", "<i8"), ("rl", "<i4"), ("gl", "<i4"), ("roffset", "<i4"),
+                         ("anchor", "<i4"), ("far", "<i4"), ("contlength", "<i4"), ("extraband", "<i2"),
+                         ("genestrand", "i1"), ("jump_late", "i1"), ("pad", "<i4"), ("defect_rate", "<f8")])
+SJ_RESULT_DTYPE = np.dtype([("pair_off", "<i8"), ("npairs", "<i4"), ("counters", "<i4", (6,)), ("pad", "<i4")])
 SEGMENT_DTYPE = np.dtype([("coord", "<u4"), ("bound", "<u4"), ("length", "<i4"), ("left", "i1"), ("revcomp", "i1"),
                           ("pad", "<i2"), ("out_off", "<i8")])
 ALIGN_REQ_DTYPE = np.dtype([("tag", "<u8"), ("read", "<u8"), ("rl", "<i4"), ("gl", "<i4"), ("roffset", "<i4"),
@@ -127,6 +131,12 @@ def load(path=None):
         "gdp_best_endpoint": (ctypes.c_int, [vp, vp, i32, i32, vp, vp, vp]),
         "gdp_bridge_cdna": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, vp]),
         "gdp_bridge_intron": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "gdp_sj_create": (vp, []),
+        "gdp_sj_destroy": (None, [vp]),
+        "gdp_sj_run": (ctypes.c_int, [vp, i32, vp, i32, vp, i64, vp, i64, i32, i32, vp]),
+        "gdp_sj_npairs": (i64, [vp]),
+        "gdp_sj_results": (vp, [vp]),
+        "gdp_sj_pairs": (vp, [vp]),
         "gdp_traceback": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp]),
         "gdp_traceback_ex": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp]),
         "gdp_traceback_ends": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp, vp]),

@@ -501,6 +501,83 @@ int gdp_export_dense(const void *d_descs, int32_t idx, const uint32_t *d_dirs, c
 int gdp_export_reference(const void *d_descs, int32_t idx, const uint32_t *d_dirs, const int16_t *d_scores,
                          int32_t stride, void *d_score, void *d_nogap, void *d_egap, void *d_fgap, void *stream);
 
+/* ---- Splice-junction ends: the whole caller on the device ------------- */
+
+/* One end of Dynprog_end5_splicejunction (dynprog_end.c:1652-1922) or
+   Dynprog_end3_splicejunction (2248-2745), 56 bytes, pad zero.  The read
+   end and its genome window are stored in sweep order: the 5' form's
+   rev_rsequence and rev_gsequence reversed (the bases from the anchor
+   outwards), the 3' form's rsequence and gsequence as they are; the window's
+   alternate allele follows it (gl bytes each from goff). */
+typedef struct gdp_sj_end {
+  int64_t qoff;        /* the read end in qbuf */
+  int64_t goff;        /* its genome window in gbuf, then the alternate allele */
+  int32_t rl, gl;      /* length1 (read), length2 (window) */
+  int32_t roffset;     /* revoffset1 (5') / offset1 (3') */
+  int32_t anchor;      /* revoffset2_anchor (5') / offset2_anchor (3') */
+  int32_t far;         /* revoffset2_far (5') / offset2_far (3') */
+  int32_t contlength;  /* the continuation column where the first walk stops */
+  int16_t extraband;   /* extraband_end */
+  int8_t genestrand;
+  int8_t jump_late;    /* the caller's jump_late_p */
+  int32_t pad;
+  double defect_rate;
+} gdp_sj_end;
+
+/* A pair record of a splice-junction end: gdp_pair with a 32-bit genomejump,
+   since the known-site gap holder (gapp 2) spans the intron. */
+typedef struct gdp_sj_pair {
+  int32_t querypos;
+  int32_t genomepos;
+  int8_t cdna;
+  int8_t comp;
+  int8_t genome;
+  int8_t genomealt;
+  int16_t gapp;
+  int16_t pad;
+  int32_t genomejump;
+} gdp_sj_pair;
+
+/* Per end: npairs -1 for the reference's NULL list, else the number of
+   records at pair_off of the batch's pair array, in the order the returned
+   list holds them from its last element to its head (the 5' form's push
+   order; the 3' form's reversed); counters = {traceback_score (finalscore),
+   missscore, nmatches, nmismatches, nopens, nindels}. */
+typedef struct gdp_sj_result {
+  int64_t pair_off;
+  int32_t npairs;
+  int32_t counters[6];
+  int32_t pad;
+} gdp_sj_result;
+
+/* A batch object owning the device and host buffers of gdp_sj_run; they
+   grow to the largest batch run through it and are reused. */
+typedef struct gdp_sj_batch gdp_sj_batch;
+gdp_sj_batch *gdp_sj_create(void);
+void gdp_sj_destroy(gdp_sj_batch *b);
+
+/* Runs n ends (five = 1: the 5' caller, 0: the 3' caller) from host buffers
+   qbuf / gbuf of qbytes / gbytes bytes: per end the upper and lower fill of
+   the read against its window (ENDQ scores, END_OPEN by defect rate with
+   END_EXTEND -2, bands widened to the far corner, int8 when the read or the
+   window is shorter than use8p_size[ENDQ], the 5' fills with the opposite
+   tie rule), the best cell of the last row
+   (find_best_endpoint_to_queryend_indels_{8,16}), and when it is not below
+   0 the two local walks (to contlength, then from where that one stopped to
+   column 0) joined by the known gap holder of far - anchor (5') or anchor -
+   far (3') genome columns, with indel pairs at the read's end dropped
+   (1797-1910, 2385-2490).  Ends of length 0 or past max_rlength /
+   max_glength, and ends whose best cell scores below 0, get npairs -1.  The
+   fills, scans, walks and the join all run on the device; the results and
+   pairs are copied back to the batch's host arrays before it returns. */
+int gdp_sj_run(gdp_sj_batch *b, int32_t five, const gdp_sj_end *ends, int32_t n, const char *qbuf, int64_t qbytes,
+               const char *gbuf, int64_t gbytes, int32_t max_rlength, int32_t max_glength, void *stream);
+/* The last run's pair count, per-end results and pair records (owned by the
+   batch, valid until its next run). */
+int64_t gdp_sj_npairs(const gdp_sj_batch *b);
+const gdp_sj_result *gdp_sj_results(const gdp_sj_batch *b);
+const gdp_sj_pair *gdp_sj_pairs(const gdp_sj_batch *b);
+
 /* ---- Serving: continuous batching of alignment requests -------------- */
 
 /* A request loop over a device-resident genome.  Each request is the DP of

@@ -70,7 +70,12 @@ def test_splicejunction(oracle, five, long_jumps):
     from gmapdp.ends import end3_splicejunction, end5_splicejunction
     rng = np.random.default_rng((36 if five else 38) + (100 if long_jumps else 0))
     ends = H.random_splice_ends(rng, 400, long_jumps)
-    got = (end5_splicejunction if five else end3_splicejunction)(ends)
+    # an empty read end and one past the workspace's 1000 rows: NULL lists
+    # (dynprog_end.c:1702-1717), as is every end whose best cell scores below 0
+    edge = [dict(ends[0], rev_rseq=b""), dict(ends[1], rev_rseq=H.rand_seq(rng, 1001), rev_gseq=H.rand_seq(rng, 1010),
+                                              rev_galt=H.rand_seq(rng, 1010))]
+    got = (end5_splicejunction if five else end3_splicejunction)(ends + edge)
+    assert got[-2] is None and got[-1] is None
     walked = 0
     for k, e in enumerate(ends):
         want = H.oracle_end5_splicejunction(oracle, e, five)
