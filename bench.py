@@ -55,6 +55,9 @@ WORKLOADS = {
                          "traceback) mixed 1:1 with single-gap logU[20,1000] problems, 2e6 + 2e6"),
     "cdna": ("cdna", 200_000, "cDNA gaps gl=U{16..160}, both ends rl=gl+8, extraband 14, open/extend -10/-7 "
                               "(four one-sided fills keeping every score, the bridge, two end tracebacks), 2e5 gaps"),
+    "intron": ("intron", 200_000, "intron gaps rl=U{40..250}, both segments gl=rl+8, extraband 14, open/extend -8/-3 "
+                                  "(four one-sided fills keeping every score, the site-level intron bridge over "
+                                  "per-column splice-site probabilities, two end tracebacks), 2e5 gaps"),
 }
 
 
@@ -259,6 +262,10 @@ def main():
         from gmapdp.workload import CdnaGaps
         wl = CdnaGaps(n)
         args.percall, args.no_cpu = 0, True  # neither form covers the bridge
+    elif config == "intron":
+        from gmapdp.workload import IntronGaps
+        wl = IntronGaps(n)
+        args.percall, args.no_cpu = 0, True
     else:
         wl = Workload(config, n, variant=1 if args.workload == "c1lowq" else 0)
     probs = wl.problems
@@ -296,6 +303,13 @@ def main():
         d_quads = torch.from_numpy(wl.quads.reshape(-1).copy()).to(b.dev)
         d_tmpl = torch.from_numpy(reqs.view(np.uint8).copy()).to(b.dev)
         d_best = torch.zeros(5 * wl.n, dtype=torch.int32, device=b.dev)
+    elif config == "intron":
+        reqs = wl.templates()
+        d_gaps = torch.from_numpy(wl.gaps.view(np.uint8).copy()).to(b.dev)
+        d_sprobs = torch.from_numpy(wl.probs).to(b.dev)
+        d_tmpl = torch.from_numpy(reqs.view(np.uint8).copy()).to(b.dev)
+        d_best = torch.zeros(5 * wl.n, dtype=torch.int32, device=b.dev)
+        d_bestp = torch.zeros(2 * wl.n, dtype=torch.float64, device=b.dev)
     else:
         reqs = wl.tb_requests()
     reqs_dev = b.prepare_traceback(reqs, genome=wl.genome())
@@ -308,6 +322,9 @@ def main():
                                      reqs_dev["d_reqs"].data_ptr() + wl.n3 * TBREQ_DTYPE.itemsize, stream=st)
         elif config == "cdna":
             b.bridge_requests(d_quads, wl.n, d_best, d_tmpl.data_ptr(), reqs_dev["d_reqs"].data_ptr(), stream=st)
+        elif config == "intron":
+            b.bridge_intron_requests(d_gaps, wl.n, d_sprobs, d_best, d_bestp, d_tmpl.data_ptr(),
+                                     reqs_dev["d_reqs"].data_ptr(), stream=st)
         b.traceback_device(reqs_dev, stream=st)
 
     def step(ev=None, tev=None):
@@ -370,12 +387,22 @@ def main():
         torch.cuda.synchronize()
         bridge_ms = bev[0].elapsed_time(bev[1]) / args.steps
         log("cDNA bridge: %.3f ms for %d gaps" % (bridge_ms, wl.n))
+    elif config == "intron":
+        bev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        bev[0].record(stream)
+        for _ in range(args.steps):
+            b.bridge_intron_requests(d_gaps, wl.n, d_sprobs, d_best, d_bestp, d_tmpl.data_ptr(),
+                                     reqs_dev["d_reqs"].data_ptr())
+        bev[1].record(stream)
+        torch.cuda.synchronize()
+        bridge_ms = bev[0].elapsed_time(bev[1]) / args.steps
+        log("intron bridge: %.3f ms for %d gaps" % (bridge_ms, wl.n))
 
     # PCIe-inclusive step: inputs from pinned host memory, the step, counts and
     # pair records back (the pair buffer is downloaded whole: rl + gl + 4
     # records per request, of which the walk fills its count).
     e2e = None
-    if not args.no_e2e and config not in (4, "cdna"):
+    if not args.no_e2e and config not in (4, "cdna", "intron"):
         h_in = [torch.empty(t.numel(), dtype=t.dtype, pin_memory=True) for t in (b.d_descs, b.d_q, b.d_g,
                                                                                   reqs_dev["d_reqs"])]
         for h, t in zip(h_in, (b.d_descs, b.d_q, b.d_g, reqs_dev["d_reqs"])):
@@ -454,11 +481,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"c1lowq": "int8", "c3": "int16+int8", "c4": "int16+int8", "cdna": "int16+int8"}.get(
+        "dtype": {"c1lowq": "int8", "c3": "int16+int8", "c4": "int16+int8", "cdna": "int16+int8",
+                  "intron": "int16+int8"}.get(
             args.workload, "int16"),
         "data": "synthetic (seeded generator of the workload)",
         "config": {"workload": desc, "problems": len(probs), "cells": total_cells,
-                   "step": {4: "fill + best end cell + traceback", "cdna": "fill + bridge + traceback"}.get(
+                   "step": {4: "fill + best end cell + traceback", "cdna": "fill + bridge + traceback",
+                            "intron": "fill + intron bridge + traceback"}.get(
                        config, "fill + traceback"),
                    "launch": "eager" if graph is None else "one HIP graph per step",
                    "lane_groups": None if params is None else {"g16_max_span": params[0],
@@ -476,7 +505,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS, "unit": "TOP/s",
                      "frac": round(achieved / PEAK_INT16_TOPS, 4), "traffic": traffic,
                      "kernel": ("gdp_fill_kernel (one fill of one-sided problems keeping every score)"
-                                if config == "cdna" else
+                                if config in ("cdna", "intron") else
                                 "gdp_fill2_kernel (one fill: every launch of gdp_fill, HIP events on its stream)"),
                      "ops_per_cell": OPS_PER_CELL},
         "cpu_baseline": cpu,

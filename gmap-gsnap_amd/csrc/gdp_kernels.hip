@@ -1696,9 +1696,12 @@ __global__ __launch_bounds__(64) void gdp_bridge_intron_kernel(
         drow = rL;
       }
     }
-    // left end on its diagonal, right end off it
-    int cR = cloR;
-    for (; cR < rR && cR < D - rL; cR++) {
+    // left end on its diagonal, right end off it: the reference's loop
+    // runs cR from cloR while cR < rR and cR < span - rL, then, skipping the
+    // column it stopped at, on while cR < chighR and cR < span - rL
+    const int e2 = gdp_imin(rR, D - rL), e3 = gdp_imin(chighR, D - rL);
+#pragma unroll 4
+    for (int cR = cloR; cR < e2; cR++) {
       const int v = sLd + rew[siteL & rdi[cR]] + score_at(scores, Rl, GRl, cR, rR);
       const double p = pLd + pR[cR];
       if (v > bv || (v == bv && p > bp)) {
@@ -1709,7 +1712,8 @@ __global__ __launch_bounds__(64) void gdp_bridge_intron_kernel(
         bcR = cR;
       }
     }
-    for (cR++; cR < chighR && cR < D - rL; cR++) {
+#pragma unroll 4
+    for (int cR = gdp_imax(cloR, e2) + 1; cR < e3; cR++) {
       const int v = sLd + rew[siteL & rdi[cR]] + score_at(scores, Ru, GRu, rR, cR);
       const double p = pLd + pR[cR];
       if (v > bv || (v == bv && p > bp)) {
@@ -1720,9 +1724,10 @@ __global__ __launch_bounds__(64) void gdp_bridge_intron_kernel(
         bcR = cR;
       }
     }
-    // right end on its diagonal, left end off it
-    int cL = cloL;
-    for (; cL < rL && cL < D - rR; cL++) {
+    // right end on its diagonal, left end off it, likewise
+    const int f2 = gdp_imin(rL, D - rR), f3 = gdp_imin(chighL, D - rR);
+#pragma unroll 4
+    for (int cL = cloL; cL < f2; cL++) {
       const int v = score_at(scores, Ll, GLl, cL, rL) + rew[ldi[cL] & siteR] + sRd;
       const double p = pL[cL] + pRd;
       if (v > bv || (v == bv && p > bp)) {
@@ -1733,7 +1738,8 @@ __global__ __launch_bounds__(64) void gdp_bridge_intron_kernel(
         bcR = rR;
       }
     }
-    for (cL++; cL < chighL && cL < D - rR; cL++) {
+#pragma unroll 4
+    for (int cL = gdp_imax(cloL, f2) + 1; cL < f3; cL++) {
       const int v = score_at(scores, Lu, GLu, rL, cL) + rew[ldi[cL] & siteR] + sRd;
       const double p = pL[cL] + pRd;
       if (v > bv || (v == bv && p > bp)) {

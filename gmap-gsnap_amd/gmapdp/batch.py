@@ -227,6 +227,15 @@ class DeviceBatch:
             return out, outp
         return out, outp, d_tb.cpu().numpy().view(TBREQ_DTYPE)
 
+    def bridge_intron_requests(self, d_gaps, m, d_probs, d_out, d_outp, d_tmpl, d_tbout, stream=None):
+        """gdp_bridge_intron on device buffers: joins into d_out / d_outp and
+        both ends' traceback requests (templates at address d_tmpl completed)
+        to address d_tbout."""
+        _check(self.L, self.L.gdp_bridge_intron(self.d_descs.data_ptr(), d_gaps.data_ptr(), m, self.d_scores.data_ptr(),
+                                                self.d_g.data_ptr(), d_probs.data_ptr(), d_out.data_ptr(),
+                                                d_outp.data_ptr(), d_tmpl, d_tbout,
+                                                self.stream if stream is None else stream), "gdp_bridge_intron")
+
     def bridge_requests(self, d_quads, m, d_out, d_tmpl, d_tbout, stream=None):
         """gdp_bridge_cdna on device buffers: joins into d_out and both ends'
         traceback requests (templates at address d_tmpl completed) to address

@@ -4,13 +4,13 @@ import ctypes
 
 import numpy as np
 
-from .lib import PROBLEM_DTYPE, TBREQ_DTYPE, load
+from .lib import INTRON_GAP_DTYPE, PROBLEM_DTYPE, TBREQ_DTYPE, load
 
 META_DTYPE = np.dtype([("queryoffset", "<i4"), ("genomeoffset", "<i4"), ("chroffset", "<u4"), ("chrhigh", "<u4"),
                        ("watsonp", "i1"), ("revp", "i1"), ("pad", "i1", (2,)), ("pair", "<i4")])
 assert META_DTYPE.itemsize == 24
 
-SEEDS = {1: 1001, 2: 1002, 3: 1003, 4: 1004, 5: 1005, "cdna": 1006}
+SEEDS = {1: 1001, 2: 1002, 3: 1003, 4: 1004, 5: 1005, "cdna": 1006, "intron": 1007}
 
 
 def _bind(L):
@@ -224,6 +224,105 @@ class CdnaGaps:
         req["queryoffset"][0::2] = self.rev_roffset  # walked with revp
         req["queryoffset"][1::2] = 0  # roffsetL
         req["genomeoffset"][0::2] = self.goffset + gl - 1  # rev_goffset
+        req["genomeoffset"][1::2] = self.goffset
+        req["revp"][0::2] = 1
+        req["chroffset"], req["chrhigh"], req["watsonp"] = 0, len(self._genome), 1
+        req["max_pairs"] = np.repeat(rl + gl + 4, 2)
+        req["pair_off"][1:] = np.cumsum(req["max_pairs"].astype(np.int64))[:-1]
+        return req
+
+
+class IntronGaps:
+    """Intron gaps as stage 3 hands them to Dynprog_genome_gap
+    (stage3.c:9538-9576): a read gap of rl = U{40..250} bases, both genome
+    segments rl + 8 long (extramaterial_paired, gmap.c:283), extraband_paired
+    14 (gmap.c:292), the paired HIGHQ penalties -8/-3 (dynprog.h:69-75) and
+    the width rule of dynprog_genome.c:3504-3508.  The read is the end of a
+    left exon (before a GT) and the start of a right exon (after an AG) with
+    2 % substitutions; the segments are random bases around them.  The
+    splice-site probabilities are synthetic per-column values in steps of
+    0.05 (the caller's site model stands behind them in GMAP).
+
+    problems[4k .. 4k+3]: left upper, left lower (the read forward against the
+    left segment), right upper, right lower (the read backwards against the
+    right segment read backwards, the other tie rule), kept with every
+    score; gaps: the rows of gdp_bridge_intron; probs: the site
+    probabilities; genome(): the segments end to end, gap k's left one at
+    goffset[k] and its right one just after."""
+
+    OPEN, EXTEND, EXTRABAND, EXTRAMATERIAL, INTRON = -8, -3, 14, 8, 5000
+
+    def __init__(self, n, seed=None, rl_range=(40, 250)):
+        L = load()
+        L.gdp_host_tables(0)
+        rng = np.random.default_rng(SEEDS["intron"] if seed is None else seed)
+        use8 = int(L.gdp_use8p_size(0))
+        acgt = np.frombuffer(b"ACGT", np.uint8)
+        rls = rng.integers(rl_range[0], rl_range[1] + 1, n).astype(np.int64)
+        gls = rls + self.EXTRAMATERIAL
+        a = rng.integers(1, rls)  # read bases from the left exon
+        seg = acgt[rng.integers(0, 4, int(2 * gls.sum()))]
+        self.goffset = np.zeros(n, np.int64)
+        self.goffset[1:] = np.cumsum(2 * gls)[:-1]
+        qbuf = np.empty(int(2 * rls.sum()) + 8, np.uint8)
+        gbuf = np.empty(int(2 * gls.sum()) + 8, np.uint8)
+        qbuf[-8:] = 0
+        gbuf[-8:] = 0
+        qo = np.zeros(n, np.int64)
+        qo[1:] = np.cumsum(2 * rls)[:-1]
+        for k in range(n):
+            rl, gl, ak, g0 = int(rls[k]), int(gls[k]), int(a[k]), int(self.goffset[k])
+            left, right = seg[g0: g0 + gl], seg[g0 + gl: g0 + 2 * gl]
+            left[ak: ak + 2] = (ord("G"), ord("T"))[: max(0, min(2, gl - ak))]
+            b = rl - ak
+            if gl - b - 2 >= 0:
+                right[gl - b - 2: gl - b] = (ord("A"), ord("G"))
+            read = np.concatenate([left[:ak], right[gl - b:]])
+            sub = rng.random(rl) < 0.02
+            read[sub] = acgt[rng.integers(0, 4, int(sub.sum()))]
+            q0 = int(qo[k])
+            qbuf[q0: q0 + rl] = read
+            qbuf[q0 + rl: q0 + 2 * rl] = read[::-1]
+            gbuf[g0: g0 + gl] = left
+            gbuf[g0 + gl: g0 + 2 * gl] = right[::-1]
+        self._genome = seg
+        probs = np.zeros(4 * n, PROBLEM_DTYPE)
+        for j in range(4):
+            p = probs[j::4]
+            right = j >= 2
+            p["qoff"] = qo + (rls if right else 0)
+            p["goff"] = self.goffset + (gls if right else 0)
+            p["galtoff"] = p["goff"]
+            p["rl"], p["gl"] = rls, gls
+            # Dynprog_compute_bands with widebandp, gl >= rl (dynprog.c:1245-1261)
+            p["lband"], p["uband"] = self.EXTRABAND, gls - rls + self.EXTRABAND
+            p["kind"] = 1 + (j & 1)
+            p["width"] = np.where((rls < use8) | (gls < use8), 8, 16)
+            p["jump_late"] = 1 if right else 0
+            p["open"], p["extend"], p["keep_scores"] = self.OPEN, self.EXTEND, 1
+        self.problems = probs
+        self.qbuf, self.gbuf = qbuf, gbuf
+        self.probs = (rng.integers(0, 21, int(2 * gls.sum())) / 20.0).astype(np.float64)
+        gaps = np.zeros(n, INTRON_GAP_DTYPE)
+        gaps["fills"] = np.stack([4 * np.arange(n) + j for j in range(4)], 1)
+        gaps["span"] = rls + self.INTRON  # rev_goffsetR - goffsetL
+        gaps["cdna_direction"] = 1
+        gaps["prob_off"] = 2 * np.concatenate([[0], np.cumsum(gls)[:-1]])
+        self.gaps = gaps
+        self.n = n
+
+    def genome(self):
+        return self._genome
+
+    def templates(self):
+        """Traceback request templates, right end then left end per gap
+        (fill, r and c are set by gdp_bridge_intron); pair_off laid out."""
+        p = self.problems
+        req = np.zeros(2 * self.n, TBREQ_DTYPE)
+        rl, gl = p["rl"][0::4], p["gl"][0::4]
+        req["queryoffset"][0::2] = rl - 1  # rev_roffset, walked with revp
+        req["queryoffset"][1::2] = 0  # roffset
+        req["genomeoffset"][0::2] = self.goffset + 2 * gl - 1  # rev_goffsetR
         req["genomeoffset"][1::2] = self.goffset
         req["revp"][0::2] = 1
         req["chroffset"], req["chrhigh"], req["watsonp"] = 0, len(self._genome), 1
