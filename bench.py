@@ -226,9 +226,12 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default lane groups instead of the autotuner's")
     ap.add_argument("--lane-groups", default="", metavar="SPAN,ROWS",
                     help="fixed lane groups (g16_max_span,g64_min_rows) instead of the autotuner's")
-    ap.add_argument("--mix-span", type=int, default=16,
-                    help="band spread allowed in mixed paired waves of the batch (0: one band per wave); "
-                         "16 measured best on c3 (fill 77.0 ms against 85.1 ms with one band per wave)")
+    ap.add_argument("--mix-span", type=int, default=-1,
+                    help="band spread allowed in mixed paired waves of the batch (0: one band per wave; default: "
+                         "the autotuner's choice, or 8 with --lane-groups / --no-tune)")
+    ap.add_argument("--mix-order", type=int, default=1, choices=(0, 1),
+                    help="how mixed waves are formed with --mix-span (gdp_plan_params.mix_order): 0 leftovers in "
+                         "order of band, 1 every problem binned by band and taken in order of rows within a bin")
     ap.add_argument("--serve-mix", type=int, default=0,
                     help="band spread allowed in the server's mixed waves (0: one band per wave; serving "
                          "saturation measured the same at 0, 24 and 64)")
@@ -283,9 +286,11 @@ def main():
         params, table = autotune(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes(), sample=min(len(probs), 1_000_000))
         tuning = {"objective": "throughput", "sample": min(len(probs), 1_000_000), "chosen": list(params),
                   "candidates": [[list(p), round(ms, 3)] for p, ms in table], "seconds": round(time.time() - t1, 1)}
-        log("autotuner: g16_max_span=%d g64_min_rows=%d (%.1fs)" % (params[0], params[1], time.time() - t1))
-    if args.mix_span:
-        params = tuple(params or (240, 8001)) + (args.mix_span,)
+        log("autotuner: g16_max_span=%d g64_min_rows=%d mix_span=%d mix_order=%d (%.1fs)" % (params + (time.time() - t1,)))
+    if args.mix_span >= 0:
+        params = tuple(params or (240, 8001))[:2] + (args.mix_span, args.mix_order)
+    elif params is None or len(params) == 2:
+        params = tuple(params or (240, 8001))[:2] + (8, 1)
     b = DeviceBatch(probs, wl.qbuf.tobytes(), wl.gbuf.tobytes(), params=params)
     stream = torch.cuda.current_stream()
     graph = None
@@ -490,9 +495,8 @@ def main():
                             "intron": "fill + intron bridge + traceback"}.get(
                        config, "fill + traceback"),
                    "launch": "eager" if graph is None else "one HIP graph per step",
-                   "lane_groups": None if params is None else {"g16_max_span": params[0],
-                                                                 "g64_min_rows": params[1]},
-                   "mix_span": args.mix_span},
+                   "lane_groups": {"g16_max_span": params[0], "g64_min_rows": params[1]},
+                   "mix_span": params[2], "mix_order": params[3]},
         "fill_ms": round(fill_ms, 3),
         "traceback_ms": round(tb_ms, 3),
         "fill_gcups": round(total_cells / (fill_ms * 1e-3) / 1e9, 3),

@@ -191,9 +191,11 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
   }
   const gdp_problem *probs = narrowed.data();
   const gdp_plan_params pp = params ? *params : g_default_params;
-  if (pp.g16_max_span < 0 || pp.g16_max_span > 240 || pp.g64_min_rows < 1 || pp.mix_span < 0)
+  if (pp.g16_max_span < 0 || pp.g16_max_span > 240 || pp.g64_min_rows < 1 || pp.mix_span < 0 ||
+      pp.mix_span > 64 || pp.mix_order < 0 || pp.mix_order > 1)
     return fail(GDP_EINVAL,
-                "gdp_plan: g16_max_span must lie in [0, 240], g64_min_rows be positive and mix_span not negative");
+                "gdp_plan: g16_max_span must lie in [0, 240], g64_min_rows be positive, mix_span in [0, 64] and "
+                "mix_order 0 or 1");
   uint8_t *base = (uint8_t *)descs_out;
   gdp_dev_problem *d = (gdp_dev_problem *)descs_out;
   int64_t dw = 0, ss = 0;
@@ -323,6 +325,52 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
   };
   std::vector<int32_t> rest;  // problems left for mixed waves
   size_t k = 0;
+  if (pp.mix_span > 0 && pp.mix_order == 1) {
+    // Bucketed mixed waves.  A wave costs (its padded row batches) x (the
+    // period of its largest bands), so its problems should agree in rows as
+    // much as in bands: problems are binned by lband / B and uband / B and
+    // taken in order of rl within a bin, full waves first; what is left of
+    // the bins is binned again by 2B.
+    auto bucketed = [&](std::vector<int32_t> &list, int B, bool full_only, std::vector<int32_t> &left) {
+      std::stable_sort(list.begin(), list.end(), [&](int32_t a, int32_t b) {
+        const gdp_problem &x = probs[a], &y = probs[b];
+        const int kx = pair_lanes(x, pp), ky = pair_lanes(y, pp);
+        if (kx != ky) return kx < ky;
+        if ((x.jump_late != 0) != (y.jump_late != 0)) return (x.jump_late != 0) < (y.jump_late != 0);
+        if (x.lband / B != y.lband / B) return x.lband / B < y.lband / B;
+        if (x.uband / B != y.uband / B) return x.uband / B < y.uband / B;
+        if (x.rl != y.rl) return x.rl < y.rl;
+        return x.gl < y.gl;
+      });
+      size_t a = 0;
+      while (a < list.size()) {
+        const gdp_problem &p0 = probs[list[a]];
+        const int G = pair_lanes(p0, pp), slots = 2 * (64 / G);
+        const int maxspan = G == 16 ? 240 : 960;
+        int lb = p0.lband, ub = p0.uband;
+        size_t e = a + 1;
+        while (e < list.size() && e - a < (size_t)slots) {
+          const gdp_problem &x = probs[list[e]];
+          if (pair_lanes(x, pp) != G || (x.jump_late != 0) != (p0.jump_late != 0) || x.lband / B != p0.lband / B ||
+              x.uband / B != p0.uband / B || gdp_imax(lb, x.lband) + gdp_imax(ub, x.uband) > maxspan)
+            break;
+          lb = gdp_imax(lb, x.lband);
+          ub = gdp_imax(ub, x.uband);
+          e++;
+        }
+        if (full_only && e - a < (size_t)slots) {
+          for (size_t j = a; j < e; j++) left.push_back(list[j]);
+        } else {
+          emit(list.data() + a, (int)(e - a), G);
+        }
+        a = e;
+      }
+    };
+    std::vector<int32_t> left, none;
+    bucketed(paired, pp.mix_span, true, left);
+    bucketed(left, 2 * pp.mix_span, false, none);
+    k = paired.size();
+  }
   while (k < paired.size()) {
     const gdp_problem &p0 = probs[paired[k]];
     const int G = pair_lanes(p0, pp), slots = 2 * (64 / G);

@@ -40,11 +40,12 @@ def _check(L, rc, what):
 PLAN_SINGLE = 1  # GDP_PLAN_SINGLE: every problem swept on its own
 
 
-def plan_params(g16_max_span=240, g64_min_rows=8001, mix_span=0):
-    """gdp_plan_params: lane-group choice of the paired sweep and the band
-    spread allowed in mixed waves (0: one band per wave)."""
+def plan_params(g16_max_span=240, g64_min_rows=8001, mix_span=0, mix_order=0):
+    """gdp_plan_params: lane-group choice of the paired sweep, the band
+    spread allowed in mixed waves (0: one band per wave) and how mixed waves
+    are formed (1: binned by band, in order of rows within a bin)."""
     p = np.zeros(1, PLAN_PARAMS_DTYPE)
-    p["g16_max_span"], p["g64_min_rows"], p["mix_span"] = g16_max_span, g64_min_rows, mix_span
+    p["g16_max_span"], p["g64_min_rows"], p["mix_span"], p["mix_order"] = g16_max_span, g64_min_rows, mix_span, mix_order
     return p
 
 

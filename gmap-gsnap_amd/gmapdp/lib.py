@@ -70,7 +70,7 @@ ALIGN_RES_DTYPE = np.dtype([("tag", "<u8"), ("status", "<i4"), ("finalscore", "<
                             ("t_done", "<f8")])
 INTRON_GAP_DTYPE = np.dtype([("fills", "<i4", (4,)), ("span", "<i4"), ("cdna_direction", "i1"), ("finalp", "i1"),
                              ("halfp", "i1"), ("pad", "i1"), ("prob_off", "<i8")])
-PLAN_PARAMS_DTYPE = np.dtype([("g16_max_span", "<i4"), ("g64_min_rows", "<i4"), ("mix_span", "<i4"), ("pad", "<i4")])
+PLAN_PARAMS_DTYPE = np.dtype([("g16_max_span", "<i4"), ("g64_min_rows", "<i4"), ("mix_span", "<i4"), ("mix_order", "<i4")])
 SERVER_CFG_DTYPE = np.dtype([("dir_budget_bytes", "<i8"), ("pair_capacity", "<i8"), ("max_batch", "<i4"),
                              ("slots", "<i4"), ("max_rl", "<i4"), ("max_gl", "<i4"), ("result_buffers", "<i4"),
                              ("long_rows", "<i4"), ("plan", PLAN_PARAMS_DTYPE)])

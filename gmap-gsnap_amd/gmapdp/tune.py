@@ -15,6 +15,15 @@ from .batch import DeviceBatch, _torch
 
 SPANS = (240, 112)
 ROWS = {"throughput": (8001, 512, 256), "latency": (8001, 512, 256, 128)}
+# Throughput candidates (g16_max_span, g64_min_rows, mix_span, mix_order):
+# mixed waves binned by band and rows (mix_order 1) against the leftover
+# mixing of one band per wave first (mix_order 0).  Measured on the c3 mix
+# (2e6 problems): (240, 8001, 4, 1) 71.9 ms, (240, 8001, 8, 1) 73.6 ms,
+# (240, 8001, 16, 1) 75.2 ms, (240, 512, 16, 1) 80.1 ms, (240, 512, 16, 0)
+# 76.8 ms, (112, 8001, 8, 1) 85.5 ms; 16-lane groups cost less per step
+# than 64-lane ones, whose band edges stretch over four times as many steps.
+THROUGHPUT = ((240, 8001, 4, 1), (240, 8001, 2, 1), (240, 8001, 8, 1), (240, 8001, 16, 1), (240, 512, 4, 1),
+              (240, 512, 16, 0))
 
 
 def _time_fill(b, reps):
@@ -31,8 +40,9 @@ def _time_fill(b, reps):
 
 
 def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", reps=2, seed=0, longest=64):
-    """Returns ((g16_max_span, g64_min_rows), table) where table lists
-    (params, milliseconds) of every candidate."""
+    """Returns (params, table) where table lists (params, milliseconds) of
+    every candidate: params = (g16_max_span, g64_min_rows, mix_span,
+    mix_order) for throughput, (g16_max_span, g64_min_rows) for latency."""
     rng = np.random.default_rng(seed)
     n = len(problems)
     if objective == "latency":
@@ -41,11 +51,11 @@ def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", rep
         idx = np.sort(rng.choice(n, size=min(sample, n), replace=False))
     sub = np.ascontiguousarray(problems[idx])
     table = []
-    for span in SPANS:
-        for rows in ROWS[objective]:
-            b = DeviceBatch(sub, qbuf, gbuf, params=(span, rows))
-            table.append(((span, rows), _time_fill(b, reps)))
-            del b
+    cands = THROUGHPUT if objective == "throughput" else [(s, r) for s in SPANS for r in ROWS[objective]]
+    for prm in cands:
+        b = DeviceBatch(sub, qbuf, gbuf, params=prm)
+        table.append((prm, _time_fill(b, reps)))
+        del b
     best = min(table, key=lambda t: t[1])[0]
     return best, table
 

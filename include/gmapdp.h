@@ -235,7 +235,13 @@ typedef struct gdp_plan_params {
      largest and every problem keeps its own band edges.  0: a wave holds
      problems of one band only. */
   int32_t mix_span;
-  int32_t pad;
+  /* With mix_span > 0, how mixed waves are formed.  0: the problems left
+     over from full waves of one band, in order of band width, lower band
+     and rl.  1: every problem binned by lband / mix_span and
+     uband / mix_span and taken in order of rl within its bin (a wave's
+     problems then agree in rows as well as in bands), full waves first,
+     what is left binned again by 2 * mix_span. */
+  int32_t mix_order;
 } gdp_plan_params;
 
 /* As gdp_plan_ex with explicit lane-group parameters (NULL = defaults). */

@@ -134,13 +134,16 @@ def _mixed_specs(seed, n, late, rl_range, lb_range, ub_extra, alph_q=b"ACGT", ge
     return specs
 
 
-@pytest.mark.parametrize("params", [(240, 8001, 16), (112, 256, 24), (0, 8001, 40), (240, 1, 8)])
+@pytest.mark.parametrize("params", [(240, 8001, 16), (112, 256, 24), (0, 8001, 40), (240, 1, 8), (240, 8001, 16, 1),
+                                    (112, 256, 8, 1), (240, 1, 24, 1), (0, 8001, 32, 1)])
 def test_mixed_band_waves(oracle, params):
     """Waves whose problems have different bands (gdp_plan_params.mix_span):
     the sweep is laid out for the wave's largest bands and each half applies
     its own problem's row starts, block prefix, bottom edge and top edge.
     Both tie rules, narrow bands (bottom edges near column 0), 16- and
-    64-lane groups, and the general lookup."""
+    64-lane groups, the general lookup, and both ways of forming the mixed
+    waves (mix_order 1 bins every problem by band, then by rows, and bins
+    what is left again at twice the spread)."""
     specs = []
     specs += _mixed_specs(51, 48, False, (1, 120), (0, 40), (0, 60))
     specs += _mixed_specs(52, 48, True, (20, 300), (3, 64), (0, 120))
