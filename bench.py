@@ -239,7 +239,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=600_000)
     ap.add_argument("--serve-batch", type=int, default=0,
                     help="largest batch the server forms (0: measured choice with --serve-long)")
-    ap.add_argument("--serve-slots", type=int, default=2, help="batches in flight (streams)")
+    ap.add_argument("--serve-slots", type=int, default=3, help="batches in flight (streams) with --serve-batch")
     ap.add_argument("--serve-long", type=int, default=-1,
                     help="reads of at least this many bases are batched apart (0: one class; -1: measured)")
     ap.add_argument("--serve-budget", type=int, default=8192, help="direction budget in MiB")

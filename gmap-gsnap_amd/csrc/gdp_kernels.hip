@@ -2093,6 +2093,56 @@ extern "C" int gdp_launch_traceback(const void *d_descs, const gdp_tb_request *d
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Copies between device memory and pinned host memory mapped into the
+// device's address space, as kernels on the caller's stream.  A copy engine
+// serves the streams of a process in submission order, so a batch's upload
+// would wait for another stream's download; a kernel keeps every batch's
+// transfers inside its own stream.
+__global__ void gdp_copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, int64_t n16) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n16; k += (int64_t)gridDim.x * blockDim.x)
+    dst[k] = src[k];
+  __threadfence_system();
+}
+
+// A served batch's results to the host: the final scores and counters
+// (head_bytes from the start of the download area), then each request's
+// pair records only (counts[6 * i] of them at its pair_off), one workgroup
+// per request.
+__global__ void gdp_serve_download_kernel(const uint8_t *__restrict__ d_down, uint8_t *__restrict__ h_dst,
+                                          int64_t head_bytes, int64_t off_counts, int64_t off_pairs,
+                                          const gdp_tb_request *__restrict__ reqs, int n) {
+  const int64_t nh = head_bytes / 16;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nh; k += (int64_t)gridDim.x * blockDim.x)
+    ((uint4 *)h_dst)[k] = ((const uint4 *)d_down)[k];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int32_t np = ((const int32_t *)(d_down + off_counts))[6 * i];
+    const int64_t po = reqs[i].pair_off;
+    const uint4 *src = (const uint4 *)(d_down + off_pairs) + po;
+    uint4 *dst = (uint4 *)(h_dst + off_pairs) + po;
+    for (int j = threadIdx.x; j < np; j += blockDim.x) dst[j] = src[j];
+  }
+  __threadfence_system();
+}
+
+extern "C" int gdp_launch_copy(void *dst, const void *src, int64_t bytes, hipStream_t stream) {
+  if (bytes <= 0) return 0;
+  if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) return -1;
+  const int64_t n16 = bytes / 16;
+  const int blocks = (int)gdp_imin(1024, (int)((n16 + 255) / 256));
+  hipLaunchKernelGGL(gdp_copy_kernel, dim3(blocks), dim3(256), 0, stream, (uint4 *)dst, (const uint4 *)src, n16);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int gdp_launch_serve_download(const uint8_t *d_down, uint8_t *h_dst, int64_t head_bytes,
+                                         int64_t off_counts, int64_t off_pairs, const gdp_tb_request *d_reqs, int n,
+                                         hipStream_t stream) {
+  if (n <= 0) return 0;
+  if ((head_bytes & 15) || (off_pairs & 15) || ((uintptr_t)d_down & 15) || ((uintptr_t)h_dst & 15)) return -1;
+  hipLaunchKernelGGL(gdp_serve_download_kernel, dim3(gdp_imin(n, 4096)), dim3(64), 0, stream, d_down, h_dst,
+                     head_bytes, off_counts, off_pairs, d_reqs, n);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 extern "C" int gdp_launch_segments(const void *d_genome, int fmt, uint32_t glen, const gdp_segment *d_segs, int n,
                                    char *d_out, hipStream_t stream) {
   if (n <= 0) return 0;

@@ -28,6 +28,10 @@
 
 #include "gdp_internal.h"
 
+extern "C" int gdp_launch_copy(void *, const void *, int64_t, hipStream_t);
+extern "C" int gdp_launch_serve_download(const uint8_t *, uint8_t *, int64_t, int64_t, int64_t,
+                                         const gdp_tb_request *, int, hipStream_t);
+
 namespace {
 
 inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -111,6 +115,7 @@ struct Slot {
   hipEvent_t ev = nullptr, ev0 = nullptr;  // end and start of the batch on the stream
   double t_launch = 0;
   uint8_t *h_up = nullptr, *d_up = nullptr, *d_down = nullptr;
+  uint8_t *h_up_dev = nullptr;  // h_up as the device addresses it
   char *d_g = nullptr;
   int buf = -1;  // result buffer of the batch in flight
   bool busy = false;
@@ -149,6 +154,7 @@ struct gdp_server {
   std::condition_variable cv_done;
   std::deque<DoneBatch> done;
   std::vector<uint8_t *> rbuf;  // pinned result buffers (down_cap bytes each)
+  std::vector<uint8_t *> rbuf_dev;  // the same as the device addresses them
   std::vector<int> rfree;       // buffers neither in flight nor held by the consumer
   std::vector<int> handed;      // buffers of results the last poll returned
 
@@ -301,7 +307,9 @@ bool gdp_server::launch(Slot &s) {
       qpos += q.r.rl;
       gpos += q.r.gl;
     }
-    const gdp_plan_params *pp = (cfg.plan.g16_max_span || cfg.plan.g64_min_rows || cfg.plan.mix_span) ? &cfg.plan : nullptr;
+    const gdp_plan_params *pp =
+        (cfg.plan.g16_max_span || cfg.plan.g64_min_rows || cfg.plan.mix_span || cfg.plan.mix_order) ? &cfg.plan
+                                                                                                      : nullptr;
     if (gdp_plan_tuned(probs.data(), n, 0, pp, s.h_up, &dw, &ss) != GDP_OK) {
       // Requests and the plan parameters were validated before; a planning
       // failure is a bug: refuse this batch and stop like a HIP failure.
@@ -419,8 +427,9 @@ bool gdp_server::launch(Slot &s) {
   uint32_t *dirs = (uint32_t *)(d_dirs + s.dir_off);
   s.t_launch = now_s();
   hipError_t e = hipEventRecord(s.ev0, s.stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(s.d_up, s.h_up, (size_t)up_bytes, hipMemcpyHostToDevice, s.stream);
+  // transfers run as kernels on the slot's stream (gdp_launch_copy)
   int rc = e == hipSuccess ? GDP_OK : GDP_EHIP;
+  if (rc == GDP_OK && gdp_launch_copy(s.d_up, s.h_up_dev, align_up(up_bytes, 16), s.stream) != 0) rc = GDP_EHIP;
   if (rc == GDP_OK)
     rc = gdp_genome_segments(genome, fmt, glen, (const gdp_segment *)(s.d_up + off_segs), n, s.d_g, s.stream);
   if (rc == GDP_OK)
@@ -430,9 +439,12 @@ bool gdp_server::launch(Slot &s) {
     rc = gdp_traceback_ex(d_descs, (const gdp_tb_request *)(s.d_up + off_tb), n, (const char *)(s.d_up + off_q),
                           s.d_g, dirs, genome, fmt, glen, (gdp_pair *)(s.d_down + s.off_pairs),
                           (int32_t *)(s.d_down + s.off_counts), s.stream);
+  if (rc == GDP_OK &&
+      gdp_launch_serve_download(s.d_down, rbuf_dev[s.buf], s.off_pairs, s.off_counts, s.off_pairs,
+                                (const gdp_tb_request *)(s.d_up + off_tb), n, s.stream) != 0)
+    rc = GDP_EHIP;
   if (rc == GDP_OK) {
-    e = hipMemcpyAsync(rbuf[s.buf], s.d_down, (size_t)s.down_bytes, hipMemcpyDeviceToHost, s.stream);
-    if (e == hipSuccess) e = hipEventRecord(s.ev, s.stream);
+    e = hipEventRecord(s.ev, s.stream);
     rc = e == hipSuccess ? GDP_OK : GDP_EHIP;
   }
   if (rc != GDP_OK) {
@@ -543,11 +555,12 @@ extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const voi
     return nullptr;
   }
   // lane groups: all zero for gdp_plan's defaults, else what gdp_plan_tuned accepts
-  if ((cfg->plan.g16_max_span != 0 || cfg->plan.g64_min_rows != 0 || cfg->plan.mix_span != 0) &&
+  if ((cfg->plan.g16_max_span != 0 || cfg->plan.g64_min_rows != 0 || cfg->plan.mix_span != 0 ||
+       cfg->plan.mix_order != 0) &&
       (cfg->plan.g16_max_span < 0 || cfg->plan.g16_max_span > 240 || cfg->plan.g64_min_rows < 1 ||
-       cfg->plan.mix_span < 0)) {
-    set_err("gdp_server_create: plan.g16_max_span must lie in [0, 240], plan.g64_min_rows be positive and "
-            "plan.mix_span not negative");
+       cfg->plan.mix_span < 0 || cfg->plan.mix_span > 64 || cfg->plan.mix_order < 0 || cfg->plan.mix_order > 1)) {
+    set_err("gdp_server_create: plan.g16_max_span must lie in [0, 240], plan.g64_min_rows be positive, "
+            "plan.mix_span lie in [0, 64] and plan.mix_order be 0 or 1");
     return nullptr;
   }
   if (genome_format != GDP_GENOME_ASCII && genome_format != GDP_GENOME_COMP) {
@@ -574,7 +587,8 @@ extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const voi
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&sl.ev);
     if (e == hipSuccess) e = hipEventCreate(&sl.ev0);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_up, (size_t)s->up_cap, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_up, (size_t)s->up_cap + 16, hipHostMallocMapped);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&sl.h_up_dev, sl.h_up, 0);
     if (e == hipSuccess) e = hipMalloc((void **)&sl.d_up, (size_t)s->up_cap);
     if (e == hipSuccess) e = hipMalloc((void **)&sl.d_down, (size_t)s->down_cap);
     if (e == hipSuccess) e = hipMalloc((void **)&sl.d_g, (size_t)s->g_cap);
@@ -582,10 +596,14 @@ extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const voi
   }
   const int nbuf = cfg->result_buffers > 0 ? std::max(cfg->result_buffers, cfg->slots) : cfg->slots + 2;
   for (int k = 0; k < nbuf && e == hipSuccess; k++) {
-    uint8_t *h = nullptr;
-    e = hipHostMalloc((void **)&h, (size_t)s->down_cap, hipHostMallocDefault);
+    uint8_t *h = nullptr, *hd = nullptr;
+    e = hipHostMalloc((void **)&h, (size_t)s->down_cap, hipHostMallocMapped);
     if (e == hipSuccess) {
       s->rbuf.push_back(h);
+      e = hipHostGetDevicePointer((void **)&hd, h, 0);
+    }
+    if (e == hipSuccess) {
+      s->rbuf_dev.push_back(hd);
       s->rfree.push_back(k);
     }
   }

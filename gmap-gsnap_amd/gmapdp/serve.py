@@ -117,10 +117,12 @@ class Server:
         cfg["pair_capacity"] = pair_capacity or max_batch * (max_rl + max_gl + 2) // 2
         cfg["result_buffers"] = result_buffers
         cfg["long_rows"] = long_rows
-        if plan is not None:  # (g16_max_span, g64_min_rows[, mix_span])
+        if plan is not None:  # (g16_max_span, g64_min_rows[, mix_span[, mix_order]])
             cfg["plan"]["g16_max_span"], cfg["plan"]["g64_min_rows"] = plan[0], plan[1]
             if len(plan) > 2:
                 cfg["plan"]["mix_span"] = plan[2]
+            if len(plan) > 3:
+                cfg["plan"]["mix_order"] = plan[3]
         self.h = L.gdp_server_create(cfg.ctypes.data, genome.ptr, genome.fmt, genome.nbases)
         if not self.h:
             raise RuntimeError("gdp_server_create failed: %s" % L.gdp_last_error().decode())

@@ -63,13 +63,13 @@ def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", rep
 # (slots, max_batch, long_rows) candidates of the request server: batches in
 # flight, requests per batch, and the query length from which reads are
 # batched apart.  At half load a request waits for a batch slot and then for
-# its batch, whose time the longest reads set; a third slot lets short batches
-# run beside a long one.  Measured on the configuration-5 mix (3 runs each at
-# half load): 2 slots of 2048 with reads of 500 rows and more apart kept p99
-# near 4.3 ms at the highest saturation; 4096-request batches and 4 slots
-# stayed above 5 ms; (3, 1024, 200) trades a third of the saturation for
-# p99 near 4.2 ms.
-SERVER_KNOBS = ((2, 2048, 500), (2, 2048, 300), (3, 2048, 200), (3, 1024, 200))
+# its batch, whose time the longest reads set.  Measured on the
+# configuration-5 mix (3 runs each at half load) once each batch's transfers
+# ran as kernels on its own stream (before, a copy engine shared by the
+# streams ran one batch at a time): (3, 4096, 300) 2.54e6 requests/s at p99
+# 3.9 ms, (3, 8192, 300) 2.50e6 at 4.2-4.4 ms, (4, 8192, 300) 2.61e6 at
+# 5.9-6.2 ms, (2, 2048, 300) 1.46e6 at 4.5 ms.
+SERVER_KNOBS = ((3, 4096, 300), (3, 8192, 300), (3, 4096, 500), (2, 2048, 300))
 
 
 def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, seconds=1.0, margin=0.9, **server_kw):
