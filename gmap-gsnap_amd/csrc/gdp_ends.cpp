@@ -1,10 +1,17 @@
-// gdp_ends.cpp -- Dynprog_end5_splicejunction / Dynprog_end3_splicejunction
-// (reference dynprog_end.c:1652-1922, 2248-2745) for a batch of ends, with
-// every step on the device: the two one-sided fills per end keeping their
-// best last-row cell, the merge of the two cells, both local walks, and the
-// join of the walks around the known-site gap holder.  The host builds the
-// problems, the plan and the walk templates, and sizes the output from the
-// per-end lengths the device reports.
+// gdp_ends.cpp -- GMAP's DP callers for a batch, every step on the device.
+//
+// Dynprog_end5_splicejunction / Dynprog_end3_splicejunction (reference
+// dynprog_end.c:1652-1922, 2248-2745): the two one-sided fills per end
+// keeping their best last-row cell, the merge of the two cells, both local
+// walks, and the join of the walks around the known-site gap holder.
+//
+// Dynprog_single_gap (dynprog_single.c:428-675): the genome windows cut from
+// a device genome, single_gap_simple for windows as long as their read, the
+// fill and walk of the rest, and the choice of the list the reference
+// returns.
+//
+// The host builds the problems, the plan and the walk templates, and sizes
+// the output from the per-call lengths the device reports.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -19,7 +26,14 @@ extern "C" int gdp_launch_sj(int stage, int five, int m, const int32_t *best, co
                              const int32_t *c2, int32_t *len, int32_t *trim, const int64_t *off, const int32_t *aux,
                              gdp_sj_pair *out, gdp_sj_result *res, hipStream_t stream);
 
+extern "C" int gdp_launch_sg(int stage, int m, const gdp_sg_dev *calls, const char *qbuf, const char *gbuf,
+                             gdp_pair *ps, int32_t *cs, const gdp_tb_request *reqs, const gdp_pair *pd,
+                             const int32_t *cd, int32_t *len, int32_t *choice, const int64_t *off, gdp_pair *out,
+                             gdp_sg_result *res, hipStream_t stream);
+
 static_assert(sizeof(gdp_sj_end) == 56, "gdp_sj_end is 56 bytes");
+static_assert(sizeof(gdp_sg_call) == 48, "gdp_sg_call is 48 bytes");
+static_assert(sizeof(gdp_sg_result) == 40, "gdp_sg_result is 40 bytes");
 static_assert(sizeof(gdp_sj_pair) == 20, "gdp_sj_pair is 20 bytes");
 static_assert(sizeof(gdp_sj_result) == 40, "gdp_sj_result is 40 bytes");
 
@@ -265,6 +279,226 @@ extern "C" int gdp_sj_run(gdp_sj_batch *b, int32_t five, const gdp_sj_end *ends,
                                    stream) != hipSuccess) ||
       hipStreamSynchronize(stream) != hipSuccess)
     return err(GDP_EHIP, "gdp_sj_run: result copy failed");
+  for (int j = 0; j < m; j++) b->results[b->live[j]] = b->lres[j];
+  b->npairs = total;
+  return GDP_OK;
+}
+
+// ------------------------------------------------------------ single gaps
+
+namespace {
+
+// Dynprog_single_gap's scoring by defect rate (dynprog_single.c:458-481;
+// DEFECT_HIGHQ 0.003 and DEFECT_MEDQ 0.014, dynprog.h:58-59;
+// SINGLE_OPEN/EXTEND_{HIGHQ,MEDQ,LOWQ}, dynprog.h:61-67).
+void single_scoring(double defect_rate, int &mt, int &open, int &extend) {
+  if (defect_rate < 0.003) {
+    mt = GDP_HIGHQ;
+    open = -8;
+    extend = -3;
+  } else if (defect_rate < 0.014) {
+    mt = GDP_MEDQ;
+    open = -7;
+    extend = -2;
+  } else {
+    mt = GDP_LOWQ;
+    open = -6;
+    extend = -1;
+  }
+}
+
+}  // namespace
+
+struct gdp_sg_batch {
+  Grow q, descs, segs, g, dirs, fin, reqs, devs, ps, cs, pd, cd, len, choice, off, out, res;
+  std::vector<gdp_problem> probs;
+  std::vector<uint8_t> hdescs;
+  std::vector<gdp_segment> hsegs;
+  std::vector<gdp_tb_request> hreqs;
+  std::vector<gdp_sg_dev> hdevs;
+  std::vector<int32_t> live, hlen;
+  std::vector<int64_t> hoff;
+  std::vector<gdp_sg_result> results, lres;
+  PinnedGrow pairs;
+  int64_t npairs = 0;
+  ~gdp_sg_batch() {
+    for (Grow *x : {&q, &descs, &segs, &g, &dirs, &fin, &reqs, &devs, &ps, &cs, &pd, &cd, &len, &choice, &off, &out,
+                    &res})
+      x->release();
+    pairs.release();
+  }
+};
+
+extern "C" gdp_sg_batch *gdp_sg_create(void) { return new gdp_sg_batch(); }
+extern "C" void gdp_sg_destroy(gdp_sg_batch *b) { delete b; }
+extern "C" int64_t gdp_sg_npairs(const gdp_sg_batch *b) { return b ? b->npairs : 0; }
+extern "C" const gdp_sg_result *gdp_sg_results(const gdp_sg_batch *b) { return b ? b->results.data() : nullptr; }
+extern "C" const gdp_pair *gdp_sg_pairs(const gdp_sg_batch *b) {
+  return b ? static_cast<const gdp_pair *>(b->pairs.p) : nullptr;
+}
+
+extern "C" int gdp_sg_run(gdp_sg_batch *b, const gdp_sg_call *calls, int32_t n, const char *qbuf, int64_t qbytes,
+                          const void *d_genome, int32_t genome_format, uint32_t genome_len, int32_t max_rlength,
+                          int32_t max_glength, int32_t user_dynprog, int32_t user_open, int32_t user_extend,
+                          void *stream_) {
+  if (b == nullptr || n < 0 || (n > 0 && (calls == nullptr || qbuf == nullptr || d_genome == nullptr)) || qbytes < 0)
+    return err(GDP_EINVAL, "gdp_sg_run: null or negative argument");
+  if (genome_format != GDP_GENOME_ASCII && genome_format != GDP_GENOME_COMP)
+    return err(GDP_EINVAL, "gdp_sg_run: unknown genome format");
+  if (user_dynprog && (user_open > 0 || user_open < -127 || user_extend > 0 || user_extend < -127))
+    return err(GDP_EINVAL, "gdp_sg_run: gap penalties must lie in [-127, 0]");
+  hipStream_t stream = (hipStream_t)stream_;
+  b->npairs = 0;
+  // lengths the workspace cannot hold: NULL with NEG_INFINITY_32
+  b->results.assign((size_t)n, gdp_sg_result{0, -1, {-32768, 0, 0, 0, 0}, 0, 0});
+  b->live.clear();
+  for (int32_t k = 0; k < n; k++) {
+    const gdp_sg_call &c = calls[k];
+    if (c.rl <= 0 || c.gl <= 0 || c.rl > max_rlength || c.gl > max_glength) continue;
+    if (c.qoff < 0 || c.qoff + c.rl > qbytes)
+      return err(GDP_EINVAL, "gdp_sg_run: call " + std::to_string(k) + " reads past its buffer");
+    if (c.genestrand < 0 || c.genestrand > 2 || c.extraband < 0)
+      return err(GDP_EINVAL, "gdp_sg_run: call " + std::to_string(k) + " has a bad genestrand or extraband");
+    b->live.push_back(k);
+  }
+  const int m = (int)b->live.size();
+  if (m == 0) return GDP_OK;
+
+  b->probs.assign((size_t)m, gdp_problem{});
+  b->hsegs.assign((size_t)m, gdp_segment{});
+  b->hreqs.assign((size_t)m, gdp_tb_request{});
+  b->hdevs.assign((size_t)m, gdp_sg_dev{});
+  int64_t gpos = 0, pcap = 0, scap = 0;
+  for (int j = 0; j < m; j++) {
+    const gdp_sg_call &c = calls[b->live[j]];
+    int mt, open, extend;
+    single_scoring(c.defect_rate, mt, open, extend);
+    if (user_dynprog) {
+      open = user_open;
+      extend = user_extend;
+    }
+    int32_t lb = 0, ub = 0;
+    gdp_compute_bands(&lb, &ub, c.rl, c.gl, c.extraband, c.widebandp);
+    const int use8 = gdp_use8p_size(mt);
+    gdp_problem &p = b->probs[j];
+    p.qoff = c.qoff;
+    p.goff = p.galtoff = gpos;
+    p.rl = c.rl;
+    p.gl = c.gl;
+    p.lband = (int16_t)std::min<int32_t>(lb, 32767);
+    p.uband = (int16_t)std::min<int32_t>(ub, 32767);
+    p.kind = GDP_KIND_FULL;
+    p.width = (c.rl < use8 && c.gl < use8) ? 8 : 16;
+    p.mismatchtype = (int8_t)mt;
+    p.jump_late = c.jump_late ? 1 : 0;
+    p.open = (int8_t)open;
+    p.extend = (int8_t)extend;
+    gdp_segment &sg = b->hsegs[j];
+    if (c.watsonp) {
+      sg.coord = c.chroffset + (uint32_t)c.goffset;
+      sg.bound = c.chrhigh;
+    } else {
+      sg.coord = c.chrhigh - (uint32_t)c.goffset + 1u;
+      sg.bound = c.chroffset;
+      sg.left = 1;
+      sg.revcomp = 1;
+    }
+    sg.length = c.gl;
+    sg.out_off = gpos;
+    gdp_tb_request &t = b->hreqs[j];
+    t.fill = j;
+    t.r = c.rl;
+    t.c = c.gl;
+    t.queryoffset = c.roffset;
+    t.genomeoffset = c.goffset;
+    t.chroffset = c.chroffset;
+    t.chrhigh = c.chrhigh;
+    t.watsonp = c.watsonp ? 1 : 0;
+    t.genestrand = c.genestrand;
+    t.pair_off = pcap;
+    t.max_pairs = c.rl + c.gl + 2;
+    gdp_sg_dev &d = b->hdevs[j];
+    d.qoff = c.qoff;
+    d.goff = gpos;
+    d.simple_off = c.gl == c.rl ? scap : -1;
+    d.rl = c.rl;
+    d.gl = c.gl;
+    d.roffset = c.roffset;
+    d.goffset = c.goffset;
+    d.genestrand = c.genestrand;
+    if (c.gl == c.rl) scap += c.rl;
+    gpos += c.gl;
+    pcap += t.max_pairs;
+  }
+  b->hdescs.resize((size_t)gdp_plan_bytes(m));
+  int64_t dw = 0, ss = 0;
+  int rc = gdp_plan_ex(b->probs.data(), m, 0, b->hdescs.data(), &dw, &ss);
+  if (rc != GDP_OK) return rc;
+  const uint32_t kernels = gdp_plan_kernels(b->hdescs.data(), m);
+  const int64_t used = gdp_plan_used_bytes(b->hdescs.data(), m);
+  if (!b->q.ensure((size_t)qbytes + 16) || !b->descs.ensure((size_t)used) ||
+      !b->segs.ensure(sizeof(gdp_segment) * (size_t)m) || !b->g.ensure((size_t)gpos + 16) ||
+      !b->dirs.ensure(sizeof(uint32_t) * (size_t)(dw > 0 ? dw : 1)) || !b->fin.ensure(sizeof(int32_t) * (size_t)m) ||
+      !b->reqs.ensure(sizeof(gdp_tb_request) * (size_t)m) || !b->devs.ensure(sizeof(gdp_sg_dev) * (size_t)m) ||
+      !b->ps.ensure(sizeof(gdp_pair) * (size_t)(scap > 0 ? scap : 1)) ||
+      !b->cs.ensure(sizeof(int32_t) * 6 * (size_t)m) || !b->pd.ensure(sizeof(gdp_pair) * (size_t)pcap) ||
+      !b->cd.ensure(sizeof(int32_t) * 6 * (size_t)m) || !b->len.ensure(sizeof(int32_t) * (size_t)m) ||
+      !b->choice.ensure(sizeof(int32_t) * (size_t)m) || !b->off.ensure(sizeof(int64_t) * (size_t)m) ||
+      !b->res.ensure(sizeof(gdp_sg_result) * (size_t)m))
+    return err(GDP_ENOMEM, "gdp_sg_run: device allocation failed");
+  auto up = [&](Grow &d, const void *h, size_t bytes) {
+    return hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, stream) == hipSuccess;
+  };
+  if (!up(b->q, qbuf, (size_t)qbytes) || !up(b->descs, b->hdescs.data(), (size_t)used) ||
+      !up(b->segs, b->hsegs.data(), sizeof(gdp_segment) * (size_t)m) ||
+      !up(b->reqs, b->hreqs.data(), sizeof(gdp_tb_request) * (size_t)m) ||
+      !up(b->devs, b->hdevs.data(), sizeof(gdp_sg_dev) * (size_t)m))
+    return err(GDP_EHIP, "gdp_sg_run: upload failed");
+  const char *dq = b->q.as<char>(), *dg = b->g.as<char>();
+  if ((rc = gdp_genome_segments(d_genome, genome_format, genome_len, b->segs.as<gdp_segment>(), m, b->g.as<char>(),
+                                stream)) != GDP_OK)
+    return rc;
+  if (gdp_launch_sg(0, m, b->devs.as<gdp_sg_dev>(), dq, dg, b->ps.as<gdp_pair>(), b->cs.as<int32_t>(), nullptr,
+                    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, stream))
+    return err(GDP_EHIP, "gdp_sg_run: single_gap_simple failed");
+  if ((rc = gdp_fill_ex(b->descs.p, m, kernels, dq, dg, b->dirs.as<uint32_t>(), nullptr, b->fin.as<int32_t>(),
+                        stream)) != GDP_OK)
+    return rc;
+  if ((rc = gdp_traceback_ex(b->descs.p, b->reqs.as<gdp_tb_request>(), m, dq, dg, b->dirs.as<uint32_t>(), d_genome,
+                             genome_format, genome_len, b->pd.as<gdp_pair>(), b->cd.as<int32_t>(), stream)) != GDP_OK)
+    return rc;
+  if (gdp_launch_sg(1, m, nullptr, nullptr, nullptr, nullptr, b->cs.as<int32_t>(), nullptr, nullptr,
+                    b->cd.as<int32_t>(), b->len.as<int32_t>(), b->choice.as<int32_t>(), nullptr, nullptr, nullptr,
+                    stream))
+    return err(GDP_EHIP, "gdp_sg_run: lengths failed");
+  b->hlen.resize((size_t)m);
+  if (hipMemcpyAsync(b->hlen.data(), b->len.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    return err(GDP_EHIP, "gdp_sg_run: lengths copy failed");
+  b->hoff.resize((size_t)m);
+  int64_t total = 0;
+  for (int j = 0; j < m; j++) {
+    b->hoff[j] = total;
+    total += b->hlen[j] > 0 ? b->hlen[j] : 0;
+  }
+  if (!b->out.ensure(sizeof(gdp_pair) * (size_t)(total > 0 ? total : 1)) ||
+      !up(b->off, b->hoff.data(), sizeof(int64_t) * (size_t)m))
+    return err(GDP_ENOMEM, "gdp_sg_run: output allocation failed");
+  if (gdp_launch_sg(2, m, b->devs.as<gdp_sg_dev>(), nullptr, nullptr, b->ps.as<gdp_pair>(), b->cs.as<int32_t>(),
+                    b->reqs.as<gdp_tb_request>(), b->pd.as<gdp_pair>(), b->cd.as<int32_t>(), b->len.as<int32_t>(),
+                    b->choice.as<int32_t>(), b->off.as<int64_t>(), b->out.as<gdp_pair>(), b->res.as<gdp_sg_result>(),
+                    stream))
+    return err(GDP_EHIP, "gdp_sg_run: join failed");
+  b->lres.resize((size_t)m);
+  if (!b->pairs.ensure(sizeof(gdp_pair) * (size_t)(total > 0 ? total : 1)))
+    return err(GDP_ENOMEM, "gdp_sg_run: pinned host allocation failed");
+  if (hipMemcpyAsync(b->lres.data(), b->res.p, sizeof(gdp_sg_result) * m, hipMemcpyDeviceToHost, stream) !=
+          hipSuccess ||
+      (total > 0 &&
+       hipMemcpyAsync(b->pairs.p, b->out.p, sizeof(gdp_pair) * (size_t)total, hipMemcpyDeviceToHost, stream) !=
+           hipSuccess) ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    return err(GDP_EHIP, "gdp_sg_run: result copy failed");
   for (int j = 0; j < m; j++) b->results[b->live[j]] = b->lres[j];
   b->npairs = total;
   return GDP_OK;

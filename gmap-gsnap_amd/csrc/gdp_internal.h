@@ -133,6 +133,19 @@ struct gdp_pair_hdr {
 };
 static_assert(sizeof(gdp_pair_hdr) == 256, "gdp_pair_hdr must stay 256 bytes");
 
+// One call of the batched single-gap caller on the device: its read in the
+// batch's query buffer, its genome window in the batch's genome buffer, and
+// where single_gap_simple writes its pairs (-1: glength != rlength, not
+// tried).
+struct gdp_sg_dev {
+  int64_t qoff;
+  int64_t goff;
+  int64_t simple_off;
+  int32_t rl, gl, roffset, goffset;
+  int32_t genestrand, pad;
+};
+static_assert(sizeof(gdp_sg_dev) == 48, "gdp_sg_dev is 48 bytes");
+
 // Row stride (direction nibbles) of a narrow sweep over band width w
 // (uband of an upper fill, lband of a lower one), 0 when w + 1 > 32.
 GDP_HD int gdp_narrow_stride(int w) { return w < 8 ? 8 : (w < 16 ? 16 : (w < 32 ? 32 : 0)); }

@@ -2093,6 +2093,143 @@ extern "C" int gdp_launch_traceback(const void *d_descs, const gdp_tb_request *d
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// ---------------------------------------------- batched Dynprog_single_gap
+
+// single_gap_simple (dynprog_single.c:344-419) for a window as long as its
+// read, one wave per call: every column pairs read base r with genome base r;
+// '*' columns (past the chromosome) push nothing; a base equal to either
+// allele is a match ('*'), one consistent with either (consistent_array) an
+// ambiguous pair (':'), else a mismatch (' ').  The pairs are written in the
+// order of the returned list from its head (the last pushed, r = rlength,
+// first); counts = {npairs, traceback_score, nmatches, nmismatches, 0, 0},
+// npairs -1 when the call was not tried.  More than one mismatch makes the
+// reference return NULL and go on to the DP; the caller sees it in counts.
+__global__ __launch_bounds__(256) void gdp_sg_simple_kernel(const gdp_sg_dev *__restrict__ calls, int m,
+                                                            const char *__restrict__ qbuf,
+                                                            const char *__restrict__ gbuf, gdp_pair *__restrict__ pairs,
+                                                            int32_t *__restrict__ counts) {
+  __shared__ uint32_t consb[3 * 128 * 4];
+  for (int j = threadIdx.x; j < 3 * 128 * 4; j += blockDim.x) consb[j] = gdp_tabConsBits[j];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (k >= m) return;
+  const gdp_sg_dev c = calls[k];
+  int32_t *cnt = counts + 6 * (int64_t)k;
+  if (c.simple_off < 0) {
+    if (lane == 0) {
+      cnt[0] = -1;
+      for (int x = 1; x < 6; x++) cnt[x] = 0;
+    }
+    return;
+  }
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int row = (c.genestrand * 128) << 2;
+  int n = 0, nm = 0, nmm = 0;
+  gdp_pair *out = pairs + c.simple_off;
+  for (int base = c.rl - 1; base >= 0; base -= 64) {
+    const int i = base - lane;
+    const bool valid = i >= 0;
+    int c1 = 'N', c2 = '*', c2a = '*';
+    if (valid) {
+      c1 = qbuf[c.qoff + i];
+      c2 = gbuf[c.goff + i];
+      c2a = c2;  // one genome: the alternate allele is the reference's
+    }
+    const bool paired = valid && c2 != '*';
+    int comp = ' ';
+    if (paired) {
+      const int c1uc = (c1 >= 'a' && c1 <= 'z') ? c1 - 32 : c1;
+      const int r4 = row + ((c1uc & 127) << 2);
+      if (c1uc == c2 || c1uc == c2a) comp = '*';
+      else if (((consb[r4 + ((c2 & 127) >> 5)] >> (c2 & 31)) & 1) || ((consb[r4 + ((c2a & 127) >> 5)] >> (c2a & 31)) & 1))
+        comp = ':';
+    }
+    const uint64_t ma = __ballot(paired && comp != ' '), mm_ = __ballot(paired && comp == ' ');
+    nm += __popcll(ma);
+    nmm += __popcll(mm_);
+    const int qpos = c.roffset + i, gpos = c.goffset + i;
+    // Pairpool_push drops pairs at negative coordinates (pairpool.c:189-191)
+    const bool emit = paired && qpos >= 0 && gpos >= 0;
+    const uint64_t em = __ballot(emit);
+    if (emit) {
+      gdp_pair &pr = out[n + __popcll(em & below)];
+      pr.querypos = qpos;
+      pr.genomepos = gpos;
+      pr.cdna = (int8_t)c1;
+      pr.comp = (int8_t)comp;
+      pr.genome = (int8_t)c2;
+      pr.genomealt = (int8_t)c2a;
+      pr.gapp = 0;
+      pr.genomejump = 0;
+    }
+    n += __popcll(em);
+  }
+  if (lane == 0) {
+    cnt[0] = n;
+    cnt[1] = TB_MATCH * nm + TB_MISMATCH * nmm;
+    cnt[2] = nm;
+    cnt[3] = nmm;
+    cnt[4] = cnt[5] = 0;
+  }
+}
+
+// Per call, which list the reference returns: single_gap_simple's when it
+// was tried, found at most one mismatch and pushed a pair, else the DP
+// walk's (a walk that pushed nothing is NULL).  len = the chosen list's
+// length (-1 NULL), choice = 1 simple / 0 DP.
+__global__ void gdp_sg_len_kernel(int m, const int32_t *__restrict__ cs, const int32_t *__restrict__ cd,
+                                  int32_t *__restrict__ len, int32_t *__restrict__ choice) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const bool simple = cs[6 * k] > 0 && cs[6 * k + 3] <= 1;
+  choice[k] = simple ? 1 : 0;
+  len[k] = simple ? cs[6 * k] : (cd[6 * k] > 0 ? cd[6 * k] : -1);
+}
+
+// Copies each call's chosen list to its place in the compact output and
+// writes its result, one workgroup per call.
+__global__ void gdp_sg_join_kernel(int m, const gdp_sg_dev *__restrict__ calls, const gdp_tb_request *__restrict__ reqs,
+                                   const gdp_pair *__restrict__ ps, const gdp_pair *__restrict__ pd,
+                                   const int32_t *__restrict__ cs, const int32_t *__restrict__ cd,
+                                   const int32_t *__restrict__ len, const int32_t *__restrict__ choice,
+                                   const int64_t *__restrict__ off, gdp_pair *__restrict__ out,
+                                   gdp_sg_result *__restrict__ res) {
+  for (int k = blockIdx.x; k < m; k += gridDim.x) {
+    const bool simple = choice[k] != 0;
+    const int n = len[k];
+    const gdp_pair *src = simple ? ps + calls[k].simple_off : pd + reqs[k].pair_off;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) out[off[k] + j] = src[j];
+    if (threadIdx.x == 0) {
+      const int32_t *c = simple ? cs + 6 * (int64_t)k : cd + 6 * (int64_t)k;
+      gdp_sg_result r;
+      r.pair_off = off[k];
+      r.npairs = n;
+      for (int x = 0; x < 5; x++) r.counters[x] = c[1 + x];
+      r.simple = simple ? 1 : 0;
+      r.pad = 0;
+      res[k] = r;
+    }
+  }
+}
+
+extern "C" int gdp_launch_sg(int stage, int m, const gdp_sg_dev *calls, const char *qbuf, const char *gbuf,
+                             gdp_pair *ps, int32_t *cs, const gdp_tb_request *reqs, const gdp_pair *pd,
+                             const int32_t *cd, int32_t *len, int32_t *choice, const int64_t *off, gdp_pair *out,
+                             gdp_sg_result *res, hipStream_t stream) {
+  if (m <= 0) return 0;
+  if (stage == 0)
+    hipLaunchKernelGGL(gdp_sg_simple_kernel, dim3((m + 3) / 4), dim3(256), 0, stream, calls, m, qbuf, gbuf, ps, cs);
+  else if (stage == 1)
+    hipLaunchKernelGGL(gdp_sg_len_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, m, (const int32_t *)cs, cd,
+                       len, choice);
+  else
+    hipLaunchKernelGGL(gdp_sg_join_kernel, dim3(gdp_imin(m, 8192)), dim3(64), 0, stream, m, calls, reqs,
+                       (const gdp_pair *)ps, pd, (const int32_t *)cs, cd, (const int32_t *)len,
+                       (const int32_t *)choice, off, out, res);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // Copies between device memory and pinned host memory mapped into the
 // device's address space, as kernels on the caller's stream.  A copy engine
 // serves the streams of a process in submission order, so a batch's upload

@@ -1,0 +1,74 @@
+"""GMAP's single-gap caller for many gaps at once: Dynprog_single_gap
+(dynprog_single.c:428-675) run by the library (gdp_sg_run).
+
+Per call the reference picks the mismatch type and gap penalties by defect
+rate, cuts the genome window (reverse complement on the minus strand), takes
+single_gap_simple's ungapped list when the window is as long as the read and
+that list has at most one mismatch, and otherwise fills the band of
+Dynprog_compute_bands with the int8 or int16 kernel and walks back from the
+corner.  Here the windows are cut from a device genome, both lists are made
+on the device and a join kernel keeps the one the reference returns.
+"""
+import ctypes
+
+import numpy as np
+
+from .batch import _check, init
+from .lib import PAIR_DTYPE, SG_CALL_DTYPE, SG_RESULT_DTYPE
+
+_batch = None
+
+
+class SingleGapBatch:
+    """A gdp_sg_batch: device and host buffers reused across runs."""
+
+    def __init__(self):
+        self.L = init()
+        self.h = self.L.gdp_sg_create()
+
+    def run(self, calls, qbuf, genome, max_rlength=500, max_glength=2000, user_penalties=None, stream=None):
+        """calls: SG_CALL_DTYPE rows; qbuf: the reads; genome: a
+        serve.DeviceGenome.  user_penalties: (open, extend) for GMAP's
+        --indel-open / --indel-extend, None for the defect-rate penalties.
+        Returns (results SG_RESULT_DTYPE, pairs PAIR_DTYPE)."""
+        L = self.L
+        calls = np.ascontiguousarray(calls, SG_CALL_DTYPE)
+        q = np.frombuffer(bytes(qbuf), np.uint8)
+        uo, ue = user_penalties if user_penalties is not None else (0, 0)
+        _check(L, L.gdp_sg_run(self.h, calls.ctypes.data, len(calls), q.ctypes.data, len(q), genome.ptr, genome.fmt,
+                               genome.nbases, max_rlength, max_glength, 1 if user_penalties is not None else 0,
+                               uo, ue, stream), "gdp_sg_run")
+        n = len(calls)
+        res = np.ctypeslib.as_array(ctypes.cast(L.gdp_sg_results(self.h), ctypes.POINTER(ctypes.c_uint8)),
+                                    (n * SG_RESULT_DTYPE.itemsize,)).view(SG_RESULT_DTYPE).copy() if n else \
+            np.zeros(0, SG_RESULT_DTYPE)
+        m = int(L.gdp_sg_npairs(self.h))
+        pairs = np.ctypeslib.as_array(ctypes.cast(L.gdp_sg_pairs(self.h), ctypes.POINTER(ctypes.c_uint8)),
+                                      (m * PAIR_DTYPE.itemsize,)).view(PAIR_DTYPE).copy() if m else \
+            np.zeros(0, PAIR_DTYPE)
+        return res, pairs
+
+    def close(self):
+        if self.h:
+            self.L.gdp_sg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def single_gap(calls, qbuf, genome, max_rlength=500, max_glength=2000, user_penalties=None):
+    """Dynprog_single_gap for every call.  Returns per call None (the
+    reference's NULL list, with traceback_score NEG_INFINITY_32 when the
+    lengths do not fit the workspace) or (the returned list's pairs from its
+    head as PAIR_DTYPE records, {traceback_score, nmatches, nmismatches,
+    nopens, nindels}, simple)."""
+    global _batch
+    if _batch is None:
+        _batch = SingleGapBatch()
+    res, pairs = _batch.run(calls, qbuf, genome, max_rlength, max_glength, user_penalties)
+    out = [None] * len(res)
+    for k in np.nonzero(res["npairs"] >= 0)[0]:
+        o, m = int(res["pair_off"][k]), int(res["npairs"][k])
+        out[k] = (pairs[o:o + m], res["counters"][k].astype(np.int32), bool(res["simple"][k]))
+    return out, res

@@ -58,6 +58,12 @@ SJ_END_DTYPE = np.dtype([("qoff", "<i8"), ("goff", "<i8"), ("rl", "<i4"), ("gl",
                          ("anchor", "<i4"), ("far", "<i4"), ("contlength", "<i4"), ("extraband", "<i2"),
                          ("genestrand", "i1"), ("jump_late", "i1"), ("pad", "<i4"), ("defect_rate", "<f8")])
 SJ_RESULT_DTYPE = np.dtype([("pair_off", "<i8"), ("npairs", "<i4"), ("counters", "<i4", (6,)), ("pad", "<i4")])
+SG_CALL_DTYPE = np.dtype([("qoff", "<i8"), ("rl", "<i4"), ("gl", "<i4"), ("roffset", "<i4"), ("goffset", "<i4"),
+                          ("chroffset", "<u4"), ("chrhigh", "<u4"), ("extraband", "<i4"), ("watsonp", "i1"),
+                          ("genestrand", "i1"), ("jump_late", "i1"), ("widebandp", "i1"), ("defect_rate", "<f8")])
+SG_RESULT_DTYPE = np.dtype([("pair_off", "<i8"), ("npairs", "<i4"), ("counters", "<i4", (5,)), ("simple", "<i4"),
+                            ("pad", "<i4")])
+assert SG_CALL_DTYPE.itemsize == 48 and SG_RESULT_DTYPE.itemsize == 40
 SEGMENT_DTYPE = np.dtype([("coord", "<u4"), ("bound", "<u4"), ("length", "<i4"), ("left", "i1"), ("revcomp", "i1"),
                           ("pad", "<i2"), ("out_off", "<i8")])
 ALIGN_REQ_DTYPE = np.dtype([("tag", "<u8"), ("read", "<u8"), ("rl", "<i4"), ("gl", "<i4"), ("roffset", "<i4"),
@@ -137,6 +143,12 @@ def load(path=None):
         "gdp_sj_npairs": (i64, [vp]),
         "gdp_sj_results": (vp, [vp]),
         "gdp_sj_pairs": (vp, [vp]),
+        "gdp_sg_create": (vp, []),
+        "gdp_sg_destroy": (None, [vp]),
+        "gdp_sg_run": (ctypes.c_int, [vp, vp, i32, vp, i64, vp, i32, ctypes.c_uint32, i32, i32, i32, i32, i32, vp]),
+        "gdp_sg_npairs": (i64, [vp]),
+        "gdp_sg_results": (vp, [vp]),
+        "gdp_sg_pairs": (vp, [vp]),
         "gdp_traceback": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp]),
         "gdp_traceback_ex": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp]),
         "gdp_traceback_ends": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp, vp]),

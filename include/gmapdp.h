@@ -584,6 +584,78 @@ int64_t gdp_sj_npairs(const gdp_sj_batch *b);
 const gdp_sj_result *gdp_sj_results(const gdp_sj_batch *b);
 const gdp_sj_pair *gdp_sj_pairs(const gdp_sj_batch *b);
 
+/* ---- Single gaps: the whole Dynprog_single_gap caller on the device --- */
+
+/* One Dynprog_single_gap call (dynprog_single.c:428-675), 48 bytes, pad
+   zero.  The read (rsequence) is rl bytes at qoff of the batch's query
+   buffer; rsequenceuc is its upper case, as GMAP builds it.  The genome
+   window comes from a device genome as the caller takes it
+   (dynprog_single.c:544-550): Genome_get_segment_right(chroffset + goffset,
+   gl, chrhigh) on the plus strand, the reverse complement
+   Genome_get_segment_left(chrhigh - goffset + 1, gl, chroffset) on the minus
+   strand; the genome is its own alternate allele. */
+typedef struct gdp_sg_call {
+  int64_t qoff;
+  int32_t rl, gl;          /* rlength, glength */
+  int32_t roffset, goffset;
+  uint32_t chroffset, chrhigh;
+  int32_t extraband;       /* extraband_single */
+  int8_t watsonp;
+  int8_t genestrand;
+  int8_t jump_late;        /* jump_late_p */
+  int8_t widebandp;
+  double defect_rate;
+} gdp_sg_call;
+
+/* Per call: npairs -1 for the reference's NULL list, else the number of
+   records at pair_off of the batch's pair array, in the order of the
+   returned list from its head; counters = {traceback_score, nmatches,
+   nmismatches, nopens, nindels}; simple = 1 when single_gap_simple gave the
+   list.  Every call advances the caller's dynprogindex by one step, as the
+   reference's returns here all do. */
+typedef struct gdp_sg_result {
+  int64_t pair_off;
+  int32_t npairs;
+  int32_t counters[5];
+  int32_t simple;
+  int32_t pad;
+} gdp_sg_result;
+
+/* A batch object owning the device and host buffers of gdp_sg_run; they
+   grow to the largest batch run through it and are reused. */
+typedef struct gdp_sg_batch gdp_sg_batch;
+gdp_sg_batch *gdp_sg_create(void);
+void gdp_sg_destroy(gdp_sg_batch *b);
+
+/* Runs n calls of Dynprog_single_gap with reads from the host buffer qbuf
+   (qbytes bytes) against a device genome (GDP_GENOME_* format, genome_len
+   bases), the reference's steps for each:
+   - lengths of 0 or past max_rlength / max_glength: NULL, traceback_score
+     NEG_INFINITY_32 (dynprog_single.c:508-520);
+   - mismatch type and gap penalties by defect rate (DEFECT_HIGHQ 0.003,
+     DEFECT_MEDQ 0.014; SINGLE_OPEN/EXTEND of dynprog.h:61-67), or
+     user_open / user_extend when user_dynprog is set (GMAP's
+     --indel-open / --indel-extend);
+   - the window from the device genome;
+   - glength == rlength: single_gap_simple, kept when it pushes a pair with
+     at most one mismatch;
+   - else the bands of Dynprog_compute_bands, the int8 fill when rlength and
+     glength are both below use8p_size[mismatchtype], the int16 fill
+     otherwise, and the traceback from (rlength, glength), returned in push
+     order (the reference reverses the pushed list).
+   GMAP's homopolymer mode (homopolymerp) is not part of it.  Segments,
+   fills, walks and the choice between the two lists run on the device; the
+   results and pairs are copied back to the batch's host arrays before it
+   returns. */
+int gdp_sg_run(gdp_sg_batch *b, const gdp_sg_call *calls, int32_t n, const char *qbuf, int64_t qbytes,
+               const void *d_genome, int32_t genome_format, uint32_t genome_len, int32_t max_rlength,
+               int32_t max_glength, int32_t user_dynprog, int32_t user_open, int32_t user_extend, void *stream);
+/* The last run's pair count, per-call results and pair records (owned by
+   the batch, valid until its next run). */
+int64_t gdp_sg_npairs(const gdp_sg_batch *b);
+const gdp_sg_result *gdp_sg_results(const gdp_sg_batch *b);
+const gdp_pair *gdp_sg_pairs(const gdp_sg_batch *b);
+
 /* ---- Serving: continuous batching of alignment requests -------------- */
 
 /* A request loop over a device-resident genome.  Each request is the DP of
