@@ -503,3 +503,249 @@ extern "C" int gdp_sg_run(gdp_sg_batch *b, const gdp_sg_call *calls, int32_t n, 
   b->npairs = total;
   return GDP_OK;
 }
+
+// --------------------------------------------------------------- end gaps
+
+extern "C" int gdp_launch_eg(int stage, int m, int five, const int32_t *flags, const gdp_tb_request *reqs,
+                             const gdp_pair *pd, int32_t *cd, const int32_t *best, int32_t *len, int32_t *trim,
+                             const int64_t *off, gdp_pair *out, gdp_eg_result *res, hipStream_t stream);
+
+static_assert(sizeof(gdp_eg_call) == 48, "gdp_eg_call is 48 bytes");
+static_assert(sizeof(gdp_eg_result) == 40, "gdp_eg_result is 40 bytes");
+
+struct gdp_eg_batch {
+  Grow q, descs, segs, g, dirs, scores, fin, ul, best, tmpl, reqs, flags, pd, cd, len, trim, off, out, res;
+  std::vector<gdp_problem> probs;
+  std::vector<uint8_t> hdescs;
+  std::vector<char> hq;
+  std::vector<gdp_segment> hsegs;
+  std::vector<gdp_tb_request> hreqs;
+  std::vector<int32_t> live, hul, hflags, hlen;
+  std::vector<int64_t> hoff;
+  std::vector<gdp_eg_result> results, lres;
+  PinnedGrow pairs;
+  int64_t npairs = 0;
+  ~gdp_eg_batch() {
+    for (Grow *x : {&q, &descs, &segs, &g, &dirs, &scores, &fin, &ul, &best, &tmpl, &reqs, &flags, &pd, &cd, &len,
+                    &trim, &off, &out, &res})
+      x->release();
+    pairs.release();
+  }
+};
+
+extern "C" gdp_eg_batch *gdp_eg_create(void) { return new gdp_eg_batch(); }
+extern "C" void gdp_eg_destroy(gdp_eg_batch *b) { delete b; }
+extern "C" int64_t gdp_eg_npairs(const gdp_eg_batch *b) { return b ? b->npairs : 0; }
+extern "C" const gdp_eg_result *gdp_eg_results(const gdp_eg_batch *b) { return b ? b->results.data() : nullptr; }
+extern "C" const gdp_pair *gdp_eg_pairs(const gdp_eg_batch *b) {
+  return b ? static_cast<const gdp_pair *>(b->pairs.p) : nullptr;
+}
+
+extern "C" int gdp_eg_run(gdp_eg_batch *b, int32_t five, const gdp_eg_call *calls, int32_t n, const char *qbuf,
+                          int64_t qbytes, const void *d_genome, int32_t genome_format, uint32_t genome_len,
+                          int32_t max_rlength, int32_t max_glength, int32_t user_dynprog, int32_t user_open,
+                          int32_t user_extend, int32_t require_pos_score, void *stream_) {
+  if (b == nullptr || n < 0 || (n > 0 && (calls == nullptr || qbuf == nullptr || d_genome == nullptr)) || qbytes < 0)
+    return err(GDP_EINVAL, "gdp_eg_run: null or negative argument");
+  if (genome_format != GDP_GENOME_ASCII && genome_format != GDP_GENOME_COMP)
+    return err(GDP_EINVAL, "gdp_eg_run: unknown genome format");
+  if (user_dynprog && (user_open > 0 || user_open < -127 || user_extend > 0 || user_extend < -127))
+    return err(GDP_EINVAL, "gdp_eg_run: gap penalties must lie in [-127, 0]");
+  hipStream_t stream = (hipStream_t)stream_;
+  b->npairs = 0;
+  b->results.assign((size_t)n, gdp_eg_result{0, -1, {0, 0, 0, 0, 0}, 0, 0});
+  // live calls grouped by end mode: QUERYEND_GAP / BEST_LOCAL, then
+  // QUERYEND_INDELS, then QUERYEND_NOGAPS; each call's chopped lengths
+  std::vector<int32_t> group[3], rlx((size_t)n), glx((size_t)n);
+  for (int32_t k = 0; k < n; k++) {
+    const gdp_eg_call &c = calls[k];
+    if (c.endalign < 0 || c.endalign > 3)
+      return err(GDP_EINVAL, "gdp_eg_run: call " + std::to_string(k) + " has a bad endalign");
+    if (c.genestrand < 0 || c.genestrand > 2 || c.extraband < 0)
+      return err(GDP_EINVAL, "gdp_eg_run: call " + std::to_string(k) + " has a bad genestrand or extraband");
+    const bool nogaps = c.endalign == 2;
+    if (c.rl <= 0 || (five && c.goffset < 0) || c.gl <= 0) continue;  // NULL, counters 0, dynprogindex kept
+    if (c.qoff < 0 || c.qoff + c.rl > qbytes)
+      return err(GDP_EINVAL, "gdp_eg_run: call " + std::to_string(k) + " reads past its buffer");
+    rlx[k] = nogaps ? c.rl : std::min(c.rl, max_rlength);
+    glx[k] = nogaps ? c.gl : std::min(c.gl, max_glength);
+    if (rlx[k] > GDP_MAX_LEN || glx[k] > GDP_MAX_LEN)
+      return err(GDP_ETOOBIG, "gdp_eg_run: call " + std::to_string(k) + " is longer than GDP_MAX_LEN");
+    group[c.endalign == 1 ? 1 : (nogaps ? 2 : 0)].push_back(k);
+  }
+  b->live.clear();
+  for (auto &gr : group) b->live.insert(b->live.end(), gr.begin(), gr.end());
+  const int m = (int)b->live.size(), m0 = (int)group[0].size(), m1 = (int)group[1].size();
+  if (m == 0) return GDP_OK;
+
+  const int use8 = gdp_use8p_size(kEndq);
+  b->probs.clear();
+  b->hq.clear();
+  b->hsegs.assign((size_t)m, gdp_segment{});
+  b->hreqs.assign((size_t)m, gdp_tb_request{});
+  b->hflags.assign((size_t)m, 0);
+  b->hul.clear();
+  int64_t gpos = 0, pcap = 0;
+  for (int j = 0; j < m; j++) {
+    const int32_t k = b->live[j];
+    const gdp_eg_call &c = calls[k];
+    const int rl = rlx[k], gl = glx[k];
+    const bool nogaps = c.endalign == 2;
+    const int64_t qpos = (int64_t)b->hq.size();
+    // the read in sweep order: the 5' caller reads rev_rsequence[1 - r]
+    for (int i = 0; i < rl; i++) b->hq.push_back(five ? qbuf[c.qoff + c.rl - 1 - i] : qbuf[c.qoff + i]);
+    gdp_segment &sg = b->hsegs[j];
+    if (five) {
+      if (c.watsonp) {
+        sg.coord = c.chroffset + (uint32_t)c.goffset + 1u;
+        sg.bound = c.chroffset;
+        sg.left = 1;
+      } else {
+        sg.coord = c.chrhigh - (uint32_t)c.goffset;
+        sg.bound = c.chrhigh;
+        sg.revcomp = 1;
+      }
+      sg.reverse = 1;  // the 5' fills read the window from its last base
+    } else if (c.watsonp) {
+      sg.coord = c.chroffset + (uint32_t)c.goffset;
+      sg.bound = c.chrhigh;
+    } else {
+      sg.coord = c.chrhigh - (uint32_t)c.goffset + 1u;
+      sg.bound = c.chroffset;
+      sg.left = 1;
+      sg.revcomp = 1;
+    }
+    sg.length = gl;
+    sg.out_off = gpos;
+    const int open = user_dynprog ? user_open : end_open(c.defect_rate);
+    const int extend = user_dynprog ? user_extend : kEndExtend;
+    int32_t lb = 0, ub = 0;
+    gdp_compute_bands(&lb, &ub, rl, gl, c.extraband, c.endalign == 0 || c.endalign == 3);
+    const int first = (int)b->probs.size();
+    for (int t = 0; t < (nogaps ? 1 : 2); t++) {
+      gdp_problem p;
+      memset(&p, 0, sizeof p);
+      p.qoff = qpos;
+      p.goff = p.galtoff = gpos;
+      p.rl = rl;
+      p.gl = gl;
+      p.kind = t == 0 ? GDP_KIND_UPPER : GDP_KIND_LOWER;
+      // the nogaps walk reads no directions: its sequences ride on a
+      // one-column upper fill
+      p.uband = (int16_t)std::min<int32_t>(nogaps ? 0 : ub, 32767);
+      p.lband = (int16_t)std::min<int32_t>(lb, 32767);
+      p.width = (rl < use8 || gl < use8) ? 8 : 16;
+      p.mismatchtype = kEndq;
+      p.jump_late = five ? !c.jump_late : (c.jump_late != 0);
+      p.open = (int8_t)open;
+      p.extend = (int8_t)extend;
+      p.keep_scores = nogaps ? 0 : (c.endalign == 1 ? GDP_KEEP_BEST_QUERYEND : GDP_KEEP_BEST);
+      b->probs.push_back(p);
+    }
+    if (!nogaps) {
+      b->hul.push_back(first);
+      b->hul.push_back(first + 1);
+    }
+    gdp_tb_request &q = b->hreqs[j];
+    q.fill = first;
+    q.queryoffset = c.roffset;
+    q.genomeoffset = c.goffset;
+    q.chroffset = c.chroffset;
+    q.chrhigh = c.chrhigh;
+    q.watsonp = c.watsonp ? 1 : 0;
+    q.revp = five ? 1 : 0;
+    q.genestrand = c.genestrand;
+    q.pair_off = pcap;
+    q.max_pairs = rl + gl + 2;
+    if (nogaps) {  // find_best_endpoint_to_queryend_nogaps (dynprog_end.c:576-586)
+      q.r = q.c = std::min(rl, gl);
+      q.flags = GDP_TB_NOGAPS;
+    }
+    b->hflags[j] = c.endalign | (require_pos_score ? 4 : 0);
+    gpos += gl;
+    pcap += q.max_pairs;
+  }
+  const int np = (int)b->probs.size();
+  b->hdescs.resize((size_t)gdp_plan_bytes(np));
+  int64_t dw = 0, ss = 0;
+  int rc = gdp_plan_ex(b->probs.data(), np, 0, b->hdescs.data(), &dw, &ss);
+  if (rc != GDP_OK) return rc;
+  const uint32_t kernels = gdp_plan_kernels(b->hdescs.data(), np);
+  const int64_t used = gdp_plan_used_bytes(b->hdescs.data(), np);
+  const size_t tb = sizeof(gdp_tb_request) * (size_t)m;
+  if (!b->q.ensure(b->hq.size() + 16) || !b->descs.ensure((size_t)used) ||
+      !b->segs.ensure(sizeof(gdp_segment) * (size_t)m) || !b->g.ensure((size_t)gpos + 16) ||
+      !b->dirs.ensure(sizeof(uint32_t) * (size_t)(dw > 0 ? dw : 1)) ||
+      !b->scores.ensure(sizeof(int16_t) * (size_t)(ss > 0 ? ss : 1)) || !b->fin.ensure(sizeof(int32_t) * np) ||
+      !b->ul.ensure(sizeof(int32_t) * (b->hul.size() + 1)) || !b->best.ensure(sizeof(int32_t) * 3 * (size_t)m) ||
+      !b->tmpl.ensure(tb) || !b->reqs.ensure(tb) || !b->flags.ensure(sizeof(int32_t) * (size_t)m) ||
+      !b->pd.ensure(sizeof(gdp_pair) * (size_t)pcap) || !b->cd.ensure(sizeof(int32_t) * 6 * (size_t)m) ||
+      !b->len.ensure(sizeof(int32_t) * (size_t)m) || !b->trim.ensure(sizeof(int32_t) * (size_t)m) ||
+      !b->off.ensure(sizeof(int64_t) * (size_t)m) || !b->res.ensure(sizeof(gdp_eg_result) * (size_t)m))
+    return err(GDP_ENOMEM, "gdp_eg_run: device allocation failed");
+  auto up = [&](Grow &d, const void *h, size_t bytes) {
+    return bytes == 0 || hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, stream) == hipSuccess;
+  };
+  if (!up(b->q, b->hq.data(), b->hq.size()) || !up(b->descs, b->hdescs.data(), (size_t)used) ||
+      !up(b->segs, b->hsegs.data(), sizeof(gdp_segment) * (size_t)m) ||
+      !up(b->ul, b->hul.data(), sizeof(int32_t) * b->hul.size()) || !up(b->tmpl, b->hreqs.data(), tb) ||
+      !up(b->reqs, b->hreqs.data(), tb) || !up(b->flags, b->hflags.data(), sizeof(int32_t) * (size_t)m))
+    return err(GDP_EHIP, "gdp_eg_run: upload failed");
+  // the nogaps calls have no best cell: finalscore 0
+  if (hipMemsetAsync(b->best.p, 0, sizeof(int32_t) * 3 * (size_t)m, stream) != hipSuccess)
+    return err(GDP_EHIP, "gdp_eg_run: clear failed");
+  const char *dq = b->q.as<char>(), *dg = b->g.as<char>();
+  if ((rc = gdp_genome_segments(d_genome, genome_format, genome_len, b->segs.as<gdp_segment>(), m, b->g.as<char>(),
+                                stream)) != GDP_OK)
+    return rc;
+  if ((rc = gdp_fill_ex(b->descs.p, np, kernels, dq, dg, b->dirs.as<uint32_t>(), b->scores.as<int16_t>(),
+                        b->fin.as<int32_t>(), stream)) != GDP_OK)
+    return rc;
+  // best end cells of each mode's calls, writing their walks' requests
+  if (m0 > 0 &&
+      (rc = gdp_best_endpoint_ex(b->descs.p, b->ul.as<int32_t>(), m0, 0, b->scores.as<int16_t>(),
+                                 b->best.as<int32_t>(), b->tmpl.as<gdp_tb_request>(), b->reqs.as<gdp_tb_request>(),
+                                 stream)) != GDP_OK)
+    return rc;
+  if (m1 > 0 && (rc = gdp_best_endpoint_ex(b->descs.p, b->ul.as<int32_t>() + 2 * m0, m1, 1, b->scores.as<int16_t>(),
+                                           b->best.as<int32_t>() + 3 * m0, b->tmpl.as<gdp_tb_request>() + m0,
+                                           b->reqs.as<gdp_tb_request>() + m0, stream)) != GDP_OK)
+    return rc;
+  if ((rc = gdp_traceback_ex(b->descs.p, b->reqs.as<gdp_tb_request>(), m, dq, dg, b->dirs.as<uint32_t>(), d_genome,
+                             genome_format, genome_len, b->pd.as<gdp_pair>(), b->cd.as<int32_t>(), stream)) != GDP_OK)
+    return rc;
+  if (gdp_launch_eg(0, m, five, b->flags.as<int32_t>(), b->reqs.as<gdp_tb_request>(), b->pd.as<gdp_pair>(),
+                    b->cd.as<int32_t>(), nullptr, b->len.as<int32_t>(), b->trim.as<int32_t>(), nullptr, nullptr,
+                    nullptr, stream))
+    return err(GDP_EHIP, "gdp_eg_run: lengths failed");
+  b->hlen.resize((size_t)m);
+  if (hipMemcpyAsync(b->hlen.data(), b->len.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    return err(GDP_EHIP, "gdp_eg_run: lengths copy failed");
+  b->hoff.resize((size_t)m);
+  int64_t total = 0;
+  for (int j = 0; j < m; j++) {
+    b->hoff[j] = total;
+    total += b->hlen[j] > 0 ? b->hlen[j] : 0;
+  }
+  if (!b->out.ensure(sizeof(gdp_pair) * (size_t)(total > 0 ? total : 1)) ||
+      !up(b->off, b->hoff.data(), sizeof(int64_t) * (size_t)m))
+    return err(GDP_ENOMEM, "gdp_eg_run: output allocation failed");
+  if (gdp_launch_eg(1, m, five, nullptr, b->reqs.as<gdp_tb_request>(), b->pd.as<gdp_pair>(), b->cd.as<int32_t>(),
+                    b->best.as<int32_t>(), b->len.as<int32_t>(), b->trim.as<int32_t>(), b->off.as<int64_t>(),
+                    b->out.as<gdp_pair>(), b->res.as<gdp_eg_result>(), stream))
+    return err(GDP_EHIP, "gdp_eg_run: join failed");
+  b->lres.resize((size_t)m);
+  if (!b->pairs.ensure(sizeof(gdp_pair) * (size_t)(total > 0 ? total : 1)))
+    return err(GDP_ENOMEM, "gdp_eg_run: pinned host allocation failed");
+  if (hipMemcpyAsync(b->lres.data(), b->res.p, sizeof(gdp_eg_result) * m, hipMemcpyDeviceToHost, stream) !=
+          hipSuccess ||
+      (total > 0 &&
+       hipMemcpyAsync(b->pairs.p, b->out.p, sizeof(gdp_pair) * (size_t)total, hipMemcpyDeviceToHost, stream) !=
+           hipSuccess) ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    return err(GDP_EHIP, "gdp_eg_run: result copy failed");
+  for (int j = 0; j < m; j++) b->results[b->live[j]] = b->lres[j];
+  b->npairs = total;
+  return GDP_OK;
+}

@@ -799,7 +799,7 @@ __global__ __launch_bounds__(256) void gdp_segments_kernel(const void *__restric
     const int64_t j = sg.revcomp ? len - 1 - i : i;
     int ch = (j < lo || j >= hi) ? '*' : genome_at(genome, fmt, glen, (uint32_t)(start + j));
     if (sg.revcomp) ch = compl_char(ch);
-    o[i] = (char)ch;
+    o[sg.reverse ? len - 1 - i : i] = (char)ch;
   }
 }
 
@@ -2227,6 +2227,79 @@ extern "C" int gdp_launch_sg(int stage, int m, const gdp_sg_dev *calls, const ch
     hipLaunchKernelGGL(gdp_sg_join_kernel, dim3(gdp_imin(m, 8192)), dim3(64), 0, stream, m, calls, reqs,
                        (const gdp_pair *)ps, pd, (const int32_t *)cs, cd, (const int32_t *)len,
                        (const int32_t *)choice, off, out, res);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ------------------------------------------- batched Dynprog_end{5,3}_gap
+
+// The ends' post-processing after their walks (dynprog_end.c:1622-1650,
+// 2214-2243), per call: flags = endalign (bits 0-1, Endalign_T) | 4 when
+// require_pos_score_p.  counts = the walk's {npairs, traceback_score,
+// nmatches, nmismatches, nopens, nindels}.  The reference returns NULL
+// - for require_pos_score_p outside QUERYEND_NOGAPS (it tests the score it
+//   has just set to 0), with every counter 0;
+// - for QUERYEND_GAP and BEST_LOCAL when nmatches + 1 < nmismatches, with
+//   traceback_score 0;
+// and otherwise drops the pushed list's first pairs while they are indels
+// (comp '-'); an empty list is NULL.  len = the kept pairs (-1 NULL), trim =
+// the dropped ones; the counters are fixed up in place.
+__global__ void gdp_eg_len_kernel(int m, const int32_t *__restrict__ flags, const gdp_tb_request *__restrict__ reqs,
+                                  const gdp_pair *__restrict__ pd, int32_t *__restrict__ cd, int32_t *__restrict__ len,
+                                  int32_t *__restrict__ trim) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  int32_t *c = cd + 6 * (int64_t)k;
+  const int f = flags[k], endalign = f & 3;
+  const bool gap_or_local = endalign == 0 || endalign == 3;  // QUERYEND_GAP, BEST_LOCAL
+  const int n = c[0] > 0 ? c[0] : 0;
+  int t = 0, l = -1;
+  if ((f & 4) && endalign != 2) {
+    for (int x = 1; x < 6; x++) c[x] = 0;
+  } else if (gap_or_local && c[2] + 1 < c[3]) {
+    c[1] = 0;
+  } else {
+    const gdp_pair *p = pd + reqs[k].pair_off;
+    while (t < n && p[t].comp == '-') t++;
+    l = n - t > 0 ? n - t : -1;
+  }
+  len[k] = l;
+  trim[k] = t;
+}
+
+// Copies each call's kept pairs to the compact output in the order of the
+// returned list from its head: the 5' caller returns the pushed list
+// reversed, the 3' caller as pushed.  One workgroup per call.
+__global__ void gdp_eg_join_kernel(int m, int five, const gdp_tb_request *__restrict__ reqs,
+                                   const gdp_pair *__restrict__ pd, const int32_t *__restrict__ cd,
+                                   const int32_t *__restrict__ best, const int32_t *__restrict__ len,
+                                   const int32_t *__restrict__ trim, const int64_t *__restrict__ off,
+                                   gdp_pair *__restrict__ out, gdp_eg_result *__restrict__ res) {
+  for (int k = blockIdx.x; k < m; k += gridDim.x) {
+    const int n = len[k], t = trim[k];
+    const gdp_pair *src = pd + reqs[k].pair_off + t;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) out[off[k] + j] = src[five ? n - 1 - j : j];
+    if (threadIdx.x == 0) {
+      gdp_eg_result r;
+      r.pair_off = off[k];
+      r.npairs = n;
+      for (int x = 0; x < 5; x++) r.counters[x] = cd[6 * (int64_t)k + 1 + x];
+      r.finalscore = best[3 * (int64_t)k];
+      r.advanced = 1;
+      res[k] = r;
+    }
+  }
+}
+
+extern "C" int gdp_launch_eg(int stage, int m, int five, const int32_t *flags, const gdp_tb_request *reqs,
+                             const gdp_pair *pd, int32_t *cd, const int32_t *best, int32_t *len, int32_t *trim,
+                             const int64_t *off, gdp_pair *out, gdp_eg_result *res, hipStream_t stream) {
+  if (m <= 0) return 0;
+  if (stage == 0)
+    hipLaunchKernelGGL(gdp_eg_len_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, m, flags, reqs, pd, cd, len,
+                       trim);
+  else
+    hipLaunchKernelGGL(gdp_eg_join_kernel, dim3(gdp_imin(m, 8192)), dim3(64), 0, stream, m, five, reqs, pd,
+                       (const int32_t *)cd, best, (const int32_t *)len, (const int32_t *)trim, off, out, res);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -1,5 +1,7 @@
-"""GMAP's single-gap caller for many gaps at once: Dynprog_single_gap
-(dynprog_single.c:428-675) run by the library (gdp_sg_run).
+"""GMAP's single-gap and end-gap callers for many gaps at once:
+Dynprog_single_gap (dynprog_single.c:428-675) run by the library
+(gdp_sg_run), and Dynprog_end5_gap / Dynprog_end3_gap (dynprog_end.c:
+1293-1650, 1923-2243) run likewise (gdp_eg_run).
 
 Per call the reference picks the mismatch type and gap penalties by defect
 rate, cuts the genome window (reverse complement on the minus strand), takes
@@ -8,13 +10,19 @@ that list has at most one mismatch, and otherwise fills the band of
 Dynprog_compute_bands with the int8 or int16 kernel and walks back from the
 corner.  Here the windows are cut from a device genome, both lists are made
 on the device and a join kernel keeps the one the reference returns.
+
+The end callers fill both triangles of the read end against its window
+(ENDQ scores, end penalties by defect rate), take the best end cell of the
+mode asked (or the nogaps diagonal), walk back from it, drop the result when
+it has too many mismatches, and trim indels from the far end; all of it on
+the device.
 """
 import ctypes
 
 import numpy as np
 
 from .batch import _check, init
-from .lib import PAIR_DTYPE, SG_CALL_DTYPE, SG_RESULT_DTYPE
+from .lib import EG_CALL_DTYPE, EG_RESULT_DTYPE, PAIR_DTYPE, SG_CALL_DTYPE, SG_RESULT_DTYPE
 
 _batch = None
 
@@ -71,4 +79,62 @@ def single_gap(calls, qbuf, genome, max_rlength=500, max_glength=2000, user_pena
     for k in np.nonzero(res["npairs"] >= 0)[0]:
         o, m = int(res["pair_off"][k]), int(res["npairs"][k])
         out[k] = (pairs[o:o + m], res["counters"][k].astype(np.int32), bool(res["simple"][k]))
+    return out, res
+
+
+_ebatch = None
+
+
+def _view(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_uint8)),
+                                 (n * dtype.itemsize,)).view(dtype).copy()
+
+
+class EndGapBatch:
+    """A gdp_eg_batch: device and host buffers reused across runs."""
+
+    def __init__(self):
+        self.L = init()
+        self.h = self.L.gdp_eg_create()
+
+    def run(self, five, calls, qbuf, genome, max_rlength=500, max_glength=2000, user_penalties=None,
+            require_pos_score=False, stream=None):
+        """calls: EG_CALL_DTYPE rows; returns (results EG_RESULT_DTYPE,
+        pairs PAIR_DTYPE)."""
+        L = self.L
+        calls = np.ascontiguousarray(calls, EG_CALL_DTYPE)
+        q = np.frombuffer(bytes(qbuf), np.uint8)
+        uo, ue = user_penalties if user_penalties is not None else (0, 0)
+        _check(L, L.gdp_eg_run(self.h, 1 if five else 0, calls.ctypes.data, len(calls), q.ctypes.data, len(q),
+                               genome.ptr, genome.fmt, genome.nbases, max_rlength, max_glength,
+                               1 if user_penalties is not None else 0, uo, ue, 1 if require_pos_score else 0,
+                               stream), "gdp_eg_run")
+        res = _view(L.gdp_eg_results(self.h), len(calls), EG_RESULT_DTYPE)
+        pairs = _view(L.gdp_eg_pairs(self.h), int(L.gdp_eg_npairs(self.h)), PAIR_DTYPE)
+        return res, pairs
+
+    def close(self):
+        if self.h:
+            self.L.gdp_eg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def end_gap(five, calls, qbuf, genome, max_rlength=500, max_glength=2000, user_penalties=None,
+            require_pos_score=False):
+    """Dynprog_end5_gap (five) or Dynprog_end3_gap for every call.  Returns
+    (per call None or (pairs from the returned list's head, {traceback_score,
+    nmatches, nmismatches, nopens, nindels}), the EG_RESULT_DTYPE rows)."""
+    global _ebatch
+    if _ebatch is None:
+        _ebatch = EndGapBatch()
+    res, pairs = _ebatch.run(five, calls, qbuf, genome, max_rlength, max_glength, user_penalties, require_pos_score)
+    out = [None] * len(res)
+    for k in np.nonzero(res["npairs"] >= 0)[0]:
+        o, m = int(res["pair_off"][k]), int(res["npairs"][k])
+        out[k] = (pairs[o:o + m], res["counters"][k].astype(np.int32))
     return out, res

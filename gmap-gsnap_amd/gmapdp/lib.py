@@ -63,9 +63,15 @@ SG_CALL_DTYPE = np.dtype([("qoff", "<i8"), ("rl", "<i4"), ("gl", "<i4"), ("roffs
                           ("genestrand", "i1"), ("jump_late", "i1"), ("widebandp", "i1"), ("defect_rate", "<f8")])
 SG_RESULT_DTYPE = np.dtype([("pair_off", "<i8"), ("npairs", "<i4"), ("counters", "<i4", (5,)), ("simple", "<i4"),
                             ("pad", "<i4")])
+EG_CALL_DTYPE = np.dtype([("qoff", "<i8"), ("rl", "<i4"), ("gl", "<i4"), ("roffset", "<i4"), ("goffset", "<i4"),
+                          ("chroffset", "<u4"), ("chrhigh", "<u4"), ("extraband", "<i4"), ("watsonp", "i1"),
+                          ("genestrand", "i1"), ("jump_late", "i1"), ("endalign", "i1"), ("defect_rate", "<f8")])
+EG_RESULT_DTYPE = np.dtype([("pair_off", "<i8"), ("npairs", "<i4"), ("counters", "<i4", (5,)), ("finalscore", "<i4"),
+                            ("advanced", "<i4")])
 assert SG_CALL_DTYPE.itemsize == 48 and SG_RESULT_DTYPE.itemsize == 40
+assert EG_CALL_DTYPE.itemsize == 48 and EG_RESULT_DTYPE.itemsize == 40
 SEGMENT_DTYPE = np.dtype([("coord", "<u4"), ("bound", "<u4"), ("length", "<i4"), ("left", "i1"), ("revcomp", "i1"),
-                          ("pad", "<i2"), ("out_off", "<i8")])
+                          ("reverse", "<i2"), ("out_off", "<i8")])
 ALIGN_REQ_DTYPE = np.dtype([("tag", "<u8"), ("read", "<u8"), ("rl", "<i4"), ("gl", "<i4"), ("roffset", "<i4"),
                             ("goffset", "<i4"), ("chroffset", "<u4"), ("chrhigh", "<u4"), ("extraband", "<i4"),
                             ("watsonp", "i1"), ("widebandp", "i1"), ("jump_late", "i1"), ("genestrand", "i1"),
@@ -149,6 +155,13 @@ def load(path=None):
         "gdp_sg_npairs": (i64, [vp]),
         "gdp_sg_results": (vp, [vp]),
         "gdp_sg_pairs": (vp, [vp]),
+        "gdp_eg_create": (vp, []),
+        "gdp_eg_destroy": (None, [vp]),
+        "gdp_eg_run": (ctypes.c_int, [vp, i32, vp, i32, vp, i64, vp, i32, ctypes.c_uint32, i32, i32, i32, i32, i32, i32,
+                                      vp]),
+        "gdp_eg_npairs": (i64, [vp]),
+        "gdp_eg_results": (vp, [vp]),
+        "gdp_eg_pairs": (vp, [vp]),
         "gdp_traceback": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, ctypes.c_uint32, vp, vp, vp]),
         "gdp_traceback_ex": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp]),
         "gdp_traceback_ends": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp, vp]),

@@ -445,14 +445,16 @@ int gdp_genomecomp_pack(const char *seq, uint64_t nbases, uint32_t *out);
    past bound = chrhigh read '*'; left = 1: Genome_get_segment_left
    (genome.c:11078-11120) of [coord - length, coord), those below
    bound = chroffset read '*'.  revcomp reverses and complements the segment.
-   The length bytes are written at d_out + out_off (no terminator). */
+   reverse = 1 then writes it last byte first: the order in which a fill
+   reading the segment backwards from its last base (revp) takes it.  The
+   length bytes are written at d_out + out_off (no terminator). */
 typedef struct gdp_segment {
   uint32_t coord;
   uint32_t bound;
   int32_t length;
   int8_t left;
   int8_t revcomp;
-  int16_t pad;
+  int16_t reverse;
   int64_t out_off;
 } gdp_segment;
 
@@ -655,6 +657,83 @@ int gdp_sg_run(gdp_sg_batch *b, const gdp_sg_call *calls, int32_t n, const char 
 int64_t gdp_sg_npairs(const gdp_sg_batch *b);
 const gdp_sg_result *gdp_sg_results(const gdp_sg_batch *b);
 const gdp_pair *gdp_sg_pairs(const gdp_sg_batch *b);
+
+/* ---- End gaps: the whole Dynprog_end5_gap / Dynprog_end3_gap caller ---- */
+
+/* One call of Dynprog_end5_gap (dynprog_end.c:1293-1650) or Dynprog_end3_gap
+   (1923-2243), 48 bytes, pad zero.  The read is rl bytes at qoff of the
+   batch's query buffer, first base first (the 5' caller is handed a pointer
+   to its last base, rev_rsequence = qbuf + qoff + rl - 1, and reads it
+   backwards).  roffset / goffset are the 5' caller's rev_roffset /
+   rev_goffset or the 3' caller's roffset / goffset.  endalign is Endalign_T
+   (dynprog.h:24): 0 QUERYEND_GAP, 1 QUERYEND_INDELS, 2 QUERYEND_NOGAPS,
+   3 BEST_LOCAL.  The genome window comes from a device genome as the
+   callers take it: 5' Genome_get_segment_left(chroffset + rev_goffset + 1,
+   gl, chroffset) on the plus strand and the reverse complement
+   Genome_get_segment_right(chrhigh - rev_goffset, gl, chrhigh) on the
+   minus strand; 3' as Dynprog_single_gap takes its window. */
+typedef struct gdp_eg_call {
+  int64_t qoff;
+  int32_t rl, gl;
+  int32_t roffset, goffset;
+  uint32_t chroffset, chrhigh;
+  int32_t extraband;       /* extraband_end */
+  int8_t watsonp;
+  int8_t genestrand;
+  int8_t jump_late;        /* jump_late_p */
+  int8_t endalign;
+  double defect_rate;
+} gdp_eg_call;
+
+/* Per call: npairs -1 for NULL, else the records at pair_off in the order
+   of the returned list from its head; counters = {traceback_score,
+   nmatches, nmismatches, nopens, nindels} as the caller leaves them;
+   finalscore = the best end cell's score (0 for QUERYEND_NOGAPS), which
+   the reference computes but does not return; advanced = 1 when the
+   reference advances its dynprogindex (every return but the early NULLs for
+   lengths and offsets). */
+typedef struct gdp_eg_result {
+  int64_t pair_off;
+  int32_t npairs;
+  int32_t counters[5];
+  int32_t finalscore;
+  int32_t advanced;
+} gdp_eg_result;
+
+typedef struct gdp_eg_batch gdp_eg_batch;
+gdp_eg_batch *gdp_eg_create(void);
+void gdp_eg_destroy(gdp_eg_batch *b);
+
+/* Runs n calls of the 5' caller (five = 1) or the 3' caller (five = 0)
+   against a device genome, the reference's steps for each:
+   - rl <= 0, gl <= 0, and on the 5' side rev_goffset < 0: NULL with every
+     counter 0;
+   - rl and gl chopped to max_rlength / max_glength outside
+     QUERYEND_NOGAPS;
+   - ENDQ scores, END_OPEN by defect rate and END_EXTEND -2
+     (dynprog_end.c:88-94), or user_open / user_extend when user_dynprog;
+   - QUERYEND_GAP and BEST_LOCAL: the upper and lower fills over the wide
+     bands of Dynprog_compute_bands, int8 when rl or gl is below
+     use8p_size[ENDQ], the 5' fills with the opposite tie rule, and the walk
+     from find_best_endpoint's cell in the triangle holding it;
+     QUERYEND_INDELS the same over the narrow bands from
+     find_best_endpoint_to_queryend_indels' cell; QUERYEND_NOGAPS the
+     diagonal walk traceback_nogaps from (min(rl, gl), min(rl, gl));
+   - require_pos_score_p: NULL outside QUERYEND_NOGAPS (the reference tests
+     the score it has just set to 0), every counter 0;
+   - QUERYEND_GAP and BEST_LOCAL: NULL with traceback_score 0 when
+     nmatches + 1 < nmismatches;
+   - else the pushed list less its first pairs while they are indels,
+     returned reversed (5') or as pushed (3').
+   Every step runs on the device; results and pairs are copied back to the
+   batch's host arrays before it returns. */
+int gdp_eg_run(gdp_eg_batch *b, int32_t five, const gdp_eg_call *calls, int32_t n, const char *qbuf, int64_t qbytes,
+               const void *d_genome, int32_t genome_format, uint32_t genome_len, int32_t max_rlength,
+               int32_t max_glength, int32_t user_dynprog, int32_t user_open, int32_t user_extend,
+               int32_t require_pos_score, void *stream);
+int64_t gdp_eg_npairs(const gdp_eg_batch *b);
+const gdp_eg_result *gdp_eg_results(const gdp_eg_batch *b);
+const gdp_pair *gdp_eg_pairs(const gdp_eg_batch *b);
 
 /* ---- Serving: continuous batching of alignment requests -------------- */
 
