@@ -109,6 +109,94 @@ def cpu_baseline(wl, sample_n, threads):
                       % (sample_n, sample_n, dt, dtf, threads)}
 
 
+def _splice_ends(n, seed=1007):
+    """Splice-junction ends of the end-extension mix: read ends of 10-150
+    bases against their genome window plus 0-20 bases (3 % substitutions),
+    extraband 6, continuation columns up to half the read length."""
+    rng = np.random.default_rng(seed)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    ends = []
+    for k in range(n):
+        rl = int(rng.integers(10, 151))
+        gl = rl + int(rng.integers(0, 21))
+        g = acgt[rng.integers(0, 4, gl)]
+        q = g[:rl].copy()
+        sub = rng.random(rl) < 0.03
+        q[sub] = acgt[rng.integers(0, 4, int(sub.sum()))]
+        anchor = int(rng.integers(1000, 1 << 30))
+        ends.append({"rev_rseq": q.tobytes(), "rev_gseq": g.tobytes(), "rev_galt": g.tobytes(),
+                     "rev_roffset": rl + 100, "anchor": anchor, "far": anchor - int(rng.integers(50, 5000)),
+                     "genestrand": 0, "jump_late": bool(k % 2), "extraband": 6, "defect_rate": 0.001,
+                     "contlength": int(rng.integers(0, rl // 2 + 1))})
+    return ends
+
+
+def callers_bench(args):
+    """GMAP's DP callers batched in the library, from host buffers in to host
+    results out (PCIe included): Dynprog_end5_splicejunction (gdp_sj_run),
+    Dynprog_single_gap (gdp_sg_run) and Dynprog_end5_gap (gdp_eg_run) over a
+    device genome.  Each is run once untimed, then timed over 5 runs."""
+    from gmapdp import callers, ends, serve
+    from gmapdp.lib import EG_CALL_DTYPE, SG_CALL_DTYPE
+    out = {}
+
+    def timed(f, reps=5):
+        f()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = f()
+        return (time.perf_counter() - t0) / reps, r
+
+    n = args.callers_n
+    arr, qbuf, gbuf = ends.pack_ends(_splice_ends(n), True)
+    sb = ends.SpliceBatch()
+    dt, (res, pairs) = timed(lambda: sb.run(True, arr, qbuf, gbuf, copy=False))
+    out["end5_splicejunction"] = {"calls": n, "seconds": round(dt, 4), "calls_per_s": round(n / dt, 1),
+                                  "pairs": int(len(pairs)), "walked": int((res["npairs"] >= 0).sum())}
+    log("callers: end5_splicejunction %d ends in %.4f s = %.0f ends/s" % (n, dt, n / dt))
+
+    nb = 100_000_000
+    gc = serve.gen_genomecomp(1005, nb)
+    dg = serve.DeviceGenome(gc, nb)
+    reqs, reads = serve.gen_requests(1005, 0, n, gc, nb)
+    # the reads packed end to end, as a caller hands them over
+    src = reqs["read"].astype(np.int64) - reads.ctypes.data
+    rl = reqs["rl"].astype(np.int64)
+    qoff = np.concatenate([[0], np.cumsum(rl)[:-1]])
+    qbuf = np.zeros(int(rl.sum()) + 16, np.uint8)
+    for k in range(n):
+        qbuf[qoff[k]:qoff[k] + rl[k]] = reads[src[k]:src[k] + rl[k]]
+    del reads
+    sg = np.zeros(n, SG_CALL_DTYPE)
+    sg["qoff"] = qoff
+    for f in ("rl", "gl", "roffset", "goffset", "chroffset", "chrhigh", "extraband", "watsonp", "genestrand",
+              "jump_late", "widebandp"):
+        sg[f] = reqs[f]
+    sg["defect_rate"] = np.where(np.arange(n) % 2 == 0, 0.001, 0.02)
+    b = callers.SingleGapBatch()
+    dt, (res, pairs) = timed(lambda: b.run(sg, qbuf, dg, max_rlength=1000, max_glength=1500))
+    out["single_gap"] = {"calls": n, "seconds": round(dt, 4), "calls_per_s": round(n / dt, 1),
+                         "pairs": int(len(pairs)), "simple": int(res["simple"].sum()),
+                         "sample": "the serving mix (rl log-uniform 20-1000) on a 1e8-base device genome, "
+                                   "HIGHQ and LOWQ alternating, max_rlength 1000, max_glength 1500"}
+    log("callers: single_gap %d calls in %.4f s = %.0f calls/s" % (n, dt, n / dt))
+    eg = np.zeros(n, EG_CALL_DTYPE)
+    eg["qoff"] = qoff
+    for f in ("rl", "roffset", "chroffset", "chrhigh", "extraband", "watsonp", "genestrand", "jump_late"):
+        eg[f] = reqs[f]
+    eg["gl"] = reqs["rl"] + 20
+    eg["goffset"] = reqs["goffset"] + reqs["gl"]
+    eg["endalign"] = np.arange(n) % 4
+    eg["defect_rate"] = 0.001
+    b2 = callers.EndGapBatch()
+    dt, (res, pairs) = timed(lambda: b2.run(True, eg, qbuf, dg, max_rlength=1000, max_glength=1500))
+    out["end5_gap"] = {"calls": n, "seconds": round(dt, 4), "calls_per_s": round(n / dt, 1), "pairs": int(len(pairs)),
+                       "sample": "read ends of the serving mix against windows 20 bases longer, the four end "
+                                 "modes in turn"}
+    log("callers: end5_gap %d calls in %.4f s = %.0f calls/s" % (n, dt, n / dt))
+    return out
+
+
 def serving_bench(args):
     """Open-loop Poisson load against the request loop over a device genome."""
     from gmapdp import serve
@@ -235,6 +323,9 @@ def main():
     ap.add_argument("--serve-mix", type=int, default=0,
                     help="band spread allowed in the server's mixed waves (0: one band per wave; serving "
                          "saturation measured the same at 0, 24 and 64)")
+    ap.add_argument("--callers", action="store_true",
+                    help="also time GMAP's batched DP callers (splice-junction ends, single gaps, end gaps)")
+    ap.add_argument("--callers-n", type=int, default=200_000, help="calls of each caller with --callers")
     ap.add_argument("--serve-genome", type=int, default=3_100_000_000, help="bases of the device genome")
     ap.add_argument("--serve-requests", type=int, default=600_000)
     ap.add_argument("--serve-batch", type=int, default=0,
@@ -462,10 +553,12 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_fill")
 
     serving = None
-    if not args.no_serve:
+    if not args.no_serve or args.callers:
         del b, reqs_dev
         torch.cuda.empty_cache()
+    if not args.no_serve:
         serving = serving_bench(args)
+    callers_out = callers_bench(args) if args.callers else None
 
     cpu = None
     if not args.no_cpu:
@@ -515,6 +608,7 @@ def main():
         "cpu_baseline": cpu,
         "bridge_ms": None if bridge_ms is None else round(bridge_ms, 3),
         "serving": serving,
+        "callers": callers_out,
         "tuning": tuning,
     }
     print(json.dumps(out), flush=True)

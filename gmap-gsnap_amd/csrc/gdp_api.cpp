@@ -175,6 +175,18 @@ static int pair_lanes(const gdp_problem &p, const gdp_plan_params &pp) {
 // Wave class: the ring of a class holds at least P step slots.
 static int pair_class(int G, int P) { return G == 64 ? 2 : (P <= 128 ? 0 : 1); }
 
+// Sorts indices by a key computed once per index, ties in their present
+// order: what a stable sort of the indices by that key gives.
+template <class KeyFn>
+static void sort_by_key(std::vector<int32_t> &ix, KeyFn key) {
+  std::vector<std::pair<uint64_t, int32_t>> kv(ix.size());
+  for (size_t j = 0; j < ix.size(); j++) kv[j] = {key(ix[j]), (int32_t)j};
+  std::sort(kv.begin(), kv.end());
+  std::vector<int32_t> out(ix.size());
+  for (size_t j = 0; j < ix.size(); j++) out[j] = ix[kv[j].second];
+  ix.swap(out);
+}
+
 extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t flags, const gdp_plan_params *params,
                               void *descs_out, int64_t *dir_words, int64_t *score_slots) {
   if (n < 0 || (n > 0 && probs_in == nullptr) || descs_out == nullptr)
@@ -254,16 +266,15 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
   // pp.mix_span > 0 the problems left over from full waves of one band go
   // to mixed waves, whose problems' lower and upper bands each lie within
   // mix_span of the wave's largest (the sweep is laid out for those).
-  std::stable_sort(paired.begin(), paired.end(), [&](int32_t a, int32_t b) {
-    const gdp_problem &x = probs[a], &y = probs[b];
-    const int kx = pair_lanes(x, pp), ky = pair_lanes(y, pp);
-    if (kx != ky) return kx < ky;
-    if ((x.jump_late != 0) != (y.jump_late != 0)) return (x.jump_late != 0) < (y.jump_late != 0);
-    if (x.lband != y.lband) return x.lband < y.lband;
-    if (x.uband != y.uband) return x.uband < y.uband;
-    if (x.rl != y.rl) return x.rl < y.rl;
-    return x.gl < y.gl;
-  });
+  // key: lane groups, tie rule, lband, uband, rl, gl (bands and lengths are
+  // below 2^15 after narrowing, so each fits its field)
+  auto band_key = [&](int32_t i) {
+    const gdp_problem &x = probs[i];
+    return ((uint64_t)(pair_lanes(x, pp) == 64) << 63) | ((uint64_t)(x.jump_late != 0) << 62) |
+           ((uint64_t)(uint16_t)x.lband << 46) | ((uint64_t)(uint16_t)x.uband << 30) | ((uint64_t)x.rl << 15) |
+           (uint64_t)x.gl;
+  };
+  sort_by_key(paired, band_key);
   std::vector<gdp_wave> waves[GDP_NCLS];
   auto emit = [&](const int32_t *ix, int cnt, int G) {
     gdp_wave w;
@@ -332,15 +343,11 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
     // taken in order of rl within a bin, full waves first; what is left of
     // the bins is binned again by 2B.
     auto bucketed = [&](std::vector<int32_t> &list, int B, bool full_only, std::vector<int32_t> &left) {
-      std::stable_sort(list.begin(), list.end(), [&](int32_t a, int32_t b) {
-        const gdp_problem &x = probs[a], &y = probs[b];
-        const int kx = pair_lanes(x, pp), ky = pair_lanes(y, pp);
-        if (kx != ky) return kx < ky;
-        if ((x.jump_late != 0) != (y.jump_late != 0)) return (x.jump_late != 0) < (y.jump_late != 0);
-        if (x.lband / B != y.lband / B) return x.lband / B < y.lband / B;
-        if (x.uband / B != y.uband / B) return x.uband / B < y.uband / B;
-        if (x.rl != y.rl) return x.rl < y.rl;
-        return x.gl < y.gl;
+      sort_by_key(list, [&](int32_t i) {
+        const gdp_problem &x = probs[i];
+        return ((uint64_t)(pair_lanes(x, pp) == 64) << 63) | ((uint64_t)(x.jump_late != 0) << 62) |
+               ((uint64_t)(x.lband / B) << 46) | ((uint64_t)(x.uband / B) << 30) | ((uint64_t)x.rl << 15) |
+               (uint64_t)x.gl;
       });
       size_t a = 0;
       while (a < list.size()) {
@@ -391,15 +398,10 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
   }
   if (!rest.empty()) {
     // neighbours in band width and lower band share a wave
-    std::stable_sort(rest.begin(), rest.end(), [&](int32_t a, int32_t b) {
-      const gdp_problem &x = probs[a], &y = probs[b];
-      const int kx = pair_lanes(x, pp), ky = pair_lanes(y, pp);
-      if (kx != ky) return kx < ky;
-      if ((x.jump_late != 0) != (y.jump_late != 0)) return (x.jump_late != 0) < (y.jump_late != 0);
-      const int wx = x.lband + x.uband, wy = y.lband + y.uband;
-      if (wx != wy) return wx < wy;
-      if (x.lband != y.lband) return x.lband < y.lband;
-      return x.rl < y.rl;
+    sort_by_key(rest, [&](int32_t i) {
+      const gdp_problem &x = probs[i];
+      return ((uint64_t)(pair_lanes(x, pp) == 64) << 63) | ((uint64_t)(x.jump_late != 0) << 62) |
+             ((uint64_t)(x.lband + x.uband) << 44) | ((uint64_t)(uint16_t)x.lband << 28) | (uint64_t)x.rl;
     });
     size_t a = 0;
     while (a < rest.size()) {

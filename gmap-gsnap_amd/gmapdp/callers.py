@@ -41,7 +41,8 @@ class SingleGapBatch:
         Returns (results SG_RESULT_DTYPE, pairs PAIR_DTYPE)."""
         L = self.L
         calls = np.ascontiguousarray(calls, SG_CALL_DTYPE)
-        q = np.frombuffer(bytes(qbuf), np.uint8)
+        q = qbuf if isinstance(qbuf, np.ndarray) else np.frombuffer(bytes(qbuf), np.uint8)
+        q = np.ascontiguousarray(q).view(np.uint8)
         uo, ue = user_penalties if user_penalties is not None else (0, 0)
         _check(L, L.gdp_sg_run(self.h, calls.ctypes.data, len(calls), q.ctypes.data, len(q), genome.ptr, genome.fmt,
                                genome.nbases, max_rlength, max_glength, 1 if user_penalties is not None else 0,
@@ -105,7 +106,8 @@ class EndGapBatch:
         pairs PAIR_DTYPE)."""
         L = self.L
         calls = np.ascontiguousarray(calls, EG_CALL_DTYPE)
-        q = np.frombuffer(bytes(qbuf), np.uint8)
+        q = qbuf if isinstance(qbuf, np.ndarray) else np.frombuffer(bytes(qbuf), np.uint8)
+        q = np.ascontiguousarray(q).view(np.uint8)
         uo, ue = user_penalties if user_penalties is not None else (0, 0)
         _check(L, L.gdp_eg_run(self.h, 1 if five else 0, calls.ctypes.data, len(calls), q.ctypes.data, len(q),
                                genome.ptr, genome.fmt, genome.nbases, max_rlength, max_glength,

@@ -138,55 +138,3 @@ def _splicejunction(ends, five, max_rlength, max_glength):
         o, m = int(res["pair_off"][k]), int(res["npairs"][k])
         out[k] = (pairs[o:o + m], res["counters"][k].astype(np.int32))
     return out
-
-
-def _random_ends(n, seed=1007):
-    """Splice-junction ends of the end-extension mix: read ends of 10-150
-    bases against their genome window plus 0-20 bases (3 % substitutions,
-    1 % indels), extraband 6, continuation columns up to the read length."""
-    rng = np.random.default_rng(seed)
-    acgt = np.frombuffer(b"ACGT", np.uint8)
-    ends = []
-    for k in range(n):
-        rl = int(rng.integers(10, 151))
-        gl = rl + int(rng.integers(0, 21))
-        g = acgt[rng.integers(0, 4, gl)]
-        q = g[:rl].copy()
-        sub = rng.random(rl) < 0.03
-        q[sub] = acgt[rng.integers(0, 4, int(sub.sum()))]
-        anchor = int(rng.integers(1000, 1 << 30))
-        ends.append({"rev_rseq": q.tobytes(), "rev_gseq": g.tobytes(), "rev_galt": g.tobytes(),
-                     "rev_roffset": rl + 100, "anchor": anchor, "far": anchor - int(rng.integers(50, 5000)),
-                     "genestrand": 0, "jump_late": bool(k % 2), "extraband": 6, "defect_rate": 0.001,
-                     "contlength": int(rng.integers(0, rl // 2 + 1))})
-    return ends
-
-
-if __name__ == "__main__":
-    # python3 -m gmapdp.ends [N]: times Dynprog_end5_splicejunction over N ends
-    # of the end-extension mix: the library call on packed ends (host buffers
-    # in and out, so the copies over PCIe are inside), then the Python API
-    # with its packing of the dicts and the per-end views (each after one
-    # untimed run of the same size).
-    import sys
-    import time
-
-    n = int(sys.argv[1]) if len(sys.argv) > 1 else 200_000
-    ends = _random_ends(n)
-    arr, qbuf, gbuf = pack_ends(ends, True)
-    sb = SpliceBatch()
-    sb.run(True, arr, qbuf, gbuf)
-    reps = 5
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        res, pairs = sb.run(True, arr, qbuf, gbuf, copy=False)
-    dt = (time.perf_counter() - t0) / reps
-    walked = int((res["npairs"] >= 0).sum())
-    print("gdp_sj_run (5'): %d ends (%d walked, %d pairs) in %.4f s = %.0f ends/s" % (n, walked, len(pairs), dt, n / dt),
-          flush=True)
-    end5_splicejunction(ends)
-    t0 = time.perf_counter()
-    out = end5_splicejunction(ends)
-    dt = time.perf_counter() - t0
-    print("end5_splicejunction (dicts in, per-end lists out): %d ends in %.3f s = %.0f ends/s" % (n, dt, n / dt),
-          flush=True)
