@@ -494,9 +494,9 @@ def main():
         bridge_ms = bev[0].elapsed_time(bev[1]) / args.steps
         log("intron bridge: %.3f ms for %d gaps" % (bridge_ms, wl.n))
 
-    # PCIe-inclusive step: inputs from pinned host memory, the step, counts and
-    # pair records back (the pair buffer is downloaded whole: rl + gl + 4
-    # records per request, of which the walk fills its count).
+    # PCIe-inclusive step: inputs from pinned host memory, the step, and each
+    # request's counts and the pair records its walk wrote back to pinned
+    # memory (gdp_download_results).
     e2e = None
     if not args.no_e2e and config not in (4, "cdna", "intron"):
         h_in = [torch.empty(t.numel(), dtype=t.dtype, pin_memory=True) for t in (b.d_descs, b.d_q, b.d_g,
@@ -512,16 +512,21 @@ def main():
             for h, t in zip(h_in, (b.d_descs, b.d_q, b.d_g, reqs_dev["d_reqs"])):
                 t.copy_(h, non_blocking=True)
             step()
-            h_cnt.copy_(reqs_dev["d_counts"], non_blocking=True)
-            h_pairs.copy_(reqs_dev["d_pairs"], non_blocking=True)
+            rc = b.L.gdp_download_results(reqs_dev["d_reqs"].data_ptr(), len(reqs), reqs_dev["d_counts"].data_ptr(),
+                                          reqs_dev["d_pairs"].data_ptr(), h_cnt.data_ptr(), h_pairs.data_ptr(),
+                                          stream.cuda_stream)
+            if rc != 0:
+                raise RuntimeError("gdp_download_results: %s" % b.L.gdp_last_error().decode())
         torch.cuda.synchronize()
         dte = (time.perf_counter() - t1) / reps
         up = sum(h.numel() * h.element_size() for h in h_in)
-        down = h_cnt.numel() * 4 + h_pairs.numel()
+        npairs = h_cnt.view(-1, 6)[:, 0].clamp(min=0).sum().item()
+        down = h_cnt.numel() * 4 + npairs * 16
         e2e = {"gcups": round(total_cells / dte / 1e9, 3), "ms_per_step": round(dte * 1e3, 3),
                "upload_bytes": int(up), "download_bytes": int(down),
                "method": "descriptors, sequences and traceback requests uploaded from pinned memory, the step, "
-                         "counts and the whole pair buffer downloaded to pinned memory, one synchronize"}
+                         "then each request's counts and the pair records its walk wrote written to pinned memory "
+                         "by a kernel (gdp_download_results), one synchronize"}
         log("PCIe-inclusive step: %.2f ms, %.1f GCUPS (%.2f GB up, %.2f GB down)" % (dte * 1e3, e2e["gcups"],
                                                                                     up / 1e9, down / 1e9))
         del h_in, h_cnt, h_pairs

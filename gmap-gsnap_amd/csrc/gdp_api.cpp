@@ -728,6 +728,23 @@ extern "C" int gdp_traceback(const void *d_descs, const gdp_tb_request *d_reqs, 
                           d_pairs, d_counts, stream);
 }
 
+extern "C" int gdp_launch_download_results(const gdp_tb_request *, int, const int32_t *, const gdp_pair *,
+                                           int32_t *, gdp_pair *, hipStream_t);
+
+extern "C" int gdp_download_results(const gdp_tb_request *d_reqs, int32_t nreq, const int32_t *d_counts,
+                                    const gdp_pair *d_pairs, int32_t *h_counts, gdp_pair *h_pairs, void *stream) {
+  if (nreq < 0 || (nreq > 0 && (!d_reqs || !d_counts || !d_pairs || !h_counts || !h_pairs)))
+    return fail(GDP_EINVAL, "gdp_download_results: null or negative argument");
+  if (nreq == 0) return GDP_OK;
+  void *dc = nullptr, *dp = nullptr;
+  if (hipHostGetDevicePointer(&dc, h_counts, 0) != hipSuccess || hipHostGetDevicePointer(&dp, h_pairs, 0) != hipSuccess)
+    return fail(GDP_EINVAL, "gdp_download_results: the host buffers are not pinned (hipHostMalloc)");
+  if (gdp_launch_download_results(d_reqs, nreq, d_counts, d_pairs, (int32_t *)dc, (gdp_pair *)dp,
+                                  (hipStream_t)stream) != 0)
+    return hip_fail("gdp_download_results", hipGetLastError());
+  return GDP_OK;
+}
+
 extern "C" int gdp_genome_segments(const void *d_genome, int32_t format, uint32_t genome_len, const gdp_segment *d_segs,
                                    int32_t n, char *d_out, void *stream) {
   if (n < 0 || (n > 0 && (!d_segs || !d_out))) return fail(GDP_EINVAL, "gdp_genome_segments: bad argument");

@@ -2334,6 +2334,35 @@ __global__ void gdp_serve_download_kernel(const uint8_t *__restrict__ d_down, ui
   __threadfence_system();
 }
 
+// A batch's walk results to host memory mapped into the device's address
+// space: each request's counts and only the pair records its walk wrote,
+// at the same offsets as on the device.  One wave per request.
+__global__ __launch_bounds__(256) void gdp_download_results_kernel(const gdp_tb_request *__restrict__ reqs, int n,
+                                                                   const int32_t *__restrict__ counts,
+                                                                   const gdp_pair *__restrict__ pairs,
+                                                                   int32_t *__restrict__ h_counts,
+                                                                   gdp_pair *__restrict__ h_pairs) {
+  const int lane = threadIdx.x & 63;
+  for (int k = blockIdx.x * 4 + (threadIdx.x >> 6); k < n; k += gridDim.x * 4) {
+    const int32_t *c = counts + 6 * (int64_t)k;
+    if (lane < 6) h_counts[6 * (int64_t)k + lane] = c[lane];
+    const int np = c[0] < reqs[k].max_pairs ? c[0] : reqs[k].max_pairs;
+    const uint4 *src = (const uint4 *)(pairs + reqs[k].pair_off);
+    uint4 *dst = (uint4 *)(h_pairs + reqs[k].pair_off);
+    for (int j = lane; j < np; j += 64) dst[j] = src[j];
+  }
+  __threadfence_system();
+}
+
+extern "C" int gdp_launch_download_results(const gdp_tb_request *d_reqs, int n, const int32_t *d_counts,
+                                           const gdp_pair *d_pairs, int32_t *h_counts, gdp_pair *h_pairs,
+                                           hipStream_t stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(gdp_download_results_kernel, dim3(gdp_imin((n + 3) / 4, 65536)), dim3(256), 0, stream, d_reqs,
+                     n, d_counts, d_pairs, h_counts, h_pairs);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 extern "C" int gdp_launch_copy(void *dst, const void *src, int64_t bytes, hipStream_t stream) {
   if (bytes <= 0) return 0;
   if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) return -1;

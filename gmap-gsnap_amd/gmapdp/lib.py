@@ -166,6 +166,7 @@ def load(path=None):
         "gdp_traceback_ex": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp]),
         "gdp_traceback_ends": (ctypes.c_int, [vp, vp, i32, vp, vp, vp, vp, i32, ctypes.c_uint32, vp, vp, vp, vp]),
         "gdp_genome_segments": (ctypes.c_int, [vp, i32, ctypes.c_uint32, vp, i32, vp, vp]),
+        "gdp_download_results": (ctypes.c_int, [vp, i32, vp, vp, vp, vp, vp]),
         "gdp_genomecomp_words": (i64, [ctypes.c_uint64]),
         "gdp_genomecomp_pack": (ctypes.c_int, [vp, ctypes.c_uint64, vp]),
         "gdp_server_create": (vp, [vp, vp, i32, ctypes.c_uint32]),

@@ -419,6 +419,14 @@ int gdp_bridge_intron(const void *d_descs, const gdp_intron_gap *d_gaps, int32_t
 
 /* ---- Genome in device memory ------------------------------------------ */
 
+/* Copies a batch's walk results to pinned host memory (hipHostMalloc'd
+   buffers as large as d_counts and d_pairs): each request's six counts and
+   only the pair records its walk wrote (counts[6k], at most max_pairs), at
+   the request's pair_off, as a kernel on the stream that writes across PCIe.
+   The rest of h_pairs is left as it was.  No counterpart in the reference. */
+int gdp_download_results(const gdp_tb_request *d_reqs, int32_t nreq, const int32_t *d_counts, const gdp_pair *d_pairs,
+                         int32_t *h_counts, gdp_pair *h_pairs, void *stream);
+
 /* Formats of a device genome handed to gdp_traceback_ex and
    gdp_genome_segments: ASCII bytes indexed by position, or GMAP's genomecomp
    image.  A genomecomp image holds three 32-bit words per 32 bases,
