@@ -1,13 +1,16 @@
-"""Autotuner of the paired sweep's lane groups over a measured batch.
+"""Autotuner of the paired sweep's plan over a measured batch.
 
-The plan's two knobs (gdp_plan_params) trade padding against sweep length:
+The plan's knobs (gdp_plan_params) trade padding against sweep length:
 16-lane groups pack eight problems into a wave with little padding, 64-lane
-groups sweep a long problem in about a quarter of the steps.  The tuner
-times real fills of a seeded sample of the caller's problems on the device
-for each candidate and keeps the best: the whole sample's fill time for
-throughput (the sample large enough that the slowest wave does not decide
-it), or the fill time of a batch of the longest problems for latency (the
-time a request waits on the slowest problem of its batch).
+groups sweep a long problem in about a quarter of the steps, and mixed waves
+(problems of neighbouring bands and rows in one wave) fill waves whose
+problems rarely share a band.  The tuner times real fills of a seeded sample
+of the caller's problems on the device for each candidate and keeps the
+best: the whole sample's fill time for throughput (the sample large enough
+that the slowest wave does not decide it), or the fill time of a batch of
+the longest reads for latency (the time a request waits on the slowest
+problem of its batch).  A second tuner picks the request server's batch
+slots, batch size and length split by measured saturation and p99.
 """
 import numpy as np
 
@@ -39,14 +42,18 @@ def _time_fill(b, reps):
     return float(np.median([a.elapsed_time(z) for a, z in ev]))
 
 
-def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", reps=2, seed=0, longest=64):
+def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", reps=2, seed=0, longest=4096):
     """Returns (params, table) where table lists (params, milliseconds) of
     every candidate: params = (g16_max_span, g64_min_rows, mix_span,
     mix_order) for throughput, (g16_max_span, g64_min_rows) for latency."""
     rng = np.random.default_rng(seed)
     n = len(problems)
     if objective == "latency":
-        idx = np.argsort(problems["rl"].astype(np.int64) * (problems["lband"] + problems["uband"] + 1))[-longest:]
+        # a batch of long reads (a server batches reads of 256 bases and more
+        # apart), whatever their bands: the slowest sweeps set a serving
+        # batch's time, and the lane groups decide how long they are
+        long_idx = np.nonzero(problems["rl"] >= 256)[0]
+        idx = np.sort(rng.choice(long_idx, size=min(longest, len(long_idx)), replace=False))
     else:
         idx = np.sort(rng.choice(n, size=min(sample, n), replace=False))
     sub = np.ascontiguousarray(problems[idx])
@@ -67,18 +74,18 @@ def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", rep
 # configuration-5 mix (3 runs each at half load) once each batch's transfers
 # ran as kernels on its own stream (before, a copy engine shared by the
 # streams ran one batch at a time): (3, 4096, 300) 2.54e6 requests/s at p99
-# 3.9 ms, (3, 8192, 300) 2.50e6 at 4.2-4.4 ms, (4, 8192, 300) 2.61e6 at
-# 5.9-6.2 ms, (2, 2048, 300) 1.46e6 at 4.5 ms.
-SERVER_KNOBS = ((3, 4096, 300), (3, 8192, 300), (3, 4096, 500), (2, 2048, 300))
+# 3.9 ms, (3, 8192, 300) 2.50e6-3.30e6 at 4.2-4.4 ms, (4, 8192, 300) 2.61e6
+# at 5.9-6.2 ms, (2, 2048, 300) 1.46e6 at 4.5 ms.
+SERVER_KNOBS = ((3, 8192, 300), (3, 4096, 300), (3, 4096, 200), (2, 2048, 300))
 
 
-def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, seconds=1.0, margin=0.9, **server_kw):
+def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, seconds=1.5, margin=0.9, **server_kw):
     """Picks (slots, max_batch, long_rows) for the request server by measurement:
     for each candidate, the saturation throughput (all requests at once) and
-    p99 latency of Poisson arrivals at half of it.  Keeps the candidate with
-    the highest saturation whose p99 is within margin * target (p99 of a
-    short run moves by several percent from run to run), else within the
-    target, else the one with the lowest p99.
+    the median p99 latency of three runs of Poisson arrivals at half of it.
+    Keeps the candidate with the highest saturation whose p99 is within
+    margin * target (p99 of a short run moves by several percent from run to
+    run), else within the target, else the one with the lowest p99.
     Returns (knob, table of (knob, saturation req/s, p99 ms))."""
     from .serve import Server
     table = []
@@ -90,10 +97,10 @@ def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, secon
             sat = len(reqs) / el
             m = int(min(len(reqs), max(2000, 0.5 * sat * seconds)))
             p99 = []
-            for rep in range(2):  # the larger of two runs: a slow stretch of the host shows in either
+            for rep in range(3):  # the median of three runs
                 lat, el, ok = srv.drive(reqs[:m], 0.5 * sat, seed=11 + rep)
                 p99.append(float(np.percentile(lat, 99)) * 1e3)
-            table.append(((slots, mb, lr), sat, max(p99)))
+            table.append(((slots, mb, lr), sat, float(np.median(p99))))
     good = [t for t in table if t[2] <= margin * target_p99_ms] or [t for t in table if t[2] <= target_p99_ms]
     best = max(good, key=lambda t: t[1]) if good else min(table, key=lambda t: t[2])
     return best[0], table
