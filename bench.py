@@ -570,15 +570,17 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_serve:
         # The server runs in a process of its own, as a deployment would run
-        # it: its streams then get hardware queues of their own instead of
-        # sharing them with the streams this process has made (measured: p99
-        # at half load 5.2-5.7 ms here against 3.9-4.2 ms in a fresh process
-        # with three batch slots).
+        # it, with a hardware queue per batch slot (GPU_MAX_HW_QUEUES; HIP's
+        # default of 4 is shared by every stream of a process, and slots
+        # sharing a queue run one after another): measured with three slots,
+        # p99 at half load 5.2-5.7 ms in this process against 3.9-4.2 ms in a
+        # fresh one; with 16 queues, 4-8 slots reach 3.4-3.6e6 requests/s.
         cmd = [sys.executable, os.path.abspath(__file__), "--serve-only"]
         for flag in ("serve_mix", "serve_genome", "serve_requests", "serve_batch", "serve_slots", "serve_long",
                      "serve_budget", "serve_seconds"):
             cmd.append("--%s=%s" % (flag.replace("_", "-"), getattr(args, flag)))
-        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+        env = dict(os.environ, GPU_MAX_HW_QUEUES="16")
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, env=env)
         if r.returncode != 0:
             raise RuntimeError("serving measurement failed (exit %d)" % r.returncode)
         serving = json.loads(r.stdout.strip().splitlines()[-1])

@@ -548,7 +548,7 @@ extern "C" double gdp_server_now(void) { return now_s(); }
 
 extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const void *d_genome, int32_t genome_format,
                                          uint32_t genome_len) {
-  if (!cfg || cfg->max_batch <= 0 || cfg->slots < 1 || cfg->slots > 4 || cfg->max_rl <= 0 ||
+  if (!cfg || cfg->max_batch <= 0 || cfg->slots < 1 || cfg->slots > 16 || cfg->max_rl <= 0 ||
       cfg->max_rl > GDP_MAX_LEN || cfg->max_gl <= 0 || cfg->max_gl > GDP_MAX_LEN || cfg->dir_budget_bytes <= 0 ||
       cfg->pair_capacity < (int64_t)cfg->max_rl + cfg->max_gl + 2) {
     set_err("gdp_server_create: bad configuration");

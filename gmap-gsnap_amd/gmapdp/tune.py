@@ -74,9 +74,12 @@ def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", rep
 # configuration-5 mix (3 runs each at half load) once each batch's transfers
 # ran as kernels on its own stream (before, a copy engine shared by the
 # streams ran one batch at a time): (3, 4096, 300) 2.54e6 requests/s at p99
-# 3.9 ms, (3, 8192, 300) 2.50e6-3.30e6 at 4.2-4.4 ms, (4, 8192, 300) 2.61e6
-# at 5.9-6.2 ms, (2, 2048, 300) 1.46e6 at 4.5 ms.
-SERVER_KNOBS = ((3, 8192, 300), (3, 4096, 300), (3, 4096, 200), (2, 2048, 300))
+# 3.9 ms, (3, 8192, 300) 2.50e6-3.30e6 at 4.2-4.4 ms, (2, 2048, 300) 1.46e6
+# at 4.5 ms.  With a hardware queue per slot (GPU_MAX_HW_QUEUES 16) more slots
+# overlap more batches: (4, 8192, 300) 3.42e6 at 3.8 ms, (6, 8192, 300)
+# 3.54e6 at 3.6 ms, (8, 8192, 300) 3.49e6 at 3.4 ms, (8, 4096, 300) 3.38e6 at
+# 3.3 ms; with HIP's default of 4 queues, 4 slots stayed above 5 ms.
+SERVER_KNOBS = ((8, 8192, 300), (6, 8192, 300), (4, 8192, 300), (3, 8192, 300))
 
 
 def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, seconds=1.5, margin=0.9, **server_kw):

@@ -768,7 +768,8 @@ typedef struct gdp_server_config {
   int64_t dir_budget_bytes; /* device memory for the direction words of all batches in flight */
   int64_t pair_capacity;    /* pair records one batch may produce (sum over requests of rl + gl + 2) */
   int32_t max_batch;        /* requests per batch */
-  int32_t slots;            /* batches in flight at once, one HIP stream each (1..4) */
+  int32_t slots;            /* batches in flight at once, one HIP stream each (1..16); batches overlap
+                               when each stream has a hardware queue of its own (GPU_MAX_HW_QUEUES) */
   int32_t max_rl;           /* longest read accepted (<= GDP_MAX_LEN) */
   int32_t max_gl;           /* longest genome segment accepted (<= GDP_MAX_LEN) */
   int32_t result_buffers;   /* pinned result buffers (>= slots; 0 = slots + 2) */
