@@ -223,23 +223,21 @@ def serving_bench(args):
     lane, table = autotune(tw.problems, tw.qbuf.tobytes(), tw.gbuf.tobytes(), objective="latency")
     del tw
     log("serving lane groups (latency autotuner): g16_max_span=%d g64_min_rows=%d" % lane)
-    if args.serve_mix > 0:
-        # requests rarely share a band: mixed paired waves hold neighbours in band
-        lane = tuple(lane) + (args.serve_mix,)
-        log("serving mixed waves: bands within %d columns share a wave" % args.serve_mix)
     knob_table = None
     if args.serve_batch and args.serve_long >= 0:
-        knob = (args.serve_slots, args.serve_batch, args.serve_long)
+        knob = (args.serve_slots, args.serve_batch, args.serve_long, max(args.serve_mix, 0), args.serve_mix_order)
     else:
-        # batch size and length split: the highest saturation whose p99 at half
-        # of it stays within 90 % of 5 ms (measured; see gmapdp.tune.tune_server)
+        # slots, batch size, length split and wave mixing: the highest
+        # saturation whose p99 at half of it stays within 90 % of 5 ms
+        # (measured; see gmapdp.tune.tune_server)
         from gmapdp.tune import tune_server
         knob, knob_table = tune_server(dg, reqs, lane, dir_budget_bytes=args.serve_budget << 20, max_rl=1000,
                                        max_gl=1500)
-        log("serving knobs (slots, max_batch, long_rows): %s" % (knob,))
-    slots, max_batch, long_rows = knob
+        log("serving knobs (slots, max_batch, long_rows, mix_span, mix_order): %s" % (knob,))
+    slots, max_batch, long_rows, mix_span, mix_order = knob
+    plan = tuple(lane[:2]) + (mix_span, mix_order)
     srv = serve.Server(dg, dir_budget_bytes=args.serve_budget << 20, max_batch=max_batch, slots=slots,
-                       max_rl=1000, max_gl=1500, pair_capacity=max_batch * 700, plan=lane, long_rows=long_rows)
+                       max_rl=1000, max_gl=1500, pair_capacity=max_batch * 700, plan=plan, long_rows=long_rows)
     try:
         srv.drive(reqs[:min(nreq, 20000)], 0.0)  # warm-up
         lat, el, ok = srv.drive(reqs, 0.0)
@@ -280,10 +278,12 @@ def serving_bench(args):
             "saturation_req_s": round(sat, 1), "saturation_gcups": round(cells.sum() / (nreq / sat) / 1e9, 2),
             "p50_ms_at_50pct": half["p50_ms"], "p99_ms_at_50pct": half["p99_ms"], "points": points,
             "batches": int(st["batches"]), "max_batch": max_batch, "slots": slots, "long_rows": long_rows,
+            "mix_span": mix_span, "mix_order": mix_order,
             "per_batch_ms": {k: round(float(st[k + "_seconds"]) / max(int(st["batches"]), 1) * 1e3, 3)
                              for k in ("prep", "retire", "batch", "device")},
             "knob_candidates": None if knob_table is None else
-            [{"slots": k[0], "max_batch": k[1], "long_rows": k[2], "saturation_req_s": round(sat_, 1),
+            [{"slots": k[0], "max_batch": k[1], "long_rows": k[2], "mix_span": k[3], "mix_order": k[4],
+              "saturation_req_s": round(sat_, 1),
               "p99_ms_at_50pct": round(p99_, 3)} for k, sat_, p99_ in knob_table],
             "lane_groups": {"g16_max_span": lane[0], "g64_min_rows": lane[1],
                             "candidates_ms": [[list(p), round(ms, 3)] for p, ms in table]},
@@ -321,9 +321,11 @@ def main():
     ap.add_argument("--mix-order", type=int, default=1, choices=(0, 1),
                     help="how mixed waves are formed with --mix-span (gdp_plan_params.mix_order): 0 leftovers in "
                          "order of band, 1 every problem binned by band and taken in order of rows within a bin")
-    ap.add_argument("--serve-mix", type=int, default=0,
-                    help="band spread allowed in the server's mixed waves (0: one band per wave; serving "
-                         "saturation measured the same at 0, 24 and 64)")
+    ap.add_argument("--serve-mix", type=int, default=8,
+                    help="mix_span of the server's batches with --serve-batch and --serve-long (0: one band per "
+                         "wave)")
+    ap.add_argument("--serve-mix-order", type=int, default=1, choices=(0, 1),
+                    help="mix_order of the server's batches with --serve-batch and --serve-long")
     ap.add_argument("--callers", action="store_true",
                     help="also time GMAP's batched DP callers (splice-junction ends, single gaps, end gaps)")
     ap.add_argument("--callers-n", type=int, default=200_000, help="calls of each caller with --callers")
@@ -331,7 +333,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=600_000)
     ap.add_argument("--serve-batch", type=int, default=0,
                     help="largest batch the server forms (0: measured choice with --serve-long)")
-    ap.add_argument("--serve-slots", type=int, default=3, help="batches in flight (streams) with --serve-batch")
+    ap.add_argument("--serve-slots", type=int, default=15, help="batches in flight (streams) with --serve-batch")
     ap.add_argument("--serve-long", type=int, default=-1,
                     help="reads of at least this many bases are batched apart (0: one class; -1: measured)")
     ap.add_argument("--serve-budget", type=int, default=8192, help="direction budget in MiB")
@@ -574,10 +576,10 @@ def main():
         # default of 4 is shared by every stream of a process, and slots
         # sharing a queue run one after another): measured with three slots,
         # p99 at half load 5.2-5.7 ms in this process against 3.9-4.2 ms in a
-        # fresh one; with 16 queues, 4-8 slots reach 3.4-3.6e6 requests/s.
+        # fresh one; with 16 queues, 15 slots reach 5.8e6 requests/s.
         cmd = [sys.executable, os.path.abspath(__file__), "--serve-only"]
-        for flag in ("serve_mix", "serve_genome", "serve_requests", "serve_batch", "serve_slots", "serve_long",
-                     "serve_budget", "serve_seconds"):
+        for flag in ("serve_mix", "serve_mix_order", "serve_genome", "serve_requests", "serve_batch", "serve_slots",
+                     "serve_long", "serve_budget", "serve_seconds"):
             cmd.append("--%s=%s" % (flag.replace("_", "-"), getattr(args, flag)))
         env = dict(os.environ, GPU_MAX_HW_QUEUES="16")
         r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, env=env)

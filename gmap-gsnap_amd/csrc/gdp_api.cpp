@@ -176,15 +176,42 @@ static int pair_lanes(const gdp_problem &p, const gdp_plan_params &pp) {
 static int pair_class(int G, int P) { return G == 64 ? 2 : (P <= 128 ? 0 : 1); }
 
 // Sorts indices by a key computed once per index, ties in their present
-// order: what a stable sort of the indices by that key gives.
+// order: what a stable sort of the indices by that key gives.  An LSD radix
+// sort over 11-bit digits, each pass stable, skipping the digits in which
+// all keys agree.
 template <class KeyFn>
 static void sort_by_key(std::vector<int32_t> &ix, KeyFn key) {
-  std::vector<std::pair<uint64_t, int32_t>> kv(ix.size());
-  for (size_t j = 0; j < ix.size(); j++) kv[j] = {key(ix[j]), (int32_t)j};
-  std::sort(kv.begin(), kv.end());
-  std::vector<int32_t> out(ix.size());
-  for (size_t j = 0; j < ix.size(); j++) out[j] = ix[kv[j].second];
-  ix.swap(out);
+  const size_t n = ix.size();
+  if (n < 2) return;
+  std::vector<uint64_t> k(n), k2(n);
+  std::vector<int32_t> v2(n);
+  uint64_t all_or = 0, all_and = ~0ull;
+  for (size_t j = 0; j < n; j++) {
+    k[j] = key(ix[j]);
+    all_or |= k[j];
+    all_and &= k[j];
+  }
+  const uint64_t varies = all_or ^ all_and;
+  constexpr int D = 11, NB = 1 << D;
+  std::vector<uint32_t> cnt(NB);
+  for (int sh = 0; sh < 64; sh += D) {
+    if (((varies >> sh) & (NB - 1)) == 0) continue;
+    std::fill(cnt.begin(), cnt.end(), 0u);
+    for (size_t j = 0; j < n; j++) cnt[(k[j] >> sh) & (NB - 1)]++;
+    uint32_t s = 0;
+    for (int b = 0; b < NB; b++) {
+      const uint32_t c = cnt[b];
+      cnt[b] = s;
+      s += c;
+    }
+    for (size_t j = 0; j < n; j++) {
+      const uint32_t pos = cnt[(k[j] >> sh) & (NB - 1)]++;
+      k2[pos] = k[j];
+      v2[pos] = ix[j];
+    }
+    k.swap(k2);
+    ix.swap(v2);
+  }
 }
 
 extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t flags, const gdp_plan_params *params,
@@ -309,7 +336,8 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
     w.mixed = (lbmin != lb || ubmin != ub) ? 1 : 0;
     w.late = probs[ix[0]].jump_late ? 1 : 0;
     w.glog = G == 16 ? 4 : 6;
-    std::vector<std::pair<int32_t, int8_t>> fin;
+    std::pair<int32_t, int8_t> fin[8];
+    int nfin = 0;
     for (int s = 0; s < cnt; s++) {
       const int idx = ix[s];
       const gdp_problem &x = probs[idx];
@@ -325,10 +353,10 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
       q.dir_off = dw;
       dw += (int64_t)(w.nsteps / 8) * G;
       if (x.rl <= x.gl + x.lband && x.rl >= x.gl - x.uband)
-        fin.push_back({gdp_p_step(G, P, lb, x.rl, x.gl), (int8_t)s});
+        fin[nfin++] = {gdp_p_step(G, P, lb, x.rl, x.gl), (int8_t)s};
     }
-    std::sort(fin.begin(), fin.end());
-    for (size_t j = 0; j < fin.size(); j++) {
+    std::sort(fin, fin + nfin);
+    for (int j = 0; j < nfin; j++) {
       w.fin_t[j] = fin[j].first;
       w.fin_slot[j] = fin[j].second;
     }
@@ -436,10 +464,14 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
     for (const gdp_wave &w : waves[c]) sink_words = std::max<int64_t>(sink_words, (int64_t)(w.nsteps / 8) * (1 << w.glog));
   const int64_t sink_off = dw;
   dw += sink_words;
-  // Longest sweeps first, so the queue ends on short ones.
-  for (int c = 0; c < GDP_NCLS; c++)
-    std::stable_sort(waves[c].begin(), waves[c].end(),
-                     [](const gdp_wave &a, const gdp_wave &b) { return a.nsteps > b.nsteps; });
+  // Longest sweeps first, so the queue ends on short ones (ties in the
+  // order the waves were formed).
+  std::vector<int32_t> worder[GDP_NCLS];
+  for (int c = 0; c < GDP_NCLS; c++) {
+    worder[c].resize(waves[c].size());
+    for (size_t j = 0; j < waves[c].size(); j++) worder[c][j] = (int32_t)j;
+    sort_by_key(worder[c], [&](int32_t j) { return (uint64_t)(INT32_MAX - waves[c][j].nsteps); });
+  }
 
   gdp_pair_hdr h;
   memset(&h, 0, sizeof h);
@@ -451,7 +483,8 @@ extern "C" int gdp_plan_tuned(const gdp_problem *probs_in, int32_t n, int32_t fl
   for (int c = 0; c < GDP_NCLS; c++) {
     h.nw[c] = (int32_t)waves[c].size();
     h.off[c] = off;
-    if (!waves[c].empty()) memcpy(base + off, waves[c].data(), waves[c].size() * sizeof(gdp_wave));
+    gdp_wave *wo = (gdp_wave *)(base + off);
+    for (size_t j = 0; j < waves[c].size(); j++) memcpy(wo + j, &waves[c][worder[c][j]], sizeof(gdp_wave));
     off += (int64_t)waves[c].size() * (int64_t)sizeof(gdp_wave);
   }
   for (int c = 0; c < GDP_NCLS; c++) {

@@ -10,6 +10,11 @@
 // fill and walk of the rest, and the choice of the list the reference
 // returns.
 //
+// Dynprog_end5_gap / Dynprog_end3_gap (dynprog_end.c:1293-1650,
+// 1923-2243): the windows chopped to max_glength, the two one-sided fills
+// per end (one for QUERYEND_NOGAPS), the best end cell of each end mode,
+// the walk, the mismatch filter and the leading-indel trim.
+//
 // The host builds the problems, the plan and the walk templates, and sizes
 // the output from the per-call lengths the device reports.
 #include <hip/hip_runtime.h>

@@ -1,15 +1,17 @@
 // gdp_serve.cpp -- continuous batching of alignment requests over a device
 // genome (the gdp_server_* functions of include/gmapdp.h).
 //
-// One server thread owns every HIP call.  It keeps up to `slots` batches in
-// flight, each on a stream of its own with its own pinned staging buffers
-// and device buffers; the direction words of all batches in flight share one
-// device budget handed out first-fit.  Whenever a slot is free and requests
-// wait, the thread takes the oldest requests that fit the slot and the free
-// budget, plans them, and queues on the slot's stream: one upload, the
-// genome-segment extraction, the fills, the tracebacks, one download and an
-// event.  Finished batches are turned into results as their events fire, so
-// new requests enter while earlier batches still run.
+// The server keeps up to `slots` batches in flight, each on a HIP stream of
+// its own with its own pinned staging buffers and device buffers, and each
+// driven by a thread of its own; the direction words of all batches in
+// flight share one device budget handed out first-fit.  Whenever its slot is
+// free and requests wait, a slot's thread takes the oldest requests that fit
+// the slot and the free budget, plans them, and queues on the slot's stream:
+// one upload, the genome-segment extraction, the fills, the tracebacks, one
+// download and an event.  It turns the finished batch into results when the
+// event fires.  Slots plan in parallel, so new batches are formed while
+// earlier ones still run, and the host's planning does not hold back the
+// device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -111,6 +113,7 @@ class Arena {
 };
 
 struct Slot {
+  std::thread th;  // the slot's thread: forms, launches and retires its batches
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr, ev0 = nullptr;  // end and start of the batch on the stream
   double t_launch = 0;
@@ -120,7 +123,11 @@ struct Slot {
   int buf = -1;  // result buffer of the batch in flight
   bool busy = false;
   std::vector<Req> reqs;
+  std::vector<gdp_problem> probs;
   std::vector<int64_t> pair_off;
+  // Ring offset of the oldest read the slot still has to copy out of the
+  // read ring (INT64_MAX: none); under gdp_server::mu.
+  int64_t copy_lo = INT64_MAX;
   int64_t dir_off = 0, dir_bytes = 0;
   int64_t off_counts = 0, off_pairs = 0, down_bytes = 0;
 };
@@ -139,10 +146,12 @@ struct gdp_server {
   uint32_t glen = 0;
   int device = 0;
   uint8_t *d_dirs = nullptr;
+  std::mutex amu;  // guards arena
   Arena arena;
+  int64_t arena_size = 0;
   std::vector<Slot> slots;
+  std::atomic<int> nbusy{0};  // slots with a batch in flight
   int64_t up_cap = 0, g_cap = 0, down_cap = 0, q_cap = 0;
-  std::vector<gdp_problem> probs;
 
   std::mutex mu;
   std::condition_variable cv_work;
@@ -163,16 +172,18 @@ struct gdp_server {
   int64_t dir_in_flight = 0;
 
   std::atomic<bool> stop{false};
-  // Set once a HIP call of the server thread failed: from then on no batch
+  // Set once a HIP call of a slot's thread failed: from then on no batch
   // starts, pending and newly submitted requests are refused with GDP_EHIP.
+  // Batches already in flight on other slots still complete.
   std::atomic<bool> failed{false};
   std::string fatal;  // what failed (under smu)
-  std::thread th;
 
-  void run();
-  bool launch(Slot &s);
+  enum { LAUNCHED, IDLE, BLOCKED };
+  void run_slot(Slot &s);
+  int launch(Slot &s);
   void retire(Slot &s);
-  void fail_all(const std::string &what);
+  void fail_all(Slot *own, const std::string &what);
+  void update_freed();
   void post(DoneBatch &&b) {
     {
       std::lock_guard<std::mutex> lk(dmu);
@@ -206,7 +217,20 @@ static gdp_align_result refused(const Req &q, double t) {
   return r;
 }
 
-void gdp_server::fail_all(const std::string &what) {
+// Read-ring bytes below the oldest read that a pending request or a slot
+// still copying its batch holds may be overwritten.  Called under mu.
+void gdp_server::update_freed() {
+  int64_t oldest = ring.next;
+  for (const auto &pq : pending)
+    if (!pq.empty()) oldest = std::min(oldest, pq.front().roff);
+  for (const Slot &o : slots) oldest = std::min(oldest, o.copy_lo);
+  ring.freed = oldest;
+}
+
+// Stops the server after a failure: refuses the calling slot's batch (own,
+// may be null) and every pending request.  Other slots' batches in flight
+// complete on their own threads.
+void gdp_server::fail_all(Slot *own, const std::string &what) {
   {
     std::lock_guard<std::mutex> lk(smu);
     if (fatal.empty()) fatal = what;
@@ -214,18 +238,19 @@ void gdp_server::fail_all(const std::string &what) {
   failed = true;
   DoneBatch b;
   const double t = now_s();
-  for (Slot &s : slots) {
-    if (!s.busy) continue;
-    for (Req &q : s.reqs) {
+  if (own && own->busy) {
+    for (Req &q : own->reqs) {
       q.status = (int8_t)-GDP_EHIP;
       b.res.push_back(refused(q, t));
     }
-    s.busy = false;
-    if (s.buf >= 0) {
-      std::lock_guard<std::mutex> lk(dmu);
-      rfree.push_back(s.buf);
-      s.buf = -1;
-    }
+    own->reqs.clear();
+    own->busy = false;
+    nbusy--;
+  }
+  if (own && own->buf >= 0) {
+    std::lock_guard<std::mutex> lk(dmu);
+    rfree.push_back(own->buf);
+    own->buf = -1;
   }
   std::deque<Req> rest;
   {
@@ -234,11 +259,13 @@ void gdp_server::fail_all(const std::string &what) {
       for (Req &q : pq) rest.push_back(q);
     pending[0].clear();
     pending[1].clear();
+    if (own) own->copy_lo = INT64_MAX;
     // no read waits in the ring any more: a submitter blocked on ring space
     // wakes up and is refused
     ring.freed = ring.next;
   }
   cv_space.notify_all();
+  cv_work.notify_all();
   for (Req &q : rest) {
     q.status = (int8_t)-GDP_EHIP;
     b.res.push_back(refused(q, t));
@@ -251,15 +278,24 @@ void gdp_server::fail_all(const std::string &what) {
 }
 
 // Takes the oldest pending requests that fit the slot and the free direction
-// budget, and queues their work on the slot's stream.  Returns false when
-// nothing could be started.
-bool gdp_server::launch(Slot &s) {
+// budget, and queues their work on the slot's stream.  Returns LAUNCHED when
+// it started a batch (or refused requests), IDLE when no request waits, and
+// BLOCKED when requests wait for a result buffer or for direction budget.
+int gdp_server::launch(Slot &s) {
   {
     std::lock_guard<std::mutex> lk(dmu);
-    if (rfree.empty()) return false;  // the consumer holds every result buffer
+    if (rfree.empty()) return BLOCKED;  // the consumer holds every result buffer
+    s.buf = rfree.back();
+    rfree.pop_back();
   }
+  auto give_back_buffer = [&] {
+    std::lock_guard<std::mutex> lk(dmu);
+    rfree.push_back(s.buf);
+    s.buf = -1;
+  };
   std::vector<Req> batch;
   int cls = 0;
+  bool more = false;
   {
     std::lock_guard<std::mutex> lk(mu);
     // a batch holds one length class: the one whose oldest request waited longest
@@ -277,20 +313,34 @@ bool gdp_server::launch(Slot &s) {
       batch.push_back(std::move(pq.front()));
       pq.pop_front();
     }
+    // the reads of the batch stay in the ring until they are copied out
+    if (!batch.empty()) s.copy_lo = batch.front().roff;
+    more = !pending[0].empty() || !pending[1].empty();
   }
-  if (batch.empty()) return false;
+  if (batch.empty()) {
+    give_back_buffer();
+    return IDLE;
+  }
+  if (more) cv_work.notify_one();  // another free slot can form the next batch meanwhile
+  auto release_reads = [&] {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      s.copy_lo = INT64_MAX;
+      update_freed();
+    }
+    cv_space.notify_all();
+  };
 
   // Plan; if the direction words do not fit the free budget, give back the
   // younger half and plan again.
-  const int64_t free_bytes = arena.largest();
   int n = (int)batch.size();
   int64_t dw = 0, ss = 0;
   for (;;) {
-    probs.resize(n);
+    s.probs.resize(n);
     int64_t qpos = 0, gpos = 0;
     for (int i = 0; i < n; i++) {
       const Req &q = batch[i];
-      gdp_problem &p = probs[i];
+      gdp_problem &p = s.probs[i];
       memset(&p, 0, sizeof p);
       p.qoff = qpos;
       p.goff = p.galtoff = gpos;
@@ -310,7 +360,7 @@ bool gdp_server::launch(Slot &s) {
     const gdp_plan_params *pp =
         (cfg.plan.g16_max_span || cfg.plan.g64_min_rows || cfg.plan.mix_span || cfg.plan.mix_order) ? &cfg.plan
                                                                                                       : nullptr;
-    if (gdp_plan_tuned(probs.data(), n, 0, pp, s.h_up, &dw, &ss) != GDP_OK) {
+    if (gdp_plan_tuned(s.probs.data(), n, 0, pp, s.h_up, &dw, &ss) != GDP_OK) {
       // Requests and the plan parameters were validated before; a planning
       // failure is a bug: refuse this batch and stop like a HIP failure.
       const std::string why = std::string("gdp_server: planning failed: ") + gdp_last_error();
@@ -325,30 +375,52 @@ bool gdp_server::launch(Slot &s) {
         st.completed += (int64_t)b.res.size();
       }
       post(std::move(b));
-      fail_all(why);
-      return true;
+      release_reads();
+      fail_all(&s, why);
+      return LAUNCHED;
     }
     const int64_t need = align_up(dw * 4, 256);
-    if (need <= free_bytes) break;
+    int64_t off = -1;
+    bool empty_arena = false;
+    {
+      std::lock_guard<std::mutex> lk(amu);
+      off = arena.alloc(need);
+      if (off < 0) empty_arena = arena.largest() == arena_size;
+    }
+    if (off >= 0) {
+      s.dir_off = off;
+      s.dir_bytes = need;
+      break;
+    }
     if (n == 1) {
-      bool idle = true;
-      for (const Slot &o : slots) idle = idle && !o.busy;
-      if (!idle) {  // wait for budget
-        std::lock_guard<std::mutex> lk(mu);
-        pending[cls].push_front(std::move(batch[0]));
-        return false;
+      if (!empty_arena) {  // wait for budget
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          pending[cls].push_front(std::move(batch[0]));
+          s.copy_lo = INT64_MAX;
+          update_freed();
+        }
+        give_back_buffer();
+        return BLOCKED;
       }
       DoneBatch b;  // one request larger than the whole budget
       batch[0].status = (int8_t)-GDP_ENOMEM;
       b.res.push_back(refused(batch[0], now_s()));
+      {
+        std::lock_guard<std::mutex> lk(smu);
+        st.completed += 1;
+      }
       post(std::move(b));
-      return true;
+      release_reads();
+      give_back_buffer();
+      return LAUNCHED;
     }
     const int keep = n / 2;
     {
       std::lock_guard<std::mutex> lk(mu);
       for (int i = n - 1; i >= keep; i--) pending[cls].push_front(std::move(batch[i]));
     }
+    cv_work.notify_one();
     batch.resize(keep);
     n = keep;
   }
@@ -398,31 +470,16 @@ bool gdp_server::launch(Slot &s) {
     ppos += t.max_pairs;
   }
   memset(qh + qpos, 0, 16);
-  {
-    // the launched reads are the oldest; their ring space is free again
-    std::lock_guard<std::mutex> lk(mu);
-    int64_t oldest = ring.next;
-    for (const auto &pq : pending)
-      if (!pq.empty()) oldest = std::min(oldest, pq.front().roff);
-    ring.freed = oldest;
-  }
-  cv_space.notify_all();
+  release_reads();
   const int64_t up_bytes = off_q + qpos + 16;
   const uint32_t kernels = gdp_plan_kernels(s.h_up, n);
 
-  s.dir_bytes = align_up(dw * 4, 256);
-  s.dir_off = arena.alloc(s.dir_bytes);
   s.off_counts = align_up((int64_t)n * 4, 16);
   s.off_pairs = align_up(s.off_counts + (int64_t)n * 24, 16);
   s.down_bytes = s.off_pairs + ppos * (int64_t)sizeof(gdp_pair);
-
-  {
-    std::lock_guard<std::mutex> lk(dmu);
-    s.buf = rfree.back();
-    rfree.pop_back();
-  }
   s.reqs = std::move(batch);
   s.busy = true;
+  const int inflight = ++nbusy;
   uint8_t *d_descs = s.d_up;
   uint32_t *dirs = (uint32_t *)(d_dirs + s.dir_off);
   s.t_launch = now_s();
@@ -448,19 +505,17 @@ bool gdp_server::launch(Slot &s) {
     rc = e == hipSuccess ? GDP_OK : GDP_EHIP;
   }
   if (rc != GDP_OK) {
-    fail_all(std::string("gdp_server launch: ") + hipGetErrorString(e != hipSuccess ? e : hipGetLastError()));
-    return true;
+    fail_all(&s, std::string("gdp_server launch: ") + hipGetErrorString(e != hipSuccess ? e : hipGetLastError()));
+    return LAUNCHED;
   }
   {
     std::lock_guard<std::mutex> lk(smu);
     st.batches++;
     dir_in_flight += s.dir_bytes;
     st.dir_bytes_peak = std::max(st.dir_bytes_peak, dir_in_flight);
-    int inflight = 0;
-    for (const Slot &o : slots) inflight += o.busy ? 1 : 0;
     st.inflight_peak = std::max(st.inflight_peak, inflight);
   }
-  return true;
+  return LAUNCHED;
 }
 
 void gdp_server::retire(Slot &s) {
@@ -486,7 +541,10 @@ void gdp_server::retire(Slot &s) {
     r.t_submit = q.t_submit;
     r.t_done = t;
   }
-  arena.release(s.dir_off, s.dir_bytes);
+  {
+    std::lock_guard<std::mutex> lk(amu);
+    arena.release(s.dir_off, s.dir_bytes);
+  }
   float dev_ms = 0;
   if (hipEventElapsedTime(&dev_ms, s.ev0, s.ev) != hipSuccess) dev_ms = 0;
   {
@@ -499,50 +557,49 @@ void gdp_server::retire(Slot &s) {
   s.reqs.clear();
   s.buf = -1;
   s.busy = false;
+  nbusy--;
   post(std::move(b));
+  cv_work.notify_all();  // budget came free: a slot waiting for it may go on
 }
 
-void gdp_server::run() {
+void gdp_server::run_slot(Slot &s) {
   (void)hipSetDevice(device);
   for (;;) {
-    bool progressed = false;
-    for (Slot &s : slots) {
-      if (!s.busy) continue;
+    if (s.busy) {
       const hipError_t e = hipEventQuery(s.ev);
       if (e == hipSuccess) {
         const double t0 = now_s();
         retire(s);
         std::lock_guard<std::mutex> lk(smu);
         st.retire_seconds += now_s() - t0;
-        progressed = true;
       } else if (e != hipErrorNotReady) {
-        fail_all(std::string("gdp_server batch: ") + hipGetErrorString(e));
-        progressed = true;
+        fail_all(&s, std::string("gdp_server batch: ") + hipGetErrorString(e));
+      } else {
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+      }
+      continue;
+    }
+    if (stop.load()) break;
+    int r = IDLE;
+    if (!failed.load()) {
+      const double t0 = now_s();
+      r = launch(s);
+      if (r == LAUNCHED) {
+        std::lock_guard<std::mutex> lk(smu);
+        st.prep_seconds += now_s() - t0;
+        continue;
       }
     }
-    bool any_busy = false;
-    for (Slot &s : slots) {
-      if (!s.busy && !failed.load() && !stop.load()) {
-        const double t0 = now_s();
-        if (launch(s)) {
-          progressed = true;
-          std::lock_guard<std::mutex> lk(smu);
-          st.prep_seconds += now_s() - t0;
-        }
-      }
-      any_busy = any_busy || s.busy;
-    }
-    if (stop.load() && !any_busy) break;
-    if (progressed) continue;
-    if (any_busy) {
-      std::this_thread::sleep_for(std::chrono::microseconds(20));
-    } else {
-      std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait_for(lk, std::chrono::milliseconds(5),
-                       [&] { return !pending[0].empty() || !pending[1].empty() || stop.load(); });
-    }
+    std::unique_lock<std::mutex> lk(mu);
+    if (r == BLOCKED)  // woken when a batch retires or the consumer returns buffers
+      cv_work.wait_for(lk, std::chrono::microseconds(200));
+    else
+      cv_work.wait_for(lk, std::chrono::milliseconds(5), [&] {
+        return stop.load() || (!failed.load() && (!pending[0].empty() || !pending[1].empty()));
+      });
   }
 }
+
 
 extern "C" double gdp_server_now(void) { return now_s(); }
 
@@ -579,7 +636,8 @@ extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const voi
   s->up_cap = align_up(gdp_plan_bytes(cfg->max_batch), 16) + align_up(nb * (int64_t)sizeof(gdp_segment), 16) +
               nb * (int64_t)sizeof(gdp_tb_request) + s->q_cap;
   s->down_cap = align_up(nb * 4, 16) + align_up(nb * 24, 16) + cfg->pair_capacity * (int64_t)sizeof(gdp_pair);
-  s->arena.init(align_up(cfg->dir_budget_bytes, 256) - 256);
+  s->arena_size = align_up(cfg->dir_budget_bytes, 256) - 256;
+  s->arena.init(s->arena_size);
   s->ring.buf.resize((size_t)std::max<int64_t>(64 << 20, 4 * nb * (int64_t)cfg->max_rl));
   hipError_t e = hipMalloc((void **)&s->d_dirs, (size_t)align_up(cfg->dir_budget_bytes, 256) + 64);
   s->slots.resize(cfg->slots);
@@ -613,7 +671,10 @@ extern "C" gdp_server *gdp_server_create(const gdp_server_config *cfg, const voi
     gdp_server_destroy(s);
     return nullptr;
   }
-  s->th = std::thread([s] { s->run(); });
+  for (Slot &sl : s->slots) {
+    Slot *p = &sl;
+    sl.th = std::thread([s, p] { s->run_slot(*p); });
+  }
   return s;
 }
 
@@ -624,61 +685,80 @@ extern "C" int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, i
   }
   const double t = now_s();
   DoneBatch b;
-  int32_t accepted = 0;
-  std::unique_lock<std::mutex> lk(s->mu);
-  for (int32_t i = 0; i < n; i++) {
-    const gdp_align_request &r = reqs[i];
-    Req q;
-    q.r = r;
-    q.r.read = nullptr;
-    q.t_submit = t;
-    q.status = 0;
-    q.roff = 0;
-    int32_t lb = 0, ub = 0;
-    if (s->failed.load()) {
-      q.status = (int8_t)-GDP_EHIP;  // the server stopped on an error
-    } else if (r.rl <= 0 || r.gl <= 0 || r.rl > s->cfg.max_rl || r.gl > s->cfg.max_gl) {
-      q.status = (int8_t)-GDP_ETOOBIG;
-    } else {
-      gdp_compute_bands(&lb, &ub, r.rl, r.gl, r.extraband, r.widebandp);
-      if (lb < 0 || ub < 0 || lb + ub > GDP_MAX_SPAN || r.mismatchtype < 0 || r.mismatchtype > 3 || r.open > 0 ||
-          r.extend > 0 || r.read == nullptr)
-        q.status = (int8_t)-GDP_EINVAL;
+  // Requests are checked and banded outside the lock, then their reads are
+  // placed in the ring a chunk at a time, so the slots' threads can take
+  // requests while a large submission is copied in.
+  constexpr int32_t CHUNK = 1024;
+  std::vector<Req> qs;
+  std::vector<const char *> rd;
+  qs.reserve((size_t)std::min(n, CHUNK));
+  rd.reserve((size_t)std::min(n, CHUNK));
+  for (int32_t i0 = 0; i0 < n; i0 += CHUNK) {
+    const int32_t i1 = std::min(n, i0 + CHUNK);
+    qs.clear();
+    rd.clear();
+    for (int32_t i = i0; i < i1; i++) {
+      const gdp_align_request &r = reqs[i];
+      Req q;
+      q.r = r;
+      q.r.read = nullptr;
+      q.t_submit = t;
+      q.status = 0;
+      q.roff = 0;
+      int32_t lb = 0, ub = 0;
+      if (r.rl <= 0 || r.gl <= 0 || r.rl > s->cfg.max_rl || r.gl > s->cfg.max_gl) {
+        q.status = (int8_t)-GDP_ETOOBIG;
+      } else {
+        gdp_compute_bands(&lb, &ub, r.rl, r.gl, r.extraband, r.widebandp);
+        if (lb < 0 || ub < 0 || lb + ub > GDP_MAX_SPAN || r.mismatchtype < 0 || r.mismatchtype > 3 || r.open > 0 ||
+            r.extend > 0 || r.read == nullptr)
+          q.status = (int8_t)-GDP_EINVAL;
+      }
+      if (q.status) {
+        b.res.push_back(refused(q, t));
+        continue;
+      }
+      q.lband = (int16_t)lb;
+      q.uband = (int16_t)ub;
+      // Dynprog_single_gap's kernel choice (dynprog_single.c:595).
+      const int u8 = gdp_use8p_size(r.mismatchtype);
+      q.width = (r.rl < u8 && r.gl < u8) ? 8 : 16;
+      qs.push_back(q);
+      rd.push_back(r.read);
     }
-    if (q.status) {
-      b.res.push_back(refused(q, t));
-      continue;
+    int32_t placed = 0;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      for (size_t j = 0; j < qs.size(); j++) {
+        Req &q = qs[j];
+        if (!s->failed.load()) {
+          int64_t v = s->ring.place(q.r.rl);
+          while (!s->ring.fits(v, q.r.rl) && !s->failed.load()) {  // the ring is full of waiting reads
+            s->cv_work.notify_all();
+            s->cv_space.wait_for(lk, std::chrono::milliseconds(1));
+            v = s->ring.place(q.r.rl);
+          }
+          if (!s->failed.load()) {
+            memcpy(s->ring.at(v), rd[j], (size_t)q.r.rl);
+            s->ring.next = v + q.r.rl;
+            q.roff = v;
+            s->pending[s->cfg.long_rows > 0 && q.r.rl >= s->cfg.long_rows ? 1 : 0].push_back(q);
+            placed++;
+            continue;
+          }
+        }
+        q.status = (int8_t)-GDP_EHIP;  // the server stopped on an error
+        b.res.push_back(refused(q, t));
+      }
     }
-    q.lband = (int16_t)lb;
-    q.uband = (int16_t)ub;
-    // Dynprog_single_gap's kernel choice (dynprog_single.c:595).
-    const int u8 = gdp_use8p_size(r.mismatchtype);
-    q.width = (r.rl < u8 && r.gl < u8) ? 8 : 16;
-    int64_t v = s->ring.place(r.rl);
-    while (!s->ring.fits(v, r.rl) && !s->failed.load()) {  // the ring is full of waiting reads
-      s->cv_work.notify_one();
-      s->cv_space.wait_for(lk, std::chrono::milliseconds(1));
-      v = s->ring.place(r.rl);
-    }
-    if (s->failed.load()) {  // failed while waiting for ring space
-      q.status = (int8_t)-GDP_EHIP;
-      b.res.push_back(refused(q, t));
-      continue;
-    }
-    memcpy(s->ring.at(v), r.read, (size_t)r.rl);
-    s->ring.next = v + r.rl;
-    q.roff = v;
-    s->pending[s->cfg.long_rows > 0 && r.rl >= s->cfg.long_rows ? 1 : 0].push_back(q);
-    accepted++;
+    if (placed) s->cv_work.notify_one();
   }
-  lk.unlock();
   {
     std::lock_guard<std::mutex> lk2(s->smu);
     s->st.submitted += n;
     s->st.completed += (int64_t)b.res.size();
   }
   if (!b.res.empty()) s->post(std::move(b));
-  if (accepted) s->cv_work.notify_one();
   return s->failed.load() ? GDP_EHIP : GDP_OK;
 }
 
@@ -688,8 +768,11 @@ extern "C" int32_t gdp_server_poll(gdp_server *s, gdp_align_result *out, int32_t
   {
     std::unique_lock<std::mutex> lk(s->dmu);
     // results of the previous call are no longer referenced
-    for (int k : s->handed) s->rfree.push_back(k);
-    s->handed.clear();
+    if (!s->handed.empty()) {
+      for (int k : s->handed) s->rfree.push_back(k);
+      s->handed.clear();
+      s->cv_work.notify_all();  // a slot waiting for a result buffer may go on
+    }
     if (s->done.empty() && wait_us > 0)
       s->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [&] { return !s->done.empty(); });
     int64_t have = 0;
@@ -735,7 +818,8 @@ extern "C" void gdp_server_destroy(gdp_server *s) {
   if (!s) return;
   s->stop = true;
   s->cv_work.notify_all();
-  if (s->th.joinable()) s->th.join();
+  for (Slot &sl : s->slots)
+    if (sl.th.joinable()) sl.th.join();
   for (Slot &sl : s->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.h_up) (void)hipHostFree(sl.h_up);

@@ -10,7 +10,8 @@ best: the whole sample's fill time for throughput (the sample large enough
 that the slowest wave does not decide it), or the fill time of a batch of
 the longest reads for latency (the time a request waits on the slowest
 problem of its batch).  A second tuner picks the request server's batch
-slots, batch size and length split by measured saturation and p99.
+slots, batch size, length split and wave mixing by measured saturation and
+p99.
 """
 import numpy as np
 
@@ -67,23 +68,26 @@ def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", rep
     return best, table
 
 
-# (slots, max_batch, long_rows) candidates of the request server: batches in
-# flight, requests per batch, and the query length from which reads are
-# batched apart.  At half load a request waits for a batch slot and then for
-# its batch, whose time the longest reads set.  Measured on the
-# configuration-5 mix (3 runs each at half load) once each batch's transfers
-# ran as kernels on its own stream (before, a copy engine shared by the
-# streams ran one batch at a time): (3, 4096, 300) 2.54e6 requests/s at p99
-# 3.9 ms, (3, 8192, 300) 2.50e6-3.30e6 at 4.2-4.4 ms, (2, 2048, 300) 1.46e6
-# at 4.5 ms.  With a hardware queue per slot (GPU_MAX_HW_QUEUES 16) more slots
-# overlap more batches: (4, 8192, 300) 3.42e6 at 3.8 ms, (6, 8192, 300)
-# 3.54e6 at 3.6 ms, (8, 8192, 300) 3.49e6 at 3.4 ms, (8, 4096, 300) 3.38e6 at
-# 3.3 ms; with HIP's default of 4 queues, 4 slots stayed above 5 ms.
-SERVER_KNOBS = ((8, 8192, 300), (6, 8192, 300), (4, 8192, 300), (3, 8192, 300))
+# (slots, max_batch, long_rows, mix_span, mix_order) candidates of the
+# request server: batches in flight, requests per batch, the query length
+# from which reads are batched apart, and how the requests of a batch share
+# paired waves (gdp_plan_params).  At half load a request waits for a batch
+# slot and then for its batch, whose time the longest reads set.  Measured
+# on the configuration-5 mix (saturation, then 3 runs at half of it) with a
+# hardware queue per slot (GPU_MAX_HW_QUEUES 16) and a thread per slot
+# forming its batches (with one thread for all slots, planning held the
+# server at 3.3e6 requests/s): waves of one band only, (6, 8192, 300) 4.39e6
+# requests/s at p99 4.2-4.3 ms; mixed waves binned by 4 columns, (6, 8192,
+# 300) 4.86e6 at 4.8 ms, (12, 8192, 300) 5.29e6-5.50e6 at 4.0-4.3 ms,
+# (15, 8192, 300) 5.53e6 at 3.7-3.9 ms, (12, 16384, 300) 6.01e6 at 4.4-4.5
+# ms, (15, 4096, 300) 5.03e6 at 3.6-3.7 ms; binned by 8 columns,
+# (15, 8192, 300) 5.81e6 at 3.9-4.0 ms.
+SERVER_KNOBS = ((15, 8192, 300, 8, 1), (15, 8192, 300, 4, 1), (12, 16384, 300, 4, 1), (8, 8192, 300, 4, 1))
 
 
 def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, seconds=1.5, margin=0.9, **server_kw):
-    """Picks (slots, max_batch, long_rows) for the request server by measurement:
+    """Picks (slots, max_batch, long_rows, mix_span, mix_order) for the
+    request server by measurement, with the lane groups plan[:2]:
     for each candidate, the saturation throughput (all requests at once) and
     the median p99 latency of three runs of Poisson arrivals at half of it.
     Keeps the candidate with the highest saturation whose p99 is within
@@ -92,9 +96,9 @@ def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, secon
     Returns (knob, table of (knob, saturation req/s, p99 ms))."""
     from .serve import Server
     table = []
-    for slots, mb, lr in knobs:
-        with Server(genome, max_batch=mb, slots=slots, pair_capacity=mb * 700, plan=plan, long_rows=lr,
-                    **server_kw) as srv:
+    for slots, mb, lr, mix, order in knobs:
+        with Server(genome, max_batch=mb, slots=slots, pair_capacity=mb * 700, plan=tuple(plan[:2]) + (mix, order),
+                    long_rows=lr, **server_kw) as srv:
             srv.drive(reqs[:min(len(reqs), 20000)], 0.0)
             lat, el, ok = srv.drive(reqs, 0.0)
             sat = len(reqs) / el
@@ -103,7 +107,7 @@ def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, secon
             for rep in range(3):  # the median of three runs
                 lat, el, ok = srv.drive(reqs[:m], 0.5 * sat, seed=11 + rep)
                 p99.append(float(np.percentile(lat, 99)) * 1e3)
-            table.append(((slots, mb, lr), sat, float(np.median(p99))))
+            table.append(((slots, mb, lr, mix, order), sat, float(np.median(p99))))
     good = [t for t in table if t[2] <= margin * target_p99_ms] or [t for t in table if t[2] <= target_p99_ms]
     best = max(good, key=lambda t: t[1]) if good else min(table, key=lambda t: t[2])
     return best[0], table

@@ -757,9 +757,10 @@ const gdp_pair *gdp_eg_pairs(const gdp_eg_batch *b);
    Dynprog_single_gap, single_gap_simple for gl == rl and the homopolymer
    mode, are not part of the request.)
 
-   A server thread takes submitted requests in arrival order into batches as
-   batch slots free up and as the direction words of the batch fit the free
-   part of a fixed device budget; every batch runs on its own stream, so
+   Each batch slot has a thread of its own, which takes submitted requests in
+   arrival order into a batch whenever its slot is free, as far as the
+   direction words of the batch fit the free part of a fixed device budget,
+   plans it, and runs it on the slot's stream; slots plan in parallel and
    batches overlap on the device, and results are returned as their batch
    completes.  A request's result does not depend on the batch it ran in. */
 typedef struct gdp_server gdp_server;
@@ -829,8 +830,8 @@ typedef struct gdp_server_stats {
   int64_t dir_bytes_peak; /* most direction bytes in flight at once */
   int32_t inflight_peak;  /* most batches in flight at once */
   int32_t pad;
-  double prep_seconds;    /* server thread: forming, planning and launching batches */
-  double retire_seconds;  /* server thread: turning finished batches into results */
+  double prep_seconds;    /* slot threads, summed: forming, planning and launching batches */
+  double retire_seconds;  /* slot threads, summed: turning finished batches into results */
   double batch_seconds;   /* sum over batches of launch-to-results wall time */
   double device_seconds;  /* sum over batches of stream time from upload to download */
 } gdp_server_stats;
@@ -860,7 +861,7 @@ int32_t gdp_server_poll(gdp_server *s, gdp_align_result *out, int32_t max, int32
    the server has stopped on a failure. */
 int gdp_server_get_stats(gdp_server *s, gdp_server_stats *st);
 
-/* Finishes the batches in flight and stops the server thread. */
+/* Finishes the batches in flight and stops the slots' threads. */
 void gdp_server_destroy(gdp_server *s);
 
 /* Seconds on the server's monotonic clock. */

@@ -156,16 +156,18 @@ def _check_result(O, genome, nbases, req, read, res, pairs):
     assert (res["counts"][1:] == co).all(), tag
 
 
-@pytest.mark.parametrize("long_rows", [0, 300])
-def test_server_matches_oracle(oracle, small_genome, long_rows):
-    """long_rows > 0 batches long and short reads apart (two queues)."""
+@pytest.mark.parametrize("long_rows,plan", [(0, (112, 128)), (300, (112, 128)), (300, (240, 256, 8, 1))])
+def test_server_matches_oracle(oracle, small_genome, long_rows, plan):
+    """long_rows > 0 batches long and short reads apart (two queues); a
+    plan with mix_span > 0 sweeps requests of neighbouring bands in shared
+    waves."""
     from gmapdp import serve
     gc, nbases, genome = small_genome
     reqs, reads = serve.gen_requests(1005, 0, 300, gc, nbases)
     reqs["genestrand"] = np.arange(300) % 3
     reqs["roffset"] = np.arange(300) % 7
     dg = serve.DeviceGenome(gc, nbases)
-    with serve.Server(dg, max_batch=64, slots=3, long_rows=long_rows, plan=(112, 128)) as srv:
+    with serve.Server(dg, max_batch=64, slots=3, long_rows=long_rows, plan=plan) as srv:
         srv.submit(reqs)
         out = srv.collect(len(reqs))
         st = srv.stats()
@@ -175,15 +177,17 @@ def test_server_matches_oracle(oracle, small_genome, long_rows):
         _check_result(oracle, genome, nbases, r, _read(reads, r), res, pairs)
 
 
-def test_server_result_alone_equals_result_in_mix(oracle, small_genome):
+@pytest.mark.parametrize("slots,plan", [(3, None), (12, (240, 256, 8, 1))])
+def test_server_result_alone_equals_result_in_mix(oracle, small_genome, slots, plan):
     """Each request's result is the same served alone and served inside a
-    stream of other requests arriving while batches run."""
+    stream of other requests arriving while batches run (slots form their
+    batches on threads of their own)."""
     from gmapdp import serve
     gc, nbases, genome = small_genome
     reqs, reads = serve.gen_requests(77, 0, 2000, gc, nbases)
     dg = serve.DeviceGenome(gc, nbases)
     probe = [3, 50, 400, 1234, 1999]
-    with serve.Server(dg, max_batch=256, slots=3) as srv:
+    with serve.Server(dg, max_batch=256, slots=slots, plan=plan) as srv:
         alone = {}
         for k in probe:
             srv.submit(reqs[k:k + 1])
