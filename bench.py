@@ -175,7 +175,7 @@ def callers_bench(args):
         sg[f] = reqs[f]
     sg["defect_rate"] = np.where(np.arange(n) % 2 == 0, 0.001, 0.02)
     b = callers.SingleGapBatch()
-    dt, (res, pairs) = timed(lambda: b.run(sg, qbuf, dg, max_rlength=1000, max_glength=1500))
+    dt, (res, pairs) = timed(lambda: b.run(sg, qbuf, dg, max_rlength=1000, max_glength=1500, copy=False))
     out["single_gap"] = {"calls": n, "seconds": round(dt, 4), "calls_per_s": round(n / dt, 1),
                          "pairs": int(len(pairs)), "simple": int(res["simple"].sum()),
                          "sample": "the serving mix (rl log-uniform 20-1000) on a 1e8-base device genome, "
@@ -190,7 +190,7 @@ def callers_bench(args):
     eg["endalign"] = np.arange(n) % 4
     eg["defect_rate"] = 0.001
     b2 = callers.EndGapBatch()
-    dt, (res, pairs) = timed(lambda: b2.run(True, eg, qbuf, dg, max_rlength=1000, max_glength=1500))
+    dt, (res, pairs) = timed(lambda: b2.run(True, eg, qbuf, dg, max_rlength=1000, max_glength=1500, copy=False))
     out["end5_gap"] = {"calls": n, "seconds": round(dt, 4), "calls_per_s": round(n / dt, 1), "pairs": int(len(pairs)),
                        "sample": "read ends of the serving mix against windows 20 bases longer, the four end "
                                  "modes in turn"}

@@ -437,7 +437,11 @@ extern "C" int gdp_sg_run(gdp_sg_batch *b, const gdp_sg_call *calls, int32_t n, 
   }
   b->hdescs.resize((size_t)gdp_plan_bytes(m));
   int64_t dw = 0, ss = 0;
-  int rc = gdp_plan_ex(b->probs.data(), m, 0, b->hdescs.data(), &dw, &ss);
+  // Single gaps rarely share a band: mixed paired waves, problems binned by
+  // band in steps of 4 columns and by rows (what the throughput autotuner
+  // picks for this length mix).
+  const gdp_plan_params mixed = {240, GDP_MAX_LEN + 1, 4, 1};
+  int rc = gdp_plan_tuned(b->probs.data(), m, 0, &mixed, b->hdescs.data(), &dw, &ss);
   if (rc != GDP_OK) return rc;
   const uint32_t kernels = gdp_plan_kernels(b->hdescs.data(), m);
   const int64_t used = gdp_plan_used_bytes(b->hdescs.data(), m);

@@ -27,6 +27,14 @@ from .lib import EG_CALL_DTYPE, EG_RESULT_DTYPE, PAIR_DTYPE, SG_CALL_DTYPE, SG_R
 _batch = None
 
 
+def _view(ptr, n, dtype, copy=True):
+    """n records of dtype at a library pointer, copied or as a view."""
+    if n == 0:
+        return np.zeros(0, dtype)
+    v = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_uint8)), (n * dtype.itemsize,)).view(dtype)
+    return v.copy() if copy else v
+
+
 class SingleGapBatch:
     """A gdp_sg_batch: device and host buffers reused across runs."""
 
@@ -34,11 +42,14 @@ class SingleGapBatch:
         self.L = init()
         self.h = self.L.gdp_sg_create()
 
-    def run(self, calls, qbuf, genome, max_rlength=500, max_glength=2000, user_penalties=None, stream=None):
+    def run(self, calls, qbuf, genome, max_rlength=500, max_glength=2000, user_penalties=None, stream=None,
+            copy=True):
         """calls: SG_CALL_DTYPE rows; qbuf: the reads; genome: a
         serve.DeviceGenome.  user_penalties: (open, extend) for GMAP's
         --indel-open / --indel-extend, None for the defect-rate penalties.
-        Returns (results SG_RESULT_DTYPE, pairs PAIR_DTYPE)."""
+        Returns (results SG_RESULT_DTYPE, pairs PAIR_DTYPE); with
+        copy=False the pairs are a view of the batch's pinned buffer, valid
+        until its next run."""
         L = self.L
         calls = np.ascontiguousarray(calls, SG_CALL_DTYPE)
         q = qbuf if isinstance(qbuf, np.ndarray) else np.frombuffer(bytes(qbuf), np.uint8)
@@ -47,14 +58,8 @@ class SingleGapBatch:
         _check(L, L.gdp_sg_run(self.h, calls.ctypes.data, len(calls), q.ctypes.data, len(q), genome.ptr, genome.fmt,
                                genome.nbases, max_rlength, max_glength, 1 if user_penalties is not None else 0,
                                uo, ue, stream), "gdp_sg_run")
-        n = len(calls)
-        res = np.ctypeslib.as_array(ctypes.cast(L.gdp_sg_results(self.h), ctypes.POINTER(ctypes.c_uint8)),
-                                    (n * SG_RESULT_DTYPE.itemsize,)).view(SG_RESULT_DTYPE).copy() if n else \
-            np.zeros(0, SG_RESULT_DTYPE)
-        m = int(L.gdp_sg_npairs(self.h))
-        pairs = np.ctypeslib.as_array(ctypes.cast(L.gdp_sg_pairs(self.h), ctypes.POINTER(ctypes.c_uint8)),
-                                      (m * PAIR_DTYPE.itemsize,)).view(PAIR_DTYPE).copy() if m else \
-            np.zeros(0, PAIR_DTYPE)
+        res = _view(L.gdp_sg_results(self.h), len(calls), SG_RESULT_DTYPE)
+        pairs = _view(L.gdp_sg_pairs(self.h), int(L.gdp_sg_npairs(self.h)), PAIR_DTYPE, copy)
         return res, pairs
 
     def close(self):
@@ -86,13 +91,6 @@ def single_gap(calls, qbuf, genome, max_rlength=500, max_glength=2000, user_pena
 _ebatch = None
 
 
-def _view(ptr, n, dtype):
-    if n == 0:
-        return np.zeros(0, dtype)
-    return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_uint8)),
-                                 (n * dtype.itemsize,)).view(dtype).copy()
-
-
 class EndGapBatch:
     """A gdp_eg_batch: device and host buffers reused across runs."""
 
@@ -101,9 +99,9 @@ class EndGapBatch:
         self.h = self.L.gdp_eg_create()
 
     def run(self, five, calls, qbuf, genome, max_rlength=500, max_glength=2000, user_penalties=None,
-            require_pos_score=False, stream=None):
+            require_pos_score=False, stream=None, copy=True):
         """calls: EG_CALL_DTYPE rows; returns (results EG_RESULT_DTYPE,
-        pairs PAIR_DTYPE)."""
+        pairs PAIR_DTYPE); copy as in SingleGapBatch.run."""
         L = self.L
         calls = np.ascontiguousarray(calls, EG_CALL_DTYPE)
         q = qbuf if isinstance(qbuf, np.ndarray) else np.frombuffer(bytes(qbuf), np.uint8)
@@ -114,7 +112,7 @@ class EndGapBatch:
                                1 if user_penalties is not None else 0, uo, ue, 1 if require_pos_score else 0,
                                stream), "gdp_eg_run")
         res = _view(L.gdp_eg_results(self.h), len(calls), EG_RESULT_DTYPE)
-        pairs = _view(L.gdp_eg_pairs(self.h), int(L.gdp_eg_npairs(self.h)), PAIR_DTYPE)
+        pairs = _view(L.gdp_eg_pairs(self.h), int(L.gdp_eg_npairs(self.h)), PAIR_DTYPE, copy)
         return res, pairs
 
     def close(self):
