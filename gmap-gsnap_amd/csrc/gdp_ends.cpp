@@ -19,6 +19,7 @@
 // the output from the per-call lengths the device reports.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -589,20 +590,25 @@ extern "C" int gdp_eg_run(gdp_eg_batch *b, int32_t five, const gdp_eg_call *call
 
   const int use8 = gdp_use8p_size(kEndq);
   b->probs.clear();
-  b->hq.clear();
+  int64_t qtotal = 0;
+  for (int j = 0; j < m; j++) qtotal += rlx[b->live[j]];
+  b->hq.resize((size_t)qtotal);
   b->hsegs.assign((size_t)m, gdp_segment{});
   b->hreqs.assign((size_t)m, gdp_tb_request{});
   b->hflags.assign((size_t)m, 0);
   b->hul.clear();
-  int64_t gpos = 0, pcap = 0;
+  int64_t gpos = 0, pcap = 0, qpos = 0;
   for (int j = 0; j < m; j++) {
     const int32_t k = b->live[j];
     const gdp_eg_call &c = calls[k];
     const int rl = rlx[k], gl = glx[k];
     const bool nogaps = c.endalign == 2;
-    const int64_t qpos = (int64_t)b->hq.size();
     // the read in sweep order: the 5' caller reads rev_rsequence[1 - r]
-    for (int i = 0; i < rl; i++) b->hq.push_back(five ? qbuf[c.qoff + c.rl - 1 - i] : qbuf[c.qoff + i]);
+    char *qd = b->hq.data() + qpos;
+    if (five)
+      std::reverse_copy(qbuf + c.qoff + c.rl - rl, qbuf + c.qoff + c.rl, qd);
+    else
+      memcpy(qd, qbuf + c.qoff, (size_t)rl);
     gdp_segment &sg = b->hsegs[j];
     if (five) {
       if (c.watsonp) {
@@ -672,6 +678,7 @@ extern "C" int gdp_eg_run(gdp_eg_batch *b, int32_t five, const gdp_eg_call *call
     }
     b->hflags[j] = c.endalign | (require_pos_score ? 4 : 0);
     gpos += gl;
+    qpos += rl;
     pcap += q.max_pairs;
   }
   const int np = (int)b->probs.size();
