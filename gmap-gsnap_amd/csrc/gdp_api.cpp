@@ -770,8 +770,14 @@ extern "C" int gdp_download_results(const gdp_tb_request *d_reqs, int32_t nreq, 
     return fail(GDP_EINVAL, "gdp_download_results: null or negative argument");
   if (nreq == 0) return GDP_OK;
   void *dc = nullptr, *dp = nullptr;
-  if (hipHostGetDevicePointer(&dc, h_counts, 0) != hipSuccess || hipHostGetDevicePointer(&dp, h_pairs, 0) != hipSuccess)
+  if (hipHostGetDevicePointer(&dc, h_counts, 0) != hipSuccess ||
+      hipHostGetDevicePointer(&dp, h_pairs, 0) != hipSuccess) {
+    // The refused pointer leaves its error as the thread's last HIP error,
+    // which every later launch check (hipGetLastError) would report as its
+    // own failure: consume it here, where it is answered.
+    (void)hipGetLastError();
     return fail(GDP_EINVAL, "gdp_download_results: the host buffers are not pinned (hipHostMalloc)");
+  }
   if (gdp_launch_download_results(d_reqs, nreq, d_counts, d_pairs, (int32_t *)dc, (gdp_pair *)dp,
                                   (hipStream_t)stream) != 0)
     return hip_fail("gdp_download_results", hipGetLastError());

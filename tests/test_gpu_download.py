@@ -52,3 +52,10 @@ def test_download_results_matches_the_device_buffers():
     rc = b.L.gdp_download_results(h["d_reqs"].data_ptr(), 1, h["d_counts"].data_ptr(), h["d_pairs"].data_ptr(),
                                   plain.ctypes.data, h_pairs.data_ptr(), None)
     assert rc != 0
+    # and the refusal leaves no HIP error behind for the next launch to report
+    rc = b.L.gdp_download_results(h["d_reqs"].data_ptr(), len(reqs), h["d_counts"].data_ptr(),
+                                  h["d_pairs"].data_ptr(), h_cnt.data_ptr(), h_pairs.data_ptr(),
+                                  torch.cuda.current_stream().cuda_stream)
+    assert rc == 0, b.L.gdp_last_error()
+    b.fill()
+    torch.cuda.synchronize()
