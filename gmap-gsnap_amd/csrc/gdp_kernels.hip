@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
+#include <atomic>
+
 #include "gdp_internal.h"
 
 #define GDP_BLOCK 256
@@ -2002,20 +2004,24 @@ template <bool GENK>
 static int p2_launch(const void *d_descs, int n, const char *d_qbuf, const char *d_gbuf, uint32_t *d_dirs,
                      int32_t *d_final, hipStream_t stream) {
   typedef p2lds<GENK> L;
-  static int cached[64] = {0};
+  // resident workgroups per device, found once (launches may come from
+  // several host threads: each finds the same value)
+  static std::atomic<int> cached[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
-  if (cached[dev] == 0) {
+  int res = cached[dev].load(std::memory_order_relaxed);
+  if (res == 0) {
     int ncu = 0, per = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill2_kernel<GENK>, P2_THREADS, L::LDS_BYTES) !=
         hipSuccess)
       return -1;
-    cached[dev] = ncu * (per > 0 ? per : 1);
+    res = ncu * (per > 0 ? per : 1);
+    cached[dev].store(res, std::memory_order_relaxed);
   }
   // resident workgroups, but no more than the waves there can be (a wave
   // holds at least one of the n problems)
-  const int blocks = gdp_imin(cached[dev], (n + P2_THREADS / 64 - 1) / (P2_THREADS / 64));
+  const int blocks = gdp_imin(res, (n + P2_THREADS / 64 - 1) / (P2_THREADS / 64));
   hipLaunchKernelGGL((gdp_fill2_kernel<GENK>), dim3(blocks), dim3(P2_THREADS), L::LDS_BYTES, stream,
                      (const uint8_t *)d_descs, n, d_qbuf, d_gbuf, d_dirs, d_final);
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -2037,17 +2043,19 @@ extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, con
     if (ringsz > cap || (wide && std_fill)) return -1;
     const int threads = wide ? 64 : GDP_BLOCK, groups = threads / GDP_GROUP;
     const size_t lds = (size_t)groups * (ringsz * sizeof(uint32_t) + GDP_CCAP);
-    static int cached[64][2][2][16] = {{{{0}}}};
+    static std::atomic<int> cached[64][2][2][16];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
     const int slot = rlog ? rlog : (wide ? 13 : 10);
     auto kern = std_fill ? gdp_fill_std_kernel : gdp_fill_kernel;
-    int &res = cached[dev][std_fill][wide][slot];
+    std::atomic<int> &cres = cached[dev][std_fill][wide][slot];
+    int res = cres.load(std::memory_order_relaxed);
     if (res == 0) {
       int ncu = 0, per = 0;
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, lds) != hipSuccess) return -1;
       res = ncu * (per > 0 ? per : 1);
+      cres.store(res, std::memory_order_relaxed);
     }
     const int blocks = gdp_imin(res, (n + groups - 1) / groups);
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), lds, stream, (const gdp_dev_problem *)d_descs, n,
@@ -2058,18 +2066,20 @@ extern "C" int gdp_launch_fill(const void *d_descs, int n, uint32_t kernels, con
   if ((kernels & GDP_FILL_WIDE) && single(false, (kernels >> 16) & 15, true)) return -1;
   if ((kernels & GDP_FILL_STD) && single(true, (kernels >> 12) & 15, false)) return -1;
   if (kernels & GDP_FILL_NARROW) {
-    static int cached_n[64] = {0};
+    static std::atomic<int> cached_n[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
-    if (cached_n[dev] == 0) {
+    int res = cached_n[dev].load(std::memory_order_relaxed);
+    if (res == 0) {
       int ncu = 0, per = 0;
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gdp_fill_narrow_kernel, GDP_BLOCK, 0) != hipSuccess)
         return -1;
-      cached_n[dev] = ncu * (per > 0 ? per : 1);
+      res = ncu * (per > 0 ? per : 1);
+      cached_n[dev].store(res, std::memory_order_relaxed);
     }
     // resident blocks, but no more than the list's waves need
-    const int blocks = gdp_imin(cached_n[dev], (n + GDP_BLOCK - 1) / GDP_BLOCK);
+    const int blocks = gdp_imin(res, (n + GDP_BLOCK - 1) / GDP_BLOCK);
     hipLaunchKernelGGL(gdp_fill_narrow_kernel, dim3(blocks), dim3(GDP_BLOCK), 0, stream,
                        (const gdp_dev_problem *)d_descs, n, d_qbuf, d_gbuf, d_dirs, d_scores);
     if (hipGetLastError() != hipSuccess) return -1;
