@@ -15,6 +15,11 @@
 // per end (one for QUERYEND_NOGAPS), the best end cell of each end mode,
 // the walk, the mismatch filter and the leading-indel trim.
 //
+// Dynprog_cdna_gap (dynprog_cdna.c:786-1180): the two windows cut from a
+// device genome, the four one-sided fills keeping every score, the bridge,
+// both ends' walks, and the join of the walks around the insertion pairs
+// or the gap holder.
+//
 // The host builds the problems, the plan and the walk templates, and sizes
 // the output from the per-call lengths the device reports.
 #include <hip/hip_runtime.h>
@@ -761,6 +766,253 @@ extern "C" int gdp_eg_run(gdp_eg_batch *b, int32_t five, const gdp_eg_call *call
            hipSuccess) ||
       hipStreamSynchronize(stream) != hipSuccess)
     return err(GDP_EHIP, "gdp_eg_run: result copy failed");
+  for (int j = 0; j < m; j++) b->results[b->live[j]] = b->lres[j];
+  b->npairs = total;
+  return GDP_OK;
+}
+
+// --------------------------------------------------------------- cDNA gaps
+
+extern "C" int gdp_launch_cd(int stage, int m, const gdp_cd_dev *calls, const char *qbuf, const void *genome,
+                             int fmt, uint32_t glen, const int32_t *bout, const gdp_tb_request *reqs,
+                             const gdp_pair *pd, const int32_t *cd, int32_t *len, const int64_t *off, gdp_pair *out,
+                             gdp_cd_result *res, hipStream_t stream);
+
+static_assert(sizeof(gdp_cd_call) == 64, "gdp_cd_call is 64 bytes");
+static_assert(sizeof(gdp_cd_result) == 32, "gdp_cd_result is 32 bytes");
+
+namespace {
+constexpr int kCdnaOpen = -10, kCdnaExtend = -7;  // CDNA_OPEN_*, CDNA_EXTEND_* (dynprog_cdna.c:31-37)
+}
+
+struct gdp_cd_batch {
+  Grow q, descs, segs, g, dirs, scores, fin, quads, bout, tmpl, reqs, devs, pd, cd, len, off, out, res;
+  std::vector<gdp_problem> probs;
+  std::vector<uint8_t> hdescs;
+  std::vector<char> hq;
+  std::vector<gdp_segment> hsegs;
+  std::vector<gdp_tb_request> hreqs;
+  std::vector<gdp_cd_dev> hdevs;
+  std::vector<int32_t> live, hquads, hlen;
+  std::vector<int64_t> hoff;
+  std::vector<gdp_cd_result> results, lres;
+  PinnedGrow pairs;
+  int64_t npairs = 0;
+  ~gdp_cd_batch() {
+    for (Grow *x : {&q, &descs, &segs, &g, &dirs, &scores, &fin, &quads, &bout, &tmpl, &reqs, &devs, &pd, &cd, &len,
+                    &off, &out, &res})
+      x->release();
+    pairs.release();
+  }
+};
+
+extern "C" gdp_cd_batch *gdp_cd_create(void) { return new gdp_cd_batch(); }
+extern "C" void gdp_cd_destroy(gdp_cd_batch *b) { delete b; }
+extern "C" int64_t gdp_cd_npairs(const gdp_cd_batch *b) { return b ? b->npairs : 0; }
+extern "C" const gdp_cd_result *gdp_cd_results(const gdp_cd_batch *b) { return b ? b->results.data() : nullptr; }
+extern "C" const gdp_pair *gdp_cd_pairs(const gdp_cd_batch *b) {
+  return b ? static_cast<const gdp_pair *>(b->pairs.p) : nullptr;
+}
+
+extern "C" int gdp_cd_run(gdp_cd_batch *b, const gdp_cd_call *calls, int32_t n, const char *qbuf, int64_t qbytes,
+                          const void *d_genome, int32_t genome_format, uint32_t genome_len, int32_t max_rlength,
+                          int32_t max_glength, void *stream_) {
+  if (b == nullptr || n < 0 || (n > 0 && (calls == nullptr || qbuf == nullptr || d_genome == nullptr)) || qbytes < 0)
+    return err(GDP_EINVAL, "gdp_cd_run: null or negative argument");
+  if (genome_format != GDP_GENOME_ASCII && genome_format != GDP_GENOME_COMP)
+    return err(GDP_EINVAL, "gdp_cd_run: unknown genome format");
+  hipStream_t stream = (hipStream_t)stream_;
+  b->npairs = 0;
+  b->results.assign((size_t)n, gdp_cd_result{0, -1, 0, 0, 0, 0, 0});
+  b->live.clear();
+  for (int32_t k = 0; k < n; k++) {
+    const gdp_cd_call &c = calls[k];
+    if (c.glength <= 1) continue;  // NULL before anything else (dynprog_cdna.c:828-830)
+    if (c.glength > max_glength || c.rlengthR > max_rlength || c.rlengthL > max_rlength) {
+      b->results[k].advanced = 1;  // (dynprog_cdna.c:864-893)
+      continue;
+    }
+    if (c.rlengthL <= 0 || c.rlengthR <= 0 || c.extraband < 0 || c.genestrand < 0 || c.genestrand > 2)
+      return err(GDP_EINVAL, "gdp_cd_run: call " + std::to_string(k) + " has a bad length, extraband or genestrand");
+    // the left read up to the right end's first base (a join's cDNA
+    // insertion copies those bytes), the right read backwards
+    const int64_t spanL = std::max<int64_t>(c.rlengthL, (int64_t)c.rev_roffsetR - c.roffsetL + 1);
+    if (c.qoffL < 0 || c.qoffL + spanL > qbytes || c.qoffR >= qbytes || c.qoffR - c.rlengthR + 1 < 0)
+      return err(GDP_EINVAL, "gdp_cd_run: call " + std::to_string(k) + " reads past its buffer");
+    b->live.push_back(k);
+  }
+  const int m = (int)b->live.size();
+  if (m == 0) return GDP_OK;
+
+  // four one-sided fills per call: left upper, left lower, right upper,
+  // right lower (dynprog_cdna.c:1016-1040); the right ones over the reversed
+  // read, stored after the caller's bytes, and the reversed window
+  int64_t qtotal = 0;
+  for (int j = 0; j < m; j++) qtotal += calls[b->live[j]].rlengthR;
+  const int64_t qbaseR = (qbytes + 15) & ~(int64_t)15;
+  b->hq.resize((size_t)qtotal);
+  b->probs.assign(4 * (size_t)m, gdp_problem{});
+  b->hsegs.assign(2 * (size_t)m, gdp_segment{});
+  b->hreqs.assign(2 * (size_t)m, gdp_tb_request{});
+  b->hdevs.assign((size_t)m, gdp_cd_dev{});
+  b->hquads.resize(5 * (size_t)m);
+  int64_t gpos = 0, pcap = 0, qr = 0;
+  for (int j = 0; j < m; j++) {
+    const gdp_cd_call &c = calls[b->live[j]];
+    const int gl = c.glength, rev_goffset = c.goffset + gl - 1;
+    // mismatch type by defect rate (dynprog_cdna.c:846-862; DEFECT_HIGHQ
+    // 0.003 and DEFECT_MEDQ 0.014, dynprog.h:58-59)
+    const int mt = c.defect_rate < 0.003 ? GDP_HIGHQ : (c.defect_rate < 0.014 ? GDP_MEDQ : GDP_LOWQ);
+    const int u8 = gdp_use8p_size(mt);
+    const int width = (gl < u8 || (c.rlengthL < u8 && c.rlengthR <= u8)) ? 8 : 16;  // (906-912)
+    int32_t lbL = 0, ubL = 0, lbR = 0, ubR = 0;
+    gdp_compute_bands(&lbL, &ubL, c.rlengthL, gl, c.extraband, 1);
+    gdp_compute_bands(&lbR, &ubR, c.rlengthR, gl, c.extraband, 1);
+    for (int i = 0; i < c.rlengthR; i++) b->hq[(size_t)(qr + i)] = qbuf[c.qoffR - i];
+    const int64_t gL = gpos, gR = gpos + gl;
+    for (int t = 0; t < 4; t++) {
+      const bool right = t >= 2;
+      gdp_problem &p = b->probs[4 * (size_t)j + t];
+      p.qoff = right ? qbaseR + qr : c.qoffL;
+      p.goff = p.galtoff = right ? gR : gL;
+      p.rl = right ? c.rlengthR : c.rlengthL;
+      p.gl = gl;
+      p.lband = (int16_t)std::min<int32_t>(right ? lbR : lbL, 32767);
+      p.uband = (int16_t)std::min<int32_t>(right ? ubR : ubL, 32767);
+      p.kind = (t & 1) ? GDP_KIND_LOWER : GDP_KIND_UPPER;
+      p.width = (int8_t)width;
+      p.mismatchtype = (int8_t)mt;
+      p.jump_late = right ? !c.jump_late : (c.jump_late != 0);  // for revp the other rule
+      p.open = (int8_t)kCdnaOpen;
+      p.extend = (int8_t)kCdnaExtend;
+      p.keep_scores = GDP_KEEP_ALL;
+    }
+    // the windows (dynprog_cdna.c:921-931): gsequence from goffset and
+    // rev_gsequence up to rev_goffset, read backwards by the right fills
+    gdp_segment &sL = b->hsegs[2 * (size_t)j], &sR = b->hsegs[2 * (size_t)j + 1];
+    if (c.watsonp) {
+      sL.coord = c.chroffset + (uint32_t)c.goffset;
+      sL.bound = c.chrhigh;
+      sR.coord = c.chroffset + (uint32_t)rev_goffset + 1u;
+      sR.bound = c.chroffset;
+      sR.left = 1;
+    } else {
+      sL.coord = c.chrhigh - (uint32_t)c.goffset + 1u;
+      sL.bound = c.chroffset;
+      sL.left = 1;
+      sL.revcomp = 1;
+      sR.coord = c.chrhigh - (uint32_t)rev_goffset;
+      sR.bound = c.chrhigh;
+      sR.revcomp = 1;
+    }
+    sL.length = sR.length = gl;
+    sL.out_off = gL;
+    sR.out_off = gR;
+    sR.reverse = 1;
+    int32_t *qd = &b->hquads[5 * (size_t)j];
+    qd[0] = 4 * j;
+    qd[1] = 4 * j + 1;
+    qd[2] = 4 * j + 2;
+    qd[3] = 4 * j + 3;
+    qd[4] = c.rev_roffsetR - c.roffsetL;
+    // walk templates, the right end first (the bridge sets fill, r and c)
+    for (int t = 0; t < 2; t++) {
+      gdp_tb_request &q = b->hreqs[2 * (size_t)j + t];
+      const bool right = t == 0;
+      q.queryoffset = right ? c.rev_roffsetR : c.roffsetL;
+      q.genomeoffset = right ? rev_goffset : c.goffset;
+      q.chroffset = c.chroffset;
+      q.chrhigh = c.chrhigh;
+      q.watsonp = c.watsonp ? 1 : 0;
+      q.genestrand = c.genestrand;
+      q.revp = right ? 1 : 0;
+      q.pair_off = pcap;
+      q.max_pairs = (right ? c.rlengthR : c.rlengthL) + gl + 4;
+      pcap += q.max_pairs;
+    }
+    gdp_cd_dev &d = b->hdevs[(size_t)j];
+    d.qoffL = c.qoffL;
+    d.roffsetL = c.roffsetL;
+    d.rev_roffsetR = c.rev_roffsetR;
+    d.goffset = c.goffset;
+    d.rev_goffset = rev_goffset;
+    d.chroffset = c.chroffset;
+    d.chrhigh = c.chrhigh;
+    d.watsonp = c.watsonp ? 1 : 0;
+    gpos += 2 * (int64_t)gl;
+    qr += c.rlengthR;
+  }
+  const int np = 4 * m;
+  b->hdescs.resize((size_t)gdp_plan_bytes(np));
+  int64_t dw = 0, ss = 0;
+  int rc = gdp_plan_ex(b->probs.data(), np, 0, b->hdescs.data(), &dw, &ss);
+  if (rc != GDP_OK) return rc;
+  const uint32_t kernels = gdp_plan_kernels(b->hdescs.data(), np);
+  const int64_t used = gdp_plan_used_bytes(b->hdescs.data(), np);
+  const size_t tb = sizeof(gdp_tb_request) * 2 * (size_t)m;
+  if (!b->q.ensure((size_t)(qbaseR + qtotal) + 16) || !b->descs.ensure((size_t)used) ||
+      !b->segs.ensure(sizeof(gdp_segment) * 2 * (size_t)m) || !b->g.ensure((size_t)gpos + 16) ||
+      !b->dirs.ensure(sizeof(uint32_t) * (size_t)(dw > 0 ? dw : 1)) ||
+      !b->scores.ensure(sizeof(int16_t) * (size_t)(ss > 0 ? ss : 1)) || !b->fin.ensure(sizeof(int32_t) * np) ||
+      !b->quads.ensure(sizeof(int32_t) * 5 * (size_t)m) || !b->bout.ensure(sizeof(int32_t) * 5 * (size_t)m) ||
+      !b->tmpl.ensure(tb) || !b->reqs.ensure(tb) || !b->devs.ensure(sizeof(gdp_cd_dev) * (size_t)m) ||
+      !b->pd.ensure(sizeof(gdp_pair) * (size_t)pcap) || !b->cd.ensure(sizeof(int32_t) * 12 * (size_t)m) ||
+      !b->len.ensure(sizeof(int32_t) * (size_t)m) || !b->off.ensure(sizeof(int64_t) * (size_t)m) ||
+      !b->res.ensure(sizeof(gdp_cd_result) * (size_t)m))
+    return err(GDP_ENOMEM, "gdp_cd_run: device allocation failed");
+  auto up = [&](void *d, const void *h, size_t bytes) {
+    return bytes == 0 || hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream) == hipSuccess;
+  };
+  if (!up(b->q.p, qbuf, (size_t)qbytes) || !up(b->q.as<char>() + qbaseR, b->hq.data(), b->hq.size()) ||
+      !up(b->descs.p, b->hdescs.data(), (size_t)used) ||
+      !up(b->segs.p, b->hsegs.data(), sizeof(gdp_segment) * 2 * (size_t)m) ||
+      !up(b->quads.p, b->hquads.data(), sizeof(int32_t) * 5 * (size_t)m) || !up(b->tmpl.p, b->hreqs.data(), tb) ||
+      !up(b->devs.p, b->hdevs.data(), sizeof(gdp_cd_dev) * (size_t)m))
+    return err(GDP_EHIP, "gdp_cd_run: upload failed");
+  const char *dq = b->q.as<char>(), *dg = b->g.as<char>();
+  if ((rc = gdp_genome_segments(d_genome, genome_format, genome_len, b->segs.as<gdp_segment>(), 2 * m,
+                                b->g.as<char>(), stream)) != GDP_OK)
+    return rc;
+  if ((rc = gdp_fill_ex(b->descs.p, np, kernels, dq, dg, b->dirs.as<uint32_t>(), b->scores.as<int16_t>(),
+                        b->fin.as<int32_t>(), stream)) != GDP_OK)
+    return rc;
+  if ((rc = gdp_bridge_cdna(b->descs.p, b->quads.as<int32_t>(), m, b->scores.as<int16_t>(), b->bout.as<int32_t>(),
+                            b->tmpl.as<gdp_tb_request>(), b->reqs.as<gdp_tb_request>(), stream)) != GDP_OK)
+    return rc;
+  if ((rc = gdp_traceback_ex(b->descs.p, b->reqs.as<gdp_tb_request>(), 2 * m, dq, dg, b->dirs.as<uint32_t>(),
+                             d_genome, genome_format, genome_len, b->pd.as<gdp_pair>(), b->cd.as<int32_t>(),
+                             stream)) != GDP_OK)
+    return rc;
+  if (gdp_launch_cd(0, m, b->devs.as<gdp_cd_dev>(), nullptr, nullptr, 0, 0, b->bout.as<int32_t>(), nullptr, nullptr,
+                    b->cd.as<int32_t>(), b->len.as<int32_t>(), nullptr, nullptr, nullptr, stream))
+    return err(GDP_EHIP, "gdp_cd_run: lengths failed");
+  b->hlen.resize((size_t)m);
+  if (hipMemcpyAsync(b->hlen.data(), b->len.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    return err(GDP_EHIP, "gdp_cd_run: lengths copy failed");
+  b->hoff.resize((size_t)m);
+  int64_t total = 0;
+  for (int j = 0; j < m; j++) {
+    b->hoff[j] = total;
+    total += b->hlen[j] > 0 ? b->hlen[j] : 0;
+  }
+  if (!b->out.ensure(sizeof(gdp_pair) * (size_t)(total > 0 ? total : 1)) ||
+      !up(b->off.p, b->hoff.data(), sizeof(int64_t) * (size_t)m))
+    return err(GDP_ENOMEM, "gdp_cd_run: output allocation failed");
+  if (gdp_launch_cd(1, m, b->devs.as<gdp_cd_dev>(), dq, d_genome, genome_format, genome_len, b->bout.as<int32_t>(),
+                    b->reqs.as<gdp_tb_request>(), b->pd.as<gdp_pair>(), b->cd.as<int32_t>(), b->len.as<int32_t>(),
+                    b->off.as<int64_t>(), b->out.as<gdp_pair>(), b->res.as<gdp_cd_result>(), stream))
+    return err(GDP_EHIP, "gdp_cd_run: join failed");
+  b->lres.resize((size_t)m);
+  if (!b->pairs.ensure(sizeof(gdp_pair) * (size_t)(total > 0 ? total : 1)))
+    return err(GDP_ENOMEM, "gdp_cd_run: pinned host allocation failed");
+  if (hipMemcpyAsync(b->lres.data(), b->res.p, sizeof(gdp_cd_result) * m, hipMemcpyDeviceToHost, stream) !=
+          hipSuccess ||
+      (total > 0 &&
+       hipMemcpyAsync(b->pairs.p, b->out.p, sizeof(gdp_pair) * (size_t)total, hipMemcpyDeviceToHost, stream) !=
+           hipSuccess) ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    return err(GDP_EHIP, "gdp_cd_run: result copy failed");
   for (int j = 0; j < m; j++) b->results[b->live[j]] = b->lres[j];
   b->npairs = total;
   return GDP_OK;

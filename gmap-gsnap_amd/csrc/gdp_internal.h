@@ -146,6 +146,17 @@ struct gdp_sg_dev {
 };
 static_assert(sizeof(gdp_sg_dev) == 48, "gdp_sg_dev is 48 bytes");
 
+// One Dynprog_cdna_gap call as its join kernels read it: where the left
+// end's read starts in the query buffer, and the query and genome
+// positions of both ends (rev_goffset = goffset + glength - 1).
+struct gdp_cd_dev {
+  int64_t qoffL;
+  int32_t roffsetL, rev_roffsetR, goffset, rev_goffset;
+  uint32_t chroffset, chrhigh;
+  int32_t watsonp, pad;
+};
+static_assert(sizeof(gdp_cd_dev) == 40, "gdp_cd_dev is 40 bytes");
+
 // Row stride (direction nibbles) of a narrow sweep over band width w
 // (uband of an upper fill, lband of a lower one), 0 when w + 1 > 32.
 GDP_HD int gdp_narrow_stride(int w) { return w < 8 ? 8 : (w < 16 ? 16 : (w < 32 ? 32 : 0)); }

@@ -2313,6 +2313,124 @@ extern "C" int gdp_launch_eg(int stage, int m, int five, const int32_t *flags, c
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// ------------------------------------------------ Dynprog_cdna_gap's join
+
+// get_genomic_nt (pairpool.c:1016-1063) over the device genome, the
+// alternate genome equal to it: '*' outside the chromosome, the complement
+// on the minus strand.
+__device__ __forceinline__ int gdp_genomic_nt(const void *genome, int fmt, uint32_t glen, uint32_t chroffset,
+                                              uint32_t chrhigh, int watsonp, int gpos) {
+  const uint32_t pos = watsonp ? chroffset + (uint32_t)gpos : chrhigh - (uint32_t)gpos;
+  if (pos < chroffset || pos >= chrhigh) return '*';
+  const int ch = genome_at(genome, fmt, glen, pos);
+  return watsonp ? ch : compl_char(ch);
+}
+
+// Query and genome positions the bridge's cells leave between the two
+// ends' walks (dynprog_cdna.c:1067-1068).
+__device__ __forceinline__ void gdp_cd_jumps(const gdp_cd_dev &c, const int32_t *o, int &qj, int &gj) {
+  qj = (c.rev_roffsetR - o[4]) - (c.roffsetL + o[3]) + 1;
+  gj = (c.rev_goffset - o[2]) - (c.goffset + o[1]) + 1;
+}
+
+// Per call the length of the returned list: the right walk, 18 insertion
+// pairs when 9 query and 9 genome positions are left (INSERT_PAIRS,
+// dynprog_cdna.c:39, 1072) or else one gap holder, the left walk; -1 (NULL)
+// for the gap holder alone (1170-1173).
+__global__ void gdp_cd_len_kernel(int m, const gdp_cd_dev *__restrict__ calls, const int32_t *__restrict__ bout,
+                                  const int32_t *__restrict__ cd, int32_t *__restrict__ len) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  int qj, gj;
+  gdp_cd_jumps(calls[k], bout + 5 * (int64_t)k, qj, gj);
+  const int nR = gdp_imax(cd[12 * (int64_t)k], 0), nL = gdp_imax(cd[12 * (int64_t)k + 6], 0);
+  const int tot = nR + (qj == 9 && gj == 9 ? 18 : 1) + nL;
+  len[k] = tot == 1 ? -1 : tot;
+}
+
+// Writes each call's returned list from its head: the right end's walk
+// reversed (its pairs were reversed before the rest was pushed, and the
+// whole list at the end), then the insertion pairs or the gap holder and the
+// left end's walk in push order (dynprog_cdna.c:1062-1175).  One workgroup
+// per call.
+__global__ void gdp_cd_join_kernel(int m, const gdp_cd_dev *__restrict__ calls, const char *__restrict__ qbuf,
+                                   const void *__restrict__ genome, int fmt, uint32_t glen,
+                                   const int32_t *__restrict__ bout, const gdp_tb_request *__restrict__ reqs,
+                                   const gdp_pair *__restrict__ pd, const int32_t *__restrict__ cd,
+                                   const int32_t *__restrict__ len, const int64_t *__restrict__ off,
+                                   gdp_pair *__restrict__ out, gdp_cd_result *__restrict__ res) {
+  for (int k = blockIdx.x; k < m; k += gridDim.x) {
+    const gdp_cd_dev c = calls[k];
+    const int32_t *o = bout + 5 * (int64_t)k;
+    int qj, gj;
+    gdp_cd_jumps(c, o, qj, gj);
+    const bool ins = qj == 9 && gj == 9;
+    const int n = len[k];
+    const int32_t *cR = cd + 12 * (int64_t)k, *cL = cR + 6;
+    if (n > 0) {
+      const int nR = gdp_imax(cR[0], 0), nL = gdp_imax(cL[0], 0), mid = ins ? 18 : 1;
+      const gdp_pair *pR = pd + reqs[2 * (int64_t)k].pair_off, *pL = pd + reqs[2 * (int64_t)k + 1].pair_off;
+      gdp_pair *dst = out + off[k];
+      for (int j = threadIdx.x; j < nR; j += blockDim.x) dst[j] = pR[nR - 1 - j];
+      for (int j = threadIdx.x; j < mid; j += blockDim.x) {
+        gdp_pair p;
+        if (!ins) {  // Pairpool_push_gapholder (pairpool.c:374-452), knownp false
+          p.querypos = -1;
+          p.genomepos = -1;
+          p.cdna = p.comp = p.genome = p.genomealt = ' ';
+          p.gapp = 1;
+          p.genomejump = (int16_t)gj;
+        } else if (j < 9) {  // the cDNA insertion, query positions descending
+          const int q = c.rev_roffsetR - o[4] - j;
+          p.querypos = q;
+          p.genomepos = c.rev_goffset - o[2] + 1;
+          p.cdna = qbuf[c.qoffL + (q - c.roffsetL)];
+          p.comp = '~';  // SHORTGAP_COMP (comp.h:10)
+          p.genome = p.genomealt = ' ';
+          p.gapp = 0;
+          p.genomejump = 0;
+        } else {  // the genome insertion, genome positions descending
+          const int g = c.rev_goffset - o[2] - (j - 9);
+          const int ch = gdp_genomic_nt(genome, fmt, glen, c.chroffset, c.chrhigh, c.watsonp, g);
+          p.querypos = c.roffsetL + o[3];
+          p.genomepos = g;
+          p.cdna = ' ';
+          p.comp = '~';
+          p.genome = p.genomealt = (int8_t)ch;
+          p.gapp = 0;
+          p.genomejump = 0;
+        }
+        dst[nR + j] = p;
+      }
+      for (int j = threadIdx.x; j < nL; j += blockDim.x) dst[nR + mid + j] = pL[j];
+    }
+    if (threadIdx.x == 0) {
+      gdp_cd_result r;
+      r.pair_off = off[k];
+      r.npairs = n;
+      r.traceback_score = cR[1] + cL[1];
+      r.incomplete = ins ? 0 : 1;
+      r.advanced = 1;
+      r.finalscore = o[0];
+      r.pad = 0;
+      res[k] = r;
+    }
+  }
+}
+
+extern "C" int gdp_launch_cd(int stage, int m, const gdp_cd_dev *calls, const char *qbuf, const void *genome,
+                             int fmt, uint32_t glen, const int32_t *bout, const gdp_tb_request *reqs,
+                             const gdp_pair *pd, const int32_t *cd, int32_t *len, const int64_t *off, gdp_pair *out,
+                             gdp_cd_result *res, hipStream_t stream) {
+  if (m <= 0) return 0;
+  if (stage == 0)
+    hipLaunchKernelGGL(gdp_cd_len_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, m, calls, bout, cd, len);
+  else
+    hipLaunchKernelGGL(gdp_cd_join_kernel, dim3(gdp_imin(m, 8192)), dim3(64), 0, stream, m, calls, qbuf, genome, fmt,
+                       glen, bout, reqs, pd, cd, (const int32_t *)len, off, out, res);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // Copies between device memory and pinned host memory mapped into the
 // device's address space, as kernels on the caller's stream.  A copy engine
 // serves the streams of a process in submission order, so a batch's upload

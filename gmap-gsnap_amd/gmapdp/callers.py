@@ -16,13 +16,19 @@ The end callers fill both triangles of the read end against its window
 mode asked (or the nogaps diagonal), walk back from it, drop the result when
 it has too many mismatches, and trim indels from the far end; all of it on
 the device.
+
+The cDNA gap caller (Dynprog_cdna_gap, dynprog_cdna.c:786-1180, run by
+gdp_cd_run) fills both ends' triangles with every score kept, bridges them
+on the device, walks both ends and joins the walks around the insertion
+pairs or a gap holder, as the reference assembles its list.
 """
 import ctypes
 
 import numpy as np
 
 from .batch import _check, init
-from .lib import EG_CALL_DTYPE, EG_RESULT_DTYPE, PAIR_DTYPE, SG_CALL_DTYPE, SG_RESULT_DTYPE
+from .lib import (CD_CALL_DTYPE, CD_RESULT_DTYPE, EG_CALL_DTYPE, EG_RESULT_DTYPE, PAIR_DTYPE, SG_CALL_DTYPE,
+                  SG_RESULT_DTYPE)
 
 _batch = None
 
@@ -137,4 +143,51 @@ def end_gap(five, calls, qbuf, genome, max_rlength=500, max_glength=2000, user_p
     for k in np.nonzero(res["npairs"] >= 0)[0]:
         o, m = int(res["pair_off"][k]), int(res["npairs"][k])
         out[k] = (pairs[o:o + m], res["counters"][k].astype(np.int32))
+    return out, res
+
+
+class CdnaGapBatch:
+    """A gdp_cd_batch: device and host buffers reused across runs."""
+
+    def __init__(self):
+        self.L = init()
+        self.h = self.L.gdp_cd_create()
+
+    def run(self, calls, qbuf, genome, max_rlength=500, max_glength=2000, stream=None, copy=True):
+        """calls: CD_CALL_DTYPE rows; returns (results CD_RESULT_DTYPE, pairs
+        PAIR_DTYPE); copy as in SingleGapBatch.run."""
+        L = self.L
+        calls = np.ascontiguousarray(calls, CD_CALL_DTYPE)
+        q = qbuf if isinstance(qbuf, np.ndarray) else np.frombuffer(bytes(qbuf), np.uint8)
+        q = np.ascontiguousarray(q).view(np.uint8)
+        _check(L, L.gdp_cd_run(self.h, calls.ctypes.data, len(calls), q.ctypes.data, len(q), genome.ptr, genome.fmt,
+                               genome.nbases, max_rlength, max_glength, stream), "gdp_cd_run")
+        res = _view(L.gdp_cd_results(self.h), len(calls), CD_RESULT_DTYPE)
+        pairs = _view(L.gdp_cd_pairs(self.h), int(L.gdp_cd_npairs(self.h)), PAIR_DTYPE, copy)
+        return res, pairs
+
+    def close(self):
+        if self.h:
+            self.L.gdp_cd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+_cbatch = None
+
+
+def cdna_gap(calls, qbuf, genome, max_rlength=500, max_glength=2000):
+    """Dynprog_cdna_gap for every call.  Returns (per call None or (pairs
+    from the returned list's head, traceback_score, incompletep), the
+    CD_RESULT_DTYPE rows)."""
+    global _cbatch
+    if _cbatch is None:
+        _cbatch = CdnaGapBatch()
+    res, pairs = _cbatch.run(calls, qbuf, genome, max_rlength, max_glength)
+    out = [None] * len(res)
+    for k in np.nonzero(res["npairs"] >= 0)[0]:
+        o, m = int(res["pair_off"][k]), int(res["npairs"][k])
+        out[k] = (pairs[o:o + m], int(res["traceback_score"][k]), bool(res["incomplete"][k]))
     return out, res

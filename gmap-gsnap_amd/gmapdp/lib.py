@@ -68,8 +68,15 @@ EG_CALL_DTYPE = np.dtype([("qoff", "<i8"), ("rl", "<i4"), ("gl", "<i4"), ("roffs
                           ("genestrand", "i1"), ("jump_late", "i1"), ("endalign", "i1"), ("defect_rate", "<f8")])
 EG_RESULT_DTYPE = np.dtype([("pair_off", "<i8"), ("npairs", "<i4"), ("counters", "<i4", (5,)), ("finalscore", "<i4"),
                             ("advanced", "<i4")])
+CD_CALL_DTYPE = np.dtype([("qoffL", "<i8"), ("qoffR", "<i8"), ("rlengthL", "<i4"), ("rlengthR", "<i4"),
+                          ("glength", "<i4"), ("roffsetL", "<i4"), ("rev_roffsetR", "<i4"), ("goffset", "<i4"),
+                          ("chroffset", "<u4"), ("chrhigh", "<u4"), ("extraband", "<i4"), ("watsonp", "i1"),
+                          ("genestrand", "i1"), ("jump_late", "i1"), ("pad", "i1"), ("defect_rate", "<f8")])
+CD_RESULT_DTYPE = np.dtype([("pair_off", "<i8"), ("npairs", "<i4"), ("traceback_score", "<i4"), ("incomplete", "<i4"),
+                            ("advanced", "<i4"), ("finalscore", "<i4"), ("pad", "<i4")])
 assert SG_CALL_DTYPE.itemsize == 48 and SG_RESULT_DTYPE.itemsize == 40
 assert EG_CALL_DTYPE.itemsize == 48 and EG_RESULT_DTYPE.itemsize == 40
+assert CD_CALL_DTYPE.itemsize == 64 and CD_RESULT_DTYPE.itemsize == 32
 SEGMENT_DTYPE = np.dtype([("coord", "<u4"), ("bound", "<u4"), ("length", "<i4"), ("left", "i1"), ("revcomp", "i1"),
                           ("reverse", "<i2"), ("out_off", "<i8")])
 ALIGN_REQ_DTYPE = np.dtype([("tag", "<u8"), ("read", "<u8"), ("rl", "<i4"), ("gl", "<i4"), ("roffset", "<i4"),
@@ -159,6 +166,12 @@ def load(path=None):
         "gdp_eg_destroy": (None, [vp]),
         "gdp_eg_run": (ctypes.c_int, [vp, i32, vp, i32, vp, i64, vp, i32, ctypes.c_uint32, i32, i32, i32, i32, i32, i32,
                                       vp]),
+        "gdp_cd_create": (vp, []),
+        "gdp_cd_destroy": (None, [vp]),
+        "gdp_cd_run": (ctypes.c_int, [vp, vp, i32, vp, i64, vp, i32, ctypes.c_uint32, i32, i32, vp]),
+        "gdp_cd_npairs": (i64, [vp]),
+        "gdp_cd_results": (vp, [vp]),
+        "gdp_cd_pairs": (vp, [vp]),
         "gdp_eg_npairs": (i64, [vp]),
         "gdp_eg_results": (vp, [vp]),
         "gdp_eg_pairs": (vp, [vp]),

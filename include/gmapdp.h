@@ -743,6 +743,73 @@ int64_t gdp_eg_npairs(const gdp_eg_batch *b);
 const gdp_eg_result *gdp_eg_results(const gdp_eg_batch *b);
 const gdp_pair *gdp_eg_pairs(const gdp_eg_batch *b);
 
+/* ---- Batched Dynprog_cdna_gap ----------------------------------------- */
+
+/* One call of Dynprog_cdna_gap (dynprog_cdna.c:786-1180), as stage 3 makes
+   it: the left end reads qbuf[qoffL ...] forward (rsequenceL, roffsetL its
+   query position), the right end reads qbuf[qoffR], qbuf[qoffR - 1], ...
+   backwards (rev_rsequenceR, rev_roffsetR), over the glength genome columns
+   from goffset (rev_goffset = goffset + glength - 1).  The left end's bytes
+   past rlengthL up to the right end's first base are read too: the cDNA
+   insertion of a join copies them. */
+typedef struct gdp_cd_call {
+  int64_t qoffL, qoffR;
+  int32_t rlengthL, rlengthR, glength;
+  int32_t roffsetL, rev_roffsetR, goffset;
+  uint32_t chroffset, chrhigh;
+  int32_t extraband;       /* extraband_paired */
+  int8_t watsonp;
+  int8_t genestrand;
+  int8_t jump_late;        /* jump_late_p */
+  int8_t pad;
+  double defect_rate;
+} gdp_cd_call;
+
+/* Per call: npairs -1 for NULL, else the records at pair_off in the order of
+   the returned list from its head (the right end's walk, the cDNA and genome
+   insertion pairs or the gap holder, the left end's walk); traceback_score
+   and incomplete (incompletep) as the caller leaves them; finalscore = the
+   bridge's score; advanced = 1 when the caller advances its dynprogindex
+   (every return but the NULL for glength <= 1). */
+typedef struct gdp_cd_result {
+  int64_t pair_off;
+  int32_t npairs;
+  int32_t traceback_score;
+  int32_t incomplete;
+  int32_t advanced;
+  int32_t finalscore;
+  int32_t pad;
+} gdp_cd_result;
+
+typedef struct gdp_cd_batch gdp_cd_batch;
+gdp_cd_batch *gdp_cd_create(void);
+void gdp_cd_destroy(gdp_cd_batch *b);
+
+/* Runs n calls of Dynprog_cdna_gap against a device genome, the reference's
+   steps for each:
+   - glength <= 1: NULL, counters 0, dynprogindex kept;
+   - glength > max_glength, or rlengthR or rlengthL > max_rlength: NULL,
+     dynprogindex advanced;
+   - the mismatch type by defect rate and the cDNA penalties (-10, -7), the
+     int8 fills when glength or both ends are under use8p_size;
+   - the genome windows cut from the device genome (Genome_get_segment_right
+     from goffset and Genome_get_segment_left to rev_goffset on the plus
+     strand, their reverse complements on the minus strand), the four
+     one-sided fills with bands from Dynprog_compute_bands (widebandp), the
+     right end's over the reversed read and window with the other tie rule;
+   - the bridge (gdp_bridge_cdna), the right end's walk, then the cDNA and
+     genome insertion pairs when the join leaves 9 query and 9 genome
+     positions (INSERT_PAIRS), else a gap holder with incomplete set, then
+     the left end's walk; a list of the gap holder alone is NULL.
+   Every step runs on the device; results and pairs are copied back to the
+   batch's host arrays before it returns. */
+int gdp_cd_run(gdp_cd_batch *b, const gdp_cd_call *calls, int32_t n, const char *qbuf, int64_t qbytes,
+               const void *d_genome, int32_t genome_format, uint32_t genome_len, int32_t max_rlength,
+               int32_t max_glength, void *stream);
+int64_t gdp_cd_npairs(const gdp_cd_batch *b);
+const gdp_cd_result *gdp_cd_results(const gdp_cd_batch *b);
+const gdp_pair *gdp_cd_pairs(const gdp_cd_batch *b);
+
 /* ---- Serving: continuous batching of alignment requests -------------- */
 
 /* A request loop over a device-resident genome.  Each request is the DP of
