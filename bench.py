@@ -135,8 +135,9 @@ def _splice_ends(n, seed=1007):
 def callers_bench(args):
     """GMAP's DP callers batched in the library, from host buffers in to host
     results out (PCIe included): Dynprog_end5_splicejunction (gdp_sj_run),
-    Dynprog_single_gap (gdp_sg_run) and Dynprog_end5_gap (gdp_eg_run) over a
-    device genome.  Each is run once untimed, then timed over 5 runs."""
+    Dynprog_single_gap (gdp_sg_run), Dynprog_end5_gap (gdp_eg_run) and
+    Dynprog_cdna_gap (gdp_cd_run) over a device genome.  Each is run once
+    untimed, then timed over 5 runs."""
     from gmapdp import callers, ends, serve
     from gmapdp.lib import EG_CALL_DTYPE, SG_CALL_DTYPE
     out = {}
@@ -195,6 +196,41 @@ def callers_bench(args):
                        "sample": "read ends of the serving mix against windows 20 bases longer, the four end "
                                  "modes in turn"}
     log("callers: end5_gap %d calls in %.4f s = %.0f calls/s" % (n, dt, n / dt))
+
+    # Dynprog_cdna_gap: genome gaps of 16-160 columns on the plus strand under
+    # query gaps holding 10-60 extra bases, both ends reading gl + 8 bases
+    from gmapdp.lib import CD_CALL_DTYPE
+    rng = np.random.default_rng(1006)
+    region, flank = 20_000_000, 200
+    G = serve.decode(gc, nb, 0, region)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    gls = rng.integers(16, 161, n)
+    ins = rng.integers(10, 61, n)
+    cuts = rng.integers(0, gls + 1)
+    goff = rng.integers(flank + 10, region - 600, n)
+    parts, qlen = [], np.zeros(n, np.int64)
+    for k in range(n):
+        g0, gl, a = int(goff[k]), int(gls[k]), int(cuts[k])
+        q = np.concatenate([G[g0 - flank:g0 + a], acgt[rng.integers(0, 4, int(ins[k]))], G[g0 + a:g0 + gl + flank]])
+        parts.append(q)
+        qlen[k] = len(q)
+    cq = np.concatenate(parts + [np.zeros(16, np.uint8)])
+    qstart = np.concatenate([[0], np.cumsum(qlen)[:-1]])
+    mid = gls + ins
+    cd = np.zeros(n, CD_CALL_DTYPE)
+    cd["qoffL"], cd["qoffR"] = qstart + flank, qstart + flank + mid - 1
+    cd["roffsetL"], cd["rev_roffsetR"] = flank, flank + mid - 1
+    cd["rlengthL"] = cd["rlengthR"] = np.minimum(gls + 8, flank)
+    cd["glength"], cd["goffset"] = gls, goff
+    cd["chroffset"], cd["chrhigh"], cd["watsonp"] = 0, region, 1
+    cd["extraband"], cd["defect_rate"] = 14, 0.001
+    b3 = callers.CdnaGapBatch()
+    dt, (res, pairs) = timed(lambda: b3.run(cd, cq, dg, max_rlength=500, max_glength=2000, copy=False))
+    out["cdna_gap"] = {"calls": n, "seconds": round(dt, 4), "calls_per_s": round(n / dt, 1), "pairs": int(len(pairs)),
+                       "joined": int((res["npairs"] >= 0).sum()),
+                       "sample": "genome gaps of 16-160 columns under query gaps with 10-60 extra bases, ends of "
+                                 "gl + 8 bases, on a 1e8-base device genome"}
+    log("callers: cdna_gap %d calls in %.4f s = %.0f calls/s" % (n, dt, n / dt))
     return out
 
 
@@ -327,7 +363,8 @@ def main():
     ap.add_argument("--serve-mix-order", type=int, default=1, choices=(0, 1),
                     help="mix_order of the server's batches with --serve-batch and --serve-long")
     ap.add_argument("--callers", action="store_true",
-                    help="also time GMAP's batched DP callers (splice-junction ends, single gaps, end gaps)")
+                    help="also time GMAP's batched DP callers (splice-junction ends, single gaps, end gaps, "
+                         "cDNA gaps)")
     ap.add_argument("--callers-n", type=int, default=200_000, help="calls of each caller with --callers")
     ap.add_argument("--serve-genome", type=int, default=3_100_000_000, help="bases of the device genome")
     ap.add_argument("--serve-requests", type=int, default=600_000)
