@@ -706,12 +706,23 @@ extern "C" int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, i
       q.status = 0;
       q.roff = 0;
       int32_t lb = 0, ub = 0;
-      if (r.rl <= 0 || r.gl <= 0 || r.rl > s->cfg.max_rl || r.gl > s->cfg.max_gl) {
+      if (r.scoring == 1) {
+        // Dynprog_single_gap's choice by defect rate (dynprog_single.c:458-481;
+        // DEFECT_HIGHQ 0.003, DEFECT_MEDQ 0.014, dynprog.h:58-59;
+        // SINGLE_OPEN/EXTEND_{HIGHQ,MEDQ,LOWQ}, dynprog.h:61-67)
+        const bool hq = r.defect_rate < 0.003, mq = r.defect_rate < 0.014;
+        q.r.mismatchtype = (int8_t)(hq ? GDP_HIGHQ : (mq ? GDP_MEDQ : GDP_LOWQ));
+        q.r.open = (int8_t)(hq ? -8 : (mq ? -7 : -6));
+        q.r.extend = (int8_t)(hq ? -3 : (mq ? -2 : -1));
+      }
+      if (r.scoring < 0 || r.scoring > 1) {
+        q.status = (int8_t)-GDP_EINVAL;
+      } else if (r.rl <= 0 || r.gl <= 0 || r.rl > s->cfg.max_rl || r.gl > s->cfg.max_gl) {
         q.status = (int8_t)-GDP_ETOOBIG;
       } else {
         gdp_compute_bands(&lb, &ub, r.rl, r.gl, r.extraband, r.widebandp);
-        if (lb < 0 || ub < 0 || lb + ub > GDP_MAX_SPAN || r.mismatchtype < 0 || r.mismatchtype > 3 || r.open > 0 ||
-            r.extend > 0 || r.read == nullptr)
+        if (lb < 0 || ub < 0 || lb + ub > GDP_MAX_SPAN || q.r.mismatchtype < 0 || q.r.mismatchtype > 3 ||
+            q.r.open > 0 || q.r.extend > 0 || r.read == nullptr)
           q.status = (int8_t)-GDP_EINVAL;
       }
       if (q.status) {
@@ -721,7 +732,7 @@ extern "C" int gdp_server_submit(gdp_server *s, const gdp_align_request *reqs, i
       q.lband = (int16_t)lb;
       q.uband = (int16_t)ub;
       // Dynprog_single_gap's kernel choice (dynprog_single.c:595).
-      const int u8 = gdp_use8p_size(r.mismatchtype);
+      const int u8 = gdp_use8p_size(q.r.mismatchtype);
       q.width = (r.rl < u8 && r.gl < u8) ? 8 : 16;
       qs.push_back(q);
       rd.push_back(r.read);

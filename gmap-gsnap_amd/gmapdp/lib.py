@@ -82,8 +82,8 @@ SEGMENT_DTYPE = np.dtype([("coord", "<u4"), ("bound", "<u4"), ("length", "<i4"),
 ALIGN_REQ_DTYPE = np.dtype([("tag", "<u8"), ("read", "<u8"), ("rl", "<i4"), ("gl", "<i4"), ("roffset", "<i4"),
                             ("goffset", "<i4"), ("chroffset", "<u4"), ("chrhigh", "<u4"), ("extraband", "<i4"),
                             ("watsonp", "i1"), ("widebandp", "i1"), ("jump_late", "i1"), ("genestrand", "i1"),
-                            ("mismatchtype", "i1"), ("open", "i1"), ("extend", "i1"), ("pad", "i1"),
-                            ("pad2", "<i4", (3,))])
+                            ("mismatchtype", "i1"), ("open", "i1"), ("extend", "i1"), ("scoring", "i1"),
+                            ("pad", "<i4"), ("defect_rate", "<f8")])
 ALIGN_RES_DTYPE = np.dtype([("tag", "<u8"), ("status", "<i4"), ("finalscore", "<i4"), ("counts", "<i4", (6,)),
                             ("width", "<i4"), ("pad", "<i4"), ("pairs", "<u8"), ("t_submit", "<f8"),
                             ("t_done", "<f8")])

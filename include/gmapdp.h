@@ -848,7 +848,10 @@ typedef struct gdp_server_config {
 /* One alignment request (64 bytes).  read: rl query bytes, copied by
    gdp_server_submit.  roffset/goffset/chroffset/chrhigh/watsonp/genestrand
    have Dynprog_single_gap's meaning; extraband and widebandp feed
-   Dynprog_compute_bands; open/extend/mismatchtype are the fill's scoring. */
+   Dynprog_compute_bands; open/extend/mismatchtype are the fill's scoring,
+   or, with scoring = 1, chosen from defect_rate as Dynprog_single_gap
+   chooses them (dynprog_single.c:458-481: HIGHQ -8/-3 below 0.003, MEDQ
+   -7/-2 below 0.014, LOWQ -6/-1 above). */
 typedef struct gdp_align_request {
   uint64_t tag;
   const char *read;
@@ -866,8 +869,9 @@ typedef struct gdp_align_request {
   int8_t mismatchtype;
   int8_t open;
   int8_t extend;
-  int8_t pad;
-  int32_t pad2[3];
+  int8_t scoring;          /* 0: mismatchtype/open/extend as given; 1: by defect_rate */
+  int32_t pad;
+  double defect_rate;
 } gdp_align_request;
 
 /* One result (72 bytes).  status GDP_OK, or GDP_ETOOBIG for a request the

@@ -212,6 +212,43 @@ def test_server_result_alone_equals_result_in_mix(oracle, small_genome, slots, p
         _check_result(oracle, genome, nbases, reqs[k], _read(reads, reqs[k]), rm, pm)
 
 
+def test_server_scoring_by_defect_rate(oracle, small_genome):
+    """scoring = 1 picks Dynprog_single_gap's mismatch type and penalties
+    from defect_rate (dynprog_single.c:458-481); the results equal those of
+    the same requests with the choice written out, and the oracle's."""
+    from gmapdp import serve
+    gc, nbases, genome = small_genome
+    reqs, reads = serve.gen_requests(1007, 0, 240, gc, nbases)
+    rates = np.array([0.0, 0.001, 0.0029999, 0.003, 0.01, 0.0139999, 0.014, 0.05])
+    by_rate = reqs.copy()
+    by_rate["scoring"] = 1
+    by_rate["defect_rate"] = rates[np.arange(len(reqs)) % len(rates)]
+    by_rate["mismatchtype"], by_rate["open"], by_rate["extend"] = 0, 0, 0  # ignored with scoring 1
+    explicit = reqs.copy()
+    hq, mq = by_rate["defect_rate"] < 0.003, by_rate["defect_rate"] < 0.014
+    explicit["mismatchtype"] = np.where(hq, 0, np.where(mq, 1, 2))
+    explicit["open"] = np.where(hq, -8, np.where(mq, -7, -6))
+    explicit["extend"] = np.where(hq, -3, np.where(mq, -2, -1))
+    dg = serve.DeviceGenome(gc, nbases)
+    with serve.Server(dg, max_batch=64, slots=3) as srv:
+        srv.submit(by_rate)
+        got = srv.collect(len(reqs))
+        srv.submit(explicit)
+        want = srv.collect(len(reqs))
+    for k in range(len(reqs)):
+        tag = int(reqs["tag"][k])
+        rg, pg = got[tag]
+        rw, pw = want[tag]
+        assert rg["status"] == 0 and rw["status"] == 0, k
+        for f in ("finalscore", "width"):
+            assert rg[f] == rw[f], (k, f)
+        assert (rg["counts"] == rw["counts"]).all(), k
+        assert (pg == pw).all(), k
+    for k in range(0, len(reqs), 23):
+        rg, pg = got[int(reqs["tag"][k])]
+        _check_result(oracle, genome, nbases, explicit[k], _read(reads, explicit[k]), rg, pg)
+
+
 def test_server_admission_by_direction_budget(oracle, small_genome):
     """A direction budget far below one batch's worth forces the server to
     split and serialise batches; results stay exact and the bytes in flight
