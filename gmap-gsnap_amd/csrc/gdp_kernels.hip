@@ -1237,6 +1237,7 @@ __global__ __launch_bounds__(256) void gdp_best_endpoint_kernel(
 #define GDP_BRIDGE_ROWS 1024  // rows of one end's column held in LDS
 #define GDP_BRIDGE_LDS 8192   // words of LDS: right prefix tables and one left column
 #define GDP_BRIDGE_LDS_SMALL 4096  // the launch for gaps whose tables fit in half of it
+#define GDP_BRIDGE_LDS_TINY 2048   // and the one for gaps whose tables fit in a quarter
 
 // A score (int16 range) and its row in one word: the score in the high half.
 __device__ __forceinline__ int bridge_pack(int v, int r) { return (int)(((uint32_t)v << 16) | (uint32_t)(r & 0xffff)); }
@@ -1277,9 +1278,10 @@ __device__ __forceinline__ int bridge_score(const int16_t *scores, const gdp_dev
 // and shared evenly over the lanes.  Gaps wider than the LDS tables take the
 // direct form: per cR a prefix table of the right column, lanes over cL and
 // every left row.
-// Two launches share the gaps: LDSW = GDP_BRIDGE_LDS_SMALL takes the gaps
-// whose tables fit in that many words (twice the blocks per CU), the full
-// size takes all others, including those that fit neither form.
+// Three launches share the gaps: LDSW = GDP_BRIDGE_LDS_TINY takes the gaps
+// whose tables fit in that many words (four times the blocks per CU of the
+// full size), GDP_BRIDGE_LDS_SMALL those that fit in its words (twice), the
+// full size all others, including those that fit neither form.
 template <int LDSW>
 __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
     const gdp_dev_problem *__restrict__ probs, const int32_t *__restrict__ quads, int m,
@@ -1326,8 +1328,11 @@ __global__ __launch_bounds__(64) void gdp_bridge_cdna_kernel(
   const int nLmax = ubL + lbL + 1;
   const bool table = fast && 2 * (gl + 1) + nRsum + 2 * nLmax <= LDSW;
   // the same test at the small size, in both launches (nRsum is the same)
-  const bool small = fast && 2 * (gl + 1) + nRsum + 2 * nLmax <= GDP_BRIDGE_LDS_SMALL;
-  if (small != (LDSW == GDP_BRIDGE_LDS_SMALL)) return;  // the other launch's gap
+  // the launch a gap belongs to, from the same test in every launch (nRsum
+  // and fast are the same wherever the tables fit the smaller sizes)
+  const int need = 2 * (gl + 1) + nRsum + 2 * nLmax;
+  const int cls = fast && need <= GDP_BRIDGE_LDS_TINY ? 0 : (fast && need <= GDP_BRIDGE_LDS_SMALL ? 1 : 2);
+  if (cls != (LDSW == GDP_BRIDGE_LDS_TINY ? 0 : (LDSW == GDP_BRIDGE_LDS_SMALL ? 1 : 2))) return;  // another launch's
   const bool ok = ok0 && (table || ubR + lbR + 1 <= GDP_BRIDGE_ROWS);
   if (ok && table) {
     const gdp_geom GLu = gdp_make_geom(Lu), GLl = gdp_make_geom(Ll), GRu = gdp_make_geom(Ru), GRl = gdp_make_geom(Rl);
@@ -2531,6 +2536,8 @@ extern "C" int gdp_launch_bridge_cdna(const void *d_descs, const int32_t *d_quad
                                       int32_t *d_out, const gdp_tb_request *d_tmpl, gdp_tb_request *d_tbout,
                                       hipStream_t stream) {
   if (m <= 0) return 0;
+  hipLaunchKernelGGL(gdp_bridge_cdna_kernel<GDP_BRIDGE_LDS_TINY>, dim3(m), dim3(64), 0, stream,
+                     (const gdp_dev_problem *)d_descs, d_quads, m, d_scores, d_out, d_tmpl, d_tbout);
   hipLaunchKernelGGL(gdp_bridge_cdna_kernel<GDP_BRIDGE_LDS_SMALL>, dim3(m), dim3(64), 0, stream,
                      (const gdp_dev_problem *)d_descs, d_quads, m, d_scores, d_out, d_tmpl, d_tbout);
   hipLaunchKernelGGL(gdp_bridge_cdna_kernel<GDP_BRIDGE_LDS>, dim3(m), dim3(64), 0, stream,

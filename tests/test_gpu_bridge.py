@@ -48,18 +48,20 @@ def _gaps(rng, n):
     return out
 
 
-BRIDGE_LDS, BRIDGE_LDS_SMALL, BRIDGE_ROWS = 8192, 4096, 1024  # gdp_kernels.hip
+BRIDGE_LDS, BRIDGE_LDS_SMALL, BRIDGE_LDS_TINY, BRIDGE_ROWS = 8192, 4096, 2048, 1024  # gdp_kernels.hip
 
 
 def _form(sp):
-    """The kernel's form for a gap, from the same sizes it tests: 'small'
-    (table form in the 16 KB launch), 'table' (table form in the 32 KB
-    launch) or 'direct'."""
+    """The kernel's form for a gap, from the same sizes it tests: 'tiny'
+    (table form in the 8 KB launch), 'small' (the 16 KB launch), 'table'
+    (the 32 KB launch) or 'direct'."""
     gl, rlR = sp[0].gl, sp[2].rl
     lbL, ubL, lbR, ubR = sp[1].lband, sp[0].uband, sp[3].lband, sp[2].uband
     fast = gl + 1 <= BRIDGE_LDS and ubL + lbL + 1 <= BRIDGE_ROWS
     nRsum = sum(max(0, max(min(c - 1, rlR), min(rlR - 1, c + lbR)) - max(1, c - ubR) + 1) for c in range(gl))
     need = 2 * (gl + 1) + nRsum + 2 * (ubL + lbL + 1)
+    if fast and need <= BRIDGE_LDS_TINY:
+        return "tiny"
     if fast and need <= BRIDGE_LDS_SMALL:
         return "small"
     if fast and need <= BRIDGE_LDS:
@@ -71,9 +73,9 @@ def test_bridge_and_end_tracebacks(oracle):
     from gmapdp.batch import DeviceBatch
     rng = np.random.default_rng(2024)
     gaps = _gaps(rng, 90)
-    # every form of the two launches is taken by some gap of the batch
+    # every form of the three launches is taken by some gap of the batch
     forms = [_form(sp) for sp, _ in gaps]
-    assert all(forms.count(f) > 0 for f in ("small", "table", "direct")), forms
+    assert all(forms.count(f) > 0 for f in ("tiny", "small", "table", "direct")), forms
     specs = [s for sp, _ in gaps for s in sp]
     probs, qbuf, gbuf = H.pack_batch(specs, keep_scores=True)
     b = DeviceBatch(probs, qbuf, gbuf)
