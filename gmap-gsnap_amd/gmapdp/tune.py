@@ -81,8 +81,11 @@ def autotune(problems, qbuf, gbuf, sample=1_000_000, objective="throughput", rep
 # 300) 4.86e6 at 4.8 ms, (12, 8192, 300) 5.29e6-5.50e6 at 4.0-4.3 ms,
 # (15, 8192, 300) 5.53e6 at 3.7-3.9 ms, (12, 16384, 300) 6.01e6 at 4.4-4.5
 # ms, (15, 4096, 300) 5.03e6 at 3.6-3.7 ms; binned by 8 columns,
-# (15, 8192, 300) 5.81e6 at 3.9-4.0 ms.
-SERVER_KNOBS = ((15, 8192, 300, 8, 1), (15, 8192, 300, 4, 1), (12, 16384, 300, 4, 1), (8, 8192, 300, 4, 1))
+# (15, 8192, 300) 5.81e6 at 3.9-4.0 ms.  With the latency tuner's lane
+# groups 112,128: (12, 16384, 300, 4) 5.80e6 at 3.6 ms, (12, 24576, 300, 4)
+# 5.86e6 at 3.7 ms, (15, 16384, 300, 8) 5.99e6 at 3.5-3.6 ms, (15, 12288,
+# 300, 4) 5.69e6 at 3.4-3.5 ms: saturation levels off near 2.8e2 GCUPS.
+SERVER_KNOBS = ((15, 16384, 300, 8, 1), (15, 8192, 300, 8, 1), (12, 16384, 300, 4, 1), (15, 8192, 300, 4, 1))
 
 
 def tune_server(genome, reqs, plan, target_p99_ms=5.0, knobs=SERVER_KNOBS, seconds=1.5, margin=0.9, **server_kw):
