@@ -13,8 +13,10 @@ introntype} and splice-site probabilities {left_prob, right_prob}; and
 tests/golden/callers_cdna_golden.npz: 600 calls of Dynprog_cdna_gap
 (dynprog_cdna.c:786-1180) from helpers.cdna_gap_calls (seed 1003), with
 their pair lists (the gap holder between the ends marks the cDNA insertion)
-and counters {dynprogindex, traceback_score, incompletep}.  With the
-argument `genome` or `cdna` only that file is written.
+and counters {dynprogindex, traceback_score, incompletep}; and
+tests/golden/callers_cdna_insert_golden.npz: the same for calls whose join
+takes the insertion-pairs branch (main_cdna_insert).  With the argument
+`genome`, `cdna` or `cdna_insert` only that file is written.
 
 Run in a container where the reference tree is mounted, after
 `make -C oracle ref`, which builds oracle/_ref/libgmapref_callers.so from
@@ -43,13 +45,17 @@ import helpers as H  # noqa: E402
 OUT = os.path.join(HERE, "callers_golden.npz")
 OUT_GENOME = os.path.join(HERE, "callers_genome_golden.npz")
 OUT_CDNA = os.path.join(HERE, "callers_cdna_golden.npz")
+OUT_CDNA_INSERT = os.path.join(HERE, "callers_cdna_insert_golden.npz")
 SEED, SEED_GENOME, SEED_CDNA, N_EXTRA = 1001, 1002, 1003, 600
+SEED_CDNA_INSERT, N_INSERT = 1004, 300
 
 
 def main():
     seq, _ = golden_io.skip_genome_image()
     if sys.argv[1:] in (["genome"], ["cdna"]):
         return main_extra(seq, sys.argv[1])
+    if sys.argv[1:] == ["cdna_insert"]:
+        return main_cdna_insert(seq)
     cases = H.caller_cases(np.random.default_rng(SEED), seq, 2000, 1000)
     with tempfile.TemporaryDirectory() as d:
         cpath, opath = os.path.join(d, "cases.npz"), os.path.join(d, "out.npz")
@@ -60,6 +66,7 @@ def main():
     print("wrote %s: %d calls" % (OUT, len(cases)))
     main_extra(seq, "genome")
     main_extra(seq, "cdna")
+    main_cdna_insert(seq)
 
 
 def main_extra(seq, kind):
@@ -76,6 +83,34 @@ def main_extra(seq, kind):
         z, o = np.load(cpath), np.load(opath)
         np.savez_compressed(out, **{k: z[k] for k in z.files}, **{"out_" + k: o[k] for k in o.files})
     print("wrote %s: %d calls" % (out, len(cases)))
+
+
+def main_cdna_insert(seq):
+    """Dynprog_cdna_gap calls whose join leaves 9 query and 9 genome positions
+    (INSERT_PAIRS, dynprog_cdna.c:1007-1029): query gaps as long as their
+    genome gaps with 9 bases replaced (helpers.cdna_gap_calls, block 9).  Run
+    one after another in one process, some of these calls leave the
+    reference's caller in a state that crashes a later call, while every call
+    finishes in a fresh process; so each call runs in a child process of its
+    own and the outputs are joined in order."""
+    import subprocess
+    cases = H.cdna_gap_calls(np.random.default_rng(SEED_CDNA_INSERT), seq, N_INSERT, block=9)
+    outs = []
+    with tempfile.TemporaryDirectory() as d:
+        for k in range(len(cases)):
+            cpath, opath = os.path.join(d, "c.npz"), os.path.join(d, "o.npz")
+            callers_run.save_cdna_cases(cpath, cases[k:k + 1])
+            code = ("import sys; sys.path[:0] = %r; import callers_run; callers_run.run_cdna('ref', %r, %r)"
+                    % (sys.path[:3], cpath, opath))
+            subprocess.run([sys.executable, "-c", code], stdout=subprocess.DEVNULL, check=True)
+            o = np.load(opath)
+            outs.append({f: o[f] for f in o.files})
+        cpath = os.path.join(d, "cases.npz")
+        callers_run.save_cdna_cases(cpath, cases)
+        z = np.load(cpath)
+        out = {"out_" + f: np.concatenate([x[f] for x in outs]) for f in outs[0]}
+        np.savez_compressed(OUT_CDNA_INSERT, **{k: z[k] for k in z.files}, **out)
+    print("wrote %s: %d calls" % (OUT_CDNA_INSERT, len(cases)))
 
 
 if __name__ == "__main__":

@@ -615,7 +615,7 @@ def genome_gap_calls(rng, genome_seq, n, chromosomes=2):
     return out
 
 
-def cdna_gap_calls(rng, genome_seq, n, chromosomes=2, flank=260):
+def cdna_gap_calls(rng, genome_seq, n, chromosomes=2, flank=260, block=0):
     """Calls of Dynprog_cdna_gap as stage 3 makes them (stage3.c:9274-9321: a
     query gap longer than its genome gap by more than MININTRONLEN, both ends
     read rlengthL = rlengthR = genomejump + extramaterial_paired bases): a
@@ -647,7 +647,15 @@ def cdna_gap_calls(rng, genome_seq, n, chromosomes=2, flank=260):
 
         a = int(rng.integers(0, glength + 1))
         seg = S(goffset, glength)
-        mid = mutate(rng, seg[:a], 0.01, 0.0) + rand_seq(rng, ins) + mutate(rng, seg[a:], 0.01, 0.0)
+        if block:
+            # no insertion: `block` bases of the gap replaced by others, which
+            # a join can leave to both sides as cDNA and genome insertions
+            a = min(a, max(glength - block, 0))
+            swap = bytes(b"ACGT"[(b"ACGT".find(bytes([x])) + 1 + int(rng.integers(0, 3))) % 4]
+                         for x in seg[a:a + block])
+            mid = seg[:a] + swap + seg[a + block:]
+        else:
+            mid = mutate(rng, seg[:a], 0.01, 0.0) + rand_seq(rng, ins) + mutate(rng, seg[a:], 0.01, 0.0)
         query = S(goffset - flank, flank) + mid + S(goffset + glength, flank)
         qdp5 = flank
         qdp3 = flank + len(mid) - 1
