@@ -2340,7 +2340,7 @@ __device__ __forceinline__ void gdp_cd_jumps(const gdp_cd_dev &c, const int32_t 
 
 // Per call the length of the returned list: the right walk, 18 insertion
 // pairs when 9 query and 9 genome positions are left (INSERT_PAIRS,
-// dynprog_cdna.c:39, 1072) or else one gap holder, the left walk; -1 (NULL)
+// dynprog_cdna.c:39, 1007-1030 and 1136-1159) or else one gap holder, the left walk; -1 (NULL)
 // for the gap holder alone (1170-1173).
 __global__ void gdp_cd_len_kernel(int m, const gdp_cd_dev *__restrict__ calls, const int32_t *__restrict__ bout,
                                   const int32_t *__restrict__ cd, int32_t *__restrict__ len) {

@@ -87,7 +87,7 @@ def main_extra(seq, kind):
 
 def main_cdna_insert(seq):
     """Dynprog_cdna_gap calls whose join leaves 9 query and 9 genome positions
-    (INSERT_PAIRS, dynprog_cdna.c:1007-1029): query gaps as long as their
+    (INSERT_PAIRS, dynprog_cdna.c:1007-1030): query gaps as long as their
     genome gaps with 9 bases replaced (helpers.cdna_gap_calls, block 9).  Run
     one after another in one process, some of these calls leave the
     reference's caller in a state that crashes a later call, while every call

@@ -10,7 +10,7 @@ insertion pairs or the gap holder, the left walk), traceback_score,
 incompletep and whether dynprogindex advanced must match.  The 300 calls of
 tests/golden/callers_cdna_insert_golden.npz hold query gaps as long as their
 genome gaps with 9 bases replaced, so that most joins take the reference's
-insertion-pairs branch (dynprog_cdna.c:1007-1029: 9 cDNA-insertion and 9
+insertion-pairs branch (dynprog_cdna.c:1007-1030: 9 cDNA-insertion and 9
 genome-insertion pairs in place of the gap holder)."""
 import os
 
